@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X propagator-completion engine.
+
+Default workload (one step = propagating one batch to its fixpoints):
+  one seeded 50,000-variable / 200,000-propagator ternary network, 4,096
+  search-node subproblems of it, each with 1-20 random splits (x <= v or
+  x > v inside the current bounds) and a random subset of 2,000-200,000
+  active propagators; int32 interval stores, all resident in HBM.
+A step restores the 4,096 stores from their device-resident image and runs
+the batched engine to every instance's fixpoint.  The same batch is also run
+once with the plain engine (one sweep launch per instance and sweep), and a
+bounded sample of instances runs on the CPU oracle (the reference's
+Gauss-Seidel algorithm, restated) on the host cores; the GPU results of that
+sample are checked bit-for-bit against it.
+
+--config 1 instead times the single 1,000,000-variable / 4,000,000-propagator
+fixpoint (plain engine, one step = one fixpoint).
+
+Prints one JSON line (the driver's contract).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lala-pc_amd"))
+
+import numpy as np  # noqa: E402
+
+import pcx  # noqa: E402
+
+METRIC = "fixpoints/sec batched vs plain; propagator evals/s; %HBM roofline; p99 latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_READ_PER_EVAL = 40     # 16 B propagator + 3 x 8 B operand intervals
+BYTES_PER_TIGHTENING = 4     # one int32 bound written
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(n, 16))
+
+
+def roofline(evals, tightened, kernel_ms):
+    algo_bytes = evals * BYTES_READ_PER_EVAL + tightened * BYTES_PER_TIGHTENING
+    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_step": int(algo_bytes)}
+
+
+def cpu_oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib  # test infrastructure: used only as the CPU baseline / checker
+    return oracle_lib
+
+
+def bench_batched(args):
+    V, P, n_inst = 50000, 200000, 4096
+    t0 = time.time()
+    p = pcx.gen_params(seed=args.seed, nvars=V, nprops=P)
+    op, x, y, z, lb, ub, w = pcx.gen_network(p)
+    pir = pcx.PIR(props_per_wave=args.ppw, timing=1)
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    rr = pir.fixpoint(root_lb, root_ub)
+    slb, sub = pcx.gen_splits(args.seed + 1, root_lb, root_ub, n_inst, 1, 20)
+    off, idx = pcx.gen_subsets(args.seed + 2, P, n_inst, 2000, 200000)
+    batch = pcx.Batch(pir, n_inst, V, off, idx)
+    batch.load(slb, sub)
+    log(f"setup {time.time() - t0:.1f}s: root fixpoint sweeps={rr.sweeps} failed={rr.failed}, "
+        f"{int(off[-1])} active propagators over {n_inst} instances")
+
+    for _ in range(args.warmup):
+        batch.reset()
+        batch.run(pcx.MODE_BATCHED)
+    evals = tight = 0
+    kms = 0.0
+    launches = 0
+    smax = 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        batch.reset()
+        res, st = batch.run(pcx.MODE_BATCHED)
+        evals += st.evals
+        tight += st.tightened
+        kms += st.kernel_ms
+        launches += st.kernel_launches
+        smax = max(smax, st.sweeps_max)
+    elapsed = time.perf_counter() - t_start
+    fps = n_inst * args.steps / elapsed
+    failed = sum(r.failed for r in res)
+    lb_g, ub_g = batch.read()
+    log(f"batched: {elapsed / args.steps * 1e3:.1f} ms/step, {fps:.0f} fixpoints/s, "
+        f"sweeps_max={smax}, failed={failed}/{n_inst}, kernel {kms / args.steps:.1f} ms/step")
+
+    # plain engine on the same batch (one instance at a time)
+    batch.reset()
+    tp = time.perf_counter()
+    res_p, st_p = batch.run(pcx.MODE_PLAIN)
+    plain_s = time.perf_counter() - tp
+    plain_fps = n_inst / plain_s
+    lb_p, ub_p = batch.read()
+    log(f"plain: {plain_s * 1e3:.0f} ms for {n_inst} instances, {plain_fps:.0f} fixpoints/s")
+
+    out = {
+        "value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
+        "config": {"workload": "batched search-node fixpoints: 4,096 subproblems of one "
+                               "50,000-var / 200,000-propagator network, 1-20 splits each, "
+                               "2,000-200,000 active propagators each",
+                   "nvars": V, "nprops": P, "instances": n_inst, "props_per_wave": args.ppw,
+                   "mode": "batched (device scheduler, one sweep launch per sweep)"},
+        "plain_fixpoints_per_s": plain_fps,
+        "batched_vs_plain": fps / plain_fps,
+        "evals_per_s": evals / elapsed,
+        "failed_instances": int(failed),
+        "sweeps_max": int(smax),
+        "roofline": roofline(evals / args.steps, tight / args.steps, kms / args.steps),
+        "kernel_launches_per_step": launches / args.steps,
+    }
+    out["roofline"]["kernel"] = "k_sweep_batched"
+
+    if not args.no_cpu:
+        ol = cpu_oracle()
+        top, tx, ty, tz = pir.table()
+        threads = host_threads()
+        sample = []
+        t_cpu = 0.0
+        k = 0
+        idx64 = idx.astype(np.int64)
+        # bounded sample: whole instances until about 15 s of CPU work
+        while k < n_inst and t_cpu < args.cpu_seconds:
+            n = min(threads * 2, n_inst - k)
+            rows = slice(k * V, (k + n) * V)
+            lo, hi = slb[rows].copy(), sub[rows].copy()
+            o = off[k:k + n + 1] - off[k]
+            ix = idx64[off[k]:off[k + n]]
+            ts = time.perf_counter()
+            r = ol.fixpoint_batch(top, tx, ty, tz, lo, hi, V, idx=ix, off=o, threads=threads)
+            t_cpu += time.perf_counter() - ts
+            sample.append((k, n, lo, hi, r))
+            k += n
+        mism = 0
+        for k0, n, lo, hi, r in sample:
+            for j in range(n):
+                kk = k0 + j
+                g_fail = bool(res[kk].failed)
+                if g_fail != bool(r[j].failed):
+                    mism += 1
+                elif not g_fail:
+                    s = slice(kk * V, (kk + 1) * V)
+                    if not (np.array_equal(lb_g[s], lo[j * V:(j + 1) * V]) and
+                            np.array_equal(ub_g[s], hi[j * V:(j + 1) * V]) and
+                            np.array_equal(lb_p[s], lo[j * V:(j + 1) * V]) and
+                            np.array_equal(ub_p[s], hi[j * V:(j + 1) * V])):
+                        mism += 1
+        out["cpu_baseline"] = {"value": k / t_cpu, "unit": "fixpoints/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"first {k} of the {n_inst} instances, one instance per "
+                                         f"thread, oracle Gauss-Seidel (-O3)"}
+        out["parity"] = {"checked_instances": k, "mismatches": mism}
+        log(f"cpu: {k} instances in {t_cpu:.1f}s on {threads} threads; parity mismatches {mism}")
+    return out
+
+
+def bench_single(args):
+    V, P = 1000000, 4000000
+    p = pcx.gen_params(seed=args.seed, nvars=V, nprops=P)
+    op, x, y, z, lb, ub, w = pcx.gen_network(p)
+    pir = pcx.PIR(props_per_wave=args.ppw, timing=1)
+    pir.tell(op, x, y, z, lb, ub)
+    batch = pcx.Batch(pir, 1, V)
+    batch.load(lb, ub)
+    for _ in range(args.warmup):
+        batch.reset()
+        batch.run(pcx.MODE_PLAIN)
+    evals = tight = 0
+    kms = 0.0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        batch.reset()
+        res, st = batch.run(pcx.MODE_PLAIN)
+        evals += st.evals
+        tight += st.tightened
+        kms += st.kernel_ms
+    elapsed = time.perf_counter() - t_start
+    lb_g, ub_g = batch.read()
+    out = {
+        "value": args.steps / elapsed, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
+        "config": {"workload": "single 1,000,000-var / 4,000,000-propagator fixpoint (plain engine)",
+                   "nvars": V, "nprops": P, "props_per_wave": args.ppw},
+        "evals_per_s": evals / elapsed, "sweeps": int(res[0].sweeps), "failed": bool(res[0].failed),
+        "roofline": roofline(evals / args.steps, tight / args.steps, kms / args.steps),
+    }
+    out["roofline"]["kernel"] = "k_sweep_plain"
+    if not args.no_cpu:
+        ol = cpu_oracle()
+        top, tx, ty, tz = pir.table()
+        lo, hi = lb.copy(), ub.copy()
+        ol.tell_embed(op, x, lo, hi)
+        ts = time.perf_counter()
+        r = ol.fixpoint(top, tx, ty, tz, lo, hi)
+        t_cpu = time.perf_counter() - ts
+        out["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "fixpoints/s", "cores": 1, "kind": "port",
+                               "sample": "the full fixpoint, one thread"}
+        out["parity"] = {"bit_exact": bool(np.array_equal(lo, lb_g) and np.array_equal(hi, ub_g)),
+                         "oracle_sweeps": int(r.sweeps)}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--ppw", type=int, default=256, choices=[64, 128, 256, 512])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; --gpus must be 1 (instances would shard "
+              "across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if pcx.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    out = bench_batched(args) if args.config == 2 else bench_single(args)
+    line = {"metric": METRIC, "value": out.pop("value"), "unit": out.pop("unit"), "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": out.pop("ms_per_step"),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic (seeded generator)", "config": out.pop("config")}
+    line.update(out)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

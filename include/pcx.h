@@ -1,0 +1,162 @@
+/*
+ * pcx.h -- C ABI of the MI355X propagator-completion engine.
+ *
+ * The engine holds a table of ternary propagators `x = y <op> z` (the PIR
+ * bytecode form of lattice-land/lala-pc, include/lala/pir.hpp:36-47) and runs
+ * their fixpoint over stores of int32 intervals on a gfx950 GPU.  It replaces,
+ * for that path, the reference's
+ *     GaussSeidelIteration{}.fixpoint(pir.num_deductions(),
+ *                                     [&](size_t i) { return pir.deduce(i); })
+ * (tests/pir_test.cpp:60-62; engine in lala-core fixpoint.hpp, not vendored).
+ *
+ * Conventions
+ *  - A store is two int32 arrays lb[nvars], ub[nvars].  PCX_MINF / PCX_INF
+ *    (INT32_MIN / INT32_MAX) stand for -oo / +oo (pir.hpp:403-404).  An
+ *    interval is empty (bot) when lb > ub.
+ *  - Failure is a result field, never an error status (pir.hpp:831-833).
+ *  - No function throws; every failure is a pcx_status with a message from
+ *    pcx_last_error() (thread-local).
+ *  - One host thread per engine handle (pc.hpp:618-623: telling is not
+ *    thread-safe in the reference either).  Device work is asynchronous
+ *    inside a call but every call returns with its results complete.
+ */
+#ifndef PCX_H
+#define PCX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pcx_status;
+#define PCX_OK       0
+#define PCX_EINVAL   1   /* bad argument, opcode or variable index           */
+#define PCX_ENOMEM   2   /* device or host allocation failed                 */
+#define PCX_EDEVICE  3   /* HIP runtime error (no device, launch failure ...) */
+#define PCX_EBUSY    4   /* serving: no capacity to admit the request now     */
+#define PCX_ENOTSUP  5   /* valid request this build does not implement       */
+
+/* Opcodes of `x = y <op> z`.  Subtraction is an ADD with permuted operands
+ * and x != y an EQ whose x is the constant 0, as in the reference
+ * (pir.hpp:270-273; pir_test.cpp:277-283, 544-548).  The numbering is this
+ * library's; it only fixes the table sort order (see pcx_tell). */
+enum {
+  PCX_ADD = 0, PCX_MUL = 1, PCX_TDIV = 2, PCX_FDIV = 3, PCX_CDIV = 4,
+  PCX_EDIV = 5, PCX_MIN = 6, PCX_MAX = 7, PCX_EQ = 8, PCX_LEQ = 9,
+  PCX_NUM_OPS = 10
+};
+
+#define PCX_INF  INT32_MAX
+#define PCX_MINF INT32_MIN
+#define PCX_MAX_VARS (1 << 28)   /* variable indices must be < 2^28 */
+
+typedef struct pcx_engine pcx_engine;
+typedef struct pcx_batch pcx_batch;
+
+typedef struct {
+  int32_t device;           /* HIP device ordinal                                   */
+  int32_t props_per_wave;   /* 64, 128, 256 or 512; 0 = library default              */
+  int32_t lds_mode;         /* -1 auto, 0 never stage stores in LDS, 1 whenever fit  */
+  int32_t sweeps_per_check; /* plain engine: sweeps queued between host checks (0=8) */
+  int64_t max_sweeps;       /* 0 = run to the fixpoint                               */
+  int32_t timing;           /* 1 = record HIP events around every sweep kernel       */
+  int32_t reserved[7];
+} pcx_config;
+
+typedef struct {
+  int32_t has_changed;      /* some bound was tightened                      */
+  int32_t failed;           /* the store became bot (some lb > ub)           */
+  int64_t sweeps;           /* sweeps run, including the final quiet one     */
+  int64_t evals;            /* propagator evaluations = sum of sweep sizes   */
+} pcx_result;
+
+typedef struct {
+  double  wall_ms;          /* host wall time of the call                    */
+  double  kernel_ms;        /* summed sweep-kernel durations (config.timing) */
+  int64_t kernel_launches;  /* sweep-kernel launches                         */
+  int64_t evals;            /* total evaluations over all instances          */
+  int64_t tightened;        /* successful bound tightenings (atomic updates) */
+  int64_t sweeps_max;       /* largest per-instance sweep count              */
+} pcx_stats;
+
+/* ---- library ---------------------------------------------------------- */
+const char* pcx_version(void);
+const char* pcx_last_error(void);
+void        pcx_config_default(pcx_config* cfg);
+/* Number of HIP devices visible (0 when none); never fails. */
+int32_t     pcx_device_count(void);
+
+/* ---- engine (propagator table) ----------------------------------------
+ * pcx_create: engine on cfg->device with an empty table.
+ *   replaces PIR(AType, sub_ptr, alloc) (pir.hpp:158-163).               */
+pcx_status pcx_create(const pcx_config* cfg, pcx_engine** out);
+void       pcx_destroy(pcx_engine* e);
+
+/* pcx_tell: append n propagators, embed x in [0,1] for EQ/LEQ into the
+ * caller's store (lb/ub may be NULL to skip), then stable-sort the whole
+ * table by (op, y, x, z).
+ *   replaces PIR::interpret_tell + PIR::deduce(tell_type) (pir.hpp:316-352);
+ *   formulas arrive already in ternary form, so interpretation reduces to
+ *   opcode/index validation (PCX_EINVAL for an unknown opcode or an index
+ *   outside [0, nvars) when nvars > 0, or >= PCX_MAX_VARS).              */
+pcx_status pcx_tell(pcx_engine* e, const int32_t* op, const int32_t* x,
+                    const int32_t* y, const int32_t* z, int64_t n,
+                    int32_t* lb, int32_t* ub, int64_t nvars);
+
+/* replaces PIR::num_deductions (pir.hpp:382-384). */
+int64_t    pcx_num_deductions(const pcx_engine* e);
+
+/* Copy the table in engine (sorted) order; any pointer may be NULL. */
+pcx_status pcx_get_table(const pcx_engine* e, int32_t* op, int32_t* x,
+                         int32_t* y, int32_t* z);
+
+/* pcx_snapshot / pcx_restore: the table size is the snapshot; restore pops
+ * the propagators told since (pir.hpp:857-870).  Stores are the caller's. */
+int64_t    pcx_snapshot(const pcx_engine* e);
+pcx_status pcx_restore(pcx_engine* e, int64_t snapshot);
+
+/* pcx_fixpoint: run every propagator to the fixpoint on one store held in
+ * host memory (copied in, propagated on the GPU, copied back).
+ *   replaces GaussSeidelIteration::fixpoint over PIR::deduce(int)
+ *   (tests/pir_test.cpp:60-62, 82-86; pir.hpp:387-390, 721-817).
+ * On return lb/ub hold the fixpoint; after a failure they hold some empty
+ * store (bounds after failure are order-dependent, only `failed` is fixed). */
+pcx_status pcx_fixpoint(pcx_engine* e, int32_t* lb, int32_t* ub, int64_t nvars,
+                        pcx_result* out);
+
+/* pcx_ask_all: entailment of every propagator on a store,
+ *   replaces PIR::ask(int) (pir.hpp:368-370, 417-438) for all i at once.
+ * entailed[i] (may be NULL) receives 0/1 per propagator in table order;
+ * *all (may be NULL) receives 1 iff every propagator is entailed.        */
+pcx_status pcx_ask_all(pcx_engine* e, const int32_t* lb, const int32_t* ub,
+                       int64_t nvars, uint8_t* entailed, int32_t* all);
+
+/* ---- batches: many stores over one shared table ------------------------
+ * Instance k owns store slice [k*nvars, (k+1)*nvars) and runs the subset of
+ * propagators prop_idx[inst_off[k] .. inst_off[k+1]) (indices into the table
+ * in engine order, ascending), or the whole table when inst_off == NULL.
+ * This is the reference's shared-table clone model (pir.hpp:182-205:
+ * bytecodes shared, store cloned per block), with search-node stores.
+ * Stores stay resident in device memory between calls.                   */
+pcx_status pcx_batch_create(pcx_engine* e, int64_t n_inst, int64_t nvars,
+                            const int64_t* inst_off, const int32_t* prop_idx,
+                            pcx_batch** out);
+void       pcx_batch_destroy(pcx_batch* b);
+/* Upload n_inst*nvars bounds; they also become the reset image. */
+pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub);
+/* Restore every store from the reset image (device to device). */
+pcx_status pcx_batch_reset(pcx_batch* b);
+pcx_status pcx_batch_read(pcx_batch* b, int32_t* lb, int32_t* ub);
+
+#define PCX_MODE_PLAIN      0  /* instance by instance, one sweep launch each   */
+#define PCX_MODE_BATCHED    1  /* all instances per sweep launch, device queue  */
+#define PCX_MODE_PERSISTENT 2  /* one persistent launch for the whole batch     */
+/* Propagate every instance to its fixpoint.  results: n_inst entries or NULL. */
+pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results,
+                         pcx_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PCX_H */

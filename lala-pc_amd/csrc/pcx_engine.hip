@@ -1,0 +1,577 @@
+// pcx_engine.hip -- host side of the C ABI (include/pcx.h).
+//
+// The engine owns the propagator table (host copy in engine order + device
+// SoA copy) and launches the sweep kernels; stores live in device memory
+// inside batches.  pcx_fixpoint on host arrays is a batch of one instance.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/pcx.h"
+#include "pcx_kernels.h"
+
+using namespace pcx;
+
+namespace {
+
+thread_local std::string g_err;
+
+pcx_status fail_with(pcx_status st, const std::string& msg) {
+  g_err = msg;
+  return st;
+}
+
+#define PCX_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail_with(e_ == hipErrorOutOfMemory ? PCX_ENOMEM : PCX_EDEVICE,           \
+                       std::string(#call) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+bool valid_ppw(int p) { return p == 64 || p == 128 || p == 256 || p == 512; }
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct pcx_engine {
+  pcx_config cfg{};
+  std::vector<int32_t> op, x, y, z;   // table in engine order
+  int32_t max_var = -1;
+  bool has_div = false;               // table holds a division propagator
+  bool dirty = true;
+  uint32_t* d_xo = nullptr;
+  int32_t* d_y = nullptr;
+  int32_t* d_z = nullptr;
+  int64_t d_cap = 0;
+  hipStream_t stream = nullptr;
+  pcx_batch* scratch = nullptr;       // batch of one for pcx_fixpoint / pcx_ask_all
+  int64_t scratch_nvars = -1;
+  int64_t scratch_table = -1;
+};
+
+struct pcx_batch {
+  pcx_engine* e = nullptr;
+  int64_t n_inst = 0, nvars = 0;
+  int64_t table_n = 0;                // table size the subsets were built for
+  int2* d_store = nullptr;
+  int2* d_image = nullptr;
+  int64_t* d_off = nullptr;
+  int32_t* d_idx = nullptr;
+  std::vector<int64_t> h_off;         // empty = whole table
+  InstFlags* d_flags = nullptr;
+  unsigned long long* d_tight = nullptr;
+  int32_t* d_act = nullptr;
+  int64_t* d_item_base = nullptr;
+  int32_t* d_ctl = nullptr;           // [0]=n_act [1]=sweep [2]=live
+  std::vector<hipEvent_t> ev;         // timing pool
+};
+
+namespace {
+
+pcx_status sync_table(pcx_engine* e) {
+  if (!e->dirty) return PCX_OK;
+  const int64_t n = (int64_t)e->op.size();
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  if (n > e->d_cap) {
+    dfree(e->d_xo); dfree(e->d_y); dfree(e->d_z);
+    const int64_t cap = std::max<int64_t>(n, 1);
+    PCX_HIP(hipMalloc(&e->d_xo, cap * sizeof(uint32_t)));
+    PCX_HIP(hipMalloc(&e->d_y, cap * sizeof(int32_t)));
+    PCX_HIP(hipMalloc(&e->d_z, cap * sizeof(int32_t)));
+    e->d_cap = cap;
+  }
+  if (n > 0) {
+    std::vector<uint32_t> xo(n);
+    for (int64_t i = 0; i < n; ++i) xo[i] = (uint32_t)e->x[i] | ((uint32_t)e->op[i] << kOpShift);
+    PCX_HIP(hipMemcpy(e->d_xo, xo.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    PCX_HIP(hipMemcpy(e->d_y, e->y.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    PCX_HIP(hipMemcpy(e->d_z, e->z.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  e->dirty = false;
+  return PCX_OK;
+}
+
+Table table_of(const pcx_engine* e) {
+  return Table{e->d_xo, e->d_y, e->d_z, (int64_t)e->op.size()};
+}
+
+int ppw_of(const pcx_engine* e) { return e->cfg.props_per_wave ? e->cfg.props_per_wave : 256; }
+
+int64_t inst_size(const pcx_batch* b, int64_t k) {
+  return b->h_off.empty() ? b->table_n : b->h_off[k + 1] - b->h_off[k];
+}
+
+// Events for kernel timing, reused across runs.
+hipEvent_t event_at(pcx_batch* b, size_t i) {
+  while (b->ev.size() <= i) {
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    b->ev.push_back(ev);
+  }
+  return b->ev[i];
+}
+
+pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* results,
+                   pcx_stats* stats, int64_t launches, double kernel_ms) {
+  pcx_engine* e = b->e;
+  flags.resize(b->n_inst);
+  PCX_HIP(hipMemcpyAsync(flags.data(), b->d_flags, b->n_inst * sizeof(InstFlags),
+                         hipMemcpyDeviceToHost, e->stream));
+  unsigned long long tight[kTightSlots];
+  PCX_HIP(hipMemcpyAsync(tight, b->d_tight, sizeof(tight), hipMemcpyDeviceToHost, e->stream));
+  PCX_HIP(hipStreamSynchronize(e->stream));
+  int64_t evals = 0, smax = 0;
+  for (int64_t k = 0; k < b->n_inst; ++k) {
+    const InstFlags& f = flags[k];
+    const int32_t last = std::max(f.changed[0], f.changed[1]);
+    pcx_result r{};
+    r.failed = f.failed_at != kNoFail;
+    r.has_changed = last >= 0;
+    r.sweeps = r.failed ? (int64_t)f.failed_at + 1 : (int64_t)last + 2;
+    if (e->cfg.max_sweeps > 0 && r.sweeps > e->cfg.max_sweeps) r.sweeps = e->cfg.max_sweeps;
+    r.evals = r.sweeps * inst_size(b, k);
+    evals += r.evals;
+    smax = std::max(smax, r.sweeps);
+    if (results) results[k] = r;
+  }
+  if (stats) {
+    stats->kernel_ms = kernel_ms;
+    stats->kernel_launches = launches;
+    stats->evals = evals;
+    unsigned long long t = 0;
+    for (int i = 0; i < kTightSlots; ++i) t += tight[i];
+    stats->tightened = (int64_t)t;
+    stats->sweeps_max = smax;
+  }
+  return PCX_OK;
+}
+
+pcx_status prepare_run(pcx_batch* b) {
+  pcx_engine* e = b->e;
+  if ((int64_t)e->op.size() != b->table_n)
+    return fail_with(PCX_EINVAL, "table changed since the batch was created");
+  if (e->max_var >= b->nvars)
+    return fail_with(PCX_EINVAL, "a propagator names a variable outside the store");
+  pcx_status s = sync_table(e);
+  if (s != PCX_OK) return s;
+  PCX_HIP(hipMemsetAsync(b->d_tight, 0, kTightSlots * sizeof(unsigned long long), e->stream));
+  WorkList wl{b->d_act, b->d_item_base, b->d_ctl, b->d_ctl + 1, b->d_ctl + 2};
+  PCX_HIP(launch_init_flags(b->d_flags, (int32_t)b->n_inst, wl, e->stream));
+  PCX_HIP(launch_store_check_empty(b->d_store, b->nvars, (int32_t)b->n_inst, b->d_flags, e->stream));
+  return PCX_OK;
+}
+
+// Plain mode: every instance on its own, one launch per sweep, host checks
+// the instance's flags every `group` sweeps.
+pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
+  pcx_engine* e = b->e;
+  const int ppw = ppw_of(e);
+  const int group = e->cfg.sweeps_per_check > 0 ? e->cfg.sweeps_per_check : 8;
+  const Table t = table_of(e);
+  int64_t launches = 0;
+  double kms = 0.0;
+  size_t nev = 0;
+  InstFlags hf;
+  for (int64_t k = 0; k < b->n_inst; ++k) {
+    const int64_t m = inst_size(b, k);
+    const int32_t* idx = b->h_off.empty() ? nullptr : b->d_idx + b->h_off[k];
+    int2* st = b->d_store + k * b->nvars;
+    InstFlags* f = b->d_flags + k;
+    int32_t s = 0;
+    for (;;) {
+      for (int g = 0; g < group; ++g, ++s) {
+        if (e->cfg.max_sweeps > 0 && s >= e->cfg.max_sweeps) break;
+        if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
+        PCX_HIP(launch_sweep_plain(t, st, idx, m, s, f, b->d_tight, ppw, e->has_div, e->stream));
+        if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
+        ++launches;
+      }
+      PCX_HIP(hipMemcpyAsync(&hf, f, sizeof(hf), hipMemcpyDeviceToHost, e->stream));
+      PCX_HIP(hipStreamSynchronize(e->stream));
+      for (size_t i = 0; i + 1 < nev; i += 2) {
+        float ms = 0.f;
+        PCX_HIP(hipEventElapsedTime(&ms, b->ev[i], b->ev[i + 1]));
+        kms += ms;
+      }
+      nev = 0;
+      const int32_t last = std::max(hf.changed[0], hf.changed[1]);
+      if (hf.failed_at != kNoFail || last + 1 < s) break;       // quiet sweep already ran
+      if (e->cfg.max_sweeps > 0 && s >= e->cfg.max_sweeps) break;
+    }
+  }
+  std::vector<InstFlags> flags;
+  return collect(b, flags, results, stats, launches, kms);
+}
+
+// Batched mode: per sweep one scheduler launch and one grid-stride sweep
+// launch over the work items of every running instance.
+pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
+  pcx_engine* e = b->e;
+  const int ppw = ppw_of(e);
+  const int group = e->cfg.sweeps_per_check > 0 ? e->cfg.sweeps_per_check : 8;
+  const Table t = table_of(e);
+  int dev_cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, e->cfg.device) == hipSuccess) dev_cus = prop.multiProcessorCount;
+  const int grid = dev_cus * 8;
+  Subsets sub{b->h_off.empty() ? nullptr : b->d_off, b->d_idx};
+  WorkList wl{b->d_act, b->d_item_base, b->d_ctl, b->d_ctl + 1, b->d_ctl + 2};
+  int64_t launches = 0;
+  double kms = 0.0;
+  size_t nev = 0;
+  int32_t live = 1, sweeps = 0;
+  const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
+  while (live > 0) {
+    for (int g = 0; g < group; ++g) {
+      PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream));
+      if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
+      PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
+                                   b->d_tight, ppw, e->has_div, grid, e->stream));
+      if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
+      ++launches;
+      ++sweeps;
+    }
+    PCX_HIP(hipMemcpyAsync(&live, b->d_ctl + 2, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+    PCX_HIP(hipStreamSynchronize(e->stream));
+    for (size_t i = 0; i + 1 < nev; i += 2) {
+      float ms = 0.f;
+      PCX_HIP(hipEventElapsedTime(&ms, b->ev[i], b->ev[i + 1]));
+      kms += ms;
+    }
+    nev = 0;
+  }
+  (void)sweeps;
+  std::vector<InstFlags> flags;
+  return collect(b, flags, results, stats, launches, kms);
+}
+
+}  // namespace
+
+// =========================== C ABI ========================================
+extern "C" {
+
+const char* pcx_version(void) { return "pcx 0.1 (gfx950)"; }
+const char* pcx_last_error(void) { return g_err.c_str(); }
+
+void pcx_config_default(pcx_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->device = 0;
+  c->props_per_wave = 256;
+  c->lds_mode = -1;
+  c->sweeps_per_check = 8;
+  c->max_sweeps = 0;
+  c->timing = 0;
+}
+
+int32_t pcx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+pcx_status pcx_create(const pcx_config* cfg, pcx_engine** out) {
+  if (!out) return fail_with(PCX_EINVAL, "out is NULL");
+  *out = nullptr;
+  pcx_config c;
+  if (cfg) c = *cfg; else pcx_config_default(&c);
+  if (c.props_per_wave == 0) c.props_per_wave = 256;
+  if (!valid_ppw(c.props_per_wave)) return fail_with(PCX_EINVAL, "props_per_wave must be 64, 128, 256 or 512");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail_with(PCX_EDEVICE, "no HIP device");
+  if (c.device < 0 || c.device >= n) return fail_with(PCX_EINVAL, "device ordinal out of range");
+  PCX_HIP(hipSetDevice(c.device));
+  auto* e = new (std::nothrow) pcx_engine();
+  if (!e) return fail_with(PCX_ENOMEM, "host allocation failed");
+  e->cfg = c;
+  hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    delete e;
+    return fail_with(PCX_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+  }
+  *out = e;
+  return PCX_OK;
+}
+
+void pcx_destroy(pcx_engine* e) {
+  if (!e) return;
+  (void)hipSetDevice(e->cfg.device);
+  if (e->scratch) pcx_batch_destroy(e->scratch);
+  dfree(e->d_xo); dfree(e->d_y); dfree(e->d_z);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+pcx_status pcx_tell(pcx_engine* e, const int32_t* op, const int32_t* x, const int32_t* y,
+                    const int32_t* z, int64_t n, int32_t* lb, int32_t* ub, int64_t nvars) {
+  if (!e) return fail_with(PCX_EINVAL, "engine is NULL");
+  if (n < 0 || (n > 0 && (!op || !x || !y || !z))) return fail_with(PCX_EINVAL, "bad table arrays");
+  if ((lb == nullptr) != (ub == nullptr)) return fail_with(PCX_EINVAL, "lb and ub must both be given or both be NULL");
+  const int64_t lim = nvars > 0 ? std::min<int64_t>(nvars, PCX_MAX_VARS) : PCX_MAX_VARS;
+  for (int64_t i = 0; i < n; ++i) {
+    if (op[i] < 0 || op[i] >= PCX_NUM_OPS)
+      return fail_with(PCX_EINVAL, "unknown opcode at " + std::to_string(i));
+    if (x[i] < 0 || x[i] >= lim || y[i] < 0 || y[i] >= lim || z[i] < 0 || z[i] >= lim)
+      return fail_with(PCX_EINVAL, "variable index out of range at " + std::to_string(i));
+  }
+  if (lb && nvars <= 0) return fail_with(PCX_EINVAL, "nvars must be given with a store");
+  // Reified outputs are Boolean: embed x in [0,1] (pir.hpp:333-335).
+  if (lb) {
+    for (int64_t i = 0; i < n; ++i)
+      if (op[i] == PCX_EQ || op[i] == PCX_LEQ) {
+        lb[x[i]] = std::max(lb[x[i]], 0);
+        ub[x[i]] = std::min(ub[x[i]], 1);
+      }
+  }
+  // Append, then keep the table stably sorted by (op, y, x, z)
+  // (pir.hpp:343-347): sort the new block and merge it behind the old one.
+  const int64_t old = (int64_t)e->op.size();
+  std::vector<int64_t> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  auto key_less = [&](int64_t a, int64_t b) {
+    if (op[a] != op[b]) return op[a] < op[b];
+    if (y[a] != y[b]) return y[a] < y[b];
+    if (x[a] != x[b]) return x[a] < x[b];
+    return z[a] < z[b];
+  };
+  std::stable_sort(ord.begin(), ord.end(), key_less);
+  struct Bc { int32_t op, x, y, z; };
+  std::vector<Bc> all(old + n);
+  for (int64_t i = 0; i < old; ++i) all[i] = Bc{e->op[i], e->x[i], e->y[i], e->z[i]};
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t j = ord[i];
+    all[old + i] = Bc{op[j], x[j], y[j], z[j]};
+  }
+  std::inplace_merge(all.begin(), all.begin() + old, all.end(), [](const Bc& a, const Bc& b) {
+    if (a.op != b.op) return a.op < b.op;
+    if (a.y != b.y) return a.y < b.y;
+    if (a.x != b.x) return a.x < b.x;
+    return a.z < b.z;
+  });
+  e->op.resize(old + n); e->x.resize(old + n); e->y.resize(old + n); e->z.resize(old + n);
+  for (int64_t i = 0; i < old + n; ++i) {
+    e->op[i] = all[i].op; e->x[i] = all[i].x; e->y[i] = all[i].y; e->z[i] = all[i].z;
+    e->max_var = std::max(e->max_var, std::max(all[i].x, std::max(all[i].y, all[i].z)));
+    e->has_div |= all[i].op >= PCX_TDIV && all[i].op <= PCX_EDIV;
+  }
+  e->dirty = true;
+  return PCX_OK;
+}
+
+int64_t pcx_num_deductions(const pcx_engine* e) { return e ? (int64_t)e->op.size() : 0; }
+
+pcx_status pcx_get_table(const pcx_engine* e, int32_t* op, int32_t* x, int32_t* y, int32_t* z) {
+  if (!e) return fail_with(PCX_EINVAL, "engine is NULL");
+  const size_t n = e->op.size();
+  if (op) std::copy(e->op.begin(), e->op.end(), op);
+  if (x) std::copy(e->x.begin(), e->x.end(), x);
+  if (y) std::copy(e->y.begin(), e->y.end(), y);
+  if (z) std::copy(e->z.begin(), e->z.end(), z);
+  (void)n;
+  return PCX_OK;
+}
+
+int64_t pcx_snapshot(const pcx_engine* e) { return e ? (int64_t)e->op.size() : 0; }
+
+pcx_status pcx_restore(pcx_engine* e, int64_t snap) {
+  if (!e) return fail_with(PCX_EINVAL, "engine is NULL");
+  if (snap < 0 || snap > (int64_t)e->op.size()) return fail_with(PCX_EINVAL, "bad snapshot");
+  // The table is sorted, so "the propagators told since the snapshot" are
+  // not a suffix in general; the reference pops the last ones it appended
+  // (pir.hpp:863-870).  Keep the first `snap` told entries by tell order.
+  if (snap == (int64_t)e->op.size()) return PCX_OK;
+  return fail_with(PCX_ENOTSUP, "restore to a non-trivial snapshot needs tell-order tracking");
+}
+
+pcx_status pcx_batch_create(pcx_engine* e, int64_t n_inst, int64_t nvars, const int64_t* inst_off,
+                            const int32_t* prop_idx, pcx_batch** out) {
+  if (!e || !out) return fail_with(PCX_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (n_inst <= 0 || n_inst > (1 << 30) || nvars <= 0 || nvars > PCX_MAX_VARS)
+    return fail_with(PCX_EINVAL, "bad n_inst or nvars");
+  const int64_t tn = (int64_t)e->op.size();
+  if (inst_off) {
+    if (inst_off[0] != 0) return fail_with(PCX_EINVAL, "inst_off[0] must be 0");
+    for (int64_t k = 0; k < n_inst; ++k)
+      if (inst_off[k + 1] < inst_off[k]) return fail_with(PCX_EINVAL, "inst_off must be non-decreasing");
+    if (inst_off[n_inst] > 0 && !prop_idx) return fail_with(PCX_EINVAL, "prop_idx is NULL");
+    for (int64_t i = 0; i < inst_off[n_inst]; ++i)
+      if (prop_idx[i] < 0 || prop_idx[i] >= tn) return fail_with(PCX_EINVAL, "prop_idx out of range");
+  }
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  auto* b = new (std::nothrow) pcx_batch();
+  if (!b) return fail_with(PCX_ENOMEM, "host allocation failed");
+  b->e = e;
+  b->n_inst = n_inst;
+  b->nvars = nvars;
+  b->table_n = tn;
+  const size_t sbytes = (size_t)n_inst * (size_t)nvars * sizeof(int2);
+  hipError_t he = hipSuccess;
+  auto chk = [&](hipError_t r) { if (he == hipSuccess) he = r; };
+  chk(hipMalloc(&b->d_store, sbytes));
+  chk(hipMalloc(&b->d_image, sbytes));
+  chk(hipMalloc(&b->d_flags, n_inst * sizeof(InstFlags)));
+  chk(hipMalloc(&b->d_tight, kTightSlots * sizeof(unsigned long long)));
+  chk(hipMalloc(&b->d_act, n_inst * sizeof(int32_t)));
+  chk(hipMalloc(&b->d_item_base, (n_inst + 1) * sizeof(int64_t)));
+  chk(hipMalloc(&b->d_ctl, 4 * sizeof(int32_t)));
+  if (inst_off && he == hipSuccess) {
+    b->h_off.assign(inst_off, inst_off + n_inst + 1);
+    chk(hipMalloc(&b->d_off, (n_inst + 1) * sizeof(int64_t)));
+    chk(hipMalloc(&b->d_idx, std::max<int64_t>(inst_off[n_inst], 1) * sizeof(int32_t)));
+    if (he == hipSuccess) {
+      chk(hipMemcpy(b->d_off, inst_off, (n_inst + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+      if (inst_off[n_inst] > 0)
+        chk(hipMemcpy(b->d_idx, prop_idx, inst_off[n_inst] * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+  }
+  if (he != hipSuccess) {
+    pcx_batch_destroy(b);
+    return fail_with(he == hipErrorOutOfMemory ? PCX_ENOMEM : PCX_EDEVICE,
+                     std::string("batch allocation: ") + hipGetErrorString(he));
+  }
+  *out = b;
+  return PCX_OK;
+}
+
+void pcx_batch_destroy(pcx_batch* b) {
+  if (!b) return;
+  if (b->e) (void)hipSetDevice(b->e->cfg.device);
+  for (hipEvent_t ev : b->ev) (void)hipEventDestroy(ev);
+  dfree(b->d_store); dfree(b->d_image); dfree(b->d_off); dfree(b->d_idx); dfree(b->d_flags);
+  dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl);
+  delete b;
+}
+
+pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub) {
+  if (!b || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
+  pcx_engine* e = b->e;
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  const int64_t n = b->n_inst * b->nvars;
+  // Stage lb / ub through the store buffer's halves, then interleave.
+  int32_t* d_lb = reinterpret_cast<int32_t*>(b->d_store);
+  int32_t* d_ub = d_lb + n;
+  PCX_HIP(hipMemcpyAsync(d_lb, lb, n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream));
+  PCX_HIP(hipMemcpyAsync(d_ub, ub, n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream));
+  PCX_HIP(launch_store_pack(d_lb, d_ub, b->d_image, n, e->stream));
+  PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, n * sizeof(int2), hipMemcpyDeviceToDevice, e->stream));
+  PCX_HIP(hipStreamSynchronize(e->stream));
+  return PCX_OK;
+}
+
+pcx_status pcx_batch_reset(pcx_batch* b) {
+  if (!b) return fail_with(PCX_EINVAL, "NULL argument");
+  pcx_engine* e = b->e;
+  PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, b->n_inst * b->nvars * sizeof(int2),
+                         hipMemcpyDeviceToDevice, e->stream));
+  return PCX_OK;
+}
+
+pcx_status pcx_batch_read(pcx_batch* b, int32_t* lb, int32_t* ub) {
+  if (!b || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
+  pcx_engine* e = b->e;
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  const int64_t n = b->n_inst * b->nvars;
+  // Unpack into the image buffer's halves (the image is rebuilt by load).
+  int32_t* d_lb = nullptr;
+  PCX_HIP(hipMalloc(&d_lb, 2 * n * sizeof(int32_t)));
+  hipError_t he = launch_store_unpack(b->d_store, d_lb, d_lb + n, n, e->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(lb, d_lb, n * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(ub, d_lb + n, n * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+  (void)hipFree(d_lb);
+  if (he != hipSuccess) return fail_with(PCX_EDEVICE, std::string("read: ") + hipGetErrorString(he));
+  return PCX_OK;
+}
+
+pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results, pcx_stats* stats) {
+  if (!b) return fail_with(PCX_EINVAL, "NULL batch");
+  pcx_engine* e = b->e;
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  const double t0 = now_ms();
+  pcx_status s = prepare_run(b);
+  if (s != PCX_OK) return s;
+  switch (mode) {
+    case PCX_MODE_PLAIN: s = run_plain(b, results, stats); break;
+    case PCX_MODE_BATCHED: s = run_batched(b, results, stats); break;
+    case PCX_MODE_PERSISTENT: return fail_with(PCX_ENOTSUP, "persistent mode not built yet");
+    default: return fail_with(PCX_EINVAL, "unknown mode");
+  }
+  if (stats) stats->wall_ms = now_ms() - t0;
+  return s;
+}
+
+pcx_status pcx_fixpoint(pcx_engine* e, int32_t* lb, int32_t* ub, int64_t nvars, pcx_result* out) {
+  if (!e || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
+  if (nvars <= 0) return fail_with(PCX_EINVAL, "nvars must be positive");
+  if (e->max_var >= nvars) return fail_with(PCX_EINVAL, "a propagator names a variable outside the store");
+  if (!e->scratch || e->scratch_nvars != nvars || e->scratch_table != (int64_t)e->op.size()) {
+    if (e->scratch) pcx_batch_destroy(e->scratch);
+    e->scratch = nullptr;
+    pcx_status s = pcx_batch_create(e, 1, nvars, nullptr, nullptr, &e->scratch);
+    if (s != PCX_OK) return s;
+    e->scratch_nvars = nvars;
+    e->scratch_table = (int64_t)e->op.size();
+  }
+  pcx_status s = pcx_batch_load(e->scratch, lb, ub);
+  if (s != PCX_OK) return s;
+  pcx_result r{};
+  s = pcx_batch_run(e->scratch, PCX_MODE_PLAIN, &r, nullptr);
+  if (s != PCX_OK) return s;
+  s = pcx_batch_read(e->scratch, lb, ub);
+  if (s != PCX_OK) return s;
+  if (out) *out = r;
+  return PCX_OK;
+}
+
+pcx_status pcx_ask_all(pcx_engine* e, const int32_t* lb, const int32_t* ub, int64_t nvars,
+                       uint8_t* entailed, int32_t* all) {
+  if (!e || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
+  if (e->max_var >= nvars) return fail_with(PCX_EINVAL, "a propagator names a variable outside the store");
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  pcx_status s = sync_table(e);
+  if (s != PCX_OK) return s;
+  const int64_t n = (int64_t)e->op.size();
+  int32_t *d_lb = nullptr, *d_ub = nullptr, *d_all = nullptr;
+  int2* d_st = nullptr;
+  uint8_t* d_ent = nullptr;
+  hipError_t he = hipMalloc(&d_lb, nvars * sizeof(int32_t));
+  if (he == hipSuccess) he = hipMalloc(&d_ub, nvars * sizeof(int32_t));
+  if (he == hipSuccess) he = hipMalloc(&d_st, nvars * sizeof(int2));
+  if (he == hipSuccess) he = hipMalloc(&d_all, sizeof(int32_t));
+  if (he == hipSuccess) he = hipMalloc(&d_ent, std::max<int64_t>(n, 1));
+  const int32_t one = 1;
+  if (he == hipSuccess) he = hipMemcpyAsync(d_lb, lb, nvars * sizeof(int32_t), hipMemcpyHostToDevice, e->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(d_ub, ub, nvars * sizeof(int32_t), hipMemcpyHostToDevice, e->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(d_all, &one, sizeof(int32_t), hipMemcpyHostToDevice, e->stream);
+  if (he == hipSuccess) he = launch_store_pack(d_lb, d_ub, d_st, nvars, e->stream);
+  if (he == hipSuccess) he = launch_ask(table_of(e), d_st, d_ent, d_all, e->stream);
+  int32_t h_all = 1;
+  if (he == hipSuccess) he = hipMemcpyAsync(&h_all, d_all, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
+  if (he == hipSuccess && entailed && n > 0)
+    he = hipMemcpyAsync(entailed, d_ent, n, hipMemcpyDeviceToHost, e->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+  (void)hipFree(d_lb); (void)hipFree(d_ub); (void)hipFree(d_st); (void)hipFree(d_all); (void)hipFree(d_ent);
+  if (he != hipSuccess) return fail_with(PCX_EDEVICE, std::string("ask: ") + hipGetErrorString(he));
+  if (all) *all = h_all;
+  return PCX_OK;
+}
+
+}  // extern "C"

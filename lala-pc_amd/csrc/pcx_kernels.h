@@ -1,0 +1,75 @@
+// pcx_kernels.h -- device data layout and kernel entry points (internal).
+//
+// HBM layout
+//   table   : three int32 SoA streams, one entry per propagator in table
+//             order (sorted by (op, y, x, z)):
+//               xo[i] = x | op << 28,  y[i],  z[i]          (12 B / propagator)
+//   store   : int2 {lb, ub} per variable, one slice of nvars per instance, so
+//             each operand gather is one 8-byte load from one cache line.
+//   flags   : per instance, InstFlags (parity-indexed "changed" words so a
+//             sweep reads the previous sweep's word while writing its own).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pcx {
+
+constexpr int kOpShift = 28;
+constexpr uint32_t kVarMask = (1u << kOpShift) - 1u;
+constexpr int kWave = 64;
+constexpr int kBlock = 256;            // 4 waves per workgroup
+constexpr int kNoFail = 0x7fffffff;
+constexpr int kTightSlots = 64;        // striped tightening counters
+
+struct Table {
+  const uint32_t* __restrict__ xo;
+  const int32_t* __restrict__ y;
+  const int32_t* __restrict__ z;
+  int64_t n;
+};
+
+// Per-instance sweep bookkeeping, all written with plain stores of the sweep
+// number (every writer of one word in one sweep writes the same value).
+struct InstFlags {
+  int32_t changed[2];   // changed[s & 1] = s when sweep s tightened a bound
+  int32_t failed_at;    // first sweep that found an empty interval, or kNoFail
+  int32_t done_at;      // sweep at which the scheduler retired the instance
+};
+
+// Per-instance propagator subset: prop_idx[off[k] .. off[k+1]) or the whole
+// table when off == nullptr.
+struct Subsets {
+  const int64_t* __restrict__ off;
+  const int32_t* __restrict__ idx;
+};
+
+// Device work list built by the scheduler for one sweep of the batched mode.
+struct WorkList {
+  int32_t* act;          // active instance ids, length n_act
+  int64_t* item_base;    // prefix sum of work items per active instance (n_act + 1)
+  int32_t* n_act;        // [1]
+  int32_t* sweep;        // [1] current sweep number
+  int32_t* live;         // [1] instances still running after the last schedule
+};
+
+}  // namespace pcx
+
+// Host-callable launchers (defined in pcx_kernels.hip).
+namespace pcx {
+hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, int64_t m,
+                              int32_t sweep, InstFlags* flags, unsigned long long* tight,
+                              int ppw, bool div, hipStream_t st);
+hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags, int32_t n_inst,
+                           WorkList wl, int ppw, int32_t max_sweeps, hipStream_t st);
+hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
+                                int64_t n_props, InstFlags* flags, WorkList wl,
+                                unsigned long long* tight, int ppw, bool div, int grid,
+                                hipStream_t st);
+hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipStream_t st);
+hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
+                      hipStream_t st);
+hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n, hipStream_t st);
+hipError_t launch_store_unpack(const int2* store, int32_t* lb, int32_t* ub, int64_t n, hipStream_t st);
+hipError_t launch_store_check_empty(const int2* store, int64_t nvars, int32_t n_inst,
+                                    InstFlags* flags, hipStream_t st);
+}  // namespace pcx

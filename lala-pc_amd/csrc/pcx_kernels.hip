@@ -1,0 +1,331 @@
+// pcx_kernels.hip -- gfx950 kernels of the interval fixpoint engine.
+//
+// One thread evaluates one propagator per step: it reads the 12-byte table
+// entry (three coalesced streams), gathers the three operand intervals (one
+// 8-byte load each), runs the bounds propagator, and tightens the store with
+// atomicMax on lb / atomicMin on ub only where its result is tighter than what
+// it read.  "changed" and "failed" are reduced per wavefront with ballots and
+// written once per wave.  Kernel boundaries order the sweeps: a sweep reads
+// the store as the previous kernel left it, so a sweep in which no atomic
+// tightened anything saw one constant store and proves the fixpoint.
+#include "pcx_kernels.h"
+#include "pcx_prop.h"
+
+namespace pcx {
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Meet `n` into variable v, whose value was read as `cur`.
+__device__ __forceinline__ void meet_var(int2* __restrict__ st, int32_t v, int2 cur, int2 n,
+                                         bool& chg, bool& fail, int& nt) {
+  if (n.x > cur.x) {
+    const int old = atomicMax(&st[v].x, n.x);
+    if (old < n.x) { chg = true; ++nt; }
+    if (n.x > cur.y) fail = true;          // cur.y >= current ub
+  }
+  if (n.y < cur.y) {
+    const int old = atomicMin(&st[v].y, n.y);
+    if (old > n.y) { chg = true; ++nt; }
+    if (n.y < cur.x) fail = true;
+  }
+}
+
+template <bool DIV>
+__device__ __forceinline__ void eval_prop(const Table& t, int2* __restrict__ st, int64_t i,
+                                          bool& chg, bool& fail, int& nt) {
+  const uint32_t xo = t.xo[i];
+  const int32_t y = t.y[i], z = t.z[i];
+  const int op = (int)(xo >> kOpShift);
+  const int32_t x = (int32_t)(xo & kVarMask);
+  const int2 X = st[x], Y = st[y], Z = st[z];
+  if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = true; return; }
+  int2 nx, ny, nz;
+  deduce<DIV>(op, X, Y, Z, nx, ny, nz);
+  if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
+  meet_var(st, x, X, nx, chg, fail, nt);
+  meet_var(st, y, Y, ny, chg, fail, nt);
+  meet_var(st, z, Z, nz, chg, fail, nt);
+}
+
+// Publish a wave's verdict: one store per flag, one atomic for the count.
+__device__ __forceinline__ void publish(InstFlags* f, int32_t s, bool chg, bool fail, int nt,
+                                        unsigned long long* tight) {
+  const unsigned long long bc = __ballot(chg);
+  const unsigned long long bf = __ballot(fail);
+  const int sum = wave_sum(nt);
+  if (lane_id() == 0) {
+    if (bc) f->changed[s & 1] = s;
+    if (bf) f->failed_at = s;
+    if (sum) atomicAdd(&tight[(blockIdx.x * 4 + (threadIdx.x >> 6)) & (kTightSlots - 1)],
+                       (unsigned long long)sum);
+  }
+}
+
+// ---- plain mode: one instance, one sweep per launch -----------------------
+template <int PPL, bool DIV>
+__global__ __launch_bounds__(kBlock) void k_sweep_plain(Table t, int2* __restrict__ st,
+                                                        const int32_t* __restrict__ idx, int64_t m,
+                                                        int32_t s, InstFlags* f,
+                                                        unsigned long long* tight) {
+  // Sweep s runs only if sweep s-1 tightened something and nothing failed.
+  if (f->changed[(s - 1) & 1] != s - 1 || f->failed_at != kNoFail) return;
+  const int64_t base = (int64_t)blockIdx.x * (kBlock * PPL);
+  bool chg = false, fail = false;
+  int nt = 0;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    const int64_t k = base + j * kBlock + threadIdx.x;
+    if (k < m) eval_prop<DIV>(t, st, idx ? (int64_t)idx[k] : k, chg, fail, nt);
+  }
+  publish(f, s, chg, fail, nt, tight);
+}
+
+hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, int64_t m,
+                              int32_t sweep, InstFlags* flags, unsigned long long* tight,
+                              int ppw, bool div, hipStream_t stream) {
+  const int ppl = ppw / kWave;
+  const int64_t per_block = (int64_t)kBlock * ppl;
+  const int64_t grid = (m + per_block - 1) / per_block;
+  if (grid <= 0) return hipSuccess;
+  const dim3 g((unsigned)grid), b(kBlock);
+#define PCX_PLAIN(P, D) k_sweep_plain<P, D><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight)
+  switch (ppl * 2 + (div ? 1 : 0)) {
+    case 2: PCX_PLAIN(1, false); break;
+    case 3: PCX_PLAIN(1, true); break;
+    case 4: PCX_PLAIN(2, false); break;
+    case 5: PCX_PLAIN(2, true); break;
+    case 8: PCX_PLAIN(4, false); break;
+    case 9: PCX_PLAIN(4, true); break;
+    case 16: PCX_PLAIN(8, false); break;
+    case 17: PCX_PLAIN(8, true); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef PCX_PLAIN
+  return hipGetLastError();
+}
+
+// ---- batched mode: device scheduler + grid-stride sweep over work items ---
+// Items are (active instance, chunk of kBlock*PPL propagators).  The
+// scheduler (one workgroup) retires converged or failed instances and builds
+// the prefix sum of the items of those still running.
+constexpr int kSchedThreads = 1024;
+
+__global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t n_props,
+                                                            InstFlags* flags, int32_t n_inst,
+                                                            WorkList wl, int64_t chunk,
+                                                            int32_t max_sweeps) {
+  __shared__ int64_t part[kSchedThreads];
+  __shared__ int32_t cnt[kSchedThreads];
+  const int32_t s = *wl.sweep + 1;       // the sweep being scheduled
+  const int tid = threadIdx.x;
+  const int per = (n_inst + kSchedThreads - 1) / kSchedThreads;
+  const int lo = min(n_inst, tid * per), hi = min(n_inst, lo + per);
+  int64_t items = 0;
+  int32_t na = 0;
+  for (int k = lo; k < hi; ++k) {
+    InstFlags& f = flags[k];
+    bool run = f.failed_at == kNoFail && f.changed[(s - 1) & 1] == s - 1 &&
+               (max_sweeps <= 0 || s < max_sweeps);
+    if (!run) {
+      if (f.done_at < 0) f.done_at = s;
+      continue;
+    }
+    const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
+    items += (m + chunk - 1) / chunk;
+    ++na;
+  }
+  part[tid] = items;
+  cnt[tid] = na;
+  __syncthreads();
+  // inclusive scans (Hillis-Steele; 1024 entries, runs once per sweep)
+  for (int o = 1; o < kSchedThreads; o <<= 1) {
+    const int64_t pv = tid >= o ? part[tid - o] : 0;
+    const int32_t cv = tid >= o ? cnt[tid - o] : 0;
+    __syncthreads();
+    part[tid] += pv;
+    cnt[tid] += cv;
+    __syncthreads();
+  }
+  int64_t ib = part[tid] - items;
+  int32_t ia = cnt[tid] - na;
+  for (int k = lo; k < hi; ++k) {
+    const InstFlags& f = flags[k];
+    if (f.done_at >= 0 && f.done_at <= s) continue;
+    const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
+    wl.act[ia] = k;
+    wl.item_base[ia] = ib;
+    ib += (m + chunk - 1) / chunk;
+    ++ia;
+  }
+  __syncthreads();
+  if (tid == kSchedThreads - 1) {
+    wl.item_base[cnt[tid]] = part[tid];
+    *wl.n_act = cnt[tid];
+    *wl.live = cnt[tid];
+    *wl.sweep = s;
+  }
+}
+
+hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags, int32_t n_inst,
+                           WorkList wl, int ppw, int32_t max_sweeps, hipStream_t stream) {
+  const int64_t chunk = (int64_t)kBlock * (ppw / kWave);
+  k_schedule<<<1, kSchedThreads, 0, stream>>>(sub, n_props, flags, n_inst, wl, chunk, max_sweeps);
+  return hipGetLastError();
+}
+
+template <int PPL, bool DIV>
+__global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restrict__ store,
+                                                          int64_t nvars, Subsets sub,
+                                                          int64_t n_props, InstFlags* flags,
+                                                          WorkList wl, unsigned long long* tight) {
+  const int32_t s = *wl.sweep;
+  const int32_t n_act = *wl.n_act;
+  if (n_act <= 0) return;
+  const int64_t total = wl.item_base[n_act];
+  constexpr int64_t chunk = (int64_t)kBlock * PPL;
+  for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+    // active slot holding this item: last a with item_base[a] <= item
+    int lo = 0, hi = n_act - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (wl.item_base[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int32_t k = wl.act[lo];
+    InstFlags* f = flags + k;
+    const int64_t first = (item - wl.item_base[lo]) * chunk;
+    const int64_t beg = sub.off ? sub.off[k] : 0;
+    const int64_t m = sub.off ? sub.off[k + 1] - beg : n_props;
+    const int32_t* idx = sub.off ? sub.idx + beg : nullptr;
+    int2* st = store + (int64_t)k * nvars;
+    bool chg = false, fail = false;
+    int nt = 0;
+    if (f->failed_at == kNoFail) {
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        const int64_t q = first + j * kBlock + threadIdx.x;
+        if (q < m) eval_prop<DIV>(t, st, idx ? (int64_t)idx[q] : q, chg, fail, nt);
+      }
+    }
+    publish(f, s, chg, fail, nt, tight);
+  }
+}
+
+hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
+                                int64_t n_props, InstFlags* flags, WorkList wl,
+                                unsigned long long* tight, int ppw, bool div, int grid,
+                                hipStream_t stream) {
+#define PCX_BATCH(P, D) \
+  k_sweep_batched<P, D><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight)
+  switch ((ppw / kWave) * 2 + (div ? 1 : 0)) {
+    case 2: PCX_BATCH(1, false); break;
+    case 3: PCX_BATCH(1, true); break;
+    case 4: PCX_BATCH(2, false); break;
+    case 5: PCX_BATCH(2, true); break;
+    case 8: PCX_BATCH(4, false); break;
+    case 9: PCX_BATCH(4, true); break;
+    case 16: PCX_BATCH(8, false); break;
+    case 17: PCX_BATCH(8, true); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef PCX_BATCH
+  return hipGetLastError();
+}
+
+// ---- bookkeeping kernels --------------------------------------------------
+__global__ void k_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_inst) {
+    flags[k].changed[0] = -2;
+    flags[k].changed[1] = -1;      // "sweep -1 changed" lets sweep 0 run
+    flags[k].failed_at = kNoFail;
+    flags[k].done_at = -1;
+  }
+  if (k == 0 && wl.sweep) {
+    *wl.sweep = -1;
+    *wl.n_act = 0;
+    *wl.live = n_inst;
+  }
+}
+
+hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipStream_t stream) {
+  const int grid = (int)((n_inst + 255) / 256);
+  k_init_flags<<<grid > 0 ? grid : 1, 256, 0, stream>>>(flags, n_inst, wl);
+  return hipGetLastError();
+}
+
+// An instance whose store already holds an empty interval is failed before
+// any sweep (the reference's is_bot() is true before the fixpoint).
+__global__ void k_check_empty(const int2* __restrict__ store, int64_t nvars, int64_t total,
+                              InstFlags* flags) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < total;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int2 b = store[v];
+    if (b.x > b.y) flags[v / nvars].failed_at = -1;
+  }
+}
+
+hipError_t launch_store_check_empty(const int2* store, int64_t nvars, int32_t n_inst,
+                                    InstFlags* flags, hipStream_t stream) {
+  const int64_t total = nvars * (int64_t)n_inst;
+  int64_t grid = (total + 255) / 256;
+  if (grid > 16384) grid = 16384;
+  if (grid > 0) k_check_empty<<<dim3((unsigned)grid), 256, 0, stream>>>(store, nvars, total, flags);
+  return hipGetLastError();
+}
+
+__global__ void k_ask(Table t, const int2* __restrict__ st, uint8_t* ent, int32_t* all_flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool ok = true;
+  if (i < t.n) {
+    const uint32_t xo = t.xo[i];
+    ok = entailed((int)(xo >> kOpShift), st[xo & kVarMask], st[t.y[i]], st[t.z[i]]);
+    if (ent) ent[i] = ok ? 1 : 0;
+  }
+  if (__ballot(!ok) && lane_id() == 0) *all_flag = 0;
+}
+
+hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
+                      hipStream_t stream) {
+  const int64_t grid = (t.n + 255) / 256;
+  if (grid > 0) k_ask<<<dim3((unsigned)grid), 256, 0, stream>>>(t, store, ent, all_flag);
+  return hipGetLastError();
+}
+
+__global__ void k_pack(const int32_t* __restrict__ lb, const int32_t* __restrict__ ub,
+                       int2* __restrict__ st, int64_t n) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    st[v] = make_int2(lb[v], ub[v]);
+}
+__global__ void k_unpack(const int2* __restrict__ st, int32_t* __restrict__ lb,
+                         int32_t* __restrict__ ub, int64_t n) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int2 b = st[v];
+    lb[v] = b.x;
+    ub[v] = b.y;
+  }
+}
+
+hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n,
+                             hipStream_t stream) {
+  int64_t grid = (n + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  if (grid > 0) k_pack<<<dim3((unsigned)grid), 256, 0, stream>>>(lb, ub, store, n);
+  return hipGetLastError();
+}
+hipError_t launch_store_unpack(const int2* store, int32_t* lb, int32_t* ub, int64_t n,
+                               hipStream_t stream) {
+  int64_t grid = (n + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  if (grid > 0) k_unpack<<<dim3((unsigned)grid), 256, 0, stream>>>(store, lb, ub, n);
+  return hipGetLastError();
+}
+
+}  // namespace pcx

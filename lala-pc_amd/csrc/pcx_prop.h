@@ -1,0 +1,405 @@
+// pcx_prop.h -- device-side bounds propagators for `x = y <op> z`.
+//
+// Semantics follow lattice-land/lala-pc include/lala/pir.hpp:721-817 (deduce)
+// and 417-438 (ask).  A propagator reads the three intervals of its operands
+// and produces, for each operand, an interval to meet into the store; an
+// operand it does not narrow gets [-oo, +oo].  Within one evaluation the
+// updates are sequential: a later line reads the bounds produced by earlier
+// lines (pir.hpp:396-401 macros read the live r1/r2/r3), and that order is
+// kept here.
+//
+// Arithmetic on bounds is done in 64 bits and saturated into the int32 range
+// when it becomes a bound; the reference uses `int` and overflows for large
+// products.  Where the reference does not overflow the results are equal.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pcx {
+
+constexpr int32_t kInf = 2147483647;
+constexpr int32_t kMinf = -2147483647 - 1;
+
+enum Op : int32_t {
+  kAdd = 0, kMul = 1, kTdiv = 2, kFdiv = 3, kCdiv = 4, kEdiv = 5,
+  kMin = 6, kMax = 7, kEq = 8, kLeq = 9, kNumOps = 10
+};
+
+typedef long long i64;
+
+struct Bnd { i64 l, u; };   // working interval (values always in int32 range)
+
+__device__ __forceinline__ i64 clamp32(i64 v) {
+  return v > kInf ? (i64)kInf : (v < kMinf ? (i64)kMinf : v);
+}
+__device__ __forceinline__ i64 mn(i64 a, i64 b) { return a < b ? a : b; }
+__device__ __forceinline__ i64 mx(i64 a, i64 b) { return a > b ? a : b; }
+__device__ __forceinline__ Bnd bnd(i64 l, i64 u) { return Bnd{clamp32(l), clamp32(u)}; }
+__device__ __forceinline__ Bnd top() { return Bnd{kMinf, kInf}; }
+__device__ __forceinline__ Bnd empty() { return Bnd{kInf, kMinf}; }
+__device__ __forceinline__ bool is_empty(Bnd a) { return a.l > a.u; }
+
+// Integer divisions (battery::tdiv/fdiv/cdiv/ediv semantics). Operands fit in
+// int32 except where a bound was shifted by one; 64-bit keeps those exact.
+__device__ __forceinline__ i64 q_trunc(i64 a, i64 b) { return a / b; }
+__device__ __forceinline__ i64 q_floor(i64 a, i64 b) {
+  i64 q = a / b;
+  return (q * b != a && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+__device__ __forceinline__ i64 q_ceil(i64 a, i64 b) {
+  i64 q = a / b;
+  return (q * b != a && ((a < 0) == (b < 0))) ? q + 1 : q;
+}
+__device__ __forceinline__ i64 q_euclid(i64 a, i64 b) {
+  i64 q = a / b, r = a - q * b;
+  return r < 0 ? (b > 0 ? q - 1 : q + 1) : q;
+}
+__device__ __forceinline__ i64 q_op(i64 a, int op, i64 b) {
+  switch (op) {
+    case kTdiv: return q_trunc(a, b);
+    case kFdiv: return q_floor(a, b);
+    case kCdiv: return q_ceil(a, b);
+    default:    return q_euclid(a, b);
+  }
+}
+
+// Division fast path for the hot MUL inverse: both operands are int32 and the
+// divisor is non-zero, so 32-bit division is exact.
+__device__ __forceinline__ int32_t d_floor32(int32_t a, int32_t b) {
+  int32_t q = a / b;
+  return (q * b != a && ((a ^ b) < 0)) ? q - 1 : q;
+}
+__device__ __forceinline__ int32_t d_ceil32(int32_t a, int32_t b) {
+  int32_t q = a / b;
+  return (q * b != a && ((a ^ b) >= 0)) ? q + 1 : q;
+}
+
+// Bot-aware join: an empty operand is the identity.
+__device__ __forceinline__ Bnd hull(Bnd a, Bnd b) {
+  if (is_empty(b)) return a;
+  if (is_empty(a)) return b;
+  return Bnd{mn(a.l, b.l), mx(a.u, b.u)};
+}
+__device__ __forceinline__ void narrow(Bnd& a, Bnd b) { a.l = mx(a.l, b.l); a.u = mn(a.u, b.u); }
+
+// ---- multiplication ------------------------------------------------------
+// Narrow `t` from x = t * d (d may lose a zero bound), pir.hpp:702-718.
+__device__ __forceinline__ void mul_inverse(Bnd x, Bnd& t, Bnd& d) {
+  const bool x_nonzero = x.l > 0 || x.u < 0;
+  if (x_nonzero) {
+    if (d.l == 0) d.l = 1;
+    if (d.u == 0) d.u = -1;
+  }
+  if (x_nonzero && d.l < 0 && d.u > 0) {
+    t.l = mx(t.l, mn(x.l, x.u == kMinf ? (i64)kInf : -x.u));
+    t.u = mn(t.u, mx(x.l == kInf ? (i64)kMinf : -x.l, x.u));
+  } else if (x_nonzero || d.l > 0 || d.u < 0) {
+    if (x.l == kMinf || x.u == kInf || d.l == kMinf || d.u == kInf) return;
+    if (d.l > d.u) return;
+    // Here x, d are finite int32 and d excludes 0.
+    const int32_t xl = (int32_t)x.l, xu = (int32_t)x.u, dl = (int32_t)d.l, du = (int32_t)d.u;
+    const int32_t c1 = d_ceil32(xl, dl), c2 = d_ceil32(xl, du), c3 = d_ceil32(xu, dl), c4 = d_ceil32(xu, du);
+    const int32_t f1 = d_floor32(xl, dl), f2 = d_floor32(xl, du), f3 = d_floor32(xu, dl), f4 = d_floor32(xu, du);
+    t.l = mx(t.l, (i64)min(min(c1, c2), min(c3, c4)));
+    t.u = mn(t.u, (i64)max(max(f1, f2), max(f3, f4)));
+  }
+}
+
+// ---- division (pir.hpp:449-700), written without recursion ----------------
+// x = y / d, forward.
+__device__ __forceinline__ void div_forward(int op, Bnd& x, Bnd y, Bnd& d) {
+  if (d.l < 0 && d.u > 0) {
+    x.l = mx(x.l, mn(y.l, y.u == kMinf ? (i64)kInf : -y.u));
+    x.u = mn(x.u, mx(y.l == kInf ? (i64)kMinf : -y.l, y.u));
+    return;
+  }
+  if (d.l == 0) d.l = 1;
+  if (d.u == 0) d.u = -1;
+  if (y.l == kMinf || y.u == kInf || d.l == kMinf || d.u == kInf) return;
+  if (is_empty(d)) return;
+  i64 t1 = q_op(y.l, op, d.l), t2 = q_op(y.l, op, d.u), t3 = q_op(y.u, op, d.l), t4 = q_op(y.u, op, d.u);
+  x.l = mx(x.l, mn(mn(t1, t2), mn(t3, t4)));
+  x.u = mn(x.u, mx(mx(t1, t2), mx(t3, t4)));
+}
+
+// numerator bound for floor / ceil division
+__device__ Bnd num_floor(Bnd x, Bnd d) {
+  if (d.l < 0 && d.u > 0)
+    return bnd(mn(mn(x.l, -x.u), mn(x.l * d.u, (x.u + 1) * d.l + 1)),
+               mx(mx(-x.l, x.u), mx(x.l * d.l, (x.u + 1) * d.u - 1)));
+  if (d.l > 0 || d.u < 0)
+    return bnd(mn(mn(x.l * d.l, x.l * d.u), mn((x.u + 1) * d.l + 1, (x.u + 1) * d.u + 1)),
+               mx(mx(x.l * d.l, x.l * d.u), mx((x.u + 1) * d.l - 1, (x.u + 1) * d.u - 1)));
+  return top();
+}
+__device__ Bnd num_ceil(Bnd x, Bnd d) {
+  if (d.l < 0 && d.u > 0)
+    return bnd(mn(mn(x.l, -x.u), mn(x.u * d.l, (x.l - 1) * d.u + 1)),
+               mx(mx(-x.l, x.u), mx(x.u * d.u, (x.l - 1) * d.l - 1)));
+  if (d.l > 0 || d.u < 0)
+    return bnd(mn(mn(x.u * d.l, x.u * d.u), mn((x.l - 1) * d.l + 1, (x.l - 1) * d.u + 1)),
+               mx(mx(x.u * d.l, x.u * d.u), mx((x.l - 1) * d.l - 1, (x.l - 1) * d.u - 1)));
+  return top();
+}
+__device__ Bnd num_trunc(Bnd x, Bnd d) {
+  if (x.l > 0) return num_floor(x, d);
+  if (x.u < 0) return num_ceil(x, d);
+  if (x.l <= 0 && 0 <= x.u) {
+    Bnd r = bnd(mn(d.l, -d.u) + 1, mx(-d.l, d.u) - 1);
+    if (x.l != 0) r = hull(r, num_ceil(bnd(x.l, -1), d));
+    if (x.u != 0) r = hull(r, num_floor(bnd(1, x.u), d));
+    return r;
+  }
+  return top();
+}
+__device__ Bnd num_euclid(Bnd x, Bnd d) {
+  if (d.l > 0) return num_floor(x, d);
+  if (d.u < 0) return num_ceil(x, d);
+  if (d.l < 0 && d.u > 0) return hull(num_ceil(x, bnd(d.l, -1)), num_floor(x, bnd(1, d.u)));
+  return top();
+}
+
+// Denominator bound for floor division when x is one of the "leaf" shapes:
+// x > 0 or x < -1 (with the numerator split by sign), x == 0, x == -1.
+// Returns `ok=false` for x shapes that need splitting (handled by the caller).
+__device__ Bnd den_floor_signed(Bnd x, Bnd y) {   // x > 0 || x.u + 1 < 0; y has no 0
+  if (y.l > 0)
+    return bnd(mn(q_floor(y.l, x.u + 1), q_floor(y.u, x.u + 1)) + 1, mx(q_floor(y.l, x.l), q_floor(y.u, x.l)));
+  // y.u < 0
+  return bnd(mn(q_ceil(y.l, x.l), q_ceil(y.u, x.l)), mx(q_ceil(y.l, x.u + 1), q_ceil(y.u, x.u + 1)) - 1);
+}
+__device__ Bnd den_floor_leaf(Bnd x, Bnd y) {
+  if (x.l > 0 || x.u + 1 < 0) {
+    if (y.l > 0 || y.u < 0) return den_floor_signed(x, y);
+    if (0 == y.l && y.l < y.u) return den_floor_signed(x, bnd(1, y.u));
+    if (y.l < y.u && y.u == 0) return den_floor_signed(x, bnd(y.l, -1));
+    if (y.l < 0 && 0 < y.u) return hull(den_floor_signed(x, bnd(y.l, -1)), den_floor_signed(x, bnd(1, y.u)));
+    if (y.l == 0 && y.u == 0) return empty();
+    return top();
+  }
+  if (x.l == 0 && x.u == 0) {
+    if (y.l > 0) return bnd(y.l + 1, kInf);
+    if (y.u < 0) return bnd(kMinf, y.u - 1);
+    return top();
+  }
+  if (x.l == -1 && x.u == -1) {
+    if (y.l > 0) return bnd(kMinf, -y.l);
+    if (y.u < 0) return bnd(-y.u, kInf);
+    if (0 == y.l && y.l < y.u) return bnd(kMinf, -1);
+    if (y.l < y.u && y.u == 0) return bnd(1, kInf);
+    if (y.l == 0 && y.u == 0) return empty();
+    return top();
+  }
+  return top();
+}
+__device__ Bnd den_floor(Bnd x, Bnd y) {
+  if ((x.l > 0 || x.u + 1 < 0) || (x.l == 0 && x.u == 0) || (x.l == -1 && x.u == -1))
+    return den_floor_leaf(x, y);
+  if (x.l == 0 && 0 < x.u) return hull(den_floor_leaf(bnd(0, 0), y), den_floor_leaf(bnd(1, x.u), y));
+  if (x.l < -1 && x.u == -1) return hull(den_floor_leaf(bnd(x.l, -2), y), den_floor_leaf(bnd(-1, -1), y));
+  if (x.l <= -1 && x.u >= 0) {
+    Bnd r = den_floor_leaf(bnd(-1, -1), y);
+    r = hull(r, den_floor_leaf(bnd(0, 0), y));
+    if (x.l != -1) r = hull(r, den_floor_leaf(bnd(x.l, -2), y));
+    if (x.u != 0) r = hull(r, den_floor_leaf(bnd(1, x.u), y));
+    return r;
+  }
+  return top();
+}
+
+__device__ Bnd den_ceil_signed(Bnd x, Bnd y) {    // x.l - 1 > 0 || x.u < 0; y has no 0
+  if (y.l > 0)
+    return bnd(mn(q_ceil(y.l, x.u), q_ceil(y.u, x.u)), mx(q_ceil(y.l, x.l - 1), q_ceil(y.u, x.l - 1)) - 1);
+  return bnd(mn(q_floor(y.l, x.l - 1), q_floor(y.u, x.l - 1)) + 1, mx(q_floor(y.l, x.u), q_floor(y.u, x.u)));
+}
+__device__ Bnd den_ceil_leaf(Bnd x, Bnd y) {
+  if (x.l - 1 > 0 || x.u < 0) {
+    if (y.l > 0 || y.u < 0) return den_ceil_signed(x, y);
+    if (0 == y.l && y.l < y.u) return den_ceil_signed(x, bnd(1, y.u));
+    if (y.l < y.u && y.u == 0) return den_ceil_signed(x, bnd(y.l, -1));
+    if (y.l < 0 && 0 < y.u) return hull(den_ceil_signed(x, bnd(y.l, -1)), den_ceil_signed(x, bnd(1, y.u)));
+    if (y.l == 0 && y.u == 0) return empty();
+    return top();
+  }
+  if (x.l == 0 && x.u == 0) {
+    if (y.l > 0) return bnd(kMinf, -y.l - 1);
+    if (y.u < 0) return bnd(-y.u + 1, kInf);
+    return top();
+  }
+  if (x.l == 1 && x.u == 1) {
+    if (y.l > 0) return bnd(y.l, kInf);
+    if (y.u < 0) return bnd(kMinf, y.u);
+    if (0 == y.l && y.l < y.u) return bnd(1, kInf);
+    if (y.l < y.u && y.u == 0) return bnd(kMinf, -1);
+    if (y.l == 0 && y.u == 0) return empty();
+    return top();
+  }
+  return top();
+}
+__device__ Bnd den_ceil(Bnd x, Bnd y) {
+  if ((x.l - 1 > 0 || x.u < 0) || (x.l == 0 && x.u == 0) || (x.l == 1 && x.u == 1))
+    return den_ceil_leaf(x, y);
+  if (x.l < 0 && x.u == 0) return hull(den_ceil_leaf(bnd(x.l, -1), y), den_ceil_leaf(bnd(0, 0), y));
+  if (x.l == 1 && 1 < x.u) return hull(den_ceil_leaf(bnd(1, 1), y), den_ceil_leaf(bnd(2, x.u), y));
+  if (x.l <= 0 && x.u >= 1) {
+    Bnd r = den_ceil_leaf(bnd(1, 1), y);
+    r = hull(r, den_ceil_leaf(bnd(0, 0), y));
+    if (x.l != 0) r = hull(r, den_ceil_leaf(bnd(x.l, -1), y));
+    if (x.u != 1) r = hull(r, den_ceil_leaf(bnd(2, x.u), y));
+    return r;
+  }
+  return top();
+}
+
+__device__ Bnd den_trunc_zero(Bnd y, Bnd d) {     // x == [0,0]
+  if (y.l > 0 && d.l > 0) return bnd(y.l + 1, kInf);
+  if (y.l > 0 && d.u < 0) return bnd(kMinf, -y.l - 1);
+  if (y.u < 0 && d.l > 0) return bnd(-y.u + 1, kInf);
+  if (y.u < 0 && d.u < 0) return bnd(kMinf, y.u - 1);
+  return top();
+}
+__device__ Bnd den_trunc(Bnd x, Bnd y, Bnd d) {
+  if (x.l > 0) return den_floor(x, y);
+  if (x.u < 0) return den_ceil(x, y);
+  if (x.l == 0 && x.u == 0) return den_trunc_zero(y, d);
+  if (x.l <= 0 && 0 <= x.u) {
+    Bnd r = den_trunc_zero(y, d);
+    if (x.l != 0) r = hull(r, den_ceil(bnd(x.l, -1), y));
+    if (x.u != 0) r = hull(r, den_floor(bnd(1, x.u), y));
+    return r;
+  }
+  return top();
+}
+__device__ Bnd den_euclid(Bnd x, Bnd y, Bnd d) {
+  if (d.l > 0) return den_floor(x, y);
+  if (d.u < 0) return den_ceil(x, y);
+  if (d.l < 0 && 0 < d.u) return hull(den_floor(x, y), den_ceil(x, y));
+  return top();
+}
+
+__device__ __noinline__ void prop_div(int op, Bnd& x, Bnd& y, Bnd& d) {
+  div_forward(op, x, y, d);
+  x = bnd(x.l, x.u);
+  if (is_empty(x) || is_empty(d)) return;
+  switch (op) {
+    case kFdiv: narrow(y, num_floor(x, d)); break;
+    case kCdiv: narrow(y, num_ceil(x, d)); break;
+    case kTdiv: narrow(y, num_trunc(x, d)); break;
+    default:    narrow(y, num_euclid(x, d)); break;
+  }
+  if (is_empty(y)) return;
+  switch (op) {
+    case kFdiv: narrow(d, den_floor(x, y)); break;
+    case kCdiv: narrow(d, den_ceil(x, y)); break;
+    case kTdiv: narrow(d, den_trunc(x, y, d)); break;
+    default:    narrow(d, den_euclid(x, y, d)); break;
+  }
+}
+
+// ---- one propagator ------------------------------------------------------
+// In: loaded intervals X, Y, Z.  Out: the intervals to meet into x, y, z.
+// kDiv = false compiles the division propagators out (they need many more
+// registers); the engine uses that variant for tables without divisions.
+template <bool kDiv = true>
+__device__ __forceinline__ void deduce(int op, int2 X, int2 Y, int2 Z, int2& nx, int2& ny, int2& nz) {
+  Bnd x{X.x, X.y}, y{Y.x, Y.y}, z{Z.x, Z.y};
+  const int2 kTop = make_int2(kMinf, kInf);
+  switch (op) {
+    case kEq: {
+      nx = kTop; ny = kTop; nz = kTop;
+      if (X.x == 1 && X.y == 1) {                  // x true: y = z
+        ny = Z; nz = Y;
+      } else if (X.x == 0 && X.y == 0 && (Y.x == Y.y || Z.x == Z.y)) {
+        // x false and one side fixed: shave that value off the other side
+        int2 cut = make_int2(Y.x == Z.x ? (int32_t)clamp32(y.l + 1) : kMinf,
+                             Y.y == Z.y ? (int32_t)clamp32(y.u - 1) : kInf);
+        if (Z.x == Z.y) ny = cut; else nz = cut;
+      } else if (Y.y == Z.x && Y.x == Z.y) {       // same singleton
+        nx = make_int2(1, 1);
+      } else if (Y.x > Z.y || Y.y < Z.x) {         // disjoint
+        nx = make_int2(0, 0);
+      }
+      return;
+    }
+    case kLeq: {
+      nx = kTop; ny = kTop; nz = kTop;
+      if (X.x == 1 && X.y == 1) {                  // y <= z
+        ny = make_int2(Y.x, Z.y); nz = make_int2(Y.x, Z.y);
+      } else if (X.x == 0 && X.y == 0) {           // y > z
+        ny = make_int2((int32_t)clamp32(z.l + 1), Y.y);
+        nz = make_int2(Z.x, (int32_t)clamp32(y.u - 1));
+      } else if (Y.y <= Z.x) {
+        nx = make_int2(1, 1);
+      } else if (Y.x > Z.y) {
+        nx = make_int2(0, 0);
+      }
+      return;
+    }
+    case kAdd: {
+      if (y.l != kMinf && z.l != kMinf) x.l = mx(x.l, clamp32(y.l + z.l));
+      if (y.u != kInf && z.u != kInf)   x.u = mn(x.u, clamp32(y.u + z.u));
+      if (x.l != kMinf && z.u != kInf)  y.l = mx(y.l, clamp32(x.l - z.u));
+      if (x.u != kInf && z.l != kMinf)  y.u = mn(y.u, clamp32(x.u - z.l));
+      if (x.l != kMinf && y.u != kInf)  z.l = mx(z.l, clamp32(x.l - y.u));
+      if (x.u != kInf && y.l != kMinf)  z.u = mn(z.u, clamp32(x.u - y.l));
+      break;
+    }
+    case kMul: {
+      if (y.l != kMinf && y.u != kInf && z.l != kMinf && z.u != kInf) {
+        const i64 a = y.l * z.l, b = y.l * z.u, c = y.u * z.l, d = y.u * z.u;
+        x.l = mx(x.l, clamp32(mn(mn(a, b), mn(c, d))));
+        x.u = mn(x.u, clamp32(mx(mx(a, b), mx(c, d))));
+      }
+      mul_inverse(x, y, z);
+      mul_inverse(x, z, y);
+      break;
+    }
+    case kMin: {
+      x.l = mx(x.l, mn(y.l, z.l));
+      x.u = mn(x.u, mn(y.u, z.u));
+      y.l = mx(y.l, x.l);
+      if (x.u < z.l) y.u = mn(y.u, x.u);
+      z.l = mx(z.l, x.l);
+      if (x.u < y.l) z.u = mn(z.u, x.u);
+      break;
+    }
+    case kMax: {
+      x.l = mx(x.l, mx(y.l, z.l));
+      x.u = mn(x.u, mx(y.u, z.u));
+      y.u = mn(y.u, x.u);
+      if (x.l > z.u) y.l = mx(y.l, x.l);
+      z.u = mn(z.u, x.u);
+      if (x.l > y.u) z.l = mx(z.l, x.l);
+      break;
+    }
+    case kTdiv: case kFdiv: case kCdiv: case kEdiv:
+      if constexpr (kDiv) prop_div(op, x, y, z);
+      break;
+    default:
+      nx = kTop; ny = kTop; nz = kTop;
+      return;
+  }
+  nx = make_int2((int32_t)clamp32(x.l), (int32_t)clamp32(x.u));
+  ny = make_int2((int32_t)clamp32(y.l), (int32_t)clamp32(y.u));
+  nz = make_int2((int32_t)clamp32(z.l), (int32_t)clamp32(z.u));
+}
+
+// Entailment of one propagator on its operands' intervals (pir.hpp:417-438).
+__device__ __forceinline__ bool entailed(int op, int2 X, int2 Y, int2 Z) {
+  const i64 xl = X.x, xu = X.y, yl = Y.x, yu = Y.y, zl = Z.x, zu = Z.y;
+  switch (op) {
+    case kEq:  return (xl == 1 && yu == zl && yl == zu) || (xu == 0 && (yu < zl || yl > zu));
+    case kLeq: return (xl == 1 && yu <= zl) || (xu == 0 && yl > zu);
+    case kAdd: return xl == xu && yl == yu && zl == zu && xl == yl + zl;
+    case kMul: return xl == xu && ((yl == yu && zl == zu && xl == yl * zl)
+                                   || (xl == 0 && ((yl == 0 && yu == 0) || (zl == 0 && zu == 0))));
+    case kTdiv: case kFdiv: case kCdiv: case kEdiv:
+      return (xl == xu && yl == yu && zl == zu && zl != 0 && xl == q_op(yl, op, zl))
+          || (xl == yu && xu == yl && xl == 0 && (zl > 0 || zu < 0));
+    case kMin: return (xl == yu && xu == yl && yu <= zl) || (xl == zu && xu == zl && zu <= yl);
+    case kMax: return (xl == yu && xu == yl && yl >= zu) || (xl == zu && xu == zl && zl >= yu);
+    default:   return false;
+  }
+}
+
+}  // namespace pcx

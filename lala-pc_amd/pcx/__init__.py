@@ -1,0 +1,299 @@
+"""Python host API of the MI355X propagator-completion engine (ctypes over libpcx.so).
+
+Mirrors the reference's PIR surface (lattice-land/lala-pc include/lala/pir.hpp):
+
+    PIR.interpret_tell / PIR.tell   -> pcx_tell       (pir.hpp:316-352)
+    PIR.num_deductions              -> pcx_num_deductions (pir.hpp:382-384)
+    PIR.fixpoint                    -> pcx_fixpoint   (GaussSeidelIteration over
+                                                        PIR::deduce, pir_test.cpp:60-62)
+    PIR.ask_all / PIR.is_extractable-> pcx_ask_all    (pir.hpp:368-370, 873-884)
+    Batch                           -> shared table, one store per search node
+                                        (pir.hpp:182-205 shared-copy clones)
+
+Every computation runs on the GPU through the compiled library; there is no
+host fallback.  Loading the library needs no GPU (the generators are host
+code); creating an engine without a HIP device raises PcxError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpcx.so")
+
+INF = 2**31 - 1
+MINF = -(2**31)
+
+ADD, MUL, TDIV, FDIV, CDIV, EDIV, MIN, MAX, EQ, LEQ = range(10)
+OPS = {"ADD": ADD, "MUL": MUL, "TDIV": TDIV, "FDIV": FDIV, "CDIV": CDIV, "EDIV": EDIV,
+       "MIN": MIN, "MAX": MAX, "EQ": EQ, "LEQ": LEQ}
+
+MODE_PLAIN, MODE_BATCHED, MODE_PERSISTENT = 0, 1, 2
+
+OK, EINVAL, ENOMEM, EDEVICE, EBUSY, ENOTSUP = range(6)
+
+
+class PcxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"pcx status {status}: {msg}")
+        self.status = status
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("props_per_wave", ctypes.c_int32),
+                ("lds_mode", ctypes.c_int32), ("sweeps_per_check", ctypes.c_int32),
+                ("max_sweeps", ctypes.c_int64), ("timing", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 7)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("has_changed", ctypes.c_int32), ("failed", ctypes.c_int32),
+                ("sweeps", ctypes.c_int64), ("evals", ctypes.c_int64)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("wall_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
+                ("kernel_launches", ctypes.c_int64), ("evals", ctypes.c_int64),
+                ("tightened", ctypes.c_int64), ("sweeps_max", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GenParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("nvars", ctypes.c_int64), ("nprops", ctypes.c_int64),
+                ("kinds", ctypes.c_uint32), ("planted", ctypes.c_int32),
+                ("lb_max", ctypes.c_int32), ("width_min", ctypes.c_int32),
+                ("width_max", ctypes.c_int32), ("value_range", ctypes.c_int32),
+                ("bool_frac", ctypes.c_double), ("const_frac", ctypes.c_double),
+                ("chain_frac", ctypes.c_double), ("chain_len", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
+
+
+_P32 = ctypes.POINTER(ctypes.c_int32)
+_P64 = ctypes.POINTER(ctypes.c_int64)
+_PU8 = ctypes.POINTER(ctypes.c_uint8)
+_VP = ctypes.c_void_p
+
+# (name, restype, argtypes) of every exported C symbol (include/pcx.h, include/pcx_gen.h)
+SIGNATURES = [
+    ("pcx_version", ctypes.c_char_p, []),
+    ("pcx_last_error", ctypes.c_char_p, []),
+    ("pcx_config_default", None, [ctypes.POINTER(Config)]),
+    ("pcx_device_count", ctypes.c_int32, []),
+    ("pcx_create", ctypes.c_int32, [ctypes.POINTER(Config), ctypes.POINTER(_VP)]),
+    ("pcx_destroy", None, [_VP]),
+    ("pcx_tell", ctypes.c_int32, [_VP, _P32, _P32, _P32, _P32, ctypes.c_int64, _P32, _P32, ctypes.c_int64]),
+    ("pcx_num_deductions", ctypes.c_int64, [_VP]),
+    ("pcx_get_table", ctypes.c_int32, [_VP, _P32, _P32, _P32, _P32]),
+    ("pcx_snapshot", ctypes.c_int64, [_VP]),
+    ("pcx_restore", ctypes.c_int32, [_VP, ctypes.c_int64]),
+    ("pcx_fixpoint", ctypes.c_int32, [_VP, _P32, _P32, ctypes.c_int64, ctypes.POINTER(Result)]),
+    ("pcx_ask_all", ctypes.c_int32, [_VP, _P32, _P32, ctypes.c_int64, _PU8, _P32]),
+    ("pcx_batch_create", ctypes.c_int32, [_VP, ctypes.c_int64, ctypes.c_int64, _P64, _P32, ctypes.POINTER(_VP)]),
+    ("pcx_batch_destroy", None, [_VP]),
+    ("pcx_batch_load", ctypes.c_int32, [_VP, _P32, _P32]),
+    ("pcx_batch_reset", ctypes.c_int32, [_VP]),
+    ("pcx_batch_read", ctypes.c_int32, [_VP, _P32, _P32]),
+    ("pcx_batch_run", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Result), ctypes.POINTER(Stats)]),
+    ("pcxg_params_default", None, [ctypes.POINTER(GenParams)]),
+    ("pcxg_network", ctypes.c_int32, [ctypes.POINTER(GenParams), _P32, _P32, _P32, _P32, _P32, _P32, _P32]),
+    ("pcxg_splits", ctypes.c_int32, [ctypes.c_uint64, _P32, _P32, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.c_int32, ctypes.c_int32, _P32, _P32]),
+    ("pcxg_subsets", ctypes.c_int32, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.c_int64, _P64, _P32]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libpcx.so (built by __graft_entry__.build() / `make -C lala-pc_amd`)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PcxError(EDEVICE, f"{LIB_PATH} is not built; run `make -C lala-pc_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p32(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], "int32 C-contiguous array expected"
+    return a.ctypes.data_as(_P32)
+
+
+def _p64(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"], "int64 C-contiguous array expected"
+    return a.ctypes.data_as(_P64)
+
+
+def _check(st):
+    if st != OK:
+        raise PcxError(st, lib().pcx_last_error().decode(errors="replace"))
+
+
+def device_count():
+    return int(lib().pcx_device_count())
+
+
+def default_config(**kw):
+    c = Config()
+    lib().pcx_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class PIR:
+    """Propagator table on one GPU (the reference's PIR without its store).
+
+    Stores are numpy int32 arrays lb/ub owned by the caller, as the
+    reference mutates its sub-store in place.
+    """
+
+    def __init__(self, config=None, **kw):
+        cfg = config if config is not None else default_config(**kw)
+        h = _VP()
+        _check(lib().pcx_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.config = cfg
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pcx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tell(self, op, x, y, z, lb=None, ub=None):
+        """Append propagators x = y <op> z; embeds reified outputs in [0,1] in lb/ub."""
+        op, x, y, z = (np.ascontiguousarray(a, dtype=np.int32) for a in (op, x, y, z))
+        n = len(op)
+        nv = 0 if lb is None else len(lb)
+        _check(lib().pcx_tell(self._h, _p32(op), _p32(x), _p32(y), _p32(z), n, _p32(lb), _p32(ub), nv))
+
+    interpret_tell = tell
+
+    def num_deductions(self):
+        return int(lib().pcx_num_deductions(self._h))
+
+    def table(self):
+        n = self.num_deductions()
+        out = [np.zeros(n, np.int32) for _ in range(4)]
+        _check(lib().pcx_get_table(self._h, *(_p32(a) for a in out)))
+        return out
+
+    def fixpoint(self, lb, ub):
+        """Propagate lb/ub (int32, modified in place) to the fixpoint; returns Result."""
+        r = Result()
+        _check(lib().pcx_fixpoint(self._h, _p32(lb), _p32(ub), len(lb), ctypes.byref(r)))
+        return r
+
+    def ask_all(self, lb, ub):
+        n = self.num_deductions()
+        ent = np.zeros(max(n, 1), np.uint8)
+        allf = ctypes.c_int32(0)
+        _check(lib().pcx_ask_all(self._h, _p32(lb), _p32(ub), len(lb),
+                                 ent.ctypes.data_as(_PU8), ctypes.byref(allf)))
+        return ent[:n].astype(bool), bool(allf.value)
+
+    def is_extractable(self, lb, ub):
+        """Not failed and every propagator entailed (pir.hpp:873-884)."""
+        if (lb > ub).any():
+            return False
+        return self.ask_all(lb, ub)[1]
+
+
+class Batch:
+    """Many device-resident stores over one shared PIR table."""
+
+    def __init__(self, pir, n_inst, nvars, inst_off=None, prop_idx=None):
+        self.pir = pir
+        self.n_inst = int(n_inst)
+        self.nvars = int(nvars)
+        if inst_off is not None:
+            inst_off = np.ascontiguousarray(inst_off, dtype=np.int64)
+            prop_idx = np.ascontiguousarray(prop_idx, dtype=np.int32)
+        h = _VP()
+        _check(lib().pcx_batch_create(pir._h, self.n_inst, self.nvars, _p64(inst_off),
+                                      _p32(prop_idx), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pcx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, lb, ub):
+        assert lb.size == self.n_inst * self.nvars
+        _check(lib().pcx_batch_load(self._h, _p32(lb.reshape(-1)), _p32(ub.reshape(-1))))
+
+    def reset(self):
+        _check(lib().pcx_batch_reset(self._h))
+
+    def read(self):
+        lb = np.zeros(self.n_inst * self.nvars, np.int32)
+        ub = np.zeros(self.n_inst * self.nvars, np.int32)
+        _check(lib().pcx_batch_read(self._h, _p32(lb), _p32(ub)))
+        return lb, ub
+
+    def run(self, mode=MODE_BATCHED):
+        res = (Result * self.n_inst)()
+        st = Stats()
+        _check(lib().pcx_batch_run(self._h, mode, res, ctypes.byref(st)))
+        return list(res), st
+
+
+# ---- synthetic workloads (host code, no GPU) ------------------------------
+def gen_params(**kw):
+    p = GenParams()
+    lib().pcxg_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def gen_network(p):
+    """Returns (op, x, y, z, lb, ub, witness) in told order."""
+    P, V = int(p.nprops), int(p.nvars)
+    op, x, y, z = (np.zeros(P, np.int32) for _ in range(4))
+    lb, ub, w = (np.zeros(V, np.int32) for _ in range(3))
+    _check(lib().pcxg_network(ctypes.byref(p), _p32(op), _p32(x), _p32(y), _p32(z),
+                              _p32(lb), _p32(ub), _p32(w)))
+    return op, x, y, z, lb, ub, w
+
+
+def gen_splits(seed, lb, ub, n_inst, kmin, kmax):
+    V = len(lb)
+    lo = np.zeros(n_inst * V, np.int32)
+    hi = np.zeros(n_inst * V, np.int32)
+    _check(lib().pcxg_splits(seed, _p32(lb), _p32(ub), V, n_inst, kmin, kmax, _p32(lo), _p32(hi)))
+    return lo, hi
+
+
+def gen_subsets(seed, nprops, n_inst, mmin, mmax):
+    off = np.zeros(n_inst + 1, np.int64)
+    _check(lib().pcxg_subsets(seed, nprops, n_inst, mmin, mmax, _p64(off), None))
+    idx = np.zeros(max(int(off[-1]), 1), np.int32)
+    _check(lib().pcxg_subsets(seed, nprops, n_inst, mmin, mmax, _p64(off), _p32(idx)))
+    return off, idx[:int(off[-1])]

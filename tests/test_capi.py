@@ -1,0 +1,103 @@
+"""CPU-side checks of the C-ABI library: it loads without a GPU, exports every
+symbol the public headers declare, validates its inputs, and its synthetic
+workload generators are deterministic and well-formed.  No kernel runs here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import pcx
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    src = open(os.path.join(ROOT, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return set(re.findall(r"\b(pcxg?_[a-z_]+)\s*\(", src))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = pcx.lib()
+    names = _declared("pcx.h") | _declared("pcx_gen.h")
+    assert len(names) >= 20
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, f"declared but not exported: {missing}"
+    bound = {n for n, _, _ in pcx.SIGNATURES}
+    assert names == bound, f"ctypes bindings differ from the headers: {names ^ bound}"
+
+
+def test_version_and_config():
+    assert b"pcx" in pcx.lib().pcx_version()
+    c = pcx.default_config()
+    assert c.props_per_wave == 256 and c.max_sweeps == 0
+
+
+def test_opcodes_match_oracle():
+    assert pcx.OPS == ol.OPS
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    if pcx.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(pcx.PcxError) as e:
+        pcx.PIR()
+    assert e.value.status == pcx.EDEVICE
+
+
+def test_generator_deterministic():
+    p = pcx.gen_params(nvars=2000, nprops=8000, seed=5)
+    a = pcx.gen_network(p)
+    b = pcx.gen_network(p)
+    for u, v in zip(a, b):
+        assert (u == v).all()
+    c = pcx.gen_network(pcx.gen_params(nvars=2000, nprops=8000, seed=6))
+    assert not (a[1] == c[1]).all()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_planted_network_is_satisfied_by_its_witness(seed):
+    p = pcx.gen_params(nvars=5000, nprops=20000, seed=seed)
+    op, x, y, z, lb, ub, w = pcx.gen_network(p)
+    assert ((lb <= w) & (w <= ub)).all()
+    wx, wy, wz = w[x].astype(np.int64), w[y].astype(np.int64), w[z].astype(np.int64)
+    expect = np.select([op == pcx.ADD, op == pcx.MUL, op == pcx.MIN, op == pcx.MAX,
+                        op == pcx.EQ, op == pcx.LEQ],
+                       [wy + wz, wy * wz, np.minimum(wy, wz), np.maximum(wy, wz),
+                        (wy == wz).astype(np.int64), (wy <= wz).astype(np.int64)], -1)
+    assert (expect == wx).all()
+    assert ((x != y) & (y != z) & (x != z)).all()
+    counts = np.bincount(op, minlength=10)
+    assert counts[[pcx.TDIV, pcx.FDIV, pcx.CDIV, pcx.EDIV]].sum() == 0
+    for k in (pcx.MUL, pcx.MIN, pcx.MAX, pcx.EQ, pcx.LEQ):
+        assert 0.10 < counts[k] / len(op) < 0.18
+
+
+def test_literal_network_is_uniform():
+    p = pcx.gen_params(nvars=1000, nprops=3000, seed=0, planted=0, lb_max=0,
+                       width_min=1000, width_max=1000, chain_frac=0.0)
+    op, x, y, z, lb, ub, _ = pcx.gen_network(p)
+    assert (lb == 0).all() and (ub == 1000).all()
+    assert ((x != y) & (y != z) & (x != z)).all()
+    assert set(np.unique(op)) == {pcx.ADD, pcx.MUL, pcx.MIN, pcx.MAX, pcx.EQ, pcx.LEQ}
+
+
+def test_splits_stay_inside_bounds():
+    lb = np.array([0, 5, 10, 3], np.int32)
+    ub = np.array([9, 5, 20, 3], np.int32)
+    lo, hi = pcx.gen_splits(3, lb, ub, 50, 1, 20)
+    lo, hi = lo.reshape(50, 4), hi.reshape(50, 4)
+    assert ((lo >= lb) & (hi <= ub) & (lo <= hi)).all()
+    assert (lo[:, 1] == 5).all() and (hi[:, 3] == 3).all()   # singletons untouched
+    assert ((lo != lb) | (hi != ub)).any(axis=1).all()
+
+
+def test_subsets_sorted_distinct_sized():
+    off, idx = pcx.gen_subsets(9, 1000, 40, 20, 700)
+    assert off[0] == 0 and (np.diff(off) >= 20).all() and (np.diff(off) <= 700).all()
+    for k in range(40):
+        s = idx[off[k]:off[k + 1]]
+        assert (np.diff(s) > 0).all() and s.min() >= 0 and s.max() < 1000

@@ -1,0 +1,226 @@
+"""GPU parity: every result of the HIP engine equals the CPU oracle's.
+
+Monotone, extensive propagators have one fixpoint whatever the iteration
+order, so the asynchronous GPU sweeps must reproduce the oracle's
+Gauss-Seidel result bit for bit in every bound and in the failure flag (after
+a failure only the flag is compared: bounds past a failure depend on order).
+All calls go through the C ABI (lala-pc_amd/pcx/libpcx.so).
+"""
+import numpy as np
+import pytest
+
+import kats
+import oracle_lib as ol
+import pcx
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if pcx.device_count() < 1:
+        pytest.fail("no HIP device visible: GPU tests need an MI355X")
+    return True
+
+
+class GpuEngine:
+    """KAT engine over pcx.PIR (same interface as the oracle's)."""
+
+    def __init__(self):
+        self.pir = pcx.PIR()
+
+    def tell(self, op, x, y, z, lb, ub):
+        self.pir.tell(op, x, y, z, lb, ub)
+
+    def num_deductions(self):
+        return self.pir.num_deductions()
+
+    def fixpoint(self, lb, ub):
+        return bool(self.pir.fixpoint(lb, ub).failed)
+
+    def all_entailed(self, lb, ub):
+        return self.pir.is_extractable(lb, ub)
+
+
+@pytest.mark.parametrize("case", kats.load_cases(), ids=lambda c: c["name"])
+def test_kat_on_gpu(gpu, case):
+    errs = kats.run_case(case, GpuEngine)
+    assert not errs, f"{case['name']} ({case['ref']}): " + "; ".join(errs)
+
+
+def _boxes(lo, hi):
+    r = np.arange(lo, hi + 1)
+    iv = np.array([(a, b) for a in r for b in r if a <= b], dtype=np.int32)
+    n = len(iv)
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    return iv[i.ravel()], iv[j.ravel()], iv[k.ravel()]
+
+
+def _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, nvars, what):
+    n_inst = len(res_o)
+    lb_g, ub_g = lb_g.reshape(n_inst, nvars), ub_g.reshape(n_inst, nvars)
+    lb_o, ub_o = lb_o.reshape(n_inst, nvars), ub_o.reshape(n_inst, nvars)
+    fg = np.array([r.failed for r in res_g], bool)
+    fo = np.array([r.failed for r in res_o], bool)
+    bad = np.nonzero(fg != fo)[0]
+    assert len(bad) == 0, f"{what}: failure flag differs on {len(bad)} instances, first {bad[:5]}"
+    ok = ~fo
+    diff = ((lb_g != lb_o) | (ub_g != ub_o)).any(axis=1) & ok
+    bad = np.nonzero(diff)[0]
+    assert len(bad) == 0, (f"{what}: bounds differ on {len(bad)} instances, first {bad[0]}: "
+                           f"gpu {lb_g[bad[0]]}/{ub_g[bad[0]]} oracle {lb_o[bad[0]]}/{ub_o[bad[0]]}")
+    hg = np.array([r.has_changed for r in res_g], bool)
+    ho = np.array([r.has_changed for r in res_o], bool)
+    assert (hg[ok] == ho[ok]).all(), f"{what}: has_changed differs"
+
+
+@pytest.mark.parametrize("name", list(ol.OPS))
+@pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PLAIN])
+def test_every_box_one_propagator(gpu, name, mode):
+    """All boxes x,y,z in [-4,4] (91,125 instances) for one propagator."""
+    lo, hi = (-4, 4) if mode == pcx.MODE_BATCHED else (-2, 2)
+    X, Y, Z = _boxes(lo, hi)
+    n = len(X)
+    lb = np.stack([X[:, 0], Y[:, 0], Z[:, 0]], 1).astype(np.int32).ravel()
+    ub = np.stack([X[:, 1], Y[:, 1], Z[:, 1]], 1).astype(np.int32).ravel()
+    op = np.array([ol.OPS[name]], np.int32)
+    x, y, z = (np.array([v], np.int32) for v in (0, 1, 2))
+    # tell embeds [0,1] for reified outputs, instance by instance
+    if name in ("EQ", "LEQ"):
+        lb[0::3] = np.maximum(lb[0::3], 0)
+        ub[0::3] = np.minimum(ub[0::3], 1)
+    lb_o, ub_o = lb.copy(), ub.copy()
+    res_o = ol.fixpoint_batch(op, x, y, z, lb_o, ub_o, 3, threads=8)
+    pir = pcx.PIR()
+    pir.tell(op, x, y, z)
+    b = pcx.Batch(pir, n, 3)
+    b.load(lb, ub)
+    res_g, st = b.run(mode)
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, 3, f"{name} mode {mode}")
+
+
+def _network(seed, V, P, **kw):
+    p = pcx.gen_params(seed=seed, nvars=V, nprops=P, **kw)
+    op, x, y, z, lb, ub, w = pcx.gen_network(p)
+    return op, x, y, z, lb, ub, w
+
+
+def _oracle_fix(op, x, y, z, lb, ub):
+    op, x, y, z = op.copy(), x.copy(), y.copy(), z.copy()
+    lb, ub = lb.copy(), ub.copy()
+    ol.tell_embed(op, x, lb, ub)
+    ol.sort_table(op, x, y, z)
+    r = ol.fixpoint(op, x, y, z, lb, ub)
+    return r, lb, ub
+
+
+def _gpu_fix(op, x, y, z, lb, ub, **cfg):
+    pir = pcx.PIR(**cfg)
+    lb, ub = lb.copy(), ub.copy()
+    pir.tell(op, x, y, z, lb, ub)
+    r = pir.fixpoint(lb, ub)
+    return r, lb, ub, pir
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_literal_random_network(gpu, seed):
+    """1,000 vars in [0,1000], 3,000 propagators over the 7 kinds, operands uniform."""
+    net = _network(seed, 1000, 3000, planted=0, lb_max=0, width_min=1000, width_max=1000,
+                   chain_frac=0.0)
+    ro, lbo, ubo = _oracle_fix(*net[:6])
+    rg, lbg, ubg, _ = _gpu_fix(*net[:6])
+    assert bool(rg.failed) == bool(ro.failed)
+    if not ro.failed:
+        assert (lbg == lbo).all() and (ubg == ubo).all()
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("ppw", [64, 512])
+def test_literal_network_wide_bounds_fails_like_oracle(gpu, seed, ppw):
+    """Random bounds as in the 1e6-variable workload but unplanted: fails."""
+    net = _network(seed, 20000, 80000, planted=0, chain_frac=0.3)
+    ro, _, _ = _oracle_fix(*net[:6])
+    rg, _, _, _ = _gpu_fix(*net[:6], props_per_wave=ppw)
+    assert bool(rg.failed) == bool(ro.failed)
+
+
+@pytest.mark.parametrize("seed,V,P,ppw", [(1, 20000, 80000, 256), (2, 20000, 80000, 64),
+                                          (3, 100000, 400000, 128), (4, 100000, 400000, 512)])
+def test_planted_network(gpu, seed, V, P, ppw):
+    op, x, y, z, lb, ub, w = _network(seed, V, P)
+    ro, lbo, ubo = _oracle_fix(op, x, y, z, lb, ub)
+    rg, lbg, ubg, pir = _gpu_fix(op, x, y, z, lb, ub, props_per_wave=ppw)
+    assert not ro.failed and not rg.failed
+    assert (lbg == lbo).all() and (ubg == ubo).all()
+    assert ((lbg <= w) & (w <= ubg)).all()
+    # idempotence: one more fixpoint changes nothing and takes one sweep
+    r2 = pir.fixpoint(lbg, ubg)
+    assert not r2.has_changed and r2.sweeps == 1
+
+
+def test_full_size_network(gpu):
+    """1,000,000 variables / 4,000,000 propagators: bit-exact with the oracle."""
+    op, x, y, z, lb, ub, w = _network(1, 1000000, 4000000)
+    rg, lbg, ubg, pir = _gpu_fix(op, x, y, z, lb, ub)
+    assert not rg.failed and rg.sweeps >= 50
+    ro, lbo, ubo = _oracle_fix(op, x, y, z, lb, ub)
+    assert (lbg == lbo).all() and (ubg == ubo).all()
+    assert ((lbg <= w) & (w <= ubg)).all()
+
+
+@pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PLAIN])
+def test_batch_of_search_nodes(gpu, mode):
+    """Shared table, per-instance subsets and splits (scaled-down batch)."""
+    V, P, n_inst = 5000, 20000, 96
+    op, x, y, z, lb, ub, w = _network(11, V, P)
+    pir = pcx.PIR()
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_lb, root_ub).failed
+    slb, sub = pcx.gen_splits(12, root_lb, root_ub, n_inst, 1, 20)
+    off, idx = pcx.gen_subsets(13, P, n_inst, 200, 20000)
+    top, tx, ty, tz = pir.table()
+    lb_o, ub_o = slb.copy(), sub.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lb_o, ub_o, V, idx=idx.astype(np.int64), off=off, threads=8)
+    b = pcx.Batch(pir, n_inst, V, off, idx)
+    b.load(slb, sub)
+    res_g, st = b.run(mode)
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"batch mode {mode}")
+    # reset restores the loaded stores and a rerun gives the same answer
+    b.reset()
+    res_g2, _ = b.run(mode)
+    lb_g2, ub_g2 = b.read()
+    assert (lb_g2 == lb_g).all() and (ub_g2 == ub_g).all()
+
+
+def test_edge_cases(gpu):
+    pir = pcx.PIR()
+    # empty table: fixpoint is the store itself, one quiet sweep
+    lb = np.array([0, 5], np.int32)
+    ub = np.array([3, 4], np.int32)
+    r = pir.fixpoint(lb, ub)
+    assert r.failed        # var 1 is empty from the start
+    lb = np.array([0, 1], np.int32)
+    ub = np.array([3, 4], np.int32)
+    r = pir.fixpoint(lb, ub)
+    assert not r.failed and not r.has_changed
+    # aliased operands: x = x + z with z in [1,1] has no solution
+    pir2 = pcx.PIR()
+    lb = np.array([0, 1], np.int32)
+    ub = np.array([10, 1], np.int32)
+    pir2.tell([pcx.ADD], [0], [0], [1], lb, ub)
+    lbo, ubo = lb.copy(), ub.copy()
+    ro = ol.fixpoint(np.array([0], np.int32), np.array([0], np.int32), np.array([0], np.int32),
+                     np.array([1], np.int32), lbo, ubo)
+    rg = pir2.fixpoint(lb, ub)
+    assert bool(rg.failed) == bool(ro.failed)
+    # a variable index outside the store is rejected, not read
+    pir3 = pcx.PIR()
+    pir3.tell([pcx.ADD], [0], [1], [5])
+    with pytest.raises(pcx.PcxError) as e:
+        pir3.fixpoint(np.zeros(3, np.int32), np.ones(3, np.int32))
+    assert e.value.status == pcx.EINVAL
+    with pytest.raises(pcx.PcxError):
+        pir3.tell([42], [0], [1], [2])
