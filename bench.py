@@ -76,9 +76,10 @@ def bench_batched(args):
     log(f"setup {time.time() - t0:.1f}s: root fixpoint sweeps={rr.sweeps} failed={rr.failed}, "
         f"{int(off[-1])} active propagators over {n_inst} instances")
 
+    mode = {"batched": pcx.MODE_BATCHED, "persistent": pcx.MODE_PERSISTENT}[args.mode]
     for _ in range(args.warmup):
         batch.reset()
-        batch.run(pcx.MODE_BATCHED)
+        batch.run(mode)
     evals = tight = 0
     kms = 0.0
     launches = 0
@@ -86,7 +87,7 @@ def bench_batched(args):
     t_start = time.perf_counter()
     for _ in range(args.steps):
         batch.reset()
-        res, st = batch.run(pcx.MODE_BATCHED)
+        res, st = batch.run(mode)
         evals += st.evals
         tight += st.tightened
         kms += st.kernel_ms
@@ -100,13 +101,16 @@ def bench_batched(args):
         f"sweeps_max={smax}, failed={failed}/{n_inst}, kernel {kms / args.steps:.1f} ms/step")
 
     # plain engine on the same batch (one instance at a time)
-    batch.reset()
-    tp = time.perf_counter()
-    res_p, st_p = batch.run(pcx.MODE_PLAIN)
-    plain_s = time.perf_counter() - tp
-    plain_fps = n_inst / plain_s
-    lb_p, ub_p = batch.read()
-    log(f"plain: {plain_s * 1e3:.0f} ms for {n_inst} instances, {plain_fps:.0f} fixpoints/s")
+    plain_fps = None
+    lb_p, ub_p = lb_g, ub_g
+    if not args.no_plain:
+        batch.reset()
+        tp = time.perf_counter()
+        res_p, st_p = batch.run(pcx.MODE_PLAIN)
+        plain_s = time.perf_counter() - tp
+        plain_fps = n_inst / plain_s
+        lb_p, ub_p = batch.read()
+        log(f"plain: {plain_s * 1e3:.0f} ms for {n_inst} instances, {plain_fps:.0f} fixpoints/s")
 
     out = {
         "value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
@@ -114,16 +118,16 @@ def bench_batched(args):
                                "50,000-var / 200,000-propagator network, 1-20 splits each, "
                                "2,000-200,000 active propagators each",
                    "nvars": V, "nprops": P, "instances": n_inst, "props_per_wave": args.ppw,
-                   "mode": "batched (device scheduler, one sweep launch per sweep)"},
+                   "mode": args.mode},
         "plain_fixpoints_per_s": plain_fps,
-        "batched_vs_plain": fps / plain_fps,
+        "batched_vs_plain": fps / plain_fps if plain_fps else None,
         "evals_per_s": evals / elapsed,
         "failed_instances": int(failed),
         "sweeps_max": int(smax),
         "roofline": roofline(evals / args.steps, tight / args.steps, kms / args.steps),
         "kernel_launches_per_step": launches / args.steps,
     }
-    out["roofline"]["kernel"] = "k_sweep_batched"
+    out["roofline"]["kernel"] = "k_sweep_batched" if args.mode == "batched" else "k_persistent"
 
     if not args.no_cpu:
         ol = cpu_oracle()
@@ -219,9 +223,13 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2])
-    ap.add_argument("--ppw", type=int, default=256, choices=[64, 128, 256, 512])
+    ap.add_argument("--mode", default="persistent", choices=["batched", "persistent"],
+                    help="engine for the timed batch (the plain engine is always run once beside it)")
+    ap.add_argument("--ppw", type=int, default=0, choices=[0, 64, 128, 256, 512],
+                    help="propagators per wave (0 = engine default)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -56,7 +56,7 @@ typedef struct pcx_batch pcx_batch;
 
 typedef struct {
   int32_t device;           /* HIP device ordinal                                   */
-  int32_t props_per_wave;   /* 64, 128, 256 or 512; 0 = library default              */
+  int32_t props_per_wave;   /* 64, 128, 256 or 512; 0 = chosen from the sweep size   */
   int32_t lds_mode;         /* -1 auto, 0 never stage stores in LDS, 1 whenever fit  */
   int32_t sweeps_per_check; /* plain engine: sweeps queued between host checks (0=8) */
   int64_t max_sweeps;       /* 0 = run to the fixpoint                               */

@@ -79,7 +79,8 @@ struct pcx_batch {
   unsigned long long* d_tight = nullptr;
   int32_t* d_act = nullptr;
   int64_t* d_item_base = nullptr;
-  int32_t* d_ctl = nullptr;           // [0]=n_act [1]=sweep [2]=live
+  int32_t* d_ctl = nullptr;           // [0]=n_act [1]=sweep [2]=live [3]=queue head
+  int32_t* d_order = nullptr;         // instances by decreasing size (persistent queue)
   std::vector<hipEvent_t> ev;         // timing pool
 };
 
@@ -112,7 +113,21 @@ Table table_of(const pcx_engine* e) {
   return Table{e->d_xo, e->d_y, e->d_z, (int64_t)e->op.size()};
 }
 
-int ppw_of(const pcx_engine* e) { return e->cfg.props_per_wave ? e->cfg.props_per_wave : 256; }
+// Propagators per wave.  0 = automatic: the largest of 512/256/128/64 that
+// still gives at least two workgroups per CU for an m-propagator sweep (small
+// tables need many small blocks to fill the chip, large ones amortize the
+// per-block overhead); the batched mode walks a grid-stride item list and
+// always uses 512 unless told otherwise.
+int ppw_of(const pcx_engine* e, int64_t m = -1) {
+  if (e->cfg.props_per_wave) return e->cfg.props_per_wave;
+  if (m < 0) return 512;
+  int cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, e->cfg.device) == hipSuccess) cus = prop.multiProcessorCount;
+  for (int ppw = 512; ppw > 64; ppw /= 2)
+    if (m / ((int64_t)kBlock * (ppw / kWave)) >= 2LL * cus) return ppw;
+  return 64;
+}
 
 int64_t inst_size(const pcx_batch* b, int64_t k) {
   return b->h_off.empty() ? b->table_n : b->h_off[k + 1] - b->h_off[k];
@@ -182,7 +197,6 @@ pcx_status prepare_run(pcx_batch* b) {
 // the instance's flags every `group` sweeps.
 pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
-  const int ppw = ppw_of(e);
   const int group = e->cfg.sweeps_per_check > 0 ? e->cfg.sweeps_per_check : 8;
   const Table t = table_of(e);
   int64_t launches = 0;
@@ -191,6 +205,7 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   InstFlags hf;
   for (int64_t k = 0; k < b->n_inst; ++k) {
     const int64_t m = inst_size(b, k);
+    const int ppw = ppw_of(e, m);
     const int32_t* idx = b->h_off.empty() ? nullptr : b->d_idx + b->h_off[k];
     int2* st = b->d_store + k * b->nvars;
     InstFlags* f = b->d_flags + k;
@@ -230,7 +245,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   int dev_cus = 256;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, e->cfg.device) == hipSuccess) dev_cus = prop.multiProcessorCount;
-  const int grid = dev_cus * 8;
+  const int grid = dev_cus * 8;          // a multiple of the XCD count
   Subsets sub{b->h_off.empty() ? nullptr : b->d_off, b->d_idx};
   WorkList wl{b->d_act, b->d_item_base, b->d_ctl, b->d_ctl + 1, b->d_ctl + 2};
   int64_t launches = 0;
@@ -262,6 +277,34 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   return collect(b, flags, results, stats, launches, kms);
 }
 
+// Persistent mode: one launch; workgroups pull whole instances from a queue
+// ordered largest first and run each to its fixpoint.
+pcx_status run_persistent(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
+  pcx_engine* e = b->e;
+  const Table t = table_of(e);
+  int dev_cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, e->cfg.device) == hipSuccess) dev_cus = prop.multiProcessorCount;
+  const int per_cu = persistent_blocks_per_cu(e->has_div);
+  const int grid = (int)std::min<int64_t>((int64_t)dev_cus * per_cu, b->n_inst);
+  Subsets sub{b->h_off.empty() ? nullptr : b->d_off, b->d_idx};
+  const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
+  double kms = 0.0;
+  if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, 0), e->stream));
+  PCX_HIP(launch_persistent(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, b->d_order,
+                            (int32_t)b->n_inst, b->d_ctl + 3, b->d_tight, maxs, e->has_div, grid,
+                            e->stream));
+  if (e->cfg.timing) {
+    PCX_HIP(hipEventRecord(event_at(b, 1), e->stream));
+    PCX_HIP(hipEventSynchronize(b->ev[1]));
+    float ms = 0.f;
+    PCX_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    kms = ms;
+  }
+  std::vector<InstFlags> flags;
+  return collect(b, flags, results, stats, 1, kms);
+}
+
 }  // namespace
 
 // =========================== C ABI ========================================
@@ -274,7 +317,7 @@ void pcx_config_default(pcx_config* c) {
   if (!c) return;
   std::memset(c, 0, sizeof(*c));
   c->device = 0;
-  c->props_per_wave = 256;
+  c->props_per_wave = 0;
   c->lds_mode = -1;
   c->sweeps_per_check = 8;
   c->max_sweeps = 0;
@@ -292,8 +335,8 @@ pcx_status pcx_create(const pcx_config* cfg, pcx_engine** out) {
   *out = nullptr;
   pcx_config c;
   if (cfg) c = *cfg; else pcx_config_default(&c);
-  if (c.props_per_wave == 0) c.props_per_wave = 256;
-  if (!valid_ppw(c.props_per_wave)) return fail_with(PCX_EINVAL, "props_per_wave must be 64, 128, 256 or 512");
+  if (c.props_per_wave != 0 && !valid_ppw(c.props_per_wave))
+    return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto), 64, 128, 256 or 512");
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail_with(PCX_EDEVICE, "no HIP device");
   if (c.device < 0 || c.device >= n) return fail_with(PCX_EINVAL, "device ordinal out of range");
@@ -442,6 +485,19 @@ pcx_status pcx_batch_create(pcx_engine* e, int64_t n_inst, int64_t nvars, const 
         chk(hipMemcpy(b->d_idx, prop_idx, inst_off[n_inst] * sizeof(int32_t), hipMemcpyHostToDevice));
     }
   }
+  if (he == hipSuccess) {
+    // Persistent queue order: largest instances first (longest-processing-
+    // time-first keeps the tail short when sizes are uneven).
+    std::vector<int32_t> order(n_inst);
+    std::iota(order.begin(), order.end(), 0);
+    if (inst_off)
+      std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
+        return inst_off[a + 1] - inst_off[a] > inst_off[c + 1] - inst_off[c];
+      });
+    chk(hipMalloc(&b->d_order, n_inst * sizeof(int32_t)));
+    if (he == hipSuccess)
+      chk(hipMemcpy(b->d_order, order.data(), n_inst * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
   if (he != hipSuccess) {
     pcx_batch_destroy(b);
     return fail_with(he == hipErrorOutOfMemory ? PCX_ENOMEM : PCX_EDEVICE,
@@ -456,7 +512,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   if (b->e) (void)hipSetDevice(b->e->cfg.device);
   for (hipEvent_t ev : b->ev) (void)hipEventDestroy(ev);
   dfree(b->d_store); dfree(b->d_image); dfree(b->d_off); dfree(b->d_idx); dfree(b->d_flags);
-  dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl);
+  dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl); dfree(b->d_order);
   delete b;
 }
 
@@ -511,7 +567,7 @@ pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results, pcx_st
   switch (mode) {
     case PCX_MODE_PLAIN: s = run_plain(b, results, stats); break;
     case PCX_MODE_BATCHED: s = run_batched(b, results, stats); break;
-    case PCX_MODE_PERSISTENT: return fail_with(PCX_ENOTSUP, "persistent mode not built yet");
+    case PCX_MODE_PERSISTENT: s = run_persistent(b, results, stats); break;
     default: return fail_with(PCX_EINVAL, "unknown mode");
   }
   if (stats) stats->wall_ms = now_ms() - t0;

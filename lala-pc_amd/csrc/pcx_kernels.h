@@ -20,6 +20,7 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;            // 4 waves per workgroup
 constexpr int kNoFail = 0x7fffffff;
 constexpr int kTightSlots = 64;        // striped tightening counters
+constexpr int kXcds = 8;               // accelerator complex dies on MI355X
 
 struct Table {
   const uint32_t* __restrict__ xo;
@@ -49,7 +50,8 @@ struct WorkList {
   int64_t* item_base;    // prefix sum of work items per active instance (n_act + 1)
   int32_t* n_act;        // [1]
   int32_t* sweep;        // [1] current sweep number
-  int32_t* live;         // [1] instances still running after the last schedule
+  int32_t* live;         // [1] instances still running after the last schedule;
+                         //     live[1] is the persistent mode's queue head
 };
 
 }  // namespace pcx
@@ -65,6 +67,11 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
                                 int64_t n_props, InstFlags* flags, WorkList wl,
                                 unsigned long long* tight, int ppw, bool div, int grid,
                                 hipStream_t st);
+hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
+                             int64_t n_props, InstFlags* flags, const int32_t* order,
+                             int32_t n_inst, int32_t* queue, unsigned long long* tight,
+                             int32_t max_sweeps, bool div, int grid, hipStream_t st);
+int persistent_blocks_per_cu(bool div);   // resident workgroups per CU
 hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipStream_t st);
 hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
                       hipStream_t st);

@@ -15,6 +15,12 @@ namespace pcx {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Subset indices are read once per sweep: stream them past the caches so
+// they do not evict the store and table lines that are reused.
+__device__ __forceinline__ int64_t prop_at(const int32_t* __restrict__ idx, int64_t q) {
+  return idx ? (int64_t)__builtin_nontemporal_load(idx + q) : q;
+}
+
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
@@ -81,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_plain(Table t, int2* __restric
 #pragma unroll
   for (int j = 0; j < PPL; ++j) {
     const int64_t k = base + j * kBlock + threadIdx.x;
-    if (k < m) eval_prop<DIV>(t, st, idx ? (int64_t)idx[k] : k, chg, fail, nt);
+    if (k < m) eval_prop<DIV>(t, st, prop_at(idx, k), chg, fail, nt);
   }
   publish(f, s, chg, fail, nt, tight);
 }
@@ -189,7 +195,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
   if (n_act <= 0) return;
   const int64_t total = wl.item_base[n_act];
   constexpr int64_t chunk = (int64_t)kBlock * PPL;
-  for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+  // XCD-aware split: workgroups are dispatched round-robin over the 8 XCDs,
+  // so workgroup b runs on XCD b % 8.  Each XCD takes one contiguous eighth
+  // of the (instance-major) item list and walks it in order, so at any time
+  // an XCD's L2 holds the stores of only the few instances it is sweeping.
+  const int xcd = blockIdx.x % kXcds;
+  const int64_t lanes = gridDim.x / kXcds;            // workgroups per XCD
+  const int64_t first_item = total * xcd / kXcds;
+  const int64_t end_item = total * (xcd + 1) / kXcds;
+  for (int64_t item = first_item + blockIdx.x / kXcds; item < end_item; item += lanes) {
     // active slot holding this item: last a with item_base[a] <= item
     int lo = 0, hi = n_act - 1;
     while (lo < hi) {
@@ -209,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
         const int64_t q = first + j * kBlock + threadIdx.x;
-        if (q < m) eval_prop<DIV>(t, st, idx ? (int64_t)idx[q] : q, chg, fail, nt);
+        if (q < m) eval_prop<DIV>(t, st, prop_at(idx, q), chg, fail, nt);
       }
     }
     publish(f, s, chg, fail, nt, tight);
@@ -237,6 +251,138 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
   return hipGetLastError();
 }
 
+// ---- persistent mode: one launch, a device queue of instances -------------
+// Each workgroup takes the next instance from the queue (largest first) and
+// runs all of its sweeps itself, separated by workgroup barriers, so the
+// instance's store is only touched from one CU and stays in that XCD's L2 /
+// the MALL for the whole fixpoint.  Store accesses are workgroup-scoped
+// atomics; the barrier between sweeps orders one sweep's tightenings before
+// the next sweep's reads, which is what the termination argument needs.
+constexpr int kPBlock = 512;     // 8 waves per workgroup
+
+__device__ __forceinline__ int2 wg_load(const int2* p) {
+  const unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return make_int2((int32_t)(uint32_t)v, (int32_t)(uint32_t)(v >> 32));
+}
+
+__device__ __forceinline__ void wg_meet(int2* __restrict__ st, int32_t v, int2 cur, int2 n,
+                                        bool& chg, bool& fail, int& nt) {
+  if (n.x > cur.x) {
+    const int old = __hip_atomic_fetch_max(&st[v].x, n.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old < n.x) { chg = true; ++nt; }
+    if (n.x > cur.y) fail = true;
+  }
+  if (n.y < cur.y) {
+    const int old = __hip_atomic_fetch_min(&st[v].y, n.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old > n.y) { chg = true; ++nt; }
+    if (n.y < cur.x) fail = true;
+  }
+}
+
+template <bool DIV>
+__device__ __forceinline__ void apply_wg(int2* __restrict__ st, int op, int32_t x, int32_t y,
+                                         int32_t z, int2 X, int2 Y, int2 Z, bool& chg, bool& fail,
+                                         int& nt) {
+  if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = true; return; }
+  int2 nx, ny, nz;
+  deduce<DIV>(op, X, Y, Z, nx, ny, nz);
+  if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
+  wg_meet(st, x, X, nx, chg, fail, nt);
+  wg_meet(st, y, Y, ny, chg, fail, nt);
+  wg_meet(st, z, Z, nz, chg, fail, nt);
+}
+
+template <bool DIV>
+__global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restrict__ store,
+                                                        int64_t nvars, Subsets sub, int64_t n_props,
+                                                        InstFlags* flags, const int32_t* order,
+                                                        int32_t n_inst, int32_t* queue,
+                                                        unsigned long long* tight,
+                                                        int32_t max_sweeps) {
+  __shared__ int32_t s_slot, s_chg, s_fail;
+  const int lane = lane_id();
+  for (;;) {
+    if (threadIdx.x == 0) s_slot = atomicAdd(queue, 1);
+    __syncthreads();
+    const int32_t slot = s_slot;
+    if (slot >= n_inst) return;            // same value in every wave: uniform exit
+    const int32_t k = order[slot];
+    InstFlags* f = flags + k;
+    const int64_t beg = sub.off ? sub.off[k] : 0;
+    const int64_t m = sub.off ? sub.off[k + 1] - beg : n_props;
+    const int32_t* idx = sub.off ? sub.idx + beg : nullptr;
+    int2* st = store + (int64_t)k * nvars;
+    const bool start_failed = f->failed_at != kNoFail;
+    int32_t last_changed = -1, failed_at = kNoFail, s = 0;
+    int nt = 0;
+    if (!start_failed) {
+      for (s = 0; max_sweeps <= 0 || s < max_sweeps; ++s) {
+        if (threadIdx.x == 0) { s_chg = 0; s_fail = 0; }
+        __syncthreads();
+        bool chg = false, fail = false;
+        // Two propagators per lane per step: both sets of loads are issued
+        // before either set of atomics (reading a bound that another lane is
+        // about to tighten is allowed: bounds only shrink).
+        for (int64_t q = threadIdx.x; q < m; q += 2 * kPBlock) {
+          const int64_t q2 = q + kPBlock;
+          const int64_t i1 = prop_at(idx, q);
+          const int64_t i2 = q2 < m ? prop_at(idx, q2) : -1;
+          const uint32_t xo1 = t.xo[i1];
+          const int32_t y1 = t.y[i1], z1 = t.z[i1];
+          uint32_t xo2 = 0;
+          int32_t y2 = 0, z2 = 0;
+          if (i2 >= 0) { xo2 = t.xo[i2]; y2 = t.y[i2]; z2 = t.z[i2]; }
+          const int32_t x1 = (int32_t)(xo1 & kVarMask), x2 = (int32_t)(xo2 & kVarMask);
+          const int2 X1 = wg_load(st + x1), Y1 = wg_load(st + y1), Z1 = wg_load(st + z1);
+          int2 X2 = make_int2(0, 0), Y2 = X2, Z2 = X2;
+          if (i2 >= 0) { X2 = wg_load(st + x2); Y2 = wg_load(st + y2); Z2 = wg_load(st + z2); }
+          apply_wg<DIV>(st, (int)(xo1 >> kOpShift), x1, y1, z1, X1, Y1, Z1, chg, fail, nt);
+          if (i2 >= 0) apply_wg<DIV>(st, (int)(xo2 >> kOpShift), x2, y2, z2, X2, Y2, Z2, chg, fail, nt);
+        }
+        const bool wc = __ballot(chg) != 0, wf = __ballot(fail) != 0;
+        if (lane == 0 && wc) s_chg = 1;
+        if (lane == 0 && wf) s_fail = 1;
+        __syncthreads();
+        const bool any_chg = s_chg != 0, any_fail = s_fail != 0;
+        __syncthreads();                     // all read the verdict before it is reset
+        if (any_chg) last_changed = s;
+        if (any_fail) { failed_at = s; break; }
+        if (!any_chg) break;
+      }
+    }
+    const int sum = wave_sum(nt);
+    if (lane == 0 && sum) atomicAdd(&tight[(k * 16 + (threadIdx.x >> 6)) & (kTightSlots - 1)],
+                                    (unsigned long long)sum);
+    if (threadIdx.x == 0 && !start_failed) {
+      if (last_changed >= 0) f->changed[last_changed & 1] = last_changed;
+      if (failed_at != kNoFail) f->failed_at = failed_at;
+      f->done_at = s;
+    }
+    __syncthreads();                         // s_slot is rewritten next round
+  }
+}
+
+int persistent_blocks_per_cu(bool div) {
+  int n = 0;
+  hipError_t e = div ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<true>, kPBlock, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<false>, kPBlock, 0);
+  return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
+                             int64_t n_props, InstFlags* flags, const int32_t* order,
+                             int32_t n_inst, int32_t* queue, unsigned long long* tight,
+                             int32_t max_sweeps, bool div, int grid, hipStream_t stream) {
+  if (div)
+    k_persistent<true><<<grid, kPBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, order,
+                                                      n_inst, queue, tight, max_sweeps);
+  else
+    k_persistent<false><<<grid, kPBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, order,
+                                                       n_inst, queue, tight, max_sweeps);
+  return hipGetLastError();
+}
+
 // ---- bookkeeping kernels --------------------------------------------------
 __global__ void k_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -250,6 +396,7 @@ __global__ void k_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl) {
     *wl.sweep = -1;
     *wl.n_act = 0;
     *wl.live = n_inst;
+    wl.live[1] = 0;                // persistent-mode queue head (ctl[3])
   }
 }
 

@@ -82,29 +82,6 @@ __device__ __forceinline__ Bnd hull(Bnd a, Bnd b) {
 }
 __device__ __forceinline__ void narrow(Bnd& a, Bnd b) { a.l = mx(a.l, b.l); a.u = mn(a.u, b.u); }
 
-// ---- multiplication ------------------------------------------------------
-// Narrow `t` from x = t * d (d may lose a zero bound), pir.hpp:702-718.
-__device__ __forceinline__ void mul_inverse(Bnd x, Bnd& t, Bnd& d) {
-  const bool x_nonzero = x.l > 0 || x.u < 0;
-  if (x_nonzero) {
-    if (d.l == 0) d.l = 1;
-    if (d.u == 0) d.u = -1;
-  }
-  if (x_nonzero && d.l < 0 && d.u > 0) {
-    t.l = mx(t.l, mn(x.l, x.u == kMinf ? (i64)kInf : -x.u));
-    t.u = mn(t.u, mx(x.l == kInf ? (i64)kMinf : -x.l, x.u));
-  } else if (x_nonzero || d.l > 0 || d.u < 0) {
-    if (x.l == kMinf || x.u == kInf || d.l == kMinf || d.u == kInf) return;
-    if (d.l > d.u) return;
-    // Here x, d are finite int32 and d excludes 0.
-    const int32_t xl = (int32_t)x.l, xu = (int32_t)x.u, dl = (int32_t)d.l, du = (int32_t)d.u;
-    const int32_t c1 = d_ceil32(xl, dl), c2 = d_ceil32(xl, du), c3 = d_ceil32(xu, dl), c4 = d_ceil32(xu, du);
-    const int32_t f1 = d_floor32(xl, dl), f2 = d_floor32(xl, du), f3 = d_floor32(xu, dl), f4 = d_floor32(xu, du);
-    t.l = mx(t.l, (i64)min(min(c1, c2), min(c3, c4)));
-    t.u = mn(t.u, (i64)max(max(f1, f2), max(f3, f4)));
-  }
-}
-
 // ---- division (pir.hpp:449-700), written without recursion ----------------
 // x = y / d, forward.
 __device__ __forceinline__ void div_forward(int op, Bnd& x, Bnd y, Bnd& d) {
@@ -296,92 +273,152 @@ __device__ __noinline__ void prop_div(int op, Bnd& x, Bnd& y, Bnd& d) {
   }
 }
 
+// ---- 32-bit helpers for the hot propagators ------------------------------
+// Saturating int32 add/sub: exactly clamp32 of the exact sum (one VALU op
+// with the clamp bit on CDNA).
+__device__ __forceinline__ int32_t sadd(int32_t a, int32_t b) { return __builtin_elementwise_add_sat(a, b); }
+__device__ __forceinline__ int32_t ssub(int32_t a, int32_t b) { return __builtin_elementwise_sub_sat(a, b); }
+__device__ __forceinline__ int32_t sat32(i64 v) { return (int32_t)clamp32(v); }
+
+// x = y * z, forward (pir.hpp:768-775) on int32 bounds.
+__device__ __forceinline__ void mul_forward32(int32_t& xl, int32_t& xu, int32_t yl, int32_t yu,
+                                              int32_t zl, int32_t zu) {
+  if (yl == kMinf || yu == kInf || zl == kMinf || zu == kInf) return;
+  i64 lo, hi;
+  if (min(yl, yu) >= 0 && min(zl, zu) >= 0) {
+    // all four products are non-negative and monotone in both factors
+    lo = (i64)min(yl, yu) * (i64)min(zl, zu);
+    hi = (i64)max(yl, yu) * (i64)max(zl, zu);
+  } else {
+    const i64 a = (i64)yl * zl, b = (i64)yl * zu, c = (i64)yu * zl, d = (i64)yu * zu;
+    lo = mn(mn(a, b), mn(c, d));
+    hi = mx(mx(a, b), mx(c, d));
+  }
+  xl = max(xl, sat32(lo));
+  xu = min(xu, sat32(hi));
+}
+
+// Narrow t from x = t * d (pir.hpp:702-718) on int32 bounds; d may lose a
+// zero bound.
+__device__ __forceinline__ void mul_inverse32(int32_t xl, int32_t xu, int32_t& tl, int32_t& tu,
+                                              int32_t& dl, int32_t& du) {
+  const bool x_nonzero = xl > 0 || xu < 0;
+  if (x_nonzero) {
+    if (dl == 0) dl = 1;
+    if (du == 0) du = -1;
+  }
+  if (x_nonzero && dl < 0 && du > 0) {
+    tl = max(tl, min(xl, xu == kMinf ? kInf : -xu));
+    tu = min(tu, max(xl == kInf ? kMinf : -xl, xu));
+    return;
+  }
+  if (!(x_nonzero || dl > 0 || du < 0)) return;
+  if (xl == kMinf || xu == kInf || dl == kMinf || du == kInf) return;
+  if (dl > du) return;
+  if (min(xl, xu) >= 0 && dl > 0) {
+    // Non-negative numerators, positive divisors: ceil(a/b) and floor(a/b)
+    // are increasing in a and decreasing in b, so the extreme of the four
+    // quotients is at (min a, max b) resp. (max a, min b).
+    const uint32_t a_lo = (uint32_t)min(xl, xu), a_hi = (uint32_t)max(xl, xu);
+    const uint32_t b_lo = (uint32_t)dl, b_hi = (uint32_t)du;
+    const int32_t c = (int32_t)((a_lo + b_hi - 1u) / b_hi);
+    const int32_t f = (int32_t)(a_hi / b_lo);
+    tl = max(tl, c);
+    tu = min(tu, f);
+    return;
+  }
+  const int32_t c1 = d_ceil32(xl, dl), c2 = d_ceil32(xl, du), c3 = d_ceil32(xu, dl), c4 = d_ceil32(xu, du);
+  const int32_t f1 = d_floor32(xl, dl), f2 = d_floor32(xl, du), f3 = d_floor32(xu, dl), f4 = d_floor32(xu, du);
+  tl = max(tl, min(min(c1, c2), min(c3, c4)));
+  tu = min(tu, max(max(f1, f2), max(f3, f4)));
+}
+
 // ---- one propagator ------------------------------------------------------
 // In: loaded intervals X, Y, Z.  Out: the intervals to meet into x, y, z.
 // kDiv = false compiles the division propagators out (they need many more
 // registers); the engine uses that variant for tables without divisions.
 template <bool kDiv = true>
 __device__ __forceinline__ void deduce(int op, int2 X, int2 Y, int2 Z, int2& nx, int2& ny, int2& nz) {
-  Bnd x{X.x, X.y}, y{Y.x, Y.y}, z{Z.x, Z.y};
+  int32_t xl = X.x, xu = X.y, yl = Y.x, yu = Y.y, zl = Z.x, zu = Z.y;
   const int2 kTop = make_int2(kMinf, kInf);
   switch (op) {
     case kEq: {
       nx = kTop; ny = kTop; nz = kTop;
-      if (X.x == 1 && X.y == 1) {                  // x true: y = z
+      if (xl == 1 && xu == 1) {                    // x true: y = z
         ny = Z; nz = Y;
-      } else if (X.x == 0 && X.y == 0 && (Y.x == Y.y || Z.x == Z.y)) {
+      } else if (xl == 0 && xu == 0 && (yl == yu || zl == zu)) {
         // x false and one side fixed: shave that value off the other side
-        int2 cut = make_int2(Y.x == Z.x ? (int32_t)clamp32(y.l + 1) : kMinf,
-                             Y.y == Z.y ? (int32_t)clamp32(y.u - 1) : kInf);
-        if (Z.x == Z.y) ny = cut; else nz = cut;
-      } else if (Y.y == Z.x && Y.x == Z.y) {       // same singleton
+        const int2 cut = make_int2(yl == zl ? sadd(yl, 1) : kMinf, yu == zu ? ssub(yu, 1) : kInf);
+        if (zl == zu) ny = cut; else nz = cut;
+      } else if (yu == zl && yl == zu) {           // same singleton
         nx = make_int2(1, 1);
-      } else if (Y.x > Z.y || Y.y < Z.x) {         // disjoint
+      } else if (yl > zu || yu < zl) {             // disjoint
         nx = make_int2(0, 0);
       }
       return;
     }
     case kLeq: {
       nx = kTop; ny = kTop; nz = kTop;
-      if (X.x == 1 && X.y == 1) {                  // y <= z
-        ny = make_int2(Y.x, Z.y); nz = make_int2(Y.x, Z.y);
-      } else if (X.x == 0 && X.y == 0) {           // y > z
-        ny = make_int2((int32_t)clamp32(z.l + 1), Y.y);
-        nz = make_int2(Z.x, (int32_t)clamp32(y.u - 1));
-      } else if (Y.y <= Z.x) {
+      if (xl == 1 && xu == 1) {                    // y <= z
+        ny = make_int2(yl, zu); nz = make_int2(yl, zu);
+      } else if (xl == 0 && xu == 0) {             // y > z
+        ny = make_int2(sadd(zl, 1), yu);
+        nz = make_int2(zl, ssub(yu, 1));
+      } else if (yu <= zl) {
         nx = make_int2(1, 1);
-      } else if (Y.x > Z.y) {
+      } else if (yl > zu) {
         nx = make_int2(0, 0);
       }
       return;
     }
     case kAdd: {
-      if (y.l != kMinf && z.l != kMinf) x.l = mx(x.l, clamp32(y.l + z.l));
-      if (y.u != kInf && z.u != kInf)   x.u = mn(x.u, clamp32(y.u + z.u));
-      if (x.l != kMinf && z.u != kInf)  y.l = mx(y.l, clamp32(x.l - z.u));
-      if (x.u != kInf && z.l != kMinf)  y.u = mn(y.u, clamp32(x.u - z.l));
-      if (x.l != kMinf && y.u != kInf)  z.l = mx(z.l, clamp32(x.l - y.u));
-      if (x.u != kInf && y.l != kMinf)  z.u = mn(z.u, clamp32(x.u - y.l));
+      if (yl != kMinf && zl != kMinf) xl = max(xl, sadd(yl, zl));
+      if (yu != kInf && zu != kInf)   xu = min(xu, sadd(yu, zu));
+      if (xl != kMinf && zu != kInf)  yl = max(yl, ssub(xl, zu));
+      if (xu != kInf && zl != kMinf)  yu = min(yu, ssub(xu, zl));
+      if (xl != kMinf && yu != kInf)  zl = max(zl, ssub(xl, yu));
+      if (xu != kInf && yl != kMinf)  zu = min(zu, ssub(xu, yl));
       break;
     }
     case kMul: {
-      if (y.l != kMinf && y.u != kInf && z.l != kMinf && z.u != kInf) {
-        const i64 a = y.l * z.l, b = y.l * z.u, c = y.u * z.l, d = y.u * z.u;
-        x.l = mx(x.l, clamp32(mn(mn(a, b), mn(c, d))));
-        x.u = mn(x.u, clamp32(mx(mx(a, b), mx(c, d))));
-      }
-      mul_inverse(x, y, z);
-      mul_inverse(x, z, y);
+      mul_forward32(xl, xu, yl, yu, zl, zu);
+      mul_inverse32(xl, xu, yl, yu, zl, zu);
+      mul_inverse32(xl, xu, zl, zu, yl, yu);
       break;
     }
     case kMin: {
-      x.l = mx(x.l, mn(y.l, z.l));
-      x.u = mn(x.u, mn(y.u, z.u));
-      y.l = mx(y.l, x.l);
-      if (x.u < z.l) y.u = mn(y.u, x.u);
-      z.l = mx(z.l, x.l);
-      if (x.u < y.l) z.u = mn(z.u, x.u);
+      xl = max(xl, min(yl, zl));
+      xu = min(xu, min(yu, zu));
+      yl = max(yl, xl);
+      if (xu < zl) yu = min(yu, xu);
+      zl = max(zl, xl);
+      if (xu < yl) zu = min(zu, xu);
       break;
     }
     case kMax: {
-      x.l = mx(x.l, mx(y.l, z.l));
-      x.u = mn(x.u, mx(y.u, z.u));
-      y.u = mn(y.u, x.u);
-      if (x.l > z.u) y.l = mx(y.l, x.l);
-      z.u = mn(z.u, x.u);
-      if (x.l > y.u) z.l = mx(z.l, x.l);
+      xl = max(xl, max(yl, zl));
+      xu = min(xu, max(yu, zu));
+      yu = min(yu, xu);
+      if (xl > zu) yl = max(yl, xl);
+      zu = min(zu, xu);
+      if (xl > yu) zl = max(zl, xl);
       break;
     }
     case kTdiv: case kFdiv: case kCdiv: case kEdiv:
-      if constexpr (kDiv) prop_div(op, x, y, z);
+      if constexpr (kDiv) {
+        Bnd x{xl, xu}, y{yl, yu}, z{zl, zu};
+        prop_div(op, x, y, z);
+        xl = sat32(x.l); xu = sat32(x.u); yl = sat32(y.l); yu = sat32(y.u);
+        zl = sat32(z.l); zu = sat32(z.u);
+      }
       break;
     default:
       nx = kTop; ny = kTop; nz = kTop;
       return;
   }
-  nx = make_int2((int32_t)clamp32(x.l), (int32_t)clamp32(x.u));
-  ny = make_int2((int32_t)clamp32(y.l), (int32_t)clamp32(y.u));
-  nz = make_int2((int32_t)clamp32(z.l), (int32_t)clamp32(z.u));
+  nx = make_int2(xl, xu);
+  ny = make_int2(yl, yu);
+  nz = make_int2(zl, zu);
 }
 
 // Entailment of one propagator on its operands' intervals (pir.hpp:417-438).

@@ -33,7 +33,7 @@ def test_library_loads_and_exports_every_declared_symbol():
 def test_version_and_config():
     assert b"pcx" in pcx.lib().pcx_version()
     c = pcx.default_config()
-    assert c.props_per_wave == 256 and c.max_sweeps == 0
+    assert c.props_per_wave == 0 and c.max_sweeps == 0
 
 
 def test_opcodes_match_oracle():

@@ -75,10 +75,10 @@ def _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, nvars, what):
 
 
 @pytest.mark.parametrize("name", list(ol.OPS))
-@pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PLAIN])
+@pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PERSISTENT, pcx.MODE_PLAIN])
 def test_every_box_one_propagator(gpu, name, mode):
     """All boxes x,y,z in [-4,4] (91,125 instances) for one propagator."""
-    lo, hi = (-4, 4) if mode == pcx.MODE_BATCHED else (-2, 2)
+    lo, hi = (-2, 2) if mode == pcx.MODE_PLAIN else (-4, 4)
     X, Y, Z = _boxes(lo, hi)
     n = len(X)
     lb = np.stack([X[:, 0], Y[:, 0], Z[:, 0]], 1).astype(np.int32).ravel()
@@ -169,7 +169,7 @@ def test_full_size_network(gpu):
     assert ((lbg <= w) & (w <= ubg)).all()
 
 
-@pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PLAIN])
+@pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PERSISTENT, pcx.MODE_PLAIN])
 def test_batch_of_search_nodes(gpu, mode):
     """Shared table, per-instance subsets and splits (scaled-down batch)."""
     V, P, n_inst = 5000, 20000, 96
