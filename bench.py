@@ -14,7 +14,8 @@ Gauss-Seidel algorithm, restated) on the host cores; the GPU results of that
 sample are checked bit-for-bit against it.
 
 --config 1 instead times the single 1,000,000-variable / 4,000,000-propagator
-fixpoint (plain engine, one step = one fixpoint).
+fixpoint (plain engine, one step = one fixpoint); --config 3 the bitset store
+(1,024 bit-removal subproblems of a 200,000-word / 800,000-propagator network).
 
 Prints one JSON line (the driver's contract).
 """
@@ -217,12 +218,76 @@ def bench_single(args):
     return out
 
 
+def bench_bits(args):
+    """Bitset store: 200,000 words, 800,000 propagators; single instance and
+    a batch of 1,024 bit-removal subproblems (one step = the batch)."""
+    V, P, n_inst = 200000, 800000, 1024
+    op, x, y, z, w = pcx.gen_bits_network(args.seed, V, P)
+    pir = pcx.BitsPIR(props_per_wave=args.ppw, timing=1)
+    pir.tell(op, x, y, z, w)
+    root = w.copy()
+    t0 = time.perf_counter()
+    r_single = pir.fixpoint(root)
+    single_ms = (time.perf_counter() - t0) * 1e3
+    inst = pcx.gen_bits_splits(args.seed + 1, root, n_inst, 1, 20)
+    batch = pcx.Batch(pir, n_inst, V)
+    batch.load_words(inst)
+    mode = {"batched": pcx.MODE_BATCHED, "persistent": pcx.MODE_PERSISTENT}[args.mode]
+    for _ in range(args.warmup):
+        batch.reset()
+        batch.run(mode)
+    evals = tight = 0
+    kms = 0.0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        batch.reset()
+        res, st = batch.run(mode)
+        evals += st.evals
+        tight += st.tightened
+        kms += st.kernel_ms
+    elapsed = time.perf_counter() - t_start
+    fps = n_inst * args.steps / elapsed
+    words_g = batch.read_words()
+    rf = roofline(evals / args.steps, tight / args.steps, kms / args.steps)
+    rf["kernel"] = "k_sweep_batched<bits>" if args.mode == "batched" else "k_persistent<bits>"
+    out = {"value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
+           "config": {"workload": "bitset store: 1,024 bit-removal subproblems of one 200,000-word / "
+                                  "800,000-propagator network (add, neq, leq, eq-reif)",
+                      "nvars": V, "nprops": P, "instances": n_inst, "mode": args.mode},
+           "single_fixpoint_ms": single_ms, "single_sweeps": int(r_single.sweeps),
+           "evals_per_s": evals / elapsed, "failed_instances": int(sum(r.failed for r in res)),
+           "roofline": rf}
+    if not args.no_cpu:
+        ol = cpu_oracle()
+        top, tx, ty, tz = pir.table()
+        threads = host_threads()
+        k = 0
+        t_cpu = 0.0
+        mism = 0
+        while k < n_inst and t_cpu < args.cpu_seconds:
+            n = min(threads * 2, n_inst - k)
+            wo = inst[k * V:(k + n) * V].copy()
+            ts = time.perf_counter()
+            ro = ol.bits_fixpoint_batch(top, tx, ty, tz, wo, V, threads=threads)
+            t_cpu += time.perf_counter() - ts
+            for j in range(n):
+                g_fail = bool(res[k + j].failed)
+                if g_fail != bool(ro[j].failed) or (not g_fail and not np.array_equal(
+                        words_g[(k + j) * V:(k + j + 1) * V], wo[j * V:(j + 1) * V])):
+                    mism += 1
+            k += n
+        out["cpu_baseline"] = {"value": k / t_cpu, "unit": "fixpoints/s", "cores": threads, "kind": "port",
+                               "sample": f"first {k} of the {n_inst} instances, one per thread"}
+        out["parity"] = {"checked_instances": k, "mismatches": mism}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3])
     ap.add_argument("--mode", default="persistent", choices=["batched", "persistent"],
                     help="engine for the timed batch (the plain engine is always run once beside it)")
     ap.add_argument("--ppw", type=int, default=0, choices=[0, 64, 128, 256, 512],
@@ -239,7 +304,7 @@ def main():
     if pcx.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    out = bench_batched(args) if args.config == 2 else bench_single(args)
+    out = {1: bench_single, 2: bench_batched, 3: bench_bits}[args.config](args)
     line = {"metric": METRIC, "value": out.pop("value"), "unit": out.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out.pop("ms_per_step"),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",

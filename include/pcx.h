@@ -47,6 +47,21 @@ enum {
   PCX_NUM_OPS = 10
 };
 
+/* Store domains. */
+#define PCX_DOMAIN_INTERVAL 0   /* int32 lb/ub per variable                     */
+#define PCX_DOMAIN_BITSET   1   /* one 64-bit word per variable, bit v = value v */
+
+/* Opcodes of the bitset domain, numbered in the order PC sorts its formula
+ * kinds (pc.hpp:634-641 over the variant order of formula.hpp:905-920). */
+enum {
+  PCX_BLEQ = 0,   /* y <= z              Inequality   formula.hpp:796-805      */
+  PCX_BADD = 1,   /* x = y + z           Equality + Binary<GroupAdd> residuals */
+                  /*                     formula.hpp:672-681, terms.hpp:376-397 */
+  PCX_BNEQ = 2,   /* y != z              Disequality  formula.hpp:638-662      */
+  PCX_BEQR = 3,   /* x <=> (y = z)       Biconditional formula.hpp:421-427     */
+  PCX_BITS_NUM_OPS = 4
+};
+
 #define PCX_INF  INT32_MAX
 #define PCX_MINF INT32_MIN
 #define PCX_MAX_VARS (1 << 28)   /* variable indices must be < 2^28 */
@@ -61,7 +76,8 @@ typedef struct {
   int32_t sweeps_per_check; /* plain engine: sweeps queued between host checks (0=8) */
   int64_t max_sweeps;       /* 0 = run to the fixpoint                               */
   int32_t timing;           /* 1 = record HIP events around every sweep kernel       */
-  int32_t reserved[7];
+  int32_t domain;           /* PCX_DOMAIN_INTERVAL (default) or PCX_DOMAIN_BITSET    */
+  int32_t reserved[6];
 } pcx_config;
 
 typedef struct {
@@ -155,6 +171,23 @@ pcx_status pcx_batch_read(pcx_batch* b, int32_t* lb, int32_t* ub);
 /* Propagate every instance to its fixpoint.  results: n_inst entries or NULL. */
 pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results,
                          pcx_stats* stats);
+
+/* ---- bitset domain (engine created with config.domain = PCX_DOMAIN_BITSET)
+ * The store is one uint64 word per variable (bit v <=> value v in [0,63]);
+ * a zero word is empty.  Batches are created with pcx_batch_create and run
+ * with pcx_batch_run exactly as interval batches.
+ * pcx_bits_tell: append propagators, embed EQR outputs into {0,1} in
+ *   `words` (may be NULL), keep the table stably sorted by opcode.
+ *   replaces PC::interpret_tell + PC::deduce(tell_type) (pc.hpp:605-645). */
+pcx_status pcx_bits_tell(pcx_engine* e, const int32_t* op, const int32_t* x,
+                         const int32_t* y, const int32_t* z, int64_t n,
+                         uint64_t* words, int64_t nvars);
+/* replaces GaussSeidelIteration::fixpoint over PC::deduce(int)
+ * (tests/pc_bitset_test.cpp:45-61). */
+pcx_status pcx_bits_fixpoint(pcx_engine* e, uint64_t* words, int64_t nvars,
+                             pcx_result* out);
+pcx_status pcx_bits_batch_load(pcx_batch* b, const uint64_t* words);
+pcx_status pcx_bits_batch_read(pcx_batch* b, uint64_t* words);
 
 #ifdef __cplusplus
 }

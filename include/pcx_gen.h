@@ -74,6 +74,28 @@ pcx_status pcxg_splits(uint64_t seed, const int32_t* lb, const int32_t* ub, int6
 pcx_status pcxg_subsets(uint64_t seed, int64_t nprops, int64_t n_inst, int64_t mmin, int64_t mmax,
                         int64_t* off, int32_t* idx);
 
+/* Bitset network: nvars words, each value in [0,63] present with a density
+ * drawn per variable from U[dens_lo, dens_hi] (never empty); nprops
+ * propagators with opcodes drawn uniformly from `kinds` (bit k = opcode k of
+ * the bitset domain, 0 = all four), operands distinct.
+ *   planted = 0: operands uniform (a random network; it usually fails);
+ *   planted = 1: a hidden value per variable lies in its domain and every
+ *     propagator holds under it (x = y + z picks x holding w_y + w_z, y <= z
+ *     and y != z are oriented/redrawn to hold, an EQR output holds
+ *     [w_y = w_z]), so the root store cannot fail; `witness` (may be NULL)
+ *     receives the hidden values.
+ * Told order, not sorted. */
+pcx_status pcxg_bits_network(uint64_t seed, int64_t nvars, int64_t nprops, uint32_t kinds,
+                             double dens_lo, double dens_hi, int32_t planted,
+                             int32_t* op, int32_t* x, int32_t* y, int32_t* z, uint64_t* words,
+                             int32_t* witness);
+
+/* Bit-removal splits: n_inst copies of words[nvars], each with k ~
+ * U[kmin, kmax] removals of one present value from a random variable that
+ * holds at least two values. */
+pcx_status pcxg_bits_splits(uint64_t seed, const uint64_t* words, int64_t nvars, int64_t n_inst,
+                            int32_t kmin, int32_t kmax, uint64_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,11 @@ pcx_status sync_table(pcx_engine* e) {
   return PCX_OK;
 }
 
+int dom_of(const pcx_engine* e) {
+  if (e->cfg.domain == PCX_DOMAIN_BITSET) return kBits;
+  return e->has_div ? kItvDiv : kItv;
+}
+
 Table table_of(const pcx_engine* e) {
   return Table{e->d_xo, e->d_y, e->d_z, (int64_t)e->op.size()};
 }
@@ -189,7 +194,7 @@ pcx_status prepare_run(pcx_batch* b) {
   PCX_HIP(hipMemsetAsync(b->d_tight, 0, kTightSlots * sizeof(unsigned long long), e->stream));
   WorkList wl{b->d_act, b->d_item_base, b->d_ctl, b->d_ctl + 1, b->d_ctl + 2};
   PCX_HIP(launch_init_flags(b->d_flags, (int32_t)b->n_inst, wl, e->stream));
-  PCX_HIP(launch_store_check_empty(b->d_store, b->nvars, (int32_t)b->n_inst, b->d_flags, e->stream));
+  PCX_HIP(launch_store_check_empty(b->d_store, b->nvars, (int32_t)b->n_inst, b->d_flags, dom_of(e), e->stream));
   return PCX_OK;
 }
 
@@ -214,7 +219,7 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       for (int g = 0; g < group; ++g, ++s) {
         if (e->cfg.max_sweeps > 0 && s >= e->cfg.max_sweeps) break;
         if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
-        PCX_HIP(launch_sweep_plain(t, st, idx, m, s, f, b->d_tight, ppw, e->has_div, e->stream));
+        PCX_HIP(launch_sweep_plain(t, st, idx, m, s, f, b->d_tight, ppw, dom_of(e), e->stream));
         if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
         ++launches;
       }
@@ -258,7 +263,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream));
       if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
       PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
-                                   b->d_tight, ppw, e->has_div, grid, e->stream));
+                                   b->d_tight, ppw, dom_of(e), grid, e->stream));
       if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
       ++launches;
       ++sweeps;
@@ -285,14 +290,14 @@ pcx_status run_persistent(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   int dev_cus = 256;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, e->cfg.device) == hipSuccess) dev_cus = prop.multiProcessorCount;
-  const int per_cu = persistent_blocks_per_cu(e->has_div);
+  const int per_cu = persistent_blocks_per_cu(dom_of(e));
   const int grid = (int)std::min<int64_t>((int64_t)dev_cus * per_cu, b->n_inst);
   Subsets sub{b->h_off.empty() ? nullptr : b->d_off, b->d_idx};
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
   double kms = 0.0;
   if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, 0), e->stream));
   PCX_HIP(launch_persistent(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, b->d_order,
-                            (int32_t)b->n_inst, b->d_ctl + 3, b->d_tight, maxs, e->has_div, grid,
+                            (int32_t)b->n_inst, b->d_ctl + 3, b->d_tight, maxs, dom_of(e), grid,
                             e->stream));
   if (e->cfg.timing) {
     PCX_HIP(hipEventRecord(event_at(b, 1), e->stream));
@@ -335,6 +340,8 @@ pcx_status pcx_create(const pcx_config* cfg, pcx_engine** out) {
   *out = nullptr;
   pcx_config c;
   if (cfg) c = *cfg; else pcx_config_default(&c);
+  if (c.domain != PCX_DOMAIN_INTERVAL && c.domain != PCX_DOMAIN_BITSET)
+    return fail_with(PCX_EINVAL, "unknown domain");
   if (c.props_per_wave != 0 && !valid_ppw(c.props_per_wave))
     return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto), 64, 128, 256 or 512");
   int n = 0;
@@ -365,6 +372,7 @@ void pcx_destroy(pcx_engine* e) {
 pcx_status pcx_tell(pcx_engine* e, const int32_t* op, const int32_t* x, const int32_t* y,
                     const int32_t* z, int64_t n, int32_t* lb, int32_t* ub, int64_t nvars) {
   if (!e) return fail_with(PCX_EINVAL, "engine is NULL");
+  if (e->cfg.domain != PCX_DOMAIN_INTERVAL) return fail_with(PCX_EINVAL, "pcx_tell needs an interval engine");
   if (n < 0 || (n > 0 && (!op || !x || !y || !z))) return fail_with(PCX_EINVAL, "bad table arrays");
   if ((lb == nullptr) != (ub == nullptr)) return fail_with(PCX_EINVAL, "lb and ub must both be given or both be NULL");
   const int64_t lim = nvars > 0 ? std::min<int64_t>(nvars, PCX_MAX_VARS) : PCX_MAX_VARS;
@@ -436,11 +444,17 @@ int64_t pcx_snapshot(const pcx_engine* e) { return e ? (int64_t)e->op.size() : 0
 pcx_status pcx_restore(pcx_engine* e, int64_t snap) {
   if (!e) return fail_with(PCX_EINVAL, "engine is NULL");
   if (snap < 0 || snap > (int64_t)e->op.size()) return fail_with(PCX_EINVAL, "bad snapshot");
-  // The table is sorted, so "the propagators told since the snapshot" are
-  // not a suffix in general; the reference pops the last ones it appended
-  // (pir.hpp:863-870).  Keep the first `snap` told entries by tell order.
-  if (snap == (int64_t)e->op.size()) return PCX_OK;
-  return fail_with(PCX_ENOTSUP, "restore to a non-trivial snapshot needs tell-order tracking");
+  // Like the reference (pir.hpp:863-870), pop entries off the end of the
+  // table; the table is kept sorted, so these are the last in table order.
+  e->op.resize(snap); e->x.resize(snap); e->y.resize(snap); e->z.resize(snap);
+  e->max_var = -1;
+  e->has_div = false;
+  for (int64_t i = 0; i < snap; ++i) {
+    e->max_var = std::max(e->max_var, std::max(e->x[i], std::max(e->y[i], e->z[i])));
+    e->has_div |= e->op[i] >= PCX_TDIV && e->op[i] <= PCX_EDIV;
+  }
+  e->dirty = true;
+  return PCX_OK;
 }
 
 pcx_status pcx_batch_create(pcx_engine* e, int64_t n_inst, int64_t nvars, const int64_t* inst_off,
@@ -518,6 +532,7 @@ void pcx_batch_destroy(pcx_batch* b) {
 
 pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub) {
   if (!b || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
+  if (b->e->cfg.domain != PCX_DOMAIN_INTERVAL) return fail_with(PCX_EINVAL, "interval batch expected");
   pcx_engine* e = b->e;
   PCX_HIP(hipSetDevice(e->cfg.device));
   const int64_t n = b->n_inst * b->nvars;
@@ -542,6 +557,7 @@ pcx_status pcx_batch_reset(pcx_batch* b) {
 
 pcx_status pcx_batch_read(pcx_batch* b, int32_t* lb, int32_t* ub) {
   if (!b || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
+  if (b->e->cfg.domain != PCX_DOMAIN_INTERVAL) return fail_with(PCX_EINVAL, "interval batch expected");
   pcx_engine* e = b->e;
   PCX_HIP(hipSetDevice(e->cfg.device));
   const int64_t n = b->n_inst * b->nvars;
@@ -576,6 +592,7 @@ pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results, pcx_st
 
 pcx_status pcx_fixpoint(pcx_engine* e, int32_t* lb, int32_t* ub, int64_t nvars, pcx_result* out) {
   if (!e || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
+  if (e->cfg.domain != PCX_DOMAIN_INTERVAL) return fail_with(PCX_EINVAL, "pcx_fixpoint needs an interval engine");
   if (nvars <= 0) return fail_with(PCX_EINVAL, "nvars must be positive");
   if (e->max_var >= nvars) return fail_with(PCX_EINVAL, "a propagator names a variable outside the store");
   if (!e->scratch || e->scratch_nvars != nvars || e->scratch_table != (int64_t)e->op.size()) {
@@ -600,6 +617,7 @@ pcx_status pcx_fixpoint(pcx_engine* e, int32_t* lb, int32_t* ub, int64_t nvars, 
 pcx_status pcx_ask_all(pcx_engine* e, const int32_t* lb, const int32_t* ub, int64_t nvars,
                        uint8_t* entailed, int32_t* all) {
   if (!e || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
+  if (e->cfg.domain != PCX_DOMAIN_INTERVAL) return fail_with(PCX_EINVAL, "pcx_ask_all needs an interval engine");
   if (e->max_var >= nvars) return fail_with(PCX_EINVAL, "a propagator names a variable outside the store");
   PCX_HIP(hipSetDevice(e->cfg.device));
   pcx_status s = sync_table(e);
@@ -627,6 +645,92 @@ pcx_status pcx_ask_all(pcx_engine* e, const int32_t* lb, const int32_t* ub, int6
   (void)hipFree(d_lb); (void)hipFree(d_ub); (void)hipFree(d_st); (void)hipFree(d_all); (void)hipFree(d_ent);
   if (he != hipSuccess) return fail_with(PCX_EDEVICE, std::string("ask: ") + hipGetErrorString(he));
   if (all) *all = h_all;
+  return PCX_OK;
+}
+
+// ---- bitset domain --------------------------------------------------------
+pcx_status pcx_bits_tell(pcx_engine* e, const int32_t* op, const int32_t* x, const int32_t* y,
+                         const int32_t* z, int64_t n, uint64_t* words, int64_t nvars) {
+  if (!e) return fail_with(PCX_EINVAL, "engine is NULL");
+  if (e->cfg.domain != PCX_DOMAIN_BITSET) return fail_with(PCX_EINVAL, "pcx_bits_tell needs a bitset engine");
+  if (n < 0 || (n > 0 && (!op || !x || !y || !z))) return fail_with(PCX_EINVAL, "bad table arrays");
+  if (words && nvars <= 0) return fail_with(PCX_EINVAL, "nvars must be given with a store");
+  const int64_t lim = nvars > 0 ? std::min<int64_t>(nvars, PCX_MAX_VARS) : PCX_MAX_VARS;
+  for (int64_t i = 0; i < n; ++i) {
+    if (op[i] < 0 || op[i] >= PCX_BITS_NUM_OPS)
+      return fail_with(PCX_EINVAL, "unknown bitset opcode at " + std::to_string(i));
+    if (x[i] < 0 || x[i] >= lim || y[i] < 0 || y[i] >= lim || z[i] < 0 || z[i] >= lim)
+      return fail_with(PCX_EINVAL, "variable index out of range at " + std::to_string(i));
+  }
+  // Reified outputs are Boolean literals: x <- x & {0,1}.
+  if (words)
+    for (int64_t i = 0; i < n; ++i)
+      if (op[i] == PCX_BEQR) words[x[i]] &= 3ull;
+  // PC keeps its propagators ordered by formula kind (pc.hpp:634-641); the
+  // bitset opcodes are numbered in that order, so a stable sort by opcode.
+  const int64_t old = (int64_t)e->op.size();
+  std::vector<int64_t> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return op[a] < op[b]; });
+  struct Bc { int32_t op, x, y, z; };
+  std::vector<Bc> all(old + n);
+  for (int64_t i = 0; i < old; ++i) all[i] = Bc{e->op[i], e->x[i], e->y[i], e->z[i]};
+  for (int64_t i = 0; i < n; ++i) all[old + i] = Bc{op[ord[i]], x[ord[i]], y[ord[i]], z[ord[i]]};
+  std::inplace_merge(all.begin(), all.begin() + old, all.end(),
+                     [](const Bc& a, const Bc& b) { return a.op < b.op; });
+  e->op.resize(old + n); e->x.resize(old + n); e->y.resize(old + n); e->z.resize(old + n);
+  for (int64_t i = 0; i < old + n; ++i) {
+    e->op[i] = all[i].op; e->x[i] = all[i].x; e->y[i] = all[i].y; e->z[i] = all[i].z;
+    e->max_var = std::max(e->max_var, std::max(all[i].x, std::max(all[i].y, all[i].z)));
+  }
+  e->dirty = true;
+  return PCX_OK;
+}
+
+pcx_status pcx_bits_batch_load(pcx_batch* b, const uint64_t* words) {
+  if (!b || !words) return fail_with(PCX_EINVAL, "NULL argument");
+  if (b->e->cfg.domain != PCX_DOMAIN_BITSET) return fail_with(PCX_EINVAL, "bitset batch expected");
+  pcx_engine* e = b->e;
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  const int64_t n = b->n_inst * b->nvars;
+  PCX_HIP(hipMemcpyAsync(b->d_image, words, n * sizeof(uint64_t), hipMemcpyHostToDevice, e->stream));
+  PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, e->stream));
+  PCX_HIP(hipStreamSynchronize(e->stream));
+  return PCX_OK;
+}
+
+pcx_status pcx_bits_batch_read(pcx_batch* b, uint64_t* words) {
+  if (!b || !words) return fail_with(PCX_EINVAL, "NULL argument");
+  if (b->e->cfg.domain != PCX_DOMAIN_BITSET) return fail_with(PCX_EINVAL, "bitset batch expected");
+  pcx_engine* e = b->e;
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  PCX_HIP(hipMemcpyAsync(words, b->d_store, b->n_inst * b->nvars * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost, e->stream));
+  PCX_HIP(hipStreamSynchronize(e->stream));
+  return PCX_OK;
+}
+
+pcx_status pcx_bits_fixpoint(pcx_engine* e, uint64_t* words, int64_t nvars, pcx_result* out) {
+  if (!e || !words) return fail_with(PCX_EINVAL, "NULL argument");
+  if (e->cfg.domain != PCX_DOMAIN_BITSET) return fail_with(PCX_EINVAL, "pcx_bits_fixpoint needs a bitset engine");
+  if (nvars <= 0) return fail_with(PCX_EINVAL, "nvars must be positive");
+  if (e->max_var >= nvars) return fail_with(PCX_EINVAL, "a propagator names a variable outside the store");
+  if (!e->scratch || e->scratch_nvars != nvars || e->scratch_table != (int64_t)e->op.size()) {
+    if (e->scratch) pcx_batch_destroy(e->scratch);
+    e->scratch = nullptr;
+    pcx_status s = pcx_batch_create(e, 1, nvars, nullptr, nullptr, &e->scratch);
+    if (s != PCX_OK) return s;
+    e->scratch_nvars = nvars;
+    e->scratch_table = (int64_t)e->op.size();
+  }
+  pcx_status s = pcx_bits_batch_load(e->scratch, words);
+  if (s != PCX_OK) return s;
+  pcx_result r{};
+  s = pcx_batch_run(e->scratch, PCX_MODE_PLAIN, &r, nullptr);
+  if (s != PCX_OK) return s;
+  s = pcx_bits_batch_read(e->scratch, words);
+  if (s != PCX_OK) return s;
+  if (out) *out = r;
   return PCX_OK;
 }
 

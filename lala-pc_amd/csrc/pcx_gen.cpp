@@ -418,4 +418,94 @@ pcx_status pcxg_subsets(uint64_t seed, int64_t nprops, int64_t n_inst, int64_t m
   return PCX_OK;
 }
 
+pcx_status pcxg_bits_network(uint64_t seed, int64_t nvars, int64_t nprops, uint32_t kinds,
+                             double dens_lo, double dens_hi, int32_t planted,
+                             int32_t* op, int32_t* x, int32_t* y, int32_t* z, uint64_t* words,
+                             int32_t* witness) {
+  if (!op || !x || !y || !z || !words || nvars < 3 || nvars > PCX_MAX_VARS || nprops < 0 ||
+      dens_lo < 0 || dens_hi > 1 || dens_lo > dens_hi)
+    return PCX_EINVAL;
+  const uint32_t mask = kinds ? kinds : 0xfu;
+  std::vector<int> ks;
+  for (int k = 0; k < PCX_BITS_NUM_OPS; ++k)
+    if (mask & (1u << k)) ks.push_back(k);
+  if (ks.empty()) return PCX_EINVAL;
+  SplitMix64 r(stream_seed(seed, 31));
+  std::vector<int32_t> w(nvars, 0);
+  for (int64_t v = 0; v < nvars; ++v) {
+    const double d = dens_lo + (dens_hi - dens_lo) * r.unit();
+    uint64_t word = 0;
+    for (int b = 0; b < 64; ++b)
+      if (r.unit() < d) word |= 1ull << b;
+    w[v] = (int32_t)r.below(64);
+    if (planted) word |= 1ull << w[v];
+    if (!word) word = 1ull << w[v];
+    words[v] = word;
+  }
+  if (witness) std::copy(w.begin(), w.end(), witness);
+  // value -> variables holding it as hidden value
+  std::vector<int64_t> off(65, 0);
+  std::vector<int32_t> by;
+  if (planted) {
+    for (int64_t v = 0; v < nvars; ++v) off[w[v] + 1]++;
+    for (int i = 0; i < 64; ++i) off[i + 1] += off[i];
+    by.resize(nvars);
+    std::vector<int64_t> fill(off.begin(), off.end() - 1);
+    for (int64_t v = 0; v < nvars; ++v) by[fill[w[v]]++] = (int32_t)v;
+  }
+  auto holding = [&](int64_t val, int32_t a, int32_t b) -> int32_t {
+    if (val < 0 || val > 63 || off[val + 1] == off[val]) return -1;
+    for (int t = 0; t < 8; ++t) {
+      const int32_t v = by[off[val] + (int64_t)r.below((uint64_t)(off[val + 1] - off[val]))];
+      if (v != a && v != b) return v;
+    }
+    return -1;
+  };
+  for (int64_t i = 0; i < nprops; ++i) {
+    const int k = ks[r.below(ks.size())];
+    int32_t a = -1, b = -1, c = -1;
+    for (int attempt = 0; attempt < 1024; ++attempt) {
+      b = (int32_t)r.below((uint64_t)nvars);
+      do { c = (int32_t)r.below((uint64_t)nvars); } while (c == b);
+      if (!planted) {
+        do { a = (int32_t)r.below((uint64_t)nvars); } while (a == b || a == c);
+        break;
+      }
+      if (k == PCX_BADD) { a = holding((int64_t)w[b] + w[c], b, c); if (a >= 0) break; continue; }
+      if (k == PCX_BEQR) { a = holding(w[b] == w[c] ? 1 : 0, b, c); if (a >= 0) break; continue; }
+      if (k == PCX_BNEQ && w[b] == w[c]) continue;
+      if (k == PCX_BLEQ && w[b] > w[c]) std::swap(b, c);
+      do { a = (int32_t)r.below((uint64_t)nvars); } while (a == b || a == c);
+      break;
+    }
+    if (a < 0) return PCX_EINVAL;   // no variable holds the needed value
+    op[i] = k; x[i] = a; y[i] = b; z[i] = c;
+  }
+  return PCX_OK;
+}
+
+pcx_status pcxg_bits_splits(uint64_t seed, const uint64_t* words, int64_t nvars, int64_t n_inst,
+                            int32_t kmin, int32_t kmax, uint64_t* out) {
+  if (!words || !out || nvars <= 0 || n_inst < 0 || kmin < 0 || kmax < kmin) return PCX_EINVAL;
+  parallel_for(n_inst, [&](int64_t k) {
+    SplitMix64 r(stream_seed(seed, 5000011ULL + (uint64_t)k));
+    uint64_t* w = out + k * nvars;
+    std::memcpy(w, words, nvars * sizeof(uint64_t));
+    const int64_t n = r.range(kmin, kmax);
+    for (int64_t s = 0; s < n; ++s) {
+      for (int t = 0; t < 64; ++t) {
+        const int64_t v = (int64_t)r.below((uint64_t)nvars);
+        const int pc = __builtin_popcountll(w[v]);
+        if (pc < 2) continue;
+        int pick = (int)r.below((uint64_t)pc);
+        uint64_t m = w[v];
+        while (pick--) m &= m - 1;           // drop the lowest set bits
+        w[v] &= ~(m & (~m + 1));             // remove the chosen value
+        break;
+      }
+    }
+  });
+  return PCX_OK;
+}
+
 }  // extern "C"

@@ -4,8 +4,10 @@
 //   table   : three int32 SoA streams, one entry per propagator in table
 //             order (sorted by (op, y, x, z)):
 //               xo[i] = x | op << 28,  y[i],  z[i]          (12 B / propagator)
-//   store   : int2 {lb, ub} per variable, one slice of nvars per instance, so
-//             each operand gather is one 8-byte load from one cache line.
+//   store   : int2 {lb, ub} per variable (interval domain) or one 64-bit
+//             word per variable (bitset domain), one slice of nvars per
+//             instance, so each operand gather is one 8-byte load from one
+//             cache line.
 //   flags   : per instance, InstFlags (parity-indexed "changed" words so a
 //             sweep reads the previous sweep's word while writing its own).
 #pragma once
@@ -21,6 +23,11 @@ constexpr int kBlock = 256;            // 4 waves per workgroup
 constexpr int kNoFail = 0x7fffffff;
 constexpr int kTightSlots = 64;        // striped tightening counters
 constexpr int kXcds = 8;               // accelerator complex dies on MI355X
+
+// Kernel instantiations per store domain.
+constexpr int kItv = 0;       // int32 intervals, tables without division
+constexpr int kItvDiv = 1;    // int32 intervals, any opcode
+constexpr int kBits = 2;      // 64-bit bitset words
 
 struct Table {
   const uint32_t* __restrict__ xo;
@@ -60,23 +67,23 @@ struct WorkList {
 namespace pcx {
 hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, int64_t m,
                               int32_t sweep, InstFlags* flags, unsigned long long* tight,
-                              int ppw, bool div, hipStream_t st);
+                              int ppw, int dom, hipStream_t st);
 hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags, int32_t n_inst,
                            WorkList wl, int ppw, int32_t max_sweeps, hipStream_t st);
 hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                                 int64_t n_props, InstFlags* flags, WorkList wl,
-                                unsigned long long* tight, int ppw, bool div, int grid,
+                                unsigned long long* tight, int ppw, int dom, int grid,
                                 hipStream_t st);
 hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                              int64_t n_props, InstFlags* flags, const int32_t* order,
                              int32_t n_inst, int32_t* queue, unsigned long long* tight,
-                             int32_t max_sweeps, bool div, int grid, hipStream_t st);
-int persistent_blocks_per_cu(bool div);   // resident workgroups per CU
+                             int32_t max_sweeps, int dom, int grid, hipStream_t st);
+int persistent_blocks_per_cu(int dom);   // resident workgroups per CU
 hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipStream_t st);
 hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
                       hipStream_t st);
 hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n, hipStream_t st);
 hipError_t launch_store_unpack(const int2* store, int32_t* lb, int32_t* ub, int64_t n, hipStream_t st);
 hipError_t launch_store_check_empty(const int2* store, int64_t nvars, int32_t n_inst,
-                                    InstFlags* flags, hipStream_t st);
+                                    InstFlags* flags, int dom, hipStream_t st);
 }  // namespace pcx

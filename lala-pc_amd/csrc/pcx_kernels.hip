@@ -1,13 +1,18 @@
-// pcx_kernels.hip -- gfx950 kernels of the interval fixpoint engine.
+// pcx_kernels.hip -- gfx950 kernels of the fixpoint engine.
 //
 // One thread evaluates one propagator per step: it reads the 12-byte table
-// entry (three coalesced streams), gathers the three operand intervals (one
-// 8-byte load each), runs the bounds propagator, and tightens the store with
-// atomicMax on lb / atomicMin on ub only where its result is tighter than what
-// it read.  "changed" and "failed" are reduced per wavefront with ballots and
-// written once per wave.  Kernel boundaries order the sweeps: a sweep reads
-// the store as the previous kernel left it, so a sweep in which no atomic
-// tightened anything saw one constant store and proves the fixpoint.
+// entry (three coalesced streams), gathers its three operands (one 8-byte
+// load each: an {lb, ub} interval or a 64-bit bitset word), runs the
+// propagator, and tightens the store with atomics (atomicMax on lb /
+// atomicMin on ub, atomicAnd on words) only where its result is tighter than
+// what it read.  "changed" and "failed" are reduced per wavefront with
+// ballots and written once per wave.  Kernel boundaries order the sweeps: a
+// sweep reads the store as the previous kernel left it, so a sweep in which
+// no atomic tightened anything saw one constant store and proves the
+// fixpoint.  Every kernel is instantiated per domain D:
+//   kItv     interval store, no division propagators (fewer registers)
+//   kItvDiv  interval store, all ten opcodes
+//   kBits    bitset store
 #include "pcx_kernels.h"
 #include "pcx_prop.h"
 
@@ -27,36 +32,80 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
-// Meet `n` into variable v, whose value was read as `cur`.
-__device__ __forceinline__ void meet_var(int2* __restrict__ st, int32_t v, int2 cur, int2 n,
+// ---- store access policies ------------------------------------------------
+// Agent scope (sweep kernels: ordered by kernel boundaries) or workgroup
+// scope (persistent kernel: one workgroup owns an instance).
+template <int SCOPE>
+__device__ __forceinline__ u64 load_word(const void* p) {
+  if constexpr (SCOPE == __HIP_MEMORY_SCOPE_AGENT) return *reinterpret_cast<const u64*>(p);
+  else return __hip_atomic_load(reinterpret_cast<const u64*>(p), __ATOMIC_RELAXED, SCOPE);
+}
+__device__ __forceinline__ int2 as_itv(u64 v) {
+  return make_int2((int32_t)(uint32_t)v, (int32_t)(uint32_t)(v >> 32));
+}
+
+// Meet interval `n` into variable v, whose value was read as `cur`.
+template <int SCOPE>
+__device__ __forceinline__ void meet_itv(int2* __restrict__ st, int32_t v, int2 cur, int2 n,
                                          bool& chg, bool& fail, int& nt) {
   if (n.x > cur.x) {
-    const int old = atomicMax(&st[v].x, n.x);
+    const int old = __hip_atomic_fetch_max(&st[v].x, n.x, __ATOMIC_RELAXED, SCOPE);
     if (old < n.x) { chg = true; ++nt; }
     if (n.x > cur.y) fail = true;          // cur.y >= current ub
   }
   if (n.y < cur.y) {
-    const int old = atomicMin(&st[v].y, n.y);
+    const int old = __hip_atomic_fetch_min(&st[v].y, n.y, __ATOMIC_RELAXED, SCOPE);
     if (old > n.y) { chg = true; ++nt; }
     if (n.y < cur.x) fail = true;
   }
 }
 
-template <bool DIV>
+// AND mask `n` into word v, whose value was read as `cur`.
+template <int SCOPE>
+__device__ __forceinline__ void meet_word(u64* __restrict__ st, int32_t v, u64 cur, u64 n,
+                                          bool& chg, bool& fail, int& nt) {
+  if ((cur & n) != cur) {
+    const u64 old = __hip_atomic_fetch_and(&st[v], n, __ATOMIC_RELAXED, SCOPE);
+    if ((old & n) != old) { chg = true; ++nt; }
+    if ((old & n) == 0) fail = true;       // the word is now empty
+  }
+}
+
+// Apply one propagator to operands already loaded.
+template <int D, int SCOPE>
+__device__ __forceinline__ void apply(int2* __restrict__ st, int op, int32_t x, int32_t y,
+                                      int32_t z, u64 wx, u64 wy, u64 wz, bool& chg, bool& fail,
+                                      int& nt) {
+  if constexpr (D == kBits) {
+    u64* ws = reinterpret_cast<u64*>(st);
+    if (!wx || !wy || !wz) { fail = true; return; }
+    u64 nx, ny, nz;
+    deduce_bits(op, wx, wy, wz, nx, ny, nz);
+    meet_word<SCOPE>(ws, x, wx, nx, chg, fail, nt);
+    meet_word<SCOPE>(ws, y, wy, ny, chg, fail, nt);
+    meet_word<SCOPE>(ws, z, wz, nz, chg, fail, nt);
+  } else {
+    const int2 X = as_itv(wx), Y = as_itv(wy), Z = as_itv(wz);
+    if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = true; return; }
+    int2 nx, ny, nz;
+    deduce<D == kItvDiv>(op, X, Y, Z, nx, ny, nz);
+    if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
+    meet_itv<SCOPE>(st, x, X, nx, chg, fail, nt);
+    meet_itv<SCOPE>(st, y, Y, ny, chg, fail, nt);
+    meet_itv<SCOPE>(st, z, Z, nz, chg, fail, nt);
+  }
+}
+
+template <int D>
 __device__ __forceinline__ void eval_prop(const Table& t, int2* __restrict__ st, int64_t i,
                                           bool& chg, bool& fail, int& nt) {
   const uint32_t xo = t.xo[i];
   const int32_t y = t.y[i], z = t.z[i];
-  const int op = (int)(xo >> kOpShift);
   const int32_t x = (int32_t)(xo & kVarMask);
-  const int2 X = st[x], Y = st[y], Z = st[z];
-  if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = true; return; }
-  int2 nx, ny, nz;
-  deduce<DIV>(op, X, Y, Z, nx, ny, nz);
-  if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
-  meet_var(st, x, X, nx, chg, fail, nt);
-  meet_var(st, y, Y, ny, chg, fail, nt);
-  meet_var(st, z, Z, nz, chg, fail, nt);
+  const u64 wx = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + x);
+  const u64 wy = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + y);
+  const u64 wz = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + z);
+  apply<D, __HIP_MEMORY_SCOPE_AGENT>(st, (int)(xo >> kOpShift), x, y, z, wx, wy, wz, chg, fail, nt);
 }
 
 // Publish a wave's verdict: one store per flag, one atomic for the count.
@@ -74,7 +123,7 @@ __device__ __forceinline__ void publish(InstFlags* f, int32_t s, bool chg, bool 
 }
 
 // ---- plain mode: one instance, one sweep per launch -----------------------
-template <int PPL, bool DIV>
+template <int PPL, int D>
 __global__ __launch_bounds__(kBlock) void k_sweep_plain(Table t, int2* __restrict__ st,
                                                         const int32_t* __restrict__ idx, int64_t m,
                                                         int32_t s, InstFlags* f,
@@ -87,33 +136,39 @@ __global__ __launch_bounds__(kBlock) void k_sweep_plain(Table t, int2* __restric
 #pragma unroll
   for (int j = 0; j < PPL; ++j) {
     const int64_t k = base + j * kBlock + threadIdx.x;
-    if (k < m) eval_prop<DIV>(t, st, prop_at(idx, k), chg, fail, nt);
+    if (k < m) eval_prop<D>(t, st, prop_at(idx, k), chg, fail, nt);
   }
   publish(f, s, chg, fail, nt, tight);
 }
 
-hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, int64_t m,
-                              int32_t sweep, InstFlags* flags, unsigned long long* tight,
-                              int ppw, bool div, hipStream_t stream) {
-  const int ppl = ppw / kWave;
+template <int D>
+hipError_t launch_plain_d(const Table& t, int2* store, const int32_t* idx, int64_t m,
+                          int32_t sweep, InstFlags* flags, unsigned long long* tight, int ppl,
+                          hipStream_t stream) {
   const int64_t per_block = (int64_t)kBlock * ppl;
   const int64_t grid = (m + per_block - 1) / per_block;
   if (grid <= 0) return hipSuccess;
   const dim3 g((unsigned)grid), b(kBlock);
-#define PCX_PLAIN(P, D) k_sweep_plain<P, D><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight)
-  switch (ppl * 2 + (div ? 1 : 0)) {
-    case 2: PCX_PLAIN(1, false); break;
-    case 3: PCX_PLAIN(1, true); break;
-    case 4: PCX_PLAIN(2, false); break;
-    case 5: PCX_PLAIN(2, true); break;
-    case 8: PCX_PLAIN(4, false); break;
-    case 9: PCX_PLAIN(4, true); break;
-    case 16: PCX_PLAIN(8, false); break;
-    case 17: PCX_PLAIN(8, true); break;
+  switch (ppl) {
+    case 1: k_sweep_plain<1, D><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
+    case 2: k_sweep_plain<2, D><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
+    case 4: k_sweep_plain<4, D><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
+    case 8: k_sweep_plain<8, D><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
     default: return hipErrorInvalidValue;
   }
-#undef PCX_PLAIN
   return hipGetLastError();
+}
+
+hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, int64_t m,
+                              int32_t sweep, InstFlags* flags, unsigned long long* tight,
+                              int ppw, int dom, hipStream_t stream) {
+  const int ppl = ppw / kWave;
+  switch (dom) {
+    case kItv: return launch_plain_d<kItv>(t, store, idx, m, sweep, flags, tight, ppl, stream);
+    case kItvDiv: return launch_plain_d<kItvDiv>(t, store, idx, m, sweep, flags, tight, ppl, stream);
+    case kBits: return launch_plain_d<kBits>(t, store, idx, m, sweep, flags, tight, ppl, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // ---- batched mode: device scheduler + grid-stride sweep over work items ---
@@ -185,7 +240,7 @@ hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags
   return hipGetLastError();
 }
 
-template <int PPL, bool DIV>
+template <int PPL, int D>
 __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restrict__ store,
                                                           int64_t nvars, Subsets sub,
                                                           int64_t n_props, InstFlags* flags,
@@ -223,83 +278,56 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
         const int64_t q = first + j * kBlock + threadIdx.x;
-        if (q < m) eval_prop<DIV>(t, st, prop_at(idx, q), chg, fail, nt);
+        if (q < m) eval_prop<D>(t, st, prop_at(idx, q), chg, fail, nt);
       }
     }
     publish(f, s, chg, fail, nt, tight);
   }
 }
 
-hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
-                                int64_t n_props, InstFlags* flags, WorkList wl,
-                                unsigned long long* tight, int ppw, bool div, int grid,
-                                hipStream_t stream) {
-#define PCX_BATCH(P, D) \
-  k_sweep_batched<P, D><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight)
-  switch ((ppw / kWave) * 2 + (div ? 1 : 0)) {
-    case 2: PCX_BATCH(1, false); break;
-    case 3: PCX_BATCH(1, true); break;
-    case 4: PCX_BATCH(2, false); break;
-    case 5: PCX_BATCH(2, true); break;
-    case 8: PCX_BATCH(4, false); break;
-    case 9: PCX_BATCH(4, true); break;
-    case 16: PCX_BATCH(8, false); break;
-    case 17: PCX_BATCH(8, true); break;
+template <int D>
+hipError_t launch_batched_d(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
+                            int64_t n_props, InstFlags* flags, WorkList wl,
+                            unsigned long long* tight, int ppl, int grid, hipStream_t stream) {
+  switch (ppl) {
+    case 1: k_sweep_batched<1, D><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight); break;
+    case 2: k_sweep_batched<2, D><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight); break;
+    case 4: k_sweep_batched<4, D><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight); break;
+    case 8: k_sweep_batched<8, D><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight); break;
     default: return hipErrorInvalidValue;
   }
-#undef PCX_BATCH
   return hipGetLastError();
+}
+
+hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
+                                int64_t n_props, InstFlags* flags, WorkList wl,
+                                unsigned long long* tight, int ppw, int dom, int grid,
+                                hipStream_t stream) {
+  const int ppl = ppw / kWave;
+  switch (dom) {
+    case kItv: return launch_batched_d<kItv>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
+    case kItvDiv: return launch_batched_d<kItvDiv>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
+    case kBits: return launch_batched_d<kBits>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // ---- persistent mode: one launch, a device queue of instances -------------
 // Each workgroup takes the next instance from the queue (largest first) and
-// runs all of its sweeps itself, separated by workgroup barriers, so the
-// instance's store is only touched from one CU and stays in that XCD's L2 /
-// the MALL for the whole fixpoint.  Store accesses are workgroup-scoped
-// atomics; the barrier between sweeps orders one sweep's tightenings before
-// the next sweep's reads, which is what the termination argument needs.
+// runs all of its sweeps itself, separated by workgroup barriers.  Store
+// accesses are workgroup-scoped atomics; the barrier between sweeps orders
+// one sweep's tightenings before the next sweep's reads, which is what the
+// termination argument needs.
 constexpr int kPBlock = 512;     // 8 waves per workgroup
 
-__device__ __forceinline__ int2 wg_load(const int2* p) {
-  const unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  return make_int2((int32_t)(uint32_t)v, (int32_t)(uint32_t)(v >> 32));
-}
-
-__device__ __forceinline__ void wg_meet(int2* __restrict__ st, int32_t v, int2 cur, int2 n,
-                                        bool& chg, bool& fail, int& nt) {
-  if (n.x > cur.x) {
-    const int old = __hip_atomic_fetch_max(&st[v].x, n.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old < n.x) { chg = true; ++nt; }
-    if (n.x > cur.y) fail = true;
-  }
-  if (n.y < cur.y) {
-    const int old = __hip_atomic_fetch_min(&st[v].y, n.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old > n.y) { chg = true; ++nt; }
-    if (n.y < cur.x) fail = true;
-  }
-}
-
-template <bool DIV>
-__device__ __forceinline__ void apply_wg(int2* __restrict__ st, int op, int32_t x, int32_t y,
-                                         int32_t z, int2 X, int2 Y, int2 Z, bool& chg, bool& fail,
-                                         int& nt) {
-  if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = true; return; }
-  int2 nx, ny, nz;
-  deduce<DIV>(op, X, Y, Z, nx, ny, nz);
-  if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
-  wg_meet(st, x, X, nx, chg, fail, nt);
-  wg_meet(st, y, Y, ny, chg, fail, nt);
-  wg_meet(st, z, Z, nz, chg, fail, nt);
-}
-
-template <bool DIV>
+template <int D>
 __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restrict__ store,
                                                         int64_t nvars, Subsets sub, int64_t n_props,
                                                         InstFlags* flags, const int32_t* order,
                                                         int32_t n_inst, int32_t* queue,
                                                         unsigned long long* tight,
                                                         int32_t max_sweeps) {
+  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
   __shared__ int32_t s_slot, s_chg, s_fail;
   const int lane = lane_id();
   for (;;) {
@@ -322,7 +350,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
         __syncthreads();
         bool chg = false, fail = false;
         // Two propagators per lane per step: both sets of loads are issued
-        // before either set of atomics (reading a bound that another lane is
+        // before either set of atomics (reading a bound another lane is
         // about to tighten is allowed: bounds only shrink).
         for (int64_t q = threadIdx.x; q < m; q += 2 * kPBlock) {
           const int64_t q2 = q + kPBlock;
@@ -334,11 +362,11 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
           int32_t y2 = 0, z2 = 0;
           if (i2 >= 0) { xo2 = t.xo[i2]; y2 = t.y[i2]; z2 = t.z[i2]; }
           const int32_t x1 = (int32_t)(xo1 & kVarMask), x2 = (int32_t)(xo2 & kVarMask);
-          const int2 X1 = wg_load(st + x1), Y1 = wg_load(st + y1), Z1 = wg_load(st + z1);
-          int2 X2 = make_int2(0, 0), Y2 = X2, Z2 = X2;
-          if (i2 >= 0) { X2 = wg_load(st + x2); Y2 = wg_load(st + y2); Z2 = wg_load(st + z2); }
-          apply_wg<DIV>(st, (int)(xo1 >> kOpShift), x1, y1, z1, X1, Y1, Z1, chg, fail, nt);
-          if (i2 >= 0) apply_wg<DIV>(st, (int)(xo2 >> kOpShift), x2, y2, z2, X2, Y2, Z2, chg, fail, nt);
+          const u64 a1 = load_word<WG>(st + x1), b1 = load_word<WG>(st + y1), c1 = load_word<WG>(st + z1);
+          u64 a2 = 0, b2 = 0, c2 = 0;
+          if (i2 >= 0) { a2 = load_word<WG>(st + x2); b2 = load_word<WG>(st + y2); c2 = load_word<WG>(st + z2); }
+          apply<D, WG>(st, (int)(xo1 >> kOpShift), x1, y1, z1, a1, b1, c1, chg, fail, nt);
+          if (i2 >= 0) apply<D, WG>(st, (int)(xo2 >> kOpShift), x2, y2, z2, a2, b2, c2, chg, fail, nt);
         }
         const bool wc = __ballot(chg) != 0, wf = __ballot(fail) != 0;
         if (lane == 0 && wc) s_chg = 1;
@@ -363,23 +391,31 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
   }
 }
 
-int persistent_blocks_per_cu(bool div) {
+int persistent_blocks_per_cu(int dom) {
   int n = 0;
-  hipError_t e = div ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<true>, kPBlock, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<false>, kPBlock, 0);
+  hipError_t e;
+  switch (dom) {
+    case kItvDiv: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<kItvDiv>, kPBlock, 0); break;
+    case kBits: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<kBits>, kPBlock, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<kItv>, kPBlock, 0); break;
+  }
   return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                              int64_t n_props, InstFlags* flags, const int32_t* order,
                              int32_t n_inst, int32_t* queue, unsigned long long* tight,
-                             int32_t max_sweeps, bool div, int grid, hipStream_t stream) {
-  if (div)
-    k_persistent<true><<<grid, kPBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, order,
-                                                      n_inst, queue, tight, max_sweeps);
-  else
-    k_persistent<false><<<grid, kPBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, order,
-                                                       n_inst, queue, tight, max_sweeps);
+                             int32_t max_sweeps, int dom, int grid, hipStream_t stream) {
+#define PCX_PERSIST(D) \
+  k_persistent<D><<<grid, kPBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, order, \
+                                                n_inst, queue, tight, max_sweeps)
+  switch (dom) {
+    case kItv: PCX_PERSIST(kItv); break;
+    case kItvDiv: PCX_PERSIST(kItvDiv); break;
+    case kBits: PCX_PERSIST(kBits); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef PCX_PERSIST
   return hipGetLastError();
 }
 
@@ -406,23 +442,29 @@ hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipS
   return hipGetLastError();
 }
 
-// An instance whose store already holds an empty interval is failed before
+// An instance whose store already holds an empty domain is failed before
 // any sweep (the reference's is_bot() is true before the fixpoint).
+template <bool BITS>
 __global__ void k_check_empty(const int2* __restrict__ store, int64_t nvars, int64_t total,
                               InstFlags* flags) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < total;
        v += (int64_t)gridDim.x * blockDim.x) {
     const int2 b = store[v];
-    if (b.x > b.y) flags[v / nvars].failed_at = -1;
+    const bool empty = BITS ? (b.x == 0 && b.y == 0) : (b.x > b.y);
+    if (empty) flags[v / nvars].failed_at = -1;
   }
 }
 
 hipError_t launch_store_check_empty(const int2* store, int64_t nvars, int32_t n_inst,
-                                    InstFlags* flags, hipStream_t stream) {
+                                    InstFlags* flags, int dom, hipStream_t stream) {
   const int64_t total = nvars * (int64_t)n_inst;
   int64_t grid = (total + 255) / 256;
   if (grid > 16384) grid = 16384;
-  if (grid > 0) k_check_empty<<<dim3((unsigned)grid), 256, 0, stream>>>(store, nvars, total, flags);
+  if (grid <= 0) return hipSuccess;
+  if (dom == kBits)
+    k_check_empty<true><<<dim3((unsigned)grid), 256, 0, stream>>>(store, nvars, total, flags);
+  else
+    k_check_empty<false><<<dim3((unsigned)grid), 256, 0, stream>>>(store, nvars, total, flags);
   return hipGetLastError();
 }
 

@@ -421,6 +421,67 @@ __device__ __forceinline__ void deduce(int op, int2 X, int2 Y, int2 Z, int2& nx,
   nz = make_int2(zl, zu);
 }
 
+// ---- bitset domains -------------------------------------------------------
+// One 64-bit word per variable, bit v <=> value v in [0,63]; empty word =
+// failed.  Propagator semantics follow the reference's formula propagators
+// over NBitset (include/lala/formula.hpp, terms.hpp): see the opcode list.
+enum BitsOp : int32_t { kBLeq = 0, kBAdd = 1, kBNeq = 2, kBEqr = 3, kBitsOps = 4 };
+
+typedef unsigned long long u64;
+
+// {i + j : i in a, j in b} within [0,63]
+__device__ __forceinline__ u64 word_sum(u64 a, u64 b) {
+  u64 r = 0;
+#pragma unroll 8
+  for (int j = 0; j < 64; ++j) r |= ((b >> j) & 1ull) ? (a << j) : 0ull;
+  return r;
+}
+// {i - j : i in a, j in b} within [0,63]
+__device__ __forceinline__ u64 word_diff(u64 a, u64 b) {
+  u64 r = 0;
+#pragma unroll 8
+  for (int j = 0; j < 64; ++j) r |= ((b >> j) & 1ull) ? (a >> j) : 0ull;
+  return r;
+}
+__device__ __forceinline__ bool word_single(u64 a) { return a && !(a & (a - 1)); }
+// values <= v / values >= v
+__device__ __forceinline__ u64 word_upto(int v) { return v >= 63 ? ~0ull : (v < 0 ? 0ull : ((2ull << v) - 1ull)); }
+__device__ __forceinline__ u64 word_from(int v) { return v <= 0 ? ~0ull : (v > 63 ? 0ull : (~0ull << v)); }
+
+// In: words X, Y, Z.  Out: masks to AND into x, y, z (all-ones = untouched).
+__device__ __forceinline__ void deduce_bits(int op, u64 X, u64 Y, u64 Z, u64& nx, u64& ny, u64& nz) {
+  nx = ~0ull; ny = ~0ull; nz = ~0ull;
+  switch (op) {
+    case kBLeq: {                         // y <= z: y <= max z, then z >= min y
+      if (Z) ny = word_upto(63 - __builtin_clzll(Z));
+      const u64 y2 = Y & ny;
+      if (y2) nz = word_from(__builtin_ctzll(y2));
+      return;
+    }
+    case kBAdd: {                         // x = y + z: y, z from residuals, then x
+      const u64 y2 = Y & word_diff(X, Z);
+      const u64 z2 = Z & word_diff(X, y2);
+      ny = y2; nz = z2;
+      nx = X & word_sum(y2, z2);
+      return;
+    }
+    case kBNeq:                           // y != z: shave a fixed side off the other
+      if (word_single(Y)) nz = ~Y;
+      else if (word_single(Z)) ny = ~Z;
+      return;
+    case kBEqr: {                         // x <=> (y = z), x a Boolean literal
+      if (!(X & 1ull)) { nz = Y; ny = Y & Z; }               // x true
+      else if (X == 1ull) {                                  // x false
+        if (word_single(Y)) nz = ~Y;
+        else if (word_single(Z)) ny = ~Z;
+      } else if (Y == Z && word_single(Y)) nx = 2ull;        // entailed: x = 1
+      else if (!(Y & Z)) nx = 1ull;                          // disentailed: x = 0
+      return;
+    }
+    default: return;
+  }
+}
+
 // Entailment of one propagator on its operands' intervals (pir.hpp:417-438).
 __device__ __forceinline__ bool entailed(int op, int2 X, int2 Y, int2 Z) {
   const i64 xl = X.x, xu = X.y, yl = Y.x, yu = Y.y, zl = Z.x, zu = Z.y;

@@ -31,6 +31,10 @@ OPS = {"ADD": ADD, "MUL": MUL, "TDIV": TDIV, "FDIV": FDIV, "CDIV": CDIV, "EDIV":
 
 MODE_PLAIN, MODE_BATCHED, MODE_PERSISTENT = 0, 1, 2
 
+DOMAIN_INTERVAL, DOMAIN_BITSET = 0, 1
+BLEQ, BADD, BNEQ, BEQR = range(4)
+BITS_OPS = {"LEQ": BLEQ, "ADD": BADD, "NEQ": BNEQ, "EQR": BEQR}
+
 OK, EINVAL, ENOMEM, EDEVICE, EBUSY, ENOTSUP = range(6)
 
 
@@ -44,7 +48,7 @@ class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("props_per_wave", ctypes.c_int32),
                 ("lds_mode", ctypes.c_int32), ("sweeps_per_check", ctypes.c_int32),
                 ("max_sweeps", ctypes.c_int64), ("timing", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 7)]
+                ("domain", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
 
 
 class Result(ctypes.Structure):
@@ -74,6 +78,7 @@ class GenParams(ctypes.Structure):
 _P32 = ctypes.POINTER(ctypes.c_int32)
 _P64 = ctypes.POINTER(ctypes.c_int64)
 _PU8 = ctypes.POINTER(ctypes.c_uint8)
+_PU64 = ctypes.POINTER(ctypes.c_uint64)
 _VP = ctypes.c_void_p
 
 # (name, restype, argtypes) of every exported C symbol (include/pcx.h, include/pcx_gen.h)
@@ -97,12 +102,21 @@ SIGNATURES = [
     ("pcx_batch_reset", ctypes.c_int32, [_VP]),
     ("pcx_batch_read", ctypes.c_int32, [_VP, _P32, _P32]),
     ("pcx_batch_run", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Result), ctypes.POINTER(Stats)]),
+    ("pcx_bits_tell", ctypes.c_int32, [_VP, _P32, _P32, _P32, _P32, ctypes.c_int64, _PU64, ctypes.c_int64]),
+    ("pcx_bits_fixpoint", ctypes.c_int32, [_VP, _PU64, ctypes.c_int64, ctypes.POINTER(Result)]),
+    ("pcx_bits_batch_load", ctypes.c_int32, [_VP, _PU64]),
+    ("pcx_bits_batch_read", ctypes.c_int32, [_VP, _PU64]),
     ("pcxg_params_default", None, [ctypes.POINTER(GenParams)]),
     ("pcxg_network", ctypes.c_int32, [ctypes.POINTER(GenParams), _P32, _P32, _P32, _P32, _P32, _P32, _P32]),
     ("pcxg_splits", ctypes.c_int32, [ctypes.c_uint64, _P32, _P32, ctypes.c_int64, ctypes.c_int64,
                                      ctypes.c_int32, ctypes.c_int32, _P32, _P32]),
     ("pcxg_subsets", ctypes.c_int32, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                       ctypes.c_int64, _P64, _P32]),
+    ("pcxg_bits_network", ctypes.c_int32, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint32,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                           _P32, _P32, _P32, _P32, _PU64, _P32]),
+    ("pcxg_bits_splits", ctypes.c_int32, [ctypes.c_uint64, _PU64, ctypes.c_int64, ctypes.c_int64,
+                                          ctypes.c_int32, ctypes.c_int32, _PU64]),
 ]
 
 _lib = None
@@ -135,6 +149,13 @@ def _p64(a):
         return None
     assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"], "int64 C-contiguous array expected"
     return a.ctypes.data_as(_P64)
+
+
+def _pu64(a):
+    if a is None:
+        return None
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], "uint64 C-contiguous array expected"
+    return a.ctypes.data_as(_PU64)
 
 
 def _check(st):
@@ -218,6 +239,30 @@ class PIR:
         return self.ask_all(lb, ub)[1]
 
 
+class BitsPIR(PIR):
+    """Bitset-domain propagators (PC over 64-value bitset words).
+
+    Stores are numpy uint64 arrays, one word per variable (bit v = value v).
+    """
+
+    def __init__(self, config=None, **kw):
+        kw.setdefault("domain", DOMAIN_BITSET)
+        super().__init__(config, **kw)
+
+    def tell(self, op, x, y, z, words=None):
+        op, x, y, z = (np.ascontiguousarray(a, dtype=np.int32) for a in (op, x, y, z))
+        nv = 0 if words is None else len(words)
+        _check(lib().pcx_bits_tell(self._h, _p32(op), _p32(x), _p32(y), _p32(z), len(op),
+                                   _pu64(words), nv))
+
+    interpret_tell = tell
+
+    def fixpoint(self, words):
+        r = Result()
+        _check(lib().pcx_bits_fixpoint(self._h, _pu64(words), len(words), ctypes.byref(r)))
+        return r
+
+
 class Batch:
     """Many device-resident stores over one shared PIR table."""
 
@@ -250,6 +295,15 @@ class Batch:
 
     def reset(self):
         _check(lib().pcx_batch_reset(self._h))
+
+    def load_words(self, words):
+        assert words.size == self.n_inst * self.nvars
+        _check(lib().pcx_bits_batch_load(self._h, _pu64(words.reshape(-1))))
+
+    def read_words(self):
+        w = np.zeros(self.n_inst * self.nvars, np.uint64)
+        _check(lib().pcx_bits_batch_read(self._h, _pu64(w)))
+        return w
 
     def read(self):
         lb = np.zeros(self.n_inst * self.nvars, np.int32)
@@ -289,6 +343,23 @@ def gen_splits(seed, lb, ub, n_inst, kmin, kmax):
     hi = np.zeros(n_inst * V, np.int32)
     _check(lib().pcxg_splits(seed, _p32(lb), _p32(ub), V, n_inst, kmin, kmax, _p32(lo), _p32(hi)))
     return lo, hi
+
+
+def gen_bits_network(seed, nvars, nprops, kinds=0, dens_lo=0.3, dens_hi=1.0, planted=1,
+                     witness=False):
+    """Returns (op, x, y, z, words[, witness]) of a bitset network in told order."""
+    op, x, y, z = (np.zeros(nprops, np.int32) for _ in range(4))
+    w = np.zeros(nvars, np.uint64)
+    wit = np.zeros(nvars, np.int32)
+    _check(lib().pcxg_bits_network(seed, nvars, nprops, kinds, dens_lo, dens_hi, planted,
+                                   _p32(op), _p32(x), _p32(y), _p32(z), _pu64(w), _p32(wit)))
+    return (op, x, y, z, w, wit) if witness else (op, x, y, z, w)
+
+
+def gen_bits_splits(seed, words, n_inst, kmin, kmax):
+    out = np.zeros(n_inst * len(words), np.uint64)
+    _check(lib().pcxg_bits_splits(seed, _pu64(words), len(words), n_inst, kmin, kmax, _pu64(out)))
+    return out
 
 
 def gen_subsets(seed, nprops, n_inst, mmin, mmax):

@@ -51,6 +51,15 @@ def lib():
         L.pcxo_check_singletons.argtypes = [ctypes.c_int] * 3 + [ctypes.c_char_p, ctypes.c_int]
         L.pcxo_check_mul_const.argtypes = [ctypes.c_int] * 2 + [ctypes.c_char_p, ctypes.c_int]
         L.pcxo_check_monotone.argtypes = [ctypes.c_int] * 3 + [ctypes.c_char_p, ctypes.c_int]
+        PU64 = ctypes.POINTER(ctypes.c_uint64)
+        L.pcxo_bits_tell_embed.argtypes = [ctypes.c_int64, P32, P32, PU64]
+        L.pcxo_bits_sort.argtypes = [ctypes.c_int64, P32, P32, P32, P32]
+        L.pcxo_bits_fixpoint.argtypes = [ctypes.c_int64, P32, P32, P32, P32, PU64, ctypes.c_int64,
+                                         ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(Result)]
+        L.pcxo_bits_fixpoint_batch.argtypes = [ctypes.c_int64, P32, P32, P32, P32, PU64, ctypes.c_int64,
+                                               ctypes.c_int64, ctypes.c_int32, ctypes.c_int64,
+                                               ctypes.c_int32, ctypes.POINTER(Result)]
+        L.pcxo_bits_check.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -96,3 +105,47 @@ def fixpoint_batch(op, x, y, z, lb, ub, nvars, idx=None, off=None, threads=1, ma
 
 def ask(op, x, y, z, lb, ub):
     return bool(lib().pcxo_ask(int(op), int(x), int(y), int(z), p32(lb), p32(ub)))
+
+
+BITS_OPS = {"LEQ": 0, "ADD": 1, "NEQ": 2, "EQR": 3}
+
+
+def pu64(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def bits_tell_embed(op, x, words):
+    return lib().pcxo_bits_tell_embed(len(op), p32(op), p32(x), pu64(words))
+
+
+def bits_sort(op, x, y, z):
+    """Stable sort by bitset opcode (PC's formula-kind order)."""
+    assert lib().pcxo_bits_sort(len(op), p32(op), p32(x), p32(y), p32(z)) == 0
+
+
+def bits_fixpoint(op, x, y, z, words, faithful=False, max_sweeps=0):
+    r = Result()
+    lib().pcxo_bits_fixpoint(len(op), p32(op), p32(x), p32(y), p32(z), pu64(words), len(words),
+                             int(faithful), max_sweeps, ctypes.byref(r))
+    return r
+
+
+def bits_fixpoint_batch(op, x, y, z, words, nvars, faithful=False, threads=1, max_sweeps=0):
+    n_inst = len(words) // nvars
+    res = (Result * n_inst)()
+    lib().pcxo_bits_fixpoint_batch(len(op), p32(op), p32(x), p32(y), p32(z), pu64(words), nvars,
+                                   n_inst, int(faithful), max_sweeps, threads, res)
+    return list(res)
+
+
+def word(values):
+    w = 0
+    for v in values:
+        w |= 1 << v
+    return np.uint64(w)
+
+
+def values(w):
+    w = int(w)
+    return [v for v in range(64) if w >> v & 1]

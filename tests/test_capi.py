@@ -101,3 +101,23 @@ def test_subsets_sorted_distinct_sized():
     for k in range(40):
         s = idx[off[k]:off[k + 1]]
         assert (np.diff(s) > 0).all() and s.min() >= 0 and s.max() < 1000
+
+
+def test_planted_bits_network_holds_under_witness():
+    op, x, y, z, w, wit = pcx.gen_bits_network(4, 3000, 12000, witness=True)
+    assert all((int(w[v]) >> int(wit[v])) & 1 for v in range(3000))
+    wx, wy, wz = wit[x], wit[y], wit[z]
+    assert ((op != pcx.BADD) | (wx == wy + wz)).all()
+    assert ((op != pcx.BLEQ) | (wy <= wz)).all()
+    assert ((op != pcx.BNEQ) | (wy != wz)).all()
+    assert ((op != pcx.BEQR) | (wx == (wy == wz))).all()
+    assert ((x != y) & (y != z) & (x != z)).all()
+    counts = np.bincount(op, minlength=4)
+    assert (counts > 2500).all()
+
+
+def test_bits_splits_remove_values():
+    w = np.array([0b1111, 0b1, 0b110], np.uint64)
+    out = pcx.gen_bits_splits(2, w, 40, 1, 10).reshape(40, 3)
+    assert ((out & ~w) == 0).all() and (out != 0).all()
+    assert (out[:, 1] == 1).all()

@@ -1,0 +1,97 @@
+"""GPU parity for the bitset domain: every word equals the oracle's true
+fixpoint (oracle/bits_oracle.c, restating PC over NBitset), and the failure
+flag matches.  All calls go through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import pcx
+from test_oracle_bits import load_bits_cases, run_bits_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if pcx.device_count() < 1:
+        pytest.fail("no HIP device visible: GPU tests need an MI355X")
+    return True
+
+
+@pytest.mark.parametrize("case", load_bits_cases(), ids=lambda c: c["name"])
+def test_bits_kat_on_gpu(gpu, case):
+    def fx(op, x, y, z, w):
+        pir = pcx.BitsPIR()
+        pir.tell(op, x, y, z)
+        r = pir.fixpoint(w)
+        return r.failed, r.has_changed
+    errs = run_bits_case(case, fx)
+    assert not errs, f"{case['name']} ({case['ref']}): " + "; ".join(errs)
+
+
+def _compare(wg, rg, wo, ro, nvars, what):
+    n = len(ro)
+    wg, wo = wg.reshape(n, nvars), wo.reshape(n, nvars)
+    fg = np.array([r.failed for r in rg], bool)
+    fo = np.array([r.failed for r in ro], bool)
+    bad = np.nonzero(fg != fo)[0]
+    assert len(bad) == 0, f"{what}: failure flag differs on {len(bad)} instances, first {bad[:5]}"
+    diff = (wg != wo).any(axis=1) & ~fo
+    bad = np.nonzero(diff)[0]
+    assert len(bad) == 0, f"{what}: words differ on {len(bad)} instances, first {bad[0]}: {wg[bad[0]]} vs {wo[bad[0]]}"
+
+
+@pytest.mark.parametrize("name", list(ol.BITS_OPS))
+@pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PERSISTENT, pcx.MODE_PLAIN])
+def test_every_domain_one_propagator(gpu, name, mode):
+    U = 5 if mode != pcx.MODE_PLAIN else 3
+    doms = np.arange(1, 1 << U, dtype=np.uint64)
+    xs = np.array([1, 2, 3], np.uint64) if name == "EQR" else doms
+    X, Y, Z = np.meshgrid(xs, doms, doms, indexing="ij")
+    words = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.uint64).ravel()
+    n = len(words) // 3
+    op = np.array([ol.BITS_OPS[name]], np.int32)
+    x, y, z = (np.array([v], np.int32) for v in (0, 1, 2))
+    wo = words.copy()
+    ro = ol.bits_fixpoint_batch(op, x, y, z, wo, 3, threads=8)
+    pir = pcx.BitsPIR()
+    pir.tell(op, x, y, z)
+    b = pcx.Batch(pir, n, 3)
+    b.load_words(words)
+    rg, _ = b.run(mode)
+    _compare(b.read_words(), rg, wo, ro, 3, f"{name} mode {mode}")
+
+
+@pytest.mark.parametrize("seed,V,P", [(1, 5000, 20000), (2, 20000, 80000), (3, 200000, 800000)])
+def test_bits_network(gpu, seed, V, P):
+    op, x, y, z, w = pcx.gen_bits_network(seed, V, P)
+    pir = pcx.BitsPIR()
+    wg = w.copy()
+    pir.tell(op, x, y, z, wg)
+    rg = pir.fixpoint(wg)
+    top, tx, ty, tz = pir.table()
+    wo = w.copy()
+    ol.bits_tell_embed(op, x, wo)
+    ro = ol.bits_fixpoint(top, tx, ty, tz, wo)
+    assert bool(rg.failed) == bool(ro.failed)
+    if not ro.failed:
+        assert (wg == wo).all()
+
+
+@pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PERSISTENT, pcx.MODE_PLAIN])
+def test_bits_batch_of_splits(gpu, mode):
+    V, P, n = 4000, 16000, 64
+    op, x, y, z, w = pcx.gen_bits_network(9, V, P)
+    pir = pcx.BitsPIR()
+    pir.tell(op, x, y, z, w)
+    root = w.copy()
+    r = pir.fixpoint(root)
+    assert not r.failed
+    inst = pcx.gen_bits_splits(10, root, n, 1, 40)
+    top, tx, ty, tz = pir.table()
+    wo = inst.copy()
+    ro = ol.bits_fixpoint_batch(top, tx, ty, tz, wo, V, threads=8)
+    b = pcx.Batch(pir, n, V)
+    b.load_words(inst)
+    rg, _ = b.run(mode)
+    _compare(b.read_words(), rg, wo, ro, V, f"bits batch mode {mode}")

@@ -114,7 +114,7 @@ def mutant_dir():
 
 def _build_mutant(d, k):
     out = os.path.join(d, f"libmut{k}.so")
-    src = [os.path.join(ol.ORACLE_DIR, f) for f in ("pir_oracle.c", "pir_checks.c")]
+    src = [os.path.join(ol.ORACLE_DIR, f) for f in ("pir_oracle.c", "pir_checks.c", "bits_oracle.c")]
     subprocess.check_call(["gcc", "-O1", "-fPIC", "-shared", "-std=c11", f"-DPCXO_MUTANT={k}",
                            "-o", out] + src + ["-lm", "-pthread"])
     return out
