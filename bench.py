@@ -15,7 +15,11 @@ sample are checked bit-for-bit against it.
 
 --config 1 instead times the single 1,000,000-variable / 4,000,000-propagator
 fixpoint (plain engine, one step = one fixpoint); --config 3 the bitset store
-(1,024 bit-removal subproblems of a 200,000-word / 800,000-propagator network).
+(1,024 bit-removal subproblems of a 200,000-word / 800,000-propagator network);
+--config 4 the serving stream (Poisson arrivals at 2,000 requests/s over 24
+networks of 1k-1M propagators, 1-10 splits per request; one step = one
+real-time replay of the trace through the continuous-batching server, with
+p50/p99 latency, beside the same trace served one request at a time).
 
 Prints one JSON line (the driver's contract).
 """
@@ -282,12 +286,121 @@ def bench_bits(args):
     return out
 
 
+def bench_serving(args):
+    from pcx.serving import make_trace
+    t0 = time.time()
+    pirs = []
+
+    def embed(op, x, y, z, lb, ub):
+        p = pcx.PIR(props_per_wave=args.ppw)
+        p.tell(op, x, y, z, lb, ub)
+        pirs.append(p)
+
+    tr = make_trace(args.seed, args.requests, rate=args.rate, embed=embed)
+    srv = pcx.Server(max_slots=args.slots, sweeps_per_tick=args.sweeps_per_tick,
+                     props_per_wave=args.ppw)
+    for p in pirs:
+        srv.add_network(p)
+    n = len(tr)
+    sizes = np.array([tr.nprops(i) for i in range(n)])
+    log(f"setup {time.time() - t0:.1f}s: {n} requests over {tr.arrival_s[-1]:.2f}s, "
+        f"{len(pirs)} networks of {min(len(q[0]) for q in tr.nets)}-{max(len(q[0]) for q in tr.nets)} "
+        f"propagators, mean {sizes.mean():.0f} propagators/request")
+
+    def replay(plain, count=n):
+        lo, hi = tr.lb_all.copy(), tr.ub_all.copy()
+        out, wall = srv.replay(tr.arrival_s[:count], tr.net[:count], lo, hi, tr.store_off[:count + 1],
+                               plain=plain)
+        return lo, hi, out, wall
+
+    warm = min(n, 200)
+    for _ in range(args.warmup):
+        replay(False, warm)
+        replay(True, warm)
+    walls, lat, evals = [], [], 0
+    for _ in range(args.steps):
+        lo, hi, out, wall = replay(False)
+        walls.append(wall)
+        lat.extend(o.latency_ms for o in out)
+        evals += sum(o.result.evals for o in out)
+    st = srv.stats()
+    elapsed = sum(walls)
+    lat = np.array(lat)
+    failed = sum(o.result.failed for o in out)
+    log(f"continuous: {n * args.steps / elapsed:.0f} requests/s, p50 {np.percentile(lat, 50):.2f} ms, "
+        f"p99 {np.percentile(lat, 99):.2f} ms, max running {st.max_running}, failed {failed}/{n}")
+    out_d = {
+        "value": n * args.steps / elapsed, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
+        "config": {"workload": f"serving stream: Poisson arrivals at {args.rate:.0f} requests/s, "
+                               f"{n} requests over {len(pirs)} seeded networks of 1k-1M propagators "
+                               "(log-uniform, vars = propagators/4), 1-10 splits each",
+                   "requests": n, "rate": args.rate, "slots": args.slots,
+                   "sweeps_per_tick": args.sweeps_per_tick, "props_per_wave": args.ppw},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max())},
+        "evals_per_s": evals / elapsed,
+        "failed_requests": int(failed),
+        "server": st.as_dict(),
+    }
+    res_c = (lo, hi, out)
+    if not args.no_plain:
+        lo_p, hi_p, out_p, wall_p = replay(True)
+        lat_p = np.array([o.latency_ms for o in out_p])
+        out_d["plain"] = {"requests_per_s": n / wall_p, "p50_ms": float(np.percentile(lat_p, 50)),
+                          "p99_ms": float(np.percentile(lat_p, 99)), "max_ms": float(lat_p.max())}
+        mism = 0
+        for i in range(n):
+            s = slice(int(tr.store_off[i]), int(tr.store_off[i + 1]))
+            a, b = out[i].result, out_p[i].result
+            if bool(a.failed) != bool(b.failed) or (not a.failed and not (
+                    np.array_equal(lo[s], lo_p[s]) and np.array_equal(hi[s], hi_p[s]))):
+                mism += 1
+        out_d["plain"]["mismatches_vs_continuous"] = mism
+        log(f"plain: {n / wall_p:.0f} requests/s, p50 {np.percentile(lat_p, 50):.2f} ms, "
+            f"p99 {np.percentile(lat_p, 99):.2f} ms, mismatches {mism}")
+    if not args.no_cpu:
+        ol = cpu_oracle()
+        threads = host_threads()
+        tables = [q.table() for q in pirs]
+        lo, hi, out = res_c
+        k = 0
+        t_cpu = 0.0
+        mism = 0
+        while k < n and t_cpu < args.cpu_seconds:
+            # the next chunk of requests, grouped by network, each group's
+            # requests spread over the host threads
+            j = min(n, k + threads)
+            for net in np.unique(tr.net[k:j]):
+                group = [i for i in range(k, j) if tr.net[i] == net]
+                top, tx, ty, tz = tables[net]
+                V = int(tr.store_off[group[0] + 1] - tr.store_off[group[0]])
+                clo = np.concatenate([tr.store(i)[0] for i in group])
+                chi = np.concatenate([tr.store(i)[1] for i in group])
+                ts = time.perf_counter()
+                r = ol.fixpoint_batch(top, tx, ty, tz, clo, chi, V, threads=threads)
+                t_cpu += time.perf_counter() - ts
+                for q, i in enumerate(group):
+                    g = out[i].result
+                    si = slice(int(tr.store_off[i]), int(tr.store_off[i + 1]))
+                    if bool(g.failed) != bool(r[q].failed) or (not g.failed and not (
+                            np.array_equal(lo[si], clo[q * V:(q + 1) * V]) and
+                            np.array_equal(hi[si], chi[q * V:(q + 1) * V]))):
+                        mism += 1
+            k = j
+        out_d["cpu_baseline"] = {"value": k / t_cpu, "unit": "fixpoints/s", "cores": threads, "kind": "port",
+                                 "sample": f"first {k} of the {n} requests, the requests of each network "
+                                           "spread over the threads, oracle Gauss-Seidel (-O3)"}
+        out_d["parity"] = {"checked_requests": k, "mismatches": mism}
+        log(f"cpu: {k} requests in {t_cpu:.1f}s on {threads} threads; parity mismatches {mism}")
+    return out_d
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3])
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4])
     ap.add_argument("--mode", default="persistent", choices=["batched", "persistent"],
                     help="engine for the timed batch (the plain engine is always run once beside it)")
     ap.add_argument("--ppw", type=int, default=0, choices=[0, 64, 128, 256, 512],
@@ -296,15 +409,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--requests", type=int, default=4000, help="serving: requests in the trace")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serving: Poisson arrival rate (1/s)")
+    ap.add_argument("--slots", type=int, default=256, help="serving: requests on the GPU at once")
+    ap.add_argument("--sweeps-per-tick", type=int, default=4, help="serving: sweeps between host checks")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; --gpus must be 1 (instances would shard "
-              "across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: this build measures one GPU; --gpus must be 1 (batched instances would "
+              "shard across GPUs with no exchange; a multi-GPU run is not part of this build)",
+              file=sys.stderr)
         sys.exit(2)
     if pcx.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    out = {1: bench_single, 2: bench_batched, 3: bench_bits}[args.config](args)
+    out = {1: bench_single, 2: bench_batched, 3: bench_bits, 4: bench_serving}[args.config](args)
     line = {"metric": METRIC, "value": out.pop("value"), "unit": out.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out.pop("ms_per_step"),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",

@@ -189,6 +189,76 @@ pcx_status pcx_bits_fixpoint(pcx_engine* e, uint64_t* words, int64_t nvars,
 pcx_status pcx_bits_batch_load(pcx_batch* b, const uint64_t* words);
 pcx_status pcx_bits_batch_read(pcx_batch* b, uint64_t* words);
 
+/* ---- serving: continuous batching of fixpoint requests ------------------
+ * The reference has no server; each request here is what one reference
+ * caller does per store: GaussSeidelIteration::fixpoint over PIR::deduce
+ * (tests/pir_test.cpp:60-62) on a copy of a shared network (pir.hpp:182-205).
+ * A server owns one device arena for request stores and a fixed number of
+ * slots.  Each step admits queued requests while a slot is free and the
+ * arena holds their store (admission by HBM footprint, first fit), runs
+ * sweeps_per_tick sweeps over every running request at once (requests of
+ * different networks and sizes in one launch), and completes the requests
+ * that reached their fixpoint or failed.  A request's result is the one
+ * pcx_fixpoint gives for the same network and store, whatever else runs.
+ * Interval-domain engines only. */
+typedef struct pcx_server pcx_server;
+
+typedef struct {
+  int32_t device;
+  int32_t max_slots;         /* requests on the GPU at once                  */
+  int64_t arena_bytes;       /* device bytes for request stores (8 B / var)   */
+  int32_t sweeps_per_tick;   /* sweeps launched between host checks           */
+  int32_t props_per_wave;    /* 64..512; 0 = 512                              */
+  int64_t stage_bytes;       /* pinned host staging for store copies          */
+  int32_t reserved[4];
+} pcx_server_config;
+
+typedef struct {
+  int64_t    tag;            /* the caller's tag (replay: request index)      */
+  pcx_result result;
+  double     latency_ms;     /* submit (replay: arrival) to completion        */
+} pcx_served;
+
+typedef struct {
+  int64_t ticks;             /* server steps that launched sweeps or idled    */
+  int64_t sweeps;            /* slot-batch sweeps launched                    */
+  int64_t admitted;          /* requests moved onto the GPU                   */
+  int64_t completed;         /* requests completed (including empty stores)   */
+  int64_t waits_slot;        /* admissions deferred: every slot busy          */
+  int64_t waits_arena;       /* admissions deferred: store did not fit (HBM)  */
+  int64_t arena_peak_bytes;  /* highest arena byte in use                     */
+  int32_t max_running;       /* most requests on the GPU in one step          */
+  int32_t reserved;
+} pcx_server_stats;
+
+void       pcx_server_config_default(pcx_server_config* cfg);
+pcx_status pcx_server_create(const pcx_server_config* cfg, pcx_server** out);
+void       pcx_server_destroy(pcx_server* s);
+/* Register an engine's table as network *net.  The engine must outlive the
+ * server and tell nothing more while registered. */
+pcx_status pcx_server_add_network(pcx_server* s, pcx_engine* e, int32_t* net);
+/* Queue a request.  lb/ub (nvars entries, host memory) stay the caller's
+ * until the request completes; they then hold its fixpoint. */
+pcx_status pcx_server_submit(pcx_server* s, int32_t net, int32_t* lb, int32_t* ub,
+                             int64_t nvars, int64_t tag);
+/* One tick: admit, sweep, complete.  *in_flight (may be NULL) receives the
+ * requests still queued or running. */
+pcx_status pcx_server_step(pcx_server* s, int32_t* in_flight);
+/* Counters since creation. */
+pcx_status pcx_server_get_stats(const pcx_server* s, pcx_server_stats* out);
+/* Move up to max completed requests to out; returns how many. */
+int64_t    pcx_server_poll(pcx_server* s, pcx_served* out, int64_t max);
+/* Replay a request trace in real time on an idle server: request i arrives
+ * arrival_s[i] seconds after the start (ascending), runs network net[i] on
+ * store slice [store_off[i], store_off[i+1]) of lb_all/ub_all (overwritten
+ * with the fixpoints), and out[i] receives its result and latency.
+ * plain = 1 serves the same trace one request at a time, each with its own
+ * sweep launches (FIFO), as the comparison point.  *wall_s = whole replay. */
+pcx_status pcx_server_replay(pcx_server* s, int64_t n, const double* arrival_s,
+                             const int32_t* net, int32_t* lb_all, int32_t* ub_all,
+                             const int64_t* store_off, int32_t plain, pcx_served* out,
+                             double* wall_s);
+
 #ifdef __cplusplus
 }
 #endif

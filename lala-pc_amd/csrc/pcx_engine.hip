@@ -13,12 +13,11 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pcx.h"
-#include "pcx_kernels.h"
+#include "pcx_internal.h"
 
 using namespace pcx;
 
-namespace {
+namespace pcx_internal {
 
 thread_local std::string g_err;
 
@@ -27,13 +26,19 @@ pcx_status fail_with(pcx_status st, const std::string& msg) {
   return st;
 }
 
-#define PCX_HIP(call)                                                                  \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail_with(e_ == hipErrorOutOfMemory ? PCX_ENOMEM : PCX_EDEVICE,           \
-                       std::string(#call) + ": " + hipGetErrorString(e_));             \
-  } while (0)
+int device_cus(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    return prop.multiProcessorCount;
+  return 256;
+}
+
+}  // namespace pcx_internal
+
+using pcx_internal::fail_with;
+using pcx_internal::g_err;
+
+namespace {
 
 double now_ms() {
   return std::chrono::duration<double, std::milli>(
@@ -50,41 +55,7 @@ void dfree(T*& p) {
 
 }  // namespace
 
-struct pcx_engine {
-  pcx_config cfg{};
-  std::vector<int32_t> op, x, y, z;   // table in engine order
-  int32_t max_var = -1;
-  bool has_div = false;               // table holds a division propagator
-  bool dirty = true;
-  uint32_t* d_xo = nullptr;
-  int32_t* d_y = nullptr;
-  int32_t* d_z = nullptr;
-  int64_t d_cap = 0;
-  hipStream_t stream = nullptr;
-  pcx_batch* scratch = nullptr;       // batch of one for pcx_fixpoint / pcx_ask_all
-  int64_t scratch_nvars = -1;
-  int64_t scratch_table = -1;
-};
-
-struct pcx_batch {
-  pcx_engine* e = nullptr;
-  int64_t n_inst = 0, nvars = 0;
-  int64_t table_n = 0;                // table size the subsets were built for
-  int2* d_store = nullptr;
-  int2* d_image = nullptr;
-  int64_t* d_off = nullptr;
-  int32_t* d_idx = nullptr;
-  std::vector<int64_t> h_off;         // empty = whole table
-  InstFlags* d_flags = nullptr;
-  unsigned long long* d_tight = nullptr;
-  int32_t* d_act = nullptr;
-  int64_t* d_item_base = nullptr;
-  int32_t* d_ctl = nullptr;           // [0]=n_act [1]=sweep [2]=live [3]=queue head
-  int32_t* d_order = nullptr;         // instances by decreasing size (persistent queue)
-  std::vector<hipEvent_t> ev;         // timing pool
-};
-
-namespace {
+namespace pcx_internal {
 
 pcx_status sync_table(pcx_engine* e) {
   if (!e->dirty) return PCX_OK;
@@ -113,6 +84,13 @@ int dom_of(const pcx_engine* e) {
   if (e->cfg.domain == PCX_DOMAIN_BITSET) return kBits;
   return e->has_div ? kItvDiv : kItv;
 }
+
+}  // namespace pcx_internal
+
+using pcx_internal::dom_of;
+using pcx_internal::sync_table;
+
+namespace {
 
 Table table_of(const pcx_engine* e) {
   return Table{e->d_xo, e->d_y, e->d_z, (int64_t)e->op.size()};

@@ -1,0 +1,58 @@
+// pcx_internal.h -- engine and batch objects shared by the host sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pcx.h"
+#include "pcx_kernels.h"
+
+struct pcx_engine {
+  pcx_config cfg{};
+  std::vector<int32_t> op, x, y, z;   // table in engine order
+  int32_t max_var = -1;
+  bool has_div = false;               // table holds a division propagator
+  bool dirty = true;
+  uint32_t* d_xo = nullptr;
+  int32_t* d_y = nullptr;
+  int32_t* d_z = nullptr;
+  int64_t d_cap = 0;
+  hipStream_t stream = nullptr;
+  pcx_batch* scratch = nullptr;       // batch of one for pcx_fixpoint / pcx_ask_all
+  int64_t scratch_nvars = -1;
+  int64_t scratch_table = -1;
+};
+
+struct pcx_batch {
+  pcx_engine* e = nullptr;
+  int64_t n_inst = 0, nvars = 0;
+  int64_t table_n = 0;                // table size the subsets were built for
+  int2* d_store = nullptr;
+  int2* d_image = nullptr;
+  int64_t* d_off = nullptr;
+  int32_t* d_idx = nullptr;
+  std::vector<int64_t> h_off;         // empty = whole table
+  pcx::InstFlags* d_flags = nullptr;
+  unsigned long long* d_tight = nullptr;
+  int32_t* d_act = nullptr;
+  int64_t* d_item_base = nullptr;
+  int32_t* d_ctl = nullptr;           // [0]=n_act [1]=sweep [2]=live [3]=queue head
+  int32_t* d_order = nullptr;         // instances by decreasing size (persistent queue)
+  std::vector<hipEvent_t> ev;         // timing pool
+};
+
+namespace pcx_internal {
+pcx_status fail_with(pcx_status st, const std::string& msg);
+pcx_status sync_table(pcx_engine* e);   // upload the table if it changed
+int dom_of(const pcx_engine* e);        // kernel domain for this engine
+int device_cus(int device);             // compute units of a device
+}  // namespace pcx_internal
+
+#define PCX_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return pcx_internal::fail_with(e_ == hipErrorOutOfMemory ? PCX_ENOMEM : PCX_EDEVICE, \
+                                     std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)

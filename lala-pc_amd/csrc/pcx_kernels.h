@@ -61,6 +61,15 @@ struct WorkList {
                          //     live[1] is the persistent mode's queue head
 };
 
+// One serving slot: a request's own table and store (heterogeneous batch).
+struct SlotDesc {
+  const uint32_t* xo;
+  const int32_t* y;
+  const int32_t* z;
+  int64_t m;             // propagators
+  int2* store;           // the request's store (nvars entries)
+};
+
 }  // namespace pcx
 
 // Host-callable launchers (defined in pcx_kernels.hip).
@@ -80,6 +89,11 @@ hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const S
                              int32_t max_sweeps, int dom, int grid, hipStream_t st);
 int persistent_blocks_per_cu(int dom);   // resident workgroups per CU
 hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipStream_t st);
+hipError_t launch_schedule_slots(const SlotDesc* slots, InstFlags* flags, int32_t n_slots,
+                                 WorkList wl, int ppw, hipStream_t st);
+hipError_t launch_sweep_slots(const SlotDesc* slots, InstFlags* flags, WorkList wl,
+                              unsigned long long* tight, int ppw, int dom, int grid,
+                              hipStream_t st);
 hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
                       hipStream_t st);
 hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n, hipStream_t st);

@@ -312,6 +312,133 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
   }
 }
 
+// ---- serving slots: heterogeneous batch, one table and store per slot ----
+// Same scheme as the batched mode: a one-workgroup scheduler builds the item
+// prefix sum over the slots still running, then a grid-stride XCD-aware
+// sweep.  Free slots carry failed_at != kNoFail and done_at = 0, so they are
+// never scheduled.  The sweep counter keeps running across ticks; a slot
+// admitted before sweep S is primed with changed[(S-1)&1] = S-1.
+__global__ __launch_bounds__(kSchedThreads) void k_schedule_slots(const SlotDesc* slots,
+                                                                  InstFlags* flags,
+                                                                  int32_t n_slots, WorkList wl,
+                                                                  int64_t chunk) {
+  __shared__ int64_t part[kSchedThreads];
+  __shared__ int32_t cnt[kSchedThreads];
+  const int32_t s = *wl.sweep + 1;
+  const int tid = threadIdx.x;
+  const int per = (n_slots + kSchedThreads - 1) / kSchedThreads;
+  const int lo = min(n_slots, tid * per), hi = min(n_slots, lo + per);
+  int64_t items = 0;
+  int32_t na = 0;
+  for (int k = lo; k < hi; ++k) {
+    InstFlags& f = flags[k];
+    const bool run = f.done_at < 0 && f.failed_at == kNoFail && f.changed[(s - 1) & 1] == s - 1;
+    if (!run) {
+      if (f.done_at < 0) f.done_at = s;
+      continue;
+    }
+    items += (slots[k].m + chunk - 1) / chunk;
+    ++na;
+  }
+  part[tid] = items;
+  cnt[tid] = na;
+  __syncthreads();
+  for (int o = 1; o < kSchedThreads; o <<= 1) {
+    const int64_t pv = tid >= o ? part[tid - o] : 0;
+    const int32_t cv = tid >= o ? cnt[tid - o] : 0;
+    __syncthreads();
+    part[tid] += pv;
+    cnt[tid] += cv;
+    __syncthreads();
+  }
+  int64_t ib = part[tid] - items;
+  int32_t ia = cnt[tid] - na;
+  for (int k = lo; k < hi; ++k) {
+    const InstFlags& f = flags[k];
+    if (f.done_at >= 0) continue;
+    wl.act[ia] = k;
+    wl.item_base[ia] = ib;
+    ib += (slots[k].m + chunk - 1) / chunk;
+    ++ia;
+  }
+  __syncthreads();
+  if (tid == kSchedThreads - 1) {
+    wl.item_base[cnt[tid]] = part[tid];
+    *wl.n_act = cnt[tid];
+    *wl.live = cnt[tid];
+    *wl.sweep = s;
+  }
+}
+
+hipError_t launch_schedule_slots(const SlotDesc* slots, InstFlags* flags, int32_t n_slots,
+                                 WorkList wl, int ppw, hipStream_t stream) {
+  const int64_t chunk = (int64_t)kBlock * (ppw / kWave);
+  k_schedule_slots<<<1, kSchedThreads, 0, stream>>>(slots, flags, n_slots, wl, chunk);
+  return hipGetLastError();
+}
+
+template <int PPL, int D>
+__global__ __launch_bounds__(kBlock) void k_sweep_slots(const SlotDesc* slots, InstFlags* flags,
+                                                        WorkList wl, unsigned long long* tight) {
+  const int32_t s = *wl.sweep;
+  const int32_t n_act = *wl.n_act;
+  if (n_act <= 0) return;
+  const int64_t total = wl.item_base[n_act];
+  constexpr int64_t chunk = (int64_t)kBlock * PPL;
+  const int xcd = blockIdx.x % kXcds;
+  const int64_t lanes = gridDim.x / kXcds;
+  const int64_t first_item = total * xcd / kXcds;
+  const int64_t end_item = total * (xcd + 1) / kXcds;
+  for (int64_t item = first_item + blockIdx.x / kXcds; item < end_item; item += lanes) {
+    int lo = 0, hi = n_act - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (wl.item_base[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int32_t k = wl.act[lo];
+    const SlotDesc sd = slots[k];
+    InstFlags* f = flags + k;
+    const Table t{sd.xo, sd.y, sd.z, sd.m};
+    const int64_t first = (item - wl.item_base[lo]) * chunk;
+    bool chg = false, fail = false;
+    int nt = 0;
+    if (f->failed_at == kNoFail) {
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        const int64_t q = first + j * kBlock + threadIdx.x;
+        if (q < sd.m) eval_prop<D>(t, sd.store, q, chg, fail, nt);
+      }
+    }
+    publish(f, s, chg, fail, nt, tight);
+  }
+}
+
+hipError_t launch_sweep_slots(const SlotDesc* slots, InstFlags* flags, WorkList wl,
+                              unsigned long long* tight, int ppw, int dom, int grid,
+                              hipStream_t stream) {
+  const int ppl = ppw / kWave;
+#define PCX_SLOTS(P, D) k_sweep_slots<P, D><<<grid, kBlock, 0, stream>>>(slots, flags, wl, tight)
+  if (dom == kItvDiv) {
+    switch (ppl) {
+      case 1: PCX_SLOTS(1, kItvDiv); break;
+      case 2: PCX_SLOTS(2, kItvDiv); break;
+      case 4: PCX_SLOTS(4, kItvDiv); break;
+      case 8: PCX_SLOTS(8, kItvDiv); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (ppl) {
+      case 1: PCX_SLOTS(1, kItv); break;
+      case 2: PCX_SLOTS(2, kItv); break;
+      case 4: PCX_SLOTS(4, kItv); break;
+      case 8: PCX_SLOTS(8, kItv); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
+#undef PCX_SLOTS
+  return hipGetLastError();
+}
+
 // ---- persistent mode: one launch, a device queue of instances -------------
 // Each workgroup takes the next instance from the queue (largest first) and
 // runs all of its sweeps itself, separated by workgroup barriers.  Store

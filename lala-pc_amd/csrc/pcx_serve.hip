@@ -1,0 +1,484 @@
+// pcx_serve.hip -- continuous-batching server for propagation requests.
+//
+// Requests (a store over one of the registered networks) wait in a host
+// queue.  Each tick admits queued requests into free device slots while the
+// store arena has room (admission by HBM footprint), launches a few sweeps
+// of the heterogeneous slot batch -- every live request advances by one sweep
+// per launch, whatever its network -- and hands back the requests that
+// reached their fixpoint, which frees their slot and arena block for the
+// next tick.  Requests that arrive while others are on the GPU thus join the
+// running batch instead of waiting for it to drain.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+#include <vector>
+
+#include "pcx_internal.h"
+
+using namespace pcx;
+using pcx_internal::fail_with;
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over the store arena (units: 8-byte store entries).
+class Arena {
+ public:
+  void init(int64_t words) { free_.clear(); if (words > 0) free_[0] = words; }
+  int64_t alloc(int64_t n) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        const int64_t off = it->first, rest = it->second - n;
+        free_.erase(it);
+        if (rest > 0) free_[off + n] = rest;
+        return off;
+      }
+    }
+    return -1;
+  }
+  void release(int64_t off, int64_t n) {
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+ private:
+  std::map<int64_t, int64_t> free_;
+};
+
+struct Request {
+  int32_t net;
+  int32_t* lb;
+  int32_t* ub;
+  int64_t nvars;
+  int64_t tag;
+  double t0;            // submit or arrival time (latency origin)
+};
+
+struct Slot {
+  bool busy = false;
+  Request req{};
+  int64_t off = 0;      // arena offset
+  int32_t start = 0;    // first sweep number
+};
+
+constexpr int32_t kFreeFailed = -2;   // failed_at marker of a free slot
+
+}  // namespace
+
+struct pcx_server {
+  pcx_server_config cfg{};
+  hipStream_t stream = nullptr;
+  std::vector<pcx_engine*> nets;
+  int32_t n_slots = 0;
+  SlotDesc* d_slots = nullptr;
+  InstFlags* d_flags = nullptr;
+  InstFlags* d_solo = nullptr;        // flags of a request run alone
+  int32_t* d_act = nullptr;
+  int64_t* d_item_base = nullptr;
+  int32_t* d_ctl = nullptr;
+  unsigned long long* d_tight = nullptr;
+  int2* d_arena = nullptr;
+  Arena arena;
+  std::vector<Slot> slots;
+  SlotDesc* h_slots = nullptr;        // pinned mirrors
+  InstFlags* h_flags = nullptr;
+  int2* h_stage = nullptr;            // pinned staging ring for store uploads
+  int64_t stage_words = 0;
+  std::deque<Request> pending;
+  std::vector<pcx_served> done;
+  int32_t next_sweep = 0;
+  pcx_server_stats stats{};
+};
+
+namespace {
+
+WorkList worklist(pcx_server* s) {
+  return WorkList{s->d_act, s->d_item_base, s->d_ctl, s->d_ctl + 1, s->d_ctl + 2};
+}
+
+pcx_status upload_slot_meta(pcx_server* s, int32_t k) {
+  PCX_HIP(hipMemcpyAsync(s->d_slots + k, s->h_slots + k, sizeof(SlotDesc), hipMemcpyHostToDevice, s->stream));
+  PCX_HIP(hipMemcpyAsync(s->d_flags + k, s->h_flags + k, sizeof(InstFlags), hipMemcpyHostToDevice, s->stream));
+  return PCX_OK;
+}
+
+void finish(pcx_server* s, const Request& rq, const pcx_result& r) {
+  pcx_served out{};
+  out.tag = rq.tag;
+  out.result = r;
+  out.latency_ms = (now_s() - rq.t0) * 1e3;
+  s->done.push_back(out);
+  ++s->stats.completed;
+}
+
+// Move queued requests into free slots while the arena and staging ring
+// have room.  Stores with an empty domain finish at once (failed, 0 sweeps).
+pcx_status admit(pcx_server* s) {
+  int64_t staged = 0;
+  while (!s->pending.empty()) {
+    Request& rq = s->pending.front();
+    bool empty = false;
+    for (int64_t v = 0; v < rq.nvars && !empty; ++v) empty = rq.lb[v] > rq.ub[v];
+    if (empty) {
+      pcx_result r{};
+      r.failed = 1;
+      finish(s, rq, r);
+      s->pending.pop_front();
+      continue;
+    }
+    int32_t k = -1;
+    for (int32_t i = 0; i < s->n_slots; ++i)
+      if (!s->slots[i].busy) { k = i; break; }
+    if (k < 0) { ++s->stats.waits_slot; break; }
+    if (staged + rq.nvars > s->stage_words) break;
+    const int64_t off = s->arena.alloc(rq.nvars);
+    if (off < 0) { ++s->stats.waits_arena; break; }   // no HBM room: wait for a slot to drain
+    int2* stage = s->h_stage + staged;
+    for (int64_t v = 0; v < rq.nvars; ++v) stage[v] = make_int2(rq.lb[v], rq.ub[v]);
+    PCX_HIP(hipMemcpyAsync(s->d_arena + off, stage, rq.nvars * sizeof(int2), hipMemcpyHostToDevice, s->stream));
+    staged += rq.nvars;
+    pcx_engine* e = s->nets[rq.net];
+    const int32_t S = s->next_sweep;
+    s->h_slots[k] = SlotDesc{e->d_xo, e->d_y, e->d_z, (int64_t)e->op.size(), s->d_arena + off};
+    InstFlags f{};
+    f.changed[(S - 1) & 1] = S - 1;            // primes sweep S
+    f.changed[S & 1] = S - 2;
+    f.failed_at = kNoFail;
+    f.done_at = -1;
+    s->h_flags[k] = f;
+    pcx_status st = upload_slot_meta(s, k);
+    if (st != PCX_OK) return st;
+    Slot& sl = s->slots[k];
+    sl.busy = true;
+    sl.req = rq;
+    sl.off = off;
+    sl.start = S;
+    s->pending.pop_front();
+    ++s->stats.admitted;
+    s->stats.arena_peak_bytes = std::max(s->stats.arena_peak_bytes,
+                                         (int64_t)((off + rq.nvars) * sizeof(int2)));
+  }
+  return PCX_OK;
+}
+
+// Hand back the slots whose request reached its fixpoint or failed.
+pcx_status harvest(pcx_server* s) {
+  for (int32_t k = 0; k < s->n_slots; ++k) {
+    Slot& sl = s->slots[k];
+    if (!sl.busy) continue;
+    const InstFlags& f = s->h_flags[k];
+    const int32_t last = std::max(f.changed[0], f.changed[1]);
+    const bool failed = f.failed_at != kNoFail;
+    const bool quiet = last < s->next_sweep - 1;     // the latest sweep left it unchanged
+    if (!failed && !quiet) continue;
+    pcx_result r{};
+    r.failed = failed;
+    r.has_changed = last >= sl.start;
+    r.sweeps = failed ? (int64_t)f.failed_at - sl.start + 1
+                      : (int64_t)std::max(last, sl.start - 1) - sl.start + 2;
+    r.evals = r.sweeps * s->h_slots[k].m;
+    int2* stage = s->h_stage;
+    PCX_HIP(hipMemcpyAsync(stage, s->d_arena + sl.off, sl.req.nvars * sizeof(int2), hipMemcpyDeviceToHost, s->stream));
+    PCX_HIP(hipStreamSynchronize(s->stream));
+    for (int64_t v = 0; v < sl.req.nvars; ++v) { sl.req.lb[v] = stage[v].x; sl.req.ub[v] = stage[v].y; }
+    finish(s, sl.req, r);
+    s->arena.release(sl.off, sl.req.nvars);
+    sl.busy = false;
+    // park the slot: never scheduled until re-admitted
+    s->h_flags[k].failed_at = kFreeFailed;
+    s->h_flags[k].done_at = 0;
+    pcx_status st = upload_slot_meta(s, k);
+    if (st != PCX_OK) return st;
+  }
+  return PCX_OK;
+}
+
+int32_t busy_slots(const pcx_server* s) {
+  int32_t n = 0;
+  for (const Slot& sl : s->slots) n += sl.busy;
+  return n;
+}
+
+pcx_status tick(pcx_server* s) {
+  pcx_status st = admit(s);
+  if (st != PCX_OK) return st;
+  const int32_t running = busy_slots(s);
+  s->stats.max_running = std::max(s->stats.max_running, running);
+  if (running > 0) {
+    const int ppw = s->cfg.props_per_wave ? s->cfg.props_per_wave : 512;
+    const int grid = pcx_internal::device_cus(s->cfg.device) * 8;
+    bool div = false;
+    for (const Slot& sl : s->slots)
+      if (sl.busy) div |= s->nets[sl.req.net]->has_div;
+    const WorkList wl = worklist(s);
+    for (int g = 0; g < s->cfg.sweeps_per_tick; ++g) {
+      PCX_HIP(launch_schedule_slots(s->d_slots, s->d_flags, s->n_slots, wl, ppw, s->stream));
+      PCX_HIP(launch_sweep_slots(s->d_slots, s->d_flags, wl, s->d_tight, ppw, div ? kItvDiv : kItv,
+                                 grid, s->stream));
+      ++s->next_sweep;
+      ++s->stats.sweeps;
+    }
+    PCX_HIP(hipMemcpyAsync(s->h_flags, s->d_flags, s->n_slots * sizeof(InstFlags), hipMemcpyDeviceToHost, s->stream));
+    PCX_HIP(hipStreamSynchronize(s->stream));
+    st = harvest(s);
+    if (st != PCX_OK) return st;
+  } else {
+    PCX_HIP(hipStreamSynchronize(s->stream));
+  }
+  ++s->stats.ticks;
+  return PCX_OK;
+}
+
+// One request on its own: one sweep launch per sweep, host check every 8.
+pcx_status run_alone(pcx_server* s, const Request& rq, pcx_result* out) {
+  pcx_engine* e = s->nets[rq.net];
+  for (int64_t v = 0; v < rq.nvars; ++v)
+    if (rq.lb[v] > rq.ub[v]) { *out = pcx_result{}; out->failed = 1; return PCX_OK; }
+  for (int64_t v = 0; v < rq.nvars; ++v) s->h_stage[v] = make_int2(rq.lb[v], rq.ub[v]);
+  PCX_HIP(hipMemcpyAsync(s->d_arena, s->h_stage, rq.nvars * sizeof(int2), hipMemcpyHostToDevice, s->stream));
+  InstFlags f{};
+  f.changed[0] = -2; f.changed[1] = -1; f.failed_at = kNoFail; f.done_at = -1;
+  InstFlags* h = s->h_flags + s->n_slots;     // spare pinned entry
+  *h = f;
+  PCX_HIP(hipMemcpyAsync(s->d_solo, h, sizeof(InstFlags), hipMemcpyHostToDevice, s->stream));
+  const Table t{e->d_xo, e->d_y, e->d_z, (int64_t)e->op.size()};
+  const int64_t m = (int64_t)e->op.size();
+  int ppw = 64;
+  for (int p = 512; p > 64; p /= 2)
+    if (m / ((int64_t)kBlock * (p / kWave)) >= 2LL * pcx_internal::device_cus(s->cfg.device)) { ppw = p; break; }
+  int32_t sw = 0;
+  for (;;) {
+    for (int g = 0; g < 8; ++g, ++sw)
+      PCX_HIP(launch_sweep_plain(t, s->d_arena, nullptr, m, sw, s->d_solo, s->d_tight, ppw,
+                                 pcx_internal::dom_of(e), s->stream));
+    PCX_HIP(hipMemcpyAsync(h, s->d_solo, sizeof(InstFlags), hipMemcpyDeviceToHost, s->stream));
+    PCX_HIP(hipStreamSynchronize(s->stream));
+    const int32_t last = std::max(h->changed[0], h->changed[1]);
+    if (h->failed_at != kNoFail || last + 1 < sw) break;
+  }
+  const int32_t last = std::max(h->changed[0], h->changed[1]);
+  pcx_result r{};
+  r.failed = h->failed_at != kNoFail;
+  r.has_changed = last >= 0;
+  r.sweeps = r.failed ? (int64_t)h->failed_at + 1 : (int64_t)last + 2;
+  r.evals = r.sweeps * m;
+  PCX_HIP(hipMemcpyAsync(s->h_stage, s->d_arena, rq.nvars * sizeof(int2), hipMemcpyDeviceToHost, s->stream));
+  PCX_HIP(hipStreamSynchronize(s->stream));
+  for (int64_t v = 0; v < rq.nvars; ++v) { rq.lb[v] = s->h_stage[v].x; rq.ub[v] = s->h_stage[v].y; }
+  *out = r;
+  return PCX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pcx_server_config_default(pcx_server_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->device = 0;
+  c->max_slots = 256;
+  c->arena_bytes = (int64_t)4 << 30;
+  c->sweeps_per_tick = 4;
+  c->props_per_wave = 0;
+  c->stage_bytes = (int64_t)64 << 20;
+}
+
+void pcx_server_destroy(pcx_server* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->cfg.device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void* p : {(void*)s->d_slots, (void*)s->d_flags, (void*)s->d_solo, (void*)s->d_act, (void*)s->d_item_base,
+                  (void*)s->d_ctl, (void*)s->d_tight, (void*)s->d_arena})
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)s->h_slots, (void*)s->h_flags, (void*)s->h_stage})
+    if (p) (void)hipHostFree(p);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+pcx_status pcx_server_create(const pcx_server_config* cfg, pcx_server** out) {
+  if (!out) return fail_with(PCX_EINVAL, "out is NULL");
+  *out = nullptr;
+  pcx_server_config c;
+  if (cfg) c = *cfg; else pcx_server_config_default(&c);
+  if (c.max_slots <= 0 || c.max_slots > (1 << 20) || c.arena_bytes < 8 || c.sweeps_per_tick <= 0 ||
+      c.stage_bytes < 8)
+    return fail_with(PCX_EINVAL, "bad server configuration");
+  if (c.props_per_wave && c.props_per_wave != 64 && c.props_per_wave != 128 &&
+      c.props_per_wave != 256 && c.props_per_wave != 512)
+    return fail_with(PCX_EINVAL, "props_per_wave must be 0, 64, 128, 256 or 512");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail_with(PCX_EDEVICE, "no HIP device");
+  if (c.device < 0 || c.device >= n) return fail_with(PCX_EINVAL, "device ordinal out of range");
+  PCX_HIP(hipSetDevice(c.device));
+  auto* s = new (std::nothrow) pcx_server();
+  if (!s) return fail_with(PCX_ENOMEM, "host allocation failed");
+  s->cfg = c;
+  s->n_slots = c.max_slots;
+  s->slots.resize(c.max_slots);
+  const int64_t words = c.arena_bytes / (int64_t)sizeof(int2);
+  s->stage_words = c.stage_bytes / (int64_t)sizeof(int2);
+  hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  auto chk = [&](hipError_t r) { if (he == hipSuccess) he = r; };
+  chk(hipMalloc(&s->d_slots, c.max_slots * sizeof(SlotDesc)));
+  chk(hipMalloc(&s->d_flags, c.max_slots * sizeof(InstFlags)));
+  chk(hipMalloc(&s->d_solo, sizeof(InstFlags)));
+  chk(hipMalloc(&s->d_act, c.max_slots * sizeof(int32_t)));
+  chk(hipMalloc(&s->d_item_base, (c.max_slots + 1) * sizeof(int64_t)));
+  chk(hipMalloc(&s->d_ctl, 4 * sizeof(int32_t)));
+  chk(hipMalloc(&s->d_tight, kTightSlots * sizeof(unsigned long long)));
+  chk(hipMalloc(&s->d_arena, words * sizeof(int2)));
+  chk(hipHostMalloc(&s->h_slots, c.max_slots * sizeof(SlotDesc), hipHostMallocDefault));
+  chk(hipHostMalloc(&s->h_flags, (c.max_slots + 1) * sizeof(InstFlags), hipHostMallocDefault));
+  chk(hipHostMalloc(&s->h_stage, s->stage_words * sizeof(int2), hipHostMallocDefault));
+  if (he != hipSuccess) {
+    pcx_server_destroy(s);
+    return fail_with(he == hipErrorOutOfMemory ? PCX_ENOMEM : PCX_EDEVICE,
+                     std::string("server allocation: ") + hipGetErrorString(he));
+  }
+  s->arena.init(words);
+  for (int32_t k = 0; k < c.max_slots; ++k) {
+    s->h_slots[k] = SlotDesc{nullptr, nullptr, nullptr, 0, nullptr};
+    InstFlags f{};
+    f.changed[0] = -2; f.changed[1] = -2; f.failed_at = kFreeFailed; f.done_at = 0;
+    s->h_flags[k] = f;
+  }
+  he = hipMemcpy(s->d_slots, s->h_slots, c.max_slots * sizeof(SlotDesc), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(s->d_flags, s->h_flags, c.max_slots * sizeof(InstFlags), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemset(s->d_tight, 0, kTightSlots * sizeof(unsigned long long));
+  const int32_t ctl[4] = {0, -1, 0, 0};          // n_act, sweep, live, unused
+  if (he == hipSuccess) he = hipMemcpy(s->d_ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice);
+  if (he != hipSuccess) {
+    pcx_server_destroy(s);
+    return fail_with(PCX_EDEVICE, std::string("server init: ") + hipGetErrorString(he));
+  }
+  *out = s;
+  return PCX_OK;
+}
+
+pcx_status pcx_server_add_network(pcx_server* s, pcx_engine* e, int32_t* net) {
+  if (!s || !e || !net) return fail_with(PCX_EINVAL, "NULL argument");
+  if (e->cfg.domain != PCX_DOMAIN_INTERVAL) return fail_with(PCX_EINVAL, "serving takes interval engines");
+  if (e->cfg.device != s->cfg.device) return fail_with(PCX_EINVAL, "engine and server are on different devices");
+  pcx_status st = pcx_internal::sync_table(e);
+  if (st != PCX_OK) return st;
+  s->nets.push_back(e);
+  *net = (int32_t)s->nets.size() - 1;
+  return PCX_OK;
+}
+
+pcx_status pcx_server_submit(pcx_server* s, int32_t net, int32_t* lb, int32_t* ub, int64_t nvars,
+                             int64_t tag) {
+  if (!s || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
+  if (net < 0 || net >= (int32_t)s->nets.size()) return fail_with(PCX_EINVAL, "unknown network");
+  pcx_engine* e = s->nets[net];
+  if (e->dirty) return fail_with(PCX_EINVAL, "network told more propagators after registration");
+  if (nvars <= e->max_var) return fail_with(PCX_EINVAL, "store smaller than the network's variables");
+  if (nvars > s->stage_words || nvars * (int64_t)sizeof(int2) > s->cfg.arena_bytes)
+    return fail_with(PCX_EINVAL, "store larger than the server's arena or staging buffer");
+  s->pending.push_back(Request{net, lb, ub, nvars, tag, now_s()});
+  return PCX_OK;
+}
+
+pcx_status pcx_server_step(pcx_server* s, int32_t* in_flight) {
+  if (!s) return fail_with(PCX_EINVAL, "NULL server");
+  PCX_HIP(hipSetDevice(s->cfg.device));
+  pcx_status st = tick(s);
+  if (in_flight) *in_flight = busy_slots(s) + (int32_t)s->pending.size();
+  return st;
+}
+
+pcx_status pcx_server_get_stats(const pcx_server* s, pcx_server_stats* out) {
+  if (!s || !out) return fail_with(PCX_EINVAL, "NULL argument");
+  *out = s->stats;
+  return PCX_OK;
+}
+
+int64_t pcx_server_poll(pcx_server* s, pcx_served* out, int64_t max) {
+  if (!s || !out || max <= 0) return 0;
+  const int64_t n = std::min<int64_t>(max, (int64_t)s->done.size());
+  std::copy(s->done.begin(), s->done.begin() + n, out);
+  s->done.erase(s->done.begin(), s->done.begin() + n);
+  return n;
+}
+
+pcx_status pcx_server_replay(pcx_server* s, int64_t n, const double* arrival_s, const int32_t* net,
+                             int32_t* lb_all, int32_t* ub_all, const int64_t* store_off,
+                             int32_t plain, pcx_served* out, double* wall_s) {
+  if (!s || n < 0 || (n > 0 && (!arrival_s || !net || !lb_all || !ub_all || !store_off || !out)))
+    return fail_with(PCX_EINVAL, "NULL argument");
+  if (!s->pending.empty() || busy_slots(s) > 0 || !s->done.empty())
+    return fail_with(PCX_EBUSY, "replay needs an idle server (no queued, running or unpolled requests)");
+  PCX_HIP(hipSetDevice(s->cfg.device));
+  for (int64_t i = 0; i < n; ++i) {
+    if (net[i] < 0 || net[i] >= (int32_t)s->nets.size()) return fail_with(PCX_EINVAL, "unknown network");
+    if (s->nets[net[i]]->dirty) return fail_with(PCX_EINVAL, "network told more propagators after registration");
+    const int64_t nv = store_off[i + 1] - store_off[i];
+    if (nv <= s->nets[net[i]]->max_var) return fail_with(PCX_EINVAL, "store smaller than the network's variables");
+    if (nv > s->stage_words || nv * (int64_t)sizeof(int2) > s->cfg.arena_bytes)
+      return fail_with(PCX_EINVAL, "store larger than the server's arena or staging buffer");
+    if (i && arrival_s[i] < arrival_s[i - 1]) return fail_with(PCX_EINVAL, "arrivals must be sorted");
+  }
+  const double t_start = now_s();
+  int64_t next = 0, completed = 0;
+  std::vector<double> arrive(n);
+  for (int64_t i = 0; i < n; ++i) arrive[i] = t_start + arrival_s[i];
+  if (plain) {
+    for (int64_t i = 0; i < n; ++i) {
+      while (now_s() < arrive[i]) std::this_thread::yield();
+      Request rq{net[i], lb_all + store_off[i], ub_all + store_off[i], store_off[i + 1] - store_off[i], i, arrive[i]};
+      pcx_result r{};
+      pcx_status st = run_alone(s, rq, &r);
+      if (st != PCX_OK) return st;
+      out[i].tag = i;
+      out[i].result = r;
+      out[i].latency_ms = (now_s() - arrive[i]) * 1e3;
+    }
+  } else {
+    std::vector<pcx_served> buf(256);
+    while (completed < n) {
+      const double now = now_s();
+      while (next < n && arrive[next] <= now) {
+        Request rq{net[next], lb_all + store_off[next], ub_all + store_off[next],
+                   store_off[next + 1] - store_off[next], next, arrive[next]};
+        s->pending.push_back(rq);
+        ++next;
+      }
+      if (s->pending.empty() && busy_slots(s) == 0) {
+        if (next < n) std::this_thread::yield();
+        continue;
+      }
+      pcx_status st = tick(s);
+      if (st != PCX_OK) return st;
+      int64_t got;
+      while ((got = pcx_server_poll(s, buf.data(), (int64_t)buf.size())) > 0)
+        for (int64_t j = 0; j < got; ++j) {
+          out[buf[j].tag] = buf[j];
+          ++completed;
+        }
+    }
+  }
+  if (wall_s) *wall_s = now_s() - t_start;
+  return PCX_OK;
+}
+
+}  // extern "C"

@@ -75,6 +75,27 @@ class GenParams(ctypes.Structure):
                 ("reserved", ctypes.c_int32 * 3)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("max_slots", ctypes.c_int32),
+                ("arena_bytes", ctypes.c_int64), ("sweeps_per_tick", ctypes.c_int32),
+                ("props_per_wave", ctypes.c_int32), ("stage_bytes", ctypes.c_int64),
+                ("reserved", ctypes.c_int32 * 4)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("ticks", ctypes.c_int64), ("sweeps", ctypes.c_int64), ("admitted", ctypes.c_int64),
+                ("completed", ctypes.c_int64), ("waits_slot", ctypes.c_int64),
+                ("waits_arena", ctypes.c_int64), ("arena_peak_bytes", ctypes.c_int64),
+                ("max_running", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class Served(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_int64), ("result", Result), ("latency_ms", ctypes.c_double)]
+
+
 _P32 = ctypes.POINTER(ctypes.c_int32)
 _P64 = ctypes.POINTER(ctypes.c_int64)
 _PU8 = ctypes.POINTER(ctypes.c_uint8)
@@ -106,6 +127,18 @@ SIGNATURES = [
     ("pcx_bits_fixpoint", ctypes.c_int32, [_VP, _PU64, ctypes.c_int64, ctypes.POINTER(Result)]),
     ("pcx_bits_batch_load", ctypes.c_int32, [_VP, _PU64]),
     ("pcx_bits_batch_read", ctypes.c_int32, [_VP, _PU64]),
+    ("pcx_server_config_default", None, [ctypes.POINTER(ServerConfig)]),
+    ("pcx_server_create", ctypes.c_int32, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_VP)]),
+    ("pcx_server_destroy", None, [_VP]),
+    ("pcx_server_add_network", ctypes.c_int32, [_VP, _VP, _P32]),
+    ("pcx_server_submit", ctypes.c_int32, [_VP, ctypes.c_int32, _P32, _P32, ctypes.c_int64,
+                                           ctypes.c_int64]),
+    ("pcx_server_step", ctypes.c_int32, [_VP, _P32]),
+    ("pcx_server_get_stats", ctypes.c_int32, [_VP, ctypes.POINTER(ServerStats)]),
+    ("pcx_server_poll", ctypes.c_int64, [_VP, ctypes.POINTER(Served), ctypes.c_int64]),
+    ("pcx_server_replay", ctypes.c_int32, [_VP, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), _P32,
+                                           _P32, _P32, _P64, ctypes.c_int32, ctypes.POINTER(Served),
+                                           ctypes.POINTER(ctypes.c_double)]),
     ("pcxg_params_default", None, [ctypes.POINTER(GenParams)]),
     ("pcxg_network", ctypes.c_int32, [ctypes.POINTER(GenParams), _P32, _P32, _P32, _P32, _P32, _P32, _P32]),
     ("pcxg_splits", ctypes.c_int32, [ctypes.c_uint64, _P32, _P32, ctypes.c_int64, ctypes.c_int64,
@@ -316,6 +349,82 @@ class Batch:
         st = Stats()
         _check(lib().pcx_batch_run(self._h, mode, res, ctypes.byref(st)))
         return list(res), st
+
+
+class Server:
+    """Continuous-batching fixpoint server over registered PIR networks.
+
+    submit() queues a store (int32 lb/ub arrays that stay owned by the
+    caller and receive the fixpoint); step() runs one tick; poll() returns
+    completed (tag, Result, latency_ms) tuples.  replay() drives a whole
+    arrival trace in real time inside the library.
+    """
+
+    def __init__(self, **kw):
+        c = ServerConfig()
+        lib().pcx_server_config_default(ctypes.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        h = _VP()
+        _check(lib().pcx_server_create(ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+        self.config = c
+        self._nets = []
+        self._live = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pcx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_network(self, pir):
+        n = ctypes.c_int32(-1)
+        _check(lib().pcx_server_add_network(self._h, pir._h, ctypes.byref(n)))
+        self._nets.append(pir)          # keeps the engine alive while registered
+        return n.value
+
+    def submit(self, net, lb, ub, tag):
+        _check(lib().pcx_server_submit(self._h, int(net), _p32(lb), _p32(ub), len(lb), int(tag)))
+        self._live[int(tag)] = (lb, ub)   # the arrays must outlive the request
+
+    def step(self):
+        n = ctypes.c_int32(0)
+        _check(lib().pcx_server_step(self._h, ctypes.byref(n)))
+        return n.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().pcx_server_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def poll(self, max_items=256):
+        buf = (Served * max_items)()
+        got = int(lib().pcx_server_poll(self._h, buf, max_items))
+        out = []
+        for i in range(got):
+            self._live.pop(buf[i].tag, None)
+            out.append((buf[i].tag, buf[i].result, buf[i].latency_ms))
+        return out
+
+    def replay(self, arrival_s, net, lb_all, ub_all, store_off, plain=False):
+        """Serve a trace; returns (list of Served, wall seconds)."""
+        n = len(arrival_s)
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        net = np.ascontiguousarray(net, dtype=np.int32)
+        store_off = np.ascontiguousarray(store_off, dtype=np.int64)
+        assert len(store_off) == n + 1 and lb_all.size >= store_off[-1] and ub_all.size >= store_off[-1]
+        out = (Served * max(n, 1))()
+        wall = ctypes.c_double(0.0)
+        _check(lib().pcx_server_replay(self._h, n, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       _p32(net), _p32(lb_all), _p32(ub_all), _p64(store_off),
+                                       1 if plain else 0, out, ctypes.byref(wall)))
+        return list(out)[:n], wall.value
 
 
 # ---- synthetic workloads (host code, no GPU) ------------------------------
