@@ -39,7 +39,7 @@ import pcx  # noqa: E402
 METRIC = "fixpoints/sec batched vs plain; propagator evals/s; %HBM roofline; p99 latency"
 HBM_PEAK_GBS = 8000.0
 BYTES_READ_PER_EVAL = 40     # 16 B propagator + 3 x 8 B operand intervals
-BYTES_PER_TIGHTENING = 4     # one int32 bound written
+BYTES_PER_TIGHTENING = 4     # one int32 bound written (a bitset word change writes 8 B)
 
 
 def log(msg):
@@ -51,12 +51,30 @@ def host_threads():
     return max(1, min(n, 16))
 
 
-def roofline(evals, tightened, kernel_ms):
-    algo_bytes = evals * BYTES_READ_PER_EVAL + tightened * BYTES_PER_TIGHTENING
-    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or
+    None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return int(float(line.split("=")[1]))
+    return None
+
+
+def roofline(evals, tightened, kernel_ms, launches, kernel, workload, tight_bytes=BYTES_PER_TIGHTENING):
+    """Algorithmic bytes per launch / average launch duration (HIP events on
+    the engine's stream around every launch of the dominant kernel)."""
+    algo_bytes = evals * BYTES_READ_PER_EVAL + tightened * tight_bytes
+    launches = max(1, launches)
+    achieved = (algo_bytes / launches) / (kernel_ms * 1e-3 / launches) / 1e9 if kernel_ms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "algorithmic_bytes_per_step": int(algo_bytes)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel, workload),
+            "kernel": kernel, "launches_per_step": launches,
+            "algorithmic_bytes_per_launch": int(algo_bytes / launches),
+            "avg_launch_us": round(kernel_ms * 1e3 / launches, 2)}
 
 
 def cpu_oracle():
@@ -129,10 +147,9 @@ def bench_batched(args):
         "evals_per_s": evals / elapsed,
         "failed_instances": int(failed),
         "sweeps_max": int(smax),
-        "roofline": roofline(evals / args.steps, tight / args.steps, kms / args.steps),
-        "kernel_launches_per_step": launches / args.steps,
+        "roofline": roofline(evals / args.steps, tight / args.steps, kms / args.steps, launches / args.steps,
+                             "k_sweep_batched" if args.mode == "batched" else "k_persistent", "cfg2"),
     }
-    out["roofline"]["kernel"] = "k_sweep_batched" if args.mode == "batched" else "k_persistent"
 
     if not args.no_cpu:
         ol = cpu_oracle()
@@ -188,7 +205,7 @@ def bench_single(args):
     for _ in range(args.warmup):
         batch.reset()
         batch.run(pcx.MODE_PLAIN)
-    evals = tight = 0
+    evals = tight = launches = 0
     kms = 0.0
     t_start = time.perf_counter()
     for _ in range(args.steps):
@@ -197,6 +214,7 @@ def bench_single(args):
         evals += st.evals
         tight += st.tightened
         kms += st.kernel_ms
+        launches += st.kernel_launches
     elapsed = time.perf_counter() - t_start
     lb_g, ub_g = batch.read()
     out = {
@@ -204,9 +222,9 @@ def bench_single(args):
         "config": {"workload": "single 1,000,000-var / 4,000,000-propagator fixpoint (plain engine)",
                    "nvars": V, "nprops": P, "props_per_wave": args.ppw},
         "evals_per_s": evals / elapsed, "sweeps": int(res[0].sweeps), "failed": bool(res[0].failed),
-        "roofline": roofline(evals / args.steps, tight / args.steps, kms / args.steps),
+        "roofline": roofline(evals / args.steps, tight / args.steps, kms / args.steps, launches / args.steps,
+                             "k_sweep_plain", "cfg1"),
     }
-    out["roofline"]["kernel"] = "k_sweep_plain"
     if not args.no_cpu:
         ol = cpu_oracle()
         top, tx, ty, tz = pir.table()
@@ -240,7 +258,7 @@ def bench_bits(args):
     for _ in range(args.warmup):
         batch.reset()
         batch.run(mode)
-    evals = tight = 0
+    evals = tight = launches = 0
     kms = 0.0
     t_start = time.perf_counter()
     for _ in range(args.steps):
@@ -249,11 +267,13 @@ def bench_bits(args):
         evals += st.evals
         tight += st.tightened
         kms += st.kernel_ms
+        launches += st.kernel_launches
     elapsed = time.perf_counter() - t_start
     fps = n_inst * args.steps / elapsed
     words_g = batch.read_words()
-    rf = roofline(evals / args.steps, tight / args.steps, kms / args.steps)
-    rf["kernel"] = "k_sweep_batched<bits>" if args.mode == "batched" else "k_persistent<bits>"
+    rf = roofline(evals / args.steps, tight / args.steps, kms / args.steps, launches / args.steps,
+                  "k_sweep_batched_bits" if args.mode == "batched" else "k_persistent_bits", "cfg3",
+                  tight_bytes=8)
     out = {"value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
            "config": {"workload": "bitset store: 1,024 bit-removal subproblems of one 200,000-word / "
                                   "800,000-propagator network (add, neq, leq, eq-reif)",
@@ -401,7 +421,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4])
-    ap.add_argument("--mode", default="persistent", choices=["batched", "persistent"],
+    ap.add_argument("--mode", default="batched", choices=["batched", "persistent"],
                     help="engine for the timed batch (the plain engine is always run once beside it)")
     ap.add_argument("--ppw", type=int, default=0, choices=[0, 64, 128, 256, 512],
                     help="propagators per wave (0 = engine default)")
