@@ -77,7 +77,12 @@ typedef struct {
   int64_t max_sweeps;       /* 0 = run to the fixpoint                               */
   int32_t timing;           /* 1 = record HIP events around every sweep kernel       */
   int32_t domain;           /* PCX_DOMAIN_INTERVAL (default) or PCX_DOMAIN_BITSET    */
-  int32_t reserved[6];
+  int32_t compact_tables;   /* batches with subsets: -1 auto, 0 read the shared table
+                               through the subset indices, 1 give each instance its
+                               own copy of its subset (12 B per active propagator)  */
+  int32_t stream_tables;    /* -1 auto (stream per-instance tables only), 0 cached
+                               table loads, 1 non-temporal table loads               */
+  int32_t reserved[4];
 } pcx_config;
 
 typedef struct {

@@ -77,6 +77,7 @@ pcx_status sync_table(pcx_engine* e) {
     PCX_HIP(hipMemcpy(e->d_z, e->z.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
   }
   e->dirty = false;
+  ++e->version;
   return PCX_OK;
 }
 
@@ -161,6 +162,58 @@ pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* resu
   return PCX_OK;
 }
 
+// Per-instance subset copies.  Reading a shared table through subset
+// indices gathers 12-byte entries out of 128-byte lines and keeps the shared
+// table competing with the instance stores for L2; a contiguous copy per
+// instance is read once per sweep as a coalesced stream (non-temporal, so the
+// stores stay cached) at the price of 12 B of HBM per active propagator.
+// Automatic mode copies when that costs at most a quarter of free memory.
+pcx_status prepare_tables(pcx_batch* b) {
+  pcx_engine* e = b->e;
+  bool want = false;
+  const int64_t total = b->h_off.empty() ? 0 : b->h_off.back();
+  if (!b->h_off.empty() && total > 0) {
+    if (e->cfg.compact_tables > 0) {
+      want = true;
+    } else if (e->cfg.compact_tables < 0) {
+      size_t free_b = 0, total_b = 0;
+      if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+        want = b->d_cxo != nullptr || (uint64_t)total * 12 <= free_b / 4;
+    }
+  }
+  if (want && (!b->d_cxo || b->ctab_version != e->version)) {
+    if (!b->d_cxo) {
+      hipError_t he = hipMalloc(&b->d_cxo, total * sizeof(uint32_t));
+      if (he == hipSuccess) he = hipMalloc(&b->d_cy, total * sizeof(int32_t));
+      if (he == hipSuccess) he = hipMalloc(&b->d_cz, total * sizeof(int32_t));
+      if (he != hipSuccess) {
+        dfree(b->d_cxo); dfree(b->d_cy); dfree(b->d_cz);
+        (void)hipGetLastError();
+        if (e->cfg.compact_tables > 0)
+          return fail_with(PCX_ENOMEM, "no device memory for per-instance tables");
+        want = false;                  // automatic mode: read through the indices
+      }
+    }
+    if (want) {
+      PCX_HIP(launch_compact_table(table_of(e), b->d_idx, total, b->d_cxo, b->d_cy, b->d_cz, e->stream));
+      b->ctab_version = e->version;
+    }
+  }
+  b->compact = want;
+  b->stream = e->cfg.stream_tables > 0 || (e->cfg.stream_tables < 0 && b->compact);
+  return PCX_OK;
+}
+
+// Table and subsets a sweep kernel reads for this batch.
+Table run_table(const pcx_batch* b) {
+  if (b->compact) return Table{b->d_cxo, b->d_cy, b->d_cz, b->h_off.back()};
+  return table_of(b->e);
+}
+Subsets run_subsets(const pcx_batch* b) {
+  if (b->h_off.empty()) return Subsets{nullptr, nullptr};
+  return Subsets{b->d_off, b->compact ? nullptr : b->d_idx};
+}
+
 pcx_status prepare_run(pcx_batch* b) {
   pcx_engine* e = b->e;
   if ((int64_t)e->op.size() != b->table_n)
@@ -170,10 +223,10 @@ pcx_status prepare_run(pcx_batch* b) {
   pcx_status s = sync_table(e);
   if (s != PCX_OK) return s;
   PCX_HIP(hipMemsetAsync(b->d_tight, 0, kTightSlots * sizeof(unsigned long long), e->stream));
-  WorkList wl{b->d_act, b->d_item_base, b->d_ctl, b->d_ctl + 1, b->d_ctl + 2};
+  const WorkList wl = worklist_of(b->d_act, b->d_item_base, b->d_ctl);
   PCX_HIP(launch_init_flags(b->d_flags, (int32_t)b->n_inst, wl, e->stream));
   PCX_HIP(launch_store_check_empty(b->d_store, b->nvars, (int32_t)b->n_inst, b->d_flags, dom_of(e), e->stream));
-  return PCX_OK;
+  return prepare_tables(b);
 }
 
 // Plain mode: every instance on its own, one launch per sweep, host checks
@@ -189,7 +242,13 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   for (int64_t k = 0; k < b->n_inst; ++k) {
     const int64_t m = inst_size(b, k);
     const int ppw = ppw_of(e, m);
-    const int32_t* idx = b->h_off.empty() ? nullptr : b->d_idx + b->h_off[k];
+    Table tk = t;
+    const int32_t* idx = nullptr;
+    if (!b->h_off.empty()) {
+      const int64_t beg = b->h_off[k];
+      if (b->compact) tk = Table{b->d_cxo + beg, b->d_cy + beg, b->d_cz + beg, m};
+      else idx = b->d_idx + beg;
+    }
     int2* st = b->d_store + k * b->nvars;
     InstFlags* f = b->d_flags + k;
     int32_t s = 0;
@@ -197,7 +256,7 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       for (int g = 0; g < group; ++g, ++s) {
         if (e->cfg.max_sweeps > 0 && s >= e->cfg.max_sweeps) break;
         if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
-        PCX_HIP(launch_sweep_plain(t, st, idx, m, s, f, b->d_tight, ppw, dom_of(e), e->stream));
+        PCX_HIP(launch_sweep_plain(tk, st, idx, m, s, f, b->d_tight, ppw, dom_of(e), b->stream, e->stream));
         if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
         ++launches;
       }
@@ -224,13 +283,10 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
   const int ppw = ppw_of(e);
   const int group = e->cfg.sweeps_per_check > 0 ? e->cfg.sweeps_per_check : 8;
-  const Table t = table_of(e);
-  int dev_cus = 256;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, e->cfg.device) == hipSuccess) dev_cus = prop.multiProcessorCount;
-  const int grid = dev_cus * 8;          // a multiple of the XCD count
-  Subsets sub{b->h_off.empty() ? nullptr : b->d_off, b->d_idx};
-  WorkList wl{b->d_act, b->d_item_base, b->d_ctl, b->d_ctl + 1, b->d_ctl + 2};
+  const Table t = run_table(b);
+  const int grid = pcx_internal::device_cus(e->cfg.device) * 8;   // a multiple of the XCD count
+  const Subsets sub = run_subsets(b);
+  const WorkList wl = worklist_of(b->d_act, b->d_item_base, b->d_ctl);
   int64_t launches = 0;
   double kms = 0.0;
   size_t nev = 0;
@@ -241,7 +297,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream));
       if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
       PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
-                                   b->d_tight, ppw, dom_of(e), grid, e->stream));
+                                   b->d_tight, ppw, dom_of(e), b->stream, grid, e->stream));
       if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
       ++launches;
       ++sweeps;
@@ -264,13 +320,11 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
 // ordered largest first and run each to its fixpoint.
 pcx_status run_persistent(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
-  const Table t = table_of(e);
-  int dev_cus = 256;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, e->cfg.device) == hipSuccess) dev_cus = prop.multiProcessorCount;
+  const Table t = run_table(b);
+  const int dev_cus = pcx_internal::device_cus(e->cfg.device);
   const int per_cu = persistent_blocks_per_cu(dom_of(e));
   const int grid = (int)std::min<int64_t>((int64_t)dev_cus * per_cu, b->n_inst);
-  Subsets sub{b->h_off.empty() ? nullptr : b->d_off, b->d_idx};
+  const Subsets sub = run_subsets(b);
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
   double kms = 0.0;
   if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, 0), e->stream));
@@ -305,6 +359,8 @@ void pcx_config_default(pcx_config* c) {
   c->sweeps_per_check = 8;
   c->max_sweeps = 0;
   c->timing = 0;
+  c->compact_tables = -1;
+  c->stream_tables = -1;
 }
 
 int32_t pcx_device_count(void) {
@@ -466,7 +522,7 @@ pcx_status pcx_batch_create(pcx_engine* e, int64_t n_inst, int64_t nvars, const 
   chk(hipMalloc(&b->d_tight, kTightSlots * sizeof(unsigned long long)));
   chk(hipMalloc(&b->d_act, n_inst * sizeof(int32_t)));
   chk(hipMalloc(&b->d_item_base, (n_inst + 1) * sizeof(int64_t)));
-  chk(hipMalloc(&b->d_ctl, 4 * sizeof(int32_t)));
+  chk(hipMalloc(&b->d_ctl, kCtlWords * sizeof(int32_t)));
   if (inst_off && he == hipSuccess) {
     b->h_off.assign(inst_off, inst_off + n_inst + 1);
     chk(hipMalloc(&b->d_off, (n_inst + 1) * sizeof(int64_t)));
@@ -505,6 +561,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   for (hipEvent_t ev : b->ev) (void)hipEventDestroy(ev);
   dfree(b->d_store); dfree(b->d_image); dfree(b->d_off); dfree(b->d_idx); dfree(b->d_flags);
   dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl); dfree(b->d_order);
+  dfree(b->d_cxo); dfree(b->d_cy); dfree(b->d_cz);
   delete b;
 }
 

@@ -14,6 +14,7 @@ struct pcx_engine {
   int32_t max_var = -1;
   bool has_div = false;               // table holds a division propagator
   bool dirty = true;
+  uint64_t version = 0;               // bumped on every upload of the table
   uint32_t* d_xo = nullptr;
   int32_t* d_y = nullptr;
   int32_t* d_z = nullptr;
@@ -33,11 +34,19 @@ struct pcx_batch {
   int64_t* d_off = nullptr;
   int32_t* d_idx = nullptr;
   std::vector<int64_t> h_off;         // empty = whole table
+  // Per-instance copies of the subsets (instance k's propagators at
+  // [h_off[k], h_off[k+1]) of these arrays), built on first run.
+  uint32_t* d_cxo = nullptr;
+  int32_t* d_cy = nullptr;
+  int32_t* d_cz = nullptr;
+  uint64_t ctab_version = 0;          // engine table version they were built from
+  bool compact = false;               // this run reads the per-instance copies
+  bool stream = false;                // this run streams table loads
   pcx::InstFlags* d_flags = nullptr;
   unsigned long long* d_tight = nullptr;
   int32_t* d_act = nullptr;
   int64_t* d_item_base = nullptr;
-  int32_t* d_ctl = nullptr;           // [0]=n_act [1]=sweep [2]=live [3]=queue head
+  int32_t* d_ctl = nullptr;           // kCtlWords control words (see WorkList)
   int32_t* d_order = nullptr;         // instances by decreasing size (persistent queue)
   std::vector<hipEvent_t> ev;         // timing pool
 };

@@ -44,12 +44,18 @@ struct InstFlags {
   int32_t done_at;      // sweep at which the scheduler retired the instance
 };
 
-// Per-instance propagator subset: prop_idx[off[k] .. off[k+1]) or the whole
-// table when off == nullptr.
+// Per-instance propagator subset: prop_idx[off[k] .. off[k+1]) of the shared
+// table, the whole table when off == nullptr, or -- off without idx -- the
+// instance's own copy of its subset at [off[k], off[k+1]) of the table.
 struct Subsets {
   const int64_t* __restrict__ off;
   const int32_t* __restrict__ idx;
 };
+
+// Control words of a batch: [0] n_act, [1] sweep, [2] live, [3] persistent
+// queue head, [8 .. 8+kXcds) per-XCD item counters of the current sweep.
+constexpr int kCtlWords = 16;
+constexpr int kCtlXq = 8;
 
 // Device work list built by the scheduler for one sweep of the batched mode.
 struct WorkList {
@@ -59,7 +65,11 @@ struct WorkList {
   int32_t* sweep;        // [1] current sweep number
   int32_t* live;         // [1] instances still running after the last schedule;
                          //     live[1] is the persistent mode's queue head
+  int32_t* xq;           // [kXcds] items claimed so far on each XCD (zeroed per sweep)
 };
+inline WorkList worklist_of(int32_t* act, int64_t* item_base, int32_t* ctl) {
+  return WorkList{act, item_base, ctl, ctl + 1, ctl + 2, ctl + kCtlXq};
+}
 
 // One serving slot: a request's own table and store (heterogeneous batch).
 struct SlotDesc {
@@ -76,13 +86,13 @@ struct SlotDesc {
 namespace pcx {
 hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, int64_t m,
                               int32_t sweep, InstFlags* flags, unsigned long long* tight,
-                              int ppw, int dom, hipStream_t st);
+                              int ppw, int dom, bool stream_table, hipStream_t st);
 hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags, int32_t n_inst,
                            WorkList wl, int ppw, int32_t max_sweeps, hipStream_t st);
 hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                                 int64_t n_props, InstFlags* flags, WorkList wl,
-                                unsigned long long* tight, int ppw, int dom, int grid,
-                                hipStream_t st);
+                                unsigned long long* tight, int ppw, int dom, bool stream_table,
+                                int grid, hipStream_t st);
 hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                              int64_t n_props, InstFlags* flags, const int32_t* order,
                              int32_t n_inst, int32_t* queue, unsigned long long* tight,
@@ -94,6 +104,8 @@ hipError_t launch_schedule_slots(const SlotDesc* slots, InstFlags* flags, int32_
 hipError_t launch_sweep_slots(const SlotDesc* slots, InstFlags* flags, WorkList wl,
                               unsigned long long* tight, int ppw, int dom, int grid,
                               hipStream_t st);
+hipError_t launch_compact_table(const Table& t, const int32_t* idx, int64_t n, uint32_t* xo,
+                                int32_t* y, int32_t* z, hipStream_t st);
 hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
                       hipStream_t st);
 hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n, hipStream_t st);

@@ -96,11 +96,20 @@ __device__ __forceinline__ void apply(int2* __restrict__ st, int op, int32_t x, 
   }
 }
 
-template <int D>
+// Table loads: cached (a shared table that other instances re-read) or
+// non-temporal (NT: a table read once per sweep, streamed past the caches so
+// that the store lines, which every propagator gathers from, stay in L2).
+template <bool NT, class T>
+__device__ __forceinline__ T table_load(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+template <int D, bool NT>
 __device__ __forceinline__ void eval_prop(const Table& t, int2* __restrict__ st, int64_t i,
                                           bool& chg, bool& fail, int& nt) {
-  const uint32_t xo = t.xo[i];
-  const int32_t y = t.y[i], z = t.z[i];
+  const uint32_t xo = table_load<NT>(t.xo + i);
+  const int32_t y = table_load<NT>(t.y + i), z = table_load<NT>(t.z + i);
   const int32_t x = (int32_t)(xo & kVarMask);
   const u64 wx = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + x);
   const u64 wy = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + y);
@@ -122,8 +131,24 @@ __device__ __forceinline__ void publish(InstFlags* f, int32_t s, bool chg, bool 
   }
 }
 
+// Instance k's propagators: the whole table (no subsets), the shared table
+// through subset indices, or its own copy of its subset (off without idx:
+// the table argument holds every instance's copy back to back).
+struct InstTable {
+  Table t;
+  const int32_t* idx;
+  int64_t m;
+};
+__device__ __forceinline__ InstTable inst_table(const Table& t, const Subsets& sub, int64_t k,
+                                                int64_t n_props) {
+  if (!sub.off) return InstTable{t, nullptr, n_props};
+  const int64_t beg = sub.off[k], m = sub.off[k + 1] - beg;
+  if (sub.idx) return InstTable{t, sub.idx + beg, m};
+  return InstTable{Table{t.xo + beg, t.y + beg, t.z + beg, m}, nullptr, m};
+}
+
 // ---- plain mode: one instance, one sweep per launch -----------------------
-template <int PPL, int D>
+template <int PPL, int D, bool NT>
 __global__ __launch_bounds__(kBlock) void k_sweep_plain(Table t, int2* __restrict__ st,
                                                         const int32_t* __restrict__ idx, int64_t m,
                                                         int32_t s, InstFlags* f,
@@ -136,12 +161,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_plain(Table t, int2* __restric
 #pragma unroll
   for (int j = 0; j < PPL; ++j) {
     const int64_t k = base + j * kBlock + threadIdx.x;
-    if (k < m) eval_prop<D>(t, st, prop_at(idx, k), chg, fail, nt);
+    if (k < m) eval_prop<D, NT>(t, st, prop_at(idx, k), chg, fail, nt);
   }
   publish(f, s, chg, fail, nt, tight);
 }
 
-template <int D>
+template <int D, bool NT>
 hipError_t launch_plain_d(const Table& t, int2* store, const int32_t* idx, int64_t m,
                           int32_t sweep, InstFlags* flags, unsigned long long* tight, int ppl,
                           hipStream_t stream) {
@@ -150,25 +175,34 @@ hipError_t launch_plain_d(const Table& t, int2* store, const int32_t* idx, int64
   if (grid <= 0) return hipSuccess;
   const dim3 g((unsigned)grid), b(kBlock);
   switch (ppl) {
-    case 1: k_sweep_plain<1, D><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
-    case 2: k_sweep_plain<2, D><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
-    case 4: k_sweep_plain<4, D><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
-    case 8: k_sweep_plain<8, D><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
+    case 1: k_sweep_plain<1, D, NT><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
+    case 2: k_sweep_plain<2, D, NT><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
+    case 4: k_sweep_plain<4, D, NT><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
+    case 8: k_sweep_plain<8, D, NT><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, int64_t m,
-                              int32_t sweep, InstFlags* flags, unsigned long long* tight,
-                              int ppw, int dom, hipStream_t stream) {
-  const int ppl = ppw / kWave;
+template <bool NT>
+hipError_t launch_plain_nt(const Table& t, int2* store, const int32_t* idx, int64_t m,
+                           int32_t sweep, InstFlags* flags, unsigned long long* tight, int ppl,
+                           int dom, hipStream_t stream) {
   switch (dom) {
-    case kItv: return launch_plain_d<kItv>(t, store, idx, m, sweep, flags, tight, ppl, stream);
-    case kItvDiv: return launch_plain_d<kItvDiv>(t, store, idx, m, sweep, flags, tight, ppl, stream);
-    case kBits: return launch_plain_d<kBits>(t, store, idx, m, sweep, flags, tight, ppl, stream);
+    case kItv: return launch_plain_d<kItv, NT>(t, store, idx, m, sweep, flags, tight, ppl, stream);
+    case kItvDiv: return launch_plain_d<kItvDiv, NT>(t, store, idx, m, sweep, flags, tight, ppl, stream);
+    case kBits: return launch_plain_d<kBits, NT>(t, store, idx, m, sweep, flags, tight, ppl, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, int64_t m,
+                              int32_t sweep, InstFlags* flags, unsigned long long* tight,
+                              int ppw, int dom, bool stream_table, hipStream_t stream) {
+  const int ppl = ppw / kWave;
+  return stream_table
+             ? launch_plain_nt<true>(t, store, idx, m, sweep, flags, tight, ppl, dom, stream)
+             : launch_plain_nt<false>(t, store, idx, m, sweep, flags, tight, ppl, dom, stream);
 }
 
 // ---- batched mode: device scheduler + grid-stride sweep over work items ---
@@ -185,6 +219,7 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
   __shared__ int32_t cnt[kSchedThreads];
   const int32_t s = *wl.sweep + 1;       // the sweep being scheduled
   const int tid = threadIdx.x;
+  if (tid < kXcds) wl.xq[tid] = 0;
   const int per = (n_inst + kSchedThreads - 1) / kSchedThreads;
   const int lo = min(n_inst, tid * per), hi = min(n_inst, lo + per);
   int64_t items = 0;
@@ -240,7 +275,7 @@ hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags
   return hipGetLastError();
 }
 
-template <int PPL, int D>
+template <int PPL, int D, bool NT>
 __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restrict__ store,
                                                           int64_t nvars, Subsets sub,
                                                           int64_t n_props, InstFlags* flags,
@@ -252,13 +287,24 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
   constexpr int64_t chunk = (int64_t)kBlock * PPL;
   // XCD-aware split: workgroups are dispatched round-robin over the 8 XCDs,
   // so workgroup b runs on XCD b % 8.  Each XCD takes one contiguous eighth
-  // of the (instance-major) item list and walks it in order, so at any time
-  // an XCD's L2 holds the stores of only the few instances it is sweeping.
+  // of the (instance-major) item list, and its workgroups claim that range's
+  // items in order from a per-XCD counter.  Claiming (rather than a fixed
+  // stride) keeps the items in flight on an XCD a narrow window even when
+  // workgroups run at different speeds, so its L2 holds the stores of only
+  // the few instances being swept.  The next item is claimed while the
+  // current one runs.
   const int xcd = blockIdx.x % kXcds;
-  const int64_t lanes = gridDim.x / kXcds;            // workgroups per XCD
   const int64_t first_item = total * xcd / kXcds;
   const int64_t end_item = total * (xcd + 1) / kXcds;
-  for (int64_t item = first_item + blockIdx.x / kXcds; item < end_item; item += lanes) {
+  int32_t* qc = wl.xq + xcd;
+  __shared__ int32_t s_next;
+  if (threadIdx.x == 0) s_next = atomicAdd(qc, 1);
+  __syncthreads();
+  int64_t item = first_item + s_next;
+  __syncthreads();
+  while (item < end_item) {
+    int32_t nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(qc, 1);
     // active slot holding this item: last a with item_base[a] <= item
     int lo = 0, hi = n_act - 1;
     while (lo < hi) {
@@ -268,9 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
     const int32_t k = wl.act[lo];
     InstFlags* f = flags + k;
     const int64_t first = (item - wl.item_base[lo]) * chunk;
-    const int64_t beg = sub.off ? sub.off[k] : 0;
-    const int64_t m = sub.off ? sub.off[k + 1] - beg : n_props;
-    const int32_t* idx = sub.off ? sub.idx + beg : nullptr;
+    const InstTable it = inst_table(t, sub, k, n_props);
     int2* st = store + (int64_t)k * nvars;
     bool chg = false, fail = false;
     int nt = 0;
@@ -278,38 +322,55 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
         const int64_t q = first + j * kBlock + threadIdx.x;
-        if (q < m) eval_prop<D>(t, st, prop_at(idx, q), chg, fail, nt);
+        if (q < it.m) eval_prop<D, NT>(it.t, st, prop_at(it.idx, q), chg, fail, nt);
       }
     }
     publish(f, s, chg, fail, nt, tight);
+    if (threadIdx.x == 0) s_next = nxt;
+    __syncthreads();
+    item = first_item + s_next;
+    __syncthreads();
   }
 }
 
-template <int D>
+template <int D, bool NT>
 hipError_t launch_batched_d(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                             int64_t n_props, InstFlags* flags, WorkList wl,
                             unsigned long long* tight, int ppl, int grid, hipStream_t stream) {
+#define PCX_BATCHED(P) \
+  k_sweep_batched<P, D, NT><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight)
   switch (ppl) {
-    case 1: k_sweep_batched<1, D><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight); break;
-    case 2: k_sweep_batched<2, D><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight); break;
-    case 4: k_sweep_batched<4, D><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight); break;
-    case 8: k_sweep_batched<8, D><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight); break;
+    case 1: PCX_BATCHED(1); break;
+    case 2: PCX_BATCHED(2); break;
+    case 4: PCX_BATCHED(4); break;
+    case 8: PCX_BATCHED(8); break;
     default: return hipErrorInvalidValue;
   }
+#undef PCX_BATCHED
   return hipGetLastError();
+}
+
+template <bool NT>
+hipError_t launch_batched_nt(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
+                             int64_t n_props, InstFlags* flags, WorkList wl,
+                             unsigned long long* tight, int ppl, int dom, int grid,
+                             hipStream_t stream) {
+  switch (dom) {
+    case kItv: return launch_batched_d<kItv, NT>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
+    case kItvDiv: return launch_batched_d<kItvDiv, NT>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
+    case kBits: return launch_batched_d<kBits, NT>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                                 int64_t n_props, InstFlags* flags, WorkList wl,
-                                unsigned long long* tight, int ppw, int dom, int grid,
-                                hipStream_t stream) {
+                                unsigned long long* tight, int ppw, int dom, bool stream_table,
+                                int grid, hipStream_t stream) {
   const int ppl = ppw / kWave;
-  switch (dom) {
-    case kItv: return launch_batched_d<kItv>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
-    case kItvDiv: return launch_batched_d<kItvDiv>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
-    case kBits: return launch_batched_d<kBits>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
-    default: return hipErrorInvalidValue;
-  }
+  return stream_table
+             ? launch_batched_nt<true>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, dom, grid, stream)
+             : launch_batched_nt<false>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, dom, grid, stream);
 }
 
 // ---- serving slots: heterogeneous batch, one table and store per slot ----
@@ -326,6 +387,7 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule_slots(const SlotDesc
   __shared__ int32_t cnt[kSchedThreads];
   const int32_t s = *wl.sweep + 1;
   const int tid = threadIdx.x;
+  if (tid < kXcds) wl.xq[tid] = 0;
   const int per = (n_slots + kSchedThreads - 1) / kSchedThreads;
   const int lo = min(n_slots, tid * per), hi = min(n_slots, lo + per);
   int64_t items = 0;
@@ -386,10 +448,17 @@ __global__ __launch_bounds__(kBlock) void k_sweep_slots(const SlotDesc* slots, I
   const int64_t total = wl.item_base[n_act];
   constexpr int64_t chunk = (int64_t)kBlock * PPL;
   const int xcd = blockIdx.x % kXcds;
-  const int64_t lanes = gridDim.x / kXcds;
   const int64_t first_item = total * xcd / kXcds;
   const int64_t end_item = total * (xcd + 1) / kXcds;
-  for (int64_t item = first_item + blockIdx.x / kXcds; item < end_item; item += lanes) {
+  int32_t* qc = wl.xq + xcd;
+  __shared__ int32_t s_next;
+  if (threadIdx.x == 0) s_next = atomicAdd(qc, 1);
+  __syncthreads();
+  int64_t item = first_item + s_next;
+  __syncthreads();
+  while (item < end_item) {
+    int32_t nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(qc, 1);
     int lo = 0, hi = n_act - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -406,10 +475,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep_slots(const SlotDesc* slots, I
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
         const int64_t q = first + j * kBlock + threadIdx.x;
-        if (q < sd.m) eval_prop<D>(t, sd.store, q, chg, fail, nt);
+        if (q < sd.m) eval_prop<D, false>(t, sd.store, q, chg, fail, nt);
       }
     }
     publish(f, s, chg, fail, nt, tight);
+    if (threadIdx.x == 0) s_next = nxt;
+    __syncthreads();
+    item = first_item + s_next;
+    __syncthreads();
   }
 }
 
@@ -464,9 +537,10 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
     if (slot >= n_inst) return;            // same value in every wave: uniform exit
     const int32_t k = order[slot];
     InstFlags* f = flags + k;
-    const int64_t beg = sub.off ? sub.off[k] : 0;
-    const int64_t m = sub.off ? sub.off[k + 1] - beg : n_props;
-    const int32_t* idx = sub.off ? sub.idx + beg : nullptr;
+    const InstTable it = inst_table(t, sub, k, n_props);
+    const Table& tk = it.t;
+    const int32_t* idx = it.idx;
+    const int64_t m = it.m;
     int2* st = store + (int64_t)k * nvars;
     const bool start_failed = f->failed_at != kNoFail;
     int32_t last_changed = -1, failed_at = kNoFail, s = 0;
@@ -483,11 +557,11 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
           const int64_t q2 = q + kPBlock;
           const int64_t i1 = prop_at(idx, q);
           const int64_t i2 = q2 < m ? prop_at(idx, q2) : -1;
-          const uint32_t xo1 = t.xo[i1];
-          const int32_t y1 = t.y[i1], z1 = t.z[i1];
+          const uint32_t xo1 = tk.xo[i1];
+          const int32_t y1 = tk.y[i1], z1 = tk.z[i1];
           uint32_t xo2 = 0;
           int32_t y2 = 0, z2 = 0;
-          if (i2 >= 0) { xo2 = t.xo[i2]; y2 = t.y[i2]; z2 = t.z[i2]; }
+          if (i2 >= 0) { xo2 = tk.xo[i2]; y2 = tk.y[i2]; z2 = tk.z[i2]; }
           const int32_t x1 = (int32_t)(xo1 & kVarMask), x2 = (int32_t)(xo2 & kVarMask);
           const u64 a1 = load_word<WG>(st + x1), b1 = load_word<WG>(st + y1), c1 = load_word<WG>(st + z1);
           u64 a2 = 0, b2 = 0, c2 = 0;
@@ -592,6 +666,27 @@ hipError_t launch_store_check_empty(const int2* store, int64_t nvars, int32_t n_
     k_check_empty<true><<<dim3((unsigned)grid), 256, 0, stream>>>(store, nvars, total, flags);
   else
     k_check_empty<false><<<dim3((unsigned)grid), 256, 0, stream>>>(store, nvars, total, flags);
+  return hipGetLastError();
+}
+
+// Per-instance copies of the subsets: out[i] = table[idx[i]] for all i.
+__global__ void k_compact(Table t, const int32_t* __restrict__ idx, int64_t n,
+                          uint32_t* __restrict__ xo, int32_t* __restrict__ y,
+                          int32_t* __restrict__ z) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t j = idx[i];
+    xo[i] = t.xo[j];
+    y[i] = t.y[j];
+    z[i] = t.z[j];
+  }
+}
+
+hipError_t launch_compact_table(const Table& t, const int32_t* idx, int64_t n, uint32_t* xo,
+                                int32_t* y, int32_t* z, hipStream_t stream) {
+  int64_t grid = (n + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  if (grid > 0) k_compact<<<dim3((unsigned)grid), 256, 0, stream>>>(t, idx, n, xo, y, z);
   return hipGetLastError();
 }
 

@@ -111,7 +111,7 @@ struct pcx_server {
 namespace {
 
 WorkList worklist(pcx_server* s) {
-  return WorkList{s->d_act, s->d_item_base, s->d_ctl, s->d_ctl + 1, s->d_ctl + 2};
+  return worklist_of(s->d_act, s->d_item_base, s->d_ctl);
 }
 
 pcx_status upload_slot_meta(pcx_server* s, int32_t k) {
@@ -268,7 +268,7 @@ pcx_status run_alone(pcx_server* s, const Request& rq, pcx_result* out) {
   for (;;) {
     for (int g = 0; g < 8; ++g, ++sw)
       PCX_HIP(launch_sweep_plain(t, s->d_arena, nullptr, m, sw, s->d_solo, s->d_tight, ppw,
-                                 pcx_internal::dom_of(e), s->stream));
+                                 pcx_internal::dom_of(e), false, s->stream));
     PCX_HIP(hipMemcpyAsync(h, s->d_solo, sizeof(InstFlags), hipMemcpyDeviceToHost, s->stream));
     PCX_HIP(hipStreamSynchronize(s->stream));
     const int32_t last = std::max(h->changed[0], h->changed[1]);
@@ -344,7 +344,7 @@ pcx_status pcx_server_create(const pcx_server_config* cfg, pcx_server** out) {
   chk(hipMalloc(&s->d_solo, sizeof(InstFlags)));
   chk(hipMalloc(&s->d_act, c.max_slots * sizeof(int32_t)));
   chk(hipMalloc(&s->d_item_base, (c.max_slots + 1) * sizeof(int64_t)));
-  chk(hipMalloc(&s->d_ctl, 4 * sizeof(int32_t)));
+  chk(hipMalloc(&s->d_ctl, kCtlWords * sizeof(int32_t)));
   chk(hipMalloc(&s->d_tight, kTightSlots * sizeof(unsigned long long)));
   chk(hipMalloc(&s->d_arena, words * sizeof(int2)));
   chk(hipHostMalloc(&s->h_slots, c.max_slots * sizeof(SlotDesc), hipHostMallocDefault));
@@ -365,7 +365,8 @@ pcx_status pcx_server_create(const pcx_server_config* cfg, pcx_server** out) {
   he = hipMemcpy(s->d_slots, s->h_slots, c.max_slots * sizeof(SlotDesc), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(s->d_flags, s->h_flags, c.max_slots * sizeof(InstFlags), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemset(s->d_tight, 0, kTightSlots * sizeof(unsigned long long));
-  const int32_t ctl[4] = {0, -1, 0, 0};          // n_act, sweep, live, unused
+  int32_t ctl[kCtlWords] = {0};
+  ctl[1] = -1;                                   // no sweep run yet
   if (he == hipSuccess) he = hipMemcpy(s->d_ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice);
   if (he != hipSuccess) {
     pcx_server_destroy(s);

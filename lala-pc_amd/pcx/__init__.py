@@ -48,7 +48,8 @@ class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("props_per_wave", ctypes.c_int32),
                 ("lds_mode", ctypes.c_int32), ("sweeps_per_check", ctypes.c_int32),
                 ("max_sweeps", ctypes.c_int64), ("timing", ctypes.c_int32),
-                ("domain", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+                ("domain", ctypes.c_int32), ("compact_tables", ctypes.c_int32),
+                ("stream_tables", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
 class Result(ctypes.Structure):

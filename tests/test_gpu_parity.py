@@ -169,12 +169,15 @@ def test_full_size_network(gpu):
     assert ((lbg <= w) & (w <= ubg)).all()
 
 
+@pytest.mark.parametrize("tables", [(0, 0), (0, 1), (1, -1), (1, 0)],
+                         ids=["indexed", "indexed-nt", "copies-nt", "copies-cached"])
 @pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PERSISTENT, pcx.MODE_PLAIN])
-def test_batch_of_search_nodes(gpu, mode):
-    """Shared table, per-instance subsets and splits (scaled-down batch)."""
+def test_batch_of_search_nodes(gpu, mode, tables):
+    """Shared table, per-instance subsets and splits (scaled-down batch), with
+    the subsets read through indices or from per-instance copies."""
     V, P, n_inst = 5000, 20000, 96
     op, x, y, z, lb, ub, w = _network(11, V, P)
-    pir = pcx.PIR()
+    pir = pcx.PIR(compact_tables=tables[0], stream_tables=tables[1])
     pir.tell(op, x, y, z, lb, ub)
     root_lb, root_ub = lb.copy(), ub.copy()
     assert not pir.fixpoint(root_lb, root_ub).failed
