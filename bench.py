@@ -89,7 +89,7 @@ def bench_batched(args):
     p = pcx.gen_params(seed=args.seed, nvars=V, nprops=P)
     op, x, y, z, lb, ub, w = pcx.gen_network(p)
     pir = pcx.PIR(props_per_wave=args.ppw, timing=1, compact_tables=args.compact,
-                  stream_tables=args.stream)
+                  stream_tables=args.stream, event_driven=args.events)
     pir.tell(op, x, y, z, lb, ub)
     root_lb, root_ub = lb.copy(), ub.copy()
     rr = pir.fixpoint(root_lb, root_ub)
@@ -142,7 +142,8 @@ def bench_batched(args):
                                "50,000-var / 200,000-propagator network, 1-20 splits each, "
                                "2,000-200,000 active propagators each",
                    "nvars": V, "nprops": P, "instances": n_inst, "props_per_wave": args.ppw,
-                   "mode": args.mode, "compact_tables": args.compact, "stream_tables": args.stream},
+                   "mode": args.mode, "compact_tables": args.compact, "stream_tables": args.stream,
+                   "event_driven": args.events},
         "plain_fixpoints_per_s": plain_fps,
         "batched_vs_plain": fps / plain_fps if plain_fps else None,
         "evals_per_s": evals / elapsed,
@@ -199,7 +200,8 @@ def bench_single(args):
     V, P = 1000000, 4000000
     p = pcx.gen_params(seed=args.seed, nvars=V, nprops=P)
     op, x, y, z, lb, ub, w = pcx.gen_network(p)
-    pir = pcx.PIR(props_per_wave=args.ppw, timing=1, stream_tables=args.stream)
+    pir = pcx.PIR(props_per_wave=args.ppw, timing=1, stream_tables=args.stream,
+                  event_driven=args.events)
     pir.tell(op, x, y, z, lb, ub)
     batch = pcx.Batch(pir, 1, V)
     batch.load(lb, ub)
@@ -246,7 +248,8 @@ def bench_bits(args):
     a batch of 1,024 bit-removal subproblems (one step = the batch)."""
     V, P, n_inst = 200000, 800000, 1024
     op, x, y, z, w = pcx.gen_bits_network(args.seed, V, P)
-    pir = pcx.BitsPIR(props_per_wave=args.ppw, timing=1, stream_tables=args.stream)
+    pir = pcx.BitsPIR(props_per_wave=args.ppw, timing=1, stream_tables=args.stream,
+                      event_driven=args.events)
     pir.tell(op, x, y, z, w)
     root = w.copy()
     t0 = time.perf_counter()
@@ -431,6 +434,8 @@ def main():
                     help="per-instance subset tables: -1 auto, 0 shared table via indices, 1 copies")
     ap.add_argument("--stream", type=int, default=-1, choices=[-1, 0, 1],
                     help="table loads: -1 auto, 0 cached, 1 non-temporal")
+    ap.add_argument("--events", type=int, default=-1, choices=[-1, 0, 1],
+                    help="sweeps after the first: -1 auto, 0 every propagator, 1 marked only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -82,7 +82,10 @@ typedef struct {
                                own copy of its subset (12 B per active propagator)  */
   int32_t stream_tables;    /* -1 auto (stream per-instance tables only), 0 cached
                                table loads, 1 non-temporal table loads               */
-  int32_t reserved[4];
+  int32_t event_driven;     /* -1 auto, 0 every sweep evaluates every propagator,
+                               1 sweeps after the first evaluate only propagators
+                               with an operand tightened in the previous sweep       */
+  int32_t reserved[3];
 } pcx_config;
 
 typedef struct {

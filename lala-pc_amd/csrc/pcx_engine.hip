@@ -145,7 +145,10 @@ pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* resu
     r.has_changed = last >= 0;
     r.sweeps = r.failed ? (int64_t)f.failed_at + 1 : (int64_t)last + 2;
     if (e->cfg.max_sweeps > 0 && r.sweeps > e->cfg.max_sweeps) r.sweeps = e->cfg.max_sweeps;
-    r.evals = r.sweeps * inst_size(b, k);
+    if (b->events)       // one full sweep (unless empty at the start) + the marked evaluations
+      r.evals = (f.failed_at == -1 ? 0 : inst_size(b, k)) + f.marked_evals;
+    else
+      r.evals = r.sweeps * inst_size(b, k);
     evals += r.evals;
     smax = std::max(smax, r.sweeps);
     if (results) results[k] = r;
@@ -204,6 +207,70 @@ pcx_status prepare_tables(pcx_batch* b) {
   return PCX_OK;
 }
 
+// Variable -> table positions adjacency of the engine's table (each
+// propagator listed once per distinct variable), built on the host when the
+// table changed and uploaded.
+pcx_status ensure_adjacency(pcx_engine* e) {
+  if (e->d_adj_off && e->adj_version == e->version) return PCX_OK;
+  const int64_t n = (int64_t)e->op.size();
+  const int64_t V = (int64_t)e->max_var + 1;
+  std::vector<int32_t> off(V + 1, 0);
+  auto each = [&](int64_t i, auto&& f) {
+    const int32_t x = e->x[i], y = e->y[i], z = e->z[i];
+    f(x);
+    if (y != x) f(y);
+    if (z != x && z != y) f(z);
+  };
+  for (int64_t i = 0; i < n; ++i) each(i, [&](int32_t v) { ++off[v + 1]; });
+  for (int64_t v = 0; v < V; ++v) off[v + 1] += off[v];
+  std::vector<int32_t> adj(std::max<int32_t>(off[V], 1)), pos(off.begin(), off.end() - 1);
+  for (int64_t i = 0; i < n; ++i) each(i, [&](int32_t v) { adj[pos[v]++] = (int32_t)i; });
+  dfree(e->d_adj_off); dfree(e->d_adj);
+  PCX_HIP(hipMalloc(&e->d_adj_off, (V + 1) * sizeof(int32_t)));
+  PCX_HIP(hipMalloc(&e->d_adj, adj.size() * sizeof(int32_t)));
+  PCX_HIP(hipMemcpy(e->d_adj_off, off.data(), (V + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  PCX_HIP(hipMemcpy(e->d_adj, adj.data(), adj.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  e->adj_version = e->version;
+  return PCX_OK;
+}
+
+int64_t bitmap_words(const pcx_batch* b) { return (b->table_n + 31) / 32; }
+
+// Event-driven sweeps need the adjacency, the member bitmaps of subset
+// batches and two dirty bitmaps per instance; automatic mode uses them when
+// they take at most a quarter of free device memory.  The persistent mode
+// always evaluates every propagator.
+pcx_status prepare_events(pcx_batch* b, int32_t mode) {
+  pcx_engine* e = b->e;
+  b->events = false;
+  if (mode == PCX_MODE_PERSISTENT || e->cfg.event_driven == 0 || b->table_n == 0) return PCX_OK;
+  if (3 * b->table_n >= ((int64_t)1 << 31)) return PCX_OK;    // adjacency indices are int32
+  const int64_t words = bitmap_words(b);
+  const int64_t rows = b->n_inst * words;
+  const uint64_t need = (uint64_t)rows * 4 * (b->h_off.empty() ? 2 : 3);
+  if (e->cfg.event_driven < 0 && !b->d_dirty) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > free_b / 4) return PCX_OK;
+  }
+  pcx_status st = ensure_adjacency(e);
+  if (st != PCX_OK) return st;
+  if (!b->d_dirty) PCX_HIP(hipMalloc(&b->d_dirty, 2 * rows * sizeof(uint32_t)));
+  if (!b->h_off.empty() && (!b->d_member || b->member_version != e->version)) {
+    if (!b->d_member) PCX_HIP(hipMalloc(&b->d_member, rows * sizeof(uint32_t)));
+    PCX_HIP(hipMemsetAsync(b->d_member, 0, rows * sizeof(uint32_t), e->stream));
+    PCX_HIP(launch_build_members(b->d_off, b->d_idx, b->n_inst, b->d_member, words, e->stream));
+    b->member_version = e->version;
+  }
+  PCX_HIP(hipMemsetAsync(b->d_dirty, 0, 2 * rows * sizeof(uint32_t), e->stream));
+  b->events = true;
+  return PCX_OK;
+}
+
+EventSet event_set(const pcx_batch* b) {
+  return EventSet{b->e->d_adj_off, b->e->d_adj, b->h_off.empty() ? nullptr : b->d_member,
+                  b->d_dirty, bitmap_words(b), b->n_inst};
+}
+
 // Table and subsets a sweep kernel reads for this batch.
 Table run_table(const pcx_batch* b) {
   if (b->compact) return Table{b->d_cxo, b->d_cy, b->d_cz, b->h_off.back()};
@@ -214,7 +281,7 @@ Subsets run_subsets(const pcx_batch* b) {
   return Subsets{b->d_off, b->compact ? nullptr : b->d_idx};
 }
 
-pcx_status prepare_run(pcx_batch* b) {
+pcx_status prepare_run(pcx_batch* b, int32_t mode) {
   pcx_engine* e = b->e;
   if ((int64_t)e->op.size() != b->table_n)
     return fail_with(PCX_EINVAL, "table changed since the batch was created");
@@ -226,7 +293,9 @@ pcx_status prepare_run(pcx_batch* b) {
   const WorkList wl = worklist_of(b->d_act, b->d_item_base, b->d_ctl);
   PCX_HIP(launch_init_flags(b->d_flags, (int32_t)b->n_inst, wl, e->stream));
   PCX_HIP(launch_store_check_empty(b->d_store, b->nvars, (int32_t)b->n_inst, b->d_flags, dom_of(e), e->stream));
-  return prepare_tables(b);
+  pcx_status st = prepare_tables(b);
+  if (st != PCX_OK) return st;
+  return prepare_events(b, mode);
 }
 
 // Plain mode: every instance on its own, one launch per sweep, host checks
@@ -235,6 +304,7 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
   const int group = e->cfg.sweeps_per_check > 0 ? e->cfg.sweeps_per_check : 8;
   const Table t = table_of(e);
+  const EventSet ev = event_set(b);
   int64_t launches = 0;
   double kms = 0.0;
   size_t nev = 0;
@@ -256,7 +326,11 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       for (int g = 0; g < group; ++g, ++s) {
         if (e->cfg.max_sweeps > 0 && s >= e->cfg.max_sweeps) break;
         if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
-        PCX_HIP(launch_sweep_plain(tk, st, idx, m, s, f, b->d_tight, ppw, dom_of(e), b->stream, e->stream));
+        if (b->events && s > 0)
+          PCX_HIP(launch_sweep_marked_plain(t, st, s, f, b->d_tight, ev, k, dom_of(e), e->stream));
+        else
+          PCX_HIP(launch_sweep_plain(tk, st, idx, m, s, f, b->d_tight, ppw, dom_of(e), b->stream,
+                                     e->stream, b->events ? &ev : nullptr, k));
         if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
         ++launches;
       }
@@ -292,12 +366,21 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   size_t nev = 0;
   int32_t live = 1, sweeps = 0;
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
+  const EventSet ev = event_set(b);
+  const int64_t marked_items = (bitmap_words(b) + kMarkedWordsPerItem - 1) / kMarkedWordsPerItem;
   while (live > 0) {
     for (int g = 0; g < group; ++g) {
-      PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream));
+      const bool marked = b->events && sweeps > 0;
+      PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
+                              marked ? marked_items : 0));
       if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
-      PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
-                                   b->d_tight, ppw, dom_of(e), b->stream, grid, e->stream));
+      if (marked)
+        PCX_HIP(launch_sweep_marked(table_of(e), b->d_store, b->nvars, b->d_flags, wl, b->d_tight, ev,
+                                    dom_of(e), grid, e->stream));
+      else
+        PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
+                                     b->d_tight, ppw, dom_of(e), b->stream, grid, e->stream,
+                                     b->events ? &ev : nullptr));
       if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
       ++launches;
       ++sweeps;
@@ -361,6 +444,7 @@ void pcx_config_default(pcx_config* c) {
   c->timing = 0;
   c->compact_tables = -1;
   c->stream_tables = -1;
+  c->event_driven = -1;
 }
 
 int32_t pcx_device_count(void) {
@@ -398,7 +482,7 @@ void pcx_destroy(pcx_engine* e) {
   if (!e) return;
   (void)hipSetDevice(e->cfg.device);
   if (e->scratch) pcx_batch_destroy(e->scratch);
-  dfree(e->d_xo); dfree(e->d_y); dfree(e->d_z);
+  dfree(e->d_xo); dfree(e->d_y); dfree(e->d_z); dfree(e->d_adj_off); dfree(e->d_adj);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
@@ -561,7 +645,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   for (hipEvent_t ev : b->ev) (void)hipEventDestroy(ev);
   dfree(b->d_store); dfree(b->d_image); dfree(b->d_off); dfree(b->d_idx); dfree(b->d_flags);
   dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl); dfree(b->d_order);
-  dfree(b->d_cxo); dfree(b->d_cy); dfree(b->d_cz);
+  dfree(b->d_cxo); dfree(b->d_cy); dfree(b->d_cz); dfree(b->d_member); dfree(b->d_dirty);
   delete b;
 }
 
@@ -613,7 +697,7 @@ pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results, pcx_st
   pcx_engine* e = b->e;
   PCX_HIP(hipSetDevice(e->cfg.device));
   const double t0 = now_ms();
-  pcx_status s = prepare_run(b);
+  pcx_status s = prepare_run(b, mode);
   if (s != PCX_OK) return s;
   switch (mode) {
     case PCX_MODE_PLAIN: s = run_plain(b, results, stats); break;

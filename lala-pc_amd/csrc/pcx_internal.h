@@ -15,6 +15,11 @@ struct pcx_engine {
   bool has_div = false;               // table holds a division propagator
   bool dirty = true;
   uint64_t version = 0;               // bumped on every upload of the table
+  // variable -> table positions adjacency (event-driven sweeps), built on demand
+  int32_t* d_adj_off = nullptr;
+  int32_t* d_adj = nullptr;
+  uint64_t adj_version = 0;
+  int64_t adj_vars = 0;
   uint32_t* d_xo = nullptr;
   int32_t* d_y = nullptr;
   int32_t* d_z = nullptr;
@@ -42,6 +47,11 @@ struct pcx_batch {
   uint64_t ctab_version = 0;          // engine table version they were built from
   bool compact = false;               // this run reads the per-instance copies
   bool stream = false;                // this run streams table loads
+  // event-driven sweeps: member bitmaps (subsets only) and two dirty bitmaps
+  uint32_t* d_member = nullptr;
+  uint32_t* d_dirty = nullptr;
+  uint64_t member_version = 0;
+  bool events = false;                // this run uses marked-only sweeps
   pcx::InstFlags* d_flags = nullptr;
   unsigned long long* d_tight = nullptr;
   int32_t* d_act = nullptr;

@@ -42,6 +42,23 @@ struct InstFlags {
   int32_t changed[2];   // changed[s & 1] = s when sweep s tightened a bound
   int32_t failed_at;    // first sweep that found an empty interval, or kNoFail
   int32_t done_at;      // sweep at which the scheduler retired the instance
+  int64_t marked_evals; // evaluations done by marked-only sweeps
+};
+
+// Event-driven sweeps.  After an instance's first (full) sweep, a sweep
+// evaluates only the propagators marked in the instance's bitmap for it: a
+// tightening of variable v marks every propagator adjacent to v (adj_off /
+// adj, table positions) that belongs to the instance (member bitmap, or all
+// when member == nullptr) in the bitmap of the next sweep.  Bitmaps are
+// indexed by table position, `words` 32-bit words per instance, two per
+// instance (sweep parity): dirty[(s & 1) * n_inst * words + k * words ...].
+struct EventSet {
+  const int32_t* adj_off;   // [nvars + 1]
+  const int32_t* adj;       // [adj_off[nvars]]
+  const uint32_t* member;   // [n_inst * words] or nullptr
+  uint32_t* dirty;          // [2 * n_inst * words], or nullptr = no marking
+  int64_t words;
+  int64_t n_inst;
 };
 
 // Per-instance propagator subset: prop_idx[off[k] .. off[k+1]) of the shared
@@ -86,13 +103,27 @@ struct SlotDesc {
 namespace pcx {
 hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, int64_t m,
                               int32_t sweep, InstFlags* flags, unsigned long long* tight,
-                              int ppw, int dom, bool stream_table, hipStream_t st);
+                              int ppw, int dom, bool stream_table, hipStream_t st,
+                              const EventSet* ev = nullptr, int64_t inst = 0);
+// Marked-only sweep of one instance (plain mode), sweep >= 1.
+hipError_t launch_sweep_marked_plain(const Table& t, int2* store, int32_t sweep, InstFlags* flags,
+                                     unsigned long long* tight, const EventSet& ev, int64_t inst,
+                                     int dom, hipStream_t st);
+// Marked-only sweep over every running instance (batched mode).
+hipError_t launch_sweep_marked(const Table& t, int2* store, int64_t nvars, InstFlags* flags,
+                               WorkList wl, unsigned long long* tight, const EventSet& ev,
+                               int dom, int grid, hipStream_t st);
+// member bitmaps from subsets
+hipError_t launch_build_members(const int64_t* off, const int32_t* idx, int64_t n_inst,
+                                uint32_t* member, int64_t words, hipStream_t st);
 hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags, int32_t n_inst,
-                           WorkList wl, int ppw, int32_t max_sweeps, hipStream_t st);
+                           WorkList wl, int ppw, int32_t max_sweeps, hipStream_t st,
+                           int64_t fixed_items = 0);
 hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                                 int64_t n_props, InstFlags* flags, WorkList wl,
                                 unsigned long long* tight, int ppw, int dom, bool stream_table,
-                                int grid, hipStream_t st);
+                                int grid, hipStream_t st, const EventSet* ev = nullptr);
+constexpr int kMarkedWordsPerItem = 256;   // bitmap words per marked-sweep item
 hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                              int64_t n_props, InstFlags* flags, const int32_t* order,
                              int32_t n_inst, int32_t* queue, unsigned long long* tight,

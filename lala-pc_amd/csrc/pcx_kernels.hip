@@ -45,54 +45,143 @@ __device__ __forceinline__ int2 as_itv(u64 v) {
 }
 
 // Meet interval `n` into variable v, whose value was read as `cur`.
+// Returns true when this call tightened a bound of v.
 template <int SCOPE>
-__device__ __forceinline__ void meet_itv(int2* __restrict__ st, int32_t v, int2 cur, int2 n,
+__device__ __forceinline__ bool meet_itv(int2* __restrict__ st, int32_t v, int2 cur, int2 n,
                                          bool& chg, bool& fail, int& nt) {
+  bool t = false;
   if (n.x > cur.x) {
     const int old = __hip_atomic_fetch_max(&st[v].x, n.x, __ATOMIC_RELAXED, SCOPE);
-    if (old < n.x) { chg = true; ++nt; }
+    if (old < n.x) { t = true; ++nt; }
     if (n.x > cur.y) fail = true;          // cur.y >= current ub
   }
   if (n.y < cur.y) {
     const int old = __hip_atomic_fetch_min(&st[v].y, n.y, __ATOMIC_RELAXED, SCOPE);
-    if (old > n.y) { chg = true; ++nt; }
+    if (old > n.y) { t = true; ++nt; }
     if (n.y < cur.x) fail = true;
   }
+  chg |= t;
+  return t;
 }
 
 // AND mask `n` into word v, whose value was read as `cur`.
 template <int SCOPE>
-__device__ __forceinline__ void meet_word(u64* __restrict__ st, int32_t v, u64 cur, u64 n,
+__device__ __forceinline__ bool meet_word(u64* __restrict__ st, int32_t v, u64 cur, u64 n,
                                           bool& chg, bool& fail, int& nt) {
+  bool t = false;
   if ((cur & n) != cur) {
     const u64 old = __hip_atomic_fetch_and(&st[v], n, __ATOMIC_RELAXED, SCOPE);
-    if ((old & n) != old) { chg = true; ++nt; }
+    if ((old & n) != old) { t = true; ++nt; }
     if ((old & n) == 0) fail = true;       // the word is now empty
   }
+  chg |= t;
+  return t;
+}
+
+// Where a tightening of variable v is recorded for the next sweep of one
+// instance (dirty == nullptr: nowhere, every sweep is a full sweep).  In a
+// marked sweep the current sweep's bitmap (also_now) is marked as well: a
+// propagator whose word this sweep has not reached yet then runs in this
+// sweep, as it would in a full sweep; a bit that lands behind the sweep is
+// an extra evaluation two sweeps later, which never changes the fixpoint.
+struct Marker {
+  const int32_t* adj_off;
+  const int32_t* adj;
+  const uint32_t* member;   // instance's row, nullptr = every propagator
+  uint32_t* dirty;          // instance's row of the next sweep
+  uint32_t* also_now;       // instance's row of this sweep (marked sweeps), or nullptr
+  int32_t* list;            // workgroup's deferred variables (LDS), or nullptr
+  int32_t* list_n;
+};
+
+// Deferred marking: a tightened variable is pushed onto the workgroup's LDS
+// list and the whole workgroup marks the list's adjacency together at the
+// end of its work item (flush_marks), instead of each thread walking the
+// adjacency of every variable it tightened one after the other.
+constexpr int kMarkCap = 1024;
+
+__device__ __forceinline__ Marker marker_for(const EventSet* ev, int64_t k, int32_t s,
+                                             bool marked_sweep = false) {
+  if (!ev || !ev->dirty) return Marker{nullptr, nullptr, nullptr, nullptr, nullptr};
+  const int64_t row = k * ev->words;
+  const int64_t plane = ev->n_inst * ev->words;
+  return Marker{ev->adj_off, ev->adj, ev->member ? ev->member + row : nullptr,
+                ev->dirty + (int64_t)((s + 1) & 1) * plane + row,
+                marked_sweep ? ev->dirty + (int64_t)(s & 1) * plane + row : nullptr, nullptr,
+                nullptr};
+}
+
+// Mark the propagators adjacent to v: adjacency entries are loaded eight at
+// a time (independent loads in flight together), then marked with
+// non-returning atomics.
+__device__ __noinline__ void mark_var(const Marker mk, int32_t v) {
+  constexpr int kB = 8;
+  const int32_t e = mk.adj_off[v + 1];
+  for (int32_t i = mk.adj_off[v]; i < e; i += kB) {
+    int32_t p[kB];
+#pragma unroll
+    for (int j = 0; j < kB; ++j) p[j] = i + j < e ? mk.adj[i + j] : -1;
+    uint32_t in[kB];
+#pragma unroll
+    for (int j = 0; j < kB; ++j)
+      in[j] = (p[j] >= 0 && mk.member) ? mk.member[p[j] >> 5] : ~0u;
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      if (p[j] < 0) continue;
+      const uint32_t bit = 1u << (p[j] & 31);
+      if (!(in[j] & bit)) continue;
+      atomicOr(&mk.dirty[p[j] >> 5], bit);
+      if (mk.also_now) atomicOr(&mk.also_now[p[j] >> 5], bit);
+    }
+  }
+}
+
+__device__ __forceinline__ void defer_mark(const Marker& mk, int32_t v) {
+  if (mk.list) {
+    const int i = atomicAdd(mk.list_n, 1);
+    if (i < kMarkCap) { mk.list[i] = v; return; }
+  }
+  mark_var(mk, v);                          // list full: mark directly
+}
+
+// Every thread of the workgroup calls this at the end of a work item.
+__device__ __forceinline__ void flush_marks(const Marker& mk) {
+  if (!mk.list) return;
+  __syncthreads();
+  const int n = min(*mk.list_n, kMarkCap);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) mark_var(mk, mk.list[i]);
+  __syncthreads();
+  if (threadIdx.x == 0) *mk.list_n = 0;     // ordered before reuse by the caller's barrier
 }
 
 // Apply one propagator to operands already loaded.
 template <int D, int SCOPE>
 __device__ __forceinline__ void apply(int2* __restrict__ st, int op, int32_t x, int32_t y,
                                       int32_t z, u64 wx, u64 wy, u64 wz, bool& chg, bool& fail,
-                                      int& nt) {
+                                      int& nt, const Marker& mk) {
+  bool tx, ty, tz;
   if constexpr (D == kBits) {
     u64* ws = reinterpret_cast<u64*>(st);
     if (!wx || !wy || !wz) { fail = true; return; }
     u64 nx, ny, nz;
     deduce_bits(op, wx, wy, wz, nx, ny, nz);
-    meet_word<SCOPE>(ws, x, wx, nx, chg, fail, nt);
-    meet_word<SCOPE>(ws, y, wy, ny, chg, fail, nt);
-    meet_word<SCOPE>(ws, z, wz, nz, chg, fail, nt);
+    tx = meet_word<SCOPE>(ws, x, wx, nx, chg, fail, nt);
+    ty = meet_word<SCOPE>(ws, y, wy, ny, chg, fail, nt);
+    tz = meet_word<SCOPE>(ws, z, wz, nz, chg, fail, nt);
   } else {
     const int2 X = as_itv(wx), Y = as_itv(wy), Z = as_itv(wz);
     if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = true; return; }
     int2 nx, ny, nz;
     deduce<D == kItvDiv>(op, X, Y, Z, nx, ny, nz);
     if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
-    meet_itv<SCOPE>(st, x, X, nx, chg, fail, nt);
-    meet_itv<SCOPE>(st, y, Y, ny, chg, fail, nt);
-    meet_itv<SCOPE>(st, z, Z, nz, chg, fail, nt);
+    tx = meet_itv<SCOPE>(st, x, X, nx, chg, fail, nt);
+    ty = meet_itv<SCOPE>(st, y, Y, ny, chg, fail, nt);
+    tz = meet_itv<SCOPE>(st, z, Z, nz, chg, fail, nt);
+  }
+  if (mk.dirty) {
+    if (tx) defer_mark(mk, x);
+    if (ty) defer_mark(mk, y);
+    if (tz) defer_mark(mk, z);
   }
 }
 
@@ -107,14 +196,15 @@ __device__ __forceinline__ T table_load(const T* p) {
 
 template <int D, bool NT>
 __device__ __forceinline__ void eval_prop(const Table& t, int2* __restrict__ st, int64_t i,
-                                          bool& chg, bool& fail, int& nt) {
+                                          bool& chg, bool& fail, int& nt, const Marker& mk) {
   const uint32_t xo = table_load<NT>(t.xo + i);
   const int32_t y = table_load<NT>(t.y + i), z = table_load<NT>(t.z + i);
   const int32_t x = (int32_t)(xo & kVarMask);
   const u64 wx = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + x);
   const u64 wy = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + y);
   const u64 wz = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + z);
-  apply<D, __HIP_MEMORY_SCOPE_AGENT>(st, (int)(xo >> kOpShift), x, y, z, wx, wy, wz, chg, fail, nt);
+  apply<D, __HIP_MEMORY_SCOPE_AGENT>(st, (int)(xo >> kOpShift), x, y, z, wx, wy, wz, chg, fail, nt,
+                                     mk);
 }
 
 // Publish a wave's verdict: one store per flag, one atomic for the count.
@@ -152,57 +242,143 @@ template <int PPL, int D, bool NT>
 __global__ __launch_bounds__(kBlock) void k_sweep_plain(Table t, int2* __restrict__ st,
                                                         const int32_t* __restrict__ idx, int64_t m,
                                                         int32_t s, InstFlags* f,
-                                                        unsigned long long* tight) {
+                                                        unsigned long long* tight, EventSet ev,
+                                                        int64_t inst) {
   // Sweep s runs only if sweep s-1 tightened something and nothing failed.
   if (f->changed[(s - 1) & 1] != s - 1 || f->failed_at != kNoFail) return;
   const int64_t base = (int64_t)blockIdx.x * (kBlock * PPL);
+  __shared__ int32_t s_list[kMarkCap];
+  __shared__ int32_t s_n;
+  Marker mk = marker_for(&ev, inst, s);
+  if (mk.dirty) {
+    mk.list = s_list;
+    mk.list_n = &s_n;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+  }
   bool chg = false, fail = false;
   int nt = 0;
 #pragma unroll
   for (int j = 0; j < PPL; ++j) {
     const int64_t k = base + j * kBlock + threadIdx.x;
-    if (k < m) eval_prop<D, NT>(t, st, prop_at(idx, k), chg, fail, nt);
+    if (k < m) eval_prop<D, NT>(t, st, prop_at(idx, k), chg, fail, nt, mk);
   }
   publish(f, s, chg, fail, nt, tight);
+  flush_marks(mk);
 }
 
 template <int D, bool NT>
 hipError_t launch_plain_d(const Table& t, int2* store, const int32_t* idx, int64_t m,
                           int32_t sweep, InstFlags* flags, unsigned long long* tight, int ppl,
-                          hipStream_t stream) {
+                          const EventSet& ev, int64_t inst, hipStream_t stream) {
   const int64_t per_block = (int64_t)kBlock * ppl;
   const int64_t grid = (m + per_block - 1) / per_block;
   if (grid <= 0) return hipSuccess;
   const dim3 g((unsigned)grid), b(kBlock);
+#define PCX_PLAIN(P) \
+  k_sweep_plain<P, D, NT><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight, ev, inst)
   switch (ppl) {
-    case 1: k_sweep_plain<1, D, NT><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
-    case 2: k_sweep_plain<2, D, NT><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
-    case 4: k_sweep_plain<4, D, NT><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
-    case 8: k_sweep_plain<8, D, NT><<<g, b, 0, stream>>>(t, store, idx, m, sweep, flags, tight); break;
+    case 1: PCX_PLAIN(1); break;
+    case 2: PCX_PLAIN(2); break;
+    case 4: PCX_PLAIN(4); break;
+    case 8: PCX_PLAIN(8); break;
     default: return hipErrorInvalidValue;
   }
+#undef PCX_PLAIN
   return hipGetLastError();
 }
 
 template <bool NT>
 hipError_t launch_plain_nt(const Table& t, int2* store, const int32_t* idx, int64_t m,
                            int32_t sweep, InstFlags* flags, unsigned long long* tight, int ppl,
-                           int dom, hipStream_t stream) {
+                           int dom, const EventSet& ev, int64_t inst, hipStream_t stream) {
   switch (dom) {
-    case kItv: return launch_plain_d<kItv, NT>(t, store, idx, m, sweep, flags, tight, ppl, stream);
-    case kItvDiv: return launch_plain_d<kItvDiv, NT>(t, store, idx, m, sweep, flags, tight, ppl, stream);
-    case kBits: return launch_plain_d<kBits, NT>(t, store, idx, m, sweep, flags, tight, ppl, stream);
+    case kItv: return launch_plain_d<kItv, NT>(t, store, idx, m, sweep, flags, tight, ppl, ev, inst, stream);
+    case kItvDiv: return launch_plain_d<kItvDiv, NT>(t, store, idx, m, sweep, flags, tight, ppl, ev, inst, stream);
+    case kBits: return launch_plain_d<kBits, NT>(t, store, idx, m, sweep, flags, tight, ppl, ev, inst, stream);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, int64_t m,
                               int32_t sweep, InstFlags* flags, unsigned long long* tight,
-                              int ppw, int dom, bool stream_table, hipStream_t stream) {
+                              int ppw, int dom, bool stream_table, hipStream_t stream,
+                              const EventSet* ev, int64_t inst) {
   const int ppl = ppw / kWave;
+  const EventSet none{nullptr, nullptr, nullptr, nullptr, 0, 0};
+  const EventSet& e = ev ? *ev : none;
   return stream_table
-             ? launch_plain_nt<true>(t, store, idx, m, sweep, flags, tight, ppl, dom, stream)
-             : launch_plain_nt<false>(t, store, idx, m, sweep, flags, tight, ppl, dom, stream);
+             ? launch_plain_nt<true>(t, store, idx, m, sweep, flags, tight, ppl, dom, e, inst, stream)
+             : launch_plain_nt<false>(t, store, idx, m, sweep, flags, tight, ppl, dom, e, inst, stream);
+}
+
+// ---- marked-only sweeps ----------------------------------------------------
+// Thread t of an item owns bitmap word w: it reads the word of this sweep,
+// clears it (the bitmap is reused two sweeps later), and evaluates the
+// propagators whose bits are set, marking into the next sweep's bitmap.
+template <int D>
+__device__ __forceinline__ void sweep_marked_word(const Table& t, int2* __restrict__ st,
+                                                  uint32_t* cur_row, int64_t w, const Marker& mk,
+                                                  bool& chg, bool& fail, int& nt, int& ne) {
+  uint32_t bits = cur_row[w];
+  if (!bits) return;
+  cur_row[w] = 0;
+  while (bits) {
+    const int b = __builtin_ctz(bits);
+    bits &= bits - 1;
+    const int64_t p = w * 32 + b;
+    if (p < t.n) {
+      eval_prop<D, false>(t, st, p, chg, fail, nt, mk);
+      ++ne;
+    }
+  }
+}
+
+__device__ __forceinline__ void count_evals(InstFlags* f, int ne) {
+  const int sum = wave_sum(ne);
+  if (lane_id() == 0 && sum) atomicAdd((unsigned long long*)&f->marked_evals, (unsigned long long)sum);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sweep_marked_plain(Table t, int2* __restrict__ st,
+                                                               int32_t s, InstFlags* f,
+                                                               unsigned long long* tight,
+                                                               EventSet ev, int64_t inst) {
+  if (f->changed[(s - 1) & 1] != s - 1 || f->failed_at != kNoFail) return;
+  uint32_t* cur = ev.dirty + (int64_t)(s & 1) * ev.n_inst * ev.words + inst * ev.words;
+  __shared__ int32_t s_list[kMarkCap];
+  __shared__ int32_t s_n;
+  Marker mk = marker_for(&ev, inst, s, true);
+  mk.list = s_list;
+  mk.list_n = &s_n;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  bool chg = false, fail = false;
+  int nt = 0, ne = 0;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < ev.words;
+       base += (int64_t)gridDim.x * kBlock) {      // same trip count in every thread
+    const int64_t w = base + threadIdx.x;
+    if (w < ev.words) sweep_marked_word<D>(t, st, cur, w, mk, chg, fail, nt, ne);
+    flush_marks(mk);
+    __syncthreads();
+  }
+  publish(f, s, chg, fail, nt, tight);
+  count_evals(f, ne);
+}
+
+hipError_t launch_sweep_marked_plain(const Table& t, int2* store, int32_t sweep, InstFlags* flags,
+                                     unsigned long long* tight, const EventSet& ev, int64_t inst,
+                                     int dom, hipStream_t stream) {
+  int64_t grid = (ev.words + kBlock - 1) / kBlock;
+  if (grid > 2048) grid = 2048;
+  if (grid <= 0) return hipSuccess;
+  switch (dom) {
+    case kItv: k_sweep_marked_plain<kItv><<<(unsigned)grid, kBlock, 0, stream>>>(t, store, sweep, flags, tight, ev, inst); break;
+    case kItvDiv: k_sweep_marked_plain<kItvDiv><<<(unsigned)grid, kBlock, 0, stream>>>(t, store, sweep, flags, tight, ev, inst); break;
+    case kBits: k_sweep_marked_plain<kBits><<<(unsigned)grid, kBlock, 0, stream>>>(t, store, sweep, flags, tight, ev, inst); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 // ---- batched mode: device scheduler + grid-stride sweep over work items ---
@@ -214,7 +390,8 @@ constexpr int kSchedThreads = 1024;
 __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t n_props,
                                                             InstFlags* flags, int32_t n_inst,
                                                             WorkList wl, int64_t chunk,
-                                                            int32_t max_sweeps) {
+                                                            int32_t max_sweeps,
+                                                            int64_t fixed_items) {
   __shared__ int64_t part[kSchedThreads];
   __shared__ int32_t cnt[kSchedThreads];
   const int32_t s = *wl.sweep + 1;       // the sweep being scheduled
@@ -233,7 +410,7 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
       continue;
     }
     const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
-    items += (m + chunk - 1) / chunk;
+    items += fixed_items > 0 ? fixed_items : (m + chunk - 1) / chunk;
     ++na;
   }
   part[tid] = items;
@@ -256,7 +433,7 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
     const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
     wl.act[ia] = k;
     wl.item_base[ia] = ib;
-    ib += (m + chunk - 1) / chunk;
+    ib += fixed_items > 0 ? fixed_items : (m + chunk - 1) / chunk;
     ++ia;
   }
   __syncthreads();
@@ -269,9 +446,11 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
 }
 
 hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags, int32_t n_inst,
-                           WorkList wl, int ppw, int32_t max_sweeps, hipStream_t stream) {
+                           WorkList wl, int ppw, int32_t max_sweeps, hipStream_t stream,
+                           int64_t fixed_items) {
   const int64_t chunk = (int64_t)kBlock * (ppw / kWave);
-  k_schedule<<<1, kSchedThreads, 0, stream>>>(sub, n_props, flags, n_inst, wl, chunk, max_sweeps);
+  k_schedule<<<1, kSchedThreads, 0, stream>>>(sub, n_props, flags, n_inst, wl, chunk, max_sweeps,
+                                              fixed_items);
   return hipGetLastError();
 }
 
@@ -279,7 +458,8 @@ template <int PPL, int D, bool NT>
 __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restrict__ store,
                                                           int64_t nvars, Subsets sub,
                                                           int64_t n_props, InstFlags* flags,
-                                                          WorkList wl, unsigned long long* tight) {
+                                                          WorkList wl, unsigned long long* tight,
+                                                          EventSet ev) {
   const int32_t s = *wl.sweep;
   const int32_t n_act = *wl.n_act;
   if (n_act <= 0) return;
@@ -298,7 +478,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
   const int64_t end_item = total * (xcd + 1) / kXcds;
   int32_t* qc = wl.xq + xcd;
   __shared__ int32_t s_next;
-  if (threadIdx.x == 0) s_next = atomicAdd(qc, 1);
+  __shared__ int32_t s_list[kMarkCap];
+  __shared__ int32_t s_n;
+  if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; }
   __syncthreads();
   int64_t item = first_item + s_next;
   __syncthreads();
@@ -316,16 +498,19 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
     const int64_t first = (item - wl.item_base[lo]) * chunk;
     const InstTable it = inst_table(t, sub, k, n_props);
     int2* st = store + (int64_t)k * nvars;
+    Marker mk = marker_for(&ev, k, s);
+    if (mk.dirty) { mk.list = s_list; mk.list_n = &s_n; }
     bool chg = false, fail = false;
     int nt = 0;
     if (f->failed_at == kNoFail) {
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
         const int64_t q = first + j * kBlock + threadIdx.x;
-        if (q < it.m) eval_prop<D, NT>(it.t, st, prop_at(it.idx, q), chg, fail, nt);
+        if (q < it.m) eval_prop<D, NT>(it.t, st, prop_at(it.idx, q), chg, fail, nt, mk);
       }
     }
     publish(f, s, chg, fail, nt, tight);
+    flush_marks(mk);
     if (threadIdx.x == 0) s_next = nxt;
     __syncthreads();
     item = first_item + s_next;
@@ -336,9 +521,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
 template <int D, bool NT>
 hipError_t launch_batched_d(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                             int64_t n_props, InstFlags* flags, WorkList wl,
-                            unsigned long long* tight, int ppl, int grid, hipStream_t stream) {
+                            unsigned long long* tight, int ppl, int grid, const EventSet& ev,
+                            hipStream_t stream) {
 #define PCX_BATCHED(P) \
-  k_sweep_batched<P, D, NT><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight)
+  k_sweep_batched<P, D, NT><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight, ev)
   switch (ppl) {
     case 1: PCX_BATCHED(1); break;
     case 2: PCX_BATCHED(2); break;
@@ -354,11 +540,11 @@ template <bool NT>
 hipError_t launch_batched_nt(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                              int64_t n_props, InstFlags* flags, WorkList wl,
                              unsigned long long* tight, int ppl, int dom, int grid,
-                             hipStream_t stream) {
+                             const EventSet& ev, hipStream_t stream) {
   switch (dom) {
-    case kItv: return launch_batched_d<kItv, NT>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
-    case kItvDiv: return launch_batched_d<kItvDiv, NT>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
-    case kBits: return launch_batched_d<kBits, NT>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, stream);
+    case kItv: return launch_batched_d<kItv, NT>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, ev, stream);
+    case kItvDiv: return launch_batched_d<kItvDiv, NT>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, ev, stream);
+    case kBits: return launch_batched_d<kBits, NT>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, grid, ev, stream);
     default: return hipErrorInvalidValue;
   }
 }
@@ -366,11 +552,97 @@ hipError_t launch_batched_nt(const Table& t, int2* store, int64_t nvars, const S
 hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                                 int64_t n_props, InstFlags* flags, WorkList wl,
                                 unsigned long long* tight, int ppw, int dom, bool stream_table,
-                                int grid, hipStream_t stream) {
+                                int grid, hipStream_t stream, const EventSet* ev) {
   const int ppl = ppw / kWave;
+  const EventSet none{nullptr, nullptr, nullptr, nullptr, 0, 0};
+  const EventSet& e = ev ? *ev : none;
   return stream_table
-             ? launch_batched_nt<true>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, dom, grid, stream)
-             : launch_batched_nt<false>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, dom, grid, stream);
+             ? launch_batched_nt<true>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, dom, grid, e, stream)
+             : launch_batched_nt<false>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, dom, grid, e, stream);
+}
+
+// Marked-only sweep over every running instance: items are (instance,
+// kMarkedWordsPerItem bitmap words), claimed per XCD as in the full sweep.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restrict__ store,
+                                                         int64_t nvars, InstFlags* flags,
+                                                         WorkList wl, unsigned long long* tight,
+                                                         EventSet ev) {
+  const int32_t s = *wl.sweep;
+  const int32_t n_act = *wl.n_act;
+  if (n_act <= 0) return;
+  const int64_t total = wl.item_base[n_act];
+  const int xcd = blockIdx.x % kXcds;
+  const int64_t first_item = total * xcd / kXcds;
+  const int64_t end_item = total * (xcd + 1) / kXcds;
+  int32_t* qc = wl.xq + xcd;
+  uint32_t* cur = ev.dirty + (int64_t)(s & 1) * ev.n_inst * ev.words;
+  __shared__ int32_t s_next;
+  __shared__ int32_t s_list[kMarkCap];
+  __shared__ int32_t s_n;
+  if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; }
+  __syncthreads();
+  int64_t item = first_item + s_next;
+  __syncthreads();
+  while (item < end_item) {
+    int32_t nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(qc, 1);
+    int lo = 0, hi = n_act - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (wl.item_base[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int32_t k = wl.act[lo];
+    InstFlags* f = flags + k;
+    const int64_t w = (item - wl.item_base[lo]) * kMarkedWordsPerItem + threadIdx.x;
+    Marker mk = marker_for(&ev, k, s, true);
+    mk.list = s_list;
+    mk.list_n = &s_n;
+    bool chg = false, fail = false;
+    int nt = 0, ne = 0;
+    if (f->failed_at == kNoFail && w < ev.words)
+      sweep_marked_word<D>(t, store + (int64_t)k * nvars, cur + (int64_t)k * ev.words, w, mk, chg,
+                           fail, nt, ne);
+    publish(f, s, chg, fail, nt, tight);
+    count_evals(f, ne);
+    flush_marks(mk);
+    if (threadIdx.x == 0) s_next = nxt;
+    __syncthreads();
+    item = first_item + s_next;
+    __syncthreads();
+  }
+}
+
+hipError_t launch_sweep_marked(const Table& t, int2* store, int64_t nvars, InstFlags* flags,
+                               WorkList wl, unsigned long long* tight, const EventSet& ev,
+                               int dom, int grid, hipStream_t stream) {
+  static_assert(kMarkedWordsPerItem == kBlock, "one bitmap word per thread");
+  switch (dom) {
+    case kItv: k_sweep_marked<kItv><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, wl, tight, ev); break;
+    case kItvDiv: k_sweep_marked<kItvDiv><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, wl, tight, ev); break;
+    case kBits: k_sweep_marked<kBits><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, wl, tight, ev); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// member[k][p >> 5] bit p & 31 <=> table position p is in instance k's subset.
+__global__ void k_build_members(const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
+                                int64_t n_inst, uint32_t* member, int64_t words) {
+  for (int64_t k = blockIdx.x; k < n_inst; k += gridDim.x) {
+    uint32_t* row = member + k * words;
+    for (int64_t i = off[k] + threadIdx.x; i < off[k + 1]; i += blockDim.x) {
+      const int32_t p = idx[i];
+      atomicOr(&row[p >> 5], 1u << (p & 31));
+    }
+  }
+}
+
+hipError_t launch_build_members(const int64_t* off, const int32_t* idx, int64_t n_inst,
+                                uint32_t* member, int64_t words, hipStream_t stream) {
+  const int64_t grid = n_inst < 65536 ? n_inst : 65536;
+  if (grid > 0) k_build_members<<<(unsigned)grid, 256, 0, stream>>>(off, idx, n_inst, member, words);
+  return hipGetLastError();
 }
 
 // ---- serving slots: heterogeneous batch, one table and store per slot ----
@@ -475,7 +747,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_slots(const SlotDesc* slots, I
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
         const int64_t q = first + j * kBlock + threadIdx.x;
-        if (q < sd.m) eval_prop<D, false>(t, sd.store, q, chg, fail, nt);
+        if (q < sd.m) eval_prop<D, false>(t, sd.store, q, chg, fail, nt, Marker{});
       }
     }
     publish(f, s, chg, fail, nt, tight);
@@ -566,8 +838,9 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
           const u64 a1 = load_word<WG>(st + x1), b1 = load_word<WG>(st + y1), c1 = load_word<WG>(st + z1);
           u64 a2 = 0, b2 = 0, c2 = 0;
           if (i2 >= 0) { a2 = load_word<WG>(st + x2); b2 = load_word<WG>(st + y2); c2 = load_word<WG>(st + z2); }
-          apply<D, WG>(st, (int)(xo1 >> kOpShift), x1, y1, z1, a1, b1, c1, chg, fail, nt);
-          if (i2 >= 0) apply<D, WG>(st, (int)(xo2 >> kOpShift), x2, y2, z2, a2, b2, c2, chg, fail, nt);
+          const Marker none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+          apply<D, WG>(st, (int)(xo1 >> kOpShift), x1, y1, z1, a1, b1, c1, chg, fail, nt, none);
+          if (i2 >= 0) apply<D, WG>(st, (int)(xo2 >> kOpShift), x2, y2, z2, a2, b2, c2, chg, fail, nt, none);
         }
         const bool wc = __ballot(chg) != 0, wf = __ballot(fail) != 0;
         if (lane == 0 && wc) s_chg = 1;
@@ -628,6 +901,7 @@ __global__ void k_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl) {
     flags[k].changed[1] = -1;      // "sweep -1 changed" lets sweep 0 run
     flags[k].failed_at = kNoFail;
     flags[k].done_at = -1;
+    flags[k].marked_evals = 0;
   }
   if (k == 0 && wl.sweep) {
     *wl.sweep = -1;

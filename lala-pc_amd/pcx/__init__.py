@@ -49,7 +49,8 @@ class Config(ctypes.Structure):
                 ("lds_mode", ctypes.c_int32), ("sweeps_per_check", ctypes.c_int32),
                 ("max_sweeps", ctypes.c_int64), ("timing", ctypes.c_int32),
                 ("domain", ctypes.c_int32), ("compact_tables", ctypes.c_int32),
-                ("stream_tables", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+                ("stream_tables", ctypes.c_int32), ("event_driven", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
 
 
 class Result(ctypes.Structure):

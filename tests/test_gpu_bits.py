@@ -78,12 +78,13 @@ def test_bits_network(gpu, seed, V, P):
         assert (wg == wo).all()
 
 
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
 @pytest.mark.parametrize("stream", [0, 1])
 @pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PERSISTENT, pcx.MODE_PLAIN])
-def test_bits_batch_of_splits(gpu, mode, stream):
+def test_bits_batch_of_splits(gpu, mode, stream, events):
     V, P, n = 4000, 16000, 64
     op, x, y, z, w = pcx.gen_bits_network(9, V, P)
-    pir = pcx.BitsPIR(stream_tables=stream)
+    pir = pcx.BitsPIR(stream_tables=stream, event_driven=events)
     pir.tell(op, x, y, z, w)
     root = w.copy()
     r = pir.fixpoint(root)

@@ -145,12 +145,13 @@ def test_literal_network_wide_bounds_fails_like_oracle(gpu, seed, ppw):
     assert bool(rg.failed) == bool(ro.failed)
 
 
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
 @pytest.mark.parametrize("seed,V,P,ppw", [(1, 20000, 80000, 256), (2, 20000, 80000, 64),
                                           (3, 100000, 400000, 128), (4, 100000, 400000, 512)])
-def test_planted_network(gpu, seed, V, P, ppw):
+def test_planted_network(gpu, seed, V, P, ppw, events):
     op, x, y, z, lb, ub, w = _network(seed, V, P)
     ro, lbo, ubo = _oracle_fix(op, x, y, z, lb, ub)
-    rg, lbg, ubg, pir = _gpu_fix(op, x, y, z, lb, ub, props_per_wave=ppw)
+    rg, lbg, ubg, pir = _gpu_fix(op, x, y, z, lb, ub, props_per_wave=ppw, event_driven=events)
     assert not ro.failed and not rg.failed
     assert (lbg == lbo).all() and (ubg == ubo).all()
     assert ((lbg <= w) & (w <= ubg)).all()
@@ -169,15 +170,17 @@ def test_full_size_network(gpu):
     assert ((lbg <= w) & (w <= ubg)).all()
 
 
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
 @pytest.mark.parametrize("tables", [(0, 0), (0, 1), (1, -1), (1, 0)],
                          ids=["indexed", "indexed-nt", "copies-nt", "copies-cached"])
 @pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PERSISTENT, pcx.MODE_PLAIN])
-def test_batch_of_search_nodes(gpu, mode, tables):
+def test_batch_of_search_nodes(gpu, mode, tables, events):
     """Shared table, per-instance subsets and splits (scaled-down batch), with
-    the subsets read through indices or from per-instance copies."""
+    the subsets read through indices or from per-instance copies, and later
+    sweeps evaluating only marked propagators or every propagator."""
     V, P, n_inst = 5000, 20000, 96
     op, x, y, z, lb, ub, w = _network(11, V, P)
-    pir = pcx.PIR(compact_tables=tables[0], stream_tables=tables[1])
+    pir = pcx.PIR(compact_tables=tables[0], stream_tables=tables[1], event_driven=events)
     pir.tell(op, x, y, z, lb, ub)
     root_lb, root_ub = lb.copy(), ub.copy()
     assert not pir.fixpoint(root_lb, root_ub).failed
