@@ -77,6 +77,37 @@ def roofline(evals, tightened, kernel_ms, launches, kernel, workload, tight_byte
             "avg_launch_us": round(kernel_ms * 1e3 / launches, 2)}
 
 
+class StatSum:
+    """Sums of pcx.Stats over the timed steps, split into full sweeps (every
+    active propagator) and marked-only sweeps."""
+
+    FIELDS = ("kernel_ms", "kernel_launches", "evals", "tightened", "full_ms", "full_launches",
+              "full_evals", "full_tightened")
+
+    def __init__(self):
+        self.v = {k: 0 for k in self.FIELDS}
+
+    def add(self, st):
+        for k in self.FIELDS:
+            self.v[k] += getattr(st, k)
+
+    def roofline(self, steps, full_kernel, marked_kernel, workload, tight_bytes=BYTES_PER_TIGHTENING):
+        """Roofline of the dominant sweep kernel (the class with more kernel
+        time per step), plus the per-step figures of both classes."""
+        v = {k: x / steps for k, x in self.v.items()}
+        full = (v["full_evals"], v["full_tightened"], v["full_ms"], v["full_launches"])
+        marked = (v["evals"] - v["full_evals"], v["tightened"] - v["full_tightened"],
+                  v["kernel_ms"] - v["full_ms"], v["kernel_launches"] - v["full_launches"])
+        dom, name = (full, full_kernel) if full[2] >= marked[2] else (marked, marked_kernel)
+        rf = roofline(*dom, name, workload, tight_bytes)
+        rf["sweep_classes_per_step"] = {
+            "full": {"kernel": full_kernel, "launches": full[3], "kernel_ms": round(full[2], 3),
+                     "evals": int(full[0])},
+            "marked": {"kernel": marked_kernel, "launches": marked[3], "kernel_ms": round(marked[2], 3),
+                       "evals": int(marked[0])}}
+        return rf
+
+
 def cpu_oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib  # test infrastructure: used only as the CPU baseline / checker
@@ -104,20 +135,16 @@ def bench_batched(args):
     for _ in range(args.warmup):
         batch.reset()
         batch.run(mode)
-    evals = tight = 0
-    kms = 0.0
-    launches = 0
+    sums = StatSum()
     smax = 0
     t_start = time.perf_counter()
     for _ in range(args.steps):
         batch.reset()
         res, st = batch.run(mode)
-        evals += st.evals
-        tight += st.tightened
-        kms += st.kernel_ms
-        launches += st.kernel_launches
+        sums.add(st)
         smax = max(smax, st.sweeps_max)
     elapsed = time.perf_counter() - t_start
+    evals, kms = sums.v["evals"], sums.v["kernel_ms"]
     fps = n_inst * args.steps / elapsed
     failed = sum(r.failed for r in res)
     lb_g, ub_g = batch.read()
@@ -149,8 +176,8 @@ def bench_batched(args):
         "evals_per_s": evals / elapsed,
         "failed_instances": int(failed),
         "sweeps_max": int(smax),
-        "roofline": roofline(evals / args.steps, tight / args.steps, kms / args.steps, launches / args.steps,
-                             "k_sweep_batched" if args.mode == "batched" else "k_persistent", "cfg2"),
+        "roofline": sums.roofline(args.steps, "k_sweep_batched" if args.mode == "batched" else "k_persistent",
+                                  "k_sweep_marked", "cfg2"),
     }
 
     if not args.no_cpu:
@@ -208,25 +235,21 @@ def bench_single(args):
     for _ in range(args.warmup):
         batch.reset()
         batch.run(pcx.MODE_PLAIN)
-    evals = tight = launches = 0
-    kms = 0.0
+    sums = StatSum()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         batch.reset()
         res, st = batch.run(pcx.MODE_PLAIN)
-        evals += st.evals
-        tight += st.tightened
-        kms += st.kernel_ms
-        launches += st.kernel_launches
+        sums.add(st)
     elapsed = time.perf_counter() - t_start
+    evals = sums.v["evals"]
     lb_g, ub_g = batch.read()
     out = {
         "value": args.steps / elapsed, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
         "config": {"workload": "single 1,000,000-var / 4,000,000-propagator fixpoint (plain engine)",
                    "nvars": V, "nprops": P, "props_per_wave": args.ppw},
         "evals_per_s": evals / elapsed, "sweeps": int(res[0].sweeps), "failed": bool(res[0].failed),
-        "roofline": roofline(evals / args.steps, tight / args.steps, kms / args.steps, launches / args.steps,
-                             "k_sweep_plain", "cfg1"),
+        "roofline": sums.roofline(args.steps, "k_sweep_plain", "k_sweep_marked_plain", "cfg1"),
     }
     if not args.no_cpu:
         ol = cpu_oracle()
@@ -262,22 +285,18 @@ def bench_bits(args):
     for _ in range(args.warmup):
         batch.reset()
         batch.run(mode)
-    evals = tight = launches = 0
-    kms = 0.0
+    sums = StatSum()
     t_start = time.perf_counter()
     for _ in range(args.steps):
         batch.reset()
         res, st = batch.run(mode)
-        evals += st.evals
-        tight += st.tightened
-        kms += st.kernel_ms
-        launches += st.kernel_launches
+        sums.add(st)
     elapsed = time.perf_counter() - t_start
+    evals = sums.v["evals"]
     fps = n_inst * args.steps / elapsed
     words_g = batch.read_words()
-    rf = roofline(evals / args.steps, tight / args.steps, kms / args.steps, launches / args.steps,
-                  "k_sweep_batched_bits" if args.mode == "batched" else "k_persistent_bits", "cfg3",
-                  tight_bytes=8)
+    rf = sums.roofline(args.steps, "k_sweep_batched_bits" if args.mode == "batched" else "k_persistent_bits",
+                       "k_sweep_marked_bits", "cfg3", tight_bytes=8)
     out = {"value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
            "config": {"workload": "bitset store: 1,024 bit-removal subproblems of one 200,000-word / "
                                   "800,000-propagator network (add, neq, leq, eq-reif)",

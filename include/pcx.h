@@ -102,6 +102,12 @@ typedef struct {
   int64_t evals;            /* total evaluations over all instances          */
   int64_t tightened;        /* successful bound tightenings (atomic updates) */
   int64_t sweeps_max;       /* largest per-instance sweep count              */
+  /* the part of the above spent in full sweeps (every active propagator of
+     every running instance; with event-driven sweeps only the first sweep) */
+  double  full_ms;
+  int64_t full_launches;
+  int64_t full_evals;
+  int64_t full_tightened;
 } pcx_stats;
 
 /* ---- library ---------------------------------------------------------- */

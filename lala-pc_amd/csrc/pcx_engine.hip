@@ -127,16 +127,48 @@ hipEvent_t event_at(pcx_batch* b, size_t i) {
   return b->ev[i];
 }
 
+// HIP-event timing of sweep launches, split into full and marked sweeps.
+struct SweepTimer {
+  pcx_batch* b;
+  bool on;
+  size_t nev = 0;
+  std::vector<char> full;             // class of each recorded launch
+  double ms[2] = {0.0, 0.0};          // [0] marked, [1] full
+  int64_t launches[2] = {0, 0};
+  pcx_status begin(hipStream_t st) {
+    if (on) PCX_HIP(hipEventRecord(event_at(b, nev++), st));
+    return PCX_OK;
+  }
+  pcx_status end(hipStream_t st, bool is_full, int n = 1) {
+    launches[is_full] += n;
+    if (!on) return PCX_OK;
+    PCX_HIP(hipEventRecord(event_at(b, nev++), st));
+    full.push_back(is_full);
+    return PCX_OK;
+  }
+  // after the stream has been synchronized
+  pcx_status drain() {
+    for (size_t i = 0; i + 1 < nev; i += 2) {
+      float t = 0.f;
+      PCX_HIP(hipEventElapsedTime(&t, b->ev[i], b->ev[i + 1]));
+      ms[full[i / 2] ? 1 : 0] += t;
+    }
+    nev = 0;
+    full.clear();
+    return PCX_OK;
+  }
+};
+
 pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* results,
-                   pcx_stats* stats, int64_t launches, double kernel_ms) {
+                   pcx_stats* stats, const SweepTimer& tm) {
   pcx_engine* e = b->e;
   flags.resize(b->n_inst);
   PCX_HIP(hipMemcpyAsync(flags.data(), b->d_flags, b->n_inst * sizeof(InstFlags),
                          hipMemcpyDeviceToHost, e->stream));
-  unsigned long long tight[kTightSlots];
+  unsigned long long tight[2 * kTightSlots];    // [0, kTightSlots): full sweeps
   PCX_HIP(hipMemcpyAsync(tight, b->d_tight, sizeof(tight), hipMemcpyDeviceToHost, e->stream));
   PCX_HIP(hipStreamSynchronize(e->stream));
-  int64_t evals = 0, smax = 0;
+  int64_t evals = 0, smax = 0, full_evals = 0;
   for (int64_t k = 0; k < b->n_inst; ++k) {
     const InstFlags& f = flags[k];
     const int32_t last = std::max(f.changed[0], f.changed[1]);
@@ -145,22 +177,31 @@ pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* resu
     r.has_changed = last >= 0;
     r.sweeps = r.failed ? (int64_t)f.failed_at + 1 : (int64_t)last + 2;
     if (e->cfg.max_sweeps > 0 && r.sweeps > e->cfg.max_sweeps) r.sweeps = e->cfg.max_sweeps;
-    if (b->events)       // one full sweep (unless empty at the start) + the marked evaluations
-      r.evals = (f.failed_at == -1 ? 0 : inst_size(b, k)) + f.marked_evals;
-    else
+    if (b->events) {     // one full sweep (unless empty at the start) + the marked evaluations
+      const int64_t first = f.failed_at == -1 ? 0 : inst_size(b, k);
+      r.evals = first + f.marked_evals;
+      full_evals += first;
+    } else {
       r.evals = r.sweeps * inst_size(b, k);
+      full_evals += r.evals;
+    }
     evals += r.evals;
     smax = std::max(smax, r.sweeps);
     if (results) results[k] = r;
   }
   if (stats) {
-    stats->kernel_ms = kernel_ms;
-    stats->kernel_launches = launches;
+    stats->kernel_ms = tm.ms[0] + tm.ms[1];
+    stats->kernel_launches = tm.launches[0] + tm.launches[1];
     stats->evals = evals;
-    unsigned long long t = 0;
-    for (int i = 0; i < kTightSlots; ++i) t += tight[i];
+    unsigned long long t = 0, tf = 0;
+    for (int i = 0; i < 2 * kTightSlots; ++i) t += tight[i];
+    for (int i = 0; i < kTightSlots; ++i) tf += tight[i];
     stats->tightened = (int64_t)t;
     stats->sweeps_max = smax;
+    stats->full_ms = tm.ms[1];
+    stats->full_launches = tm.launches[1];
+    stats->full_evals = full_evals;
+    stats->full_tightened = (int64_t)tf;
   }
   return PCX_OK;
 }
@@ -289,7 +330,7 @@ pcx_status prepare_run(pcx_batch* b, int32_t mode) {
     return fail_with(PCX_EINVAL, "a propagator names a variable outside the store");
   pcx_status s = sync_table(e);
   if (s != PCX_OK) return s;
-  PCX_HIP(hipMemsetAsync(b->d_tight, 0, kTightSlots * sizeof(unsigned long long), e->stream));
+  PCX_HIP(hipMemsetAsync(b->d_tight, 0, 2 * kTightSlots * sizeof(unsigned long long), e->stream));
   const WorkList wl = worklist_of(b->d_act, b->d_item_base, b->d_ctl);
   PCX_HIP(launch_init_flags(b->d_flags, (int32_t)b->n_inst, wl, e->stream));
   PCX_HIP(launch_store_check_empty(b->d_store, b->nvars, (int32_t)b->n_inst, b->d_flags, dom_of(e), e->stream));
@@ -305,9 +346,7 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   const int group = e->cfg.sweeps_per_check > 0 ? e->cfg.sweeps_per_check : 8;
   const Table t = table_of(e);
   const EventSet ev = event_set(b);
-  int64_t launches = 0;
-  double kms = 0.0;
-  size_t nev = 0;
+  SweepTimer tm{b, e->cfg.timing != 0};
   InstFlags hf;
   for (int64_t k = 0; k < b->n_inst; ++k) {
     const int64_t m = inst_size(b, k);
@@ -325,30 +364,29 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     for (;;) {
       for (int g = 0; g < group; ++g, ++s) {
         if (e->cfg.max_sweeps > 0 && s >= e->cfg.max_sweeps) break;
-        if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
-        if (b->events && s > 0)
-          PCX_HIP(launch_sweep_marked_plain(t, st, s, f, b->d_tight, ev, k, dom_of(e), e->stream));
+        const bool marked = b->events && s > 0;
+        pcx_status ts = tm.begin(e->stream);
+        if (ts != PCX_OK) return ts;
+        if (marked)
+          PCX_HIP(launch_sweep_marked_plain(t, st, s, f, b->d_tight + kTightSlots, ev, k, dom_of(e),
+                                            e->stream));
         else
           PCX_HIP(launch_sweep_plain(tk, st, idx, m, s, f, b->d_tight, ppw, dom_of(e), b->stream,
                                      e->stream, b->events ? &ev : nullptr, k));
-        if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
-        ++launches;
+        ts = tm.end(e->stream, !marked);
+        if (ts != PCX_OK) return ts;
       }
       PCX_HIP(hipMemcpyAsync(&hf, f, sizeof(hf), hipMemcpyDeviceToHost, e->stream));
       PCX_HIP(hipStreamSynchronize(e->stream));
-      for (size_t i = 0; i + 1 < nev; i += 2) {
-        float ms = 0.f;
-        PCX_HIP(hipEventElapsedTime(&ms, b->ev[i], b->ev[i + 1]));
-        kms += ms;
-      }
-      nev = 0;
+      pcx_status ts = tm.drain();
+      if (ts != PCX_OK) return ts;
       const int32_t last = std::max(hf.changed[0], hf.changed[1]);
       if (hf.failed_at != kNoFail || last + 1 < s) break;       // quiet sweep already ran
       if (e->cfg.max_sweeps > 0 && s >= e->cfg.max_sweeps) break;
     }
   }
   std::vector<InstFlags> flags;
-  return collect(b, flags, results, stats, launches, kms);
+  return collect(b, flags, results, stats, tm);
 }
 
 // Batched mode: per sweep one scheduler launch and one grid-stride sweep
@@ -361,42 +399,76 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   const int grid = pcx_internal::device_cus(e->cfg.device) * 8;   // a multiple of the XCD count
   const Subsets sub = run_subsets(b);
   const WorkList wl = worklist_of(b->d_act, b->d_item_base, b->d_ctl);
-  int64_t launches = 0;
-  double kms = 0.0;
-  size_t nev = 0;
+  SweepTimer tm{b, e->cfg.timing != 0};
   int32_t live = 1, sweeps = 0;
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
   const EventSet ev = event_set(b);
   const int64_t marked_items = (bitmap_words(b) + kMarkedWordsPerItem - 1) / kMarkedWordsPerItem;
+  auto launch_marked = [&]() -> pcx_status {
+    PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
+                            marked_items));
+    PCX_HIP(launch_sweep_marked(table_of(e), b->d_store, b->nvars, b->d_flags, wl,
+                                b->d_tight + kTightSlots, ev, dom_of(e), grid, e->stream));
+    return PCX_OK;
+  };
+  // Groups after the first hold only marked sweeps, whose launches have the
+  // same arguments every time (the sweep number lives on the device): they
+  // are captured once into a graph and replayed.
+  const uint64_t key = (e->version << 20) ^ ((uint64_t)group << 8) ^ ((uint64_t)dom_of(e) << 4) ^
+                       (uint64_t)(maxs & 0xf) ^ ((uint64_t)(uintptr_t)e->d_xo << 1);
+  if (b->events && (!b->marked_graph || b->marked_graph_key != key)) {
+    if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
+    b->marked_graph = nullptr;
+    hipGraph_t graph = nullptr;
+    PCX_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    pcx_status cs = PCX_OK;
+    for (int g = 0; g < group && cs == PCX_OK; ++g) cs = launch_marked();
+    const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+    if (cs != PCX_OK) { if (graph) (void)hipGraphDestroy(graph); return cs; }
+    PCX_HIP(ce);
+    const hipError_t ie = hipGraphInstantiate(&b->marked_graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    PCX_HIP(ie);
+    b->marked_graph_key = key;
+  }
   while (live > 0) {
+    if (b->events && sweeps > 0) {
+      pcx_status ts = tm.begin(e->stream);
+      if (ts != PCX_OK) return ts;
+      PCX_HIP(hipGraphLaunch(b->marked_graph, e->stream));
+      ts = tm.end(e->stream, false, group);
+      if (ts != PCX_OK) return ts;
+      sweeps += group;
+      PCX_HIP(hipMemcpyAsync(&live, b->d_ctl + 2, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+      PCX_HIP(hipStreamSynchronize(e->stream));
+      ts = tm.drain();
+      if (ts != PCX_OK) return ts;
+      continue;
+    }
     for (int g = 0; g < group; ++g) {
       const bool marked = b->events && sweeps > 0;
       PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
                               marked ? marked_items : 0));
-      if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
+      pcx_status ts = tm.begin(e->stream);
+      if (ts != PCX_OK) return ts;
       if (marked)
-        PCX_HIP(launch_sweep_marked(table_of(e), b->d_store, b->nvars, b->d_flags, wl, b->d_tight, ev,
-                                    dom_of(e), grid, e->stream));
+        PCX_HIP(launch_sweep_marked(table_of(e), b->d_store, b->nvars, b->d_flags, wl,
+                                    b->d_tight + kTightSlots, ev, dom_of(e), grid, e->stream));
       else
         PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
                                      b->d_tight, ppw, dom_of(e), b->stream, grid, e->stream,
                                      b->events ? &ev : nullptr));
-      if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, nev++), e->stream));
-      ++launches;
+      ts = tm.end(e->stream, !marked);
+      if (ts != PCX_OK) return ts;
       ++sweeps;
     }
     PCX_HIP(hipMemcpyAsync(&live, b->d_ctl + 2, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
     PCX_HIP(hipStreamSynchronize(e->stream));
-    for (size_t i = 0; i + 1 < nev; i += 2) {
-      float ms = 0.f;
-      PCX_HIP(hipEventElapsedTime(&ms, b->ev[i], b->ev[i + 1]));
-      kms += ms;
-    }
-    nev = 0;
+    pcx_status ts = tm.drain();
+    if (ts != PCX_OK) return ts;
   }
-  (void)sweeps;
   std::vector<InstFlags> flags;
-  return collect(b, flags, results, stats, launches, kms);
+  return collect(b, flags, results, stats, tm);
 }
 
 // Persistent mode: one launch; workgroups pull whole instances from a queue
@@ -409,20 +481,19 @@ pcx_status run_persistent(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   const int grid = (int)std::min<int64_t>((int64_t)dev_cus * per_cu, b->n_inst);
   const Subsets sub = run_subsets(b);
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
-  double kms = 0.0;
-  if (e->cfg.timing) PCX_HIP(hipEventRecord(event_at(b, 0), e->stream));
+  SweepTimer tm{b, e->cfg.timing != 0};
+  pcx_status ts = tm.begin(e->stream);
+  if (ts != PCX_OK) return ts;
   PCX_HIP(launch_persistent(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, b->d_order,
                             (int32_t)b->n_inst, b->d_ctl + 3, b->d_tight, maxs, dom_of(e), grid,
                             e->stream));
-  if (e->cfg.timing) {
-    PCX_HIP(hipEventRecord(event_at(b, 1), e->stream));
-    PCX_HIP(hipEventSynchronize(b->ev[1]));
-    float ms = 0.f;
-    PCX_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
-    kms = ms;
-  }
+  ts = tm.end(e->stream, true);
+  if (ts != PCX_OK) return ts;
+  PCX_HIP(hipStreamSynchronize(e->stream));
+  ts = tm.drain();
+  if (ts != PCX_OK) return ts;
   std::vector<InstFlags> flags;
-  return collect(b, flags, results, stats, 1, kms);
+  return collect(b, flags, results, stats, tm);
 }
 
 }  // namespace
@@ -603,7 +674,7 @@ pcx_status pcx_batch_create(pcx_engine* e, int64_t n_inst, int64_t nvars, const 
   chk(hipMalloc(&b->d_store, sbytes));
   chk(hipMalloc(&b->d_image, sbytes));
   chk(hipMalloc(&b->d_flags, n_inst * sizeof(InstFlags)));
-  chk(hipMalloc(&b->d_tight, kTightSlots * sizeof(unsigned long long)));
+  chk(hipMalloc(&b->d_tight, 2 * kTightSlots * sizeof(unsigned long long)));
   chk(hipMalloc(&b->d_act, n_inst * sizeof(int32_t)));
   chk(hipMalloc(&b->d_item_base, (n_inst + 1) * sizeof(int64_t)));
   chk(hipMalloc(&b->d_ctl, kCtlWords * sizeof(int32_t)));
@@ -646,6 +717,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   dfree(b->d_store); dfree(b->d_image); dfree(b->d_off); dfree(b->d_idx); dfree(b->d_flags);
   dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl); dfree(b->d_order);
   dfree(b->d_cxo); dfree(b->d_cy); dfree(b->d_cz); dfree(b->d_member); dfree(b->d_dirty);
+  if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
   delete b;
 }
 

@@ -52,6 +52,9 @@ struct pcx_batch {
   uint32_t* d_dirty = nullptr;
   uint64_t member_version = 0;
   bool events = false;                // this run uses marked-only sweeps
+  // captured group of marked sweeps (batched mode), rebuilt when its key changes
+  hipGraphExec_t marked_graph = nullptr;
+  uint64_t marked_graph_key = 0;
   pcx::InstFlags* d_flags = nullptr;
   unsigned long long* d_tight = nullptr;
   int32_t* d_act = nullptr;

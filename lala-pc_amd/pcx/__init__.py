@@ -61,7 +61,9 @@ class Result(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("wall_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
                 ("kernel_launches", ctypes.c_int64), ("evals", ctypes.c_int64),
-                ("tightened", ctypes.c_int64), ("sweeps_max", ctypes.c_int64)]
+                ("tightened", ctypes.c_int64), ("sweeps_max", ctypes.c_int64),
+                ("full_ms", ctypes.c_double), ("full_launches", ctypes.c_int64),
+                ("full_evals", ctypes.c_int64), ("full_tightened", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
