@@ -108,6 +108,28 @@ class StatSum:
         return rf
 
 
+def autotune(args, pir, batch, key):
+    """Configure the engine from the tuning cache, or run the autotuner on the
+    loaded batch (setup, outside the timed region) and cache its choice."""
+    path = args.tune_cache or pcx.default_tuning_path()
+    cached = None if args.retune else pcx.load_tuning(path, key)
+    t0 = time.time()
+    if cached:
+        best = cached["best"]
+        mode = pcx.apply_tuning(pir, best)
+        how = "cached"
+    else:
+        best, points = batch.autotune(reps=2)
+        mode = best["mode"]
+        how = f"tuned over {len(points)} points in {time.time() - t0:.1f}s"
+        try:
+            pcx.save_tuning(path, key, best, points)
+        except OSError as e:
+            log(f"autotune: could not save {path}: {e}")
+    log(f"autotune ({how}): {best}")
+    return mode, dict(best, source=how)
+
+
 def cpu_oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib  # test infrastructure: used only as the CPU baseline / checker
@@ -132,6 +154,9 @@ def bench_batched(args):
         f"{int(off[-1])} active propagators over {n_inst} instances")
 
     mode = {"batched": pcx.MODE_BATCHED, "persistent": pcx.MODE_PERSISTENT}[args.mode]
+    tuning = None
+    if args.autotune:
+        mode, tuning = autotune(args, pir, batch, f"cfg2:V{V}:P{P}:n{n_inst}:seed{args.seed}")
     for _ in range(args.warmup):
         batch.reset()
         batch.run(mode)
@@ -169,14 +194,15 @@ def bench_batched(args):
                                "50,000-var / 200,000-propagator network, 1-20 splits each, "
                                "2,000-200,000 active propagators each",
                    "nvars": V, "nprops": P, "instances": n_inst, "props_per_wave": args.ppw,
-                   "mode": args.mode, "compact_tables": args.compact, "stream_tables": args.stream,
-                   "event_driven": args.events},
+                   "mode": {pcx.MODE_BATCHED: "batched", pcx.MODE_PERSISTENT: "persistent"}[mode],
+                   "compact_tables": args.compact, "stream_tables": args.stream,
+                   "event_driven": args.events, "autotuned": tuning},
         "plain_fixpoints_per_s": plain_fps,
         "batched_vs_plain": fps / plain_fps if plain_fps else None,
         "evals_per_s": evals / elapsed,
         "failed_instances": int(failed),
         "sweeps_max": int(smax),
-        "roofline": sums.roofline(args.steps, "k_sweep_batched" if args.mode == "batched" else "k_persistent",
+        "roofline": sums.roofline(args.steps, "k_sweep_batched" if mode == pcx.MODE_BATCHED else "k_persistent",
                                   "k_sweep_marked", "cfg2"),
     }
 
@@ -282,6 +308,9 @@ def bench_bits(args):
     batch = pcx.Batch(pir, n_inst, V)
     batch.load_words(inst)
     mode = {"batched": pcx.MODE_BATCHED, "persistent": pcx.MODE_PERSISTENT}[args.mode]
+    tuning = None
+    if args.autotune:
+        mode, tuning = autotune(args, pir, batch, f"cfg3:V{V}:P{P}:n{n_inst}:seed{args.seed}")
     for _ in range(args.warmup):
         batch.reset()
         batch.run(mode)
@@ -295,12 +324,14 @@ def bench_bits(args):
     evals = sums.v["evals"]
     fps = n_inst * args.steps / elapsed
     words_g = batch.read_words()
-    rf = sums.roofline(args.steps, "k_sweep_batched_bits" if args.mode == "batched" else "k_persistent_bits",
+    rf = sums.roofline(args.steps, "k_sweep_batched_bits" if mode == pcx.MODE_BATCHED else "k_persistent_bits",
                        "k_sweep_marked_bits", "cfg3", tight_bytes=8)
     out = {"value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
            "config": {"workload": "bitset store: 1,024 bit-removal subproblems of one 200,000-word / "
                                   "800,000-propagator network (add, neq, leq, eq-reif)",
-                      "nvars": V, "nprops": P, "instances": n_inst, "mode": args.mode},
+                      "nvars": V, "nprops": P, "instances": n_inst,
+                      "mode": {pcx.MODE_BATCHED: "batched", pcx.MODE_PERSISTENT: "persistent"}[mode],
+                      "autotuned": tuning},
            "single_fixpoint_ms": single_ms, "single_sweeps": int(r_single.sweeps),
            "evals_per_s": evals / elapsed, "failed_instances": int(sum(r.failed for r in res)),
            "roofline": rf}
@@ -453,6 +484,12 @@ def main():
                     help="per-instance subset tables: -1 auto, 0 shared table via indices, 1 copies")
     ap.add_argument("--stream", type=int, default=-1, choices=[-1, 0, 1],
                     help="table loads: -1 auto, 0 cached, 1 non-temporal")
+    ap.add_argument("--autotune", type=int, default=1, choices=[0, 1],
+                    help="configs 2 and 3: pick the sweep configuration with the autotuner "
+                         "(cached as JSON, see --tune-cache) before the timed steps")
+    ap.add_argument("--tune-cache", default=None, help="tuning cache file (default $PCX_TUNING or "
+                                                       "~/.cache/pcx/tuning.json)")
+    ap.add_argument("--retune", action="store_true", help="ignore a cached tuning")
     ap.add_argument("--events", type=int, default=-1, choices=[-1, 0, 1],
                     help="sweeps after the first: -1 auto, 0 every propagator, 1 marked only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")

@@ -72,7 +72,9 @@ typedef struct pcx_batch pcx_batch;
 typedef struct {
   int32_t device;           /* HIP device ordinal                                   */
   int32_t props_per_wave;   /* 64, 128, 256 or 512; 0 = chosen from the sweep size   */
-  int32_t lds_mode;         /* -1 auto, 0 never stage stores in LDS, 1 whenever fit  */
+  int32_t lds_mode;         /* persistent mode: stage each store in LDS; -1 auto
+                               (stores up to 32 KB), 0 never, 1 whenever it fits
+                               (stores up to 64 KB)                                */
   int32_t sweeps_per_check; /* plain engine: sweeps queued between host checks (0=8) */
   int64_t max_sweeps;       /* 0 = run to the fixpoint                               */
   int32_t timing;           /* 1 = record HIP events around every sweep kernel       */
@@ -134,6 +136,11 @@ pcx_status pcx_tell(pcx_engine* e, const int32_t* op, const int32_t* x,
                     const int32_t* y, const int32_t* z, int64_t n,
                     int32_t* lb, int32_t* ub, int64_t nvars);
 
+/* Read / change an engine's configuration (device and domain cannot change:
+ * PCX_EINVAL).  Takes effect at the next run. */
+pcx_status pcx_get_config(const pcx_engine* e, pcx_config* cfg);
+pcx_status pcx_set_config(pcx_engine* e, const pcx_config* cfg);
+
 /* replaces PIR::num_deductions (pir.hpp:382-384). */
 int64_t    pcx_num_deductions(const pcx_engine* e);
 
@@ -185,6 +192,29 @@ pcx_status pcx_batch_read(pcx_batch* b, int32_t* lb, int32_t* ub);
 /* Propagate every instance to its fixpoint.  results: n_inst entries or NULL. */
 pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results,
                          pcx_stats* stats);
+
+/* ---- autotuning ---------------------------------------------------------
+ * One point of the search: how a batch is swept. */
+typedef struct {
+  int32_t mode;             /* PCX_MODE_BATCHED or PCX_MODE_PERSISTENT          */
+  int32_t props_per_wave;   /* batched: 64, 128, 256, 512                       */
+  int32_t lds_mode;         /* persistent: 0 or 1 (stores staged in LDS)        */
+  int32_t compact_tables;   /* 0 or 1 (only varied for batches with subsets)    */
+  int32_t stream_tables;    /* 0 or 1                                           */
+  int32_t event_driven;     /* batched: 0 or 1                                  */
+  double  ms;               /* best host time of one reset + run; < 0: failed   */
+} pcx_tuning;
+
+/* Time every point of {batched: props_per_wave x compact x stream x
+ * event_driven} U {persistent: lds (where the store fits) x compact} on the
+ * batch's loaded image (one warm-up and `reps` timed runs per point, best
+ * kept), set the engine's configuration to the fastest point, and restore
+ * the stores to the image.  results (may be NULL): up to max_results points
+ * in the order tried; *n_results (may be NULL) receives how many were tried.
+ * No counterpart in the reference (its fixpoint has no tuning parameters). */
+pcx_status pcx_batch_autotune(pcx_batch* b, int32_t reps, pcx_tuning* best,
+                              pcx_tuning* results, int32_t max_results,
+                              int32_t* n_results);
 
 /* ---- bitset domain (engine created with config.domain = PCX_DOMAIN_BITSET)
  * The store is one uint64 word per variable (bit v <=> value v in [0,63]);

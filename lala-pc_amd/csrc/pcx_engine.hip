@@ -477,7 +477,12 @@ pcx_status run_persistent(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
   const Table t = run_table(b);
   const int dev_cus = pcx_internal::device_cus(e->cfg.device);
-  const int per_cu = persistent_blocks_per_cu(dom_of(e));
+  // LDS-resident stores: lds_mode 1 whenever the store fits, -1 (auto) when
+  // it also leaves room for two workgroups per CU.
+  const int64_t sbytes = b->nvars * 8;
+  const bool lds = (e->cfg.lds_mode > 0 && sbytes <= kLdsStoreBytes) ||
+                   (e->cfg.lds_mode < 0 && sbytes <= kLdsStoreBytes / 2);
+  const int per_cu = persistent_blocks_per_cu(dom_of(e), lds ? sbytes : 0);
   const int grid = (int)std::min<int64_t>((int64_t)dev_cus * per_cu, b->n_inst);
   const Subsets sub = run_subsets(b);
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
@@ -486,7 +491,7 @@ pcx_status run_persistent(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   if (ts != PCX_OK) return ts;
   PCX_HIP(launch_persistent(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, b->d_order,
                             (int32_t)b->n_inst, b->d_ctl + 3, b->d_tight, maxs, dom_of(e), grid,
-                            e->stream));
+                            lds, e->stream));
   ts = tm.end(e->stream, true);
   if (ts != PCX_OK) return ts;
   PCX_HIP(hipStreamSynchronize(e->stream));
@@ -612,6 +617,22 @@ pcx_status pcx_tell(pcx_engine* e, const int32_t* op, const int32_t* x, const in
     e->has_div |= all[i].op >= PCX_TDIV && all[i].op <= PCX_EDIV;
   }
   e->dirty = true;
+  return PCX_OK;
+}
+
+pcx_status pcx_get_config(const pcx_engine* e, pcx_config* cfg) {
+  if (!e || !cfg) return fail_with(PCX_EINVAL, "NULL argument");
+  *cfg = e->cfg;
+  return PCX_OK;
+}
+
+pcx_status pcx_set_config(pcx_engine* e, const pcx_config* cfg) {
+  if (!e || !cfg) return fail_with(PCX_EINVAL, "NULL argument");
+  if (cfg->device != e->cfg.device || cfg->domain != e->cfg.domain)
+    return fail_with(PCX_EINVAL, "device and domain are fixed at creation");
+  if (cfg->props_per_wave != 0 && !valid_ppw(cfg->props_per_wave))
+    return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto), 64, 128, 256 or 512");
+  e->cfg = *cfg;
   return PCX_OK;
 }
 
@@ -778,6 +799,67 @@ pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results, pcx_st
     default: return fail_with(PCX_EINVAL, "unknown mode");
   }
   if (stats) stats->wall_ms = now_ms() - t0;
+  return s;
+}
+
+pcx_status pcx_batch_autotune(pcx_batch* b, int32_t reps, pcx_tuning* best, pcx_tuning* results,
+                              int32_t max_results, int32_t* n_results) {
+  if (!b || !best) return fail_with(PCX_EINVAL, "NULL argument");
+  if (reps < 1 || reps > 100) return fail_with(PCX_EINVAL, "reps must be in [1, 100]");
+  pcx_engine* e = b->e;
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  const pcx_config saved = e->cfg;
+  std::vector<pcx_tuning> pts;
+  const bool subsets = !b->h_off.empty();
+  for (int ppw : {64, 128, 256, 512})
+    for (int c = 0; c <= (subsets ? 1 : 0); ++c)
+      for (int st = 0; st <= 1; ++st)
+        for (int ev = 0; ev <= 1; ++ev)
+          pts.push_back(pcx_tuning{PCX_MODE_BATCHED, ppw, 0, c, st, ev, 0.0});
+  const bool lds_fits = b->nvars * 8 <= kLdsStoreBytes;
+  for (int l = 0; l <= (lds_fits ? 1 : 0); ++l)
+    for (int c = 0; c <= (subsets ? 1 : 0); ++c)
+      pts.push_back(pcx_tuning{PCX_MODE_PERSISTENT, 0, l, c, 0, 0, 0.0});
+  int best_i = -1;
+  for (size_t i = 0; i < pts.size(); ++i) {
+    pcx_tuning& p = pts[i];
+    e->cfg.props_per_wave = p.props_per_wave;
+    e->cfg.lds_mode = p.lds_mode;
+    e->cfg.compact_tables = p.compact_tables;
+    e->cfg.stream_tables = p.stream_tables;
+    e->cfg.event_driven = p.event_driven;
+    e->cfg.timing = 0;
+    double bt = -1.0;
+    for (int r = 0; r <= reps; ++r) {
+      pcx_status s = pcx_batch_reset(b);
+      if (s == PCX_OK) s = hipStreamSynchronize(e->stream) == hipSuccess ? PCX_OK : PCX_EDEVICE;
+      const double t0 = now_ms();
+      if (s == PCX_OK) s = pcx_batch_run(b, p.mode, nullptr, nullptr);
+      const double t = now_ms() - t0;
+      if (s != PCX_OK) {
+        if (s != PCX_ENOMEM) { e->cfg = saved; return s; }   // a point that does not fit is skipped
+        bt = -1.0;
+        break;
+      }
+      if (r > 0 && (bt < 0 || t < bt)) bt = t;              // run 0 warms up
+    }
+    p.ms = bt;
+    if (bt >= 0 && (best_i < 0 || bt < pts[best_i].ms)) best_i = (int)i;
+  }
+  e->cfg = saved;
+  if (results)
+    for (size_t i = 0; i < pts.size() && (int32_t)i < max_results; ++i) results[i] = pts[i];
+  if (n_results) *n_results = (int32_t)pts.size();
+  if (best_i < 0) return fail_with(PCX_ENOMEM, "no configuration could run");
+  const pcx_tuning& w = pts[best_i];
+  e->cfg.props_per_wave = w.props_per_wave;
+  e->cfg.lds_mode = w.lds_mode;
+  e->cfg.compact_tables = w.compact_tables;
+  e->cfg.stream_tables = w.stream_tables;
+  e->cfg.event_driven = w.event_driven;
+  *best = w;
+  pcx_status s = pcx_batch_reset(b);
+  if (s == PCX_OK) PCX_HIP(hipStreamSynchronize(e->stream));
   return s;
 }
 

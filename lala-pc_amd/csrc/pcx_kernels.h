@@ -124,11 +124,14 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
                                 unsigned long long* tight, int ppw, int dom, bool stream_table,
                                 int grid, hipStream_t st, const EventSet* ev = nullptr);
 constexpr int kMarkedWordsPerItem = 256;   // bitmap words per marked-sweep item
+// lds: stage each instance's store in LDS (nvars * 8 <= kLdsStoreBytes).
+constexpr int64_t kLdsStoreBytes = 64 * 1024;
 hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                              int64_t n_props, InstFlags* flags, const int32_t* order,
                              int32_t n_inst, int32_t* queue, unsigned long long* tight,
-                             int32_t max_sweeps, int dom, int grid, hipStream_t st);
-int persistent_blocks_per_cu(int dom);   // resident workgroups per CU
+                             int32_t max_sweeps, int dom, int grid, bool lds, hipStream_t st);
+// resident workgroups per CU (lds_bytes of dynamic LDS per workgroup)
+int persistent_blocks_per_cu(int dom, int64_t lds_bytes);
 hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipStream_t st);
 hipError_t launch_schedule_slots(const SlotDesc* slots, InstFlags* flags, int32_t n_slots,
                                  WorkList wl, int ppw, hipStream_t st);

@@ -792,7 +792,10 @@ hipError_t launch_sweep_slots(const SlotDesc* slots, InstFlags* flags, WorkList 
 // termination argument needs.
 constexpr int kPBlock = 512;     // 8 waves per workgroup
 
-template <int D>
+// LDS: the instance's store (nvars 8-byte entries, at most kLdsStoreBytes)
+// is copied into the workgroup's LDS, swept there with LDS atomics, and
+// written back when the instance is done.
+template <int D, bool LDS>
 __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restrict__ store,
                                                         int64_t nvars, Subsets sub, int64_t n_props,
                                                         InstFlags* flags, const int32_t* order,
@@ -801,6 +804,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
                                                         int32_t max_sweeps) {
   constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
   __shared__ int32_t s_slot, s_chg, s_fail;
+  extern __shared__ u64 s_lds[];
   const int lane = lane_id();
   for (;;) {
     if (threadIdx.x == 0) s_slot = atomicAdd(queue, 1);
@@ -813,10 +817,16 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
     const Table& tk = it.t;
     const int32_t* idx = it.idx;
     const int64_t m = it.m;
-    int2* st = store + (int64_t)k * nvars;
+    int2* gst = store + (int64_t)k * nvars;
+    int2* st = LDS ? reinterpret_cast<int2*>(s_lds) : gst;
     const bool start_failed = f->failed_at != kNoFail;
     int32_t last_changed = -1, failed_at = kNoFail, s = 0;
     int nt = 0;
+    if (LDS && !start_failed) {
+      const u64* g = reinterpret_cast<const u64*>(gst);
+      for (int64_t v = threadIdx.x; v < nvars; v += kPBlock) s_lds[v] = g[v];
+      __syncthreads();
+    }
     if (!start_failed) {
       for (s = 0; max_sweeps <= 0 || s < max_sweeps; ++s) {
         if (threadIdx.x == 0) { s_chg = 0; s_fail = 0; }
@@ -853,6 +863,11 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
         if (!any_chg) break;
       }
     }
+    if (LDS && !start_failed) {
+      __syncthreads();                       // every tightening is in LDS
+      u64* g = reinterpret_cast<u64*>(gst);
+      for (int64_t v = threadIdx.x; v < nvars; v += kPBlock) g[v] = s_lds[v];
+    }
     const int sum = wave_sum(nt);
     if (lane == 0 && sum) atomicAdd(&tight[(k * 16 + (threadIdx.x >> 6)) & (kTightSlots - 1)],
                                     (unsigned long long)sum);
@@ -865,28 +880,33 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
   }
 }
 
-int persistent_blocks_per_cu(int dom) {
+int persistent_blocks_per_cu(int dom, int64_t lds_bytes) {
   int n = 0;
   hipError_t e;
+  const size_t sm = (size_t)lds_bytes;
+#define PCX_OCC(D, L) hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<D, L>, kPBlock, sm)
   switch (dom) {
-    case kItvDiv: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<kItvDiv>, kPBlock, 0); break;
-    case kBits: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<kBits>, kPBlock, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persistent<kItv>, kPBlock, 0); break;
+    case kItvDiv: e = lds_bytes ? PCX_OCC(kItvDiv, true) : PCX_OCC(kItvDiv, false); break;
+    case kBits: e = lds_bytes ? PCX_OCC(kBits, true) : PCX_OCC(kBits, false); break;
+    default: e = lds_bytes ? PCX_OCC(kItv, true) : PCX_OCC(kItv, false); break;
   }
+#undef PCX_OCC
   return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                              int64_t n_props, InstFlags* flags, const int32_t* order,
                              int32_t n_inst, int32_t* queue, unsigned long long* tight,
-                             int32_t max_sweeps, int dom, int grid, hipStream_t stream) {
-#define PCX_PERSIST(D) \
-  k_persistent<D><<<grid, kPBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, order, \
-                                                n_inst, queue, tight, max_sweeps)
+                             int32_t max_sweeps, int dom, int grid, bool lds, hipStream_t stream) {
+  if (lds && nvars * 8 > kLdsStoreBytes) return hipErrorInvalidValue;
+  const size_t sm = lds ? (size_t)nvars * 8 : 0;
+#define PCX_PERSIST(D, L) \
+  k_persistent<D, L><<<grid, kPBlock, sm, stream>>>(t, store, nvars, sub, n_props, flags, order, \
+                                                    n_inst, queue, tight, max_sweeps)
   switch (dom) {
-    case kItv: PCX_PERSIST(kItv); break;
-    case kItvDiv: PCX_PERSIST(kItvDiv); break;
-    case kBits: PCX_PERSIST(kBits); break;
+    case kItv: if (lds) PCX_PERSIST(kItv, true); else PCX_PERSIST(kItv, false); break;
+    case kItvDiv: if (lds) PCX_PERSIST(kItvDiv, true); else PCX_PERSIST(kItvDiv, false); break;
+    case kBits: if (lds) PCX_PERSIST(kBits, true); else PCX_PERSIST(kBits, false); break;
     default: return hipErrorInvalidValue;
   }
 #undef PCX_PERSIST

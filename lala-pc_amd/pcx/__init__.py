@@ -79,6 +79,16 @@ class GenParams(ctypes.Structure):
                 ("reserved", ctypes.c_int32 * 3)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("props_per_wave", ctypes.c_int32),
+                ("lds_mode", ctypes.c_int32), ("compact_tables", ctypes.c_int32),
+                ("stream_tables", ctypes.c_int32), ("event_driven", ctypes.c_int32),
+                ("ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("max_slots", ctypes.c_int32),
                 ("arena_bytes", ctypes.c_int64), ("sweeps_per_tick", ctypes.c_int32),
@@ -116,6 +126,8 @@ SIGNATURES = [
     ("pcx_destroy", None, [_VP]),
     ("pcx_tell", ctypes.c_int32, [_VP, _P32, _P32, _P32, _P32, ctypes.c_int64, _P32, _P32, ctypes.c_int64]),
     ("pcx_num_deductions", ctypes.c_int64, [_VP]),
+    ("pcx_get_config", ctypes.c_int32, [_VP, ctypes.POINTER(Config)]),
+    ("pcx_set_config", ctypes.c_int32, [_VP, ctypes.POINTER(Config)]),
     ("pcx_get_table", ctypes.c_int32, [_VP, _P32, _P32, _P32, _P32]),
     ("pcx_snapshot", ctypes.c_int64, [_VP]),
     ("pcx_restore", ctypes.c_int32, [_VP, ctypes.c_int64]),
@@ -127,6 +139,8 @@ SIGNATURES = [
     ("pcx_batch_reset", ctypes.c_int32, [_VP]),
     ("pcx_batch_read", ctypes.c_int32, [_VP, _P32, _P32]),
     ("pcx_batch_run", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Result), ctypes.POINTER(Stats)]),
+    ("pcx_batch_autotune", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Tuning),
+                                            ctypes.POINTER(Tuning), ctypes.c_int32, _P32]),
     ("pcx_bits_tell", ctypes.c_int32, [_VP, _P32, _P32, _P32, _P32, ctypes.c_int64, _PU64, ctypes.c_int64]),
     ("pcx_bits_fixpoint", ctypes.c_int32, [_VP, _PU64, ctypes.c_int64, ctypes.POINTER(Result)]),
     ("pcx_bits_batch_load", ctypes.c_int32, [_VP, _PU64]),
@@ -249,6 +263,19 @@ class PIR:
     def num_deductions(self):
         return int(lib().pcx_num_deductions(self._h))
 
+    def get_config(self):
+        c = Config()
+        _check(lib().pcx_get_config(self._h, ctypes.byref(c)))
+        return c
+
+    def set_config(self, **kw):
+        """Change tuning fields (props_per_wave, lds_mode, compact_tables, ...)."""
+        c = self.get_config()
+        for k, v in kw.items():
+            setattr(c, k, v)
+        _check(lib().pcx_set_config(self._h, ctypes.byref(c)))
+        self.config = c
+
     def table(self):
         n = self.num_deductions()
         out = [np.zeros(n, np.int32) for _ in range(4)]
@@ -354,6 +381,16 @@ class Batch:
         _check(lib().pcx_batch_run(self._h, mode, res, ctypes.byref(st)))
         return list(res), st
 
+    def autotune(self, reps=2):
+        """Time every sweep configuration on the loaded image, configure the
+        engine with the fastest; returns (best, [all points]) as dicts."""
+        cap = 128
+        allp = (Tuning * cap)()
+        best = Tuning()
+        n = ctypes.c_int32(0)
+        _check(lib().pcx_batch_autotune(self._h, reps, ctypes.byref(best), allp, cap, ctypes.byref(n)))
+        return best.as_dict(), [allp[i].as_dict() for i in range(min(n.value, cap))]
+
 
 class Server:
     """Continuous-batching fixpoint server over registered PIR networks.
@@ -429,6 +466,52 @@ class Server:
                                        _p32(net), _p32(lb_all), _p32(ub_all), _p64(store_off),
                                        1 if plain else 0, out, ctypes.byref(wall)))
         return list(out)[:n], wall.value
+
+
+def default_tuning_path():
+    """$PCX_TUNING, else ~/.cache/pcx/tuning.json."""
+    p = os.environ.get("PCX_TUNING")
+    if p:
+        return p
+    d = os.path.join(os.path.expanduser("~"), ".cache", "pcx")
+    try:
+        os.makedirs(d, exist_ok=True)
+    except OSError:
+        import tempfile
+        d = tempfile.gettempdir()
+    return os.path.join(d, "tuning.json")
+
+
+def load_tuning(path, key):
+    """A tuning saved by save_tuning for `key`, or None."""
+    import json
+    try:
+        with open(path) as f:
+            return json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def save_tuning(path, key, best, points=None):
+    """Persist an autotuning result as JSON under `key` (other keys kept)."""
+    import json
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        data = {}
+    data[key] = {"best": best, "points": points or []}
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1)
+    os.replace(tmp, path)
+
+
+def apply_tuning(pir, best):
+    """Configure the engine with a saved tuning; returns the mode to run."""
+    pir.set_config(**{k: best[k] for k in ("props_per_wave", "lds_mode", "compact_tables",
+                                           "stream_tables", "event_driven")})
+    return best["mode"]
 
 
 # ---- synthetic workloads (host code, no GPU) ------------------------------

@@ -121,3 +121,14 @@ def test_bits_splits_remove_values():
     out = pcx.gen_bits_splits(2, w, 40, 1, 10).reshape(40, 3)
     assert ((out & ~w) == 0).all() and (out != 0).all()
     assert (out[:, 1] == 1).all()
+
+
+def test_tuning_cache_round_trip(tmp_path):
+    path = str(tmp_path / "t.json")
+    assert pcx.load_tuning(path, "a") is None
+    best = {"mode": 1, "props_per_wave": 256, "lds_mode": 0, "compact_tables": 1,
+            "stream_tables": 1, "event_driven": 1, "ms": 3.5}
+    pcx.save_tuning(path, "a", best, [best])
+    pcx.save_tuning(path, "b", dict(best, mode=2))
+    assert pcx.load_tuning(path, "a")["best"] == best
+    assert pcx.load_tuning(path, "b")["best"]["mode"] == 2

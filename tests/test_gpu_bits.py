@@ -97,3 +97,21 @@ def test_bits_batch_of_splits(gpu, mode, stream, events):
     b.load_words(inst)
     rg, _ = b.run(mode)
     _compare(b.read_words(), rg, wo, ro, V, f"bits batch mode {mode}")
+
+
+@pytest.mark.parametrize("lds", [1, 0])
+def test_bits_persistent_lds(gpu, lds):
+    V, P, n = 2000, 8000, 48
+    op, x, y, z, w = pcx.gen_bits_network(31, V, P)
+    pir = pcx.BitsPIR(lds_mode=lds)
+    pir.tell(op, x, y, z, w)
+    root = w.copy()
+    assert not pir.fixpoint(root).failed
+    inst = pcx.gen_bits_splits(32, root, n, 1, 40)
+    top, tx, ty, tz = pir.table()
+    wo = inst.copy()
+    ro = ol.bits_fixpoint_batch(top, tx, ty, tz, wo, V, threads=8)
+    b = pcx.Batch(pir, n, V)
+    b.load_words(inst)
+    rg, _ = b.run(pcx.MODE_PERSISTENT)
+    _compare(b.read_words(), rg, wo, ro, V, f"bits persistent lds {lds}")

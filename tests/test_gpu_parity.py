@@ -230,3 +230,60 @@ def test_edge_cases(gpu):
     assert e.value.status == pcx.EINVAL
     with pytest.raises(pcx.PcxError):
         pir3.tell([42], [0], [1], [2])
+
+
+@pytest.mark.parametrize("lds", [1, 0, -1], ids=["lds", "global", "auto"])
+def test_persistent_lds_resident_stores(gpu, lds):
+    """Small instances (3,000 variables = 24 KB stores) in the persistent mode
+    with the stores staged in LDS or left in global memory."""
+    V, P, n_inst = 3000, 12000, 64
+    op, x, y, z, lb, ub, w = _network(21, V, P)
+    pir = pcx.PIR(lds_mode=lds)
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_lb, root_ub).failed
+    slb, sub = pcx.gen_splits(22, root_lb, root_ub, n_inst, 1, 20)
+    off, idx = pcx.gen_subsets(23, P, n_inst, 100, P)
+    top, tx, ty, tz = pir.table()
+    lb_o, ub_o = slb.copy(), sub.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lb_o, ub_o, V, idx=idx.astype(np.int64), off=off, threads=8)
+    b = pcx.Batch(pir, n_inst, V, off, idx)
+    b.load(slb, sub)
+    res_g, _ = b.run(pcx.MODE_PERSISTENT)
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"persistent lds_mode {lds}")
+
+
+def test_autotuner_choice_is_bit_exact(gpu, tmp_path):
+    """The autotuner tries every sweep configuration on the loaded batch and
+    leaves the engine configured with the fastest; results stay bit-exact and
+    the choice round-trips through the JSON cache."""
+    V, P, n_inst = 3000, 12000, 48
+    op, x, y, z, lb, ub, w = _network(41, V, P)
+    pir = pcx.PIR()
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_lb, root_ub).failed
+    slb, sub = pcx.gen_splits(42, root_lb, root_ub, n_inst, 1, 20)
+    off, idx = pcx.gen_subsets(43, P, n_inst, 100, P)
+    top, tx, ty, tz = pir.table()
+    lb_o, ub_o = slb.copy(), sub.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lb_o, ub_o, V, idx=idx.astype(np.int64), off=off, threads=8)
+    b = pcx.Batch(pir, n_inst, V, off, idx)
+    b.load(slb, sub)
+    best, points = b.autotune(reps=1)
+    assert len(points) == 32 + 4          # batched grid + persistent x {LDS, global} x {index, copy}
+    assert all(p["ms"] > 0 for p in points)
+    assert best["ms"] == min(p["ms"] for p in points)
+    c = pir.get_config()
+    assert (c.props_per_wave, c.lds_mode, c.compact_tables) == \
+        (best["props_per_wave"], best["lds_mode"], best["compact_tables"])
+    res_g, _ = b.run(best["mode"])
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"autotuned {best}")
+    path = str(tmp_path / "tuning.json")
+    pcx.save_tuning(path, "k", best, points)
+    assert pcx.load_tuning(path, "k")["best"] == best
+    pir2 = pcx.PIR()
+    assert pcx.apply_tuning(pir2, best) == best["mode"]
+    assert pir2.get_config().event_driven == best["event_driven"]
