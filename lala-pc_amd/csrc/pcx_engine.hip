@@ -276,6 +276,10 @@ pcx_status ensure_adjacency(pcx_engine* e) {
 }
 
 int64_t bitmap_words(const pcx_batch* b) { return (b->table_n + 31) / 32; }
+int64_t summary_words(const pcx_batch* b) {
+  const int64_t chunks = (bitmap_words(b) + kChunkWords - 1) / kChunkWords;
+  return (chunks + 31) / 32;
+}
 
 // Event-driven sweeps need the adjacency, the member bitmaps of subset
 // batches and two dirty bitmaps per instance; automatic mode uses them when
@@ -296,6 +300,8 @@ pcx_status prepare_events(pcx_batch* b, int32_t mode) {
   pcx_status st = ensure_adjacency(e);
   if (st != PCX_OK) return st;
   if (!b->d_dirty) PCX_HIP(hipMalloc(&b->d_dirty, 2 * rows * sizeof(uint32_t)));
+  const int64_t srows = b->n_inst * summary_words(b);
+  if (!b->d_summary) PCX_HIP(hipMalloc(&b->d_summary, 3 * srows * sizeof(uint32_t)));
   if (!b->h_off.empty() && (!b->d_member || b->member_version != e->version)) {
     if (!b->d_member) PCX_HIP(hipMalloc(&b->d_member, rows * sizeof(uint32_t)));
     PCX_HIP(hipMemsetAsync(b->d_member, 0, rows * sizeof(uint32_t), e->stream));
@@ -303,13 +309,16 @@ pcx_status prepare_events(pcx_batch* b, int32_t mode) {
     b->member_version = e->version;
   }
   PCX_HIP(hipMemsetAsync(b->d_dirty, 0, 2 * rows * sizeof(uint32_t), e->stream));
+  PCX_HIP(hipMemsetAsync(b->d_summary, 0, 2 * srows * sizeof(uint32_t), e->stream));
   b->events = true;
   return PCX_OK;
 }
 
 EventSet event_set(const pcx_batch* b) {
+  const int64_t srows = b->n_inst * summary_words(b);
   return EventSet{b->e->d_adj_off, b->e->d_adj, b->h_off.empty() ? nullptr : b->d_member,
-                  b->d_dirty, bitmap_words(b), b->n_inst};
+                  b->d_dirty, b->d_summary, b->d_summary ? b->d_summary + 2 * srows : nullptr,
+                  bitmap_words(b), summary_words(b), b->n_inst};
 }
 
 // Table and subsets a sweep kernel reads for this batch.
@@ -345,7 +354,9 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
   const int group = e->cfg.sweeps_per_check > 0 ? e->cfg.sweeps_per_check : 8;
   const Table t = table_of(e);
-  const EventSet ev = event_set(b);
+  EventSet ev = event_set(b);
+  ev.summary = nullptr;               // plain marked sweeps scan every chunk
+  ev.snap = nullptr;
   SweepTimer tm{b, e->cfg.timing != 0};
   InstFlags hf;
   for (int64_t k = 0; k < b->n_inst; ++k) {
@@ -403,10 +414,9 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   int32_t live = 1, sweeps = 0;
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
   const EventSet ev = event_set(b);
-  const int64_t marked_items = (bitmap_words(b) + kMarkedWordsPerItem - 1) / kMarkedWordsPerItem;
   auto launch_marked = [&]() -> pcx_status {
     PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
-                            marked_items));
+                            &ev));
     PCX_HIP(launch_sweep_marked(table_of(e), b->d_store, b->nvars, b->d_flags, wl,
                                 b->d_tight + kTightSlots, ev, dom_of(e), grid, e->stream));
     return PCX_OK;
@@ -448,7 +458,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     for (int g = 0; g < group; ++g) {
       const bool marked = b->events && sweeps > 0;
       PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
-                              marked ? marked_items : 0));
+                              marked ? &ev : nullptr));
       pcx_status ts = tm.begin(e->stream);
       if (ts != PCX_OK) return ts;
       if (marked)
@@ -738,6 +748,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   dfree(b->d_store); dfree(b->d_image); dfree(b->d_off); dfree(b->d_idx); dfree(b->d_flags);
   dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl); dfree(b->d_order);
   dfree(b->d_cxo); dfree(b->d_cy); dfree(b->d_cz); dfree(b->d_member); dfree(b->d_dirty);
+  dfree(b->d_summary);
   if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
   delete b;
 }

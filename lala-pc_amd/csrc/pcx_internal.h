@@ -50,6 +50,7 @@ struct pcx_batch {
   // event-driven sweeps: member bitmaps (subsets only) and two dirty bitmaps
   uint32_t* d_member = nullptr;
   uint32_t* d_dirty = nullptr;
+  uint32_t* d_summary = nullptr;      // chunk summaries (2 planes) + scheduler snapshot
   uint64_t member_version = 0;
   bool events = false;                // this run uses marked-only sweeps
   // captured group of marked sweeps (batched mode), rebuilt when its key changes

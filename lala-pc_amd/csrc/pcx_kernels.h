@@ -52,12 +52,23 @@ struct InstFlags {
 // when member == nullptr) in the bitmap of the next sweep.  Bitmaps are
 // indexed by table position, `words` 32-bit words per instance, two per
 // instance (sweep parity): dirty[(s & 1) * n_inst * words + k * words ...].
+//
+// A summary bitmap per instance and parity has one bit per chunk of
+// kChunkWords bitmap words (kChunkProps propagators) that may hold marks:
+// summary[(s & 1) * n_inst * swords + k * swords ...].  The batched scheduler
+// copies an instance's summary to snap (and clears it) when it schedules a
+// marked sweep, so the sweep's work items are the chunks set in snap.
+constexpr int kChunkWords = 256;
+constexpr int kChunkShift = 13;            // propagators per chunk = 1 << 13
 struct EventSet {
   const int32_t* adj_off;   // [nvars + 1]
   const int32_t* adj;       // [adj_off[nvars]]
   const uint32_t* member;   // [n_inst * words] or nullptr
   uint32_t* dirty;          // [2 * n_inst * words], or nullptr = no marking
+  uint32_t* summary;        // [2 * n_inst * swords]
+  uint32_t* snap;           // [n_inst * swords]
   int64_t words;
+  int64_t swords;
   int64_t n_inst;
 };
 
@@ -116,14 +127,15 @@ hipError_t launch_sweep_marked(const Table& t, int2* store, int64_t nvars, InstF
 // member bitmaps from subsets
 hipError_t launch_build_members(const int64_t* off, const int32_t* idx, int64_t n_inst,
                                 uint32_t* member, int64_t words, hipStream_t st);
+// marked != nullptr: schedule a marked sweep (items = chunks with marks).
 hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags, int32_t n_inst,
                            WorkList wl, int ppw, int32_t max_sweeps, hipStream_t st,
-                           int64_t fixed_items = 0);
+                           const EventSet* marked = nullptr);
 hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                                 int64_t n_props, InstFlags* flags, WorkList wl,
                                 unsigned long long* tight, int ppw, int dom, bool stream_table,
                                 int grid, hipStream_t st, const EventSet* ev = nullptr);
-constexpr int kMarkedWordsPerItem = 256;   // bitmap words per marked-sweep item
+constexpr int kMarkedWordsPerItem = kChunkWords;   // bitmap words per marked-sweep item
 // lds: stage each instance's store in LDS (nvars * 8 <= kLdsStoreBytes).
 constexpr int64_t kLdsStoreBytes = 64 * 1024;
 hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,

@@ -92,7 +92,16 @@ struct Marker {
   uint32_t* also_now;       // instance's row of this sweep (marked sweeps), or nullptr
   int32_t* list;            // workgroup's deferred variables (LDS), or nullptr
   int32_t* list_n;
+  uint32_t* sum;            // summary row of the next sweep
+  uint32_t* sum_now;        // summary row of this sweep (with also_now)
+  uint32_t* lsum;           // LDS accumulators of sum / sum_now during a flush, or nullptr
+  uint32_t* lsum_now;
+  int32_t swords;
 };
+
+// Summary words accumulated in LDS during a flush (larger summaries are
+// marked in global memory directly).
+constexpr int kLdsSummaryWords = 64;
 
 // Deferred marking: a tightened variable is pushed onto the workgroup's LDS
 // list and the whole workgroup marks the list's adjacency together at the
@@ -102,13 +111,32 @@ constexpr int kMarkCap = 1024;
 
 __device__ __forceinline__ Marker marker_for(const EventSet* ev, int64_t k, int32_t s,
                                              bool marked_sweep = false) {
-  if (!ev || !ev->dirty) return Marker{nullptr, nullptr, nullptr, nullptr, nullptr};
-  const int64_t row = k * ev->words;
-  const int64_t plane = ev->n_inst * ev->words;
-  return Marker{ev->adj_off, ev->adj, ev->member ? ev->member + row : nullptr,
-                ev->dirty + (int64_t)((s + 1) & 1) * plane + row,
-                marked_sweep ? ev->dirty + (int64_t)(s & 1) * plane + row : nullptr, nullptr,
-                nullptr};
+  if (!ev || !ev->dirty) return Marker{};
+  const int64_t row = k * ev->words, srow = k * ev->swords;
+  const int64_t plane = ev->n_inst * ev->words, splane = ev->n_inst * ev->swords;
+  const int64_t nx = (s + 1) & 1, cu = s & 1;
+  const bool sums = ev->summary != nullptr;
+  Marker mk{};
+  mk.adj_off = ev->adj_off;
+  mk.adj = ev->adj;
+  mk.member = ev->member ? ev->member + row : nullptr;
+  mk.dirty = ev->dirty + nx * plane + row;
+  mk.also_now = marked_sweep ? ev->dirty + cu * plane + row : nullptr;
+  mk.sum = sums ? ev->summary + nx * splane + srow : nullptr;
+  mk.sum_now = sums && marked_sweep ? ev->summary + cu * splane + srow : nullptr;
+  mk.swords = (int32_t)ev->swords;
+  return mk;
+}
+
+__device__ __forceinline__ void set_summary(uint32_t* lds, uint32_t* glob, int32_t c) {
+  const uint32_t cbit = 1u << (c & 31);
+  if (lds) {
+    atomicOr(&lds[c >> 5], cbit);
+  } else if (glob &&
+             !(__hip_atomic_load(&glob[c >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & cbit)) {
+    // a summary word covers 32 chunks: write it only when its bit is unset
+    atomicOr(&glob[c >> 5], cbit);
+  }
 }
 
 // Mark the propagators adjacent to v: adjacency entries are loaded eight at
@@ -130,8 +158,13 @@ __device__ __noinline__ void mark_var(const Marker mk, int32_t v) {
       if (p[j] < 0) continue;
       const uint32_t bit = 1u << (p[j] & 31);
       if (!(in[j] & bit)) continue;
+      const int32_t c = p[j] >> kChunkShift;
       atomicOr(&mk.dirty[p[j] >> 5], bit);
-      if (mk.also_now) atomicOr(&mk.also_now[p[j] >> 5], bit);
+      if (mk.sum) set_summary(mk.lsum, mk.sum, c);
+      if (mk.also_now) {
+        atomicOr(&mk.also_now[p[j] >> 5], bit);
+        if (mk.sum_now) set_summary(mk.lsum_now, mk.sum_now, c);
+      }
     }
   }
 }
@@ -144,13 +177,30 @@ __device__ __forceinline__ void defer_mark(const Marker& mk, int32_t v) {
   mark_var(mk, v);                          // list full: mark directly
 }
 
-// Every thread of the workgroup calls this at the end of a work item.
+// Every thread of the workgroup calls this at the end of a work item.  The
+// summary bits of the flushed marks are gathered in LDS and written with
+// one atomic per summary word.
 __device__ __forceinline__ void flush_marks(const Marker& mk) {
   if (!mk.list) return;
+  __shared__ uint32_t s_sum[2][kLdsSummaryWords];
+  const bool lds_sum = mk.sum && mk.swords <= kLdsSummaryWords;
+  if (lds_sum)
+    for (int j = threadIdx.x; j < 2 * kLdsSummaryWords; j += blockDim.x) s_sum[j >> 6][j & 63] = 0;
   __syncthreads();
+  Marker m2 = mk;
+  if (lds_sum) {
+    m2.lsum = s_sum[0];
+    m2.lsum_now = mk.sum_now ? s_sum[1] : nullptr;
+  }
   const int n = min(*mk.list_n, kMarkCap);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) mark_var(mk, mk.list[i]);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) mark_var(m2, mk.list[i]);
   __syncthreads();
+  if (lds_sum) {
+    for (int j = threadIdx.x; j < mk.swords; j += blockDim.x) {
+      if (s_sum[0][j]) atomicOr(&mk.sum[j], s_sum[0][j]);
+      if (mk.sum_now && s_sum[1][j]) atomicOr(&mk.sum_now[j], s_sum[1][j]);
+    }
+  }
   if (threadIdx.x == 0) *mk.list_n = 0;     // ordered before reuse by the caller's barrier
 }
 
@@ -369,7 +419,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked_plain(Table t, int2* __
 hipError_t launch_sweep_marked_plain(const Table& t, int2* store, int32_t sweep, InstFlags* flags,
                                      unsigned long long* tight, const EventSet& ev, int64_t inst,
                                      int dom, hipStream_t stream) {
-  int64_t grid = (ev.words + kBlock - 1) / kBlock;
+  static_assert(kChunkWords == kBlock, "one bitmap word per thread");
+  int64_t grid = (ev.words + kChunkWords - 1) / kChunkWords;
   if (grid > 2048) grid = 2048;
   if (grid <= 0) return hipSuccess;
   switch (dom) {
@@ -391,7 +442,7 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
                                                             InstFlags* flags, int32_t n_inst,
                                                             WorkList wl, int64_t chunk,
                                                             int32_t max_sweeps,
-                                                            int64_t fixed_items) {
+                                                            EventSet ev, bool marked) {
   __shared__ int64_t part[kSchedThreads];
   __shared__ int32_t cnt[kSchedThreads];
   const int32_t s = *wl.sweep + 1;       // the sweep being scheduled
@@ -409,8 +460,19 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
       if (f.done_at < 0) f.done_at = s;
       continue;
     }
-    const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
-    items += fixed_items > 0 ? fixed_items : (m + chunk - 1) / chunk;
+    if (marked) {            // the chunks with marks become the items; snapshot and clear
+      uint32_t* srow = ev.summary + (int64_t)(s & 1) * ev.n_inst * ev.swords + (int64_t)k * ev.swords;
+      uint32_t* snap = ev.snap + (int64_t)k * ev.swords;
+      for (int64_t j = 0; j < ev.swords; ++j) {
+        const uint32_t v = srow[j];
+        snap[j] = v;
+        srow[j] = 0;
+        items += __builtin_popcount(v);
+      }
+    } else {
+      const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
+      items += (m + chunk - 1) / chunk;
+    }
     ++na;
   }
   part[tid] = items;
@@ -430,10 +492,15 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
   for (int k = lo; k < hi; ++k) {
     const InstFlags& f = flags[k];
     if (f.done_at >= 0 && f.done_at <= s) continue;
-    const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
     wl.act[ia] = k;
     wl.item_base[ia] = ib;
-    ib += fixed_items > 0 ? fixed_items : (m + chunk - 1) / chunk;
+    if (marked) {
+      const uint32_t* snap = ev.snap + (int64_t)k * ev.swords;
+      for (int64_t j = 0; j < ev.swords; ++j) ib += __builtin_popcount(snap[j]);
+    } else {
+      const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
+      ib += (m + chunk - 1) / chunk;
+    }
     ++ia;
   }
   __syncthreads();
@@ -447,10 +514,10 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
 
 hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags, int32_t n_inst,
                            WorkList wl, int ppw, int32_t max_sweeps, hipStream_t stream,
-                           int64_t fixed_items) {
+                           const EventSet* marked) {
   const int64_t chunk = (int64_t)kBlock * (ppw / kWave);
   k_schedule<<<1, kSchedThreads, 0, stream>>>(sub, n_props, flags, n_inst, wl, chunk, max_sweeps,
-                                              fixed_items);
+                                              marked ? *marked : EventSet{}, marked != nullptr);
   return hipGetLastError();
 }
 
@@ -594,13 +661,26 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
     }
     const int32_t k = wl.act[lo];
     InstFlags* f = flags + k;
-    const int64_t w = (item - wl.item_base[lo]) * kMarkedWordsPerItem + threadIdx.x;
+    // the item's chunk: the r-th set bit of the instance's snapshot
+    int64_t r = item - wl.item_base[lo], chunk = -1;
+    const uint32_t* snap = ev.snap + (int64_t)k * ev.swords;
+    for (int64_t j = 0; j < ev.swords; ++j) {
+      uint32_t v = snap[j];
+      const int c = __builtin_popcount(v);
+      if (r < c) {
+        for (int q = 0; q < r; ++q) v &= v - 1;
+        chunk = j * 32 + __builtin_ctz(v);
+        break;
+      }
+      r -= c;
+    }
+    const int64_t w = chunk * kMarkedWordsPerItem + threadIdx.x;
     Marker mk = marker_for(&ev, k, s, true);
     mk.list = s_list;
     mk.list_n = &s_n;
     bool chg = false, fail = false;
     int nt = 0, ne = 0;
-    if (f->failed_at == kNoFail && w < ev.words)
+    if (f->failed_at == kNoFail && chunk >= 0 && w < ev.words)
       sweep_marked_word<D>(t, store + (int64_t)k * nvars, cur + (int64_t)k * ev.words, w, mk, chg,
                            fail, nt, ne);
     publish(f, s, chg, fail, nt, tight);
@@ -848,7 +928,7 @@ __global__ __launch_bounds__(kPBlock) void k_persistent(Table t, int2* __restric
           const u64 a1 = load_word<WG>(st + x1), b1 = load_word<WG>(st + y1), c1 = load_word<WG>(st + z1);
           u64 a2 = 0, b2 = 0, c2 = 0;
           if (i2 >= 0) { a2 = load_word<WG>(st + x2); b2 = load_word<WG>(st + y2); c2 = load_word<WG>(st + z2); }
-          const Marker none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+          const Marker none{};
           apply<D, WG>(st, (int)(xo1 >> kOpShift), x1, y1, z1, a1, b1, c1, chg, fail, nt, none);
           if (i2 >= 0) apply<D, WG>(st, (int)(xo2 >> kOpShift), x2, y2, z2, a2, b2, c2, chg, fail, nt, none);
         }
