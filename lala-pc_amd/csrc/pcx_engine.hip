@@ -341,8 +341,16 @@ pcx_status prepare_run(pcx_batch* b, int32_t mode) {
   if (s != PCX_OK) return s;
   PCX_HIP(hipMemsetAsync(b->d_tight, 0, 2 * kTightSlots * sizeof(unsigned long long), e->stream));
   const WorkList wl = worklist_of(b->d_act, b->d_item_base, b->d_ctl);
-  PCX_HIP(launch_init_flags(b->d_flags, (int32_t)b->n_inst, wl, e->stream));
-  PCX_HIP(launch_store_check_empty(b->d_store, b->nvars, (int32_t)b->n_inst, b->d_flags, dom_of(e), e->stream));
+  // A store already empty is failed before any sweep; when the stores are the
+  // image, its emptiness was found at load time.
+  if (b->store_is_image) {
+    PCX_HIP(launch_init_flags(b->d_flags, (int32_t)b->n_inst, wl, e->stream, b->d_image_empty));
+  } else {
+    PCX_HIP(launch_init_flags(b->d_flags, (int32_t)b->n_inst, wl, e->stream));
+    PCX_HIP(launch_store_check_empty(b->d_store, b->nvars, (int32_t)b->n_inst, b->d_flags, dom_of(e),
+                                     e->stream));
+  }
+  b->store_is_image = false;
   pcx_status st = prepare_tables(b);
   if (st != PCX_OK) return st;
   return prepare_events(b, mode);
@@ -705,6 +713,7 @@ pcx_status pcx_batch_create(pcx_engine* e, int64_t n_inst, int64_t nvars, const 
   chk(hipMalloc(&b->d_store, sbytes));
   chk(hipMalloc(&b->d_image, sbytes));
   chk(hipMalloc(&b->d_flags, n_inst * sizeof(InstFlags)));
+  chk(hipMalloc(&b->d_image_empty, n_inst));
   chk(hipMalloc(&b->d_tight, 2 * kTightSlots * sizeof(unsigned long long)));
   chk(hipMalloc(&b->d_act, n_inst * sizeof(int32_t)));
   chk(hipMalloc(&b->d_item_base, (n_inst + 1) * sizeof(int64_t)));
@@ -748,10 +757,23 @@ void pcx_batch_destroy(pcx_batch* b) {
   dfree(b->d_store); dfree(b->d_image); dfree(b->d_off); dfree(b->d_idx); dfree(b->d_flags);
   dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl); dfree(b->d_order);
   dfree(b->d_cxo); dfree(b->d_cy); dfree(b->d_cz); dfree(b->d_member); dfree(b->d_dirty);
-  dfree(b->d_summary);
+  dfree(b->d_summary); dfree(b->d_image_empty);
   if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
   delete b;
 }
+
+namespace {
+// After a load: which instances' images are empty, and the stores are the image.
+pcx_status image_loaded(pcx_batch* b) {
+  pcx_engine* e = b->e;
+  PCX_HIP(hipMemsetAsync(b->d_image_empty, 0, b->n_inst, e->stream));
+  PCX_HIP(launch_store_empty_map(b->d_image, b->nvars, (int32_t)b->n_inst, b->d_image_empty, dom_of(e),
+                                 e->stream));
+  PCX_HIP(hipStreamSynchronize(e->stream));
+  b->store_is_image = true;
+  return PCX_OK;
+}
+}  // namespace
 
 pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub) {
   if (!b || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
@@ -766,8 +788,7 @@ pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub) {
   PCX_HIP(hipMemcpyAsync(d_ub, ub, n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream));
   PCX_HIP(launch_store_pack(d_lb, d_ub, b->d_image, n, e->stream));
   PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, n * sizeof(int2), hipMemcpyDeviceToDevice, e->stream));
-  PCX_HIP(hipStreamSynchronize(e->stream));
-  return PCX_OK;
+  return image_loaded(b);
 }
 
 pcx_status pcx_batch_reset(pcx_batch* b) {
@@ -775,6 +796,7 @@ pcx_status pcx_batch_reset(pcx_batch* b) {
   pcx_engine* e = b->e;
   PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, b->n_inst * b->nvars * sizeof(int2),
                          hipMemcpyDeviceToDevice, e->stream));
+  b->store_is_image = true;
   return PCX_OK;
 }
 
@@ -979,8 +1001,7 @@ pcx_status pcx_bits_batch_load(pcx_batch* b, const uint64_t* words) {
   const int64_t n = b->n_inst * b->nvars;
   PCX_HIP(hipMemcpyAsync(b->d_image, words, n * sizeof(uint64_t), hipMemcpyHostToDevice, e->stream));
   PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, e->stream));
-  PCX_HIP(hipStreamSynchronize(e->stream));
-  return PCX_OK;
+  return image_loaded(b);
 }
 
 pcx_status pcx_bits_batch_read(pcx_batch* b, uint64_t* words) {

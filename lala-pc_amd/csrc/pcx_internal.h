@@ -36,6 +36,8 @@ struct pcx_batch {
   int64_t table_n = 0;                // table size the subsets were built for
   int2* d_store = nullptr;
   int2* d_image = nullptr;
+  uint8_t* d_image_empty = nullptr;   // per instance: the image holds an empty domain
+  bool store_is_image = false;        // the stores equal the image (after load / reset)
   int64_t* d_off = nullptr;
   int32_t* d_idx = nullptr;
   std::vector<int64_t> h_off;         // empty = whole table

@@ -144,7 +144,11 @@ hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const S
                              int32_t max_sweeps, int dom, int grid, bool lds, hipStream_t st);
 // resident workgroups per CU (lds_bytes of dynamic LDS per workgroup)
 int persistent_blocks_per_cu(int dom, int64_t lds_bytes);
-hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipStream_t st);
+// empty (may be nullptr): per-instance bytes, 1 = the store starts empty
+hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipStream_t st,
+                             const uint8_t* empty = nullptr);
+hipError_t launch_store_empty_map(const int2* store, int64_t nvars, int32_t n_inst, uint8_t* empty,
+                                  int dom, hipStream_t st);
 hipError_t launch_schedule_slots(const SlotDesc* slots, InstFlags* flags, int32_t n_slots,
                                  WorkList wl, int ppw, hipStream_t st);
 hipError_t launch_sweep_slots(const SlotDesc* slots, InstFlags* flags, WorkList wl,

@@ -994,12 +994,13 @@ hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const S
 }
 
 // ---- bookkeeping kernels --------------------------------------------------
-__global__ void k_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl) {
+__global__ void k_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl,
+                             const uint8_t* __restrict__ empty) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n_inst) {
     flags[k].changed[0] = -2;
     flags[k].changed[1] = -1;      // "sweep -1 changed" lets sweep 0 run
-    flags[k].failed_at = kNoFail;
+    flags[k].failed_at = (empty && empty[k]) ? -1 : kNoFail;
     flags[k].done_at = -1;
     flags[k].marked_evals = 0;
   }
@@ -1011,9 +1012,35 @@ __global__ void k_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl) {
   }
 }
 
-hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipStream_t stream) {
+hipError_t launch_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl, hipStream_t stream,
+                             const uint8_t* empty) {
   const int grid = (int)((n_inst + 255) / 256);
-  k_init_flags<<<grid > 0 ? grid : 1, 256, 0, stream>>>(flags, n_inst, wl);
+  k_init_flags<<<grid > 0 ? grid : 1, 256, 0, stream>>>(flags, n_inst, wl, empty);
+  return hipGetLastError();
+}
+
+// empty[k] = 1 when instance k's slice of `store` holds an empty domain
+// (empty must be zeroed first).
+template <bool BITS>
+__global__ void k_mark_empty(const int2* __restrict__ store, int64_t nvars, int64_t total,
+                             uint8_t* empty) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < total;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int2 b = store[v];
+    if (BITS ? (b.x == 0 && b.y == 0) : (b.x > b.y)) empty[v / nvars] = 1;
+  }
+}
+
+hipError_t launch_store_empty_map(const int2* store, int64_t nvars, int32_t n_inst, uint8_t* empty,
+                                  int dom, hipStream_t stream) {
+  const int64_t total = nvars * (int64_t)n_inst;
+  int64_t grid = (total + 255) / 256;
+  if (grid > 16384) grid = 16384;
+  if (grid <= 0) return hipSuccess;
+  if (dom == kBits)
+    k_mark_empty<true><<<dim3((unsigned)grid), 256, 0, stream>>>(store, nvars, total, empty);
+  else
+    k_mark_empty<false><<<dim3((unsigned)grid), 256, 0, stream>>>(store, nvars, total, empty);
   return hipGetLastError();
 }
 
