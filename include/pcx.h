@@ -84,9 +84,10 @@ typedef struct {
                                own copy of its subset (12 B per active propagator)  */
   int32_t stream_tables;    /* -1 auto (stream per-instance tables only), 0 cached
                                table loads, 1 non-temporal table loads               */
-  int32_t event_driven;     /* -1 auto, 0 every sweep evaluates every propagator,
-                               1 sweeps after the first evaluate only propagators
-                               with an operand tightened in the previous sweep       */
+  int32_t event_driven;     /* -1 auto (batched mode; plain mode from 2^20
+                               propagators per instance), 0 every sweep evaluates
+                               every propagator, 1 sweeps after the first evaluate
+                               only propagators with an operand tightened since     */
   int32_t reserved[3];
 } pcx_config;
 
@@ -254,7 +255,11 @@ typedef struct {
   int32_t sweeps_per_tick;   /* sweeps launched between host checks           */
   int32_t props_per_wave;    /* 64..512; 0 = 512                              */
   int64_t stage_bytes;       /* pinned host staging for store copies          */
-  int32_t reserved[4];
+  int32_t event_driven;      /* 0 full sweeps only, 1 after the first sweep
+                                evaluate only marked propagators (their bitmaps
+                                share the request's arena block), -1 auto: 1 for
+                                requests of at least 2^17 propagators          */
+  int32_t reserved[3];
 } pcx_server_config;
 
 typedef struct {
@@ -296,8 +301,9 @@ int64_t    pcx_server_poll(pcx_server* s, pcx_served* out, int64_t max);
  * arrival_s[i] seconds after the start (ascending), runs network net[i] on
  * store slice [store_off[i], store_off[i+1]) of lb_all/ub_all (overwritten
  * with the fixpoints), and out[i] receives its result and latency.
- * plain = 1 serves the same trace one request at a time, each with its own
- * sweep launches (FIFO), as the comparison point.  *wall_s = whole replay. */
+ * plain = 1 serves the same trace one request at a time (FIFO), each with
+ * pcx_fixpoint on its network's engine, as the comparison point.
+ * *wall_s = whole replay. */
 pcx_status pcx_server_replay(pcx_server* s, int64_t n, const double* arrival_s,
                              const int32_t* net, int32_t* lb_all, int32_t* ub_all,
                              const int64_t* store_off, int32_t plain, pcx_served* out,

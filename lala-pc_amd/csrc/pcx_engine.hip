@@ -86,9 +86,37 @@ int dom_of(const pcx_engine* e) {
   return e->has_div ? kItvDiv : kItv;
 }
 
+// Variable -> table positions adjacency of the engine's table (each
+// propagator listed once per distinct variable), built on the host when the
+// table changed and uploaded.
+pcx_status ensure_adjacency(pcx_engine* e) {
+  if (e->d_adj_off && e->adj_version == e->version) return PCX_OK;
+  const int64_t n = (int64_t)e->op.size();
+  const int64_t V = (int64_t)e->max_var + 1;
+  std::vector<int32_t> off(V + 1, 0);
+  auto each = [&](int64_t i, auto&& f) {
+    const int32_t x = e->x[i], y = e->y[i], z = e->z[i];
+    f(x);
+    if (y != x) f(y);
+    if (z != x && z != y) f(z);
+  };
+  for (int64_t i = 0; i < n; ++i) each(i, [&](int32_t v) { ++off[v + 1]; });
+  for (int64_t v = 0; v < V; ++v) off[v + 1] += off[v];
+  std::vector<int32_t> adj(std::max<int32_t>(off[V], 1)), pos(off.begin(), off.end() - 1);
+  for (int64_t i = 0; i < n; ++i) each(i, [&](int32_t v) { adj[pos[v]++] = (int32_t)i; });
+  dfree(e->d_adj_off); dfree(e->d_adj);
+  PCX_HIP(hipMalloc(&e->d_adj_off, (V + 1) * sizeof(int32_t)));
+  PCX_HIP(hipMalloc(&e->d_adj, adj.size() * sizeof(int32_t)));
+  PCX_HIP(hipMemcpy(e->d_adj_off, off.data(), (V + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  PCX_HIP(hipMemcpy(e->d_adj, adj.data(), adj.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  e->adj_version = e->version;
+  return PCX_OK;
+}
+
 }  // namespace pcx_internal
 
 using pcx_internal::dom_of;
+using pcx_internal::ensure_adjacency;
 using pcx_internal::sync_table;
 
 namespace {
@@ -248,34 +276,8 @@ pcx_status prepare_tables(pcx_batch* b) {
   return PCX_OK;
 }
 
-// Variable -> table positions adjacency of the engine's table (each
-// propagator listed once per distinct variable), built on the host when the
-// table changed and uploaded.
-pcx_status ensure_adjacency(pcx_engine* e) {
-  if (e->d_adj_off && e->adj_version == e->version) return PCX_OK;
-  const int64_t n = (int64_t)e->op.size();
-  const int64_t V = (int64_t)e->max_var + 1;
-  std::vector<int32_t> off(V + 1, 0);
-  auto each = [&](int64_t i, auto&& f) {
-    const int32_t x = e->x[i], y = e->y[i], z = e->z[i];
-    f(x);
-    if (y != x) f(y);
-    if (z != x && z != y) f(z);
-  };
-  for (int64_t i = 0; i < n; ++i) each(i, [&](int32_t v) { ++off[v + 1]; });
-  for (int64_t v = 0; v < V; ++v) off[v + 1] += off[v];
-  std::vector<int32_t> adj(std::max<int32_t>(off[V], 1)), pos(off.begin(), off.end() - 1);
-  for (int64_t i = 0; i < n; ++i) each(i, [&](int32_t v) { adj[pos[v]++] = (int32_t)i; });
-  dfree(e->d_adj_off); dfree(e->d_adj);
-  PCX_HIP(hipMalloc(&e->d_adj_off, (V + 1) * sizeof(int32_t)));
-  PCX_HIP(hipMalloc(&e->d_adj, adj.size() * sizeof(int32_t)));
-  PCX_HIP(hipMemcpy(e->d_adj_off, off.data(), (V + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
-  PCX_HIP(hipMemcpy(e->d_adj, adj.data(), adj.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  e->adj_version = e->version;
-  return PCX_OK;
-}
-
 int64_t bitmap_words(const pcx_batch* b) { return (b->table_n + 31) / 32; }
+constexpr int64_t kPlainMarkedMin = (int64_t)1 << 20;
 int64_t summary_words(const pcx_batch* b) {
   const int64_t chunks = (bitmap_words(b) + kChunkWords - 1) / kChunkWords;
   return (chunks + 31) / 32;
@@ -289,6 +291,15 @@ pcx_status prepare_events(pcx_batch* b, int32_t mode) {
   pcx_engine* e = b->e;
   b->events = false;
   if (mode == PCX_MODE_PERSISTENT || e->cfg.event_driven == 0 || b->table_n == 0) return PCX_OK;
+  // Automatic mode, one instance at a time: a marked sweep gives up the
+  // propagation within a sweep that a full sweep gets (marks land at the end
+  // of a work item), so small instances converge in fewer, cheap full
+  // sweeps; marked sweeps pay from about 2^20 propagators per instance.
+  if (e->cfg.event_driven < 0 && mode == PCX_MODE_PLAIN) {
+    int64_t mmax = 0;
+    for (int64_t k = 0; k < b->n_inst; ++k) mmax = std::max(mmax, inst_size(b, k));
+    if (mmax < kPlainMarkedMin) return PCX_OK;
+  }
   if (3 * b->table_n >= ((int64_t)1 << 31)) return PCX_OK;    // adjacency indices are int32
   const int64_t words = bitmap_words(b);
   const int64_t rows = b->n_inst * words;
@@ -757,7 +768,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   dfree(b->d_store); dfree(b->d_image); dfree(b->d_off); dfree(b->d_idx); dfree(b->d_flags);
   dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl); dfree(b->d_order);
   dfree(b->d_cxo); dfree(b->d_cy); dfree(b->d_cz); dfree(b->d_member); dfree(b->d_dirty);
-  dfree(b->d_summary); dfree(b->d_image_empty);
+  dfree(b->d_summary); dfree(b->d_image_empty); dfree(b->d_unpack);
   if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
   delete b;
 }
@@ -806,15 +817,13 @@ pcx_status pcx_batch_read(pcx_batch* b, int32_t* lb, int32_t* ub) {
   pcx_engine* e = b->e;
   PCX_HIP(hipSetDevice(e->cfg.device));
   const int64_t n = b->n_inst * b->nvars;
-  // Unpack into the image buffer's halves (the image is rebuilt by load).
-  int32_t* d_lb = nullptr;
-  PCX_HIP(hipMalloc(&d_lb, 2 * n * sizeof(int32_t)));
-  hipError_t he = launch_store_unpack(b->d_store, d_lb, d_lb + n, n, e->stream);
-  if (he == hipSuccess) he = hipMemcpyAsync(lb, d_lb, n * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
-  if (he == hipSuccess) he = hipMemcpyAsync(ub, d_lb + n, n * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
-  if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
-  (void)hipFree(d_lb);
-  if (he != hipSuccess) return fail_with(PCX_EDEVICE, std::string("read: ") + hipGetErrorString(he));
+  // Unpack into a staging buffer kept with the batch (lb then ub).
+  if (!b->d_unpack) PCX_HIP(hipMalloc(&b->d_unpack, 2 * n * sizeof(int32_t)));
+  int32_t* d_lb = b->d_unpack;
+  PCX_HIP(launch_store_unpack(b->d_store, d_lb, d_lb + n, n, e->stream));
+  PCX_HIP(hipMemcpyAsync(lb, d_lb, n * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+  PCX_HIP(hipMemcpyAsync(ub, d_lb + n, n * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+  PCX_HIP(hipStreamSynchronize(e->stream));
   return PCX_OK;
 }
 

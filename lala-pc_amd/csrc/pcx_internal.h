@@ -36,6 +36,7 @@ struct pcx_batch {
   int64_t table_n = 0;                // table size the subsets were built for
   int2* d_store = nullptr;
   int2* d_image = nullptr;
+  int32_t* d_unpack = nullptr;        // staging of pcx_batch_read
   uint8_t* d_image_empty = nullptr;   // per instance: the image holds an empty domain
   bool store_is_image = false;        // the stores equal the image (after load / reset)
   int64_t* d_off = nullptr;
@@ -72,6 +73,7 @@ pcx_status fail_with(pcx_status st, const std::string& msg);
 pcx_status sync_table(pcx_engine* e);   // upload the table if it changed
 int dom_of(const pcx_engine* e);        // kernel domain for this engine
 int device_cus(int device);             // compute units of a device
+pcx_status ensure_adjacency(pcx_engine* e);   // variable -> table positions, on the device
 }  // namespace pcx_internal
 
 #define PCX_HIP(call)                                                                  \

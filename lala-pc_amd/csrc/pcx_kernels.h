@@ -106,6 +106,16 @@ struct SlotDesc {
   const int32_t* z;
   int64_t m;             // propagators
   int2* store;           // the request's store (nvars entries)
+  // Marked sweeps (adj == nullptr: every sweep of the slot is a full sweep).
+  // marks = [2 * words] mark bitmaps (sweep parity), [2 * swords] summaries,
+  // [swords] scheduler snapshot; zeroed at admission.
+  const int32_t* adj_off;
+  const int32_t* adj;
+  uint32_t* marks;
+  int64_t words;
+  int64_t swords;
+  int32_t start;         // the request's first sweep (a full sweep)
+  int32_t pad;
 };
 
 }  // namespace pcx

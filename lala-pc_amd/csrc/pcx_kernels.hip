@@ -20,6 +20,19 @@ namespace pcx {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Values the whole wave agrees on (work-item indices, descriptors read
+// through them) moved to scalar registers: the compiler cannot prove a load
+// from a uniform address returns a uniform value, and would otherwise keep
+// them, and everything derived from them, in VGPRs.
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <class T>
+__device__ __forceinline__ T* uni(T* p) { return reinterpret_cast<T*>(uni((int64_t)(uintptr_t)p)); }
+
 // Subset indices are read once per sweep: stream them past the caches so
 // they do not evict the store and table lines that are reused.
 __device__ __forceinline__ int64_t prop_at(const int32_t* __restrict__ idx, int64_t q) {
@@ -96,6 +109,8 @@ struct Marker {
   uint32_t* sum_now;        // summary row of this sweep (with also_now)
   uint32_t* lsum;           // LDS accumulators of sum / sum_now during a flush, or nullptr
   uint32_t* lsum_now;
+  int32_t* overflow;        // LDS flag: the list overflowed (see flush_marks)
+  int64_t words;            // bitmap words of a row
   int32_t swords;
 };
 
@@ -124,6 +139,7 @@ __device__ __forceinline__ Marker marker_for(const EventSet* ev, int64_t k, int3
   mk.also_now = marked_sweep ? ev->dirty + cu * plane + row : nullptr;
   mk.sum = sums ? ev->summary + nx * splane + srow : nullptr;
   mk.sum_now = sums && marked_sweep ? ev->summary + cu * splane + srow : nullptr;
+  mk.words = ev->words;
   mk.swords = (int32_t)ev->swords;
   return mk;
 }
@@ -142,7 +158,7 @@ __device__ __forceinline__ void set_summary(uint32_t* lds, uint32_t* glob, int32
 // Mark the propagators adjacent to v: adjacency entries are loaded eight at
 // a time (independent loads in flight together), then marked with
 // non-returning atomics.
-__device__ __noinline__ void mark_var(const Marker mk, int32_t v) {
+__device__ __forceinline__ void mark_var(const Marker& mk, int32_t v) {
   constexpr int kB = 8;
   const int32_t e = mk.adj_off[v + 1];
   for (int32_t i = mk.adj_off[v]; i < e; i += kB) {
@@ -169,12 +185,14 @@ __device__ __noinline__ void mark_var(const Marker mk, int32_t v) {
   }
 }
 
+// Every marking kernel gives its workgroup an LDS list (mk.list).  When the
+// list is full the variable is not recorded; the flush then marks every
+// propagator of the instance for the next sweep instead (a superset of the
+// marks the lost variables needed).
 __device__ __forceinline__ void defer_mark(const Marker& mk, int32_t v) {
-  if (mk.list) {
-    const int i = atomicAdd(mk.list_n, 1);
-    if (i < kMarkCap) { mk.list[i] = v; return; }
-  }
-  mark_var(mk, v);                          // list full: mark directly
+  const int i = atomicAdd(mk.list_n, 1);
+  if (i < kMarkCap) mk.list[i] = v;
+  else *mk.overflow = 1;
 }
 
 // Every thread of the workgroup calls this at the end of a work item.  The
@@ -201,7 +219,19 @@ __device__ __forceinline__ void flush_marks(const Marker& mk) {
       if (mk.sum_now && s_sum[1][j]) atomicOr(&mk.sum_now[j], s_sum[1][j]);
     }
   }
-  if (threadIdx.x == 0) *mk.list_n = 0;     // ordered before reuse by the caller's barrier
+  if (*mk.overflow) {                       // lost variables: mark the whole instance
+    for (int64_t w = threadIdx.x; w < mk.words; w += blockDim.x)
+      atomicOr(&mk.dirty[w], mk.member ? mk.member[w] : ~0u);
+    if (mk.sum) {
+      const int64_t chunks = (mk.words + kChunkWords - 1) / kChunkWords;
+      for (int64_t j = threadIdx.x; j < mk.swords; j += blockDim.x) {
+        const int64_t left = chunks - j * 32;
+        atomicOr(&mk.sum[j], left >= 32 ? ~0u : ((1u << left) - 1u));
+      }
+    }
+  }
+  __syncthreads();                          // every thread has read the flag
+  if (threadIdx.x == 0) { *mk.list_n = 0; *mk.overflow = 0; }   // ordered by the caller's barrier
 }
 
 // Apply one propagator to operands already loaded.
@@ -287,6 +317,23 @@ __device__ __forceinline__ InstTable inst_table(const Table& t, const Subsets& s
   return InstTable{Table{t.xo + beg, t.y + beg, t.z + beg, m}, nullptr, m};
 }
 
+// Active slot holding work item `item`: the last a with item_base[a] <= item.
+// A workgroup claims items in increasing order, so the search starts at the
+// slot of its previous item (`hint`): usually one or two loads instead of a
+// full binary search of dependent loads.
+__device__ __forceinline__ int find_active(const int64_t* item_base, int n_act, int64_t item,
+                                           int hint) {
+  int lo = hint, hi = n_act - 1;
+  if (lo >= hi || item_base[lo + 1] > item) return lo;
+  ++lo;
+  if (lo >= hi || item_base[lo + 1] > item) return lo;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (item_base[mid] <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // ---- plain mode: one instance, one sweep per launch -----------------------
 template <int PPL, int D, bool NT>
 __global__ __launch_bounds__(kBlock) void k_sweep_plain(Table t, int2* __restrict__ st,
@@ -298,12 +345,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep_plain(Table t, int2* __restric
   if (f->changed[(s - 1) & 1] != s - 1 || f->failed_at != kNoFail) return;
   const int64_t base = (int64_t)blockIdx.x * (kBlock * PPL);
   __shared__ int32_t s_list[kMarkCap];
-  __shared__ int32_t s_n;
+  __shared__ int32_t s_n, s_over;
   Marker mk = marker_for(&ev, inst, s);
   if (mk.dirty) {
     mk.list = s_list;
     mk.list_n = &s_n;
-    if (threadIdx.x == 0) s_n = 0;
+    mk.overflow = &s_over;
+    if (threadIdx.x == 0) { s_n = 0; s_over = 0; }
     __syncthreads();
   }
   bool chg = false, fail = false;
@@ -397,11 +445,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked_plain(Table t, int2* __
   if (f->changed[(s - 1) & 1] != s - 1 || f->failed_at != kNoFail) return;
   uint32_t* cur = ev.dirty + (int64_t)(s & 1) * ev.n_inst * ev.words + inst * ev.words;
   __shared__ int32_t s_list[kMarkCap];
-  __shared__ int32_t s_n;
+  __shared__ int32_t s_n, s_over;
   Marker mk = marker_for(&ev, inst, s, true);
   mk.list = s_list;
   mk.list_n = &s_n;
-  if (threadIdx.x == 0) s_n = 0;
+  mk.overflow = &s_over;
+  if (threadIdx.x == 0) { s_n = 0; s_over = 0; }
   __syncthreads();
   bool chg = false, fail = false;
   int nt = 0, ne = 0;
@@ -546,27 +595,27 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
   int32_t* qc = wl.xq + xcd;
   __shared__ int32_t s_next;
   __shared__ int32_t s_list[kMarkCap];
-  __shared__ int32_t s_n;
-  if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; }
+  __shared__ int32_t s_n, s_over;
+  if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; s_over = 0; }
   __syncthreads();
-  int64_t item = first_item + s_next;
+  int64_t item = first_item + __builtin_amdgcn_readfirstlane(s_next);
   __syncthreads();
+  int hint = 0;
   while (item < end_item) {
     int32_t nxt = 0;
     if (threadIdx.x == 0) nxt = atomicAdd(qc, 1);
     // active slot holding this item: last a with item_base[a] <= item
-    int lo = 0, hi = n_act - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (wl.item_base[mid] <= item) lo = mid; else hi = mid - 1;
-    }
-    const int32_t k = wl.act[lo];
+    const int lo = uni(find_active(wl.item_base, n_act, item, hint));
+    hint = lo;
+    const int32_t k = uni(wl.act[lo]);
     InstFlags* f = flags + k;
-    const int64_t first = (item - wl.item_base[lo]) * chunk;
-    const InstTable it = inst_table(t, sub, k, n_props);
+    const int64_t first = (item - uni(wl.item_base[lo])) * chunk;
+    InstTable it = inst_table(t, sub, k, n_props);
+    it.t.xo = uni(it.t.xo); it.t.y = uni(it.t.y); it.t.z = uni(it.t.z);
+    it.idx = uni(it.idx); it.m = uni(it.m);
     int2* st = store + (int64_t)k * nvars;
     Marker mk = marker_for(&ev, k, s);
-    if (mk.dirty) { mk.list = s_list; mk.list_n = &s_n; }
+    if (mk.dirty) { mk.list = s_list; mk.list_n = &s_n; mk.overflow = &s_over; }
     bool chg = false, fail = false;
     int nt = 0;
     if (f->failed_at == kNoFail) {
@@ -580,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
     flush_marks(mk);
     if (threadIdx.x == 0) s_next = nxt;
     __syncthreads();
-    item = first_item + s_next;
+    item = first_item + __builtin_amdgcn_readfirstlane(s_next);
     __syncthreads();
   }
 }
@@ -646,23 +695,21 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
   uint32_t* cur = ev.dirty + (int64_t)(s & 1) * ev.n_inst * ev.words;
   __shared__ int32_t s_next;
   __shared__ int32_t s_list[kMarkCap];
-  __shared__ int32_t s_n;
-  if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; }
+  __shared__ int32_t s_n, s_over;
+  if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; s_over = 0; }
   __syncthreads();
-  int64_t item = first_item + s_next;
+  int64_t item = first_item + __builtin_amdgcn_readfirstlane(s_next);
   __syncthreads();
+  int hint = 0;
   while (item < end_item) {
     int32_t nxt = 0;
     if (threadIdx.x == 0) nxt = atomicAdd(qc, 1);
-    int lo = 0, hi = n_act - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (wl.item_base[mid] <= item) lo = mid; else hi = mid - 1;
-    }
-    const int32_t k = wl.act[lo];
+    const int lo = uni(find_active(wl.item_base, n_act, item, hint));
+    hint = lo;
+    const int32_t k = uni(wl.act[lo]);
     InstFlags* f = flags + k;
     // the item's chunk: the r-th set bit of the instance's snapshot
-    int64_t r = item - wl.item_base[lo], chunk = -1;
+    int64_t r = item - uni(wl.item_base[lo]), chunk = -1;
     const uint32_t* snap = ev.snap + (int64_t)k * ev.swords;
     for (int64_t j = 0; j < ev.swords; ++j) {
       uint32_t v = snap[j];
@@ -674,10 +721,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
       }
       r -= c;
     }
+    chunk = uni(chunk);
     const int64_t w = chunk * kMarkedWordsPerItem + threadIdx.x;
     Marker mk = marker_for(&ev, k, s, true);
     mk.list = s_list;
     mk.list_n = &s_n;
+    mk.overflow = &s_over;
     bool chg = false, fail = false;
     int nt = 0, ne = 0;
     if (f->failed_at == kNoFail && chunk >= 0 && w < ev.words)
@@ -688,7 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
     flush_marks(mk);
     if (threadIdx.x == 0) s_next = nxt;
     __syncthreads();
-    item = first_item + s_next;
+    item = first_item + __builtin_amdgcn_readfirstlane(s_next);
     __syncthreads();
   }
 }
@@ -726,6 +775,26 @@ hipError_t launch_build_members(const int64_t* off, const int32_t* idx, int64_t 
 }
 
 // ---- serving slots: heterogeneous batch, one table and store per slot ----
+__device__ __forceinline__ Marker slot_marker(const SlotDesc& sd, int32_t s, bool marked_sweep) {
+  if (!sd.adj) return Marker{};
+  uint32_t* dirty = sd.marks;
+  uint32_t* summary = sd.marks + 2 * sd.words;
+  const int64_t nx = (s + 1) & 1, cu = s & 1;
+  Marker mk{};
+  mk.adj_off = sd.adj_off;
+  mk.adj = sd.adj;
+  mk.dirty = dirty + nx * sd.words;
+  mk.also_now = marked_sweep ? dirty + cu * sd.words : nullptr;
+  mk.sum = summary + nx * sd.swords;
+  mk.sum_now = marked_sweep ? summary + cu * sd.swords : nullptr;
+  mk.words = sd.words;
+  mk.swords = (int32_t)sd.swords;
+  return mk;
+}
+
+__device__ __forceinline__ bool slot_full_sweep(const SlotDesc& sd, int32_t s) {
+  return !sd.adj || s == sd.start;
+}
 // Same scheme as the batched mode: a one-workgroup scheduler builds the item
 // prefix sum over the slots still running, then a grid-stride XCD-aware
 // sweep.  Free slots carry failed_at != kNoFail and done_at = 0, so they are
@@ -751,7 +820,19 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule_slots(const SlotDesc
       if (f.done_at < 0) f.done_at = s;
       continue;
     }
-    items += (slots[k].m + chunk - 1) / chunk;
+    const SlotDesc& sd = slots[k];
+    if (slot_full_sweep(sd, s)) {
+      items += (sd.m + chunk - 1) / chunk;
+    } else {                 // marked sweep: the chunks holding marks; snapshot and clear
+      uint32_t* srow = sd.marks + 2 * sd.words + (int64_t)(s & 1) * sd.swords;
+      uint32_t* snap = sd.marks + 2 * sd.words + 2 * sd.swords;
+      for (int64_t j = 0; j < sd.swords; ++j) {
+        const uint32_t v = srow[j];
+        snap[j] = v;
+        srow[j] = 0;
+        items += __builtin_popcount(v);
+      }
+    }
     ++na;
   }
   part[tid] = items;
@@ -772,7 +853,13 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule_slots(const SlotDesc
     if (f.done_at >= 0) continue;
     wl.act[ia] = k;
     wl.item_base[ia] = ib;
-    ib += (slots[k].m + chunk - 1) / chunk;
+    const SlotDesc& sd = slots[k];
+    if (slot_full_sweep(sd, s)) {
+      ib += (sd.m + chunk - 1) / chunk;
+    } else {
+      const uint32_t* snap = sd.marks + 2 * sd.words + 2 * sd.swords;
+      for (int64_t j = 0; j < sd.swords; ++j) ib += __builtin_popcount(snap[j]);
+    }
     ++ia;
   }
   __syncthreads();
@@ -804,36 +891,66 @@ __global__ __launch_bounds__(kBlock) void k_sweep_slots(const SlotDesc* slots, I
   const int64_t end_item = total * (xcd + 1) / kXcds;
   int32_t* qc = wl.xq + xcd;
   __shared__ int32_t s_next;
-  if (threadIdx.x == 0) s_next = atomicAdd(qc, 1);
+  __shared__ int32_t s_list[kMarkCap];
+  __shared__ int32_t s_n, s_over;
+  if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; s_over = 0; }
   __syncthreads();
-  int64_t item = first_item + s_next;
+  int64_t item = first_item + __builtin_amdgcn_readfirstlane(s_next);
   __syncthreads();
+  int hint = 0;
   while (item < end_item) {
     int32_t nxt = 0;
     if (threadIdx.x == 0) nxt = atomicAdd(qc, 1);
-    int lo = 0, hi = n_act - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (wl.item_base[mid] <= item) lo = mid; else hi = mid - 1;
-    }
-    const int32_t k = wl.act[lo];
-    const SlotDesc sd = slots[k];
+    const int lo = uni(find_active(wl.item_base, n_act, item, hint));
+    hint = lo;
+    const int32_t k = uni(wl.act[lo]);
+    SlotDesc sd = slots[k];
+    sd.xo = uni(sd.xo); sd.y = uni(sd.y); sd.z = uni(sd.z); sd.m = uni(sd.m);
+    sd.store = uni(sd.store); sd.adj_off = uni(sd.adj_off); sd.adj = uni(sd.adj);
+    sd.marks = uni(sd.marks); sd.words = uni(sd.words); sd.swords = uni(sd.swords);
+    sd.start = uni(sd.start);
     InstFlags* f = flags + k;
     const Table t{sd.xo, sd.y, sd.z, sd.m};
-    const int64_t first = (item - wl.item_base[lo]) * chunk;
+    const bool full = slot_full_sweep(sd, s);
+    Marker mk = slot_marker(sd, s, !full);
+    if (mk.dirty) { mk.list = s_list; mk.list_n = &s_n; mk.overflow = &s_over; }
+    const int64_t r = item - uni(wl.item_base[lo]);
     bool chg = false, fail = false;
-    int nt = 0;
-    if (f->failed_at == kNoFail) {
+    int nt = 0, ne = 0;
+    if (full) {
+      if (f->failed_at == kNoFail) {
 #pragma unroll
-      for (int j = 0; j < PPL; ++j) {
-        const int64_t q = first + j * kBlock + threadIdx.x;
-        if (q < sd.m) eval_prop<D, false>(t, sd.store, q, chg, fail, nt, Marker{});
+        for (int j = 0; j < PPL; ++j) {
+          const int64_t q = r * chunk + j * kBlock + threadIdx.x;
+          if (q < sd.m) eval_prop<D, false>(t, sd.store, q, chg, fail, nt, mk);
+        }
       }
+    } else {
+      // the item's chunk: the r-th set bit of the slot's snapshot
+      const uint32_t* snap = sd.marks + 2 * sd.words + 2 * sd.swords;
+      int64_t rr = r, c = -1;
+      for (int64_t j = 0; j < sd.swords; ++j) {
+        uint32_t v = snap[j];
+        const int pc = __builtin_popcount(v);
+        if (rr < pc) {
+          for (int q = 0; q < rr; ++q) v &= v - 1;
+          c = j * 32 + __builtin_ctz(v);
+          break;
+        }
+        rr -= pc;
+      }
+      c = uni(c);
+      const int64_t w = c * kChunkWords + threadIdx.x;
+      if (f->failed_at == kNoFail && c >= 0 && w < sd.words)
+        sweep_marked_word<D>(t, sd.store, sd.marks + (int64_t)(s & 1) * sd.words, w, mk, chg, fail,
+                             nt, ne);
     }
     publish(f, s, chg, fail, nt, tight);
+    count_evals(f, ne);
+    flush_marks(mk);
     if (threadIdx.x == 0) s_next = nxt;
     __syncthreads();
-    item = first_item + s_next;
+    item = first_item + __builtin_amdgcn_readfirstlane(s_next);
     __syncthreads();
   }
 }

@@ -63,6 +63,12 @@ class Arena {
   std::map<int64_t, int64_t> free_;
 };
 
+// Units of 8 bytes an admitted request takes in the arena: its store, then
+// (marked sweeps) two mark bitmaps, two summaries and a snapshot.
+struct Footprint {
+  int64_t words = 0, swords = 0, units = 0;
+};
+
 struct Request {
   int32_t net;
   int32_t* lb;
@@ -76,6 +82,7 @@ struct Slot {
   bool busy = false;
   Request req{};
   int64_t off = 0;      // arena offset
+  int64_t units = 0;    // arena units held
   int32_t start = 0;    // first sweep number
 };
 
@@ -90,7 +97,6 @@ struct pcx_server {
   int32_t n_slots = 0;
   SlotDesc* d_slots = nullptr;
   InstFlags* d_flags = nullptr;
-  InstFlags* d_solo = nullptr;        // flags of a request run alone
   int32_t* d_act = nullptr;
   int64_t* d_item_base = nullptr;
   int32_t* d_ctl = nullptr;
@@ -106,9 +112,36 @@ struct pcx_server {
   std::vector<pcx_served> done;
   int32_t next_sweep = 0;
   pcx_server_stats stats{};
+  hipGraphExec_t tick_graph[2] = {nullptr, nullptr};   // a tick's sweeps, per kernel domain
 };
 
 namespace {
+
+// Marked sweeps per request.  A small request is done in a few cheap full
+// sweeps, and full sweeps keep the propagation within a sweep that marked
+// sweeps give up; a large one re-evaluates its whole table every sweep for
+// ~100 sweeps, which is most of the server's work.  Automatic mode marks
+// requests of at least kSlotMarkedMin propagators.
+constexpr int64_t kSlotMarkedMin = (int64_t)1 << 17;
+
+bool events_for(const pcx_server* s, int32_t net) {
+  if (s->cfg.event_driven == 0) return false;
+  const int64_t m = (int64_t)s->nets[net]->op.size();
+  return m > 0 && (s->cfg.event_driven > 0 || m >= kSlotMarkedMin);
+}
+
+Footprint footprint(const pcx_server* s, const Request& rq) {
+  Footprint fp;
+  fp.units = rq.nvars;
+  if (events_for(s, rq.net)) {
+    const int64_t m = (int64_t)s->nets[rq.net]->op.size();
+    fp.words = (m + 31) / 32;
+    const int64_t chunks = (fp.words + kChunkWords - 1) / kChunkWords;
+    fp.swords = (chunks + 31) / 32;
+    fp.units += (2 * fp.words + 3 * fp.swords + 1) / 2;
+  }
+  return fp;
+}
 
 WorkList worklist(pcx_server* s) {
   return worklist_of(s->d_act, s->d_item_base, s->d_ctl);
@@ -149,7 +182,8 @@ pcx_status admit(pcx_server* s) {
       if (!s->slots[i].busy) { k = i; break; }
     if (k < 0) { ++s->stats.waits_slot; break; }
     if (staged + rq.nvars > s->stage_words) break;
-    const int64_t off = s->arena.alloc(rq.nvars);
+    const Footprint fp = footprint(s, rq);
+    const int64_t off = s->arena.alloc(fp.units);
     if (off < 0) { ++s->stats.waits_arena; break; }   // no HBM room: wait for a slot to drain
     int2* stage = s->h_stage + staged;
     for (int64_t v = 0; v < rq.nvars; ++v) stage[v] = make_int2(rq.lb[v], rq.ub[v]);
@@ -157,7 +191,22 @@ pcx_status admit(pcx_server* s) {
     staged += rq.nvars;
     pcx_engine* e = s->nets[rq.net];
     const int32_t S = s->next_sweep;
-    s->h_slots[k] = SlotDesc{e->d_xo, e->d_y, e->d_z, (int64_t)e->op.size(), s->d_arena + off};
+    SlotDesc sd{};
+    sd.xo = e->d_xo;
+    sd.y = e->d_y;
+    sd.z = e->d_z;
+    sd.m = (int64_t)e->op.size();
+    sd.store = s->d_arena + off;
+    sd.start = S;
+    if (fp.words > 0) {
+      sd.adj_off = e->d_adj_off;
+      sd.adj = e->d_adj;
+      sd.marks = reinterpret_cast<uint32_t*>(s->d_arena + off + rq.nvars);
+      sd.words = fp.words;
+      sd.swords = fp.swords;
+      PCX_HIP(hipMemsetAsync(sd.marks, 0, (2 * fp.words + 3 * fp.swords) * sizeof(uint32_t), s->stream));
+    }
+    s->h_slots[k] = sd;
     InstFlags f{};
     f.changed[(S - 1) & 1] = S - 1;            // primes sweep S
     f.changed[S & 1] = S - 2;
@@ -170,17 +219,21 @@ pcx_status admit(pcx_server* s) {
     sl.busy = true;
     sl.req = rq;
     sl.off = off;
+    sl.units = fp.units;
     sl.start = S;
     s->pending.pop_front();
     ++s->stats.admitted;
     s->stats.arena_peak_bytes = std::max(s->stats.arena_peak_bytes,
-                                         (int64_t)((off + rq.nvars) * sizeof(int2)));
+                                         (int64_t)((off + fp.units) * sizeof(int2)));
   }
   return PCX_OK;
 }
 
-// Hand back the slots whose request reached its fixpoint or failed.
+// Hand back the slots whose request reached its fixpoint or failed: their
+// stores are copied back together (one synchronization), then released.
 pcx_status harvest(pcx_server* s) {
+  std::vector<std::pair<int32_t, int64_t>> done;     // slot, staging offset
+  int64_t staged = 0;
   for (int32_t k = 0; k < s->n_slots; ++k) {
     Slot& sl = s->slots[k];
     if (!sl.busy) continue;
@@ -189,18 +242,32 @@ pcx_status harvest(pcx_server* s) {
     const bool failed = f.failed_at != kNoFail;
     const bool quiet = last < s->next_sweep - 1;     // the latest sweep left it unchanged
     if (!failed && !quiet) continue;
+    if (staged + sl.req.nvars > s->stage_words) break;   // the rest next tick
+    PCX_HIP(hipMemcpyAsync(s->h_stage + staged, s->d_arena + sl.off, sl.req.nvars * sizeof(int2),
+                           hipMemcpyDeviceToHost, s->stream));
+    done.emplace_back(k, staged);
+    staged += sl.req.nvars;
+  }
+  if (done.empty()) return PCX_OK;
+  PCX_HIP(hipStreamSynchronize(s->stream));
+  for (const auto& d : done) {
+    const int32_t k = d.first;
+    Slot& sl = s->slots[k];
+    const InstFlags& f = s->h_flags[k];
+    const int32_t last = std::max(f.changed[0], f.changed[1]);
     pcx_result r{};
-    r.failed = failed;
+    r.failed = f.failed_at != kNoFail;
     r.has_changed = last >= sl.start;
-    r.sweeps = failed ? (int64_t)f.failed_at - sl.start + 1
-                      : (int64_t)std::max(last, sl.start - 1) - sl.start + 2;
-    r.evals = r.sweeps * s->h_slots[k].m;
-    int2* stage = s->h_stage;
-    PCX_HIP(hipMemcpyAsync(stage, s->d_arena + sl.off, sl.req.nvars * sizeof(int2), hipMemcpyDeviceToHost, s->stream));
-    PCX_HIP(hipStreamSynchronize(s->stream));
+    r.sweeps = r.failed ? (int64_t)f.failed_at - sl.start + 1
+                        : (int64_t)std::max(last, sl.start - 1) - sl.start + 2;
+    if (s->h_slots[k].adj)          // one full sweep + the marked evaluations
+      r.evals = (r.sweeps > 0 ? s->h_slots[k].m : 0) + f.marked_evals;
+    else
+      r.evals = r.sweeps * s->h_slots[k].m;
+    const int2* stage = s->h_stage + d.second;
     for (int64_t v = 0; v < sl.req.nvars; ++v) { sl.req.lb[v] = stage[v].x; sl.req.ub[v] = stage[v].y; }
     finish(s, sl.req, r);
-    s->arena.release(sl.off, sl.req.nvars);
+    s->arena.release(sl.off, sl.units);
     sl.busy = false;
     // park the slot: never scheduled until re-admitted
     s->h_flags[k].failed_at = kFreeFailed;
@@ -228,14 +295,32 @@ pcx_status tick(pcx_server* s) {
     bool div = false;
     for (const Slot& sl : s->slots)
       if (sl.busy) div |= s->nets[sl.req.net]->has_div;
-    const WorkList wl = worklist(s);
-    for (int g = 0; g < s->cfg.sweeps_per_tick; ++g) {
-      PCX_HIP(launch_schedule_slots(s->d_slots, s->d_flags, s->n_slots, wl, ppw, s->stream));
-      PCX_HIP(launch_sweep_slots(s->d_slots, s->d_flags, wl, s->d_tight, ppw, div ? kItvDiv : kItv,
-                                 grid, s->stream));
-      ++s->next_sweep;
-      ++s->stats.sweeps;
+    // A tick's sweeps read everything (slots, flags, sweep number) from
+    // device memory, so they are captured once per kernel domain and replayed.
+    hipGraphExec_t& gx = s->tick_graph[div ? 1 : 0];
+    if (!gx) {
+      const WorkList wl = worklist(s);
+      hipGraph_t graph = nullptr;
+      PCX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t ce = hipSuccess;
+      for (int g = 0; g < s->cfg.sweeps_per_tick && ce == hipSuccess; ++g) {
+        ce = launch_schedule_slots(s->d_slots, s->d_flags, s->n_slots, wl, ppw, s->stream);
+        if (ce == hipSuccess)
+          ce = launch_sweep_slots(s->d_slots, s->d_flags, wl, s->d_tight, ppw, div ? kItvDiv : kItv,
+                                  grid, s->stream);
+      }
+      const hipError_t ee = hipStreamEndCapture(s->stream, &graph);
+      if (ce != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        PCX_HIP(ce != hipSuccess ? ce : ee);
+      }
+      const hipError_t ie = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      PCX_HIP(ie);
     }
+    PCX_HIP(hipGraphLaunch(gx, s->stream));
+    s->next_sweep += s->cfg.sweeps_per_tick;
+    s->stats.sweeps += s->cfg.sweeps_per_tick;
     PCX_HIP(hipMemcpyAsync(s->h_flags, s->d_flags, s->n_slots * sizeof(InstFlags), hipMemcpyDeviceToHost, s->stream));
     PCX_HIP(hipStreamSynchronize(s->stream));
     st = harvest(s);
@@ -247,44 +332,10 @@ pcx_status tick(pcx_server* s) {
   return PCX_OK;
 }
 
-// One request on its own: one sweep launch per sweep, host check every 8.
+// One request on its own, as a caller without the server would run it: the
+// drop-in fixpoint call on the request's network.
 pcx_status run_alone(pcx_server* s, const Request& rq, pcx_result* out) {
-  pcx_engine* e = s->nets[rq.net];
-  for (int64_t v = 0; v < rq.nvars; ++v)
-    if (rq.lb[v] > rq.ub[v]) { *out = pcx_result{}; out->failed = 1; return PCX_OK; }
-  for (int64_t v = 0; v < rq.nvars; ++v) s->h_stage[v] = make_int2(rq.lb[v], rq.ub[v]);
-  PCX_HIP(hipMemcpyAsync(s->d_arena, s->h_stage, rq.nvars * sizeof(int2), hipMemcpyHostToDevice, s->stream));
-  InstFlags f{};
-  f.changed[0] = -2; f.changed[1] = -1; f.failed_at = kNoFail; f.done_at = -1;
-  InstFlags* h = s->h_flags + s->n_slots;     // spare pinned entry
-  *h = f;
-  PCX_HIP(hipMemcpyAsync(s->d_solo, h, sizeof(InstFlags), hipMemcpyHostToDevice, s->stream));
-  const Table t{e->d_xo, e->d_y, e->d_z, (int64_t)e->op.size()};
-  const int64_t m = (int64_t)e->op.size();
-  int ppw = 64;
-  for (int p = 512; p > 64; p /= 2)
-    if (m / ((int64_t)kBlock * (p / kWave)) >= 2LL * pcx_internal::device_cus(s->cfg.device)) { ppw = p; break; }
-  int32_t sw = 0;
-  for (;;) {
-    for (int g = 0; g < 8; ++g, ++sw)
-      PCX_HIP(launch_sweep_plain(t, s->d_arena, nullptr, m, sw, s->d_solo, s->d_tight, ppw,
-                                 pcx_internal::dom_of(e), false, s->stream));
-    PCX_HIP(hipMemcpyAsync(h, s->d_solo, sizeof(InstFlags), hipMemcpyDeviceToHost, s->stream));
-    PCX_HIP(hipStreamSynchronize(s->stream));
-    const int32_t last = std::max(h->changed[0], h->changed[1]);
-    if (h->failed_at != kNoFail || last + 1 < sw) break;
-  }
-  const int32_t last = std::max(h->changed[0], h->changed[1]);
-  pcx_result r{};
-  r.failed = h->failed_at != kNoFail;
-  r.has_changed = last >= 0;
-  r.sweeps = r.failed ? (int64_t)h->failed_at + 1 : (int64_t)last + 2;
-  r.evals = r.sweeps * m;
-  PCX_HIP(hipMemcpyAsync(s->h_stage, s->d_arena, rq.nvars * sizeof(int2), hipMemcpyDeviceToHost, s->stream));
-  PCX_HIP(hipStreamSynchronize(s->stream));
-  for (int64_t v = 0; v < rq.nvars; ++v) { rq.lb[v] = s->h_stage[v].x; rq.ub[v] = s->h_stage[v].y; }
-  *out = r;
-  return PCX_OK;
+  return pcx_fixpoint(s->nets[rq.net], rq.lb, rq.ub, rq.nvars, out);
 }
 
 }  // namespace
@@ -300,13 +351,16 @@ void pcx_server_config_default(pcx_server_config* c) {
   c->sweeps_per_tick = 4;
   c->props_per_wave = 0;
   c->stage_bytes = (int64_t)64 << 20;
+  c->event_driven = -1;
 }
 
 void pcx_server_destroy(pcx_server* s) {
   if (!s) return;
   (void)hipSetDevice(s->cfg.device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  for (void* p : {(void*)s->d_slots, (void*)s->d_flags, (void*)s->d_solo, (void*)s->d_act, (void*)s->d_item_base,
+  for (hipGraphExec_t g : s->tick_graph)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (void* p : {(void*)s->d_slots, (void*)s->d_flags, (void*)s->d_act, (void*)s->d_item_base,
                   (void*)s->d_ctl, (void*)s->d_tight, (void*)s->d_arena})
     if (p) (void)hipFree(p);
   for (void* p : {(void*)s->h_slots, (void*)s->h_flags, (void*)s->h_stage})
@@ -341,14 +395,13 @@ pcx_status pcx_server_create(const pcx_server_config* cfg, pcx_server** out) {
   auto chk = [&](hipError_t r) { if (he == hipSuccess) he = r; };
   chk(hipMalloc(&s->d_slots, c.max_slots * sizeof(SlotDesc)));
   chk(hipMalloc(&s->d_flags, c.max_slots * sizeof(InstFlags)));
-  chk(hipMalloc(&s->d_solo, sizeof(InstFlags)));
   chk(hipMalloc(&s->d_act, c.max_slots * sizeof(int32_t)));
   chk(hipMalloc(&s->d_item_base, (c.max_slots + 1) * sizeof(int64_t)));
   chk(hipMalloc(&s->d_ctl, kCtlWords * sizeof(int32_t)));
   chk(hipMalloc(&s->d_tight, kTightSlots * sizeof(unsigned long long)));
   chk(hipMalloc(&s->d_arena, words * sizeof(int2)));
   chk(hipHostMalloc(&s->h_slots, c.max_slots * sizeof(SlotDesc), hipHostMallocDefault));
-  chk(hipHostMalloc(&s->h_flags, (c.max_slots + 1) * sizeof(InstFlags), hipHostMallocDefault));
+  chk(hipHostMalloc(&s->h_flags, c.max_slots * sizeof(InstFlags), hipHostMallocDefault));
   chk(hipHostMalloc(&s->h_stage, s->stage_words * sizeof(int2), hipHostMallocDefault));
   if (he != hipSuccess) {
     pcx_server_destroy(s);
@@ -357,7 +410,7 @@ pcx_status pcx_server_create(const pcx_server_config* cfg, pcx_server** out) {
   }
   s->arena.init(words);
   for (int32_t k = 0; k < c.max_slots; ++k) {
-    s->h_slots[k] = SlotDesc{nullptr, nullptr, nullptr, 0, nullptr};
+    s->h_slots[k] = SlotDesc{};
     InstFlags f{};
     f.changed[0] = -2; f.changed[1] = -2; f.failed_at = kFreeFailed; f.done_at = 0;
     s->h_flags[k] = f;
@@ -383,6 +436,14 @@ pcx_status pcx_server_add_network(pcx_server* s, pcx_engine* e, int32_t* net) {
   pcx_status st = pcx_internal::sync_table(e);
   if (st != PCX_OK) return st;
   s->nets.push_back(e);
+  if (events_for(s, (int32_t)s->nets.size() - 1)) {
+    if (3 * (int64_t)e->op.size() >= ((int64_t)1 << 31)) {
+      s->nets.pop_back();
+      return fail_with(PCX_EINVAL, "network too large for marked sweeps; set event_driven = 0");
+    }
+    st = pcx_internal::ensure_adjacency(e);
+    if (st != PCX_OK) { s->nets.pop_back(); return st; }
+  }
   *net = (int32_t)s->nets.size() - 1;
   return PCX_OK;
 }
@@ -394,9 +455,10 @@ pcx_status pcx_server_submit(pcx_server* s, int32_t net, int32_t* lb, int32_t* u
   pcx_engine* e = s->nets[net];
   if (e->dirty) return fail_with(PCX_EINVAL, "network told more propagators after registration");
   if (nvars <= e->max_var) return fail_with(PCX_EINVAL, "store smaller than the network's variables");
-  if (nvars > s->stage_words || nvars * (int64_t)sizeof(int2) > s->cfg.arena_bytes)
+  Request rq{net, lb, ub, nvars, tag, now_s()};
+  if (nvars > s->stage_words || footprint(s, rq).units * (int64_t)sizeof(int2) > s->cfg.arena_bytes)
     return fail_with(PCX_EINVAL, "store larger than the server's arena or staging buffer");
-  s->pending.push_back(Request{net, lb, ub, nvars, tag, now_s()});
+  s->pending.push_back(rq);
   return PCX_OK;
 }
 
@@ -435,7 +497,8 @@ pcx_status pcx_server_replay(pcx_server* s, int64_t n, const double* arrival_s, 
     if (s->nets[net[i]]->dirty) return fail_with(PCX_EINVAL, "network told more propagators after registration");
     const int64_t nv = store_off[i + 1] - store_off[i];
     if (nv <= s->nets[net[i]]->max_var) return fail_with(PCX_EINVAL, "store smaller than the network's variables");
-    if (nv > s->stage_words || nv * (int64_t)sizeof(int2) > s->cfg.arena_bytes)
+    const Request rq{net[i], nullptr, nullptr, nv, i, 0.0};
+    if (nv > s->stage_words || footprint(s, rq).units * (int64_t)sizeof(int2) > s->cfg.arena_bytes)
       return fail_with(PCX_EINVAL, "store larger than the server's arena or staging buffer");
     if (i && arrival_s[i] < arrival_s[i - 1]) return fail_with(PCX_EINVAL, "arrivals must be sorted");
   }

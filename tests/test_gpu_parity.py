@@ -287,3 +287,50 @@ def test_autotuner_choice_is_bit_exact(gpu, tmp_path):
     pir2 = pcx.PIR()
     assert pcx.apply_tuning(pir2, best) == best["mode"]
     assert pir2.get_config().event_driven == best["event_driven"]
+
+
+@pytest.mark.parametrize("mode", [pcx.MODE_PLAIN, pcx.MODE_BATCHED])
+def test_marked_sweeps_when_a_work_item_tightens_many_variables(gpu, mode):
+    """2,048 independent x_i = y_i + z_i whose x_i all tighten in the first
+    sweep: one 2,048-propagator work item tightens more variables than its
+    workgroup's mark list holds, so the whole instance is marked for the
+    next sweep.  A chain x_0 <= x_1 <= ... reads them afterwards."""
+    n = 2048
+    V = 3 * n
+    op = np.full(n, pcx.ADD, np.int32)
+    x = np.arange(n, dtype=np.int32)
+    y = (n + np.arange(n)).astype(np.int32)
+    z = (2 * n + np.arange(n)).astype(np.int32)
+    lb = np.zeros(V, np.int32)
+    ub = np.full(V, 1000, np.int32)
+    ub[n:] = np.arange(2 * n) % 7             # y, z small: every x_i tightens
+    # a LEQ chain x_i <= x_{i+1} over the x's (reified output fixed to 1)
+    b = np.arange(V, V + n - 1, dtype=np.int32)
+    op2 = np.concatenate([op, np.full(n - 1, pcx.LEQ, np.int32)])
+    x2 = np.concatenate([x, b])
+    y2 = np.concatenate([y, np.arange(n - 1, dtype=np.int32)])
+    z2 = np.concatenate([z, np.arange(1, n, dtype=np.int32)])
+    lb = np.concatenate([lb, np.ones(n - 1, np.int32)])
+    ub = np.concatenate([ub, np.ones(n - 1, np.int32)])
+    tables = (op2, x2, y2, z2)
+    lo, hi = lb.copy(), ub.copy()
+    top, tx, ty, tz = (a.copy() for a in tables)
+    ol.tell_embed(top, tx, lo, hi)
+    ol.sort_table(top, tx, ty, tz)
+    ro = ol.fixpoint(top, tx, ty, tz, lo, hi)
+    pir = pcx.PIR(props_per_wave=512, event_driven=1)
+    glb, gub = lb.copy(), ub.copy()
+    pir.tell(*tables, glb, gub)
+    if mode == pcx.MODE_PLAIN:
+        rg = pir.fixpoint(glb, gub)
+    else:
+        bt = pcx.Batch(pir, 2, len(glb))
+        bt.load(np.concatenate([glb, glb]), np.concatenate([gub, gub]))
+        res, _ = bt.run(mode)
+        rg = res[0]
+        l2, u2 = bt.read()
+        assert (l2[:len(glb)] == l2[len(glb):]).all() and (u2[:len(glb)] == u2[len(glb):]).all()
+        glb, gub = l2[:len(glb)], u2[:len(glb)]
+    assert bool(rg.failed) == bool(ro.failed)
+    if not ro.failed:
+        assert (glb == lo).all() and (gub == hi).all()

@@ -68,6 +68,17 @@ def _serve(tr, pirs, **cfg):
     return stores, got, srv.stats()
 
 
+def _footprint_bytes(nvars, nprops, events):
+    """Arena bytes a request takes: its store, plus mark bitmaps with marked sweeps
+    (events: 1 always, 0 never, -1 from 2^17 propagators)."""
+    units = nvars
+    if events == 1 or (events == -1 and nprops >= 1 << 17):
+        words = (nprops + 31) // 32
+        swords = ((words + 255) // 256 + 31) // 32
+        units += (2 * words + 3 * swords + 1) // 2
+    return 8 * units
+
+
 def _same(a_lo, a_hi, a_res, b_lo, b_hi, b_res):
     if bool(a_res.failed) != bool(b_res.failed):
         return False
@@ -89,9 +100,10 @@ def test_alone_matches_oracle(small):
             assert np.array_equal(lo, g_lo) and np.array_equal(hi, g_hi), f"request {i}"
 
 
-def test_mix_equals_alone(small):
+@pytest.mark.parametrize("events", [1, 0, -1], ids=["marked", "full", "auto"])
+def test_mix_equals_alone(small, events):
     tr, pirs, alone = small
-    stores, got, st = _serve(tr, pirs, max_slots=64, sweeps_per_tick=4)
+    stores, got, st = _serve(tr, pirs, max_slots=64, sweeps_per_tick=4, event_driven=events)
     assert st.max_running > 8, "requests never shared the GPU"
     assert st.completed == len(tr) and st.admitted == len(tr)
     for i in range(len(tr)):
@@ -99,13 +111,15 @@ def test_mix_equals_alone(small):
         assert _same(lo, hi, got[i], *alone[i]), f"request {i} differs in the mix"
 
 
-def test_few_slots_and_tiny_arena(small):
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
+def test_few_slots_and_tiny_arena(small, events):
     tr, pirs, alone = small
-    biggest = max(len(tr.store(i)[0]) for i in range(len(tr)))
-    stores, got, st = _serve(tr, pirs, max_slots=3, sweeps_per_tick=1,
-                             arena_bytes=8 * biggest + 8 * 300)
+    biggest = max(_footprint_bytes(len(tr.store(i)[0]), tr.nprops(i), events) for i in range(len(tr)))
+    arena = biggest + 8 * 300
+    stores, got, st = _serve(tr, pirs, max_slots=3, sweeps_per_tick=1, arena_bytes=arena,
+                             event_driven=events)
     assert st.waits_slot > 0 and st.waits_arena > 0
-    assert st.arena_peak_bytes <= 8 * biggest + 8 * 300
+    assert st.arena_peak_bytes <= arena
     for i in range(len(tr)):
         lo, hi = stores[i]
         assert _same(lo, hi, got[i], *alone[i]), f"request {i} differs under admission limits"
@@ -147,7 +161,7 @@ def test_empty_store_and_errors(gpu):
     ub = np.array([10, 3, 3], np.int32)
     p = pcx.PIR()
     p.tell(op, x, y, z, lb, ub)
-    srv = pcx.Server(max_slots=2, arena_bytes=64)
+    srv = pcx.Server(max_slots=2, arena_bytes=64)      # full sweeps: 8 B per variable
     n = srv.add_network(p)
     e_lo, e_hi = lb.copy(), ub.copy()
     e_lo[1], e_hi[1] = 5, 4                      # empty interval: fails at once
@@ -168,3 +182,23 @@ def test_empty_store_and_errors(gpu):
     assert e.value.status == pcx.EINVAL
     with pytest.raises(pcx.PcxError):
         srv.submit(5, lb.copy(), ub.copy(), tag=11)                         # unknown network
+
+
+def test_auto_marks_large_requests_only(gpu):
+    """Automatic mode: requests of at least 2^17 propagators run marked sweeps,
+    smaller ones full sweeps, in the same slot batch; every result equals the
+    request propagated alone."""
+    tr, pirs = _trace(35, 30, rate=1e6, n_nets=4, pmin=2000, pmax=300000)
+    sizes = [p.num_deductions() for p in pirs]
+    used = set(tr.net.tolist())
+    assert any(sizes[i] >= 1 << 17 for i in used) and any(sizes[i] < 1 << 17 for i in used)
+    alone = []
+    for i in range(len(tr)):
+        lo, hi = (a.copy() for a in tr.store(i))
+        alone.append((lo, hi, pirs[tr.net[i]].fixpoint(lo, hi)))
+    stores, got, st = _serve(tr, pirs, max_slots=64, sweeps_per_tick=4)
+    assert st.max_running > 4
+    for i in range(len(tr)):
+        assert _same(*stores[i], got[i], *alone[i]), f"request {i} ({sizes[tr.net[i]]} propagators)"
+        if sizes[tr.net[i]] >= 1 << 17 and not got[i].failed:
+            assert got[i].evals < got[i].sweeps * sizes[tr.net[i]]   # marked sweeps did less work
