@@ -429,24 +429,35 @@ enum BitsOp : int32_t { kBLeq = 0, kBAdd = 1, kBNeq = 2, kBEqr = 3, kBitsOps = 4
 
 typedef unsigned long long u64;
 
-// {i + j : i in a, j in b} within [0,63]
-__device__ __forceinline__ u64 word_sum(u64 a, u64 b) {
-  u64 r = 0;
-#pragma unroll 8
-  for (int j = 0; j < 64; ++j) r |= ((b >> j) & 1ull) ? (a << j) : 0ull;
-  return r;
-}
-// {i - j : i in a, j in b} within [0,63]
-__device__ __forceinline__ u64 word_diff(u64 a, u64 b) {
-  u64 r = 0;
-#pragma unroll 8
-  for (int j = 0; j < 64; ++j) r |= ((b >> j) & 1ull) ? (a >> j) : 0ull;
-  return r;
-}
 __device__ __forceinline__ bool word_single(u64 a) { return a && !(a & (a - 1)); }
 // values <= v / values >= v
 __device__ __forceinline__ u64 word_upto(int v) { return v >= 63 ? ~0ull : (v < 0 ? 0ull : ((2ull << v) - 1ull)); }
 __device__ __forceinline__ u64 word_from(int v) { return v <= 0 ? ~0ull : (v > 63 ? 0ull : (~0ull << v)); }
+
+// Membership tests for the set arithmetic of ADD over [0,63]:
+// add_residual(X, Z) = {v : exists z in Z, v + z in X} (= X - Z within [0,63]),
+// bit v set iff (X >> v) & Z != 0;
+// add_support(Y, Z) = {x : exists y in Y, x - y in Z} (= Y + Z within [0,63]),
+// bit x set iff (bitreverse(Z) >> (63 - x)) & Y != 0 (bit y of that word is
+// bit 63 - (x - y) of bitreverse(Z), i.e. x - y in Z, for y <= x).
+// One test per candidate value instead of a shifted OR per member.
+__device__ __forceinline__ u64 add_residual(u64 X, u64 Z) {
+  uint32_t lo = 0, hi = 0;
+#pragma unroll 8
+  for (int v = 0; v < 32; ++v) lo |= ((X >> v) & Z) ? (1u << v) : 0u;
+#pragma unroll 8
+  for (int v = 32; v < 64; ++v) hi |= ((X >> v) & Z) ? (1u << (v - 32)) : 0u;
+  return ((u64)hi << 32) | lo;
+}
+__device__ __forceinline__ u64 add_support(u64 Y, u64 Z) {
+  const u64 rz = __builtin_bitreverse64(Z);
+  uint32_t lo = 0, hi = 0;
+#pragma unroll 8
+  for (int x = 0; x < 32; ++x) lo |= ((rz >> (63 - x)) & Y) ? (1u << x) : 0u;
+#pragma unroll 8
+  for (int x = 32; x < 64; ++x) hi |= ((rz >> (63 - x)) & Y) ? (1u << (x - 32)) : 0u;
+  return ((u64)hi << 32) | lo;
+}
 
 // In: words X, Y, Z.  Out: masks to AND into x, y, z (all-ones = untouched).
 __device__ __forceinline__ void deduce_bits(int op, u64 X, u64 Y, u64 Z, u64& nx, u64& ny, u64& nz) {
@@ -459,10 +470,10 @@ __device__ __forceinline__ void deduce_bits(int op, u64 X, u64 Y, u64 Z, u64& nx
       return;
     }
     case kBAdd: {                         // x = y + z: y, z from residuals, then x
-      const u64 y2 = Y & word_diff(X, Z);
-      const u64 z2 = Z & word_diff(X, y2);
+      const u64 y2 = Y & add_residual(X, Z);
+      const u64 z2 = Z & add_residual(X, y2);
       ny = y2; nz = z2;
-      nx = X & word_sum(y2, z2);
+      nx = X & add_support(y2, z2);
       return;
     }
     case kBNeq:                           // y != z: shave a fixed side off the other
