@@ -477,7 +477,7 @@ def main():
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4])
     ap.add_argument("--mode", default="batched", choices=["batched", "persistent"],
                     help="engine for the timed batch (the plain engine is always run once beside it)")
-    ap.add_argument("--ppw", type=int, default=0, choices=[0, 64, 128, 256, 512],
+    ap.add_argument("--ppw", type=int, default=0, choices=[0, 64, 128, 256, 512, 1024, 2048],
                     help="propagators per wave (0 = engine default)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--compact", type=int, default=-1, choices=[-1, 0, 1],

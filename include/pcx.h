@@ -71,7 +71,9 @@ typedef struct pcx_batch pcx_batch;
 
 typedef struct {
   int32_t device;           /* HIP device ordinal                                   */
-  int32_t props_per_wave;   /* 64, 128, 256 or 512; 0 = chosen from the sweep size   */
+  int32_t props_per_wave;   /* 64 .. 512 (powers of two; batched mode also 1024 and
+                               2048: work items of 2 or 4 rounds of 512); 0 = chosen
+                               from the sweep size                                   */
   int32_t lds_mode;         /* persistent mode: stage each store in LDS; -1 auto
                                (stores up to 32 KB), 0 never, 1 whenever it fits
                                (stores up to 64 KB)                                */
@@ -198,7 +200,7 @@ pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results,
  * One point of the search: how a batch is swept. */
 typedef struct {
   int32_t mode;             /* PCX_MODE_BATCHED or PCX_MODE_PERSISTENT          */
-  int32_t props_per_wave;   /* batched: 64, 128, 256, 512                       */
+  int32_t props_per_wave;   /* batched: 64 .. 2048                              */
   int32_t lds_mode;         /* persistent: 0 or 1 (stores staged in LDS)        */
   int32_t compact_tables;   /* 0 or 1 (only varied for batches with subsets)    */
   int32_t stream_tables;    /* 0 or 1                                           */

@@ -45,7 +45,9 @@ double now_ms() {
              std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-bool valid_ppw(int p) { return p == 64 || p == 128 || p == 256 || p == 512; }
+bool valid_ppw(int p) {
+  return p == 64 || p == 128 || p == 256 || p == 512 || p == 1024 || p == 2048;
+}
 
 template <class T>
 void dfree(T*& p) {
@@ -131,7 +133,8 @@ Table table_of(const pcx_engine* e) {
 // per-block overhead); the batched mode walks a grid-stride item list and
 // always uses 512 unless told otherwise.
 int ppw_of(const pcx_engine* e, int64_t m = -1) {
-  if (e->cfg.props_per_wave) return e->cfg.props_per_wave;
+  // plain sweeps unroll at most 512 per wave (batched items go to 2048)
+  if (e->cfg.props_per_wave) return m < 0 ? e->cfg.props_per_wave : std::min(e->cfg.props_per_wave, 512);
   if (m < 0) return 512;
   int cus = 256;
   hipDeviceProp_t prop;
@@ -392,18 +395,27 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     InstFlags* f = b->d_flags + k;
     int32_t s = 0;
     for (;;) {
+      // Full sweeps are timed one by one; consecutive marked sweeps as one span.
+      int n_marked = 0;
       for (int g = 0; g < group; ++g, ++s) {
         if (e->cfg.max_sweeps > 0 && s >= e->cfg.max_sweeps) break;
         const bool marked = b->events && s > 0;
-        pcx_status ts = tm.begin(e->stream);
+        pcx_status ts = PCX_OK;
+        if (!marked || n_marked == 0) ts = tm.begin(e->stream);
         if (ts != PCX_OK) return ts;
-        if (marked)
+        if (marked) {
           PCX_HIP(launch_sweep_marked_plain(t, st, s, f, b->d_tight + kTightSlots, ev, k, dom_of(e),
                                             e->stream));
-        else
+          ++n_marked;
+        } else {
           PCX_HIP(launch_sweep_plain(tk, st, idx, m, s, f, b->d_tight, ppw, dom_of(e), b->stream,
                                      e->stream, b->events ? &ev : nullptr, k));
-        ts = tm.end(e->stream, !marked);
+          ts = tm.end(e->stream, true);
+          if (ts != PCX_OK) return ts;
+        }
+      }
+      if (n_marked > 0) {
+        pcx_status ts = tm.end(e->stream, false, n_marked);
         if (ts != PCX_OK) return ts;
       }
       PCX_HIP(hipMemcpyAsync(&hf, f, sizeof(hf), hipMemcpyDeviceToHost, e->stream));
@@ -474,22 +486,34 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       if (ts != PCX_OK) return ts;
       continue;
     }
+    // Full sweeps are timed one by one; consecutive marked sweeps (with their
+    // scheduler launches) as one span.
+    int n_marked = 0;
     for (int g = 0; g < group; ++g) {
       const bool marked = b->events && sweeps > 0;
+      pcx_status ts = PCX_OK;
+      if (marked && n_marked == 0) ts = tm.begin(e->stream);
+      if (ts != PCX_OK) return ts;
       PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
                               marked ? &ev : nullptr));
-      pcx_status ts = tm.begin(e->stream);
-      if (ts != PCX_OK) return ts;
-      if (marked)
+      if (marked) {
         PCX_HIP(launch_sweep_marked(table_of(e), b->d_store, b->nvars, b->d_flags, wl,
                                     b->d_tight + kTightSlots, ev, dom_of(e), grid, e->stream));
-      else
+        ++n_marked;
+      } else {
+        ts = tm.begin(e->stream);
+        if (ts != PCX_OK) return ts;
         PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
                                      b->d_tight, ppw, dom_of(e), b->stream, grid, e->stream,
                                      b->events ? &ev : nullptr));
-      ts = tm.end(e->stream, !marked);
-      if (ts != PCX_OK) return ts;
+        ts = tm.end(e->stream, true);
+        if (ts != PCX_OK) return ts;
+      }
       ++sweeps;
+    }
+    if (n_marked > 0) {
+      pcx_status ts = tm.end(e->stream, false, n_marked);
+      if (ts != PCX_OK) return ts;
     }
     PCX_HIP(hipMemcpyAsync(&live, b->d_ctl + 2, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
     PCX_HIP(hipStreamSynchronize(e->stream));
@@ -566,7 +590,7 @@ pcx_status pcx_create(const pcx_config* cfg, pcx_engine** out) {
   if (c.domain != PCX_DOMAIN_INTERVAL && c.domain != PCX_DOMAIN_BITSET)
     return fail_with(PCX_EINVAL, "unknown domain");
   if (c.props_per_wave != 0 && !valid_ppw(c.props_per_wave))
-    return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto), 64, 128, 256 or 512");
+    return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto) or a power of two in [64, 2048]");
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail_with(PCX_EDEVICE, "no HIP device");
   if (c.device < 0 || c.device >= n) return fail_with(PCX_EINVAL, "device ordinal out of range");
@@ -660,7 +684,7 @@ pcx_status pcx_set_config(pcx_engine* e, const pcx_config* cfg) {
   if (cfg->device != e->cfg.device || cfg->domain != e->cfg.domain)
     return fail_with(PCX_EINVAL, "device and domain are fixed at creation");
   if (cfg->props_per_wave != 0 && !valid_ppw(cfg->props_per_wave))
-    return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto), 64, 128, 256 or 512");
+    return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto) or a power of two in [64, 2048]");
   e->cfg = *cfg;
   return PCX_OK;
 }
@@ -853,7 +877,7 @@ pcx_status pcx_batch_autotune(pcx_batch* b, int32_t reps, pcx_tuning* best, pcx_
   const pcx_config saved = e->cfg;
   std::vector<pcx_tuning> pts;
   const bool subsets = !b->h_off.empty();
-  for (int ppw : {64, 128, 256, 512})
+  for (int ppw : {64, 128, 256, 512, 1024, 2048})
     for (int c = 0; c <= (subsets ? 1 : 0); ++c)
       for (int st = 0; st <= 1; ++st)
         for (int ev = 0; ev <= 1; ++ev)

@@ -411,24 +411,61 @@ hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, i
 }
 
 // ---- marked-only sweeps ----------------------------------------------------
-// Thread t of an item owns bitmap word w: it reads the word of this sweep,
-// clears it (the bitmap is reused two sweeps later), and evaluates the
-// propagators whose bits are set, marking into the next sweep's bitmap.
+// A marked chunk swept by the whole workgroup: thread t reads bitmap word
+// w0 + t (when `active`) and clears it, a block scan of the popcounts places
+// every set bit's propagator in an LDS list, and the workgroup evaluates the
+// list spread evenly over its threads (a word with many marks no longer
+// serializes one thread).  Every thread of the workgroup calls this.
+constexpr int kPropCap = 1024;
 template <int D>
-__device__ __forceinline__ void sweep_marked_word(const Table& t, int2* __restrict__ st,
-                                                  uint32_t* cur_row, int64_t w, const Marker& mk,
-                                                  bool& chg, bool& fail, int& nt, int& ne) {
-  uint32_t bits = cur_row[w];
-  if (!bits) return;
-  cur_row[w] = 0;
-  while (bits) {
-    const int b = __builtin_ctz(bits);
-    bits &= bits - 1;
-    const int64_t p = w * 32 + b;
-    if (p < t.n) {
-      eval_prop<D, false>(t, st, p, chg, fail, nt, mk);
-      ++ne;
+__device__ __forceinline__ void sweep_marked_chunk(const Table& t, int2* __restrict__ st,
+                                                   uint32_t* cur_row, int64_t w0, int64_t words,
+                                                   bool active, const Marker& mk, bool& chg,
+                                                   bool& fail, int& nt, int& ne, int32_t* s_props,
+                                                   int32_t* s_scan) {
+  const int64_t w = w0 + threadIdx.x;
+  uint32_t bits = 0;
+  if (active && w < words) {
+    bits = cur_row[w];
+    if (bits) cur_row[w] = 0;
+  }
+  const int c = __builtin_popcount(bits);
+  const int lane = lane_id();
+  int incl = c;                                   // inclusive scan within the wave
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int v = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += v;
+  }
+  const int wv = threadIdx.x / kWave;
+  if (lane == kWave - 1) s_scan[wv] = incl;
+  __syncthreads();
+  int off = incl - c, total = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / kWave; ++i) {
+    const int v = s_scan[i];
+    if (i < wv) off += v;
+    total += v;
+  }
+  for (int base = 0; base < total; base += kPropCap) {   // same trip count in every thread
+    uint32_t b2 = bits;
+    int idx = off;
+    while (b2) {
+      const int j = __builtin_ctz(b2);
+      b2 &= b2 - 1;
+      if (idx >= base && idx < base + kPropCap) s_props[idx - base] = (int32_t)(w * 32 + j);
+      ++idx;
     }
+    __syncthreads();
+    const int n = min(kPropCap, total - base);
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+      const int64_t p = s_props[i];
+      if (p < t.n) {
+        eval_prop<D, false>(t, st, p, chg, fail, nt, mk);
+        ++ne;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -445,6 +482,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked_plain(Table t, int2* __
   if (f->changed[(s - 1) & 1] != s - 1 || f->failed_at != kNoFail) return;
   uint32_t* cur = ev.dirty + (int64_t)(s & 1) * ev.n_inst * ev.words + inst * ev.words;
   __shared__ int32_t s_list[kMarkCap];
+  __shared__ int32_t s_props[kPropCap];
+  __shared__ int32_t s_scan[kBlock / kWave];
   __shared__ int32_t s_n, s_over;
   Marker mk = marker_for(&ev, inst, s, true);
   mk.list = s_list;
@@ -456,8 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked_plain(Table t, int2* __
   int nt = 0, ne = 0;
   for (int64_t base = (int64_t)blockIdx.x * kBlock; base < ev.words;
        base += (int64_t)gridDim.x * kBlock) {      // same trip count in every thread
-    const int64_t w = base + threadIdx.x;
-    if (w < ev.words) sweep_marked_word<D>(t, st, cur, w, mk, chg, fail, nt, ne);
+    sweep_marked_chunk<D>(t, st, cur, base, ev.words, true, mk, chg, fail, nt, ne, s_props, s_scan);
     flush_marks(mk);
     __syncthreads();
   }
@@ -570,7 +608,10 @@ hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags
   return hipGetLastError();
 }
 
-template <int PPL, int D, bool NT>
+// R rounds of the PPL-unrolled body per work item (props per wave above 512):
+// larger items spread the per-item costs (claim, slot search, mark flush,
+// verdict) over more propagators without more registers.
+template <int PPL, int D, bool NT, int R>
 __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restrict__ store,
                                                           int64_t nvars, Subsets sub,
                                                           int64_t n_props, InstFlags* flags,
@@ -580,7 +621,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
   const int32_t n_act = *wl.n_act;
   if (n_act <= 0) return;
   const int64_t total = wl.item_base[n_act];
-  constexpr int64_t chunk = (int64_t)kBlock * PPL;
+  constexpr int64_t chunk = (int64_t)kBlock * PPL * R;
   // XCD-aware split: workgroups are dispatched round-robin over the 8 XCDs,
   // so workgroup b runs on XCD b % 8.  Each XCD takes one contiguous eighth
   // of the (instance-major) item list, and its workgroups claim that range's
@@ -619,10 +660,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
     bool chg = false, fail = false;
     int nt = 0;
     if (f->failed_at == kNoFail) {
+      for (int rd = 0; rd < R; ++rd) {
 #pragma unroll
-      for (int j = 0; j < PPL; ++j) {
-        const int64_t q = first + j * kBlock + threadIdx.x;
-        if (q < it.m) eval_prop<D, NT>(it.t, st, prop_at(it.idx, q), chg, fail, nt, mk);
+        for (int j = 0; j < PPL; ++j) {
+          const int64_t q = first + (rd * PPL + j) * kBlock + threadIdx.x;
+          if (q < it.m) eval_prop<D, NT>(it.t, st, prop_at(it.idx, q), chg, fail, nt, mk);
+        }
       }
     }
     publish(f, s, chg, fail, nt, tight);
@@ -639,13 +682,15 @@ hipError_t launch_batched_d(const Table& t, int2* store, int64_t nvars, const Su
                             int64_t n_props, InstFlags* flags, WorkList wl,
                             unsigned long long* tight, int ppl, int grid, const EventSet& ev,
                             hipStream_t stream) {
-#define PCX_BATCHED(P) \
-  k_sweep_batched<P, D, NT><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight, ev)
+#define PCX_BATCHED(P, R) \
+  k_sweep_batched<P, D, NT, R><<<grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, flags, wl, tight, ev)
   switch (ppl) {
-    case 1: PCX_BATCHED(1); break;
-    case 2: PCX_BATCHED(2); break;
-    case 4: PCX_BATCHED(4); break;
-    case 8: PCX_BATCHED(8); break;
+    case 1: PCX_BATCHED(1, 1); break;
+    case 2: PCX_BATCHED(2, 1); break;
+    case 4: PCX_BATCHED(4, 1); break;
+    case 8: PCX_BATCHED(8, 1); break;
+    case 16: PCX_BATCHED(8, 2); break;
+    case 32: PCX_BATCHED(8, 4); break;
     default: return hipErrorInvalidValue;
   }
 #undef PCX_BATCHED
@@ -695,6 +740,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
   uint32_t* cur = ev.dirty + (int64_t)(s & 1) * ev.n_inst * ev.words;
   __shared__ int32_t s_next;
   __shared__ int32_t s_list[kMarkCap];
+  __shared__ int32_t s_props[kPropCap];
+  __shared__ int32_t s_scan[kBlock / kWave];
   __shared__ int32_t s_n, s_over;
   if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; s_over = 0; }
   __syncthreads();
@@ -722,16 +769,16 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
       r -= c;
     }
     chunk = uni(chunk);
-    const int64_t w = chunk * kMarkedWordsPerItem + threadIdx.x;
     Marker mk = marker_for(&ev, k, s, true);
     mk.list = s_list;
     mk.list_n = &s_n;
     mk.overflow = &s_over;
     bool chg = false, fail = false;
     int nt = 0, ne = 0;
-    if (f->failed_at == kNoFail && chunk >= 0 && w < ev.words)
-      sweep_marked_word<D>(t, store + (int64_t)k * nvars, cur + (int64_t)k * ev.words, w, mk, chg,
-                           fail, nt, ne);
+    const bool active = uni((int32_t)(f->failed_at == kNoFail && chunk >= 0)) != 0;
+    sweep_marked_chunk<D>(t, store + (int64_t)k * nvars, cur + (int64_t)k * ev.words,
+                          chunk * kMarkedWordsPerItem, ev.words, active, mk, chg, fail, nt, ne,
+                          s_props, s_scan);
     publish(f, s, chg, fail, nt, tight);
     count_evals(f, ne);
     flush_marks(mk);
@@ -892,6 +939,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_slots(const SlotDesc* slots, I
   int32_t* qc = wl.xq + xcd;
   __shared__ int32_t s_next;
   __shared__ int32_t s_list[kMarkCap];
+  __shared__ int32_t s_props[kPropCap];
+  __shared__ int32_t s_scan[kBlock / kWave];
   __shared__ int32_t s_n, s_over;
   if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; s_over = 0; }
   __syncthreads();
@@ -940,10 +989,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_slots(const SlotDesc* slots, I
         rr -= pc;
       }
       c = uni(c);
-      const int64_t w = c * kChunkWords + threadIdx.x;
-      if (f->failed_at == kNoFail && c >= 0 && w < sd.words)
-        sweep_marked_word<D>(t, sd.store, sd.marks + (int64_t)(s & 1) * sd.words, w, mk, chg, fail,
-                             nt, ne);
+      const bool active = uni((int32_t)(f->failed_at == kNoFail && c >= 0)) != 0;
+      sweep_marked_chunk<D>(t, sd.store, sd.marks + (int64_t)(s & 1) * sd.words, c * kChunkWords,
+                            sd.words, active, mk, chg, fail, nt, ne, s_props, s_scan);
     }
     publish(f, s, chg, fail, nt, tight);
     count_evals(f, ne);

@@ -58,6 +58,10 @@ class Result(ctypes.Structure):
                 ("sweeps", ctypes.c_int64), ("evals", ctypes.c_int64)]
 
 
+RESULT_DTYPE = np.dtype([("has_changed", np.int32), ("failed", np.int32), ("sweeps", np.int64),
+                         ("evals", np.int64)])
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("wall_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
                 ("kernel_launches", ctypes.c_int64), ("evals", ctypes.c_int64),
@@ -376,10 +380,14 @@ class Batch:
         return lb, ub
 
     def run(self, mode=MODE_BATCHED):
-        res = (Result * self.n_inst)()
+        """Propagate every instance; returns (results, Stats).  results is a
+        numpy record array (fields has_changed, failed, sweeps, evals; one
+        record per instance, attribute access r.failed)."""
+        res = np.empty(self.n_inst, RESULT_DTYPE)
         st = Stats()
-        _check(lib().pcx_batch_run(self._h, mode, res, ctypes.byref(st)))
-        return list(res), st
+        _check(lib().pcx_batch_run(self._h, mode, res.ctypes.data_as(ctypes.POINTER(Result)),
+                                   ctypes.byref(st)))
+        return res.view(np.recarray), st
 
     def autotune(self, reps=2):
         """Time every sweep configuration on the loaded image, configure the

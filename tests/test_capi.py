@@ -132,3 +132,10 @@ def test_tuning_cache_round_trip(tmp_path):
     pcx.save_tuning(path, "b", dict(best, mode=2))
     assert pcx.load_tuning(path, "a")["best"] == best
     assert pcx.load_tuning(path, "b")["best"]["mode"] == 2
+
+
+def test_result_record_layout_matches_c():
+    import ctypes
+    assert pcx.RESULT_DTYPE.itemsize == ctypes.sizeof(pcx.Result)
+    for name, _ in pcx.Result._fields_:
+        assert pcx.RESULT_DTYPE.fields[name][1] == getattr(pcx.Result, name).offset

@@ -272,7 +272,7 @@ def test_autotuner_choice_is_bit_exact(gpu, tmp_path):
     b = pcx.Batch(pir, n_inst, V, off, idx)
     b.load(slb, sub)
     best, points = b.autotune(reps=1)
-    assert len(points) == 32 + 4          # batched grid + persistent x {LDS, global} x {index, copy}
+    assert len(points) == 48 + 4          # batched grid + persistent x {LDS, global} x {index, copy}
     assert all(p["ms"] > 0 for p in points)
     assert best["ms"] == min(p["ms"] for p in points)
     c = pir.get_config()
@@ -334,3 +334,24 @@ def test_marked_sweeps_when_a_work_item_tightens_many_variables(gpu, mode):
     assert bool(rg.failed) == bool(ro.failed)
     if not ro.failed:
         assert (glb == lo).all() and (gub == hi).all()
+
+
+@pytest.mark.parametrize("ppw", [1024, 2048])
+def test_batched_long_work_items(gpu, ppw):
+    """Work items of 2 and 4 rounds of 512 propagators per wave."""
+    V, P, n_inst = 5000, 20000, 64
+    op, x, y, z, lb, ub, w = _network(51, V, P)
+    pir = pcx.PIR(props_per_wave=ppw)
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_lb, root_ub).failed       # plain sweeps cap at 512 per wave
+    slb, sub = pcx.gen_splits(52, root_lb, root_ub, n_inst, 1, 20)
+    off, idx = pcx.gen_subsets(53, P, n_inst, 200, P)
+    top, tx, ty, tz = pir.table()
+    lb_o, ub_o = slb.copy(), sub.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lb_o, ub_o, V, idx=idx.astype(np.int64), off=off, threads=8)
+    b = pcx.Batch(pir, n_inst, V, off, idx)
+    b.load(slb, sub)
+    res_g, _ = b.run(pcx.MODE_BATCHED)
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"batched ppw {ppw}")
