@@ -189,6 +189,35 @@ pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub);
 pcx_status pcx_batch_reset(pcx_batch* b);
 pcx_status pcx_batch_read(pcx_batch* b, int32_t* lb, int32_t* ub);
 
+/* Entailment over a batch, on the current stores (after pcx_batch_run):
+ *   replaces PIR::ask(int) / PIR::is_extractable (pir.hpp:368-370, 417-438,
+ *   873-884) for every instance at once.
+ * extractable (n_inst entries, may be NULL): 1 when the store holds no empty
+ *   interval and every propagator of the instance is entailed (the interval
+ *   store itself is taken as extractable).
+ * live_off (n_inst + 1, may be NULL): prefix offsets of the propagators that
+ *   are not entailed; live_idx (capacity cap, may be NULL): their table
+ *   positions, ascending per instance -- entailed propagators bring no more
+ *   information (pir.hpp:886-889), so these are the subsets a next batch of
+ *   the same search nodes needs.  PCX_EINVAL when cap is too small
+ *   (live_off[n_inst] then holds the size needed).  Interval domain only. */
+pcx_status pcx_batch_ask(pcx_batch* b, int32_t* extractable, int64_t* live_off,
+                         int32_t* live_idx, int64_t cap);
+
+/* Branching on the device: the search step after a fixpoint (the split of a
+ * search node into two, as the reference's tests build splits with embed,
+ * pir.hpp:354-356, and snapshot/restore, pir.hpp:857-870).  child must hold
+ * 2 * parent's instances over the same engine and nvars (its subsets are the
+ * caller's; typically each parent's twice).  For parent instance k, the
+ * branching variable is the first variable, in input order, that is not
+ * fixed; child 2k gets its lower part and child 2k+1 the rest: intervals are
+ * bisected ([lb, mid] / [mid + 1, ub], mid = lb + floor((ub - lb) / 2)),
+ * bitsets split into {min} / the other values.  A parent that is empty or
+ * fully fixed has two copies as children and var -1.  The children become
+ * the child batch's stores and its reset image.  var (n_inst entries of the
+ * parent, may be NULL) receives the branching variables. */
+pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var);
+
 #define PCX_MODE_PLAIN      0  /* instance by instance, one sweep launch each   */
 #define PCX_MODE_BATCHED    1  /* all instances per sweep launch, device queue  */
 #define PCX_MODE_PERSISTENT 2  /* one persistent launch for the whole batch     */

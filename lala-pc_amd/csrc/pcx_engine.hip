@@ -868,6 +868,85 @@ pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results, pcx_st
   return s;
 }
 
+pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var) {
+  if (!parent || !child) return fail_with(PCX_EINVAL, "NULL batch");
+  if (parent->e != child->e) return fail_with(PCX_EINVAL, "parent and child batches must share the engine");
+  if (child->n_inst != 2 * parent->n_inst || child->nvars != parent->nvars)
+    return fail_with(PCX_EINVAL, "child batch must hold 2 * parent instances of the same nvars");
+  if (parent == child) return fail_with(PCX_EINVAL, "parent and child must differ");
+  pcx_engine* e = parent->e;
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  int32_t* d_var = nullptr;
+  PCX_HIP(hipMalloc(&d_var, std::max<int64_t>(parent->n_inst, 1) * sizeof(int32_t)));
+  hipError_t he = launch_branch(parent->d_store, parent->nvars, parent->n_inst, d_var, child->d_store,
+                                child->d_image, dom_of(e), e->stream);
+  if (he == hipSuccess && var)
+    he = hipMemcpyAsync(var, d_var, parent->n_inst * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+  (void)hipFree(d_var);
+  if (he != hipSuccess) return fail_with(PCX_EDEVICE, std::string("branch: ") + hipGetErrorString(he));
+  return image_loaded(child);
+}
+
+pcx_status pcx_batch_ask(pcx_batch* b, int32_t* extractable, int64_t* live_off, int32_t* live_idx,
+                         int64_t cap) {
+  if (!b) return fail_with(PCX_EINVAL, "NULL batch");
+  pcx_engine* e = b->e;
+  if (e->cfg.domain != PCX_DOMAIN_INTERVAL) return fail_with(PCX_EINVAL, "entailment is defined for interval batches");
+  if (live_idx && !live_off) return fail_with(PCX_EINVAL, "live_idx needs live_off");
+  if ((int64_t)e->op.size() != b->table_n) return fail_with(PCX_EINVAL, "table changed since the batch was created");
+  if (e->max_var >= b->nvars) return fail_with(PCX_EINVAL, "a propagator names a variable outside the store");
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  pcx_status st = sync_table(e);
+  if (st != PCX_OK) return st;
+  const int64_t n = b->n_inst;
+  const Table t = table_of(e);
+  const Subsets sub{b->h_off.empty() ? nullptr : b->d_off, b->d_idx};
+  int64_t* d_live = nullptr;
+  int32_t* d_empty = nullptr;
+  int64_t* d_off = nullptr;
+  int32_t* d_out = nullptr;
+  auto release = [&]() { dfree(d_live); dfree(d_empty); dfree(d_off); dfree(d_out); };
+  std::vector<int64_t> live(n);
+  std::vector<int32_t> empty(n);
+  hipError_t he = hipMalloc(&d_live, n * sizeof(int64_t));
+  if (he == hipSuccess) he = hipMalloc(&d_empty, n * sizeof(int32_t));
+  if (he == hipSuccess)
+    he = launch_batch_ask(t, b->d_store, b->nvars, sub, b->table_n, n, d_live, d_empty, nullptr, nullptr,
+                          e->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(live.data(), d_live, n * sizeof(int64_t), hipMemcpyDeviceToHost, e->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(empty.data(), d_empty, n * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+  if (he != hipSuccess) {
+    release();
+    return fail_with(PCX_EDEVICE, std::string("batch ask: ") + hipGetErrorString(he));
+  }
+  std::vector<int64_t> off(n + 1, 0);
+  for (int64_t k = 0; k < n; ++k) off[k + 1] = off[k] + live[k];
+  if (extractable)
+    for (int64_t k = 0; k < n; ++k) extractable[k] = !empty[k] && live[k] == 0;
+  if (live_off) std::copy(off.begin(), off.end(), live_off);
+  if (live_idx) {
+    if (off[n] > cap) {
+      release();
+      return fail_with(PCX_EINVAL, "live_idx capacity too small (live_off[n_inst] holds the size needed)");
+    }
+    if (off[n] > 0) {
+      he = hipMalloc(&d_off, (n + 1) * sizeof(int64_t));
+      if (he == hipSuccess) he = hipMalloc(&d_out, off[n] * sizeof(int32_t));
+      if (he == hipSuccess) he = hipMemcpyAsync(d_off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, e->stream);
+      if (he == hipSuccess)
+        he = launch_batch_ask(t, b->d_store, b->nvars, sub, b->table_n, n, nullptr, nullptr, d_off, d_out,
+                              e->stream);
+      if (he == hipSuccess) he = hipMemcpyAsync(live_idx, d_out, off[n] * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
+      if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    }
+  }
+  release();
+  if (he != hipSuccess) return fail_with(PCX_EDEVICE, std::string("batch ask: ") + hipGetErrorString(he));
+  return PCX_OK;
+}
+
 pcx_status pcx_batch_autotune(pcx_batch* b, int32_t reps, pcx_tuning* best, pcx_tuning* results,
                               int32_t max_results, int32_t* n_results) {
   if (!b || !best) return fail_with(PCX_EINVAL, "NULL argument");

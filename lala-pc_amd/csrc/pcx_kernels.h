@@ -166,6 +166,16 @@ hipError_t launch_sweep_slots(const SlotDesc* slots, InstFlags* flags, WorkList 
                               hipStream_t st);
 hipError_t launch_compact_table(const Table& t, const int32_t* idx, int64_t n, uint32_t* xo,
                                 int32_t* y, int32_t* z, hipStream_t st);
+// Entailment over a batch: out_idx == nullptr counts (live[k] = propagators
+// not entailed, empty[k] = store holds an empty interval); otherwise writes
+// those propagators' table positions at out_off[k].
+hipError_t launch_batch_ask(const Table& t, const int2* store, int64_t nvars, const Subsets& sub,
+                            int64_t n_props, int64_t n_inst, int64_t* live, int32_t* empty,
+                            const int64_t* out_off, int32_t* out_idx, hipStream_t st);
+// Branching: var[k] = split variable of parent instance k (or -1); children
+// 2k / 2k+1 written to child and child_image.
+hipError_t launch_branch(const int2* parent, int64_t nvars, int64_t n_inst, int32_t* var, int2* child,
+                         int2* child_image, int dom, hipStream_t st);
 hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
                       hipStream_t st);
 hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n, hipStream_t st);

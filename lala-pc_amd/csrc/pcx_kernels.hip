@@ -1256,6 +1256,178 @@ hipError_t launch_compact_table(const Table& t, const int32_t* idx, int64_t n, u
   return hipGetLastError();
 }
 
+// ---- entailment over a batch ----------------------------------------------
+// One workgroup per instance (grid-stride over instances).  count: the
+// instance's propagators that are not entailed on its store, and whether the
+// store holds an empty interval.  write: those propagators' table positions,
+// ascending, at out_off[k] (a block scan per 256 propagators keeps the order).
+__device__ __forceinline__ bool inst_prop_entailed(const Table& t, const int2* st,
+                                                   const Subsets& sub, int64_t k, int64_t q,
+                                                   int64_t& p) {
+  p = sub.off ? (int64_t)sub.idx[sub.off[k] + q] : q;
+  const uint32_t xo = t.xo[p];
+  return entailed((int)(xo >> kOpShift), st[xo & kVarMask], st[t.y[p]], st[t.z[p]]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_ask_count(Table t, const int2* __restrict__ store,
+                                                            int64_t nvars, Subsets sub,
+                                                            int64_t n_props, int64_t n_inst,
+                                                            int64_t* live, int32_t* empty) {
+  __shared__ int s_cnt, s_empty;
+  for (int64_t k = blockIdx.x; k < n_inst; k += gridDim.x) {
+    if (threadIdx.x == 0) { s_cnt = 0; s_empty = 0; }
+    __syncthreads();
+    const int2* st = store + k * nvars;
+    const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
+    int c = 0;
+    for (int64_t q = threadIdx.x; q < m; q += kBlock) {
+      int64_t p;
+      c += !inst_prop_entailed(t, st, sub, k, q, p);
+    }
+    bool e = false;
+    for (int64_t v = threadIdx.x; v < nvars; v += kBlock) e |= st[v].x > st[v].y;
+    c = wave_sum(c);
+    if (lane_id() == 0 && c) atomicAdd(&s_cnt, c);
+    if (__ballot(e) && lane_id() == 0) s_empty = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) { live[k] = s_cnt; empty[k] = s_empty; }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_ask_write(Table t, const int2* __restrict__ store,
+                                                            int64_t nvars, Subsets sub,
+                                                            int64_t n_props, int64_t n_inst,
+                                                            const int64_t* out_off,
+                                                            int32_t* out_idx) {
+  __shared__ int s_scan[kBlock / kWave];
+  for (int64_t k = blockIdx.x; k < n_inst; k += gridDim.x) {
+    const int2* st = store + k * nvars;
+    const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
+    int64_t at = out_off[k];
+    for (int64_t q0 = 0; q0 < m; q0 += kBlock) {               // same trip count in every thread
+      const int64_t q = q0 + threadIdx.x;
+      int64_t p = -1;
+      const int keep = q < m && !inst_prop_entailed(t, st, sub, k, q, p);
+      const int lane = lane_id(), wv = threadIdx.x / kWave;
+      int incl = keep;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int v = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += v;
+      }
+      if (lane == kWave - 1) s_scan[wv] = incl;
+      __syncthreads();
+      int off = incl - keep, total = 0;
+#pragma unroll
+      for (int i = 0; i < kBlock / kWave; ++i) {
+        if (i < wv) off += s_scan[i];
+        total += s_scan[i];
+      }
+      if (keep) out_idx[at + off] = (int32_t)p;
+      at += total;
+      __syncthreads();                                          // s_scan is rewritten next chunk
+    }
+  }
+}
+
+hipError_t launch_batch_ask(const Table& t, const int2* store, int64_t nvars, const Subsets& sub,
+                            int64_t n_props, int64_t n_inst, int64_t* live, int32_t* empty,
+                            const int64_t* out_off, int32_t* out_idx, hipStream_t stream) {
+  const int64_t grid = n_inst < 65536 ? n_inst : 65536;
+  if (grid <= 0) return hipSuccess;
+  if (!out_idx)
+    k_batch_ask_count<<<(unsigned)grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, n_inst,
+                                                            live, empty);
+  else
+    k_batch_ask_write<<<(unsigned)grid, kBlock, 0, stream>>>(t, store, nvars, sub, n_props, n_inst,
+                                                            out_off, out_idx);
+  return hipGetLastError();
+}
+
+// ---- branching: two children per instance ---------------------------------
+// var[k] = the first variable (input order) of instance k's store that is not
+// fixed, or -1 when the store is empty or every variable is fixed.
+template <bool BITS>
+__global__ __launch_bounds__(kBlock) void k_branch_pick(const int2* __restrict__ store,
+                                                        int64_t nvars, int64_t n_inst,
+                                                        int32_t* var) {
+  __shared__ int32_t s_first, s_empty;
+  for (int64_t k = blockIdx.x; k < n_inst; k += gridDim.x) {
+    if (threadIdx.x == 0) { s_first = 0x7fffffff; s_empty = 0; }
+    __syncthreads();
+    const int2* st = store + k * nvars;
+    int32_t first = 0x7fffffff;
+    bool empty = false;
+    for (int64_t v = threadIdx.x; v < nvars; v += kBlock) {
+      const int2 b = st[v];
+      if (BITS) {
+        const u64 wv = ((u64)(uint32_t)b.y << 32) | (uint32_t)b.x;
+        empty |= wv == 0;
+        if (wv & (wv - 1)) first = min(first, (int32_t)v);
+      } else {
+        empty |= b.x > b.y;
+        if (b.x < b.y) first = min(first, (int32_t)v);
+      }
+    }
+    if (first != 0x7fffffff) atomicMin(&s_first, first);
+    if (__ballot(empty) && lane_id() == 0) s_empty = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) var[k] = (s_empty || s_first == 0x7fffffff) ? -1 : s_first;
+    __syncthreads();
+  }
+}
+
+// child[2k] and child[2k+1] = parent k with variable var[k] split: intervals
+// [lb, mid] / [mid+1, ub] (mid = lb + (ub - lb) / 2, floor), bitsets {min} /
+// the rest.  var[k] == -1: both children are copies.  Writes the stores and
+// their image.
+template <bool BITS>
+__global__ void k_branch_write(const int2* __restrict__ parent, int64_t nvars, int64_t n_inst,
+                               const int32_t* __restrict__ var, int2* child, int2* child_image) {
+  const int64_t total = n_inst * nvars;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = i / nvars, v = i - k * nvars;
+    const int2 b = parent[i];
+    int2 left = b, right = b;
+    if (var[k] == v) {
+      if (BITS) {
+        const u64 wv = ((u64)(uint32_t)b.y << 32) | (uint32_t)b.x;
+        const u64 lo = wv & (~wv + 1), hi = wv & ~lo;
+        left = make_int2((int32_t)(uint32_t)lo, (int32_t)(uint32_t)(lo >> 32));
+        right = make_int2((int32_t)(uint32_t)hi, (int32_t)(uint32_t)(hi >> 32));
+      } else {
+        const int64_t mid = (int64_t)b.x + (((int64_t)b.y - (int64_t)b.x) >> 1);
+        left.y = (int32_t)mid;
+        right.x = (int32_t)(mid + 1);
+      }
+    }
+    const int64_t c0 = (2 * k) * nvars + v, c1 = c0 + nvars;
+    child[c0] = left;
+    child[c1] = right;
+    child_image[c0] = left;
+    child_image[c1] = right;
+  }
+}
+
+hipError_t launch_branch(const int2* parent, int64_t nvars, int64_t n_inst, int32_t* var, int2* child,
+                         int2* child_image, int dom, hipStream_t stream) {
+  if (n_inst <= 0) return hipSuccess;
+  const int64_t g1 = n_inst < 65536 ? n_inst : 65536;
+  const int64_t total = n_inst * nvars;
+  int64_t g2 = (total + 255) / 256;
+  if (g2 > 65536) g2 = 65536;
+  if (dom == kBits) {
+    k_branch_pick<true><<<(unsigned)g1, kBlock, 0, stream>>>(parent, nvars, n_inst, var);
+    k_branch_write<true><<<(unsigned)g2, 256, 0, stream>>>(parent, nvars, n_inst, var, child, child_image);
+  } else {
+    k_branch_pick<false><<<(unsigned)g1, kBlock, 0, stream>>>(parent, nvars, n_inst, var);
+    k_branch_write<false><<<(unsigned)g2, 256, 0, stream>>>(parent, nvars, n_inst, var, child, child_image);
+  }
+  return hipGetLastError();
+}
+
 __global__ void k_ask(Table t, const int2* __restrict__ st, uint8_t* ent, int32_t* all_flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool ok = true;

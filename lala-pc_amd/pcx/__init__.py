@@ -143,6 +143,8 @@ SIGNATURES = [
     ("pcx_batch_reset", ctypes.c_int32, [_VP]),
     ("pcx_batch_read", ctypes.c_int32, [_VP, _P32, _P32]),
     ("pcx_batch_run", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Result), ctypes.POINTER(Stats)]),
+    ("pcx_batch_branch", ctypes.c_int32, [_VP, _VP, _P32]),
+    ("pcx_batch_ask", ctypes.c_int32, [_VP, _P32, _P64, _P32, ctypes.c_int64]),
     ("pcx_batch_autotune", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Tuning),
                                             ctypes.POINTER(Tuning), ctypes.c_int32, _P32]),
     ("pcx_bits_tell", ctypes.c_int32, [_VP, _P32, _P32, _P32, _P32, ctypes.c_int64, _PU64, ctypes.c_int64]),
@@ -388,6 +390,28 @@ class Batch:
         _check(lib().pcx_batch_run(self._h, mode, res.ctypes.data_as(ctypes.POINTER(Result)),
                                    ctypes.byref(st)))
         return res.view(np.recarray), st
+
+    def branch(self, child):
+        """Split every instance on its first unfixed variable into child
+        instances 2k / 2k+1 (child: a Batch of 2 * n_inst instances over the
+        same engine); returns the branching variables (-1: no split)."""
+        var = np.zeros(max(self.n_inst, 1), np.int32)
+        _check(lib().pcx_batch_branch(self._h, child._h, _p32(var)))
+        return var[:self.n_inst]
+
+    def ask(self, live=True):
+        """Entailment on the current stores: (extractable bool[n_inst],
+        live_off int64[n_inst+1], live_idx int32[...]) -- the propagators of
+        each instance that are not entailed (None, None without `live`)."""
+        ext = np.zeros(self.n_inst, np.int32)
+        if not live:
+            _check(lib().pcx_batch_ask(self._h, _p32(ext), None, None, 0))
+            return ext.astype(bool), None, None
+        off = np.zeros(self.n_inst + 1, np.int64)
+        _check(lib().pcx_batch_ask(self._h, _p32(ext), _p64(off), None, 0))
+        idx = np.zeros(max(int(off[-1]), 1), np.int32)
+        _check(lib().pcx_batch_ask(self._h, None, _p64(off), _p32(idx), len(idx)))
+        return ext.astype(bool), off, idx[:int(off[-1])]
 
     def autotune(self, reps=2):
         """Time every sweep configuration on the loaded image, configure the

@@ -115,3 +115,37 @@ def test_bits_persistent_lds(gpu, lds):
     b.load_words(inst)
     rg, _ = b.run(pcx.MODE_PERSISTENT)
     _compare(b.read_words(), rg, wo, ro, V, f"bits persistent lds {lds}")
+
+
+def test_bits_branching(gpu):
+    V, P, n = 1000, 4000, 16
+    op, x, y, z, w = pcx.gen_bits_network(81, V, P)
+    pir = pcx.BitsPIR()
+    pir.tell(op, x, y, z, w)
+    root = w.copy()
+    assert not pir.fixpoint(root).failed
+    inst = pcx.gen_bits_splits(82, root, n, 1, 10)
+    parent = pcx.Batch(pir, n, V)
+    parent.load_words(inst)
+    parent.run(pcx.MODE_BATCHED)
+    pw = parent.read_words().reshape(n, V)
+    child = pcx.Batch(pir, 2 * n, V)
+    var = parent.branch(child)
+    cw = child.read_words().reshape(2 * n, V)
+    for k in range(n):
+        words = pw[k]
+        multi = [v for v in range(V) if int(words[v]) & (int(words[v]) - 1)]
+        expect = -1 if (words == 0).any() or not multi else multi[0]
+        assert var[k] == expect
+        left, right = words.copy(), words.copy()
+        if expect >= 0:
+            wv = int(words[expect])
+            lowbit = wv & -wv
+            left[expect] = np.uint64(lowbit)
+            right[expect] = np.uint64(wv & ~lowbit)
+        assert (cw[2 * k] == left).all() and (cw[2 * k + 1] == right).all()
+    rg, _ = child.run(pcx.MODE_BATCHED)
+    top, tx, ty, tz = pir.table()
+    wo = cw.reshape(-1).copy()
+    ro = ol.bits_fixpoint_batch(top, tx, ty, tz, wo, V, threads=8)
+    _compare(child.read_words(), rg, wo, ro, V, "bits children")

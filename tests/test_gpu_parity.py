@@ -355,3 +355,103 @@ def test_batched_long_work_items(gpu, ppw):
     res_g, _ = b.run(pcx.MODE_BATCHED)
     lb_g, ub_g = b.read()
     _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"batched ppw {ppw}")
+
+
+def test_batch_entailment_matches_oracle(gpu):
+    """pcx_batch_ask on propagated search nodes: the live (not entailed)
+    propagators and the extractable flag of every instance equal the
+    oracle's ask (pir.hpp:417-438, 873-884); the live subsets carry the
+    whole fixpoint (re-running only them changes nothing); a store fixed to
+    the planted witness is extractable."""
+    V, P, n_inst = 2000, 8000, 40
+    op, x, y, z, lb, ub, w = _network(61, V, P)
+    pir = pcx.PIR()
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_lb, root_ub).failed
+    slb, sub = pcx.gen_splits(62, root_lb, root_ub, n_inst, 1, 10)
+    slb = slb.reshape(n_inst, V)
+    sub = sub.reshape(n_inst, V)
+    slb[0] = w                                # instance 0: the witness, every variable fixed
+    sub[0] = w
+    off, idx = pcx.gen_subsets(63, P, n_inst, 100, P)
+    b = pcx.Batch(pir, n_inst, V, off, idx)
+    b.load(slb.reshape(-1), sub.reshape(-1))
+    res, _ = b.run(pcx.MODE_BATCHED)
+    lb_g, ub_g = b.read()
+    ext, loff, lidx = b.ask()
+    top, tx, ty, tz = pir.table()
+    for k in range(n_inst):
+        lo, hi = lb_g[k * V:(k + 1) * V].copy(), ub_g[k * V:(k + 1) * V].copy()
+        props = idx[off[k]:off[k + 1]]
+        live = [int(p) for p in props if not ol.ask(top[p], tx[p], ty[p], tz[p], lo, hi)]
+        assert list(lidx[loff[k]:loff[k + 1]]) == live, f"instance {k}"
+        assert ext[k] == (not res[k].failed and not live), f"instance {k}"
+    assert ext[0]
+    # the live subsets alone keep every fixpoint: nothing changes when they run
+    b2 = pcx.Batch(pir, n_inst, V, loff, lidx)
+    b2.load(lb_g, ub_g)
+    res2, _ = b2.run(pcx.MODE_BATCHED)
+    lb2, ub2 = b2.read()
+    for k in range(n_inst):
+        if not res[k].failed:
+            assert not res2[k].failed and not res2[k].has_changed, f"instance {k}"
+    ok = np.repeat(~np.array([bool(r.failed) for r in res]), V)
+    assert (lb2[ok] == lb_g[ok]).all() and (ub2[ok] == ub_g[ok]).all()
+
+
+def _host_children(lo, hi, n_inst, V):
+    """Branching restated on the host: first unfixed variable, bisected."""
+    lo, hi = lo.reshape(n_inst, V), hi.reshape(n_inst, V)
+    var = np.full(n_inst, -1, np.int32)
+    clo = np.repeat(lo, 2, axis=0).copy()
+    chi = np.repeat(hi, 2, axis=0).copy()
+    for k in range(n_inst):
+        if (lo[k] > hi[k]).any():
+            continue
+        free = np.nonzero(lo[k] < hi[k])[0]
+        if len(free) == 0:
+            continue
+        v = int(free[0])
+        var[k] = v
+        mid = int(lo[k, v]) + (int(hi[k, v]) - int(lo[k, v])) // 2
+        chi[2 * k, v] = mid
+        clo[2 * k + 1, v] = mid + 1
+    return var, clo.reshape(-1), chi.reshape(-1)
+
+
+def test_branching_on_device(gpu):
+    """pcx_batch_branch splits each propagated node on its first unfixed
+    variable; the children equal the host restatement and propagate to the
+    oracle's fixpoints."""
+    V, P, n_inst = 3000, 12000, 24
+    op, x, y, z, lb, ub, w = _network(71, V, P)
+    pir = pcx.PIR()
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_lb, root_ub).failed
+    slb, sub = pcx.gen_splits(72, root_lb, root_ub, n_inst, 1, 6)
+    off, idx = pcx.gen_subsets(73, P, n_inst, 1000, P)
+    parent = pcx.Batch(pir, n_inst, V, off, idx)
+    parent.load(slb, sub)
+    parent.run(pcx.MODE_BATCHED)
+    plo, phi = parent.read()
+    # children inherit their parent's subset
+    sizes = np.diff(off)
+    coff = np.concatenate([[0], np.cumsum(np.repeat(sizes, 2))]).astype(np.int64)
+    cidx = np.concatenate([np.concatenate([idx[off[k]:off[k + 1]]] * 2) for k in range(n_inst)])
+    child = pcx.Batch(pir, 2 * n_inst, V, coff, cidx)
+    var = parent.branch(child)
+    hvar, clo, chi = _host_children(plo, phi, n_inst, V)
+    assert (var == hvar).all()
+    glo, ghi = child.read()
+    assert (glo == clo).all() and (ghi == chi).all()
+    res, _ = child.run(pcx.MODE_BATCHED)
+    lb_g, ub_g = child.read()
+    top, tx, ty, tz = pir.table()
+    lo_o, hi_o = clo.copy(), chi.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lo_o, hi_o, V, idx=cidx.astype(np.int64), off=coff, threads=8)
+    _compare(lb_g, ub_g, res, lo_o, hi_o, res_o, V, "children")
+    child.reset()                             # the children are the child batch's image
+    glo2, ghi2 = child.read()
+    assert (glo2 == clo).all() and (ghi2 == chi).all()
