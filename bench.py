@@ -206,6 +206,9 @@ def bench_batched(args):
                                   "k_sweep_marked", "cfg2"),
     }
 
+    if args.search_step:
+        out["search_step"] = search_step(args, pir, batch, off, idx, res, lb_g, ub_g, V, n_inst)
+
     if not args.no_cpu:
         ol = cpu_oracle()
         top, tx, ty, tz = pir.table()
@@ -246,6 +249,65 @@ def bench_batched(args):
                                          f"thread, oracle Gauss-Seidel (-O3)"}
         out["parity"] = {"checked_instances": k, "mismatches": mism}
         log(f"cpu: {k} instances in {t_cpu:.1f}s on {threads} threads; parity mismatches {mism}")
+    return out
+
+
+def search_step(args, pir, parent, off, idx, res, plo, phi, V, n_inst):
+    """One step of a search on the device after the timed batch: the 4,096
+    propagated subproblems are split on their first unfixed variable into
+    8,192 children (pcx_batch_branch), which are propagated starting from
+    their parents' fixpoints (marks on the split variable's propagators).
+    Reported beside the headline figure; a sample of children is checked
+    against the oracle."""
+    sizes = np.diff(off)
+    coff = np.concatenate([[0], np.cumsum(np.repeat(sizes, 2))]).astype(np.int64)
+    cidx = np.concatenate([np.concatenate([idx[off[k]:off[k + 1]]] * 2) for k in range(n_inst)])
+    child = pcx.Batch(pir, 2 * n_inst, V, coff, cidx)
+    # warm-up: buffers, member bitmaps, graphs
+    parent.branch(child, from_fixpoint=True)
+    child.run(pcx.MODE_BATCHED)
+    t0 = time.perf_counter()
+    var = parent.branch(child, from_fixpoint=True)
+    t1 = time.perf_counter()
+    cres, st = child.run(pcx.MODE_BATCHED)
+    t2 = time.perf_counter()
+    n_child = 2 * n_inst
+    split = int((var >= 0).sum())
+    out = {"children": n_child, "parents_split": split, "branch_ms": (t1 - t0) * 1e3,
+           "propagate_ms": (t2 - t1) * 1e3, "children_fixpoints_per_s": n_child / (t2 - t0),
+           "evals": int(st.evals), "failed_children": int(cres.failed.sum())}
+    log(f"search step: {n_child} children of {split} split parents, branch {out['branch_ms']:.2f} ms, "
+        f"propagate {out['propagate_ms']:.2f} ms, {int(st.evals)} evaluations")
+    if not args.no_cpu:
+        ol = cpu_oracle()
+        top, tx, ty, tz = pir.table()
+        k_max = min(n_inst, 32)
+        clo = np.empty(2 * k_max * V, np.int32)
+        chi = np.empty(2 * k_max * V, np.int32)
+        for k in range(k_max):
+            lo, hi = plo[k * V:(k + 1) * V], phi[k * V:(k + 1) * V]
+            for j in range(2):
+                clo[(2 * k + j) * V:(2 * k + j + 1) * V] = lo
+                chi[(2 * k + j) * V:(2 * k + j + 1) * V] = hi
+            v = int(var[k])
+            if v >= 0:
+                mid = int(lo[v]) + (int(hi[v]) - int(lo[v])) // 2
+                chi[2 * k * V + v] = mid
+                clo[(2 * k + 1) * V + v] = mid + 1
+        ro = ol.fixpoint_batch(top, tx, ty, tz, clo, chi, V, idx=cidx[:coff[2 * k_max]].astype(np.int64),
+                               off=coff[:2 * k_max + 1], threads=host_threads())
+        glo, ghi = child.read()
+        mism = 0
+        for c in range(2 * k_max):
+            if bool(cres[c].failed) != bool(ro[c].failed):
+                mism += 1
+            elif not ro[c].failed and not (
+                    np.array_equal(glo[c * V:(c + 1) * V], clo[c * V:(c + 1) * V]) and
+                    np.array_equal(ghi[c * V:(c + 1) * V], chi[c * V:(c + 1) * V])):
+                mism += 1
+        out["parity"] = {"checked_children": 2 * k_max, "mismatches": mism}
+        log(f"search step parity: {2 * k_max} children, mismatches {mism}")
+    child.close()
     return out
 
 
@@ -495,6 +557,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--search-step", type=int, default=1, choices=[0, 1],
+                    help="config 2: after the timed steps, branch the subproblems on the device and "
+                         "propagate the children (reported beside the headline figure)")
     ap.add_argument("--requests", type=int, default=4000, help="serving: requests in the trace")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: Poisson arrival rate (1/s)")
     ap.add_argument("--slots", type=int, default=256, help="serving: requests on the GPU at once")

@@ -215,8 +215,17 @@ pcx_status pcx_batch_ask(pcx_batch* b, int32_t* extractable, int64_t* live_off,
  * bitsets split into {min} / the other values.  A parent that is empty or
  * fully fixed has two copies as children and var -1.  The children become
  * the child batch's stores and its reset image.  var (n_inst entries of the
- * parent, may be NULL) receives the branching variables. */
-pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var);
+ * parent, may be NULL) receives the branching variables.
+ * from_fixpoint = 1 declares that every parent store is a fixpoint of its
+ * instance's propagators (the parent was run and not changed since) and that
+ * each child's subset is contained in its parent's: the child batch's next
+ * run then starts from marks on the propagators adjacent to the split
+ * variable instead of a full first sweep (same fixpoint, far less work).  The
+ * seeds are used by that one run (later runs after a reset sweep fully), and
+ * only when the run uses marked sweeps (not the persistent mode, not
+ * event_driven = 0). */
+pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var,
+                            int32_t from_fixpoint);
 
 #define PCX_MODE_PLAIN      0  /* instance by instance, one sweep launch each   */
 #define PCX_MODE_BATCHED    1  /* all instances per sweep launch, device queue  */

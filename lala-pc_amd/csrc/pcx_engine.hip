@@ -208,8 +208,8 @@ pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* resu
     r.has_changed = last >= 0;
     r.sweeps = r.failed ? (int64_t)f.failed_at + 1 : (int64_t)last + 2;
     if (e->cfg.max_sweeps > 0 && r.sweeps > e->cfg.max_sweeps) r.sweeps = e->cfg.max_sweeps;
-    if (b->events) {     // one full sweep (unless empty at the start) + the marked evaluations
-      const int64_t first = f.failed_at == -1 ? 0 : inst_size(b, k);
+    if (b->events) {     // one full sweep (unless empty at the start or seeded) + the marked ones
+      const int64_t first = (f.failed_at == -1 || b->seeded_run) ? 0 : inst_size(b, k);
       r.evals = first + f.marked_evals;
       full_evals += first;
     } else {
@@ -290,27 +290,11 @@ int64_t summary_words(const pcx_batch* b) {
 // batches and two dirty bitmaps per instance; automatic mode uses them when
 // they take at most a quarter of free device memory.  The persistent mode
 // always evaluates every propagator.
-pcx_status prepare_events(pcx_batch* b, int32_t mode) {
+// Adjacency, member bitmaps (subset batches) and mark bitmaps of a batch.
+pcx_status ensure_event_buffers(pcx_batch* b) {
   pcx_engine* e = b->e;
-  b->events = false;
-  if (mode == PCX_MODE_PERSISTENT || e->cfg.event_driven == 0 || b->table_n == 0) return PCX_OK;
-  // Automatic mode, one instance at a time: a marked sweep gives up the
-  // propagation within a sweep that a full sweep gets (marks land at the end
-  // of a work item), so small instances converge in fewer, cheap full
-  // sweeps; marked sweeps pay from about 2^20 propagators per instance.
-  if (e->cfg.event_driven < 0 && mode == PCX_MODE_PLAIN) {
-    int64_t mmax = 0;
-    for (int64_t k = 0; k < b->n_inst; ++k) mmax = std::max(mmax, inst_size(b, k));
-    if (mmax < kPlainMarkedMin) return PCX_OK;
-  }
-  if (3 * b->table_n >= ((int64_t)1 << 31)) return PCX_OK;    // adjacency indices are int32
   const int64_t words = bitmap_words(b);
   const int64_t rows = b->n_inst * words;
-  const uint64_t need = (uint64_t)rows * 4 * (b->h_off.empty() ? 2 : 3);
-  if (e->cfg.event_driven < 0 && !b->d_dirty) {
-    size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > free_b / 4) return PCX_OK;
-  }
   pcx_status st = ensure_adjacency(e);
   if (st != PCX_OK) return st;
   if (!b->d_dirty) PCX_HIP(hipMalloc(&b->d_dirty, 2 * rows * sizeof(uint32_t)));
@@ -322,8 +306,55 @@ pcx_status prepare_events(pcx_batch* b, int32_t mode) {
     PCX_HIP(launch_build_members(b->d_off, b->d_idx, b->n_inst, b->d_member, words, e->stream));
     b->member_version = e->version;
   }
+  return PCX_OK;
+}
+
+pcx_status clear_marks(pcx_batch* b) {
+  pcx_engine* e = b->e;
+  const int64_t rows = b->n_inst * bitmap_words(b);
+  const int64_t srows = b->n_inst * summary_words(b);
   PCX_HIP(hipMemsetAsync(b->d_dirty, 0, 2 * rows * sizeof(uint32_t), e->stream));
   PCX_HIP(hipMemsetAsync(b->d_summary, 0, 2 * srows * sizeof(uint32_t), e->stream));
+  return PCX_OK;
+}
+
+bool events_possible(const pcx_batch* b, int32_t mode) {
+  const pcx_engine* e = b->e;
+  return mode != PCX_MODE_PERSISTENT && e->cfg.event_driven != 0 && b->table_n > 0 &&
+         3 * b->table_n < ((int64_t)1 << 31);             // adjacency indices are int32
+}
+
+pcx_status prepare_events(pcx_batch* b, int32_t mode) {
+  pcx_engine* e = b->e;
+  b->events = false;
+  const bool seeded = b->seeded;
+  b->seeded = false;                        // seeds serve one run
+  b->seeded_run = false;
+  if (!events_possible(b, mode)) return PCX_OK;
+  if (seeded && b->d_dirty) {               // marks were seeded by pcx_batch_branch
+    b->events = true;
+    b->seeded_run = true;
+    return PCX_OK;
+  }
+  // Automatic mode, one instance at a time: a marked sweep gives up the
+  // propagation within a sweep that a full sweep gets (marks land at the end
+  // of a work item), so small instances converge in fewer, cheap full
+  // sweeps; marked sweeps pay from about 2^20 propagators per instance.
+  if (e->cfg.event_driven < 0 && mode == PCX_MODE_PLAIN) {
+    int64_t mmax = 0;
+    for (int64_t k = 0; k < b->n_inst; ++k) mmax = std::max(mmax, inst_size(b, k));
+    if (mmax < kPlainMarkedMin) return PCX_OK;
+  }
+  const int64_t rows = b->n_inst * bitmap_words(b);
+  const uint64_t need = (uint64_t)rows * 4 * (b->h_off.empty() ? 2 : 3);
+  if (e->cfg.event_driven < 0 && !b->d_dirty) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > free_b / 4) return PCX_OK;
+  }
+  pcx_status st = ensure_event_buffers(b);
+  if (st != PCX_OK) return st;
+  st = clear_marks(b);
+  if (st != PCX_OK) return st;
   b->events = true;
   return PCX_OK;
 }
@@ -399,7 +430,7 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       int n_marked = 0;
       for (int g = 0; g < group; ++g, ++s) {
         if (e->cfg.max_sweeps > 0 && s >= e->cfg.max_sweeps) break;
-        const bool marked = b->events && s > 0;
+        const bool marked = b->events && (s > 0 || b->seeded_run);
         pcx_status ts = PCX_OK;
         if (!marked || n_marked == 0) ts = tm.begin(e->stream);
         if (ts != PCX_OK) return ts;
@@ -473,7 +504,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     b->marked_graph_key = key;
   }
   while (live > 0) {
-    if (b->events && sweeps > 0) {
+    if (b->events && sweeps >= group) {     // groups after the first: all marked sweeps
       pcx_status ts = tm.begin(e->stream);
       if (ts != PCX_OK) return ts;
       PCX_HIP(hipGraphLaunch(b->marked_graph, e->stream));
@@ -490,7 +521,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     // scheduler launches) as one span.
     int n_marked = 0;
     for (int g = 0; g < group; ++g) {
-      const bool marked = b->events && sweeps > 0;
+      const bool marked = b->events && (sweeps > 0 || b->seeded_run);
       pcx_status ts = PCX_OK;
       if (marked && n_marked == 0) ts = tm.begin(e->stream);
       if (ts != PCX_OK) return ts;
@@ -868,7 +899,7 @@ pcx_status pcx_batch_run(pcx_batch* b, int32_t mode, pcx_result* results, pcx_st
   return s;
 }
 
-pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var) {
+pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var, int32_t from_fixpoint) {
   if (!parent || !child) return fail_with(PCX_EINVAL, "NULL batch");
   if (parent->e != child->e) return fail_with(PCX_EINVAL, "parent and child batches must share the engine");
   if (child->n_inst != 2 * parent->n_inst || child->nvars != parent->nvars)
@@ -880,12 +911,24 @@ pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var) {
   PCX_HIP(hipMalloc(&d_var, std::max<int64_t>(parent->n_inst, 1) * sizeof(int32_t)));
   hipError_t he = launch_branch(parent->d_store, parent->nvars, parent->n_inst, d_var, child->d_store,
                                 child->d_image, dom_of(e), e->stream);
+  child->seeded = false;
+  if (he == hipSuccess && from_fixpoint && events_possible(child, PCX_MODE_BATCHED)) {
+    pcx_status st = sync_table(e);
+    if (st == PCX_OK) st = ensure_event_buffers(child);
+    if (st == PCX_OK) st = clear_marks(child);
+    if (st != PCX_OK) { (void)hipFree(d_var); return st; }
+    he = launch_branch_seed(event_set(child), d_var, child->n_inst, e->stream);
+    child->seeded = he == hipSuccess;
+  }
   if (he == hipSuccess && var)
     he = hipMemcpyAsync(var, d_var, parent->n_inst * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
   if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
   (void)hipFree(d_var);
   if (he != hipSuccess) return fail_with(PCX_EDEVICE, std::string("branch: ") + hipGetErrorString(he));
-  return image_loaded(child);
+  const bool seeded = child->seeded;
+  pcx_status st = image_loaded(child);
+  child->seeded = seeded;
+  return st;
 }
 
 pcx_status pcx_batch_ask(pcx_batch* b, int32_t* extractable, int64_t* live_off, int32_t* live_idx,

@@ -56,6 +56,8 @@ struct pcx_batch {
   uint32_t* d_summary = nullptr;      // chunk summaries (2 planes) + scheduler snapshot
   uint64_t member_version = 0;
   bool events = false;                // this run uses marked-only sweeps
+  bool seeded = false;                // the next run may start from seeded marks
+  bool seeded_run = false;            // this run starts with a marked sweep
   // captured group of marked sweeps (batched mode), rebuilt when its key changes
   hipGraphExec_t marked_graph = nullptr;
   uint64_t marked_graph_key = 0;

@@ -176,6 +176,7 @@ hipError_t launch_batch_ask(const Table& t, const int2* store, int64_t nvars, co
 // 2k / 2k+1 written to child and child_image.
 hipError_t launch_branch(const int2* parent, int64_t nvars, int64_t n_inst, int32_t* var, int2* child,
                          int2* child_image, int dom, hipStream_t st);
+hipError_t launch_branch_seed(const EventSet& ev, const int32_t* var, int64_t n_child, hipStream_t st);
 hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
                       hipStream_t st);
 hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n, hipStream_t st);

@@ -1411,6 +1411,31 @@ __global__ void k_branch_write(const int2* __restrict__ parent, int64_t nvars, i
   }
 }
 
+// Seeds for children that start from their parent's fixpoint: child c's
+// marks (plane 0, the first sweep's) are the member propagators adjacent to
+// its parent's split variable.  One workgroup per child.
+__global__ __launch_bounds__(kBlock) void k_branch_seed(EventSet ev, const int32_t* __restrict__ var,
+                                                        int64_t n_child) {
+  for (int64_t c = blockIdx.x; c < n_child; c += gridDim.x) {
+    const int32_t v = var[c >> 1];
+    if (v < 0) continue;                          // uniform: no split, no marks
+    Marker mk = marker_for(&ev, c, -1);           // sweep -1 marks plane 0 (sweep 0's)
+    for (int32_t i = ev.adj_off[v] + threadIdx.x; i < ev.adj_off[v + 1]; i += kBlock) {
+      const int32_t p = ev.adj[i];
+      const uint32_t bit = 1u << (p & 31);
+      if (mk.member && !(mk.member[p >> 5] & bit)) continue;
+      atomicOr(&mk.dirty[p >> 5], bit);
+      if (mk.sum) set_summary(nullptr, mk.sum, p >> kChunkShift);
+    }
+  }
+}
+
+hipError_t launch_branch_seed(const EventSet& ev, const int32_t* var, int64_t n_child, hipStream_t stream) {
+  const int64_t grid = n_child < 65536 ? n_child : 65536;
+  if (grid > 0) k_branch_seed<<<(unsigned)grid, kBlock, 0, stream>>>(ev, var, n_child);
+  return hipGetLastError();
+}
+
 hipError_t launch_branch(const int2* parent, int64_t nvars, int64_t n_inst, int32_t* var, int2* child,
                          int2* child_image, int dom, hipStream_t stream) {
   if (n_inst <= 0) return hipSuccess;

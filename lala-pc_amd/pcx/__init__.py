@@ -143,7 +143,7 @@ SIGNATURES = [
     ("pcx_batch_reset", ctypes.c_int32, [_VP]),
     ("pcx_batch_read", ctypes.c_int32, [_VP, _P32, _P32]),
     ("pcx_batch_run", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Result), ctypes.POINTER(Stats)]),
-    ("pcx_batch_branch", ctypes.c_int32, [_VP, _VP, _P32]),
+    ("pcx_batch_branch", ctypes.c_int32, [_VP, _VP, _P32, ctypes.c_int32]),
     ("pcx_batch_ask", ctypes.c_int32, [_VP, _P32, _P64, _P32, ctypes.c_int64]),
     ("pcx_batch_autotune", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Tuning),
                                             ctypes.POINTER(Tuning), ctypes.c_int32, _P32]),
@@ -391,12 +391,14 @@ class Batch:
                                    ctypes.byref(st)))
         return res.view(np.recarray), st
 
-    def branch(self, child):
+    def branch(self, child, from_fixpoint=False):
         """Split every instance on its first unfixed variable into child
         instances 2k / 2k+1 (child: a Batch of 2 * n_inst instances over the
-        same engine); returns the branching variables (-1: no split)."""
+        same engine); returns the branching variables (-1: no split).
+        from_fixpoint: the parents were run to their fixpoints, so the
+        children's next run starts from the split variable's propagators."""
         var = np.zeros(max(self.n_inst, 1), np.int32)
-        _check(lib().pcx_batch_branch(self._h, child._h, _p32(var)))
+        _check(lib().pcx_batch_branch(self._h, child._h, _p32(var), 1 if from_fixpoint else 0))
         return var[:self.n_inst]
 
     def ask(self, live=True):

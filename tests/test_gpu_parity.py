@@ -420,10 +420,13 @@ def _host_children(lo, hi, n_inst, V):
     return var, clo.reshape(-1), chi.reshape(-1)
 
 
-def test_branching_on_device(gpu):
+@pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PLAIN])
+@pytest.mark.parametrize("incremental", [False, True], ids=["full", "from-fixpoint"])
+def test_branching_on_device(gpu, incremental, mode):
     """pcx_batch_branch splits each propagated node on its first unfixed
     variable; the children equal the host restatement and propagate to the
-    oracle's fixpoints."""
+    oracle's fixpoints, also when their run starts from the split variable's
+    propagators only (the parents are at their fixpoints)."""
     V, P, n_inst = 3000, 12000, 24
     op, x, y, z, lb, ub, w = _network(71, V, P)
     pir = pcx.PIR()
@@ -441,13 +444,15 @@ def test_branching_on_device(gpu):
     coff = np.concatenate([[0], np.cumsum(np.repeat(sizes, 2))]).astype(np.int64)
     cidx = np.concatenate([np.concatenate([idx[off[k]:off[k + 1]]] * 2) for k in range(n_inst)])
     child = pcx.Batch(pir, 2 * n_inst, V, coff, cidx)
-    var = parent.branch(child)
+    var = parent.branch(child, from_fixpoint=incremental)
     hvar, clo, chi = _host_children(plo, phi, n_inst, V)
     assert (var == hvar).all()
     glo, ghi = child.read()
     assert (glo == clo).all() and (ghi == chi).all()
-    res, _ = child.run(pcx.MODE_BATCHED)
+    res, st = child.run(mode)
     lb_g, ub_g = child.read()
+    if incremental:                           # no full first sweep
+        assert st.full_evals == 0 and st.evals < int(np.diff(coff).sum())
     top, tx, ty, tz = pir.table()
     lo_o, hi_o = clo.copy(), chi.copy()
     res_o = ol.fixpoint_batch(top, tx, ty, tz, lo_o, hi_o, V, idx=cidx.astype(np.int64), off=coff, threads=8)
