@@ -110,3 +110,42 @@ def test_bits_order_independence_and_faithful_mode(seed):
     rf = ol.bits_fixpoint(op, x, y, z, w, faithful=True)
     if not ref[0] and not rf.failed:
         assert all((int(a) & int(b)) == int(b) for a, b in zip(w, ref[1]))
+
+
+def test_faithful_mode_difference_count():
+    """Counts the networks where the faithful mode stops short of the fixpoint.
+
+    Shuffled chains of `y <= z` make a later leq in a sweep overwrite the
+    has_changed that an earlier one set (formula.hpp:803 assigns instead of
+    or-ing), so the reference's Gauss-Seidel loop can end after a sweep that
+    did tighten.  Over these 200 networks 46 end early; each such result is
+    above the true fixpoint and is not a fixpoint itself (one more true run
+    from it tightens), and the other 154 are identical to the true fixpoint.
+    """
+    differ = 0
+    for seed in range(200):
+        rng = np.random.default_rng(seed)
+        V = 12
+        words = np.array([rng.integers(1, 2**20) for _ in range(V)], np.uint64)
+        pairs = [(i, i + 1) for i in range(V - 1)]
+        rng.shuffle(pairs)
+        P = len(pairs)
+        op = np.full(P, ol.BITS_OPS["LEQ"], np.int32)
+        x = np.zeros(P, np.int32)
+        y = np.array([p[0] for p in pairs], np.int32)
+        z = np.array([p[1] for p in pairs], np.int32)
+        wt = words.copy()
+        rt = ol.bits_fixpoint(op, x, y, z, wt)
+        wf = words.copy()
+        rf = ol.bits_fixpoint(op, x, y, z, wf, faithful=True)
+        if bool(rt.failed) == bool(rf.failed) and (rt.failed or np.array_equal(wt, wf)):
+            continue
+        differ += 1
+        assert not rf.failed, f"seed {seed}: the faithful mode failed where the fixpoint did not"
+        if not rt.failed:
+            assert all((int(a) & int(b)) == int(b) for a, b in zip(wf, wt)), f"seed {seed}"
+        again = wf.copy()
+        r2 = ol.bits_fixpoint(op, x, y, z, again)
+        assert r2.failed or r2.has_changed, f"seed {seed}: an early stop that is a fixpoint"
+        assert bool(r2.failed) == bool(rt.failed) and (rt.failed or np.array_equal(again, wt))
+    assert differ == 46, f"{differ} of 200 networks differ"
