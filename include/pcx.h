@@ -299,7 +299,16 @@ typedef struct {
                                 evaluate only marked propagators (their bitmaps
                                 share the request's arena block), -1 auto: 1 for
                                 requests of at least 2^17 propagators          */
-  int32_t reserved[3];
+  int32_t burst_sweeps;      /* side lanes: a request still changing after
+                                burst_after sweeps leaves the slot batch and is
+                                propagated by one workgroup of its own, beside
+                                the batch, in launches of up to this many rounds
+                                (marked requests while their marks fit one
+                                workgroup, others up to 8,192 propagators);
+                                0 off, -1 auto (1,024)                          */
+  int32_t burst_after;       /* sweeps before a request may move to a side lane;
+                                -1 auto (256)                                   */
+  int32_t reserved[1];
 } pcx_server_config;
 
 typedef struct {

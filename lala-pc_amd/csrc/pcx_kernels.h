@@ -43,6 +43,8 @@ struct InstFlags {
   int32_t failed_at;    // first sweep that found an empty interval, or kNoFail
   int32_t done_at;      // sweep at which the scheduler retired the instance
   int64_t marked_evals; // evaluations done by marked-only sweeps
+  int32_t extra_sweeps; // workgroup-local sweeps run after a sweep (serving bursts)
+  int32_t settled_at;   // a burst after sweep s reached the fixpoint: s, else -1
 };
 
 // Event-driven sweeps.  After an instance's first (full) sweep, a sweep
@@ -115,8 +117,14 @@ struct SlotDesc {
   int64_t words;
   int64_t swords;
   int32_t start;         // the request's first sweep (a full sweep)
-  int32_t pad;
+  int32_t handover;      // side lane: the slot-batch sweep after which it left the batch
 };
+
+// done_at markers of a slot in a side lane (see launch_burst_slots): the
+// scheduler never runs a slot whose done_at is set, so the slot batch leaves
+// it alone while a burst owns it.
+constexpr int32_t kSideLane = 0x7fffff00;   // + lane: in that lane's bursts (kept when the budget ran out)
+constexpr int32_t kSideBail = 0x7ffffff1;   // the burst stopped: its marks need the slot batch
 
 }  // namespace pcx
 
@@ -163,6 +171,15 @@ hipError_t launch_schedule_slots(const SlotDesc* slots, InstFlags* flags, int32_
                                  WorkList wl, int ppw, hipStream_t st);
 hipError_t launch_sweep_slots(const SlotDesc* slots, InstFlags* flags, WorkList wl,
                               unsigned long long* tight, int ppw, int dom, int grid,
+                              hipStream_t st);
+// Side-lane bursts: one workgroup per slot whose done_at is kSideLane + lane
+// propagates it alone for up to `budget` rounds (marked slots while their
+// marks fit one workgroup, slots with full sweeps of at most kBurstFullMax
+// propagators).  Runs on its own stream beside the slot batch.
+constexpr int64_t kBurstFullMax = 8192;
+constexpr int64_t kBurstSummaryMax = 512;   // summary words of a marked slot in a burst
+hipError_t launch_burst_slots(const SlotDesc* slots, InstFlags* flags, int32_t n_slots,
+                              unsigned long long* tight, int32_t budget, int32_t lane, int dom,
                               hipStream_t st);
 hipError_t launch_compact_table(const Table& t, const int32_t* idx, int64_t n, uint32_t* xo,
                                 int32_t* y, int32_t* z, hipStream_t st);

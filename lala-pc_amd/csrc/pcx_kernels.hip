@@ -274,17 +274,16 @@ __device__ __forceinline__ T table_load(const T* p) {
   else return *p;
 }
 
-template <int D, bool NT>
+template <int D, bool NT, int SCOPE = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ void eval_prop(const Table& t, int2* __restrict__ st, int64_t i,
                                           bool& chg, bool& fail, int& nt, const Marker& mk) {
   const uint32_t xo = table_load<NT>(t.xo + i);
   const int32_t y = table_load<NT>(t.y + i), z = table_load<NT>(t.z + i);
   const int32_t x = (int32_t)(xo & kVarMask);
-  const u64 wx = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + x);
-  const u64 wy = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + y);
-  const u64 wz = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + z);
-  apply<D, __HIP_MEMORY_SCOPE_AGENT>(st, (int)(xo >> kOpShift), x, y, z, wx, wy, wz, chg, fail, nt,
-                                     mk);
+  const u64 wx = load_word<SCOPE>(st + x);
+  const u64 wy = load_word<SCOPE>(st + y);
+  const u64 wz = load_word<SCOPE>(st + z);
+  apply<D, SCOPE>(st, (int)(xo >> kOpShift), x, y, z, wx, wy, wz, chg, fail, nt, mk);
 }
 
 // Publish a wave's verdict: one store per flag, one atomic for the count.
@@ -862,7 +861,8 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule_slots(const SlotDesc
   int32_t na = 0;
   for (int k = lo; k < hi; ++k) {
     InstFlags& f = flags[k];
-    const bool run = f.done_at < 0 && f.failed_at == kNoFail && f.changed[(s - 1) & 1] == s - 1;
+    const bool run = f.done_at < 0 && f.failed_at == kNoFail && f.settled_at < 0 &&
+                     f.changed[(s - 1) & 1] == s - 1;
     if (!run) {
       if (f.done_at < 0) f.done_at = s;
       continue;
@@ -1029,6 +1029,232 @@ hipError_t launch_sweep_slots(const SlotDesc* slots, InstFlags* flags, WorkList 
   return hipGetLastError();
 }
 
+// ---- serving bursts (side lane) --------------------------------------------
+// Some requests need thousands of sweeps, most of which change a few bounds
+// each (bounds of a cycle climbing step by step).  In the slot batch each of
+// those sweeps costs two launches and a share of a tick's host round trip.
+// The server hands such a slot to a side lane: this kernel, on a stream of
+// its own, runs one workgroup per handed-over slot while the slot batch goes
+// on without it.  The slot left the batch after sweep S = sd.handover.
+//  - Marked slots: the propagators marked for sweep S+1 are gathered into
+//    an LDS ring (when they fit).  Each round pops up to kBurstPop of them,
+//    clears their bits, evaluates them, and marks the adjacency of every
+//    variable they tightened: a mark whose bit was clear (returning
+//    atomicOr) is appended to the ring, so the ring always holds exactly
+//    the set bits.  Every unmarked propagator is at its fixpoint (the
+//    marked-sweep invariant), so an empty ring is the fixpoint.  At exit the
+//    summary row is rewritten from the ring, so bitmap and summary again
+//    describe the marks of sweep S+1.
+//  - Slots with full sweeps (at most kBurstFullMax propagators): a round is
+//    a sweep over every propagator; one that tightens nothing is the fixpoint.
+// Exit: settled (settled_at = S), failed (failed_at = S), budget spent
+// (done_at stays kSideLane + lane), or the marks do not fit (done_at = kSideBail:
+// the slot goes back to the batch).  The slot's store and marks are touched
+// by this workgroup alone, so the store is read and tightened with
+// workgroup-scope operations and barriers separate rounds, as in the
+// persistent kernel; mark bits are device-scope atomics.
+constexpr int kBurstBlock = 512;
+constexpr int kBurstQ = 4096;        // ring of marked propagators (power of two)
+constexpr int kBurstPop = kMarkCap / 3;   // propagators evaluated per round
+constexpr int kBurstChunks = 256;    // marked chunks gathered at the start
+static_assert(kBurstSummaryMax <= kBurstBlock, "one summary word per thread");
+
+// Exclusive scan of v over a workgroup of BT threads; returns the offset,
+// *total = the sum.  s_scan holds BT / kWave entries.
+template <int BT>
+__device__ __forceinline__ int block_scan(int v, int32_t* s_scan, int* total) {
+  const int lane = lane_id();
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int u = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += u;
+  }
+  __syncthreads();
+  const int wv = threadIdx.x / kWave;
+  if (lane == kWave - 1) s_scan[wv] = incl;
+  __syncthreads();
+  int off = incl - v, t = 0;
+#pragma unroll
+  for (int i = 0; i < BT / kWave; ++i) {
+    const int u = s_scan[i];
+    if (i < wv) off += u;
+    t += u;
+  }
+  *total = t;
+  return off;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBurstBlock) void k_burst_slots(const SlotDesc* slots,
+                                                             InstFlags* flags, int32_t n_slots,
+                                                             unsigned long long* tight,
+                                                             int32_t budget, int32_t lane) {
+  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+  constexpr int BT = kBurstBlock;
+  const int32_t k = blockIdx.x;
+  if (k >= n_slots) return;
+  InstFlags* f = flags + k;
+  if (uni((int32_t)(f->done_at == kSideLane + lane && f->failed_at == kNoFail &&
+                    f->settled_at < 0)) == 0)
+    return;
+  SlotDesc sd = slots[k];
+  sd.xo = uni(sd.xo); sd.y = uni(sd.y); sd.z = uni(sd.z); sd.m = uni(sd.m);
+  sd.store = uni(sd.store); sd.adj_off = uni(sd.adj_off); sd.adj = uni(sd.adj);
+  sd.marks = uni(sd.marks); sd.words = uni(sd.words); sd.swords = uni(sd.swords);
+  const int32_t S = uni(sd.handover);
+  const bool marked = sd.adj != nullptr;
+  const int tid = threadIdx.x;
+  if (!(marked ? sd.swords <= kBurstSummaryMax : sd.m <= kBurstFullMax)) {
+    if (tid == 0) f->done_at = kSideBail;
+    return;
+  }
+
+  __shared__ int32_t s_list[2][kMarkCap];          // tightened variables, by round parity
+  __shared__ int32_t s_n[2];
+  __shared__ int32_t s_q[kBurstQ];
+  __shared__ uint32_t s_sum[kBurstSummaryMax];
+  __shared__ int32_t s_chunks[kBurstChunks];
+  __shared__ int32_t s_scan[BT / kWave];
+  __shared__ int32_t s_over, s_head, s_tail;
+  const Table t{sd.xo, sd.y, sd.z, sd.m};
+  const int64_t plane = (S + 1) & 1;               // the marks of sweep S+1
+  uint32_t* cur = sd.marks + plane * sd.words;
+  uint32_t* srow = sd.marks + 2 * sd.words + plane * sd.swords;
+  if (marked) {
+    // gather every marked propagator: chunks from the summary row, then
+    // their words (counted first; the slot goes back if they do not fit)
+    const uint32_t sw = tid < sd.swords ? srow[tid] : 0u;
+    int nc;
+    int o = block_scan<BT>(__builtin_popcount(sw), s_scan, &nc);
+    bool fits = nc <= kBurstChunks;
+    if (fits) for (uint32_t b = sw; b; b &= b - 1) s_chunks[o++] = tid * 32 + __builtin_ctz(b);
+    __syncthreads();
+    const int64_t nw = fits ? (int64_t)nc * kChunkWords : 0;
+    int cnt = 0;
+    for (int64_t j = tid; j < nw; j += BT) {
+      const int64_t w = (int64_t)s_chunks[j / kChunkWords] * kChunkWords + j % kChunkWords;
+      if (w < sd.words) cnt += __builtin_popcount(cur[w]);
+    }
+    int np;
+    int base = block_scan<BT>(cnt, s_scan, &np);
+    fits = fits && np <= kBurstQ;
+    if (!fits) {
+      if (tid == 0) f->done_at = kSideBail;
+      return;
+    }
+    for (int64_t j = tid; j < nw; j += BT) {
+      const int64_t w = (int64_t)s_chunks[j / kChunkWords] * kChunkWords + j % kChunkWords;
+      if (w >= sd.words) continue;
+      for (uint32_t bits = cur[w]; bits; bits &= bits - 1)
+        s_q[base++] = (int32_t)(w * 32 + __builtin_ctz(bits));
+    }
+    if (tid == 0) { s_head = 0; s_tail = np; }
+  }
+  if (tid == 0) { s_n[0] = 0; s_n[1] = 0; s_over = 0; }
+  __syncthreads();
+  bool chg = false, fail = false, settled = false;
+  int nt = 0, ne = 0, it = 0;
+  while (it < budget) {
+    if (marked) {
+      // One round: pop up to kBurstPop entries, clear their bits, evaluate
+      // them; after the barrier, mark the adjacency of what they tightened.
+      const int par = it & 1;
+      Marker mk{};
+      mk.dirty = cur;                              // apply() lists tightened variables
+      mk.list = s_list[par];
+      mk.list_n = &s_n[par];
+      mk.overflow = &s_over;
+      const int32_t head = s_head, n = min(s_tail - head, kBurstPop);
+      if (n == 0) { settled = true; ++it; break; }         // the quiet round
+      const int32_t p = tid < n ? s_q[(head + tid) & (kBurstQ - 1)] : -1;
+      if (p >= 0) {
+        __hip_atomic_fetch_and(&cur[p >> 5], ~(1u << (p & 31)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p < t.n) { eval_prop<D, false, WG>(t, sd.store, p, chg, fail, nt, mk); ++ne; }
+      }
+      // The clears are non-returning atomics: wait for them explicitly, so a
+      // re-mark after the barrier finds the bit clear and re-queues.
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();             // bits cleared, variables listed, head and tail read
+      if (tid == 0) { s_head = head + n; s_n[par ^ 1] = 0; }
+      const int32_t limit = head + n + kBurstQ;    // ring capacity after this pop
+      const int nl = s_n[par];
+      for (int i = tid; i < nl; i += BT) {
+        const int32_t v = s_list[par][i];
+        const int32_t e = sd.adj_off[v + 1];
+        for (int32_t a = sd.adj_off[v]; a < e; a += 8) {
+          int32_t q[8];
+          uint32_t old[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) q[j] = a + j < e ? sd.adj[a + j] : -1;
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            old[j] = q[j] >= 0 ? atomicOr(&cur[q[j] >> 5], 1u << (q[j] & 31)) : ~0u;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            if (q[j] < 0 || (old[j] & (1u << (q[j] & 31)))) continue;
+            const int32_t at = atomicAdd(&s_tail, 1);
+            if (at < limit) s_q[at & (kBurstQ - 1)] = q[j];
+            else s_over = 1;                               // the bit stays set; the summary covers it
+          }
+        }
+      }
+      ++it;
+      const bool any_fail = __syncthreads_or(fail);
+      if (any_fail || s_over) break;
+    } else {
+      bool c = false;
+      for (int64_t q = tid; q < sd.m; q += BT) eval_prop<D, false, WG>(t, sd.store, q, c, fail, nt, Marker{});
+      chg |= c;
+      ++it;
+      const bool any_fail = __syncthreads_or(fail);
+      const bool any_chg = __syncthreads_or(c);
+      if (any_fail) break;
+      if (!any_chg) { settled = true; break; }             // a full sweep that changed nothing
+    }
+  }
+  static_assert(3 * kBurstPop <= kMarkCap, "a round's tightenings fit the mark list");
+  if (marked && !settled && !fail) {
+    // The summary row again covers exactly the marked chunks: those of the
+    // ring's entries, or every chunk when marks overflowed the ring.
+    for (int j = tid; j < sd.swords; j += BT) s_sum[j] = 0;
+    __syncthreads();
+    const int64_t chunks = (sd.words + kChunkWords - 1) / kChunkWords;
+    if (s_over) {
+      for (int64_t j = tid; j < sd.swords; j += BT) {
+        const int64_t left = chunks - j * 32;
+        s_sum[j] = left >= 32 ? ~0u : ((1u << left) - 1u);
+      }
+    } else {
+      const int32_t head = s_head, n = s_tail - head;
+      for (int i = tid; i < n; i += BT) {
+        const int32_t c = s_q[(head + i) & (kBurstQ - 1)] >> kChunkShift;
+        atomicOr(&s_sum[c >> 5], 1u << (c & 31));
+      }
+    }
+    __syncthreads();
+    for (int j = tid; j < sd.swords; j += BT) srow[j] = s_sum[j];
+  }
+  publish(f, S, chg, fail, nt, tight);
+  count_evals(f, ne);
+  if (tid == 0) {
+    f->extra_sweeps += it;
+    if (settled && !fail) f->settled_at = S;
+    if (marked && s_over && !fail) f->done_at = kSideBail;
+  }
+}
+
+hipError_t launch_burst_slots(const SlotDesc* slots, InstFlags* flags, int32_t n_slots,
+                              unsigned long long* tight, int32_t budget, int32_t lane, int dom,
+                              hipStream_t stream) {
+  if (n_slots <= 0 || budget <= 0) return hipSuccess;
+  if (dom == kItvDiv)
+    k_burst_slots<kItvDiv><<<n_slots, kBurstBlock, 0, stream>>>(slots, flags, n_slots, tight, budget, lane);
+  else
+    k_burst_slots<kItv><<<n_slots, kBurstBlock, 0, stream>>>(slots, flags, n_slots, tight, budget, lane);
+  return hipGetLastError();
+}
+
 // ---- persistent mode: one launch, a device queue of instances -------------
 // Each workgroup takes the next instance from the queue (largest first) and
 // runs all of its sweeps itself, separated by workgroup barriers.  Store
@@ -1168,6 +1394,8 @@ __global__ void k_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl,
     flags[k].failed_at = (empty && empty[k]) ? -1 : kNoFail;
     flags[k].done_at = -1;
     flags[k].marked_evals = 0;
+    flags[k].extra_sweeps = 0;
+    flags[k].settled_at = -1;
   }
   if (k == 0 && wl.sweep) {
     *wl.sweep = -1;

@@ -80,13 +80,27 @@ struct Request {
 
 struct Slot {
   bool busy = false;
+  bool side = false;    // owned by a side lane (bursts) instead of the slot batch
+  int32_t lane = 0;     // that side lane
   Request req{};
   int64_t off = 0;      // arena offset
   int64_t units = 0;    // arena units held
   int32_t start = 0;    // first sweep number
+  int32_t skipped = 0;  // slot-batch sweeps run while the slot was in the side lane
+  int32_t retry_at = 0; // earliest sweep to try the side lane again after a bail
 };
 
 constexpr int32_t kFreeFailed = -2;   // failed_at marker of a free slot
+
+// Side lanes: concurrent burst launches, each on a stream of its own (with
+// the slot batch's stream, four streams: the hardware queues of a process).
+constexpr int kSideLanes = 3;
+struct SideLane {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;        // burst and flag read-back done
+  InstFlags* h_flags = nullptr;     // pinned; valid for the lane's slots once done
+  bool busy = false;
+};
 
 }  // namespace
 
@@ -113,6 +127,9 @@ struct pcx_server {
   int32_t next_sweep = 0;
   pcx_server_stats stats{};
   hipGraphExec_t tick_graph[2] = {nullptr, nullptr};   // a tick's sweeps, per kernel domain
+  // Side lanes: bursts of long-running slots beside the slot batch.
+  SideLane lanes[kSideLanes];
+  hipEvent_t handed = nullptr;        // slot batch: hand-over flags uploaded
 };
 
 namespace {
@@ -212,6 +229,7 @@ pcx_status admit(pcx_server* s) {
     f.changed[S & 1] = S - 2;
     f.failed_at = kNoFail;
     f.done_at = -1;
+    f.settled_at = -1;
     s->h_flags[k] = f;
     pcx_status st = upload_slot_meta(s, k);
     if (st != PCX_OK) return st;
@@ -221,6 +239,9 @@ pcx_status admit(pcx_server* s) {
     sl.off = off;
     sl.units = fp.units;
     sl.start = S;
+    sl.side = false;
+    sl.skipped = 0;
+    sl.retry_at = 0;
     s->pending.pop_front();
     ++s->stats.admitted;
     s->stats.arena_peak_bytes = std::max(s->stats.arena_peak_bytes,
@@ -229,46 +250,54 @@ pcx_status admit(pcx_server* s) {
   return PCX_OK;
 }
 
-// Hand back the slots whose request reached its fixpoint or failed: their
-// stores are copied back together (one synchronization), then released.
-pcx_status harvest(pcx_server* s) {
-  std::vector<std::pair<int32_t, int64_t>> done;     // slot, staging offset
+// A slot's result from its flags: slot-batch sweeps up to the failing /
+// settling / last changing one (plus the quiet sweep that found nothing more
+// to do), plus every burst round, minus the slot-batch sweeps run while the
+// slot was in the side lane.
+pcx_result result_of(const pcx_server* s, int32_t k, const InstFlags& f) {
+  const Slot& sl = s->slots[k];
+  const int32_t last = std::max(f.changed[0], f.changed[1]);
+  pcx_result r{};
+  r.failed = f.failed_at != kNoFail;
+  r.has_changed = last >= sl.start;
+  if (r.failed)
+    r.sweeps = (int64_t)f.failed_at - sl.start + 1;
+  else if (f.settled_at >= 0)
+    r.sweeps = (int64_t)f.settled_at - sl.start + 1;
+  else
+    r.sweeps = (int64_t)std::max(last, sl.start - 1) - sl.start + 2;
+  r.sweeps += f.extra_sweeps - sl.skipped;
+  if (s->h_slots[k].adj)          // one full sweep + the marked evaluations
+    r.evals = (r.sweeps > 0 ? s->h_slots[k].m : 0) + f.marked_evals;
+  else
+    r.evals = r.sweeps * s->h_slots[k].m;
+  return r;
+}
+
+// Hand back finished slots (flags in fl): their stores are copied back
+// together (one synchronization), then their slot and arena block are freed.
+pcx_status complete(pcx_server* s, const std::vector<int32_t>& ks, const InstFlags* fl) {
+  if (ks.empty()) return PCX_OK;
+  std::vector<int64_t> at;
   int64_t staged = 0;
-  for (int32_t k = 0; k < s->n_slots; ++k) {
-    Slot& sl = s->slots[k];
-    if (!sl.busy) continue;
-    const InstFlags& f = s->h_flags[k];
-    const int32_t last = std::max(f.changed[0], f.changed[1]);
-    const bool failed = f.failed_at != kNoFail;
-    const bool quiet = last < s->next_sweep - 1;     // the latest sweep left it unchanged
-    if (!failed && !quiet) continue;
-    if (staged + sl.req.nvars > s->stage_words) break;   // the rest next tick
+  for (int32_t k : ks) {
+    const Slot& sl = s->slots[k];
     PCX_HIP(hipMemcpyAsync(s->h_stage + staged, s->d_arena + sl.off, sl.req.nvars * sizeof(int2),
                            hipMemcpyDeviceToHost, s->stream));
-    done.emplace_back(k, staged);
+    at.push_back(staged);
     staged += sl.req.nvars;
   }
-  if (done.empty()) return PCX_OK;
   PCX_HIP(hipStreamSynchronize(s->stream));
-  for (const auto& d : done) {
-    const int32_t k = d.first;
+  for (size_t i = 0; i < ks.size(); ++i) {
+    const int32_t k = ks[i];
     Slot& sl = s->slots[k];
-    const InstFlags& f = s->h_flags[k];
-    const int32_t last = std::max(f.changed[0], f.changed[1]);
-    pcx_result r{};
-    r.failed = f.failed_at != kNoFail;
-    r.has_changed = last >= sl.start;
-    r.sweeps = r.failed ? (int64_t)f.failed_at - sl.start + 1
-                        : (int64_t)std::max(last, sl.start - 1) - sl.start + 2;
-    if (s->h_slots[k].adj)          // one full sweep + the marked evaluations
-      r.evals = (r.sweeps > 0 ? s->h_slots[k].m : 0) + f.marked_evals;
-    else
-      r.evals = r.sweeps * s->h_slots[k].m;
-    const int2* stage = s->h_stage + d.second;
+    const pcx_result r = result_of(s, k, fl[k]);
+    const int2* stage = s->h_stage + at[i];
     for (int64_t v = 0; v < sl.req.nvars; ++v) { sl.req.lb[v] = stage[v].x; sl.req.ub[v] = stage[v].y; }
     finish(s, sl.req, r);
     s->arena.release(sl.off, sl.units);
     sl.busy = false;
+    sl.side = false;
     // park the slot: never scheduled until re-admitted
     s->h_flags[k].failed_at = kFreeFailed;
     s->h_flags[k].done_at = 0;
@@ -278,23 +307,167 @@ pcx_status harvest(pcx_server* s) {
   return PCX_OK;
 }
 
+// Slots of the slot batch whose request reached its fixpoint or failed
+// (flags of this tick's read-back), as many as the staging buffer holds.
+pcx_status harvest(pcx_server* s) {
+  std::vector<int32_t> done;
+  int64_t staged = 0;
+  for (int32_t k = 0; k < s->n_slots; ++k) {
+    const Slot& sl = s->slots[k];
+    if (!sl.busy || sl.side) continue;
+    const InstFlags& f = s->h_flags[k];
+    const int32_t last = std::max(f.changed[0], f.changed[1]);
+    const bool failed = f.failed_at != kNoFail;
+    const bool quiet = last < s->next_sweep - 1;     // the latest sweep left it unchanged
+    if (!failed && !quiet) continue;
+    if (staged + sl.req.nvars > s->stage_words) break;   // the rest next tick
+    done.push_back(k);
+    staged += sl.req.nvars;
+  }
+  return complete(s, done, s->h_flags);
+}
+
 int32_t busy_slots(const pcx_server* s) {
   int32_t n = 0;
   for (const Slot& sl : s->slots) n += sl.busy;
   return n;
 }
 
+// Side lanes.  A slot that has run burst_after sweeps and still changes
+// leaves the slot batch (done_at = kSideLane + lane: the scheduler skips it)
+// and is propagated by k_burst_slots on that lane's stream, concurrently
+// with the slot batch's ticks, up to burst_sweeps rounds per launch.  Each
+// lane has at most one launch in flight; the host looks at the lanes at each
+// tick.  For each slot of a finished launch:
+//  - settled or failed: the request completes;
+//  - budget spent: the slot stays in the lane for its next launch;
+//  - the burst stopped (its marks do not fit a workgroup): the slot rejoins
+//    the slot batch at the batch's next sweep.  Its marks are for the sweep
+//    after its hand-over; when the parities differ they move to the other
+//    bitmap plane.  The batch sweeps it skipped are not its sweeps.
+pcx_status side_collect(pcx_server* s) {
+  std::vector<int32_t> done;
+  int64_t staged = 0;
+  const int32_t S1 = s->next_sweep - 1;          // the slot batch's last sweep
+  for (int L = 0; L < kSideLanes; ++L) {
+    SideLane& ln = s->lanes[L];
+    if (!ln.busy) continue;
+    const hipError_t q = hipEventQuery(ln.done);
+    if (q == hipErrorNotReady) continue;
+    PCX_HIP(q);
+    ln.busy = false;
+    for (int32_t k = 0; k < s->n_slots; ++k) {
+      Slot& sl = s->slots[k];
+      if (!sl.busy || !sl.side || sl.lane != L) continue;
+      InstFlags f = ln.h_flags[k];
+      if (f.failed_at != kNoFail || f.settled_at >= 0) {
+        if (staged + sl.req.nvars <= s->stage_words) {
+          s->h_flags[k] = f;                     // the flags its result is read from
+          done.push_back(k);
+          staged += sl.req.nvars;
+        }
+        continue;                                // a full staging buffer: next tick
+      }
+      if (f.done_at != kSideBail) continue;      // budget spent: next launch
+      const SlotDesc& sd = s->h_slots[k];
+      const int32_t S0 = sd.handover;
+      if (sd.adj && ((S0 + 1) & 1) != ((S1 + 1) & 1)) {
+        const int64_t from = (S0 + 1) & 1, to = (S1 + 1) & 1;
+        uint32_t* mk = sd.marks;
+        PCX_HIP(hipMemcpyAsync(mk + to * sd.words, mk + from * sd.words, sd.words * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, s->stream));
+        PCX_HIP(hipMemcpyAsync(mk + 2 * sd.words + to * sd.swords, mk + 2 * sd.words + from * sd.swords,
+                               sd.swords * sizeof(uint32_t), hipMemcpyDeviceToDevice, s->stream));
+        PCX_HIP(hipMemsetAsync(mk + from * sd.words, 0, sd.words * sizeof(uint32_t), s->stream));
+        PCX_HIP(hipMemsetAsync(mk + 2 * sd.words + from * sd.swords, 0, sd.swords * sizeof(uint32_t),
+                               s->stream));
+      }
+      f.changed[S1 & 1] = S1;                    // primes sweep S1 + 1
+      f.changed[(S1 + 1) & 1] = S1 - 1;
+      f.done_at = -1;
+      s->h_flags[k] = f;
+      sl.side = false;
+      sl.skipped += S1 - S0;
+      sl.retry_at = S1 + std::max(s->cfg.burst_after, 4 * s->cfg.sweeps_per_tick);
+      pcx_status st = upload_slot_meta(s, k);
+      if (st != PCX_OK) return st;
+    }
+  }
+  return complete(s, done, s->h_flags);
+}
+
+// Hand newly eligible slots of the slot batch to a free lane, and relaunch
+// free lanes whose slots spent their budget.
+pcx_status side_launch(pcx_server* s) {
+  if (s->cfg.burst_sweeps <= 0) return PCX_OK;
+  const int32_t S = s->next_sweep - 1;
+  std::vector<int32_t> fresh;
+  for (int32_t k = 0; k < s->n_slots; ++k) {
+    const Slot& sl = s->slots[k];
+    if (!sl.busy || sl.side) continue;
+    const InstFlags& f = s->h_flags[k];
+    const SlotDesc& sd = s->h_slots[k];
+    const bool running = f.failed_at == kNoFail && f.done_at < 0 && f.changed[S & 1] == S &&
+                         S - sl.start >= s->cfg.burst_after && S >= sl.retry_at;
+    const bool fits = sd.adj ? sd.swords <= kBurstSummaryMax : sd.m <= kBurstFullMax;
+    if (running && fits) fresh.push_back(k);
+  }
+  bool uploaded = false;
+  for (int L = 0; L < kSideLanes; ++L) {
+    SideLane& ln = s->lanes[L];
+    if (ln.busy) continue;
+    if (!fresh.empty()) {
+      for (int32_t k : fresh) {
+        s->h_flags[k].done_at = kSideLane + L;
+        s->h_slots[k].handover = S;
+        s->slots[k].side = true;
+        s->slots[k].lane = L;
+        pcx_status st = upload_slot_meta(s, k);
+        if (st != PCX_OK) return st;
+      }
+      fresh.clear();
+      uploaded = true;
+    }
+    bool any = false, div = false;
+    for (const Slot& sl : s->slots)
+      if (sl.busy && sl.side && sl.lane == L) {
+        any = true;
+        div |= s->nets[sl.req.net]->has_div;
+      }
+    if (!any) continue;
+    if (uploaded) {
+      PCX_HIP(hipEventRecord(s->handed, s->stream));
+      uploaded = false;
+    }
+    PCX_HIP(hipStreamWaitEvent(ln.stream, s->handed, 0));
+    PCX_HIP(launch_burst_slots(s->d_slots, s->d_flags, s->n_slots, s->d_tight, s->cfg.burst_sweeps,
+                               L, div ? kItvDiv : kItv, ln.stream));
+    PCX_HIP(hipMemcpyAsync(ln.h_flags, s->d_flags, s->n_slots * sizeof(InstFlags),
+                           hipMemcpyDeviceToHost, ln.stream));
+    PCX_HIP(hipEventRecord(ln.done, ln.stream));
+    ln.busy = true;
+  }
+  return PCX_OK;
+}
+
 pcx_status tick(pcx_server* s) {
-  pcx_status st = admit(s);
+  pcx_status st = side_collect(s);
   if (st != PCX_OK) return st;
-  const int32_t running = busy_slots(s);
+  st = admit(s);
+  if (st != PCX_OK) return st;
+  int32_t running = 0, in_batch = 0;
+  bool div = false;
+  for (const Slot& sl : s->slots) {
+    if (!sl.busy) continue;
+    ++running;
+    if (sl.side) continue;
+    ++in_batch;
+    div |= s->nets[sl.req.net]->has_div;
+  }
   s->stats.max_running = std::max(s->stats.max_running, running);
-  if (running > 0) {
+  if (in_batch > 0) {
     const int ppw = s->cfg.props_per_wave ? s->cfg.props_per_wave : 512;
     const int grid = pcx_internal::device_cus(s->cfg.device) * 8;
-    bool div = false;
-    for (const Slot& sl : s->slots)
-      if (sl.busy) div |= s->nets[sl.req.net]->has_div;
     // A tick's sweeps read everything (slots, flags, sweep number) from
     // device memory, so they are captured once per kernel domain and replayed.
     hipGraphExec_t& gx = s->tick_graph[div ? 1 : 0];
@@ -328,6 +501,8 @@ pcx_status tick(pcx_server* s) {
   } else {
     PCX_HIP(hipStreamSynchronize(s->stream));
   }
+  st = side_launch(s);
+  if (st != PCX_OK) return st;
   ++s->stats.ticks;
   return PCX_OK;
 }
@@ -352,11 +527,15 @@ void pcx_server_config_default(pcx_server_config* c) {
   c->props_per_wave = 0;
   c->stage_bytes = (int64_t)64 << 20;
   c->event_driven = -1;
+  c->burst_sweeps = -1;
+  c->burst_after = -1;
 }
 
 void pcx_server_destroy(pcx_server* s) {
   if (!s) return;
   (void)hipSetDevice(s->cfg.device);
+  for (SideLane& ln : s->lanes)
+    if (ln.stream) (void)hipStreamSynchronize(ln.stream);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (hipGraphExec_t g : s->tick_graph)
     if (g) (void)hipGraphExecDestroy(g);
@@ -365,6 +544,12 @@ void pcx_server_destroy(pcx_server* s) {
     if (p) (void)hipFree(p);
   for (void* p : {(void*)s->h_slots, (void*)s->h_flags, (void*)s->h_stage})
     if (p) (void)hipHostFree(p);
+  for (SideLane& ln : s->lanes) {
+    if (ln.h_flags) (void)hipHostFree(ln.h_flags);
+    if (ln.done) (void)hipEventDestroy(ln.done);
+    if (ln.stream) (void)hipStreamDestroy(ln.stream);
+  }
+  if (s->handed) (void)hipEventDestroy(s->handed);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -375,8 +560,10 @@ pcx_status pcx_server_create(const pcx_server_config* cfg, pcx_server** out) {
   pcx_server_config c;
   if (cfg) c = *cfg; else pcx_server_config_default(&c);
   if (c.max_slots <= 0 || c.max_slots > (1 << 20) || c.arena_bytes < 8 || c.sweeps_per_tick <= 0 ||
-      c.stage_bytes < 8)
+      c.stage_bytes < 8 || c.burst_sweeps < -1 || c.burst_after < -1)
     return fail_with(PCX_EINVAL, "bad server configuration");
+  if (c.burst_sweeps == -1) c.burst_sweeps = 1024;
+  if (c.burst_after == -1) c.burst_after = 256;
   if (c.props_per_wave && c.props_per_wave != 64 && c.props_per_wave != 128 &&
       c.props_per_wave != 256 && c.props_per_wave != 512)
     return fail_with(PCX_EINVAL, "props_per_wave must be 0, 64, 128, 256 or 512");
@@ -393,6 +580,12 @@ pcx_status pcx_server_create(const pcx_server_config* cfg, pcx_server** out) {
   s->stage_words = c.stage_bytes / (int64_t)sizeof(int2);
   hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   auto chk = [&](hipError_t r) { if (he == hipSuccess) he = r; };
+  chk(hipEventCreateWithFlags(&s->handed, hipEventDisableTiming));
+  for (SideLane& ln : s->lanes) {
+    chk(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+    chk(hipEventCreateWithFlags(&ln.done, hipEventDisableTiming));
+    chk(hipHostMalloc(&ln.h_flags, c.max_slots * sizeof(InstFlags), hipHostMallocDefault));
+  }
   chk(hipMalloc(&s->d_slots, c.max_slots * sizeof(SlotDesc)));
   chk(hipMalloc(&s->d_flags, c.max_slots * sizeof(InstFlags)));
   chk(hipMalloc(&s->d_act, c.max_slots * sizeof(int32_t)));

@@ -97,7 +97,8 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("max_slots", ctypes.c_int32),
                 ("arena_bytes", ctypes.c_int64), ("sweeps_per_tick", ctypes.c_int32),
                 ("props_per_wave", ctypes.c_int32), ("stage_bytes", ctypes.c_int64),
-                ("event_driven", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("event_driven", ctypes.c_int32), ("burst_sweeps", ctypes.c_int32),
+                ("burst_after", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
 
 
 class ServerStats(ctypes.Structure):

@@ -202,3 +202,112 @@ def test_auto_marks_large_requests_only(gpu):
         assert _same(*stores[i], got[i], *alone[i]), f"request {i} ({sizes[tr.net[i]]} propagators)"
         if sizes[tr.net[i]] >= 1 << 17 and not got[i].failed:
             assert got[i].evals < got[i].sweeps * sizes[tr.net[i]]   # marked sweeps did less work
+
+
+def _climbers(n, seed, big=10**6):
+    """n copies of a = b + 1, t = a + 1, b = min(t, cap): the lower bounds of
+    a, b, t climb by about one per sweep until b reaches cap, so a request
+    needs about max(cap) sweeps.  Variable 5g+4 is gadget g's cap and 5g its
+    a (an upper bound on a below cap + 1 makes the request fail after about
+    that many sweeps)."""
+    rng = np.random.default_rng(seed)
+    op, x, y, z, lb, ub = [], [], [], [], [], []
+    for g in range(n):
+        a, b, t, one, cap = range(5 * g, 5 * g + 5)
+        K = int(rng.integers(10, 1200))
+        lb += [0, 0, 0, 1, K]
+        ub += [big, big, big, 1, K]
+        op += [pcx.ADD, pcx.ADD, pcx.MIN]
+        x += [a, t, b]
+        y += [b, a, t]
+        z += [one, one, cap]
+    return [np.array(v, np.int32) for v in (op, x, y, z, lb, ub)]
+
+
+@pytest.fixture(scope="module")
+def climbers(gpu):
+    """Two networks (50 and 4,000 gadgets) and 24 requests with random caps,
+    a quarter of them made to fail; the oracle's fixpoint and the request
+    alone for each."""
+    nets, pirs = [], []
+    for n, seed in ((50, 1), (4000, 2)):
+        op, x, y, z, lb, ub = _climbers(n, seed)
+        p = pcx.PIR()
+        p.tell(op, x, y, z, lb, ub)
+        nets.append((lb, ub))
+        pirs.append(p)
+    rng = np.random.default_rng(7)
+    reqs = []
+    for i in range(24):
+        j = i % 2
+        lo, hi = (a.copy() for a in nets[j])
+        G = len(lo) // 5
+        caps = rng.integers(10, 1500, G)
+        lo[4::5] = caps
+        hi[4::5] = caps
+        if i % 4 == 3:                           # one gadget's a cannot reach cap + 1
+            g = int(rng.integers(G))
+            hi[5 * g] = int(caps[g]) // 2 + 5
+        reqs.append((j, lo, hi))
+    oracle, alone = [], []
+    for j, lo, hi in reqs:
+        top, tx, ty, tz = pirs[j].table()
+        o_lo, o_hi = lo.copy(), hi.copy()
+        oracle.append((o_lo, o_hi, ol.fixpoint(top, tx, ty, tz, o_lo, o_hi)))
+        a_lo, a_hi = lo.copy(), hi.copy()
+        alone.append((a_lo, a_hi, pirs[j].fixpoint(a_lo, a_hi)))
+    return pirs, reqs, oracle, alone
+
+
+@pytest.mark.parametrize("spt", [2, 3])
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
+@pytest.mark.parametrize("burst", [(-1, -1), (5, 0), (0, 0)], ids=["auto", "short", "off"])
+def test_bursts_slow_convergence(climbers, events, burst, spt):
+    """Requests needing up to ~1,500 sweeps, served with the side lane (long
+    bursts, bursts of 5 rounds relaunched again and again, or none): every
+    result equals the oracle's and the request alone.  The 4,000-gadget
+    network starts with more marks than a burst holds, so its requests go
+    back to the slot batch (with 3 sweeps per tick, at a sweep of the other
+    parity) and return to the side lane later."""
+    pirs, reqs, oracle, alone = climbers
+    for i, (o, a) in enumerate(zip(oracle, alone)):
+        assert _same(*o, *a), f"request {i}: alone differs from the oracle"
+    assert sum(bool(o[2].failed) for o in oracle) == len(reqs) // 4
+    got, st, stores = _serve_climbers(pirs, reqs, range(len(reqs)), events, *burst, spt=spt)
+    for i in range(len(reqs)):
+        assert _same(*stores[i], got[i], *oracle[i]), f"request {i} (net {reqs[i][0]})"
+        if not got[i].failed:
+            assert got[i].sweeps >= 10
+    if burst[0] == 0:
+        assert st.sweeps >= max(r.sweeps for r in got.values())
+
+
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
+def test_bursts_take_the_long_requests(climbers, events):
+    """The small network's requests (150 propagators: bursts apply with or
+    without marks) do most of their sweeps in the side lane: the slot batch
+    runs far fewer sweeps than they need."""
+    pirs, reqs, oracle, _ = climbers
+    sel = [i for i in range(len(reqs)) if reqs[i][0] == 0]
+    got, st, stores = _serve_climbers(pirs, reqs, sel, events, -1, 8)
+    for i in sel:
+        assert _same(*stores[i], got[i], *oracle[i]), f"request {i}"
+    longest = max(r.sweeps for r in got.values())
+    assert longest > 400 and st.sweeps < longest // 4, (st.sweeps, longest)
+
+
+def _serve_climbers(pirs, reqs, sel, events, burst_sweeps, burst_after, spt=2):
+    srv = pcx.Server(max_slots=8, sweeps_per_tick=spt, event_driven=events,
+                     burst_sweeps=burst_sweeps, burst_after=burst_after)
+    for p in pirs:
+        srv.add_network(p)
+    stores = {i: (reqs[i][1].copy(), reqs[i][2].copy()) for i in sel}
+    for i in sel:
+        srv.submit(reqs[i][0], *stores[i], tag=i)
+    got = {}
+    while len(got) < len(stores):
+        srv.step()
+        for tag, r, _ in srv.poll():
+            got[tag] = r
+        assert srv.stats().ticks < 10000000, "server made no progress"
+    return got, srv.stats(), stores
