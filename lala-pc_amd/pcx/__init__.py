@@ -175,6 +175,15 @@ SIGNATURES = [
                                            _P32, _P32, _P32, _P32, _PU64, _P32]),
     ("pcxg_bits_splits", ctypes.c_int32, [ctypes.c_uint64, _PU64, ctypes.c_int64, ctypes.c_int64,
                                           ctypes.c_int32, ctypes.c_int32, _PU64]),
+    ("pcx_model_parse", ctypes.c_int32, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
+    ("pcx_model_add", ctypes.c_int32, [_VP, ctypes.c_char_p]),
+    ("pcx_model_destroy", None, [_VP]),
+    ("pcx_model_num_vars", ctypes.c_int64, [_VP]),
+    ("pcx_model_num_props", ctypes.c_int64, [_VP]),
+    ("pcx_model_table", ctypes.c_int32, [_VP, ctypes.c_int64, _P32, _P32, _P32, _P32]),
+    ("pcx_model_store", ctypes.c_int32, [_VP, _P32, _P32]),
+    ("pcx_model_var", ctypes.c_int32, [_VP, ctypes.c_char_p]),
+    ("pcx_model_var_name", ctypes.c_char_p, [_VP, ctypes.c_int32]),
 ]
 
 _lib = None
@@ -599,3 +608,85 @@ def gen_subsets(seed, nprops, n_inst, mmin, mmax):
     idx = np.zeros(max(int(off[-1]), 1), np.int32)
     _check(lib().pcxg_subsets(seed, nprops, n_inst, mmin, mmax, _p64(off), _p32(idx)))
     return off, idx[:int(off[-1])]
+
+
+class Model:
+    """A FlatZinc-style model decomposed to PIR ternary form (include/pcx_model.h).
+
+    Model(text) parses declarations and constraints; add(text) tells more, as
+    the reference's tests tell further constraints into a PIR
+    (tests/pir_test.cpp:577-581).  table(frm) gives the propagators created
+    since `frm` (for PIR.tell), store() the declared and restricted domains,
+    var(name) a variable's index.
+    """
+
+    def __init__(self, text):
+        h = _VP()
+        _check(lib().pcx_model_parse(text.encode(), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pcx_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, text):
+        _check(lib().pcx_model_add(self._h, text.encode()))
+
+    def num_vars(self):
+        return int(lib().pcx_model_num_vars(self._h))
+
+    def num_props(self):
+        return int(lib().pcx_model_num_props(self._h))
+
+    def table(self, frm=0):
+        n = self.num_props() - frm
+        op, x, y, z = (np.zeros(max(n, 0), np.int32) for _ in range(4))
+        _check(lib().pcx_model_table(self._h, frm, _p32(op), _p32(x), _p32(y), _p32(z)))
+        return op, x, y, z
+
+    def store(self):
+        n = self.num_vars()
+        lb, ub = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        _check(lib().pcx_model_store(self._h, _p32(lb), _p32(ub)))
+        return lb, ub
+
+    def var(self, name):
+        return int(lib().pcx_model_var(self._h, name.encode()))
+
+    def var_name(self, v):
+        r = lib().pcx_model_var_name(self._h, v)
+        return None if r is None else r.decode()
+
+    def meet_store(self, lb, ub):
+        """The current store met with the model's domains; variables created
+        since the store was made are appended with their domains."""
+        mlb, mub = self.store()
+        n = len(lb)
+        out_lb, out_ub = mlb.copy(), mub.copy()
+        out_lb[:n] = np.maximum(lb, mlb[:n])
+        out_ub[:n] = np.minimum(ub, mub[:n])
+        return out_lb, out_ub
+
+
+_FZN = {ADD: "int_plus({y}, {z}, {x})", MUL: "int_times({y}, {z}, {x})",
+        TDIV: "int_tdiv({y}, {z}, {x})", FDIV: "int_fdiv({y}, {z}, {x})",
+        CDIV: "int_cdiv({y}, {z}, {x})", EDIV: "int_ediv({y}, {z}, {x})",
+        MIN: "int_min({y}, {z}, {x})", MAX: "int_max({y}, {z}, {x})",
+        EQ: "int_eq_reif({y}, {z}, {x})", LEQ: "int_le_reif({y}, {z}, {x})"}
+
+
+def flatzinc(op, x, y, z, lb, ub):
+    """A ternary network as FlatZinc text (variables v0, v1, ...): each
+    propagator x = y <op> z as the predicate that Model decomposes back into
+    exactly that propagator."""
+    out = [f"var {int(a)}..{int(b)}: v{i};" for i, (a, b) in enumerate(zip(lb, ub))]
+    for o, a, b, c in zip(op.tolist(), x.tolist(), y.tolist(), z.tolist()):
+        out.append("constraint " + _FZN[o].format(x=f"v{a}", y=f"v{b}", z=f"v{c}") + ";")
+    return "\n".join(out)

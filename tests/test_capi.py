@@ -22,7 +22,7 @@ def _declared(header):
 
 def test_library_loads_and_exports_every_declared_symbol():
     L = pcx.lib()
-    names = _declared("pcx.h") | _declared("pcx_gen.h")
+    names = _declared("pcx.h") | _declared("pcx_gen.h") | _declared("pcx_model.h")
     assert len(names) >= 20
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, f"declared but not exported: {missing}"

@@ -1,0 +1,115 @@
+"""The front-end (include/pcx_model.h) on the reference's own test models.
+
+tests/golden/fzn_kats.json holds the FlatZinc models of tests/pir_test.cpp
+and the stores that test file expects before and after each Gauss-Seidel
+fixpoint.  Here the front-end's decomposition is run through the oracle's
+fixpoint (the GPU engine does the same in test_gpu_frontend.py): every
+variable of the original model must end as the reference expects, failures
+included.  The propagator counts are lala-core ternarize's, which the
+front-end matches only where the decomposition is the obvious one; they are
+reported, not pinned.  Parse errors and unsupported predicates are errors.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import pcx
+from frontend_cases import load_cases, run_case
+
+CASES = load_cases()
+
+
+def oracle_backend(op, x, y, z, lb, ub, before, _state={}):
+    raise NotImplementedError
+
+
+class Oracle:
+    """The oracle as the fixpoint: keeps the whole table, sorted as tell does."""
+
+    def __init__(self):
+        self.t = [np.zeros(0, np.int32) for _ in range(4)]
+
+    def __call__(self, op, x, y, z, lb, ub, before):
+        op, x = op.copy(), x.copy()
+        ol.tell_embed(op, x, lb, ub)
+        self.t = [np.concatenate([a, b]) for a, b in zip(self.t, (op, x, y, z))]
+        t = [a.copy() for a in self.t]
+        ol.sort_table(*t)
+        before()
+        r = ol.fixpoint(*t, lb, ub)
+        return bool(r.failed)
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c["name"])
+def test_reference_models_through_the_oracle(case):
+    errs, _ = run_case(case, Oracle())
+    assert not errs, f"{case['name']} ({case['ref']}): " + "; ".join(errs)
+
+
+def test_fixture_covers_the_model_tests():
+    assert len(CASES) >= 70
+    assert sum(len(c["steps"]) for c in CASES) >= 100
+    assert any(s["failed"] for c in CASES for s in c["steps"])
+
+
+def test_propagator_counts_reported():
+    same = total = 0
+    for c in CASES:
+        _, counts = run_case(c, Oracle())
+        for s, n in zip(c["steps"], counts):
+            total += 1
+            same += s["nd"] == n
+    print(f"front-end propagator counts equal to the reference's in {same} of {total} steps")
+    assert total > 0
+
+
+@pytest.mark.parametrize("text,status", [
+    ("var int: x; constraint int_le(x, ;", pcx.EINVAL),
+    ("var int: x; constraint foo_bar(x, 3);", pcx.ENOTSUP),
+    ("var int: x; var int: x;", pcx.EINVAL),
+    ("var int: x; constraint int_le(y, 3);", pcx.EINVAL),
+    ("var 1..99999999999: x;", pcx.EINVAL),
+    ("predicate p(var int: x);", pcx.EINVAL),
+])
+def test_errors(text, status):
+    with pytest.raises(pcx.PcxError) as e:
+        pcx.Model(text)
+    assert e.value.status == status
+    assert "line 1" in str(e.value)
+
+
+def test_decomposition_shapes():
+    """The decomposition's shape on small models: unary constraints are
+    domains, x = term puts the term in x, x < y is one LEQ with 0."""
+    m = pcx.Model("var 0..10: x; var 0..10: y; var int: z; constraint int_le(x, 7);"
+                  "constraint int_eq(z, int_plus(x, y)); constraint int_lt(x, y);"
+                  "constraint int_ne(x, 3);")
+    lb, ub = m.store()
+    assert (lb[m.var("x")], ub[m.var("x")]) == (0, 7)
+    op, x, y, z = m.table()
+    X, Y, Z = m.var("x"), m.var("y"), m.var("z")
+    c0, c3 = m.var("__CONSTANT_0"), m.var("__CONSTANT_3")
+    rows = sorted(zip(op.tolist(), x.tolist(), y.tolist(), z.tolist()))
+    assert rows == sorted([(pcx.ADD, Z, X, Y), (pcx.LEQ, c0, Y, X), (pcx.EQ, c0, X, c3)])
+    # incremental: more constraints, more variables, domains only shrink
+    n0, v0 = m.num_props(), m.num_vars()
+    m.add("var bool: b; constraint int_le_reif(x, y, b); constraint int_ge(x, 2);")
+    assert m.num_props() == n0 + 1 and m.num_vars() == v0 + 1
+    lb2, ub2 = m.meet_store(lb, ub)
+    assert (lb2[X], ub2[X]) == (2, 7) and (lb2[m.var("b")], ub2[m.var("b")]) == (0, 1)
+    assert m.var("nope") == -1 and m.var_name(10**6) is None
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_flatzinc_round_trip(seed):
+    """A generated network written as FlatZinc decomposes back into exactly
+    its table and store, and the fixpoint is the oracle's on the table."""
+    p = pcx.gen_params(nvars=1000, nprops=3000, seed=seed)
+    op, x, y, z, lb, ub, _ = pcx.gen_network(p)
+    m = pcx.Model(pcx.flatzinc(op, x, y, z, lb, ub))
+    assert m.num_vars() == len(lb) and m.num_props() == len(op)
+    mop, mx, my, mz = m.table()
+    assert np.array_equal(mop, op) and np.array_equal(mx, x)
+    assert np.array_equal(my, y) and np.array_equal(mz, z)
+    mlb, mub = m.store()
+    assert np.array_equal(mlb, lb) and np.array_equal(mub, ub)
