@@ -65,6 +65,7 @@ pcx_status sync_table(pcx_engine* e) {
   PCX_HIP(hipSetDevice(e->cfg.device));
   if (n > e->d_cap) {
     dfree(e->d_xo); dfree(e->d_y); dfree(e->d_z);
+    e->d_cap = 0;                      // a failed allocation below leaves no stale capacity
     const int64_t cap = std::max<int64_t>(n, 1);
     PCX_HIP(hipMalloc(&e->d_xo, cap * sizeof(uint32_t)));
     PCX_HIP(hipMalloc(&e->d_y, cap * sizeof(int32_t)));
