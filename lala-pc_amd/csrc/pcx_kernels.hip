@@ -416,17 +416,27 @@ hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, i
 // list spread evenly over its threads (a word with many marks no longer
 // serializes one thread).  Every thread of the workgroup calls this.
 constexpr int kPropCap = 1024;
+constexpr int kMarkedPasses = 16;  // at most, passes over a chunk per marked sweep
 template <int D>
-__device__ __forceinline__ void sweep_marked_chunk(const Table& t, int2* __restrict__ st,
-                                                   uint32_t* cur_row, int64_t w0, int64_t words,
-                                                   bool active, const Marker& mk, bool& chg,
-                                                   bool& fail, int& nt, int& ne, int32_t* s_props,
-                                                   int32_t* s_scan) {
+// `fresh`: read the words with device-scope atomics (a pass over a chunk
+// this workgroup already swept in this launch, whose lines the CU's L1 may
+// hold from before other workgroups marked it).  Returns the number of
+// marked propagators the pass evaluated (the same in every thread).
+__device__ __forceinline__ int sweep_marked_chunk(const Table& t, int2* __restrict__ st,
+                                                  uint32_t* cur_row, int64_t w0, int64_t words,
+                                                  bool active, const Marker& mk, bool& chg,
+                                                  bool& fail, int& nt, int& ne, int32_t* s_props,
+                                                  int32_t* s_scan, bool fresh = false) {
   const int64_t w = w0 + threadIdx.x;
   uint32_t bits = 0;
   if (active && w < words) {
-    bits = cur_row[w];
-    if (bits) cur_row[w] = 0;
+    if (fresh) {
+      bits = __hip_atomic_load(&cur_row[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (bits) bits = __hip_atomic_exchange(&cur_row[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      bits = cur_row[w];
+      if (bits) cur_row[w] = 0;
+    }
   }
   const int c = __builtin_popcount(bits);
   const int lane = lane_id();
@@ -466,6 +476,7 @@ __device__ __forceinline__ void sweep_marked_chunk(const Table& t, int2* __restr
     }
     __syncthreads();
   }
+  return total;
 }
 
 __device__ __forceinline__ void count_evals(InstFlags* f, int ne) {
@@ -494,9 +505,16 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked_plain(Table t, int2* __
   int nt = 0, ne = 0;
   for (int64_t base = (int64_t)blockIdx.x * kBlock; base < ev.words;
        base += (int64_t)gridDim.x * kBlock) {      // same trip count in every thread
-    sweep_marked_chunk<D>(t, st, cur, base, ev.words, true, mk, chg, fail, nt, ne, s_props, s_scan);
-    flush_marks(mk);
-    __syncthreads();
+    // Passes over the chunk while they find marks: marks that other
+    // workgroups (or this one) put in it during this sweep are evaluated in
+    // this sweep, so a chain can advance several links per launch.
+    for (int pass = 0;; ++pass) {
+      const int total = sweep_marked_chunk<D>(t, st, cur, base, ev.words, true, mk, chg, fail, nt, ne,
+                                              s_props, s_scan, pass > 0);
+      flush_marks(mk);
+      __syncthreads();
+      if (total == 0 || pass + 1 >= kMarkedPasses) break;
+    }
   }
   publish(f, s, chg, fail, nt, tight);
   count_evals(f, ne);
@@ -775,12 +793,16 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
     bool chg = false, fail = false;
     int nt = 0, ne = 0;
     const bool active = uni((int32_t)(f->failed_at == kNoFail && chunk >= 0)) != 0;
-    sweep_marked_chunk<D>(t, store + (int64_t)k * nvars, cur + (int64_t)k * ev.words,
-                          chunk * kMarkedWordsPerItem, ev.words, active, mk, chg, fail, nt, ne,
-                          s_props, s_scan);
+    for (int pass = 0;; ++pass) {        // passes while they find marks (see k_sweep_marked_plain)
+      const int total = sweep_marked_chunk<D>(t, store + (int64_t)k * nvars, cur + (int64_t)k * ev.words,
+                                              chunk * kMarkedWordsPerItem, ev.words, active, mk, chg,
+                                              fail, nt, ne, s_props, s_scan, pass > 0);
+      flush_marks(mk);
+      __syncthreads();
+      if (total == 0 || pass + 1 >= kMarkedPasses) break;
+    }
     publish(f, s, chg, fail, nt, tight);
     count_evals(f, ne);
-    flush_marks(mk);
     if (threadIdx.x == 0) s_next = nxt;
     __syncthreads();
     item = first_item + __builtin_amdgcn_readfirstlane(s_next);
@@ -990,6 +1012,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_slots(const SlotDesc* slots, I
       }
       c = uni(c);
       const bool active = uni((int32_t)(f->failed_at == kNoFail && c >= 0)) != 0;
+      // One pass per chunk: repeated passes shorten long requests but lengthen
+      // the tick that every request in the batch waits for.
       sweep_marked_chunk<D>(t, sd.store, sd.marks + (int64_t)(s & 1) * sd.words, c * kChunkWords,
                             sd.words, active, mk, chg, fail, nt, ne, s_props, s_scan);
     }

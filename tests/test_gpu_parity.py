@@ -161,11 +161,15 @@ def test_planted_network(gpu, seed, V, P, ppw, events):
 
 
 def test_full_size_network(gpu):
-    """1,000,000 variables / 4,000,000 propagators: bit-exact with the oracle."""
+    """1,000,000 variables / 4,000,000 propagators: bit-exact with the oracle.
+    The planted chains make the reference's Gauss-Seidel loop take at least
+    50 sweeps; the GPU's marked sweeps advance a chain several links per
+    sweep (repeated passes over a chunk), so they may take fewer."""
     op, x, y, z, lb, ub, w = _network(1, 1000000, 4000000)
     rg, lbg, ubg, pir = _gpu_fix(op, x, y, z, lb, ub)
-    assert not rg.failed and rg.sweeps >= 50
+    assert not rg.failed
     ro, lbo, ubo = _oracle_fix(op, x, y, z, lb, ub)
+    assert ro.sweeps >= 50
     assert (lbg == lbo).all() and (ubg == ubo).all()
     assert ((lbg <= w) & (w <= ubg)).all()
 
