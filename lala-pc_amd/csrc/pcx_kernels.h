@@ -83,7 +83,8 @@ struct Subsets {
 };
 
 // Control words of a batch: [0] n_act, [1] sweep, [2] live, [3] persistent
-// queue head, [8 .. 8+kXcds) per-XCD item counters of the current sweep.
+// queue head, [4] marked chunks per work item of the current sweep,
+// [8 .. 8+kXcds) per-XCD item counters of the current sweep.
 constexpr int kCtlWords = 16;
 constexpr int kCtlXq = 8;
 
@@ -96,9 +97,10 @@ struct WorkList {
   int32_t* live;         // [1] instances still running after the last schedule;
                          //     live[1] is the persistent mode's queue head
   int32_t* xq;           // [kXcds] items claimed so far on each XCD (zeroed per sweep)
+  int32_t* group;        // [1] marked sweeps: chunks with marks per work item
 };
 inline WorkList worklist_of(int32_t* act, int64_t* item_base, int32_t* ctl) {
-  return WorkList{act, item_base, ctl, ctl + 1, ctl + 2, ctl + kCtlXq};
+  return WorkList{act, item_base, ctl, ctl + 1, ctl + 2, ctl + kCtlXq, ctl + 4};
 }
 
 // One serving slot: a request's own table and store (heterogeneous batch).

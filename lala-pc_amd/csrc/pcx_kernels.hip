@@ -417,6 +417,8 @@ hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, i
 // serializes one thread).  Every thread of the workgroup calls this.
 constexpr int kPropCap = 1024;
 constexpr int kMarkedPasses = 16;  // at most, passes over a chunk per marked sweep
+constexpr int kMarkedGroupMax = 8;        // marked chunks per batched work item, at most
+constexpr int64_t kMarkedTargetItems = 4096;   // work items a marked sweep keeps, at least
 template <int D>
 // `fresh`: read the words with device-scope atomics (a pass over a chunk
 // this workgroup already swept in this launch, whose lines the CU's L1 may
@@ -464,6 +466,74 @@ __device__ __forceinline__ int sweep_marked_chunk(const Table& t, int2* __restri
       b2 &= b2 - 1;
       if (idx >= base && idx < base + kPropCap) s_props[idx - base] = (int32_t)(w * 32 + j);
       ++idx;
+    }
+    __syncthreads();
+    const int n = min(kPropCap, total - base);
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+      const int64_t p = s_props[i];
+      if (p < t.n) {
+        eval_prop<D, false>(t, st, p, chg, fail, nt, mk);
+        ++ne;
+      }
+    }
+    __syncthreads();
+  }
+  return total;
+}
+
+// The marked propagators of ng chunks (ids in LDS), scanned together: thread
+// t reads word t of each chunk, one block scan places every set bit in the
+// LDS list, and the workgroup evaluates the list spread over its threads.
+// Same contract as sweep_marked_chunk.
+template <int D>
+__device__ __forceinline__ int sweep_marked_group(const Table& t, int2* __restrict__ st,
+                                                  uint32_t* cur_row, const int32_t* chunks, int ng,
+                                                  int64_t words, bool active, const Marker& mk,
+                                                  bool& chg, bool& fail, int& nt, int& ne,
+                                                  int32_t* s_props, int32_t* s_scan, bool fresh) {
+  uint32_t bits[kMarkedGroupMax];
+  int c = 0;
+#pragma unroll
+  for (int g = 0; g < kMarkedGroupMax; ++g) {
+    bits[g] = 0;
+    const int64_t w = (int64_t)chunks[g] * kChunkWords + threadIdx.x;
+    if (active && g < ng && w < words) {
+      if (fresh) {
+        bits[g] = __hip_atomic_load(&cur_row[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (bits[g]) bits[g] = __hip_atomic_exchange(&cur_row[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        bits[g] = cur_row[w];
+        if (bits[g]) cur_row[w] = 0;
+      }
+    }
+    c += __builtin_popcount(bits[g]);
+  }
+  const int lane = lane_id();
+  int incl = c;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int v = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += v;
+  }
+  const int wv = threadIdx.x / kWave;
+  if (lane == kWave - 1) s_scan[wv] = incl;
+  __syncthreads();
+  int off = incl - c, total = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / kWave; ++i) {
+    const int v = s_scan[i];
+    if (i < wv) off += v;
+    total += v;
+  }
+  for (int base = 0; base < total; base += kPropCap) {   // same trip count in every thread
+    int idx = off;
+#pragma unroll
+    for (int g = 0; g < kMarkedGroupMax; ++g) {
+      const int64_t w = (int64_t)chunks[g] * kChunkWords + threadIdx.x;
+      for (uint32_t b2 = bits[g]; b2; b2 &= b2 - 1) {
+        if (idx >= base && idx < base + kPropCap) s_props[idx - base] = (int32_t)(w * 32 + __builtin_ctz(b2));
+        ++idx;
+      }
     }
     __syncthreads();
     const int n = min(kPropCap, total - base);
@@ -591,6 +661,36 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
     cnt[tid] += cv;
     __syncthreads();
   }
+  // Marked sweeps: when many chunks hold marks (typically few marks each,
+  // right after a full sweep), a work item takes up to kMarkedGroupMax of an
+  // instance's marked chunks, so the per-item cost (scan, barriers, mark
+  // flush) is paid once for several chunks; the items still fill the chip.
+  int group = 1;
+  if (marked) {
+    const int64_t total_chunks = part[kSchedThreads - 1];
+    group = (int)max((int64_t)1, min((int64_t)kMarkedGroupMax, total_chunks / kMarkedTargetItems));
+    if (group > 1) {
+      int64_t gi = 0;
+      for (int k = lo; k < hi; ++k) {
+        const InstFlags& f = flags[k];
+        if (f.done_at >= 0 && f.done_at <= s) continue;
+        const uint32_t* snap = ev.snap + (int64_t)k * ev.swords;
+        int64_t c = 0;
+        for (int64_t j = 0; j < ev.swords; ++j) c += __builtin_popcount(snap[j]);
+        gi += (c + group - 1) / group;
+      }
+      __syncthreads();                   // every thread has read the first scan's total
+      part[tid] = gi;
+      __syncthreads();
+      for (int o = 1; o < kSchedThreads; o <<= 1) {
+        const int64_t pv = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += pv;
+        __syncthreads();
+      }
+      items = gi;
+    }
+  }
   int64_t ib = part[tid] - items;
   int32_t ia = cnt[tid] - na;
   for (int k = lo; k < hi; ++k) {
@@ -600,7 +700,9 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
     wl.item_base[ia] = ib;
     if (marked) {
       const uint32_t* snap = ev.snap + (int64_t)k * ev.swords;
-      for (int64_t j = 0; j < ev.swords; ++j) ib += __builtin_popcount(snap[j]);
+      int64_t c = 0;
+      for (int64_t j = 0; j < ev.swords; ++j) c += __builtin_popcount(snap[j]);
+      ib += (c + group - 1) / group;
     } else {
       const int64_t m = sub.off ? sub.off[k + 1] - sub.off[k] : n_props;
       ib += (m + chunk - 1) / chunk;
@@ -613,6 +715,7 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
     *wl.n_act = cnt[tid];
     *wl.live = cnt[tid];
     *wl.sweep = s;
+    *wl.group = group;
   }
 }
 
@@ -755,11 +858,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
   const int64_t end_item = total * (xcd + 1) / kXcds;
   int32_t* qc = wl.xq + xcd;
   uint32_t* cur = ev.dirty + (int64_t)(s & 1) * ev.n_inst * ev.words;
+  const int group = uni(*wl.group);
   __shared__ int32_t s_next;
   __shared__ int32_t s_list[kMarkCap];
   __shared__ int32_t s_props[kPropCap];
   __shared__ int32_t s_scan[kBlock / kWave];
-  __shared__ int32_t s_n, s_over;
+  __shared__ int32_t s_chunks[kMarkedGroupMax];
+  __shared__ int32_t s_n, s_over, s_ng;
   if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; s_over = 0; }
   __syncthreads();
   int64_t item = first_item + __builtin_amdgcn_readfirstlane(s_next);
@@ -772,31 +877,33 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
     hint = lo;
     const int32_t k = uni(wl.act[lo]);
     InstFlags* f = flags + k;
-    // the item's chunk: the r-th set bit of the instance's snapshot
-    int64_t r = item - uni(wl.item_base[lo]), chunk = -1;
-    const uint32_t* snap = ev.snap + (int64_t)k * ev.swords;
-    for (int64_t j = 0; j < ev.swords; ++j) {
-      uint32_t v = snap[j];
-      const int c = __builtin_popcount(v);
-      if (r < c) {
-        for (int q = 0; q < r; ++q) v &= v - 1;
-        chunk = j * 32 + __builtin_ctz(v);
-        break;
+    // the item's chunks: set bits [r * group, r * group + group) of the
+    // instance's snapshot, listed by thread 0
+    if (threadIdx.x == 0) {
+      const int64_t first = (item - wl.item_base[lo]) * group;
+      const uint32_t* snap = ev.snap + (int64_t)k * ev.swords;
+      int64_t seen = 0;
+      int ng = 0;
+      for (int64_t j = 0; j < ev.swords && ng < group; ++j) {
+        for (uint32_t v = snap[j]; v && ng < group; v &= v - 1, ++seen)
+          if (seen >= first) s_chunks[ng++] = (int32_t)(j * 32 + __builtin_ctz(v));
       }
-      r -= c;
+      for (int g = ng; g < kMarkedGroupMax; ++g) s_chunks[g] = 0;
+      s_ng = ng;
     }
-    chunk = uni(chunk);
+    __syncthreads();
+    const int ng = s_ng;
     Marker mk = marker_for(&ev, k, s, true);
     mk.list = s_list;
     mk.list_n = &s_n;
     mk.overflow = &s_over;
     bool chg = false, fail = false;
     int nt = 0, ne = 0;
-    const bool active = uni((int32_t)(f->failed_at == kNoFail && chunk >= 0)) != 0;
+    const bool active = uni((int32_t)(f->failed_at == kNoFail && ng > 0)) != 0;
     for (int pass = 0;; ++pass) {        // passes while they find marks (see k_sweep_marked_plain)
-      const int total = sweep_marked_chunk<D>(t, store + (int64_t)k * nvars, cur + (int64_t)k * ev.words,
-                                              chunk * kMarkedWordsPerItem, ev.words, active, mk, chg,
-                                              fail, nt, ne, s_props, s_scan, pass > 0);
+      const int total = sweep_marked_group<D>(t, store + (int64_t)k * nvars, cur + (int64_t)k * ev.words,
+                                              s_chunks, ng, ev.words, active, mk, chg, fail, nt, ne,
+                                              s_props, s_scan, pass > 0);
       flush_marks(mk);
       __syncthreads();
       if (total == 0 || pass + 1 >= kMarkedPasses) break;

@@ -205,6 +205,37 @@ def test_batch_of_search_nodes(gpu, mode, tables, events):
     assert (lb_g2 == lb_g).all() and (ub_g2 == ub_g).all()
 
 
+def test_marked_sweeps_grouped_items(gpu):
+    """A batch whose marked sweeps group several marked chunks per work item:
+    2,048 instances of a 200,000-propagator network with 150k-200k active
+    propagators each leave more than 8,192 chunks holding marks after the
+    first sweep.  Every instance equals the oracle and the same batch run
+    with full sweeps only."""
+    V, P, n_inst = 50000, 200000, 2048
+    op, x, y, z, lb, ub, w = _network(21, V, P)
+    pir = pcx.PIR(event_driven=1)
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_lb, root_ub).failed
+    slb, sub = pcx.gen_splits(22, root_lb, root_ub, n_inst, 1, 20)
+    off, idx = pcx.gen_subsets(23, P, n_inst, 150000, 200000)
+    top, tx, ty, tz = pir.table()
+    lb_o, ub_o = slb.copy(), sub.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lb_o, ub_o, V, idx=idx.astype(np.int64), off=off,
+                              threads=16)
+    b = pcx.Batch(pir, n_inst, V, off, idx)
+    b.load(slb, sub)
+    res_g, st = b.run(pcx.MODE_BATCHED)
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, "grouped marked items")
+    assert st.evals > st.full_evals > 0           # marked sweeps ran after the full one
+    pir.set_config(event_driven=0)
+    b.reset()
+    res_f, _ = b.run(pcx.MODE_BATCHED)
+    lb_f, ub_f = b.read()
+    _compare(lb_g, ub_g, res_g, lb_f, ub_f, res_f, V, "grouped marked items vs full sweeps")
+
+
 def test_edge_cases(gpu):
     pir = pcx.PIR()
     # empty table: fixpoint is the store itself, one quiet sweep
