@@ -931,13 +931,31 @@ hipError_t launch_sweep_marked(const Table& t, int2* store, int64_t nvars, InstF
 }
 
 // member[k][p >> 5] bit p & 31 <=> table position p is in instance k's subset.
+// Lanes of a wave take consecutive subset entries; subsets are ascending, so
+// neighbouring lanes mostly set bits of the same word.  A segmented OR over
+// lanes with equal words (shuffles) leaves each word's bits in the last lane
+// of its run, and only that lane writes: a few atomics per wave instead of
+// 64.  (Every lane's bits reach a lane that writes them whatever the order.)
 __global__ void k_build_members(const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
                                 int64_t n_inst, uint32_t* member, int64_t words) {
+  const int lane = lane_id();
   for (int64_t k = blockIdx.x; k < n_inst; k += gridDim.x) {
     uint32_t* row = member + k * words;
-    for (int64_t i = off[k] + threadIdx.x; i < off[k + 1]; i += blockDim.x) {
-      const int32_t p = idx[i];
-      atomicOr(&row[p >> 5], 1u << (p & 31));
+    const int64_t end = off[k + 1];
+    for (int64_t base = off[k]; base < end; base += blockDim.x) {   // same trip count in every thread
+      const int64_t i = base + threadIdx.x;
+      const bool ok = i < end;
+      const int32_t p = ok ? idx[i] : 0;
+      const int32_t w = ok ? (p >> 5) : -1;
+      uint32_t b = ok ? (1u << (p & 31)) : 0u;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t b2 = __shfl_up(b, o, kWave);
+        const int32_t w2 = __shfl_up(w, o, kWave);
+        if (lane >= o && w2 == w) b |= b2;
+      }
+      const int32_t wn = __shfl_down(w, 1, kWave);
+      if (ok && (lane == kWave - 1 || wn != w)) atomicOr(&row[w], b);
     }
   }
 }
