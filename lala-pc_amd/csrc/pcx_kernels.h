@@ -45,6 +45,7 @@ struct InstFlags {
   int64_t marked_evals; // evaluations done by marked-only sweeps
   int32_t extra_sweeps; // workgroup-local sweeps run after a sweep (serving bursts)
   int32_t settled_at;   // a burst after sweep s reached the fixpoint: s, else -1
+  int32_t all_marked[2];// all_marked[s & 1] = s: every propagator is marked for sweep s
 };
 
 // Event-driven sweeps.  After an instance's first (full) sweep, a sweep

@@ -112,6 +112,8 @@ struct Marker {
   int32_t* overflow;        // LDS flag: the list overflowed (see flush_marks)
   int64_t words;            // bitmap words of a row
   int32_t swords;
+  int32_t* all_marked;      // instance's all_marked word of the next sweep, or nullptr
+  int32_t all_marked_s;     // the next sweep's number
 };
 
 // Summary words accumulated in LDS during a flush (larger summaries are
@@ -188,7 +190,8 @@ __device__ __forceinline__ void mark_var(const Marker& mk, int32_t v) {
 // Every marking kernel gives its workgroup an LDS list (mk.list).  When the
 // list is full the variable is not recorded; the flush then marks every
 // propagator of the instance for the next sweep instead (a superset of the
-// marks the lost variables needed).
+// marks the lost variables needed): by one word (mk.all_marked, the plain
+// kernels) or by OR-ing the member row into the bitmap.
 __device__ __forceinline__ void defer_mark(const Marker& mk, int32_t v) {
   const int i = atomicAdd(mk.list_n, 1);
   if (i < kMarkCap) mk.list[i] = v;
@@ -204,7 +207,18 @@ __device__ __forceinline__ void flush_marks(const Marker& mk) {
   const bool lds_sum = mk.sum && mk.swords <= kLdsSummaryWords;
   if (lds_sum)
     for (int j = threadIdx.x; j < 2 * kLdsSummaryWords; j += blockDim.x) s_sum[j >> 6][j & 63] = 0;
+  // Another workgroup already marked everything for the next sweep: these
+  // marks add nothing (a stale read only means marking them anyway).
+  if (mk.all_marked && threadIdx.x == 0 &&
+      __hip_atomic_load(mk.all_marked, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == mk.all_marked_s)
+    *mk.overflow = 1;
   __syncthreads();
+  if (*mk.overflow && mk.all_marked) {       // lost variables: every propagator for the next sweep
+    if (threadIdx.x == 0) *mk.all_marked = mk.all_marked_s;
+    __syncthreads();                        // every thread has read the flag
+    if (threadIdx.x == 0) { *mk.list_n = 0; *mk.overflow = 0; }   // ordered by the caller's barrier
+    return;
+  }
   Marker m2 = mk;
   if (lds_sum) {
     m2.lsum = s_sum[0];
@@ -350,6 +364,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_plain(Table t, int2* __restric
     mk.list = s_list;
     mk.list_n = &s_n;
     mk.overflow = &s_over;
+    mk.all_marked = &f->all_marked[(s + 1) & 1];
+    mk.all_marked_s = s + 1;
     if (threadIdx.x == 0) { s_n = 0; s_over = 0; }
     __syncthreads();
   }
@@ -428,11 +444,15 @@ __device__ __forceinline__ int sweep_marked_chunk(const Table& t, int2* __restri
                                                   uint32_t* cur_row, int64_t w0, int64_t words,
                                                   bool active, const Marker& mk, bool& chg,
                                                   bool& fail, int& nt, int& ne, int32_t* s_props,
-                                                  int32_t* s_scan, bool fresh = false) {
+                                                  int32_t* s_scan, bool fresh = false,
+                                                  bool all = false) {
   const int64_t w = w0 + threadIdx.x;
   uint32_t bits = 0;
   if (active && w < words) {
-    if (fresh) {
+    if (all && !fresh) {                   // every member propagator is marked
+      bits = mk.member ? mk.member[w] : ~0u;
+      if (cur_row[w]) cur_row[w] = 0;
+    } else if (fresh) {
       bits = __hip_atomic_load(&cur_row[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (bits) bits = __hip_atomic_exchange(&cur_row[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
@@ -569,6 +589,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked_plain(Table t, int2* __
   mk.list = s_list;
   mk.list_n = &s_n;
   mk.overflow = &s_over;
+  mk.all_marked = &f->all_marked[(s + 1) & 1];
+  mk.all_marked_s = s + 1;
+  const bool all = uni((int32_t)(f->all_marked[s & 1] == s)) != 0;
   if (threadIdx.x == 0) { s_n = 0; s_over = 0; }
   __syncthreads();
   bool chg = false, fail = false;
@@ -580,7 +603,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked_plain(Table t, int2* __
     // this sweep, so a chain can advance several links per launch.
     for (int pass = 0;; ++pass) {
       const int total = sweep_marked_chunk<D>(t, st, cur, base, ev.words, true, mk, chg, fail, nt, ne,
-                                              s_props, s_scan, pass > 0);
+                                              s_props, s_scan, pass > 0, all);
       flush_marks(mk);
       __syncthreads();
       if (total == 0 || pass + 1 >= kMarkedPasses) break;
@@ -1545,6 +1568,8 @@ __global__ void k_init_flags(InstFlags* flags, int32_t n_inst, WorkList wl,
     flags[k].marked_evals = 0;
     flags[k].extra_sweeps = 0;
     flags[k].settled_at = -1;
+    flags[k].all_marked[0] = -1;
+    flags[k].all_marked[1] = -1;
   }
   if (k == 0 && wl.sweep) {
     *wl.sweep = -1;

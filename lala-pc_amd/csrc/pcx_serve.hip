@@ -230,6 +230,7 @@ pcx_status admit(pcx_server* s) {
     f.failed_at = kNoFail;
     f.done_at = -1;
     f.settled_at = -1;
+    f.all_marked[0] = f.all_marked[1] = -1;
     s->h_flags[k] = f;
     pcx_status st = upload_slot_meta(s, k);
     if (st != PCX_OK) return st;
