@@ -1008,8 +1008,10 @@ __device__ __forceinline__ Marker slot_marker(const SlotDesc& sd, int32_t s, boo
   return mk;
 }
 
-__device__ __forceinline__ bool slot_full_sweep(const SlotDesc& sd, int32_t s) {
-  return !sd.adj || s == sd.start;
+// A slot's sweep s is a full sweep: no marks, its first sweep, or a sweep
+// for which a workgroup marked everything (all_marked).
+__device__ __forceinline__ bool slot_full_sweep(const SlotDesc& sd, const InstFlags& f, int32_t s) {
+  return !sd.adj || s == sd.start || f.all_marked[s & 1] == s;
 }
 // Same scheme as the batched mode: a one-workgroup scheduler builds the item
 // prefix sum over the slots still running, then a grid-stride XCD-aware
@@ -1038,7 +1040,7 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule_slots(const SlotDesc
       continue;
     }
     const SlotDesc& sd = slots[k];
-    if (slot_full_sweep(sd, s)) {
+    if (slot_full_sweep(sd, f, s)) {
       items += (sd.m + chunk - 1) / chunk;
     } else {                 // marked sweep: the chunks holding marks; snapshot and clear
       uint32_t* srow = sd.marks + 2 * sd.words + (int64_t)(s & 1) * sd.swords;
@@ -1071,7 +1073,7 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule_slots(const SlotDesc
     wl.act[ia] = k;
     wl.item_base[ia] = ib;
     const SlotDesc& sd = slots[k];
-    if (slot_full_sweep(sd, s)) {
+    if (slot_full_sweep(sd, f, s)) {
       ib += (sd.m + chunk - 1) / chunk;
     } else {
       const uint32_t* snap = sd.marks + 2 * sd.words + 2 * sd.swords;
@@ -1130,9 +1132,13 @@ __global__ __launch_bounds__(kBlock) void k_sweep_slots(const SlotDesc* slots, I
     sd.start = uni(sd.start);
     InstFlags* f = flags + k;
     const Table t{sd.xo, sd.y, sd.z, sd.m};
-    const bool full = slot_full_sweep(sd, s);
+    const bool full = uni((int32_t)slot_full_sweep(sd, *f, s)) != 0;
     Marker mk = slot_marker(sd, s, !full);
-    if (mk.dirty) { mk.list = s_list; mk.list_n = &s_n; mk.overflow = &s_over; }
+    if (mk.dirty) {
+      mk.list = s_list; mk.list_n = &s_n; mk.overflow = &s_over;
+      mk.all_marked = &f->all_marked[(s + 1) & 1];
+      mk.all_marked_s = s + 1;
+    }
     const int64_t r = item - uni(wl.item_base[lo]);
     bool chg = false, fail = false;
     int nt = 0, ne = 0;
@@ -1293,6 +1299,10 @@ __global__ __launch_bounds__(kBurstBlock) void k_burst_slots(const SlotDesc* slo
   const int64_t plane = (S + 1) & 1;               // the marks of sweep S+1
   uint32_t* cur = sd.marks + plane * sd.words;
   uint32_t* srow = sd.marks + 2 * sd.words + plane * sd.swords;
+  if (marked && f->all_marked[(S + 1) & 1] == S + 1) {   // every propagator marked: the batch's work
+    if (tid == 0) f->done_at = kSideBail;
+    return;
+  }
   if (marked) {
     // gather every marked propagator: chunks from the summary row, then
     // their words (counted first; the slot goes back if they do not fit)

@@ -383,8 +383,11 @@ pcx_status side_collect(pcx_server* s) {
         PCX_HIP(hipMemsetAsync(mk + 2 * sd.words + from * sd.swords, 0, sd.swords * sizeof(uint32_t),
                                s->stream));
       }
+      const bool all = f.all_marked[(S0 + 1) & 1] == S0 + 1;
       f.changed[S1 & 1] = S1;                    // primes sweep S1 + 1
       f.changed[(S1 + 1) & 1] = S1 - 1;
+      f.all_marked[0] = f.all_marked[1] = -1;
+      if (all) f.all_marked[(S1 + 1) & 1] = S1 + 1;   // sweep S1 + 1 is then a full sweep
       f.done_at = -1;
       s->h_flags[k] = f;
       sl.side = false;
