@@ -190,8 +190,8 @@ __device__ __forceinline__ void mark_var(const Marker& mk, int32_t v) {
 // Every marking kernel gives its workgroup an LDS list (mk.list).  When the
 // list is full the variable is not recorded; the flush then marks every
 // propagator of the instance for the next sweep instead (a superset of the
-// marks the lost variables needed): by one word (mk.all_marked, the plain
-// kernels) or by OR-ing the member row into the bitmap.
+// marks the lost variables needed): by one word (mk.all_marked) where the
+// kernel sets it, or by OR-ing the member row into the bitmap.
 __device__ __forceinline__ void defer_mark(const Marker& mk, int32_t v) {
   const int i = atomicAdd(mk.list_n, 1);
   if (i < kMarkCap) mk.list[i] = v;
@@ -510,7 +510,8 @@ __device__ __forceinline__ int sweep_marked_group(const Table& t, int2* __restri
                                                   uint32_t* cur_row, const int32_t* chunks, int ng,
                                                   int64_t words, bool active, const Marker& mk,
                                                   bool& chg, bool& fail, int& nt, int& ne,
-                                                  int32_t* s_props, int32_t* s_scan, bool fresh) {
+                                                  int32_t* s_props, int32_t* s_scan, bool fresh,
+                                                  bool all) {
   uint32_t bits[kMarkedGroupMax];
   int c = 0;
 #pragma unroll
@@ -518,7 +519,10 @@ __device__ __forceinline__ int sweep_marked_group(const Table& t, int2* __restri
     bits[g] = 0;
     const int64_t w = (int64_t)chunks[g] * kChunkWords + threadIdx.x;
     if (active && g < ng && w < words) {
-      if (fresh) {
+      if (all && !fresh) {                 // every member propagator is marked
+        bits[g] = mk.member ? mk.member[w] : ~0u;
+        if (cur_row[w]) cur_row[w] = 0;
+      } else if (fresh) {
         bits[g] = __hip_atomic_load(&cur_row[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (bits[g]) bits[g] = __hip_atomic_exchange(&cur_row[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
@@ -660,8 +664,11 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
     if (marked) {            // the chunks with marks become the items; snapshot and clear
       uint32_t* srow = ev.summary + (int64_t)(s & 1) * ev.n_inst * ev.swords + (int64_t)k * ev.swords;
       uint32_t* snap = ev.snap + (int64_t)k * ev.swords;
+      const bool all = f.all_marked[s & 1] == s;    // every chunk (see k_sweep_marked)
+      const int64_t chunks = (ev.words + kChunkWords - 1) / kChunkWords;
       for (int64_t j = 0; j < ev.swords; ++j) {
-        const uint32_t v = srow[j];
+        const int64_t left = chunks - j * 32;
+        const uint32_t v = all ? (left >= 32 ? ~0u : ((1u << left) - 1u)) : srow[j];
         snap[j] = v;
         srow[j] = 0;
         items += __builtin_popcount(v);
@@ -799,7 +806,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batched(Table t, int2* __restr
     it.idx = uni(it.idx); it.m = uni(it.m);
     int2* st = store + (int64_t)k * nvars;
     Marker mk = marker_for(&ev, k, s);
-    if (mk.dirty) { mk.list = s_list; mk.list_n = &s_n; mk.overflow = &s_over; }
+    if (mk.dirty) {
+      mk.list = s_list; mk.list_n = &s_n; mk.overflow = &s_over;
+      mk.all_marked = &f->all_marked[(s + 1) & 1];
+      mk.all_marked_s = s + 1;
+    }
     bool chg = false, fail = false;
     int nt = 0;
     if (f->failed_at == kNoFail) {
@@ -920,13 +931,16 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
     mk.list = s_list;
     mk.list_n = &s_n;
     mk.overflow = &s_over;
+    mk.all_marked = &f->all_marked[(s + 1) & 1];
+    mk.all_marked_s = s + 1;
+    const bool all = uni((int32_t)(f->all_marked[s & 1] == s)) != 0;
     bool chg = false, fail = false;
     int nt = 0, ne = 0;
     const bool active = uni((int32_t)(f->failed_at == kNoFail && ng > 0)) != 0;
     for (int pass = 0;; ++pass) {        // passes while they find marks (see k_sweep_marked_plain)
       const int total = sweep_marked_group<D>(t, store + (int64_t)k * nvars, cur + (int64_t)k * ev.words,
                                               s_chunks, ng, ev.words, active, mk, chg, fail, nt, ne,
-                                              s_props, s_scan, pass > 0);
+                                              s_props, s_scan, pass > 0, all);
       flush_marks(mk);
       __syncthreads();
       if (total == 0 || pass + 1 >= kMarkedPasses) break;

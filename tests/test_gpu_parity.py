@@ -236,6 +236,30 @@ def test_marked_sweeps_grouped_items(gpu):
     _compare(lb_g, ub_g, res_g, lb_f, ub_f, res_f, V, "grouped marked items vs full sweeps")
 
 
+@pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PLAIN])
+def test_dense_first_sweep_marks_everything(gpu, mode):
+    """Batches of unpropagated stores (splits of the told store, not of its
+    fixpoint): the first sweep tightens most bounds, work items overflow their
+    mark lists and mark every propagator for the next sweep by one word.
+    Every instance equals the oracle."""
+    V, P, n_inst = 20000, 80000, 48
+    op, x, y, z, lb, ub, w = _network(31, V, P)
+    pir = pcx.PIR(event_driven=1, props_per_wave=2048 if mode == pcx.MODE_BATCHED else 512)
+    pir.tell(op, x, y, z, lb, ub)
+    slb, sub = pcx.gen_splits(32, lb, ub, n_inst, 1, 20)
+    off, idx = pcx.gen_subsets(33, P, n_inst, 40000, 80000)
+    top, tx, ty, tz = pir.table()
+    lb_o, ub_o = slb.copy(), sub.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lb_o, ub_o, V, idx=idx.astype(np.int64), off=off,
+                              threads=16)
+    b = pcx.Batch(pir, n_inst, V, off, idx)
+    b.load(slb, sub)
+    res_g, st = b.run(mode)
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"dense first sweep, mode {mode}")
+    assert st.full_tightened > n_inst * 1000     # dense: far more than one mark list per item
+
+
 def test_edge_cases(gpu):
     pir = pcx.PIR()
     # empty table: fixpoint is the store itself, one quiet sweep
