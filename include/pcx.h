@@ -292,7 +292,8 @@ typedef struct {
   int32_t device;
   int32_t max_slots;         /* requests on the GPU at once                  */
   int64_t arena_bytes;       /* device bytes for request stores (8 B / var)   */
-  int32_t sweeps_per_tick;   /* sweeps launched between host checks           */
+  int32_t sweeps_per_tick;   /* sweeps launched between host checks (one in a
+                                tick that admitted requests)                   */
   int32_t props_per_wave;    /* 64..512; 0 = 512                              */
   int64_t stage_bytes;       /* pinned host staging for store copies          */
   int32_t event_driven;      /* 0 full sweeps only, 1 after the first sweep

@@ -126,7 +126,9 @@ struct pcx_server {
   std::vector<pcx_served> done;
   int32_t next_sweep = 0;
   pcx_server_stats stats{};
-  hipGraphExec_t tick_graph[2] = {nullptr, nullptr};   // a tick's sweeps, per kernel domain
+  // a tick's sweeps, per kernel domain: [dom][0] sweeps_per_tick sweeps,
+  // [dom][1] one sweep (ticks that admitted requests)
+  hipGraphExec_t tick_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   // Side lanes: bursts of long-running slots beside the slot batch.
   SideLane lanes[kSideLanes];
   hipEvent_t handed = nullptr;        // slot batch: hand-over flags uploaded
@@ -457,8 +459,14 @@ pcx_status side_launch(pcx_server* s) {
 pcx_status tick(pcx_server* s) {
   pcx_status st = side_collect(s);
   if (st != PCX_OK) return st;
+  const int64_t admitted = s->stats.admitted;
   st = admit(s);
   if (st != PCX_OK) return st;
+  // A tick that admitted requests runs one sweep: most requests settle or
+  // fail in their first sweep and are handed back at the end of the tick.
+  // Other ticks run sweeps_per_tick sweeps for the requests still going.
+  const bool fresh = s->stats.admitted > admitted;
+  const int n_sweeps = fresh ? 1 : s->cfg.sweeps_per_tick;
   int32_t running = 0, in_batch = 0;
   bool div = false;
   for (const Slot& sl : s->slots) {
@@ -474,13 +482,13 @@ pcx_status tick(pcx_server* s) {
     const int grid = pcx_internal::device_cus(s->cfg.device) * 8;
     // A tick's sweeps read everything (slots, flags, sweep number) from
     // device memory, so they are captured once per kernel domain and replayed.
-    hipGraphExec_t& gx = s->tick_graph[div ? 1 : 0];
+    hipGraphExec_t& gx = s->tick_graph[div ? 1 : 0][fresh ? 1 : 0];
     if (!gx) {
       const WorkList wl = worklist(s);
       hipGraph_t graph = nullptr;
       PCX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
       hipError_t ce = hipSuccess;
-      for (int g = 0; g < s->cfg.sweeps_per_tick && ce == hipSuccess; ++g) {
+      for (int g = 0; g < n_sweeps && ce == hipSuccess; ++g) {
         ce = launch_schedule_slots(s->d_slots, s->d_flags, s->n_slots, wl, ppw, s->stream);
         if (ce == hipSuccess)
           ce = launch_sweep_slots(s->d_slots, s->d_flags, wl, s->d_tight, ppw, div ? kItvDiv : kItv,
@@ -496,8 +504,8 @@ pcx_status tick(pcx_server* s) {
       PCX_HIP(ie);
     }
     PCX_HIP(hipGraphLaunch(gx, s->stream));
-    s->next_sweep += s->cfg.sweeps_per_tick;
-    s->stats.sweeps += s->cfg.sweeps_per_tick;
+    s->next_sweep += n_sweeps;
+    s->stats.sweeps += n_sweeps;
     PCX_HIP(hipMemcpyAsync(s->h_flags, s->d_flags, s->n_slots * sizeof(InstFlags), hipMemcpyDeviceToHost, s->stream));
     PCX_HIP(hipStreamSynchronize(s->stream));
     st = harvest(s);
@@ -541,8 +549,9 @@ void pcx_server_destroy(pcx_server* s) {
   for (SideLane& ln : s->lanes)
     if (ln.stream) (void)hipStreamSynchronize(ln.stream);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  for (hipGraphExec_t g : s->tick_graph)
-    if (g) (void)hipGraphExecDestroy(g);
+  for (auto& row : s->tick_graph)
+    for (hipGraphExec_t g : row)
+      if (g) (void)hipGraphExecDestroy(g);
   for (void* p : {(void*)s->d_slots, (void*)s->d_flags, (void*)s->d_act, (void*)s->d_item_base,
                   (void*)s->d_ctl, (void*)s->d_tight, (void*)s->d_arena})
     if (p) (void)hipFree(p);
