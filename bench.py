@@ -86,10 +86,12 @@ class StatSum:
 
     def __init__(self):
         self.v = {k: 0 for k in self.FIELDS}
+        self.lockstep = 0
 
     def add(self, st):
         for k in self.FIELDS:
             self.v[k] += getattr(st, k)
+        self.lockstep = int(st.lockstep)
 
     def roofline(self, steps, full_kernel, marked_kernel, workload, tight_bytes=BYTES_PER_TIGHTENING):
         """Roofline of the dominant sweep kernel (the class with more kernel
@@ -106,6 +108,14 @@ class StatSum:
             "marked": {"kernel": marked_kernel, "launches": marked[3], "kernel_ms": round(marked[2], 3),
                        "evals": int(marked[0])}}
         return rf
+
+
+def full_kernel(mode, sums):
+    """Name of the kernel of the full sweeps: the lockstep sweep's span also
+    holds the interleave of the stores before it."""
+    if mode != pcx.MODE_BATCHED:
+        return "k_persistent"
+    return "k_sweep_lockstep" if sums.lockstep else "k_sweep_batched"
 
 
 def autotune(args, pir, batch, key):
@@ -142,7 +152,7 @@ def bench_batched(args):
     p = pcx.gen_params(seed=args.seed, nvars=V, nprops=P)
     op, x, y, z, lb, ub, w = pcx.gen_network(p)
     pir = pcx.PIR(props_per_wave=args.ppw, timing=1, compact_tables=args.compact,
-                  stream_tables=args.stream, event_driven=args.events)
+                  stream_tables=args.stream, event_driven=args.events, lockstep=args.lockstep)
     pir.tell(op, x, y, z, lb, ub)
     root_lb, root_ub = lb.copy(), ub.copy()
     rr = pir.fixpoint(root_lb, root_ub)
@@ -196,14 +206,13 @@ def bench_batched(args):
                    "nvars": V, "nprops": P, "instances": n_inst, "props_per_wave": args.ppw,
                    "mode": {pcx.MODE_BATCHED: "batched", pcx.MODE_PERSISTENT: "persistent"}[mode],
                    "compact_tables": args.compact, "stream_tables": args.stream,
-                   "event_driven": args.events, "autotuned": tuning},
+                   "event_driven": args.events, "lockstep": sums.lockstep, "autotuned": tuning},
         "plain_fixpoints_per_s": plain_fps,
         "batched_vs_plain": fps / plain_fps if plain_fps else None,
         "evals_per_s": evals / elapsed,
         "failed_instances": int(failed),
         "sweeps_max": int(smax),
-        "roofline": sums.roofline(args.steps, "k_sweep_batched" if mode == pcx.MODE_BATCHED else "k_persistent",
-                                  "k_sweep_marked", "cfg2"),
+        "roofline": sums.roofline(args.steps, full_kernel(mode, sums), "k_sweep_marked", "cfg2"),
     }
 
     if args.search_step:
@@ -360,7 +369,7 @@ def bench_bits(args):
     V, P, n_inst = 200000, 800000, 1024
     op, x, y, z, w = pcx.gen_bits_network(args.seed, V, P)
     pir = pcx.BitsPIR(props_per_wave=args.ppw, timing=1, stream_tables=args.stream,
-                      event_driven=args.events)
+                      event_driven=args.events, lockstep=args.lockstep)
     pir.tell(op, x, y, z, w)
     root = w.copy()
     t0 = time.perf_counter()
@@ -386,14 +395,14 @@ def bench_bits(args):
     evals = sums.v["evals"]
     fps = n_inst * args.steps / elapsed
     words_g = batch.read_words()
-    rf = sums.roofline(args.steps, "k_sweep_batched_bits" if mode == pcx.MODE_BATCHED else "k_persistent_bits",
-                       "k_sweep_marked_bits", "cfg3", tight_bytes=8)
+    rf = sums.roofline(args.steps, full_kernel(mode, sums) + "_bits", "k_sweep_marked_bits", "cfg3",
+                       tight_bytes=8)
     out = {"value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
            "config": {"workload": "bitset store: 1,024 bit-removal subproblems of one 200,000-word / "
                                   "800,000-propagator network (add, neq, leq, eq-reif)",
                       "nvars": V, "nprops": P, "instances": n_inst,
                       "mode": {pcx.MODE_BATCHED: "batched", pcx.MODE_PERSISTENT: "persistent"}[mode],
-                      "autotuned": tuning},
+                      "lockstep": sums.lockstep, "autotuned": tuning},
            "single_fixpoint_ms": single_ms, "single_sweeps": int(r_single.sweeps),
            "evals_per_s": evals / elapsed, "failed_instances": int(sum(r.failed for r in res)),
            "roofline": rf}
@@ -554,6 +563,9 @@ def main():
     ap.add_argument("--retune", action="store_true", help="ignore a cached tuning")
     ap.add_argument("--events", type=int, default=-1, choices=[-1, 0, 1],
                     help="sweeps after the first: -1 auto, 0 every propagator, 1 marked only")
+    ap.add_argument("--lockstep", type=int, default=-1, choices=[-1, 0, 16, 32, 64],
+                    help="configs 2 and 3, full sweeps: -1 auto, 0 per-instance work items, "
+                         "16/32/64 instances per lockstep group")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

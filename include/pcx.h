@@ -90,7 +90,11 @@ typedef struct {
                                propagators per instance), 0 every sweep evaluates
                                every propagator, 1 sweeps after the first evaluate
                                only propagators with an operand tightened since     */
-  int32_t reserved[3];
+  int32_t lockstep;         /* batched mode, full sweeps: 0 per-instance work items;
+                               16, 32 or 64: one propagator for that many instances
+                               per step, over an instance-interleaved copy of the
+                               stores (8 B per variable per instance); -1 auto     */
+  int32_t reserved[2];
 } pcx_config;
 
 typedef struct {
@@ -113,6 +117,8 @@ typedef struct {
   int64_t full_launches;
   int64_t full_evals;
   int64_t full_tightened;
+  int64_t lockstep;         /* instances per lockstep group of the full sweeps
+                               (batched mode), 0 for per-instance work items  */
 } pcx_stats;
 
 /* ---- library ---------------------------------------------------------- */
@@ -243,11 +249,12 @@ typedef struct {
   int32_t compact_tables;   /* 0 or 1 (only varied for batches with subsets)    */
   int32_t stream_tables;    /* 0 or 1                                           */
   int32_t event_driven;     /* batched: 0 or 1                                  */
+  int32_t lockstep;         /* batched: 0, 16, 32 or 64                         */
   double  ms;               /* best host time of one reset + run; < 0: failed   */
 } pcx_tuning;
 
 /* Time every point of {batched: props_per_wave x compact x stream x
- * event_driven} U {persistent: lds (where the store fits) x compact} on the
+ * event_driven, and lockstep 16/32/64 with event_driven} U {persistent: lds (where the store fits) x compact} on the
  * batch's loaded image (one warm-up and `reps` timed runs per point, best
  * kept), set the engine's configuration to the fastest point, and restore
  * the stores to the image.  results (may be NULL): up to max_results points

@@ -49,6 +49,8 @@ bool valid_ppw(int p) {
   return p == 64 || p == 128 || p == 256 || p == 512 || p == 1024 || p == 2048;
 }
 
+bool valid_lockstep(int g) { return g == -1 || g == 0 || g == 16 || g == 32 || g == 64; }
+
 template <class T>
 void dfree(T*& p) {
   if (p) (void)hipFree((void*)p);
@@ -167,6 +169,7 @@ struct SweepTimer {
   std::vector<char> full;             // class of each recorded launch
   double ms[2] = {0.0, 0.0};          // [0] marked, [1] full
   int64_t launches[2] = {0, 0};
+  int64_t lockstep = 0;               // group size of the lockstep full sweeps
   pcx_status begin(hipStream_t st) {
     if (on) PCX_HIP(hipEventRecord(event_at(b, nev++), st));
     return PCX_OK;
@@ -234,6 +237,7 @@ pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* resu
     stats->full_launches = tm.launches[1];
     stats->full_evals = full_evals;
     stats->full_tightened = (int64_t)tf;
+    stats->lockstep = tm.lockstep;
   }
   return PCX_OK;
 }
@@ -287,6 +291,20 @@ int64_t summary_words(const pcx_batch* b) {
   return (chunks + 31) / 32;
 }
 
+// Member bitmaps of a subset batch (bit p of row k: instance k has
+// propagator p), rebuilt when the table changed.
+pcx_status ensure_members(pcx_batch* b) {
+  pcx_engine* e = b->e;
+  if (b->h_off.empty() || (b->d_member && b->member_version == e->version)) return PCX_OK;
+  const int64_t words = bitmap_words(b);
+  const int64_t rows = b->n_inst * words;
+  if (!b->d_member) PCX_HIP(hipMalloc(&b->d_member, rows * sizeof(uint32_t)));
+  PCX_HIP(hipMemsetAsync(b->d_member, 0, rows * sizeof(uint32_t), e->stream));
+  PCX_HIP(launch_build_members(b->d_off, b->d_idx, b->n_inst, b->d_member, words, e->stream));
+  b->member_version = e->version;
+  return PCX_OK;
+}
+
 // Event-driven sweeps need the adjacency, the member bitmaps of subset
 // batches and two dirty bitmaps per instance; automatic mode uses them when
 // they take at most a quarter of free device memory.  The persistent mode
@@ -301,13 +319,7 @@ pcx_status ensure_event_buffers(pcx_batch* b) {
   if (!b->d_dirty) PCX_HIP(hipMalloc(&b->d_dirty, 2 * rows * sizeof(uint32_t)));
   const int64_t srows = b->n_inst * summary_words(b);
   if (!b->d_summary) PCX_HIP(hipMalloc(&b->d_summary, 3 * srows * sizeof(uint32_t)));
-  if (!b->h_off.empty() && (!b->d_member || b->member_version != e->version)) {
-    if (!b->d_member) PCX_HIP(hipMalloc(&b->d_member, rows * sizeof(uint32_t)));
-    PCX_HIP(hipMemsetAsync(b->d_member, 0, rows * sizeof(uint32_t), e->stream));
-    PCX_HIP(launch_build_members(b->d_off, b->d_idx, b->n_inst, b->d_member, words, e->stream));
-    b->member_version = e->version;
-  }
-  return PCX_OK;
+  return ensure_members(b);
 }
 
 pcx_status clear_marks(pcx_batch* b) {
@@ -396,6 +408,7 @@ pcx_status prepare_run(pcx_batch* b, int32_t mode) {
     PCX_HIP(launch_store_check_empty(b->d_store, b->nvars, (int32_t)b->n_inst, b->d_flags, dom_of(e),
                                      e->stream));
   }
+  b->run_from_image = b->store_is_image;
   b->store_is_image = false;
   pcx_status st = prepare_tables(b);
   if (st != PCX_OK) return st;
@@ -463,17 +476,56 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   return collect(b, flags, results, stats, tm);
 }
 
+// Instances per lockstep group of this run's full sweeps, 0 for per-instance
+// work items.  Automatic mode takes the largest group whose interleaved
+// stores fit in kLockGroupBytes (the groups one XCD sweeps at a time stay
+// in the memory-side cache; larger groups were slower) and that the batch
+// fills four times over, when the interleaved copy takes at most a quarter
+// of free device memory.
+constexpr int64_t kLockGroupBytes = (int64_t)32 << 20;
+int lockstep_of(const pcx_batch* b) {
+  const pcx_engine* e = b->e;
+  int g = e->cfg.lockstep;
+  if (b->n_inst < 2 || b->table_n == 0) return 0;
+  if (g < 0) {
+    g = 0;
+    for (int c : {64, 32, 16})
+      if (c * b->nvars * 8 <= kLockGroupBytes && b->n_inst >= 4 * c) { g = c; break; }
+    if (g == 0) return 0;
+    if (!b->d_shadow) {
+      size_t free_b = 0, total_b = 0;
+      const uint64_t need = (uint64_t)((b->n_inst + g - 1) / g) * g * b->nvars * 8;
+      if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > free_b / 4) return 0;
+    }
+  }
+  return g;
+}
+
+pcx_status ensure_shadow(pcx_batch* b, int gi) {
+  const int64_t words = (b->n_inst + gi - 1) / gi * gi * b->nvars;
+  if (b->shadow_words >= words) return PCX_OK;
+  dfree(b->d_shadow);
+  b->shadow_words = 0;
+  b->shadow_is_image = false;
+  PCX_HIP(hipMalloc(&b->d_shadow, words * sizeof(int2)));
+  b->shadow_words = words;
+  return PCX_OK;
+}
+
 // Batched mode: per sweep one scheduler launch and one grid-stride sweep
-// launch over the work items of every running instance.
+// launch over the work items of every running instance (or, for full sweeps
+// in lockstep, over (group, table block) items).
 pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
   const int ppw = ppw_of(e);
+  const int gi = lockstep_of(b);
+  SweepTimer tm{b, e->cfg.timing != 0};
+  tm.lockstep = gi;
   const int group = e->cfg.sweeps_per_check > 0 ? e->cfg.sweeps_per_check : 8;
   const Table t = run_table(b);
   const int grid = pcx_internal::device_cus(e->cfg.device) * 8;   // a multiple of the XCD count
   const Subsets sub = run_subsets(b);
   const WorkList wl = worklist_of(b->d_act, b->d_item_base, b->d_ctl);
-  SweepTimer tm{b, e->cfg.timing != 0};
   int32_t live = 1, sweeps = 0;
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
   const EventSet ev = event_set(b);
@@ -535,9 +587,27 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       } else {
         ts = tm.begin(e->stream);
         if (ts != PCX_OK) return ts;
-        PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
-                                     b->d_tight, ppw, dom_of(e), b->stream, grid, e->stream,
-                                     b->events ? &ev : nullptr));
+        if (gi) {                            // snapshot of the stores, then the sweep reading it
+          ts = ensure_shadow(b, gi);
+          if (ts == PCX_OK) ts = ensure_members(b);
+          if (ts != PCX_OK) return ts;
+          // A run that starts from the image reuses the image's interleaved
+          // copy (the sweep only reads it) until the next load.
+          const bool from_image = b->run_from_image && sweeps == 0;
+          if (!(from_image && b->shadow_is_image && b->shadow_gi == gi)) {
+            PCX_HIP(launch_interleave(b->d_store, b->d_shadow, b->nvars, b->n_inst, gi, e->stream));
+            b->shadow_is_image = from_image;
+            b->shadow_gi = gi;
+          }
+          PCX_HIP(launch_sweep_lockstep(table_of(e), b->d_store, b->d_shadow, b->nvars,
+                                        (int32_t)b->n_inst, b->h_off.empty() ? nullptr : b->d_member,
+                                        bitmap_words(b), b->d_flags, wl, b->d_tight, gi, dom_of(e),
+                                        grid, e->stream, b->events ? &ev : nullptr));
+        } else {
+          PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
+                                       b->d_tight, ppw, dom_of(e), b->stream, grid, e->stream,
+                                       b->events ? &ev : nullptr));
+        }
         ts = tm.end(e->stream, true);
         if (ts != PCX_OK) return ts;
       }
@@ -606,6 +676,7 @@ void pcx_config_default(pcx_config* c) {
   c->compact_tables = -1;
   c->stream_tables = -1;
   c->event_driven = -1;
+  c->lockstep = -1;
 }
 
 int32_t pcx_device_count(void) {
@@ -717,6 +788,8 @@ pcx_status pcx_set_config(pcx_engine* e, const pcx_config* cfg) {
     return fail_with(PCX_EINVAL, "device and domain are fixed at creation");
   if (cfg->props_per_wave != 0 && !valid_ppw(cfg->props_per_wave))
     return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto) or a power of two in [64, 2048]");
+  if (!valid_lockstep(cfg->lockstep))
+    return fail_with(PCX_EINVAL, "lockstep must be -1 (auto), 0 (off), 16, 32 or 64");
   e->cfg = *cfg;
   return PCX_OK;
 }
@@ -824,7 +897,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   dfree(b->d_store); dfree(b->d_image); dfree(b->d_off); dfree(b->d_idx); dfree(b->d_flags);
   dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl); dfree(b->d_order);
   dfree(b->d_cxo); dfree(b->d_cy); dfree(b->d_cz); dfree(b->d_member); dfree(b->d_dirty);
-  dfree(b->d_summary); dfree(b->d_image_empty); dfree(b->d_unpack);
+  dfree(b->d_summary); dfree(b->d_image_empty); dfree(b->d_unpack); dfree(b->d_shadow);
   if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
   delete b;
 }
@@ -838,6 +911,7 @@ pcx_status image_loaded(pcx_batch* b) {
                                  e->stream));
   PCX_HIP(hipStreamSynchronize(e->stream));
   b->store_is_image = true;
+  b->shadow_is_image = false;
   return PCX_OK;
 }
 }  // namespace
@@ -1004,11 +1078,16 @@ pcx_status pcx_batch_autotune(pcx_batch* b, int32_t reps, pcx_tuning* best, pcx_
     for (int c = 0; c <= (subsets ? 1 : 0); ++c)
       for (int st = 0; st <= 1; ++st)
         for (int ev = 0; ev <= 1; ++ev)
-          pts.push_back(pcx_tuning{PCX_MODE_BATCHED, ppw, 0, c, st, ev, 0.0});
+          pts.push_back(pcx_tuning{PCX_MODE_BATCHED, ppw, 0, c, st, ev, 0, 0.0});
+  // lockstep full sweeps read the shared table: props_per_wave, compact and
+  // stream only shape the marked sweeps that follow
+  if (b->n_inst >= 2)
+    for (int gi : {16, 32, 64})
+      pts.push_back(pcx_tuning{PCX_MODE_BATCHED, 512, 0, 0, 0, 1, gi, 0.0});
   const bool lds_fits = b->nvars * 8 <= kLdsStoreBytes;
   for (int l = 0; l <= (lds_fits ? 1 : 0); ++l)
     for (int c = 0; c <= (subsets ? 1 : 0); ++c)
-      pts.push_back(pcx_tuning{PCX_MODE_PERSISTENT, 0, l, c, 0, 0, 0.0});
+      pts.push_back(pcx_tuning{PCX_MODE_PERSISTENT, 0, l, c, 0, 0, 0, 0.0});
   int best_i = -1;
   for (size_t i = 0; i < pts.size(); ++i) {
     pcx_tuning& p = pts[i];
@@ -1017,6 +1096,7 @@ pcx_status pcx_batch_autotune(pcx_batch* b, int32_t reps, pcx_tuning* best, pcx_
     e->cfg.compact_tables = p.compact_tables;
     e->cfg.stream_tables = p.stream_tables;
     e->cfg.event_driven = p.event_driven;
+    e->cfg.lockstep = p.lockstep;
     e->cfg.timing = 0;
     double bt = -1.0;
     for (int r = 0; r <= reps; ++r) {
@@ -1046,6 +1126,7 @@ pcx_status pcx_batch_autotune(pcx_batch* b, int32_t reps, pcx_tuning* best, pcx_
   e->cfg.compact_tables = w.compact_tables;
   e->cfg.stream_tables = w.stream_tables;
   e->cfg.event_driven = w.event_driven;
+  e->cfg.lockstep = w.lockstep;
   *best = w;
   pcx_status s = pcx_batch_reset(b);
   if (s == PCX_OK) PCX_HIP(hipStreamSynchronize(e->stream));

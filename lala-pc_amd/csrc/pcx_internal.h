@@ -55,6 +55,12 @@ struct pcx_batch {
   uint32_t* d_dirty = nullptr;
   uint32_t* d_summary = nullptr;      // chunk summaries (2 planes) + scheduler snapshot
   uint64_t member_version = 0;
+  // lockstep full sweeps: instance-interleaved copy of the stores
+  int2* d_shadow = nullptr;
+  int64_t shadow_words = 0;
+  int shadow_gi = 0;                  // group size of its layout
+  bool shadow_is_image = false;       // it holds the image (interleaved since the last load)
+  bool run_from_image = false;        // this run started from stores equal to the image
   bool events = false;                // this run uses marked-only sweeps
   bool seeded = false;                // the next run may start from seeded marks
   bool seeded_run = false;            // this run starts with a marked sweep

@@ -156,6 +156,16 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
                                 int64_t n_props, InstFlags* flags, WorkList wl,
                                 unsigned long long* tight, int ppw, int dom, bool stream_table,
                                 int grid, hipStream_t st, const EventSet* ev = nullptr);
+// Lockstep full sweep (gi in {16, 32, 64} instances per group): shadow is the
+// instance-interleaved copy of the stores made by launch_interleave (groups
+// of gi instances; ceil(n_inst / gi) * gi * nvars words), member the member
+// bitmaps of a subset batch (nullptr: every propagator).
+hipError_t launch_interleave(const int2* store, int2* shadow, int64_t nvars, int64_t n_inst, int gi,
+                             hipStream_t st);
+hipError_t launch_sweep_lockstep(const Table& t, int2* store, int2* shadow, int64_t nvars,
+                                 int32_t n_inst, const uint32_t* member, int64_t words,
+                                 InstFlags* flags, WorkList wl, unsigned long long* tight, int gi,
+                                 int dom, int grid, hipStream_t st, const EventSet* ev = nullptr);
 constexpr int kMarkedWordsPerItem = kChunkWords;   // bitmap words per marked-sweep item
 // lds: stage each instance's store in LDS (nvars * 8 <= kLdsStoreBytes).
 constexpr int64_t kLdsStoreBytes = 64 * 1024;

@@ -876,6 +876,244 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
              : launch_batched_nt<false>(t, store, nvars, sub, n_props, flags, wl, tight, ppl, dom, grid, e, stream);
 }
 
+// ---- lockstep full sweeps: GI instances side by side ---------------------
+// A full sweep of many instances of one table evaluates the same
+// propagators in every instance.  The lockstep sweep runs a propagator for
+// GI instances at once (lane = instance slot i + GI * propagator), reading an
+// instance-interleaved copy of the stores in which variable v of the group's
+// GI instances is one contiguous run of GI words: its three operand gathers
+// are each one 8*GI-byte run per propagator instead of GI scattered words.
+// The copy is a snapshot taken before the sweep and only read during it;
+// tightenings go to the stores (atomics, as in every sweep).  A sweep that
+// reads the stores as they were at its start is a valid sweep (what it reads
+// is at least as tight as the store at its start), and with a read-only copy
+// the operand loads of several propagators are issued together.
+
+// Interleaved copy: group g = instances [g*GI, g*GI + GI), variable v of
+// slot i at word (g * nvars + v) * GI + i.  One workgroup transposes a tile
+// of GI instances x (4096 / GI) variables through LDS; both sides are
+// contiguous runs.
+template <int GI>
+__global__ __launch_bounds__(kBlock) void k_interleave(const u64* __restrict__ store,
+                                                      u64* __restrict__ sh, int64_t nvars,
+                                                      int64_t n_inst, int64_t tiles_per_group) {
+  constexpr int kTV = 4096 / GI;              // variables per tile
+  __shared__ u64 tile[GI][kTV + 1];
+  const int64_t g = blockIdx.x / tiles_per_group;
+  const int64_t v0 = (blockIdx.x - g * tiles_per_group) * kTV;
+#pragma unroll
+  for (int r = 0; r < 4096 / kBlock; ++r) {
+    const int e = r * kBlock + threadIdx.x;
+    const int i = e / kTV, j = e - i * kTV;
+    const int64_t k = g * GI + i;
+    tile[i][j] = (k < n_inst && v0 + j < nvars) ? store[k * nvars + v0 + j] : 0;
+  }
+  __syncthreads();
+  u64* out = sh + (g * nvars + v0) * GI;
+  const int64_t lim = min((int64_t)kTV, nvars - v0) * GI;
+#pragma unroll
+  for (int r = 0; r < 4096 / kBlock; ++r) {
+    const int e = r * kBlock + threadIdx.x;
+    if (e < lim) out[e] = tile[e % GI][e / GI];
+  }
+}
+
+hipError_t launch_interleave(const int2* store, int2* shadow, int64_t nvars, int64_t n_inst, int gi,
+                             hipStream_t stream) {
+  if (nvars <= 0 || n_inst <= 0) return hipSuccess;
+  const int64_t groups = (n_inst + gi - 1) / gi;
+  const int64_t tv = 4096 / gi;
+  const int64_t tpg = (nvars + tv - 1) / tv;
+  const int64_t grid = groups * tpg;
+  if (grid > 0x7fffffff) return hipErrorInvalidValue;
+  const u64* in = reinterpret_cast<const u64*>(store);
+  u64* out = reinterpret_cast<u64*>(shadow);
+  switch (gi) {
+    case 16: k_interleave<16><<<(unsigned)grid, kBlock, 0, stream>>>(in, out, nvars, n_inst, tpg); break;
+    case 32: k_interleave<32><<<(unsigned)grid, kBlock, 0, stream>>>(in, out, nvars, n_inst, tpg); break;
+    case 64: k_interleave<64><<<(unsigned)grid, kBlock, 0, stream>>>(in, out, nvars, n_inst, tpg); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Tightened (instance, variable) pairs are deferred to an LDS list and
+// marked by the whole workgroup at the end of the work item; a pair that
+// does not fit marks its whole instance for the next sweep instead (its
+// all_marked word).
+constexpr int kLockMarkCap = 1024;
+constexpr int kLockWordsPerWave = 8;       // 32-propagator table words per wave per item
+constexpr int kLockUnroll = 2;             // propagators whose operands are loaded together
+
+template <int D, int GI>
+__global__ __launch_bounds__(kBlock) void k_sweep_lockstep(Table t, int2* __restrict__ store,
+                                                          const int2* __restrict__ shadow, int64_t nvars,
+                                                          int32_t n_inst,
+                                                          const uint32_t* __restrict__ member,
+                                                          int64_t words, InstFlags* flags,
+                                                          WorkList wl, unsigned long long* tight,
+                                                          EventSet ev) {
+  constexpr int kPpi = kWave / GI;                           // propagators per wave step
+  constexpr int kWpi = kLockWordsPerWave * (kBlock / kWave);  // table words per item
+  const int32_t s = *wl.sweep;
+  const int64_t tw = (t.n + 31) / 32;
+  const int64_t ipg = (tw + kWpi - 1) / kWpi;
+  const int64_t groups = ((int64_t)n_inst + GI - 1) / GI;
+  const int64_t total = groups * ipg;
+  // XCD-aware: each XCD takes a contiguous range of (group-major) items and
+  // claims them in order, so its workgroups share the few groups in flight.
+  const int xcd = blockIdx.x % kXcds;
+  const int64_t first_item = total * xcd / kXcds;
+  const int64_t end_item = total * (xcd + 1) / kXcds;
+  int32_t* qc = wl.xq + xcd;
+  __shared__ int32_t s_next;
+  __shared__ int32_t s_n;
+  __shared__ int32_t s_mk[kLockMarkCap], s_mv[kLockMarkCap];
+  if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; }
+  __syncthreads();
+  int64_t item = first_item + __builtin_amdgcn_readfirstlane(s_next);
+  __syncthreads();
+  const int lane = lane_id();
+  const int slot = lane % GI, sub = lane / GI;
+  const int wave = threadIdx.x / kWave;
+  while (item < end_item) {
+    int32_t nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(qc, 1);
+    const int64_t g = uni(item / ipg);
+    const int64_t w0 = (item - g * ipg) * kWpi + wave * kLockWordsPerWave;
+    const int64_t k = g * GI + slot;
+    bool run = false, am = true;
+    InstFlags* f = flags + (k < n_inst ? k : 0);
+    if (k < n_inst) {
+      run = !(f->done_at >= 0 && f->done_at <= s) && f->failed_at == kNoFail;
+      am = !ev.dirty || f->all_marked[(s + 1) & 1] == s + 1;
+    }
+    int2* sk = store + k * nvars;
+    const int2* sg = shadow + g * nvars * GI + slot;
+    const uint32_t* mrow = member ? member + k * words : nullptr;
+    bool chg = false, fail = false;
+    int nt = 0;
+    auto mark = [&](int32_t v) {
+      if (am) return;
+      const int i = atomicAdd(&s_n, 1);
+      if (i < kLockMarkCap) { s_mk[i] = (int32_t)k; s_mv[i] = v; }
+      else { f->all_marked[(s + 1) & 1] = s + 1; am = true; }
+    };
+    for (int64_t w = w0; w < min(w0 + kLockWordsPerWave, tw); ++w) {
+      uint32_t m = 0;
+      if (run) {
+        m = mrow ? mrow[w] : ~0u;
+        const int64_t left = t.n - w * 32;
+        if (left < 32) m &= (1u << left) - 1u;
+      }
+      if (!__ballot(m != 0)) continue;
+      const int64_t p = w * 32 + (lane & 31);
+      const bool pin = p < t.n;
+      const uint32_t txo = pin ? t.xo[p] : 0u;
+      const int32_t ty = pin ? t.y[p] : 0, tz = pin ? t.z[p] : 0;
+      for (int j0 = 0; j0 < 32; j0 += kPpi * kLockUnroll) {
+        uint32_t xo[kLockUnroll];
+        int32_t y[kLockUnroll], z[kLockUnroll];
+        bool on[kLockUnroll];
+        u64 wx[kLockUnroll], wy[kLockUnroll], wz[kLockUnroll];
+#pragma unroll
+        for (int u = 0; u < kLockUnroll; ++u) {
+          const int src = j0 + u * kPpi + sub;
+          if constexpr (GI == kWave) {
+            xo[u] = (uint32_t)__builtin_amdgcn_readlane((int)txo, j0 + u);
+            y[u] = __builtin_amdgcn_readlane(ty, j0 + u);
+            z[u] = __builtin_amdgcn_readlane(tz, j0 + u);
+          } else {
+            xo[u] = (uint32_t)__shfl((int)txo, src, kWave);
+            y[u] = __shfl(ty, src, kWave);
+            z[u] = __shfl(tz, src, kWave);
+          }
+          on[u] = (m >> src) & 1u;
+        }
+#pragma unroll
+        for (int u = 0; u < kLockUnroll; ++u) {
+          if (!on[u]) continue;
+          const int32_t x = (int32_t)(xo[u] & kVarMask);
+          wx[u] = *reinterpret_cast<const u64*>(sg + (int64_t)x * GI);
+          wy[u] = *reinterpret_cast<const u64*>(sg + (int64_t)y[u] * GI);
+          wz[u] = *reinterpret_cast<const u64*>(sg + (int64_t)z[u] * GI);
+        }
+#pragma unroll
+        for (int u = 0; u < kLockUnroll; ++u) {
+          if (!on[u]) continue;
+          const int op = (int)(xo[u] >> kOpShift);
+          const int32_t x = (int32_t)(xo[u] & kVarMask);
+          bool tx, ty2, tz2;
+          if constexpr (D == kBits) {
+            if (!wx[u] || !wy[u] || !wz[u]) { fail = true; continue; }
+            u64 nx, ny, nz;
+            deduce_bits(op, wx[u], wy[u], wz[u], nx, ny, nz);
+            u64* sk64 = reinterpret_cast<u64*>(sk);
+            tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, x, wx[u], nx, chg, fail, nt);
+            ty2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, y[u], wy[u], ny, chg, fail, nt);
+            tz2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, z[u], wz[u], nz, chg, fail, nt);
+          } else {
+            const int2 X = as_itv(wx[u]), Y = as_itv(wy[u]), Z = as_itv(wz[u]);
+            if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = true; continue; }
+            int2 nx, ny, nz;
+            deduce<D == kItvDiv>(op, X, Y, Z, nx, ny, nz);
+            if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
+            tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, x, X, nx, chg, fail, nt);
+            ty2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, y[u], Y, ny, chg, fail, nt);
+            tz2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, z[u], Z, nz, chg, fail, nt);
+          }
+          if (tx) mark(x);
+          if (ty2) mark(y[u]);
+          if (tz2) mark(z[u]);
+          // one propagator after the other: interleaving their arithmetic
+          // would hold every one's temporaries at once
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+    if (chg) f->changed[s & 1] = s;
+    if (fail) f->failed_at = s;
+    const int sum = wave_sum(nt);
+    if (lane == 0 && sum)
+      atomicAdd(&tight[(blockIdx.x * 4 + wave) & (kTightSlots - 1)], (unsigned long long)sum);
+    __syncthreads();                            // the item's marks are listed
+    const int n = min(s_n, kLockMarkCap);
+    for (int i = threadIdx.x; i < n; i += kBlock) mark_var(marker_for(&ev, s_mk[i], s), s_mv[i]);
+    __syncthreads();                            // every thread has read s_n and the list
+    if (threadIdx.x == 0) { s_next = nxt; s_n = 0; }
+    __syncthreads();
+    item = first_item + __builtin_amdgcn_readfirstlane(s_next);
+    __syncthreads();
+  }
+}
+
+hipError_t launch_sweep_lockstep(const Table& t, int2* store, int2* shadow, int64_t nvars,
+                                 int32_t n_inst, const uint32_t* member, int64_t words,
+                                 InstFlags* flags, WorkList wl, unsigned long long* tight, int gi,
+                                 int dom, int grid, hipStream_t stream, const EventSet* ev) {
+  const EventSet none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0};
+  const EventSet& e = ev ? *ev : none;
+  if (t.n <= 0 || n_inst <= 0) return hipSuccess;
+#define PCX_LOCK(D, G) \
+  k_sweep_lockstep<D, G><<<grid, kBlock, 0, stream>>>(t, store, shadow, nvars, n_inst, member, words, flags, wl, tight, e)
+#define PCX_LOCK_GI(D)                   \
+  switch (gi) {                          \
+    case 16: PCX_LOCK(D, 16); break;     \
+    case 32: PCX_LOCK(D, 32); break;     \
+    case 64: PCX_LOCK(D, 64); break;     \
+    default: return hipErrorInvalidValue; \
+  }
+  switch (dom) {
+    case kItv: PCX_LOCK_GI(kItv); break;
+    case kItvDiv: PCX_LOCK_GI(kItvDiv); break;
+    case kBits: PCX_LOCK_GI(kBits); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef PCX_LOCK_GI
+#undef PCX_LOCK
+  return hipGetLastError();
+}
+
 // Marked-only sweep over every running instance: items are (instance,
 // kMarkedWordsPerItem bitmap words), claimed per XCD as in the full sweep.
 template <int D>

@@ -50,7 +50,7 @@ class Config(ctypes.Structure):
                 ("max_sweeps", ctypes.c_int64), ("timing", ctypes.c_int32),
                 ("domain", ctypes.c_int32), ("compact_tables", ctypes.c_int32),
                 ("stream_tables", ctypes.c_int32), ("event_driven", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("lockstep", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class Result(ctypes.Structure):
@@ -67,7 +67,8 @@ class Stats(ctypes.Structure):
                 ("kernel_launches", ctypes.c_int64), ("evals", ctypes.c_int64),
                 ("tightened", ctypes.c_int64), ("sweeps_max", ctypes.c_int64),
                 ("full_ms", ctypes.c_double), ("full_launches", ctypes.c_int64),
-                ("full_evals", ctypes.c_int64), ("full_tightened", ctypes.c_int64)]
+                ("full_evals", ctypes.c_int64), ("full_tightened", ctypes.c_int64),
+                ("lockstep", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -87,7 +88,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("props_per_wave", ctypes.c_int32),
                 ("lds_mode", ctypes.c_int32), ("compact_tables", ctypes.c_int32),
                 ("stream_tables", ctypes.c_int32), ("event_driven", ctypes.c_int32),
-                ("ms", ctypes.c_double)]
+                ("lockstep", ctypes.c_int32), ("ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -554,7 +555,8 @@ def save_tuning(path, key, best, points=None):
 def apply_tuning(pir, best):
     """Configure the engine with a saved tuning; returns the mode to run."""
     pir.set_config(**{k: best[k] for k in ("props_per_wave", "lds_mode", "compact_tables",
-                                           "stream_tables", "event_driven")})
+                                           "stream_tables", "event_driven", "lockstep")
+                      if k in best})
     return best["mode"]
 
 

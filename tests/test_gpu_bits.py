@@ -99,6 +99,27 @@ def test_bits_batch_of_splits(gpu, mode, stream, events):
     _compare(b.read_words(), rg, wo, ro, V, f"bits batch mode {mode}")
 
 
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
+@pytest.mark.parametrize("gi", [16, 32, 64])
+def test_bits_lockstep(gpu, gi, events):
+    """Bitset full sweeps in lockstep groups of gi over 100 instances."""
+    V, P, n = 4000, 16000, 100
+    op, x, y, z, w = pcx.gen_bits_network(61, V, P)
+    pir = pcx.BitsPIR(event_driven=events, lockstep=gi)
+    pir.tell(op, x, y, z, w)
+    root = w.copy()
+    assert not pir.fixpoint(root).failed
+    inst = pcx.gen_bits_splits(62, root, n, 1, 40)
+    top, tx, ty, tz = pir.table()
+    wo = inst.copy()
+    ro = ol.bits_fixpoint_batch(top, tx, ty, tz, wo, V, threads=8)
+    b = pcx.Batch(pir, n, V)
+    b.load_words(inst)
+    rg, st = b.run(pcx.MODE_BATCHED)
+    assert st.lockstep == gi
+    _compare(b.read_words(), rg, wo, ro, V, f"bits lockstep {gi}")
+
+
 @pytest.mark.parametrize("lds", [1, 0])
 def test_bits_persistent_lds(gpu, lds):
     V, P, n = 2000, 8000, 48

@@ -260,6 +260,97 @@ def test_dense_first_sweep_marks_everything(gpu, mode):
     assert st.full_tightened > n_inst * 1000     # dense: far more than one mark list per item
 
 
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
+@pytest.mark.parametrize("gi", [16, 32, 64])
+def test_lockstep_full_sweeps(gpu, gi, events):
+    """Full sweeps in lockstep (gi instances per step over the
+    instance-interleaved copy of the stores): 100 instances (not a multiple
+    of gi), per-instance subsets, splits of the root fixpoint.  Every instance
+    equals the oracle, and a rerun after reset gives the same stores."""
+    V, P, n_inst = 5000, 20000, 100
+    op, x, y, z, lb, ub, w = _network(41, V, P)
+    pir = pcx.PIR(event_driven=events, lockstep=gi)
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_lb, root_ub).failed
+    slb, sub = pcx.gen_splits(42, root_lb, root_ub, n_inst, 1, 20)
+    off, idx = pcx.gen_subsets(43, P, n_inst, 200, 20000)
+    top, tx, ty, tz = pir.table()
+    lb_o, ub_o = slb.copy(), sub.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lb_o, ub_o, V, idx=idx.astype(np.int64), off=off, threads=8)
+    b = pcx.Batch(pir, n_inst, V, off, idx)
+    b.load(slb, sub)
+    res_g, st = b.run(pcx.MODE_BATCHED)
+    assert st.lockstep == gi
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"lockstep {gi} events {events}")
+    # a rerun from the same image reuses its interleaved copy
+    b.reset()
+    res_g2, _ = b.run(pcx.MODE_BATCHED)
+    lb_g2, ub_g2 = b.read()
+    assert (lb_g2 == lb_g).all() and (ub_g2 == ub_g).all()
+    # a new load replaces it
+    slb3, sub3 = pcx.gen_splits(44, root_lb, root_ub, n_inst, 1, 20)
+    lb_o3, ub_o3 = slb3.copy(), sub3.copy()
+    res_o3 = ol.fixpoint_batch(top, tx, ty, tz, lb_o3, ub_o3, V, idx=idx.astype(np.int64), off=off,
+                               threads=8)
+    b.load(slb3, sub3)
+    res_g3, _ = b.run(pcx.MODE_BATCHED)
+    lb_g3, ub_g3 = b.read()
+    _compare(lb_g3, ub_g3, res_g3, lb_o3, ub_o3, res_o3, V, f"lockstep {gi} after a new load")
+
+
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
+@pytest.mark.parametrize("gi", [16, 64])
+def test_lockstep_dense_first_sweep(gpu, gi, events):
+    """Unpropagated stores and no subsets: the lockstep first sweep tightens
+    most bounds, overflows its workgroups' mark lists (whole instances marked
+    for the next sweep) and, with full sweeps only, every later sweep is a
+    lockstep sweep over the same interleaved copy.  48 instances: one
+    partial group at gi = 64.  Every instance equals the oracle."""
+    V, P, n_inst = 20000, 80000, 48
+    op, x, y, z, lb, ub, w = _network(51, V, P)
+    pir = pcx.PIR(event_driven=events, lockstep=gi)
+    pir.tell(op, x, y, z, lb, ub)
+    slb, sub = pcx.gen_splits(52, lb, ub, n_inst, 1, 20)
+    top, tx, ty, tz = pir.table()
+    lb_o, ub_o = slb.copy(), sub.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lb_o, ub_o, V, threads=16)
+    b = pcx.Batch(pir, n_inst, V)
+    b.load(slb, sub)
+    res_g, st = b.run(pcx.MODE_BATCHED)
+    assert st.lockstep == gi
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"lockstep {gi} dense first sweep")
+    assert st.full_tightened > n_inst * 1000
+
+
+@pytest.mark.parametrize("name", list(ol.OPS))
+@pytest.mark.parametrize("gi", [32, 64])
+def test_lockstep_every_box(gpu, name, gi):
+    """All boxes x,y,z in [-3,3] (21,952 instances) for one propagator, swept
+    in lockstep groups of gi."""
+    X, Y, Z = _boxes(-3, 3)
+    n = len(X)
+    lb = np.stack([X[:, 0], Y[:, 0], Z[:, 0]], 1).astype(np.int32).ravel()
+    ub = np.stack([X[:, 1], Y[:, 1], Z[:, 1]], 1).astype(np.int32).ravel()
+    op = np.array([ol.OPS[name]], np.int32)
+    x, y, z = (np.array([v], np.int32) for v in (0, 1, 2))
+    if name in ("EQ", "LEQ"):
+        lb[0::3] = np.maximum(lb[0::3], 0)
+        ub[0::3] = np.minimum(ub[0::3], 1)
+    lb_o, ub_o = lb.copy(), ub.copy()
+    res_o = ol.fixpoint_batch(op, x, y, z, lb_o, ub_o, 3, threads=8)
+    pir = pcx.PIR(lockstep=gi)
+    pir.tell(op, x, y, z)
+    b = pcx.Batch(pir, n, 3)
+    b.load(lb, ub)
+    res_g, st = b.run(pcx.MODE_BATCHED)
+    assert st.lockstep == gi
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, 3, f"{name} lockstep {gi}")
+
+
 def test_edge_cases(gpu):
     pir = pcx.PIR()
     # empty table: fixpoint is the store itself, one quiet sweep
@@ -289,6 +380,9 @@ def test_edge_cases(gpu):
     assert e.value.status == pcx.EINVAL
     with pytest.raises(pcx.PcxError):
         pir3.tell([42], [0], [1], [2])
+    with pytest.raises(pcx.PcxError) as e:
+        pir3.set_config(lockstep=8)
+    assert e.value.status == pcx.EINVAL
 
 
 @pytest.mark.parametrize("lds", [1, 0, -1], ids=["lds", "global", "auto"])
@@ -331,12 +425,14 @@ def test_autotuner_choice_is_bit_exact(gpu, tmp_path):
     b = pcx.Batch(pir, n_inst, V, off, idx)
     b.load(slb, sub)
     best, points = b.autotune(reps=1)
-    assert len(points) == 48 + 4          # batched grid + persistent x {LDS, global} x {index, copy}
+    # batched grid + lockstep {16, 32, 64} + persistent x {LDS, global} x {index, copy}
+    assert len(points) == 48 + 3 + 4
+    assert sorted(p["lockstep"] for p in points if p["lockstep"]) == [16, 32, 64]
     assert all(p["ms"] > 0 for p in points)
     assert best["ms"] == min(p["ms"] for p in points)
     c = pir.get_config()
-    assert (c.props_per_wave, c.lds_mode, c.compact_tables) == \
-        (best["props_per_wave"], best["lds_mode"], best["compact_tables"])
+    assert (c.props_per_wave, c.lds_mode, c.compact_tables, c.lockstep) == \
+        (best["props_per_wave"], best["lds_mode"], best["compact_tables"], best["lockstep"])
     res_g, _ = b.run(best["mode"])
     lb_g, ub_g = b.read()
     _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"autotuned {best}")
@@ -346,6 +442,7 @@ def test_autotuner_choice_is_bit_exact(gpu, tmp_path):
     pir2 = pcx.PIR()
     assert pcx.apply_tuning(pir2, best) == best["mode"]
     assert pir2.get_config().event_driven == best["event_driven"]
+    assert pir2.get_config().lockstep == best["lockstep"]
 
 
 @pytest.mark.parametrize("mode", [pcx.MODE_PLAIN, pcx.MODE_BATCHED])
