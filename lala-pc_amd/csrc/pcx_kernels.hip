@@ -16,6 +16,8 @@
 #include "pcx_kernels.h"
 #include "pcx_prop.h"
 
+#include <type_traits>
+
 namespace pcx {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -999,75 +1001,148 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lockstep(Table t, int2* __rest
       if (i < kLockMarkCap) { s_mk[i] = (int32_t)k; s_mv[i] = v; }
       else { f->all_marked[(s + 1) & 1] = s + 1; am = true; }
     };
-    for (int64_t w = w0; w < min(w0 + kLockWordsPerWave, tw); ++w) {
-      uint32_t m = 0;
+    // The member word and the table entries of the next word are loaded
+    // while this word is swept.
+    const int64_t wend = min(w0 + kLockWordsPerWave, tw);
+    uint32_t m_n = 0, txo_n = 0;
+    int32_t ty_n = 0, tz_n = 0;
+    auto fetch = [&](int64_t w) {
+      m_n = 0;
       if (run) {
-        m = mrow ? mrow[w] : ~0u;
+        m_n = mrow ? mrow[w] : ~0u;
         const int64_t left = t.n - w * 32;
-        if (left < 32) m &= (1u << left) - 1u;
+        if (left < 32) m_n &= (1u << left) - 1u;
       }
-      if (!__ballot(m != 0)) continue;
       const int64_t p = w * 32 + (lane & 31);
       const bool pin = p < t.n;
-      const uint32_t txo = pin ? t.xo[p] : 0u;
-      const int32_t ty = pin ? t.y[p] : 0, tz = pin ? t.z[p] : 0;
-      for (int j0 = 0; j0 < 32; j0 += kPpi * kLockUnroll) {
-        uint32_t xo[kLockUnroll];
-        int32_t y[kLockUnroll], z[kLockUnroll];
-        bool on[kLockUnroll];
-        u64 wx[kLockUnroll], wy[kLockUnroll], wz[kLockUnroll];
+      txo_n = pin ? t.xo[p] : 0u;
+      ty_n = pin ? t.y[p] : 0;
+      tz_n = pin ? t.z[p] : 0;
+    };
+    if (w0 < wend) fetch(w0);
+    for (int64_t w = w0; w < wend; ++w) {
+      const uint32_t m = m_n, txo = txo_n;
+      const int32_t ty = ty_n, tz = tz_n;
+      if (w + 1 < wend) fetch(w + 1);
+      if (!__ballot(m != 0)) continue;
+      // The rounds of this word, for one opcode OPC (a compile-time constant:
+      // the propagator is branch-free up to a tightening, which is rare) or
+      // for mixed opcodes (OPC < 0, read per propagator).
+      auto rounds = [&](auto opc) {
+        constexpr int OPC = decltype(opc)::value;
+        constexpr bool kFlat = OPC >= 0 && (D == kBits || OPC < kTdiv || OPC > kEdiv);
+#pragma unroll 1
+        for (int j0 = 0; j0 < 32; j0 += kPpi * kLockUnroll) {
+          uint32_t xo[kLockUnroll];
+          int32_t y[kLockUnroll], z[kLockUnroll];
+          bool on[kLockUnroll];
+          u64 wx[kLockUnroll], wy[kLockUnroll], wz[kLockUnroll];
 #pragma unroll
-        for (int u = 0; u < kLockUnroll; ++u) {
-          const int src = j0 + u * kPpi + sub;
-          if constexpr (GI == kWave) {
-            xo[u] = (uint32_t)__builtin_amdgcn_readlane((int)txo, j0 + u);
-            y[u] = __builtin_amdgcn_readlane(ty, j0 + u);
-            z[u] = __builtin_amdgcn_readlane(tz, j0 + u);
-          } else {
-            xo[u] = (uint32_t)__shfl((int)txo, src, kWave);
-            y[u] = __shfl(ty, src, kWave);
-            z[u] = __shfl(tz, src, kWave);
+          for (int u = 0; u < kLockUnroll; ++u) {
+            const int src = j0 + u * kPpi + sub;
+            if constexpr (GI == kWave) {
+              xo[u] = (uint32_t)__builtin_amdgcn_readlane((int)txo, j0 + u);
+              y[u] = __builtin_amdgcn_readlane(ty, j0 + u);
+              z[u] = __builtin_amdgcn_readlane(tz, j0 + u);
+            } else {
+              xo[u] = (uint32_t)__shfl((int)txo, src, kWave);
+              y[u] = __shfl(ty, src, kWave);
+              z[u] = __shfl(tz, src, kWave);
+            }
+            on[u] = (m >> src) & 1u;
           }
-          on[u] = (m >> src) & 1u;
-        }
+          // Every lane loads (its table entry names valid variables and its
+          // slot lies inside the padded copy), so the loads issue together.
 #pragma unroll
-        for (int u = 0; u < kLockUnroll; ++u) {
-          if (!on[u]) continue;
-          const int32_t x = (int32_t)(xo[u] & kVarMask);
-          wx[u] = *reinterpret_cast<const u64*>(sg + (int64_t)x * GI);
-          wy[u] = *reinterpret_cast<const u64*>(sg + (int64_t)y[u] * GI);
-          wz[u] = *reinterpret_cast<const u64*>(sg + (int64_t)z[u] * GI);
-        }
-#pragma unroll
-        for (int u = 0; u < kLockUnroll; ++u) {
-          if (!on[u]) continue;
-          const int op = (int)(xo[u] >> kOpShift);
-          const int32_t x = (int32_t)(xo[u] & kVarMask);
-          bool tx, ty2, tz2;
-          if constexpr (D == kBits) {
-            if (!wx[u] || !wy[u] || !wz[u]) { fail = true; continue; }
-            u64 nx, ny, nz;
-            deduce_bits(op, wx[u], wy[u], wz[u], nx, ny, nz);
-            u64* sk64 = reinterpret_cast<u64*>(sk);
-            tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, x, wx[u], nx, chg, fail, nt);
-            ty2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, y[u], wy[u], ny, chg, fail, nt);
-            tz2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, z[u], wz[u], nz, chg, fail, nt);
-          } else {
-            const int2 X = as_itv(wx[u]), Y = as_itv(wy[u]), Z = as_itv(wz[u]);
-            if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = true; continue; }
-            int2 nx, ny, nz;
-            deduce<D == kItvDiv>(op, X, Y, Z, nx, ny, nz);
-            if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
-            tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, x, X, nx, chg, fail, nt);
-            ty2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, y[u], Y, ny, chg, fail, nt);
-            tz2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, z[u], Z, nz, chg, fail, nt);
+          for (int u = 0; u < kLockUnroll; ++u) {
+            const int32_t x = (int32_t)(xo[u] & kVarMask);
+            wx[u] = *reinterpret_cast<const u64*>(sg + (int64_t)x * GI);
+            wy[u] = *reinterpret_cast<const u64*>(sg + (int64_t)y[u] * GI);
+            wz[u] = *reinterpret_cast<const u64*>(sg + (int64_t)z[u] * GI);
           }
-          if (tx) mark(x);
-          if (ty2) mark(y[u]);
-          if (tz2) mark(z[u]);
-          // one propagator after the other: interleaving their arithmetic
-          // would hold every one's temporaries at once
-          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 0; u < kLockUnroll; ++u) {
+            const int op = OPC >= 0 ? OPC : (int)(xo[u] >> kOpShift);
+            const int32_t x = (int32_t)(xo[u] & kVarMask);
+            bool tx = false, ty2 = false, tz2 = false;
+            if constexpr (D == kBits) {
+              u64 nx, ny, nz;
+              if constexpr (kFlat) {
+                deduce_bits(op, wx[u], wy[u], wz[u], nx, ny, nz);
+                const bool empty_in = !wx[u] || !wy[u] || !wz[u];
+                fail |= on[u] && empty_in;
+                const bool tight = on[u] && !empty_in &&
+                                   ((wx[u] & nx) != wx[u] || (wy[u] & ny) != wy[u] || (wz[u] & nz) != wz[u]);
+                if (tight) {
+                  u64* sk64 = reinterpret_cast<u64*>(sk);
+                  tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, x, wx[u], nx, chg, fail, nt);
+                  ty2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, y[u], wy[u], ny, chg, fail, nt);
+                  tz2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, z[u], wz[u], nz, chg, fail, nt);
+                }
+              } else {
+                if (!on[u]) continue;
+                if (!wx[u] || !wy[u] || !wz[u]) { fail = true; continue; }
+                deduce_bits(op, wx[u], wy[u], wz[u], nx, ny, nz);
+                u64* sk64 = reinterpret_cast<u64*>(sk);
+                tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, x, wx[u], nx, chg, fail, nt);
+                ty2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, y[u], wy[u], ny, chg, fail, nt);
+                tz2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, z[u], wz[u], nz, chg, fail, nt);
+              }
+            } else {
+              const int2 X = as_itv(wx[u]), Y = as_itv(wy[u]), Z = as_itv(wz[u]);
+              int2 nx, ny, nz;
+              if constexpr (kFlat) {
+                deduce<D == kItvDiv>(op, X, Y, Z, nx, ny, nz);
+                const bool empty_in = X.x > X.y || Y.x > Y.y || Z.x > Z.y;
+                fail |= on[u] && (empty_in || nx.x > nx.y || ny.x > ny.y || nz.x > nz.y);
+                const bool tight = on[u] && !empty_in &&
+                                   (nx.x > X.x || nx.y < X.y || ny.x > Y.x || ny.y < Y.y ||
+                                    nz.x > Z.x || nz.y < Z.y);
+                if (tight) {
+                  tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, x, X, nx, chg, fail, nt);
+                  ty2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, y[u], Y, ny, chg, fail, nt);
+                  tz2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, z[u], Z, nz, chg, fail, nt);
+                }
+              } else {
+                if (!on[u]) continue;
+                if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = true; continue; }
+                deduce<D == kItvDiv>(op, X, Y, Z, nx, ny, nz);
+                if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
+                tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, x, X, nx, chg, fail, nt);
+                ty2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, y[u], Y, ny, chg, fail, nt);
+                tz2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, z[u], Z, nz, chg, fail, nt);
+              }
+            }
+            if (tx) mark(x);
+            if (ty2) mark(y[u]);
+            if (tz2) mark(z[u]);
+          }
+        }
+      };
+      // One opcode in the whole word (the table is sorted by opcode)?
+      const int op0 = __builtin_amdgcn_readfirstlane((int)(txo >> kOpShift));
+      const bool pin = w * 32 + (lane & 31) < t.n;
+      const bool one_op = !__ballot(lane < 32 && pin && (int)(txo >> kOpShift) != op0);
+      using std::integral_constant;
+      if (!one_op) {
+        rounds(integral_constant<int, -1>{});
+      } else if constexpr (D == kBits) {
+        switch (op0) {
+          case kBLeq: rounds(integral_constant<int, kBLeq>{}); break;
+          case kBAdd: rounds(integral_constant<int, kBAdd>{}); break;
+          case kBNeq: rounds(integral_constant<int, kBNeq>{}); break;
+          case kBEqr: rounds(integral_constant<int, kBEqr>{}); break;
+          default: rounds(integral_constant<int, -1>{}); break;
+        }
+      } else {
+        switch (op0) {
+          case kAdd: rounds(integral_constant<int, kAdd>{}); break;
+          case kMul: rounds(integral_constant<int, kMul>{}); break;
+          case kMin: rounds(integral_constant<int, kMin>{}); break;
+          case kMax: rounds(integral_constant<int, kMax>{}); break;
+          case kEq: rounds(integral_constant<int, kEq>{}); break;
+          case kLeq: rounds(integral_constant<int, kLeq>{}); break;
+          default: rounds(integral_constant<int, -1>{}); break;   // division: per propagator
         }
       }
     }
