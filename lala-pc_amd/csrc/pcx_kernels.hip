@@ -435,6 +435,7 @@ hipError_t launch_sweep_plain(const Table& t, int2* store, const int32_t* idx, i
 // serializes one thread).  Every thread of the workgroup calls this.
 constexpr int kPropCap = 1024;
 constexpr int kMarkedPasses = 16;  // at most, passes over a chunk per marked sweep
+constexpr int kBatchedMarkedPasses = 4;    // the same in batched marked sweeps (short chains)
 constexpr int kMarkedGroupMax = 8;        // marked chunks per batched work item, at most
 constexpr int64_t kMarkedTargetItems = 4096;   // work items a marked sweep keeps, at least
 template <int D>
@@ -1256,7 +1257,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restri
                                               s_props, s_scan, pass > 0, all);
       flush_marks(mk);
       __syncthreads();
-      if (total == 0 || pass + 1 >= kMarkedPasses) break;
+      if (total == 0 || pass + 1 >= kBatchedMarkedPasses) break;
     }
     publish(f, s, chg, fail, nt, tight);
     count_evals(f, ne);
