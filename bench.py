@@ -147,17 +147,18 @@ def cpu_oracle():
 
 
 def bench_batched(args):
-    V, P, n_inst = 50000, 200000, 4096
+    V, P, n_inst = args.nvars, args.nprops, 4096
     t0 = time.time()
     p = pcx.gen_params(seed=args.seed, nvars=V, nprops=P)
     op, x, y, z, lb, ub, w = pcx.gen_network(p)
     pir = pcx.PIR(props_per_wave=args.ppw, timing=1, compact_tables=args.compact,
-                  stream_tables=args.stream, event_driven=args.events, lockstep=args.lockstep)
+                  stream_tables=args.stream, event_driven=args.events, lockstep=args.lockstep,
+                  lds_mode=args.lds)
     pir.tell(op, x, y, z, lb, ub)
     root_lb, root_ub = lb.copy(), ub.copy()
     rr = pir.fixpoint(root_lb, root_ub)
     slb, sub = pcx.gen_splits(args.seed + 1, root_lb, root_ub, n_inst, 1, 20)
-    off, idx = pcx.gen_subsets(args.seed + 2, P, n_inst, 2000, 200000)
+    off, idx = pcx.gen_subsets(args.seed + 2, P, n_inst, min(2000, P), P)
     batch = pcx.Batch(pir, n_inst, V, off, idx)
     batch.load(slb, sub)
     log(f"setup {time.time() - t0:.1f}s: root fixpoint sweeps={rr.sweeps} failed={rr.failed}, "
@@ -200,9 +201,9 @@ def bench_batched(args):
 
     out = {
         "value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
-        "config": {"workload": "batched search-node fixpoints: 4,096 subproblems of one "
-                               "50,000-var / 200,000-propagator network, 1-20 splits each, "
-                               "2,000-200,000 active propagators each",
+        "config": {"workload": f"batched search-node fixpoints: 4,096 subproblems of one "
+                               f"{V:,}-var / {P:,}-propagator network, 1-20 splits each, "
+                               f"{min(2000, P):,}-{P:,} active propagators each",
                    "nvars": V, "nprops": P, "instances": n_inst, "props_per_wave": args.ppw,
                    "mode": {pcx.MODE_BATCHED: "batched", pcx.MODE_PERSISTENT: "persistent"}[mode],
                    "compact_tables": args.compact, "stream_tables": args.stream,
@@ -563,6 +564,12 @@ def main():
     ap.add_argument("--retune", action="store_true", help="ignore a cached tuning")
     ap.add_argument("--events", type=int, default=-1, choices=[-1, 0, 1],
                     help="sweeps after the first: -1 auto, 0 every propagator, 1 marked only")
+    ap.add_argument("--nvars", type=int, default=50000,
+                    help="config 2: variables of the network (the config's own size by default)")
+    ap.add_argument("--nprops", type=int, default=200000,
+                    help="config 2: propagators of the network (the config's own size by default)")
+    ap.add_argument("--lds", type=int, default=-1, choices=[-1, 0, 1],
+                    help="persistent mode: stores staged in LDS (-1 auto, 0 never, 1 when they fit)")
     ap.add_argument("--lockstep", type=int, default=-1, choices=[-1, 0, 16, 32, 64],
                     help="configs 2 and 3, full sweeps: -1 auto, 0 per-instance work items, "
                          "16/32/64 instances per lockstep group")

@@ -76,7 +76,7 @@ typedef struct {
                                from the sweep size                                   */
   int32_t lds_mode;         /* persistent mode: stage each store in LDS; -1 auto
                                (stores up to 32 KB), 0 never, 1 whenever it fits
-                               (stores up to 64 KB)                                */
+                               in a workgroup's LDS (160 KB on MI355X)             */
   int32_t sweeps_per_check; /* plain engine: sweeps queued between host checks (0=8) */
   int64_t max_sweeps;       /* 0 = run to the fixpoint                               */
   int32_t timing;           /* 1 = record HIP events around every sweep kernel       */

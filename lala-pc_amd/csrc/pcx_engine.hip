@@ -49,6 +49,8 @@ bool valid_ppw(int p) {
   return p == 64 || p == 128 || p == 256 || p == 512 || p == 1024 || p == 2048;
 }
 
+constexpr int64_t kLdsAutoBytes = 32 * 1024;   // automatic LDS residency up to this store size
+
 bool valid_lockstep(int g) { return g == -1 || g == 0 || g == 16 || g == 32 || g == 64; }
 
 template <class T>
@@ -632,11 +634,12 @@ pcx_status run_persistent(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
   const Table t = run_table(b);
   const int dev_cus = pcx_internal::device_cus(e->cfg.device);
-  // LDS-resident stores: lds_mode 1 whenever the store fits, -1 (auto) when
-  // it also leaves room for two workgroups per CU.
+  // LDS-resident stores: lds_mode 1 whenever the store fits in a workgroup's
+  // LDS, -1 (auto) up to kLdsAutoBytes (several workgroups per CU).
   const int64_t sbytes = b->nvars * 8;
-  const bool lds = (e->cfg.lds_mode > 0 && sbytes <= kLdsStoreBytes) ||
-                   (e->cfg.lds_mode < 0 && sbytes <= kLdsStoreBytes / 2);
+  const int64_t lds_cap = lds_store_cap(e->cfg.device);
+  const bool lds = (e->cfg.lds_mode > 0 && sbytes <= lds_cap) ||
+                   (e->cfg.lds_mode < 0 && sbytes <= kLdsAutoBytes);
   const int per_cu = persistent_blocks_per_cu(dom_of(e), lds ? sbytes : 0);
   const int grid = (int)std::min<int64_t>((int64_t)dev_cus * per_cu, b->n_inst);
   const Subsets sub = run_subsets(b);
@@ -646,7 +649,7 @@ pcx_status run_persistent(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   if (ts != PCX_OK) return ts;
   PCX_HIP(launch_persistent(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, b->d_order,
                             (int32_t)b->n_inst, b->d_ctl + 3, b->d_tight, maxs, dom_of(e), grid,
-                            lds, e->stream));
+                            lds, lds_cap, e->stream));
   ts = tm.end(e->stream, true);
   if (ts != PCX_OK) return ts;
   PCX_HIP(hipStreamSynchronize(e->stream));
@@ -1084,7 +1087,7 @@ pcx_status pcx_batch_autotune(pcx_batch* b, int32_t reps, pcx_tuning* best, pcx_
   if (b->n_inst >= 2)
     for (int gi : {16, 32, 64})
       pts.push_back(pcx_tuning{PCX_MODE_BATCHED, 512, 0, 0, 0, 1, gi, 0.0});
-  const bool lds_fits = b->nvars * 8 <= kLdsStoreBytes;
+  const bool lds_fits = b->nvars * 8 <= lds_store_cap(e->cfg.device);
   for (int l = 0; l <= (lds_fits ? 1 : 0); ++l)
     for (int c = 0; c <= (subsets ? 1 : 0); ++c)
       pts.push_back(pcx_tuning{PCX_MODE_PERSISTENT, 0, l, c, 0, 0, 0, 0.0});

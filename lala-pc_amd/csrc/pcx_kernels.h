@@ -167,12 +167,18 @@ hipError_t launch_sweep_lockstep(const Table& t, int2* store, int2* shadow, int6
                                  InstFlags* flags, WorkList wl, unsigned long long* tight, int gi,
                                  int dom, int grid, hipStream_t st, const EventSet* ev = nullptr);
 constexpr int kMarkedWordsPerItem = kChunkWords;   // bitmap words per marked-sweep item
-// lds: stage each instance's store in LDS (nvars * 8 <= kLdsStoreBytes).
+// lds: stage each instance's store in LDS (nvars * 8 <= lds_cap, at most
+// lds_store_cap(device): the device's LDS per workgroup, 160 KB on gfx950,
+// less a reserve for the kernel's static variables; kLdsStoreBytes when the
+// runtime does not report it).
 constexpr int64_t kLdsStoreBytes = 64 * 1024;
+constexpr int64_t kLdsStaticReserve = 256;
+int64_t lds_store_cap(int device);
 hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                              int64_t n_props, InstFlags* flags, const int32_t* order,
                              int32_t n_inst, int32_t* queue, unsigned long long* tight,
-                             int32_t max_sweeps, int dom, int grid, bool lds, hipStream_t st);
+                             int32_t max_sweeps, int dom, int grid, bool lds, int64_t lds_cap,
+                             hipStream_t st);
 // resident workgroups per CU (lds_bytes of dynamic LDS per workgroup)
 int persistent_blocks_per_cu(int dom, int64_t lds_bytes);
 // empty (may be nullptr): per-instance bytes, 1 = the store starts empty

@@ -1875,15 +1875,33 @@ int persistent_blocks_per_cu(int dom, int64_t lds_bytes) {
   return (e == hipSuccess && n > 0) ? n : 1;
 }
 
+int64_t lds_store_cap(int device) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess ||
+      v <= kLdsStaticReserve)
+    return kLdsStoreBytes;
+  return (int64_t)v - kLdsStaticReserve;
+}
+
 hipError_t launch_persistent(const Table& t, int2* store, int64_t nvars, const Subsets& sub,
                              int64_t n_props, InstFlags* flags, const int32_t* order,
                              int32_t n_inst, int32_t* queue, unsigned long long* tight,
-                             int32_t max_sweeps, int dom, int grid, bool lds, hipStream_t stream) {
-  if (lds && nvars * 8 > kLdsStoreBytes) return hipErrorInvalidValue;
+                             int32_t max_sweeps, int dom, int grid, bool lds, int64_t lds_cap,
+                             hipStream_t stream) {
+  if (lds && nvars * 8 > lds_cap) return hipErrorInvalidValue;
   const size_t sm = lds ? (size_t)nvars * 8 : 0;
-#define PCX_PERSIST(D, L) \
-  k_persistent<D, L><<<grid, kPBlock, sm, stream>>>(t, store, nvars, sub, n_props, flags, order, \
-                                                    n_inst, queue, tight, max_sweeps)
+  // Dynamic LDS beyond 64 KB (gfx950 has 160 KB per workgroup) is requested
+  // from the runtime first.
+#define PCX_PERSIST(D, L)                                                                        \
+  do {                                                                                         \
+    if (sm > 65536) {                                                                          \
+      const hipError_t fe = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_persistent<D, L>), \
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); \
+      if (fe != hipSuccess) return fe;                                                         \
+    }                                                                                          \
+    k_persistent<D, L><<<grid, kPBlock, sm, stream>>>(t, store, nvars, sub, n_props, flags, order, \
+                                                      n_inst, queue, tight, max_sweeps);       \
+  } while (0)
   switch (dom) {
     case kItv: if (lds) PCX_PERSIST(kItv, true); else PCX_PERSIST(kItv, false); break;
     case kItvDiv: if (lds) PCX_PERSIST(kItvDiv, true); else PCX_PERSIST(kItvDiv, false); break;

@@ -407,6 +407,27 @@ def test_persistent_lds_resident_stores(gpu, lds):
     _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"persistent lds_mode {lds}")
 
 
+@pytest.mark.parametrize("V", [12000, 19000])
+def test_persistent_lds_stores_beyond_64k(gpu, V):
+    """Persistent mode with stores of 96 KB and 152 KB staged in LDS: beyond
+    the 64 KB of a workgroup on earlier CDNA, within gfx950's 160 KB."""
+    P, n_inst = 4 * V, 48
+    op, x, y, z, lb, ub, w = _network(25, V, P)
+    pir = pcx.PIR(lds_mode=1)
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_lb, root_ub).failed
+    slb, sub = pcx.gen_splits(26, root_lb, root_ub, n_inst, 1, 20)
+    top, tx, ty, tz = pir.table()
+    lb_o, ub_o = slb.copy(), sub.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lb_o, ub_o, V, threads=8)
+    b = pcx.Batch(pir, n_inst, V)
+    b.load(slb, sub)
+    res_g, _ = b.run(pcx.MODE_PERSISTENT)
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"persistent LDS store of {V * 8} bytes")
+
+
 def test_autotuner_choice_is_bit_exact(gpu, tmp_path):
     """The autotuner tries every sweep configuration on the loaded batch and
     leaves the engine configured with the fastest; results stay bit-exact and
