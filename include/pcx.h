@@ -74,9 +74,9 @@ typedef struct {
   int32_t props_per_wave;   /* 64 .. 512 (powers of two; batched mode also 1024 and
                                2048: work items of 2 or 4 rounds of 512); 0 = chosen
                                from the sweep size                                   */
-  int32_t lds_mode;         /* persistent mode: stage each store in LDS; -1 auto
-                               (stores up to 32 KB), 0 never, 1 whenever it fits
-                               in a workgroup's LDS (160 KB on MI355X)             */
+  int32_t lds_mode;         /* persistent mode: stage each store in LDS; 0 never,
+                               1 or -1 (auto) whenever it fits in a workgroup's
+                               LDS (160 KB on MI355X)                              */
   int32_t sweeps_per_check; /* plain engine: sweeps queued between host checks (0=8) */
   int64_t max_sweeps;       /* 0 = run to the fixpoint                               */
   int32_t timing;           /* 1 = record HIP events around every sweep kernel       */
