@@ -1773,7 +1773,7 @@ hipError_t launch_burst_slots(const SlotDesc* slots, InstFlags* flags, int32_t n
 // termination argument needs.
 constexpr int kPBlock = 512;     // 8 waves per workgroup
 
-// LDS: the instance's store (nvars 8-byte entries, at most kLdsStoreBytes)
+// LDS: the instance's store (nvars 8-byte entries, at most lds_store_cap())
 // is copied into the workgroup's LDS, swept there with LDS atomics, and
 // written back when the instance is done.
 template <int D, bool LDS>

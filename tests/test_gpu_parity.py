@@ -325,6 +325,40 @@ def test_lockstep_dense_first_sweep(gpu, gi, events):
     assert st.full_tightened > n_inst * 1000
 
 
+@pytest.mark.parametrize("gi", [16, 64])
+def test_lockstep_all_opcodes_mixed(gpu, gi):
+    """A random table over all ten opcodes, divisions included (words of
+    mixed opcodes take the per-propagator path, words of one opcode the
+    specialized one), on 200 random stores with bounds in [-30, 30]."""
+    rng = np.random.default_rng(71)
+    V, P, n_inst = 60, 400, 200
+    op = rng.integers(0, len(ol.OPS), P).astype(np.int32)
+    x = rng.integers(0, V, P).astype(np.int32)
+    y = rng.integers(0, V, P).astype(np.int32)
+    z = rng.integers(0, V, P).astype(np.int32)
+    a = rng.integers(-30, 31, (n_inst, V))
+    b = rng.integers(-30, 31, (n_inst, V))
+    lb = np.minimum(a, b).astype(np.int32).ravel()
+    ub = np.maximum(a, b).astype(np.int32).ravel()
+    pir = pcx.PIR(lockstep=gi)
+    lb1, ub1 = lb[:V].copy(), ub[:V].copy()
+    pir.tell(op, x, y, z, lb1, ub1)           # embeds reified outputs in [0, 1] (first store)
+    top, tx, ty, tz = pir.table()
+    for k in range(n_inst):                   # the same embedding in every store
+        s = slice(k * V, (k + 1) * V)
+        r = top >= ol.OPS["EQ"]
+        lb[s][tx[r]] = np.maximum(lb[s][tx[r]], 0)
+        ub[s][tx[r]] = np.minimum(ub[s][tx[r]], 1)
+    lb_o, ub_o = lb.copy(), ub.copy()
+    res_o = ol.fixpoint_batch(top, tx, ty, tz, lb_o, ub_o, V, threads=8)
+    b = pcx.Batch(pir, n_inst, V)
+    b.load(lb, ub)
+    res_g, st = b.run(pcx.MODE_BATCHED)
+    assert st.lockstep == gi
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, V, f"all opcodes, lockstep {gi}")
+
+
 @pytest.mark.parametrize("name", list(ol.OPS))
 @pytest.mark.parametrize("gi", [32, 64])
 def test_lockstep_every_box(gpu, name, gi):
