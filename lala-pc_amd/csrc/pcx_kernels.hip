@@ -945,7 +945,7 @@ hipError_t launch_interleave(const int2* store, int2* shadow, int64_t nvars, int
 // does not fit marks its whole instance for the next sweep instead (its
 // all_marked word).
 constexpr int kLockMarkCap = 1024;
-constexpr int kLockWordsPerWave = 8;       // 32-propagator table words per wave per item
+constexpr int kLockWordsPerWave = 4;       // 32-propagator table words per wave per item
 constexpr int kLockUnroll = 2;             // propagators whose operands are loaded together
 
 template <int D, int GI>
