@@ -437,7 +437,7 @@ constexpr int kPropCap = 1024;
 constexpr int kMarkedPasses = 16;  // at most, passes over a chunk per marked sweep
 constexpr int kBatchedMarkedPasses = 4;    // the same in batched marked sweeps (short chains)
 constexpr int kMarkedGroupMax = 8;        // marked chunks per batched work item, at most
-constexpr int64_t kMarkedTargetItems = 4096;   // work items a marked sweep keeps, at least
+constexpr int64_t kMarkedTargetItems = 2048;   // work items a marked sweep keeps, at least
 template <int D>
 // `fresh`: read the words with device-scope atomics (a pass over a chunk
 // this workgroup already swept in this launch, whose lines the CU's L1 may
