@@ -139,3 +139,37 @@ def test_result_record_layout_matches_c():
     assert pcx.RESULT_DTYPE.itemsize == ctypes.sizeof(pcx.Result)
     for name, _ in pcx.Result._fields_:
         assert pcx.RESULT_DTYPE.fields[name][1] == getattr(pcx.Result, name).offset
+
+
+# C struct <-> ctypes mirror: every field's offset and every struct's size.
+_MIRRORS = {"pcx_config": "Config", "pcx_result": "Result", "pcx_stats": "Stats",
+            "pcx_tuning": "Tuning", "pcx_server_config": "ServerConfig",
+            "pcx_served": "Served", "pcx_server_stats": "ServerStats", "pcxg_params": "GenParams"}
+
+
+def test_struct_layouts_match_c(tmp_path):
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pcx.h"', '#include "pcx_gen.h"',
+             "int main(void) {"]
+    for cname, pyname in _MIRRORS.items():
+        cls = getattr(pcx, pyname)
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:-1]): int(l.split()[-1]) for l in out if l.strip()}
+    for cname, pyname in _MIRRORS.items():
+        cls = getattr(pcx, pyname)
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, f"{cname}.{f}"
