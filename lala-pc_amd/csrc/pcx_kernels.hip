@@ -948,8 +948,16 @@ constexpr int kLockMarkCap = 1024;
 constexpr int kLockWordsPerWave = 4;       // 32-propagator table words per wave per item
 constexpr int kLockUnroll = 2;             // propagators whose operands are loaded together
 
+// Waves per SIMD the lockstep sweep is compiled for.  It waits on memory
+// for about half its cycles, so occupancy pays even at the price of a few
+// spilled registers in its cold paths: measured best at 6 for intervals
+// (config 2: 2.75 -> 2.51 ms) and 7 for bitsets (config 3: 7.5 -> 6.4 ms).
+// The division variant's registers do not fit such a bound.
+template <int D>
+constexpr int lock_min_waves() { return D == kItv ? 6 : (D == kBits ? 7 : 1); }
+
 template <int D, int GI>
-__global__ __launch_bounds__(kBlock) void k_sweep_lockstep(Table t, int2* __restrict__ store,
+__global__ __launch_bounds__(kBlock, lock_min_waves<D>()) void k_sweep_lockstep(Table t, int2* __restrict__ store,
                                                           const int2* __restrict__ shadow, int64_t nvars,
                                                           int32_t n_inst,
                                                           const uint32_t* __restrict__ member,
