@@ -1200,8 +1200,13 @@ hipError_t launch_sweep_lockstep(const Table& t, int2* store, int2* shadow, int6
 
 // Marked-only sweep over every running instance: items are (instance,
 // kMarkedWordsPerItem bitmap words), claimed per XCD as in the full sweep.
+// Waves per SIMD of the batched marked sweep: 6 for intervals (config 2's
+// marked sweeps 0.98 -> 0.93 ms); the other variants keep their default.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_sweep_marked(Table t, int2* __restrict__ store,
+constexpr int marked_min_waves() { return D == kItv ? 6 : 1; }
+
+template <int D>
+__global__ __launch_bounds__(kBlock, marked_min_waves<D>()) void k_sweep_marked(Table t, int2* __restrict__ store,
                                                          int64_t nvars, InstFlags* flags,
                                                          WorkList wl, unsigned long long* tight,
                                                          EventSet ev) {
