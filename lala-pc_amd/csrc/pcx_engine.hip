@@ -936,8 +936,7 @@ pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub) {
 pcx_status pcx_batch_reset(pcx_batch* b) {
   if (!b) return fail_with(PCX_EINVAL, "NULL argument");
   pcx_engine* e = b->e;
-  PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, b->n_inst * b->nvars * sizeof(int2),
-                         hipMemcpyDeviceToDevice, e->stream));
+  PCX_HIP(launch_copy_words(b->d_image, b->d_store, b->n_inst * b->nvars, e->stream));
   b->store_is_image = true;
   return PCX_OK;
 }

@@ -216,6 +216,8 @@ hipError_t launch_branch_seed(const EventSet& ev, const int32_t* var, int64_t n_
 hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
                       hipStream_t st);
 hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n, hipStream_t st);
+// device copy of n 8-byte words (16-byte aligned buffers)
+hipError_t launch_copy_words(const void* src, void* dst, int64_t n, hipStream_t st);
 hipError_t launch_store_unpack(const int2* store, int32_t* lb, int32_t* ub, int64_t n, hipStream_t st);
 hipError_t launch_store_check_empty(const int2* store, int64_t nvars, int32_t n_inst,
                                     InstFlags* flags, int dom, hipStream_t st);

@@ -2258,6 +2258,36 @@ __global__ void k_unpack(const int2* __restrict__ st, int32_t* __restrict__ lb,
   }
 }
 
+// Device copy of n 8-byte words (batch reset: stores <- image): one
+// non-temporal 16-byte load and store per thread per step, a large grid.
+// Measured on the 1.64 GB config-2 reset: 562 us, against 657 us for
+// hipMemcpyAsync (and 600-750 us with 2-8 loads in flight per thread).
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy_words(const v4u* __restrict__ src,
+                                                    v4u* __restrict__ dst, int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+hipError_t launch_copy_words(const void* src, void* dst, int64_t n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  const int64_t n16 = n / 2;
+  if (n16 > 0) {
+    int64_t grid = (n16 + 255) / 256;
+    if (grid > 131072) grid = 131072;
+    k_copy_words<<<dim3((unsigned)grid), 256, 0, stream>>>(static_cast<const v4u*>(src),
+                                                           static_cast<v4u*>(dst), n16);
+  }
+  if (n & 1) {                    // odd word count: the last word
+    const hipError_t e = hipMemcpyAsync(static_cast<uint64_t*>(dst) + n - 1,
+                                        static_cast<const uint64_t*>(src) + n - 1, 8,
+                                        hipMemcpyDeviceToDevice, stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n,
                              hipStream_t stream) {
   int64_t grid = (n + 255) / 256;
