@@ -385,6 +385,31 @@ def test_lockstep_every_box(gpu, name, gi):
     _compare(lb_g, ub_g, res_g, lb_o, ub_o, res_o, 3, f"{name} lockstep {gi}")
 
 
+def test_reset_restores_the_loaded_stores(gpu):
+    """pcx_batch_reset copies the image back over the stores, here with an odd
+    number of 8-byte words (3 instances x 5 variables: the copy's last word
+    is its tail), after a run that tightened them."""
+    op = np.array([pcx.ADD, pcx.LEQ], np.int32)
+    x, y, z = np.array([0, 4], np.int32), np.array([1, 2], np.int32), np.array([2, 3], np.int32)
+    lb = np.array([0, 1, 2, 0, 0] * 3, np.int32)
+    ub = np.array([9, 3, 4, 9, 1] * 3, np.int32)
+    lb[5 + 2], ub[5 + 2] = 4, 4            # instance 1: z fixed
+    pir = pcx.PIR()
+    pir.tell(op, x, y, z)
+    b = pcx.Batch(pir, 3, 5)
+    b.load(lb, ub)
+    res1, _ = b.run(pcx.MODE_BATCHED)
+    lb1, ub1 = b.read()
+    assert (lb1 != lb).any() or (ub1 != ub).any()      # the run tightened something
+    b.reset()
+    lb0, ub0 = b.read()
+    assert (lb0 == lb).all() and (ub0 == ub).all()
+    res2, _ = b.run(pcx.MODE_BATCHED)
+    lb2, ub2 = b.read()
+    assert (lb2 == lb1).all() and (ub2 == ub1).all()
+    assert [r.failed for r in res1] == [r.failed for r in res2]
+
+
 def test_edge_cases(gpu):
     pir = pcx.PIR()
     # empty table: fixpoint is the store itself, one quiet sweep
