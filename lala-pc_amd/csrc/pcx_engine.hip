@@ -194,15 +194,36 @@ struct SweepTimer {
   }
 };
 
+// Readbacks go through pinned host memory: a copy into pageable memory is
+// staged by the runtime and cost ~70 us per call here.
+constexpr size_t kPinScratch = 256;
+size_t pinned_bytes(const pcx_batch* b) {
+  return kPinScratch + (size_t)b->n_inst * sizeof(InstFlags) + 2 * kTightSlots * sizeof(unsigned long long);
+}
+pcx_status ensure_pinned(pcx_batch* b) {
+  if (!b->h_pin) PCX_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_pin), pinned_bytes(b)));
+  return PCX_OK;
+}
+// Copy `bytes` of device memory into the pinned scratch and wait for it.
+pcx_status read_small(pcx_batch* b, const void* d, size_t bytes, void* out) {
+  PCX_HIP(hipMemcpyAsync(b->h_pin, d, bytes, hipMemcpyDeviceToHost, b->e->stream));
+  PCX_HIP(hipStreamSynchronize(b->e->stream));
+  std::memcpy(out, b->h_pin, bytes);
+  return PCX_OK;
+}
+
 pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* results,
                    pcx_stats* stats, const SweepTimer& tm) {
   pcx_engine* e = b->e;
   flags.resize(b->n_inst);
-  PCX_HIP(hipMemcpyAsync(flags.data(), b->d_flags, b->n_inst * sizeof(InstFlags),
-                         hipMemcpyDeviceToHost, e->stream));
+  const size_t fbytes = b->n_inst * sizeof(InstFlags);
   unsigned long long tight[2 * kTightSlots];    // [0, kTightSlots): full sweeps
-  PCX_HIP(hipMemcpyAsync(tight, b->d_tight, sizeof(tight), hipMemcpyDeviceToHost, e->stream));
+  PCX_HIP(hipMemcpyAsync(b->h_pin + kPinScratch, b->d_flags, fbytes, hipMemcpyDeviceToHost, e->stream));
+  PCX_HIP(hipMemcpyAsync(b->h_pin + kPinScratch + fbytes, b->d_tight, sizeof(tight),
+                         hipMemcpyDeviceToHost, e->stream));
   PCX_HIP(hipStreamSynchronize(e->stream));
+  std::memcpy(flags.data(), b->h_pin + kPinScratch, fbytes);
+  std::memcpy(tight, b->h_pin + kPinScratch + fbytes, sizeof(tight));
   int64_t evals = 0, smax = 0, full_evals = 0;
   for (int64_t k = 0; k < b->n_inst; ++k) {
     const InstFlags& f = flags[k];
@@ -410,7 +431,9 @@ pcx_status prepare_run(pcx_batch* b, int32_t mode) {
   }
   b->run_from_image = b->store_is_image;
   b->store_is_image = false;
-  pcx_status st = prepare_tables(b);
+  pcx_status st = ensure_pinned(b);
+  if (st != PCX_OK) return st;
+  st = prepare_tables(b);
   if (st != PCX_OK) return st;
   return prepare_events(b, mode);
 }
@@ -463,9 +486,9 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
         pcx_status ts = tm.end(e->stream, false, n_marked);
         if (ts != PCX_OK) return ts;
       }
-      PCX_HIP(hipMemcpyAsync(&hf, f, sizeof(hf), hipMemcpyDeviceToHost, e->stream));
-      PCX_HIP(hipStreamSynchronize(e->stream));
-      pcx_status ts = tm.drain();
+      pcx_status ts = read_small(b, f, sizeof(hf), &hf);
+      if (ts != PCX_OK) return ts;
+      ts = tm.drain();
       if (ts != PCX_OK) return ts;
       const int32_t last = std::max(hf.changed[0], hf.changed[1]);
       if (hf.failed_at != kNoFail || last + 1 < s) break;       // quiet sweep already ran
@@ -564,8 +587,8 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       ts = tm.end(e->stream, false, group);
       if (ts != PCX_OK) return ts;
       sweeps += group;
-      PCX_HIP(hipMemcpyAsync(&live, b->d_ctl + 2, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
-      PCX_HIP(hipStreamSynchronize(e->stream));
+      ts = read_small(b, b->d_ctl + 2, sizeof(int32_t), &live);
+      if (ts != PCX_OK) return ts;
       ts = tm.drain();
       if (ts != PCX_OK) return ts;
       continue;
@@ -617,9 +640,9 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       pcx_status ts = tm.end(e->stream, false, n_marked);
       if (ts != PCX_OK) return ts;
     }
-    PCX_HIP(hipMemcpyAsync(&live, b->d_ctl + 2, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
-    PCX_HIP(hipStreamSynchronize(e->stream));
-    pcx_status ts = tm.drain();
+    pcx_status ts = read_small(b, b->d_ctl + 2, sizeof(int32_t), &live);
+    if (ts != PCX_OK) return ts;
+    ts = tm.drain();
     if (ts != PCX_OK) return ts;
   }
   std::vector<InstFlags> flags;
@@ -899,6 +922,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl); dfree(b->d_order);
   dfree(b->d_cxo); dfree(b->d_cy); dfree(b->d_cz); dfree(b->d_member); dfree(b->d_dirty);
   dfree(b->d_summary); dfree(b->d_image_empty); dfree(b->d_unpack); dfree(b->d_shadow);
+  if (b->h_pin) (void)hipHostFree(b->h_pin);
   if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
   delete b;
 }

@@ -74,6 +74,10 @@ struct pcx_batch {
   int32_t* d_ctl = nullptr;           // kCtlWords control words (see WorkList)
   int32_t* d_order = nullptr;         // instances by decreasing size (persistent queue)
   std::vector<hipEvent_t> ev;         // timing pool
+  // pinned host memory for the readbacks of a run: kPinScratch bytes of small
+  // values (a flags record, the live count), then the flags of every
+  // instance, then the tightening counters
+  uint8_t* h_pin = nullptr;
 };
 
 namespace pcx_internal {
