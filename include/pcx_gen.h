@@ -74,8 +74,17 @@ pcx_status pcxg_splits(uint64_t seed, const int32_t* lb, const int32_t* ub, int6
 pcx_status pcxg_subsets(uint64_t seed, int64_t nprops, int64_t n_inst, int64_t mmin, int64_t mmax,
                         int64_t* off, int32_t* idx);
 
-/* Bitset network: nvars words, each value in [0,63] present with a density
- * drawn per variable from U[dens_lo, dens_hi] (never empty); nprops
+/* Values of generated bitset domains: [0, PCXG_BITS_VALUES).  These are the
+ * values NBitset<64> of the reference's bitset store holds exactly
+ * (tests/pc_bitset_test.cpp:23-25): its first and last bits stand for all
+ * negative values and for every value past 61 (pc_bitset_test.cpp:150-157,
+ * `var -15..5` is NBit(-1, 5)).  Propagation only removes values, so every
+ * store of a generated workload stays inside [0, 61], where a plain word and
+ * NBitset<64> describe the same set. */
+#define PCXG_BITS_VALUES 62
+
+/* Bitset network: nvars words, each value in [0, PCXG_BITS_VALUES) present
+ * with a density drawn per variable from U[dens_lo, dens_hi] (never empty); nprops
  * propagators with opcodes drawn uniformly from `kinds` (bit k = opcode k of
  * the bitset domain, 0 = all four), operands distinct.
  *   planted = 0: operands uniform (a random network; it usually fails);

@@ -435,26 +435,26 @@ pcx_status pcxg_bits_network(uint64_t seed, int64_t nvars, int64_t nprops, uint3
   for (int64_t v = 0; v < nvars; ++v) {
     const double d = dens_lo + (dens_hi - dens_lo) * r.unit();
     uint64_t word = 0;
-    for (int b = 0; b < 64; ++b)
+    for (int b = 0; b < PCXG_BITS_VALUES; ++b)
       if (r.unit() < d) word |= 1ull << b;
-    w[v] = (int32_t)r.below(64);
+    w[v] = (int32_t)r.below(PCXG_BITS_VALUES);
     if (planted) word |= 1ull << w[v];
     if (!word) word = 1ull << w[v];
     words[v] = word;
   }
   if (witness) std::copy(w.begin(), w.end(), witness);
   // value -> variables holding it as hidden value
-  std::vector<int64_t> off(65, 0);
+  std::vector<int64_t> off(PCXG_BITS_VALUES + 1, 0);
   std::vector<int32_t> by;
   if (planted) {
     for (int64_t v = 0; v < nvars; ++v) off[w[v] + 1]++;
-    for (int i = 0; i < 64; ++i) off[i + 1] += off[i];
+    for (int i = 0; i < PCXG_BITS_VALUES; ++i) off[i + 1] += off[i];
     by.resize(nvars);
     std::vector<int64_t> fill(off.begin(), off.end() - 1);
     for (int64_t v = 0; v < nvars; ++v) by[fill[w[v]]++] = (int32_t)v;
   }
   auto holding = [&](int64_t val, int32_t a, int32_t b) -> int32_t {
-    if (val < 0 || val > 63 || off[val + 1] == off[val]) return -1;
+    if (val < 0 || val >= PCXG_BITS_VALUES || off[val + 1] == off[val]) return -1;
     for (int t = 0; t < 8; ++t) {
       const int32_t v = by[off[val] + (int64_t)r.below((uint64_t)(off[val + 1] - off[val]))];
       if (v != a && v != b) return v;

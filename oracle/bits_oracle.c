@@ -7,11 +7,18 @@
  * never by the product library.
  *
  * Domains are 64-bit words, bit v <=> value v in [0, 63]; a variable is empty
- * (failed) when its word is 0.  The reference's NBitset<64> maps values outside
- * its range to sentinel buckets (pc_bitset_test.cpp:153-156); this restatement
- * uses plain [0,63] words and drops out-of-range results, so its set
- * arithmetic is "parity unpinned" against the reference at the edges (no
- * reference test exercises NBitset add/leq/eq-reif).
+ * (failed) when its word is 0.  The reference's NBitset<64> holds the values
+ * [0, 61] exactly and keeps one bucket bit for all negative values and one for
+ * every value past 61 (pc_bitset_test.cpp:150-157: `var -15..5` is
+ * NBit(-1, 5)).  This restatement uses plain words and drops results outside
+ * [0, 63].  The two agree on stores inside [0, 61], and every generated
+ * workload stays there (pcxg_bits_network / pcxg_bits_splits draw values below
+ * PCXG_BITS_VALUES = 62 and propagation only removes values).
+ * Inside that range LEQ, NEQ and EQR follow formula.hpp (lines cited below)
+ * and are pinned by the pc_bitset_test.cpp cases and the brute-force checks.
+ * ADD is "parity unpinned": its projections are NBitset::project(ADD / SUB)
+ * of lala-core v1.2.8 (not in the reference tree), which no reference test
+ * exercises; this restatement takes the exact set sum / difference.
  *
  * Propagators (restated from include/lala/formula.hpp and terms.hpp):
  *   LEQ  y <= z          Inequality<false>::deduce_impl  formula.hpp:796-805

@@ -170,3 +170,26 @@ def test_bits_branching(gpu):
     wo = cw.reshape(-1).copy()
     ro = ol.bits_fixpoint_batch(top, tx, ty, tz, wo, V, threads=8)
     _compare(child.read_words(), rg, wo, ro, V, "bits children")
+
+
+def test_bits_full_size_batch_of_1024(gpu):
+    """The bitset workload at its full size: 1,024 bit-removal subproblems of
+    one 200,000-word / 800,000-propagator network (values inside [0, 61]),
+    run batched with the automatic lockstep full sweep; every instance equals
+    the oracle."""
+    V, P, n = 200000, 800000, 1024
+    op, x, y, z, w = pcx.gen_bits_network(1, V, P)
+    pir = pcx.BitsPIR()
+    pir.tell(op, x, y, z, w)
+    root = w.copy()
+    assert not pir.fixpoint(root).failed
+    inst = pcx.gen_bits_splits(2, root, n, 1, 20)
+    assert not (inst & np.uint64((1 << 62) | (1 << 63))).any()
+    b = pcx.Batch(pir, n, V)
+    b.load_words(inst)
+    rg, st = b.run(pcx.MODE_BATCHED)
+    wg = b.read_words()
+    b.close()
+    top, tx, ty, tz = pir.table()
+    ro = ol.bits_fixpoint_batch(top, tx, ty, tz, inst, V, threads=16)
+    _compare(wg, rg, inst, ro, V, "bits 1,024 x 200k/800k")

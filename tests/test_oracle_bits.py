@@ -149,3 +149,34 @@ def test_faithful_mode_difference_count():
         assert r2.failed or r2.has_changed, f"seed {seed}: an early stop that is a fixpoint"
         assert bool(r2.failed) == bool(rt.failed) and (rt.failed or np.array_equal(again, wt))
     assert differ == 46, f"{differ} of 200 networks differ"
+
+
+BUCKET_BITS = np.uint64((1 << 62) | (1 << 63))
+
+
+@pytest.mark.parametrize("seed,V,P", [(1, 5000, 20000), (3, 200000, 800000), (61, 4000, 16000)])
+def test_generated_bitset_domains_stay_inside_nbitset_exact_range(seed, V, P):
+    """Generated bitset stores, their splits and their fixpoints hold values
+    in [0, 61] only: the values NBitset<64> represents exactly (its bits for
+    negative values and for values past 61 are buckets,
+    tests/pc_bitset_test.cpp:150-157).  Inside that range the plain words of
+    the oracle and NBitset describe the same sets."""
+    import pcx
+    op, x, y, z, w = pcx.gen_bits_network(seed, V, P)
+    assert not (w & BUCKET_BITS).any()
+    assert (w != 0).all()
+    inst = pcx.gen_bits_splits(seed + 1, w, 8, 1, 20)
+    assert not (inst & BUCKET_BITS).any()
+    if V <= 5000:
+        wo = w.copy()
+        ol.bits_tell_embed(op, x, wo)
+        ol.bits_sort(op, x, y, z)
+        ol.bits_fixpoint(op, x, y, z, wo)
+        assert not (wo & BUCKET_BITS).any()
+
+
+def test_planted_bitset_witness_inside_range():
+    import pcx
+    op, x, y, z, w, wit = pcx.gen_bits_network(7, 3000, 12000, witness=True)
+    assert wit.min() >= 0 and wit.max() <= 61
+    assert all((int(w[v]) >> int(wit[v])) & 1 for v in range(len(w)))
