@@ -64,9 +64,19 @@ struct pcx_batch {
   bool events = false;                // this run uses marked-only sweeps
   bool seeded = false;                // the next run may start from seeded marks
   bool seeded_run = false;            // this run starts with a marked sweep
-  // captured group of marked sweeps (batched mode), rebuilt when its key changes
+  // captured group of marked sweeps (batched mode), rebuilt when any launch
+  // argument it was captured with differs
+  struct MarkedGraphArgs {
+    uint64_t version;                 // engine table version
+    int32_t group, dom, max_sweeps, ppw;
+    const void *xo, *y, *z, *adj_off, *adj;
+    bool operator==(const MarkedGraphArgs& o) const {
+      return version == o.version && group == o.group && dom == o.dom && max_sweeps == o.max_sweeps &&
+             ppw == o.ppw && xo == o.xo && y == o.y && z == o.z && adj_off == o.adj_off && adj == o.adj;
+    }
+  };
   hipGraphExec_t marked_graph = nullptr;
-  uint64_t marked_graph_key = 0;
+  MarkedGraphArgs marked_graph_args{};
   pcx::InstFlags* d_flags = nullptr;
   unsigned long long* d_tight = nullptr;
   int32_t* d_act = nullptr;

@@ -200,7 +200,7 @@ pcx_status admit(pcx_server* s) {
     for (int32_t i = 0; i < s->n_slots; ++i)
       if (!s->slots[i].busy) { k = i; break; }
     if (k < 0) { ++s->stats.waits_slot; break; }
-    if (staged + rq.nvars > s->stage_words) break;
+    if (staged + rq.nvars > s->stage_words) { ++s->stats.waits_stage_admit; break; }
     const Footprint fp = footprint(s, rq);
     const int64_t off = s->arena.alloc(fp.units);
     if (off < 0) { ++s->stats.waits_arena; break; }   // no HBM room: wait for a slot to drain
@@ -323,7 +323,10 @@ pcx_status harvest(pcx_server* s) {
     const bool failed = f.failed_at != kNoFail;
     const bool quiet = last < s->next_sweep - 1;     // the latest sweep left it unchanged
     if (!failed && !quiet) continue;
-    if (staged + sl.req.nvars > s->stage_words) break;   // the rest next tick
+    if (staged + sl.req.nvars > s->stage_words) {        // the rest next tick
+      ++s->stats.waits_stage_harvest;
+      break;
+    }
     done.push_back(k);
     staged += sl.req.nvars;
   }
@@ -368,8 +371,10 @@ pcx_status side_collect(pcx_server* s) {
           s->h_flags[k] = f;                     // the flags its result is read from
           done.push_back(k);
           staged += sl.req.nvars;
-        }
-        continue;                                // a full staging buffer: next tick
+        } else {
+          ++s->stats.waits_stage_side;           // a full staging buffer: the lane's next
+        }                                        // launch finds it settled again
+        continue;
       }
       if (f.done_at != kSideBail) continue;      // budget spent: next launch
       const SlotDesc& sd = s->h_slots[k];

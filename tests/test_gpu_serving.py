@@ -311,3 +311,51 @@ def _serve_climbers(pirs, reqs, sel, events, burst_sweeps, burst_after, spt=2):
             got[tag] = r
         assert srv.stats().ticks < 10000000, "server made no progress"
     return got, srv.stats(), stores
+
+
+@pytest.mark.parametrize("burst", [(-1, 8), (0, 0)], ids=["side-lanes", "batch-only"])
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
+def test_staging_buffer_holds_one_store(climbers, events, burst):
+    """Regression test for the pinned staging buffer (stage_bytes), the
+    buffer every store passes through on its way to and from the device.  It
+    holds one and a half of these 250-variable stores, so admissions,
+    hand-backs from the slot batch and hand-backs from the side lanes each
+    find it full and defer the rest to later ticks.
+
+    Eight requests share one store; admitted one per tick (one sweep each),
+    they start on eight consecutive sweeps and need the same ~hundreds of
+    sweeps, so with 16 sweeps per tick several settle in one tick (slot
+    batch), or are handed to one side lane together and settle in one launch.
+    Four requests with other caps follow.  Every result equals the oracle's
+    and the request alone."""
+    pirs, reqs, oracle, alone = climbers
+    sel = [i for i in range(len(reqs)) if reqs[i][0] == 0]
+    twin = sel[0]
+    nv = len(reqs[twin][1])
+    cases = [twin] * 8 + sel[1:5]
+    srv = pcx.Server(max_slots=8, sweeps_per_tick=16, event_driven=events, burst_sweeps=burst[0],
+                     burst_after=burst[1], stage_bytes=nv * 8 * 3 // 2)
+    for p in pirs:
+        srv.add_network(p)
+    stores = []
+    for tag, i in enumerate(cases):
+        lo, hi = reqs[i][1].copy(), reqs[i][2].copy()
+        stores.append((lo, hi))
+        srv.submit(0, lo, hi, tag=tag)
+    got = {}
+    while len(got) < len(cases):
+        srv.step()
+        for tag, r, _ in srv.poll():
+            assert tag not in got
+            got[tag] = r
+        assert srv.stats().ticks < 10000000, "server made no progress"
+    st = srv.stats()
+    for tag, i in enumerate(cases):
+        assert _same(*stores[tag], got[tag], *oracle[i]), f"request {tag} (trace request {i}) vs oracle"
+        assert _same(*stores[tag], got[tag], *alone[i]), f"request {tag} (trace request {i}) vs alone"
+    assert not oracle[twin][2].failed and got[0].sweeps > 64
+    assert st.waits_stage_admit > 0
+    if burst[0] == 0:
+        assert st.waits_stage_harvest > 0
+    else:
+        assert st.waits_stage_side > 0
