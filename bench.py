@@ -1,25 +1,36 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X propagator-completion engine.
 
-Default workload (one step = propagating one batch to its fixpoints):
-  one seeded 50,000-variable / 200,000-propagator ternary network, 4,096
-  search-node subproblems of it, each with 1-20 random splits (x <= v or
-  x > v inside the current bounds) and a random subset of 2,000-200,000
-  active propagators; int32 interval stores, all resident in HBM.
-A step restores the 4,096 stores from their device-resident image and runs
-the batched engine to every instance's fixpoint.  The same batch is also run
-once with the plain engine (one sweep launch per instance and sweep), and a
-bounded sample of instances runs on the CPU oracle (the reference's
-Gauss-Seidel algorithm, restated) on the host cores; the GPU results of that
-sample are checked bit-for-bit against it.
+Headline workload (one step = propagating one batch of new search nodes to
+their fixpoints):
+  one seeded 50,000-variable / 200,000-propagator ternary network from the
+  planted generator (every propagator holds under a hidden assignment;
+  lb ~ U[0, 5e5], ub = lb + U[1, 5e5] around it, 10% constants, 25% Boolean
+  variables, 30% of the propagators in chains of 200), told and run to its
+  root fixpoint; 4,096 search-node subproblems, each the root fixpoint with
+  1-20 random splits (x <= v or x > v inside the current bounds) and a
+  random subset of 2,000-200,000 active propagators; int32 interval stores.
+A step loads a new set of 4,096 stores from device memory (one of four sets
+staged in HBM before the timed region, each drawn with its own seed) and
+runs the batched engine to every instance's fixpoint: the load, the
+emptiness scan and the interleaved copy of the stores are inside the timing.
+Beside it: the same batch replayed from its own reset image (no new
+stores), the plain engine (one sweep launch per instance and sweep), a
+search step on the device, and a bounded sample of instances on the CPU
+oracle (the reference's Gauss-Seidel algorithm, restated) on the host
+cores, whose results are checked bit-for-bit against the GPU's.
 
---config 1 instead times the single 1,000,000-variable / 4,000,000-propagator
-fixpoint (plain engine, one step = one fixpoint); --config 3 the bitset store
-(1,024 bit-removal subproblems of a 200,000-word / 800,000-propagator network);
---config 4 the serving stream (Poisson arrivals at 2,000 requests/s over 24
-networks of 1k-1M propagators, 1-10 splits per request; one step = one
-real-time replay of the trace through the continuous-batching server, with
-p50/p99 latency, beside the same trace served one request at a time).
+The default line also carries the other workloads, each measured the same
+way (HBM-resident inputs, HIP-event kernel timing, CPU oracle beside it):
+  single:  one 1,000,000-variable / 4,000,000-propagator fixpoint (plain
+           engine), ms per fixpoint;
+  bitset:  1,024 bit-removal subproblems of one 200,000-word /
+           800,000-propagator bitset network (values in [0, 61]);
+  serving: Poisson arrivals at 2,000 requests/s over 24 networks of 1k-1M
+           propagators, 1-10 splits per request, replayed in real time through
+           the continuous-batching server: p50 / p99 latency, beside the
+           same trace served one request at a time.
+--config 1|2|3|4 prints the line of one workload alone.
 
 Prints one JSON line (the driver's contract).
 """
@@ -46,9 +57,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+BOX_CPU_SHARE = 16   # host cores a one-GPU box grants a job (worker pools are sized to it)
+
+
+def host_cpus():
+    """(threads the CPU baseline uses, host facts): the affinity count capped
+    at the box's CPU share of 16, beside the affinity and os.cpu_count()."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    used = max(1, min(aff, BOX_CPU_SHARE))
+    return used, {"affinity_cpus": aff, "os_cpu_count": os.cpu_count(), "cpu_share": BOX_CPU_SHARE}
+
+
 def host_threads():
-    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    return max(1, min(n, 16))
+    return host_cpus()[0]
+
+
+def cpu_baseline(value, threads, kind, sample):
+    t, info = host_cpus()
+    return dict({"value": value, "unit": "fixpoints/s", "cores": threads, "kind": kind,
+                 "sample": sample}, **info)
 
 
 def measured_traffic(kernel, workload):
@@ -146,6 +173,21 @@ def cpu_oracle():
     return oracle_lib
 
 
+GEN_LABEL = ("planted generator: a hidden assignment satisfies every propagator, lb ~ U[0, 5e5], "
+             "ub = lb + U[1, 5e5] around it, 10% constants, 25% Boolean variables, 30% of the "
+             "propagators in chains of 200; opcodes uniform over add, sub, mul, min, max, eq-reif, "
+             "leq-reif")
+
+
+def useful_lane_fraction(off, n_inst, P, gi):
+    """Active (instance, propagator) evaluations / pairs the lockstep full
+    sweep loads: every table entry for every slot of every group of gi."""
+    if not gi:
+        return None
+    groups = (n_inst + gi - 1) // gi
+    return float(off[-1]) / float(groups * gi * P)
+
+
 def bench_batched(args):
     V, P, n_inst = args.nvars, args.nprops, 4096
     t0 = time.time()
@@ -168,6 +210,8 @@ def bench_batched(args):
     tuning = None
     if args.autotune:
         mode, tuning = autotune(args, pir, batch, f"cfg2:V{V}:P{P}:n{n_inst}:seed{args.seed}")
+
+    # replay: each step restores the loaded stores from the batch's image
     for _ in range(args.warmup):
         batch.reset()
         batch.run(mode)
@@ -179,13 +223,13 @@ def bench_batched(args):
         res, st = batch.run(mode)
         sums.add(st)
         smax = max(smax, st.sweeps_max)
-    elapsed = time.perf_counter() - t_start
-    evals, kms = sums.v["evals"], sums.v["kernel_ms"]
-    fps = n_inst * args.steps / elapsed
+    replay_s = time.perf_counter() - t_start
+    evals = sums.v["evals"]
+    replay_fps = n_inst * args.steps / replay_s
     failed = sum(r.failed for r in res)
     lb_g, ub_g = batch.read()
-    log(f"batched: {elapsed / args.steps * 1e3:.1f} ms/step, {fps:.0f} fixpoints/s, "
-        f"sweeps_max={smax}, failed={failed}/{n_inst}, kernel {kms / args.steps:.1f} ms/step")
+    log(f"replay: {replay_s / args.steps * 1e3:.2f} ms/step, {replay_fps:.0f} fixpoints/s, "
+        f"sweeps_max={smax}, failed={failed}/{n_inst}, kernel {sums.v['kernel_ms'] / args.steps:.2f} ms/step")
 
     # plain engine on the same batch (one instance at a time)
     plain_fps = None
@@ -200,19 +244,23 @@ def bench_batched(args):
         log(f"plain: {plain_s * 1e3:.0f} ms for {n_inst} instances, {plain_fps:.0f} fixpoints/s")
 
     out = {
-        "value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
         "config": {"workload": f"batched search-node fixpoints: 4,096 subproblems of one "
-                               f"{V:,}-var / {P:,}-propagator network, 1-20 splits each, "
-                               f"{min(2000, P):,}-{P:,} active propagators each",
+                               f"{V:,}-var / {P:,}-propagator network ({GEN_LABEL}), each the "
+                               f"network's root fixpoint with 1-20 random splits and "
+                               f"{min(2000, P):,}-{P:,} active propagators; a step loads 4,096 new "
+                               f"stores from HBM and propagates them",
                    "nvars": V, "nprops": P, "instances": n_inst, "props_per_wave": args.ppw,
                    "mode": {pcx.MODE_BATCHED: "batched", pcx.MODE_PERSISTENT: "persistent"}[mode],
                    "compact_tables": args.compact, "stream_tables": args.stream,
                    "event_driven": args.events, "lockstep": sums.lockstep, "autotuned": tuning},
+        "replay": {"fixpoints_per_s": replay_fps, "ms_per_step": replay_s / args.steps * 1e3,
+                   "what": "each step restores the same 4,096 stores from the batch's device-resident "
+                           "reset image (the image's interleaved copy and emptiness are reused)"},
         "plain_fixpoints_per_s": plain_fps,
-        "batched_vs_plain": fps / plain_fps if plain_fps else None,
-        "evals_per_s": evals / elapsed,
+        "evals_per_s": evals / replay_s,
         "failed_instances": int(failed),
         "sweeps_max": int(smax),
+        "useful_lane_frac": useful_lane_fraction(off, n_inst, P, sums.lockstep),
         "roofline": sums.roofline(args.steps, full_kernel(mode, sums), "k_sweep_marked", "cfg2"),
     }
 
@@ -227,7 +275,7 @@ def bench_batched(args):
         t_cpu = 0.0
         k = 0
         idx64 = idx.astype(np.int64)
-        # bounded sample: whole instances until about 15 s of CPU work
+        # bounded sample: whole instances until about cpu_seconds of CPU work
         while k < n_inst and t_cpu < args.cpu_seconds:
             n = min(threads * 2, n_inst - k)
             rows = slice(k * V, (k + n) * V)
@@ -253,12 +301,79 @@ def bench_batched(args):
                             np.array_equal(lb_p[s], lo[j * V:(j + 1) * V]) and
                             np.array_equal(ub_p[s], hi[j * V:(j + 1) * V])):
                         mism += 1
-        out["cpu_baseline"] = {"value": k / t_cpu, "unit": "fixpoints/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"first {k} of the {n_inst} instances, one instance per "
-                                         f"thread, oracle Gauss-Seidel (-O3)"}
+        out["cpu_baseline"] = cpu_baseline(k / t_cpu, threads, "port",
+                                           f"first {k} of the {n_inst} instances, one instance per "
+                                           f"thread, oracle Gauss-Seidel (-O3)")
         out["parity"] = {"checked_instances": k, "mismatches": mism}
         log(f"cpu: {k} instances in {t_cpu:.1f}s on {threads} threads; parity mismatches {mism}")
+
+    fr = fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V)
+    out["value"] = fr.pop("fixpoints_per_s")
+    out["unit"] = "fixpoints/s"
+    out["ms_per_step"] = fr.pop("ms_per_step")
+    out["fresh"] = fr
+    out["batched_vs_plain"] = out["value"] / plain_fps if plain_fps else None
+    batch.close()
+    return out
+
+
+def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n_sets=4):
+    """The headline steps: each loads a new set of 4,096 stores (splits of the
+    root fixpoint drawn with their own seed) from device memory into the batch
+    and propagates it.  The sets are staged in HBM before the timed region;
+    consecutive steps load different sets.  The last step's results are
+    checked on a bounded sample against the oracle."""
+    from pcx.device import DeviceArray
+    t0 = time.time()
+    sets = []
+    for j in range(n_sets):
+        lo, hi = pcx.gen_splits(args.seed + 100 + j, root_lb, root_ub, n_inst, 1, 20)
+        sets.append((DeviceArray(lo), DeviceArray(hi)))
+        if j == (args.steps - 1) % n_sets:
+            last_host = (lo, hi)
+        del lo, hi
+    log(f"fresh: {n_sets} store sets staged in HBM in {time.time() - t0:.1f}s")
+    for s in range(max(1, args.warmup)):
+        d_lo, d_hi = sets[s % n_sets]
+        batch.load_device(d_lo.ptr, d_hi.ptr)
+        batch.run(mode)
+    sums = StatSum()
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        d_lo, d_hi = sets[s % n_sets]
+        batch.load_device(d_lo.ptr, d_hi.ptr)
+        res, st = batch.run(mode)
+        sums.add(st)
+    elapsed = time.perf_counter() - t_start
+    for d_lo, d_hi in sets:
+        d_lo.close()
+        d_hi.close()
+    fps = n_inst * args.steps / elapsed
+    out = {"fixpoints_per_s": fps, "ms_per_step": elapsed / args.steps * 1e3, "store_sets": n_sets,
+           "evals_per_s": sums.v["evals"] / elapsed,
+           "kernel_ms_per_step": sums.v["kernel_ms"] / args.steps,
+           "failed_instances": int(sum(r.failed for r in res)),
+           "what": "each step loads 4,096 new stores from HBM (pcx_batch_load_device: stores, reset "
+                   "image and emptiness in one pass) and propagates them; the interleaved copy for "
+                   "the lockstep full sweep is rebuilt inside the step"}
+    log(f"fresh: {out['ms_per_step']:.2f} ms/step, {fps:.0f} fixpoints/s")
+    if not args.no_cpu:
+        ol = cpu_oracle()
+        top, tx, ty, tz = pir.table()
+        lb_g, ub_g = batch.read()
+        k = min(n_inst, args.fresh_check)
+        lo, hi = last_host[0][:k * V].copy(), last_host[1][:k * V].copy()
+        r = ol.fixpoint_batch(top, tx, ty, tz, lo, hi, V, idx=idx[:off[k]].astype(np.int64),
+                              off=off[:k + 1], threads=host_threads())
+        mism = 0
+        for j in range(k):
+            if bool(res[j].failed) != bool(r[j].failed):
+                mism += 1
+            elif not r[j].failed and not (np.array_equal(lb_g[j * V:(j + 1) * V], lo[j * V:(j + 1) * V]) and
+                                          np.array_equal(ub_g[j * V:(j + 1) * V], hi[j * V:(j + 1) * V])):
+                mism += 1
+        out["parity"] = {"checked_instances": k, "mismatches": mism, "of_step": args.steps}
+        log(f"fresh parity: {k} instances of the last step, mismatches {mism}")
     return out
 
 
@@ -344,7 +459,9 @@ def bench_single(args):
     lb_g, ub_g = batch.read()
     out = {
         "value": args.steps / elapsed, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
-        "config": {"workload": "single 1,000,000-var / 4,000,000-propagator fixpoint (plain engine)",
+        "config": {"workload": f"single 1,000,000-var / 4,000,000-propagator fixpoint from the root "
+                               f"store (plain engine; {GEN_LABEL}); each step restores the store "
+                               f"from its device-resident image",
                    "nvars": V, "nprops": P, "props_per_wave": args.ppw},
         "evals_per_s": evals / elapsed, "sweeps": int(res[0].sweeps), "failed": bool(res[0].failed),
         "roofline": sums.roofline(args.steps, "k_sweep_plain", "k_sweep_marked_plain", "cfg1"),
@@ -357,8 +474,7 @@ def bench_single(args):
         ts = time.perf_counter()
         r = ol.fixpoint(top, tx, ty, tz, lo, hi)
         t_cpu = time.perf_counter() - ts
-        out["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "fixpoints/s", "cores": 1, "kind": "port",
-                               "sample": "the full fixpoint, one thread"}
+        out["cpu_baseline"] = cpu_baseline(1.0 / t_cpu, 1, "port", "the full fixpoint, one thread")
         out["parity"] = {"bit_exact": bool(np.array_equal(lo, lb_g) and np.array_equal(hi, ub_g)),
                          "oracle_sweeps": int(r.sweeps)}
     return out
@@ -399,8 +515,11 @@ def bench_bits(args):
     rf = sums.roofline(args.steps, full_kernel(mode, sums) + "_bits", "k_sweep_marked_bits", "cfg3",
                        tight_bytes=8)
     out = {"value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
-           "config": {"workload": "bitset store: 1,024 bit-removal subproblems of one 200,000-word / "
-                                  "800,000-propagator network (add, neq, leq, eq-reif)",
+           "config": {"workload": "bitset store: 1,024 bit-removal subproblems (1-20 removals each) of "
+                                  "the root fixpoint of one 200,000-word / 800,000-propagator planted "
+                                  "network (add, neq, leq, eq-reif; domain density U[0.3, 1.0] over the "
+                                  "values 0-61); each step restores the 1,024 stores from their "
+                                  "device-resident image",
                       "nvars": V, "nprops": P, "instances": n_inst,
                       "mode": {pcx.MODE_BATCHED: "batched", pcx.MODE_PERSISTENT: "persistent"}[mode],
                       "lockstep": sums.lockstep, "autotuned": tuning},
@@ -426,8 +545,8 @@ def bench_bits(args):
                         words_g[(k + j) * V:(k + j + 1) * V], wo[j * V:(j + 1) * V])):
                     mism += 1
             k += n
-        out["cpu_baseline"] = {"value": k / t_cpu, "unit": "fixpoints/s", "cores": threads, "kind": "port",
-                               "sample": f"first {k} of the {n_inst} instances, one per thread"}
+        out["cpu_baseline"] = cpu_baseline(k / t_cpu, threads, "port",
+                                           f"first {k} of the {n_inst} instances, one per thread")
         out["parity"] = {"checked_instances": k, "mismatches": mism}
     return out
 
@@ -533,12 +652,43 @@ def bench_serving(args):
                             np.array_equal(hi[si], chi[q * V:(q + 1) * V]))):
                         mism += 1
             k = j
-        out_d["cpu_baseline"] = {"value": k / t_cpu, "unit": "fixpoints/s", "cores": threads, "kind": "port",
-                                 "sample": f"first {k} of the {n} requests, the requests of each network "
-                                           "spread over the threads, oracle Gauss-Seidel (-O3)"}
+        out_d["cpu_baseline"] = cpu_baseline(k / t_cpu, threads, "port",
+                                             f"first {k} of the {n} requests, the requests of each network "
+                                             "spread over the threads, oracle Gauss-Seidel (-O3)")
         out_d["parity"] = {"checked_requests": k, "mismatches": mism}
         log(f"cpu: {k} requests in {t_cpu:.1f}s on {threads} threads; parity mismatches {mism}")
     return out_d
+
+
+def sub_args(args, **kw):
+    """Arguments of a workload run beside the headline: bounded steps and CPU
+    time, so the whole default line finishes in a few minutes."""
+    sa = argparse.Namespace(**vars(args))
+    sa.steps = min(args.steps, 10)
+    sa.warmup = min(args.warmup, 2)
+    sa.cpu_seconds = args.sub_cpu_seconds
+    for k, v in kw.items():
+        setattr(sa, k, v)
+    return sa
+
+
+def all_workloads(args):
+    """The headline (batched search nodes, fresh stores per step) with the
+    single-instance, bitset and serving workloads beside it."""
+    import gc
+    out = bench_batched(args)
+    subs = {}
+    for name, fn, kw in (("single", bench_single, {}),
+                         ("bitset", bench_bits, {"lockstep": -1}),
+                         ("serving", bench_serving, {"steps": 1, "warmup": 1})):
+        gc.collect()
+        t0 = time.time()
+        log(f"== {name}")
+        subs[name] = fn(sub_args(args, **kw))
+        subs[name]["bench_s"] = round(time.time() - t0, 1)
+    out["workloads"] = subs
+    out["latency_ms"] = dict(subs["serving"]["latency_ms"], workload="serving")
+    return out
 
 
 def main():
@@ -546,7 +696,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4])
+    ap.add_argument("--config", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="0 (default): every workload, the batched search nodes as the headline; "
+                         "1-4: one workload's line alone")
     ap.add_argument("--mode", default="batched", choices=["batched", "persistent"],
                     help="engine for the timed batch (the plain engine is always run once beside it)")
     ap.add_argument("--ppw", type=int, default=0, choices=[0, 64, 128, 256, 512, 1024, 2048],
@@ -576,6 +728,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-engine comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--sub-cpu-seconds", type=float, default=8.0,
+                    help="CPU-baseline time of each workload beside the headline")
+    ap.add_argument("--fresh-check", type=int, default=256,
+                    help="headline: instances of the last fresh step checked against the oracle")
     ap.add_argument("--search-step", type=int, default=1, choices=[0, 1],
                     help="config 2: after the timed steps, branch the subproblems on the device and "
                          "propagate the children (reported beside the headline figure)")
@@ -592,7 +748,8 @@ def main():
     if pcx.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    out = {1: bench_single, 2: bench_batched, 3: bench_bits, 4: bench_serving}[args.config](args)
+    out = {0: all_workloads, 1: bench_single, 2: bench_batched, 3: bench_bits,
+           4: bench_serving}[args.config](args)
     line = {"metric": METRIC, "value": out.pop("value"), "unit": out.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out.pop("ms_per_step"),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",

@@ -191,6 +191,12 @@ pcx_status pcx_batch_create(pcx_engine* e, int64_t n_inst, int64_t nvars,
 void       pcx_batch_destroy(pcx_batch* b);
 /* Upload n_inst*nvars bounds; they also become the reset image. */
 pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub);
+/* The same from DEVICE memory: d_lb / d_ub hold n_inst*nvars bounds in the
+ * same order (e.g. stores a caller's kernel produced, or inputs staged in HBM
+ * ahead of time); no host transfer.  One pass writes the stores, the reset
+ * image and the per-instance emptiness.  The source may be reused or freed
+ * once the call returns.  Interval batches. */
+pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_t* d_ub);
 /* Restore every store from the reset image (device to device). */
 pcx_status pcx_batch_reset(pcx_batch* b);
 pcx_status pcx_batch_read(pcx_batch* b, int32_t* lb, int32_t* ub);
@@ -335,6 +341,12 @@ typedef struct {
   int64_t arena_peak_bytes;  /* highest arena byte in use                     */
   int32_t max_running;       /* most requests on the GPU in one step          */
   int32_t reserved;
+  /* store copies deferred to a later tick because the pinned staging buffer
+     (stage_bytes) was full: admissions, hand-backs from the slot batch and
+     hand-backs from the side lanes                                           */
+  int64_t waits_stage_admit;
+  int64_t waits_stage_harvest;
+  int64_t waits_stage_side;
 } pcx_server_stats;
 
 void       pcx_server_config_default(pcx_server_config* cfg);

@@ -561,10 +561,11 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   };
   // Groups after the first hold only marked sweeps, whose launches have the
   // same arguments every time (the sweep number lives on the device): they
-  // are captured once into a graph and replayed.
-  const uint64_t key = (e->version << 20) ^ ((uint64_t)group << 8) ^ ((uint64_t)dom_of(e) << 4) ^
-                       (uint64_t)(maxs & 0xf) ^ ((uint64_t)(uintptr_t)e->d_xo << 1);
-  if (b->events && (!b->marked_graph || b->marked_graph_key != key)) {
+  // are captured once into a graph and replayed while every argument the
+  // capture used is unchanged (the batch's own buffers never move).
+  const pcx_batch::MarkedGraphArgs args{e->version, group, dom_of(e), maxs, ppw,
+                                        e->d_xo, e->d_y, e->d_z, e->d_adj_off, e->d_adj};
+  if (b->events && (!b->marked_graph || !(b->marked_graph_args == args))) {
     if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
     b->marked_graph = nullptr;
     hipGraph_t graph = nullptr;
@@ -577,7 +578,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     const hipError_t ie = hipGraphInstantiate(&b->marked_graph, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     PCX_HIP(ie);
-    b->marked_graph_key = key;
+    b->marked_graph_args = args;
   }
   while (live > 0) {
     if (b->events && sweeps >= group) {     // groups after the first: all marked sweeps
@@ -955,6 +956,20 @@ pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub) {
   PCX_HIP(launch_store_pack(d_lb, d_ub, b->d_image, n, e->stream));
   PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, n * sizeof(int2), hipMemcpyDeviceToDevice, e->stream));
   return image_loaded(b);
+}
+
+pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_t* d_ub) {
+  if (!b || !d_lb || !d_ub) return fail_with(PCX_EINVAL, "NULL argument");
+  if (b->e->cfg.domain != PCX_DOMAIN_INTERVAL) return fail_with(PCX_EINVAL, "interval batch expected");
+  pcx_engine* e = b->e;
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  const int64_t n = b->n_inst * b->nvars;
+  PCX_HIP(hipMemsetAsync(b->d_image_empty, 0, b->n_inst, e->stream));
+  PCX_HIP(launch_ingest(d_lb, d_ub, b->d_image, b->d_store, n, b->nvars, b->d_image_empty, e->stream));
+  PCX_HIP(hipStreamSynchronize(e->stream));
+  b->store_is_image = true;
+  b->shadow_is_image = false;
+  return PCX_OK;
 }
 
 pcx_status pcx_batch_reset(pcx_batch* b) {

@@ -216,6 +216,10 @@ hipError_t launch_branch_seed(const EventSet& ev, const int32_t* var, int64_t n_
 hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
                       hipStream_t st);
 hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n, hipStream_t st);
+// image[v] = store[v] = {lb[v], ub[v]} for n device-resident bounds, and
+// empty[v / nvars] = 1 where lb > ub (empty zeroed by the caller)
+hipError_t launch_ingest(const int32_t* lb, const int32_t* ub, int2* image, int2* store, int64_t n,
+                         int64_t nvars, uint8_t* empty, hipStream_t st);
 // device copy of n 8-byte words (16-byte aligned buffers)
 hipError_t launch_copy_words(const void* src, void* dst, int64_t n, hipStream_t st);
 hipError_t launch_store_unpack(const int2* store, int32_t* lb, int32_t* ub, int64_t n, hipStream_t st);

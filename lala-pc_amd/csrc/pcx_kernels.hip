@@ -2290,6 +2290,68 @@ hipError_t launch_copy_words(const void* src, void* dst, int64_t n, hipStream_t 
   return hipGetLastError();
 }
 
+// Stores loaded from device memory: image[v] = store[v] = {lb[v], ub[v]} and
+// empty[k] = 1 when instance k holds some lb > ub (empty zeroed first), in
+// one pass.  Four variables per thread and step: two 16-byte loads, four
+// 16-byte stores; the source is read once, so its loads are non-temporal.
+typedef int32_t v4i __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_ingest4(const v4i* __restrict__ lb, const v4i* __restrict__ ub,
+                                                 v4i* __restrict__ image, v4i* __restrict__ store,
+                                                 int64_t n4, int64_t nvars, uint8_t* __restrict__ empty) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += stride) {
+    const v4i a = __builtin_nontemporal_load(lb + i);
+    const v4i b = __builtin_nontemporal_load(ub + i);
+    const v4i p0 = {a.x, b.x, a.y, b.y};
+    const v4i p1 = {a.z, b.z, a.w, b.w};
+    image[2 * i] = p0;
+    image[2 * i + 1] = p1;
+    store[2 * i] = p0;
+    store[2 * i + 1] = p1;
+    const int bad = (a.x > b.x) | ((a.y > b.y) << 1) | ((a.z > b.z) << 2) | ((a.w > b.w) << 3);
+    if (bad)
+      for (int j = 0; j < 4; ++j)
+        if (bad >> j & 1) empty[(4 * i + j) / nvars] = 1;
+  }
+}
+// The same one variable at a time: sources that are not 16-byte aligned, and
+// the tail of n % 4 variables.
+__global__ void k_ingest1(const int32_t* __restrict__ lb, const int32_t* __restrict__ ub,
+                          int2* __restrict__ image, int2* __restrict__ store, int64_t from, int64_t n,
+                          int64_t nvars, uint8_t* __restrict__ empty) {
+  for (int64_t v = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int2 p = make_int2(lb[v], ub[v]);
+    image[v] = p;
+    store[v] = p;
+    if (p.x > p.y) empty[v / nvars] = 1;
+  }
+}
+
+hipError_t launch_ingest(const int32_t* lb, const int32_t* ub, int2* image, int2* store, int64_t n,
+                         int64_t nvars, uint8_t* empty, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  const bool aligned = ((reinterpret_cast<uintptr_t>(lb) | reinterpret_cast<uintptr_t>(ub) |
+                         reinterpret_cast<uintptr_t>(image) | reinterpret_cast<uintptr_t>(store)) & 15) == 0;
+  int64_t done = 0;
+  if (aligned && n >= 4) {
+    const int64_t n4 = n / 4;
+    int64_t grid = (n4 + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    k_ingest4<<<dim3((unsigned)grid), 256, 0, stream>>>(reinterpret_cast<const v4i*>(lb),
+                                                        reinterpret_cast<const v4i*>(ub),
+                                                        reinterpret_cast<v4i*>(image),
+                                                        reinterpret_cast<v4i*>(store), n4, nvars, empty);
+    done = 4 * n4;
+  }
+  if (done < n) {
+    int64_t grid = (n - done + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    k_ingest1<<<dim3((unsigned)grid), 256, 0, stream>>>(lb, ub, image, store, done, n, nvars, empty);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n,
                              hipStream_t stream) {
   int64_t grid = (n + 255) / 256;

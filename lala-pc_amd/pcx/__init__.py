@@ -106,7 +106,9 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("ticks", ctypes.c_int64), ("sweeps", ctypes.c_int64), ("admitted", ctypes.c_int64),
                 ("completed", ctypes.c_int64), ("waits_slot", ctypes.c_int64),
                 ("waits_arena", ctypes.c_int64), ("arena_peak_bytes", ctypes.c_int64),
-                ("max_running", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("max_running", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("waits_stage_admit", ctypes.c_int64), ("waits_stage_harvest", ctypes.c_int64),
+                ("waits_stage_side", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -142,6 +144,7 @@ SIGNATURES = [
     ("pcx_batch_create", ctypes.c_int32, [_VP, ctypes.c_int64, ctypes.c_int64, _P64, _P32, ctypes.POINTER(_VP)]),
     ("pcx_batch_destroy", None, [_VP]),
     ("pcx_batch_load", ctypes.c_int32, [_VP, _P32, _P32]),
+    ("pcx_batch_load_device", ctypes.c_int32, [_VP, _VP, _VP]),
     ("pcx_batch_reset", ctypes.c_int32, [_VP]),
     ("pcx_batch_read", ctypes.c_int32, [_VP, _P32, _P32]),
     ("pcx_batch_run", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Result), ctypes.POINTER(Stats)]),
@@ -205,25 +208,41 @@ def lib():
     return _lib
 
 
-def _p32(a):
+def _ptr(a, dtype, ptype, size=None):
+    """ctypes pointer to a C-contiguous array of `dtype` holding at least
+    `size` elements.  The C side reads and writes exactly the sizes it is
+    given, so a wrong dtype, layout or length is refused here (ValueError),
+    not left to corrupt memory."""
     if a is None:
         return None
-    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], "int32 C-contiguous array expected"
-    return a.ctypes.data_as(_P32)
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError(f"{np.dtype(dtype).name} C-contiguous numpy array expected")
+    if size is not None and a.size < size:
+        raise ValueError(f"array of {a.size} elements where {size} are needed")
+    return a.ctypes.data_as(ptype)
 
 
-def _p64(a):
-    if a is None:
-        return None
-    assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"], "int64 C-contiguous array expected"
-    return a.ctypes.data_as(_P64)
+def _p32(a, size=None):
+    return _ptr(a, np.int32, _P32, size)
 
 
-def _pu64(a):
-    if a is None:
-        return None
-    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], "uint64 C-contiguous array expected"
-    return a.ctypes.data_as(_PU64)
+def _p64(a, size=None):
+    return _ptr(a, np.int64, _P64, size)
+
+
+def _pu64(a, size=None):
+    return _ptr(a, np.uint64, _PU64, size)
+
+
+def _store_pair(lb, ub):
+    """Length of an lb/ub store pair, refusing a mismatch."""
+    if lb is None or ub is None:
+        if lb is not None or ub is not None:
+            raise ValueError("lb and ub must both be given or both be None")
+        return 0
+    if lb.size != ub.size:
+        raise ValueError(f"lb has {lb.size} entries and ub {ub.size}")
+    return lb.size
 
 
 def _check(st):
@@ -272,8 +291,11 @@ class PIR:
         """Append propagators x = y <op> z; embeds reified outputs in [0,1] in lb/ub."""
         op, x, y, z = (np.ascontiguousarray(a, dtype=np.int32) for a in (op, x, y, z))
         n = len(op)
-        nv = 0 if lb is None else len(lb)
-        _check(lib().pcx_tell(self._h, _p32(op), _p32(x), _p32(y), _p32(z), n, _p32(lb), _p32(ub), nv))
+        if not (len(x) == len(y) == len(z) == n):
+            raise ValueError("op, x, y, z must have the same length")
+        nv = _store_pair(lb, ub)
+        _check(lib().pcx_tell(self._h, _p32(op), _p32(x), _p32(y), _p32(z), n, _p32(lb, nv),
+                              _p32(ub, nv), nv))
 
     interpret_tell = tell
 
@@ -293,6 +315,15 @@ class PIR:
         _check(lib().pcx_set_config(self._h, ctypes.byref(c)))
         self.config = c
 
+    def snapshot(self):
+        """The table size (pir.hpp:857-860); stores are the caller's."""
+        return int(lib().pcx_snapshot(self._h))
+
+    def restore(self, snap):
+        """Pop the propagators told since `snap` off the end of the sorted
+        table, as the reference does (pir.hpp:862-870)."""
+        _check(lib().pcx_restore(self._h, int(snap)))
+
     def table(self):
         n = self.num_deductions()
         out = [np.zeros(n, np.int32) for _ in range(4)]
@@ -302,14 +333,16 @@ class PIR:
     def fixpoint(self, lb, ub):
         """Propagate lb/ub (int32, modified in place) to the fixpoint; returns Result."""
         r = Result()
-        _check(lib().pcx_fixpoint(self._h, _p32(lb), _p32(ub), len(lb), ctypes.byref(r)))
+        nv = _store_pair(lb, ub)
+        _check(lib().pcx_fixpoint(self._h, _p32(lb, nv), _p32(ub, nv), nv, ctypes.byref(r)))
         return r
 
     def ask_all(self, lb, ub):
         n = self.num_deductions()
         ent = np.zeros(max(n, 1), np.uint8)
         allf = ctypes.c_int32(0)
-        _check(lib().pcx_ask_all(self._h, _p32(lb), _p32(ub), len(lb),
+        nv = _store_pair(lb, ub)
+        _check(lib().pcx_ask_all(self._h, _p32(lb, nv), _p32(ub, nv), nv,
                                  ent.ctypes.data_as(_PU8), ctypes.byref(allf)))
         return ent[:n].astype(bool), bool(allf.value)
 
@@ -332,7 +365,9 @@ class BitsPIR(PIR):
 
     def tell(self, op, x, y, z, words=None):
         op, x, y, z = (np.ascontiguousarray(a, dtype=np.int32) for a in (op, x, y, z))
-        nv = 0 if words is None else len(words)
+        if not (len(x) == len(y) == len(z) == len(op)):
+            raise ValueError("op, x, y, z must have the same length")
+        nv = 0 if words is None else words.size
         _check(lib().pcx_bits_tell(self._h, _p32(op), _p32(x), _p32(y), _p32(z), len(op),
                                    _pu64(words), nv))
 
@@ -340,7 +375,7 @@ class BitsPIR(PIR):
 
     def fixpoint(self, words):
         r = Result()
-        _check(lib().pcx_bits_fixpoint(self._h, _pu64(words), len(words), ctypes.byref(r)))
+        _check(lib().pcx_bits_fixpoint(self._h, _pu64(words), words.size, ctypes.byref(r)))
         return r
 
 
@@ -351,12 +386,16 @@ class Batch:
         self.pir = pir
         self.n_inst = int(n_inst)
         self.nvars = int(nvars)
+        n_idx = None
         if inst_off is not None:
             inst_off = np.ascontiguousarray(inst_off, dtype=np.int64)
             prop_idx = np.ascontiguousarray(prop_idx, dtype=np.int32)
+            if inst_off.size != self.n_inst + 1:
+                raise ValueError(f"inst_off needs n_inst + 1 = {self.n_inst + 1} entries")
+            n_idx = int(inst_off[-1])
         h = _VP()
         _check(lib().pcx_batch_create(pir._h, self.n_inst, self.nvars, _p64(inst_off),
-                                      _p32(prop_idx), ctypes.byref(h)))
+                                      _p32(prop_idx, n_idx), ctypes.byref(h)))
         self._h = h
 
     def close(self):
@@ -371,15 +410,26 @@ class Batch:
             pass
 
     def load(self, lb, ub):
-        assert lb.size == self.n_inst * self.nvars
-        _check(lib().pcx_batch_load(self._h, _p32(lb.reshape(-1)), _p32(ub.reshape(-1))))
+        n = self.n_inst * self.nvars
+        if _store_pair(lb, ub) != n:
+            raise ValueError(f"a batch load needs n_inst * nvars = {n} bounds in lb and in ub")
+        _check(lib().pcx_batch_load(self._h, _p32(lb.reshape(-1), n), _p32(ub.reshape(-1), n)))
+
+    def load_device(self, d_lb, d_ub):
+        """Load the stores from device memory: d_lb / d_ub are device addresses
+        (ints) of n_inst * nvars int32 bounds each, laid out as for load()."""
+        if not d_lb or not d_ub:
+            raise ValueError("device addresses expected")
+        _check(lib().pcx_batch_load_device(self._h, _VP(int(d_lb)), _VP(int(d_ub))))
 
     def reset(self):
         _check(lib().pcx_batch_reset(self._h))
 
     def load_words(self, words):
-        assert words.size == self.n_inst * self.nvars
-        _check(lib().pcx_bits_batch_load(self._h, _pu64(words.reshape(-1))))
+        n = self.n_inst * self.nvars
+        if words.size != n:
+            raise ValueError(f"a batch load needs n_inst * nvars = {n} words")
+        _check(lib().pcx_bits_batch_load(self._h, _pu64(words.reshape(-1), n)))
 
     def read_words(self):
         w = np.zeros(self.n_inst * self.nvars, np.uint64)
@@ -476,7 +526,8 @@ class Server:
         return n.value
 
     def submit(self, net, lb, ub, tag):
-        _check(lib().pcx_server_submit(self._h, int(net), _p32(lb), _p32(ub), len(lb), int(tag)))
+        nv = _store_pair(lb, ub)
+        _check(lib().pcx_server_submit(self._h, int(net), _p32(lb, nv), _p32(ub, nv), nv, int(tag)))
         self._live[int(tag)] = (lb, ub)   # the arrays must outlive the request
 
     def step(self):
@@ -504,7 +555,10 @@ class Server:
         arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
         net = np.ascontiguousarray(net, dtype=np.int32)
         store_off = np.ascontiguousarray(store_off, dtype=np.int64)
-        assert len(store_off) == n + 1 and lb_all.size >= store_off[-1] and ub_all.size >= store_off[-1]
+        if len(net) != n or len(store_off) != n + 1:
+            raise ValueError("net needs one entry per arrival and store_off one more")
+        if n and (lb_all.size < store_off[-1] or ub_all.size < store_off[-1]):
+            raise ValueError("lb_all / ub_all are shorter than store_off[-1]")
         out = (Served * max(n, 1))()
         wall = ctypes.c_double(0.0)
         _check(lib().pcx_server_replay(self._h, n, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
@@ -580,7 +634,7 @@ def gen_network(p):
 
 
 def gen_splits(seed, lb, ub, n_inst, kmin, kmax):
-    V = len(lb)
+    V = _store_pair(lb, ub)
     lo = np.zeros(n_inst * V, np.int32)
     hi = np.zeros(n_inst * V, np.int32)
     _check(lib().pcxg_splits(seed, _p32(lb), _p32(ub), V, n_inst, kmin, kmax, _p32(lo), _p32(hi)))

@@ -173,3 +173,21 @@ def test_struct_layouts_match_c(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for f, _ in cls._fields_:
             assert got[(cname, f)] == getattr(cls, f).offset, f"{cname}.{f}"
+
+
+def test_reference_side_adapter_compiles_against_pir_surface(tmp_path):
+    """include/lala_pcx_adapter.hpp compiles against the stand-in of PIR's
+    public surface (tests/adapter/pir_stub.hpp) and links to libpcx.so; with
+    no HIP device the adapter reports pcx's error instead of propagating."""
+    import subprocess
+    import pcx
+    here = os.path.dirname(os.path.abspath(__file__))
+    out = str(tmp_path / "adapter_kat")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(here, "adapter"), f"OUT={out}"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    if pcx.device_count() > 0:
+        return
+    r = subprocess.run([out], input="vars 3\n0 10\n0 10\n0 5\ntell 1\nADD 2 0 1\nfix\n",
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "no HIP device" in r.stderr

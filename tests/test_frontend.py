@@ -19,10 +19,6 @@ from frontend_cases import load_cases, run_case
 CASES = load_cases()
 
 
-def oracle_backend(op, x, y, z, lb, ub, before, _state={}):
-    raise NotImplementedError
-
-
 class Oracle:
     """The oracle as the fixpoint: keeps the whole table, sorted as tell does."""
 
