@@ -813,6 +813,10 @@ struct Parser {
     const long long hi = integer();
     expect("]");
     if (lo != 1 || hi < 0) fail("arrays are indexed 1..n");
+    // every element of a var array becomes a variable: refuse a size the
+    // store cannot index before creating any of them
+    if (hi > (long long)PCX_MAX_VARS - (long long)b.m.vars.size())
+      fail("array of " + std::to_string(hi) + " elements: more variables than a store indexes");
     if (!is("of")) fail("expected 'of'");
     skip();
     const bool is_var = is("var");

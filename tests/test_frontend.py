@@ -109,3 +109,15 @@ def test_flatzinc_round_trip(seed):
     assert np.array_equal(my, y) and np.array_equal(mz, z)
     mlb, mub = m.store()
     assert np.array_equal(mlb, lb) and np.array_equal(mub, ub)
+
+
+def test_oversized_array_is_refused_before_creating_variables():
+    """A var array larger than a store can index is an error at once (the
+    sanitizer corpus found the front-end creating 2^28 variables first)."""
+    import time
+    t0 = time.time()
+    with pytest.raises(pcx.PcxError) as e:
+        pcx.Model("array[1..2147483647] of var int: q;")
+    assert e.value.status == pcx.EINVAL and time.time() - t0 < 5
+    m = pcx.Model("array[1..3] of var 0..1: q; constraint bool_clause([q[1]], [q[2]]);")
+    assert m.num_vars() >= 3
