@@ -171,6 +171,25 @@ pcx_status pcx_restore(pcx_engine* e, int64_t snapshot);
 pcx_status pcx_fixpoint(pcx_engine* e, int32_t* lb, int32_t* ub, int64_t nvars,
                         pcx_result* out);
 
+/* pcx_fixpoint_noalloc: pcx_fixpoint for callers that cannot allocate
+ * (tight serving loops, harnesses that forbid hipMalloc).  No counterpart in
+ * the reference, whose fixpoint engine allocates nothing on a device.
+ * d_lb / d_ub: nvars int32 bounds each in DEVICE memory, the store in and
+ * its fixpoint out (after a failure some empty store).  The call allocates
+ * nothing: its working state (the packed store, flags and counters) lives
+ * in one fixed device array of the library per GPU, which holds stores of
+ * at most PCX_NOALLOC_MAX_VARS variables (PCX_EINVAL beyond that), and the
+ * engine's table must fit its device copy (after a tell that grew the
+ * table, one pcx_fixpoint call sizes it; PCX_EINVAL otherwise).  Sweeps are
+ * full sweeps (no marks, which would need memory per propagator); the
+ * fixpoint is the one pcx_fixpoint gives.  Calls that overlap on one GPU
+ * share that array and race: one call at a time per GPU. */
+#define PCX_NOALLOC_MAX_VARS (1 << 21)
+pcx_status pcx_fixpoint_noalloc(pcx_engine* e, int32_t* d_lb, int32_t* d_ub, int64_t nvars,
+                                pcx_result* out);
+/* PCX_NOALLOC_MAX_VARS, for bindings that do not read this header. */
+int64_t    pcx_noalloc_max_vars(void);
+
 /* pcx_ask_all: entailment of every propagator on a store,
  *   replaces PIR::ask(int) (pir.hpp:368-370, 417-438) for all i at once.
  * entailed[i] (may be NULL) receives 0/1 per propagator in table order;

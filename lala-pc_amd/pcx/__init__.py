@@ -140,6 +140,8 @@ SIGNATURES = [
     ("pcx_snapshot", ctypes.c_int64, [_VP]),
     ("pcx_restore", ctypes.c_int32, [_VP, ctypes.c_int64]),
     ("pcx_fixpoint", ctypes.c_int32, [_VP, _P32, _P32, ctypes.c_int64, ctypes.POINTER(Result)]),
+    ("pcx_fixpoint_noalloc", ctypes.c_int32, [_VP, _VP, _VP, ctypes.c_int64, ctypes.POINTER(Result)]),
+    ("pcx_noalloc_max_vars", ctypes.c_int64, []),
     ("pcx_ask_all", ctypes.c_int32, [_VP, _P32, _P32, ctypes.c_int64, _PU8, _P32]),
     ("pcx_batch_create", ctypes.c_int32, [_VP, ctypes.c_int64, ctypes.c_int64, _P64, _P32, ctypes.POINTER(_VP)]),
     ("pcx_batch_destroy", None, [_VP]),
@@ -335,6 +337,17 @@ class PIR:
         r = Result()
         nv = _store_pair(lb, ub)
         _check(lib().pcx_fixpoint(self._h, _p32(lb, nv), _p32(ub, nv), nv, ctypes.byref(r)))
+        return r
+
+    def fixpoint_noalloc(self, d_lb, d_ub, nvars):
+        """pcx_fixpoint_noalloc: the fixpoint of a store held in device memory
+        (d_lb / d_ub: device addresses of nvars int32 bounds each, updated in
+        place); the call allocates nothing.  Returns Result."""
+        if not d_lb or not d_ub:
+            raise ValueError("device addresses expected")
+        r = Result()
+        _check(lib().pcx_fixpoint_noalloc(self._h, _VP(int(d_lb)), _VP(int(d_ub)), int(nvars),
+                                          ctypes.byref(r)))
         return r
 
     def ask_all(self, lb, ub):
