@@ -1,6 +1,7 @@
-"""Runs the reference's FlatZinc test models (tests/golden/fzn_kats.json)
+"""Runs the reference's FlatZinc test models (tests/golden/fzn_kats.json from
+tests/pir_test.cpp, tests/golden/pc_kats.json from tests/pc_test.cpp)
 through the front-end and a fixpoint, step by step, as the reference's
-tests/pir_test.cpp does: parse and tell, check the store before the
+tests do: parse and tell, check the store before the
 fixpoint, run Gauss-Seidel (here: the oracle or the GPU engine), check the
 store after it, tell the next constraints into the same store.  Only the
 variables of the original model are compared (by name); the generated ones
@@ -13,11 +14,20 @@ import numpy as np
 import pcx
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fzn_kats.json")
+GOLDEN_PC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pc_kats.json")
 
 
-def load_cases():
-    with open(GOLDEN) as f:
+def load_cases(path=GOLDEN):
+    with open(path) as f:
         return json.load(f)["cases"]
+
+
+def load_pc_cases():
+    """The model tests of the reference's PC (tests/pc_test.cpp).  PC keeps
+    formulas as trees and interprets them directly; here they go through the
+    same ternary decomposition, and the stores of the original variables must
+    still be PC's (the fixpoint of the decomposition projected on them)."""
+    return load_cases(GOLDEN_PC)
 
 
 def run_case(case, backend):

@@ -1,4 +1,6 @@
-"""Writes tests/golden/fzn_kats.json from the reference's tests/pir_test.cpp.
+"""Writes tests/golden/fzn_kats.json from the reference's tests/pir_test.cpp,
+and tests/golden/pc_kats.json from its tests/pc_test.cpp (the same kind of
+test over PC, the general propagator completion, include/lala/pc.hpp).
 
 Each case is one TEST of that file whose model is a string literal: the
 model text, the constraints told after it, and, for every deduce_and_test /
@@ -18,8 +20,12 @@ import os
 import re
 import sys
 
-SRC = "/root/reference/tests/pir_test.cpp"
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fzn_kats.json")
+HERE = os.path.dirname(os.path.abspath(__file__))
+# (source, TEST suite name, output file, the reference's abstract domain)
+SOURCES = [("/root/reference/tests/pir_test.cpp", "PIRTest", os.path.join(HERE, "fzn_kats.json"), "PIR"),
+           ("/root/reference/tests/pc_test.cpp", "IPCTest", os.path.join(HERE, "pc_kats.json"), "PC")]
+# the test helpers that build the abstract element from a model string
+CREATE = r"create_and_interpret_and_tell<[^>]*>|interpret_type_and_tell2?<[^>]*>"
 INF = 2**31 - 1
 MINF = -2**31
 
@@ -116,9 +122,10 @@ def declared_names(model):
     return names
 
 
-def main():
-    src = open(SRC).read()
-    tests = list(re.finditer(r"^TEST\(PIRTest,\s*(\w+)\)\s*\{", src, flags=re.M))
+def extract(path, suite, domain):
+    src = open(path).read()
+    rel = "tests/" + os.path.basename(path)
+    tests = list(re.finditer(r"^TEST\(" + suite + r",\s*(\w+)\)\s*\{", src, flags=re.M))
     cases, skipped = [], []
     for k, t in enumerate(tests):
         name = t.group(1)
@@ -126,18 +133,18 @@ def main():
         body = src[t.end():end]
         line0 = src[:t.start()].count("\n") + 1
         line1 = line0 + body.rstrip().count("\n")
-        mk = re.search(r"create_and_interpret_and_tell<[^>]*>\(", body)
+        mk = re.search(r"(" + CREATE + r")\(", body)
         if not mk or "std::format" in body:
             skipped.append(name)
             continue
         texts, steps = [], []
         i = 0
         ok = True
-        for m in re.finditer(r"(create_and_interpret_and_tell<[^>]*>|interpret_must_succeed<[^>]*>|"
+        for m in re.finditer(r"(" + CREATE + r"|interpret_must_succeed<[^>]*>|"
                              r"deduce_and_test_bot|deduce_and_test)\(", body):
             args, _ = call_args(body, m.end() - 1)
             fn = m.group(1)
-            if fn.startswith("create_and_interpret") or fn.startswith("interpret_must_succeed"):
+            if not fn.startswith("deduce_and_test"):
                 strs = c_strings(args[0])
                 if len(strs) != 1:
                     ok = False
@@ -153,9 +160,19 @@ def main():
                 else:
                     before = itv_list(args[2])
                     after = before
+                    # deduce_and_test(x, nd, before, after, is_ua, expect_changed = true) or
+                    # deduce_and_test(x, nd, before_after, is_ua = false) (pir_test.cpp:54-89)
+                    flags = args[4:] if len(args) >= 4 and args[3].strip().startswith("{") else args[3:]
                     if len(args) >= 4 and args[3].strip().startswith("{"):
                         after = itv_list(args[3])
-                    steps.append({"nd": nd, "before": before, "after": after, "failed": False})
+                        changed = flags[1].strip() == "true" if len(flags) > 1 else True
+                    else:
+                        changed = False
+                    ua = bool(flags) and flags[0].strip() == "true"
+                    # an expected store with an empty interval is a failure (is_bot)
+                    empty = any(e == "bot" or e[0] > e[1] for e in after)
+                    steps.append({"nd": nd, "before": before, "after": after, "failed": empty,
+                                  "ua": ua, "changed": changed})
             except (ValueError, AssertionError):
                 ok = False
                 break
@@ -167,16 +184,25 @@ def main():
             names = re.findall(r"\(var\s+(\w+):", gen.group(1))
         else:
             names = declared_names(texts[0])
-        cases.append({"name": name, "ref": f"tests/pir_test.cpp:{line0}-{line1}", "names": names,
+        cases.append({"name": name, "ref": f"{rel}:{line0}-{line1}", "names": names,
                       "steps": [dict(text=x, **s) for x, s in zip(texts, steps)]})
-    about = ("Models and expected stores of the reference's tests/pir_test.cpp, extracted as data by "
-             "tests/golden/make_fzn_kats.py: for each step, the FlatZinc text told, the reference's "
-             "propagator count after it (nd), and the expected store before and after the Gauss-Seidel "
-             "fixpoint over the reference's variables in the order of `names` (positions past the "
-             "declared variables belong to the reference's generated variables), or failed = true. "
-             "Bounds are int32; -2^31 / 2^31-1 are the store's infinities.")
-    json.dump({"about": about, "skipped": skipped, "cases": cases}, open(OUT, "w"), indent=1)
-    print(f"{len(cases)} cases, skipped {skipped}", file=sys.stderr)
+    about = (f"Models and expected stores of the reference's {rel} (its {domain} over interval "
+             "stores), extracted as data by tests/golden/make_fzn_kats.py: for each step, the FlatZinc "
+             f"text told, the reference's {domain} propagator count after it (nd), and the expected "
+             "store before and after the Gauss-Seidel fixpoint over the reference's variables in the "
+             "order of `names` (positions past the declared variables belong to the reference's "
+             "generated variables), or failed = true; ua and changed are the test's expected "
+             "is_extractable() and has_changed. Bounds are int32; -2^31 / 2^31-1 are the store's "
+             "infinities; \"bot\" is an empty interval.")
+    return {"about": about, "skipped": skipped, "cases": cases}
+
+
+def main():
+    for path, suite, out, domain in SOURCES:
+        data = extract(path, suite, domain)
+        json.dump(data, open(out, "w"), indent=1)
+        print(f"{os.path.basename(out)}: {len(data['cases'])} cases, skipped {data['skipped']}",
+              file=sys.stderr)
 
 
 if __name__ == "__main__":

@@ -14,9 +14,10 @@ import pytest
 
 import oracle_lib as ol
 import pcx
-from frontend_cases import load_cases, run_case
+from frontend_cases import load_cases, load_pc_cases, run_case
 
 CASES = load_cases()
+PC_CASES = load_pc_cases()
 
 
 class Oracle:
@@ -42,7 +43,69 @@ def test_reference_models_through_the_oracle(case):
     assert not errs, f"{case['name']} ({case['ref']}): " + "; ".join(errs)
 
 
+@pytest.mark.parametrize("case", PC_CASES, ids=lambda c: c["name"])
+def test_reference_pc_models_through_the_oracle(case):
+    """tests/pc_test.cpp's models (n-ary linear, XOR, clauses, element, abs,
+    reified and nested terms): PC's expected stores on every original
+    variable, failures included."""
+    errs, _ = run_case(case, Oracle())
+    assert not errs, f"{case['name']} ({case['ref']}): " + "; ".join(errs)
+
+
+class AskingOracle(Oracle):
+    """The oracle, also recording per step whether every ternary propagator is
+    entailed (PIR's is_extractable) and whether the fixpoint changed the store."""
+
+    def __init__(self):
+        super().__init__()
+        self.steps = []
+
+    def __call__(self, op, x, y, z, lb, ub, before):
+        op, x = op.copy(), x.copy()
+        ol.tell_embed(op, x, lb, ub)
+        self.t = [np.concatenate([a, b]) for a, b in zip(self.t, (op, x, y, z))]
+        t = [a.copy() for a in self.t]
+        ol.sort_table(*t)
+        before()
+        r = ol.fixpoint(*t, lb, ub)
+        ua = not r.failed and all(ol.ask(int(t[0][i]), int(t[1][i]), int(t[2][i]), int(t[3][i]), lb, ub)
+                                  for i in range(len(t[0])))
+        self.steps.append((ua, bool(r.has_changed)))
+        return bool(r.failed)
+
+
+def _flag_agreement(cases):
+    n = ua = ch = 0
+    for c in cases:
+        o = AskingOracle()
+        errs, _ = run_case(c, o)
+        assert not errs
+        for st, (u, h) in zip(c["steps"], o.steps):
+            if st["failed"]:
+                continue
+            n += 1
+            ua += st["ua"] == u
+            ch += st["changed"] == h
+    return n, ua, ch
+
+
+def test_extractable_and_changed_flags():
+    """is_extractable and has_changed through the decomposition.  For PIR's
+    tests every is_extractable flag is the reference's; has_changed differs
+    where the front-end's auxiliary variables start wider than ternarize's
+    (their first fixpoint tightens them).  PC asks its formula trees
+    directly (`-x <= 2` is entailed once x >= -2, with no defining `0 = x + t`
+    left to entail), so for PC's tests the flags are counted, not pinned."""
+    n, ua, ch = _flag_agreement(CASES)
+    print(f"PIR tests: is_extractable {ua}/{n}, has_changed {ch}/{n}")
+    assert ua == n and ch >= n - 11
+    n, ua, ch = _flag_agreement(PC_CASES)
+    print(f"PC tests: is_extractable {ua}/{n}, has_changed {ch}/{n}")
+    assert ua >= n - 10 and ch >= n - 11
+
+
 def test_fixture_covers_the_model_tests():
+    assert len(PC_CASES) >= 60
     assert len(CASES) >= 70
     assert sum(len(c["steps"]) for c in CASES) >= 100
     assert any(s["failed"] for c in CASES for s in c["steps"])

@@ -8,11 +8,12 @@ import pytest
 
 import oracle_lib as ol
 import pcx
-from frontend_cases import load_cases, run_case
+from frontend_cases import load_cases, load_pc_cases, run_case
 
 pytestmark = pytest.mark.gpu
 
 CASES = load_cases()
+PC_CASES = load_pc_cases()
 
 
 @pytest.fixture(scope="module")
@@ -35,6 +36,14 @@ class Engine:
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: c["name"])
 def test_reference_models_on_the_gpu(gpu, case):
+    errs, _ = run_case(case, Engine())
+    assert not errs, f"{case['name']} ({case['ref']}): " + "; ".join(errs)
+
+
+@pytest.mark.parametrize("case", PC_CASES, ids=lambda c: c["name"])
+def test_reference_pc_models_on_the_gpu(gpu, case):
+    """tests/pc_test.cpp's models, decomposed and propagated on the GPU: PC's
+    stores on every original variable, failures included."""
     errs, _ = run_case(case, Engine())
     assert not errs, f"{case['name']} ({case['ref']}): " + "; ".join(errs)
 
