@@ -524,6 +524,15 @@ int lockstep_of(const pcx_batch* b) {
   return g;
 }
 
+// A reset image that lives only in the interleaved copy is written out to
+// d_image before the copy is overwritten.
+pcx_status materialize_image(pcx_batch* b) {
+  if (!b->image_in_shadow) return PCX_OK;
+  PCX_HIP(launch_deinterleave(b->d_shadow, b->d_image, b->nvars, b->n_inst, b->shadow_gi, b->e->stream));
+  b->image_in_shadow = false;
+  return PCX_OK;
+}
+
 pcx_status ensure_shadow(pcx_batch* b, int gi) {
   const int64_t words = (b->n_inst + gi - 1) / gi * gi * b->nvars;
   if (b->shadow_words >= words) return PCX_OK;
@@ -612,13 +621,15 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
         ts = tm.begin(e->stream);
         if (ts != PCX_OK) return ts;
         if (gi) {                            // snapshot of the stores, then the sweep reading it
-          ts = ensure_shadow(b, gi);
-          if (ts == PCX_OK) ts = ensure_members(b);
-          if (ts != PCX_OK) return ts;
           // A run that starts from the image reuses the image's interleaved
           // copy (the sweep only reads it) until the next load.
           const bool from_image = b->run_from_image && sweeps == 0;
-          if (!(from_image && b->shadow_is_image && b->shadow_gi == gi)) {
+          const bool reuse = from_image && b->shadow_is_image && b->shadow_gi == gi;
+          if (!reuse) ts = materialize_image(b);
+          if (ts == PCX_OK) ts = ensure_shadow(b, gi);
+          if (ts == PCX_OK) ts = ensure_members(b);
+          if (ts != PCX_OK) return ts;
+          if (!reuse) {
             PCX_HIP(launch_interleave(b->d_store, b->d_shadow, b->nvars, b->n_inst, gi, e->stream));
             b->shadow_is_image = from_image;
             b->shadow_gi = gi;
@@ -938,6 +949,7 @@ pcx_status image_loaded(pcx_batch* b) {
   PCX_HIP(hipStreamSynchronize(e->stream));
   b->store_is_image = true;
   b->shadow_is_image = false;
+  b->image_in_shadow = false;
   return PCX_OK;
 }
 }  // namespace
@@ -965,17 +977,36 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
   PCX_HIP(hipSetDevice(e->cfg.device));
   const int64_t n = b->n_inst * b->nvars;
   PCX_HIP(hipMemsetAsync(b->d_image_empty, 0, b->n_inst, e->stream));
-  PCX_HIP(launch_ingest(d_lb, d_ub, b->d_image, b->d_store, n, b->nvars, b->d_image_empty, e->stream));
+  const int gi = lockstep_of(b);
+  if (gi > 0) {
+    // Batched runs sweep a lockstep copy first: write it in the same pass
+    // as the stores.  That copy is then also the reset image (d_image is
+    // written from it only if a run is about to overwrite it).
+    b->image_in_shadow = false;
+    pcx_status st = ensure_shadow(b, gi);
+    if (st != PCX_OK) return st;
+    PCX_HIP(launch_ingest_interleave(d_lb, d_ub, b->d_store, b->d_shadow, b->nvars, b->n_inst, gi,
+                                     b->d_image_empty, e->stream));
+    b->shadow_is_image = true;
+    b->shadow_gi = gi;
+    b->image_in_shadow = true;
+  } else {
+    PCX_HIP(launch_ingest(d_lb, d_ub, b->d_image, b->d_store, n, b->nvars, b->d_image_empty, e->stream));
+    b->shadow_is_image = false;
+    b->image_in_shadow = false;
+  }
   PCX_HIP(hipStreamSynchronize(e->stream));
   b->store_is_image = true;
-  b->shadow_is_image = false;
   return PCX_OK;
 }
 
 pcx_status pcx_batch_reset(pcx_batch* b) {
   if (!b) return fail_with(PCX_EINVAL, "NULL argument");
   pcx_engine* e = b->e;
-  PCX_HIP(launch_copy_words(b->d_image, b->d_store, b->n_inst * b->nvars, e->stream));
+  if (b->image_in_shadow)
+    PCX_HIP(launch_deinterleave(b->d_shadow, b->d_store, b->nvars, b->n_inst, b->shadow_gi, e->stream));
+  else
+    PCX_HIP(launch_copy_words(b->d_image, b->d_store, b->n_inst * b->nvars, e->stream));
   b->store_is_image = true;
   return PCX_OK;
 }

@@ -60,6 +60,8 @@ struct pcx_batch {
   int64_t shadow_words = 0;
   int shadow_gi = 0;                  // group size of its layout
   bool shadow_is_image = false;       // it holds the image (interleaved since the last load)
+  bool image_in_shadow = false;       // the reset image exists only as the interleaved copy
+                                      // (a device load); d_image is written when needed
   bool run_from_image = false;        // this run started from stores equal to the image
   bool events = false;                // this run uses marked-only sweeps
   bool seeded = false;                // the next run may start from seeded marks

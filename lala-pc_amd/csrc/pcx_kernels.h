@@ -162,6 +162,15 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
 // bitmaps of a subset batch (nullptr: every propagator).
 hipError_t launch_interleave(const int2* store, int2* shadow, int64_t nvars, int64_t n_inst, int gi,
                              hipStream_t st);
+// Device-resident lb / ub (n_inst * nvars each) into the stores and their
+// interleaved copy in one pass; empty[k] = 1 where instance k has lb > ub
+// (empty zeroed by the caller).
+hipError_t launch_ingest_interleave(const int32_t* lb, const int32_t* ub, int2* store, int2* shadow,
+                                    int64_t nvars, int64_t n_inst, int gi, uint8_t* empty,
+                                    hipStream_t st);
+// dst (n_inst * nvars words, row by row) from an interleaved copy
+hipError_t launch_deinterleave(const int2* shadow, int2* dst, int64_t nvars, int64_t n_inst, int gi,
+                               hipStream_t st);
 hipError_t launch_sweep_lockstep(const Table& t, int2* store, int2* shadow, int64_t nvars,
                                  int32_t n_inst, const uint32_t* member, int64_t words,
                                  InstFlags* flags, WorkList wl, unsigned long long* tight, int gi,

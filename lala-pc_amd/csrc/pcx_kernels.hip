@@ -942,6 +942,113 @@ hipError_t launch_interleave(const int2* store, int2* shadow, int64_t nvars, int
   return hipGetLastError();
 }
 
+// Stores loaded from device memory straight into the layout a lockstep sweep
+// reads: one pass reads lb / ub (once, non-temporal), writes the packed
+// stores row by row, the interleaved copy through the same LDS tile as
+// k_interleave, and the per-instance emptiness (empty zeroed first).
+template <int GI>
+__global__ __launch_bounds__(kBlock) void k_ingest_interleave(const int32_t* __restrict__ lb,
+                                                             const int32_t* __restrict__ ub,
+                                                             u64* __restrict__ store, u64* __restrict__ sh,
+                                                             int64_t nvars, int64_t n_inst,
+                                                             int64_t tiles_per_group,
+                                                             uint8_t* __restrict__ empty) {
+  constexpr int kTV = 4096 / GI;
+  __shared__ u64 tile[GI][kTV + 1];
+  const int64_t g = blockIdx.x / tiles_per_group;
+  const int64_t v0 = (blockIdx.x - g * tiles_per_group) * kTV;
+#pragma unroll
+  for (int r = 0; r < 4096 / kBlock; ++r) {
+    const int e = r * kBlock + threadIdx.x;
+    const int i = e / kTV, j = e - i * kTV;
+    const int64_t k = g * GI + i;
+    u64 val = 0;
+    if (k < n_inst && v0 + j < nvars) {
+      const int64_t at = k * nvars + v0 + j;
+      const int32_t l = __builtin_nontemporal_load(lb + at);
+      const int32_t u = __builtin_nontemporal_load(ub + at);
+      val = (u64)(uint32_t)l | ((u64)(uint32_t)u << 32);     // int2 {lb, ub}
+      store[at] = val;
+      if (l > u) empty[k] = 1;
+    }
+    tile[i][j] = val;
+  }
+  __syncthreads();
+  u64* out = sh + (g * nvars + v0) * GI;
+  const int64_t lim = min((int64_t)kTV, nvars - v0) * GI;
+#pragma unroll
+  for (int r = 0; r < 4096 / kBlock; ++r) {
+    const int e = r * kBlock + threadIdx.x;
+    if (e < lim) out[e] = tile[e % GI][e / GI];
+  }
+}
+
+// The inverse of k_interleave: rows of the stores from the interleaved copy
+// (a batch whose reset image lives only in the interleaved copy).
+template <int GI>
+__global__ __launch_bounds__(kBlock) void k_deinterleave(const u64* __restrict__ sh, u64* __restrict__ dst,
+                                                        int64_t nvars, int64_t n_inst,
+                                                        int64_t tiles_per_group) {
+  constexpr int kTV = 4096 / GI;
+  __shared__ u64 tile[GI][kTV + 1];
+  const int64_t g = blockIdx.x / tiles_per_group;
+  const int64_t v0 = (blockIdx.x - g * tiles_per_group) * kTV;
+  const u64* in = sh + (g * nvars + v0) * GI;
+  const int64_t lim = min((int64_t)kTV, nvars - v0) * GI;
+#pragma unroll
+  for (int r = 0; r < 4096 / kBlock; ++r) {
+    const int e = r * kBlock + threadIdx.x;
+    if (e < lim) tile[e % GI][e / GI] = in[e];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4096 / kBlock; ++r) {
+    const int e = r * kBlock + threadIdx.x;
+    const int i = e / kTV, j = e - i * kTV;
+    const int64_t k = g * GI + i;
+    if (k < n_inst && v0 + j < nvars) dst[k * nvars + v0 + j] = tile[i][j];
+  }
+}
+
+hipError_t launch_ingest_interleave(const int32_t* lb, const int32_t* ub, int2* store, int2* shadow,
+                                    int64_t nvars, int64_t n_inst, int gi, uint8_t* empty,
+                                    hipStream_t stream) {
+  if (nvars <= 0 || n_inst <= 0) return hipSuccess;
+  const int64_t groups = (n_inst + gi - 1) / gi;
+  const int64_t tv = 4096 / gi;
+  const int64_t tpg = (nvars + tv - 1) / tv;
+  const int64_t grid = groups * tpg;
+  if (grid > 0x7fffffff) return hipErrorInvalidValue;
+  u64* st = reinterpret_cast<u64*>(store);
+  u64* out = reinterpret_cast<u64*>(shadow);
+  switch (gi) {
+    case 16: k_ingest_interleave<16><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, out, nvars, n_inst, tpg, empty); break;
+    case 32: k_ingest_interleave<32><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, out, nvars, n_inst, tpg, empty); break;
+    case 64: k_ingest_interleave<64><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, out, nvars, n_inst, tpg, empty); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_deinterleave(const int2* shadow, int2* dst, int64_t nvars, int64_t n_inst, int gi,
+                               hipStream_t stream) {
+  if (nvars <= 0 || n_inst <= 0) return hipSuccess;
+  const int64_t groups = (n_inst + gi - 1) / gi;
+  const int64_t tv = 4096 / gi;
+  const int64_t tpg = (nvars + tv - 1) / tv;
+  const int64_t grid = groups * tpg;
+  if (grid > 0x7fffffff) return hipErrorInvalidValue;
+  const u64* in = reinterpret_cast<const u64*>(shadow);
+  u64* out = reinterpret_cast<u64*>(dst);
+  switch (gi) {
+    case 16: k_deinterleave<16><<<(unsigned)grid, kBlock, 0, stream>>>(in, out, nvars, n_inst, tpg); break;
+    case 32: k_deinterleave<32><<<(unsigned)grid, kBlock, 0, stream>>>(in, out, nvars, n_inst, tpg); break;
+    case 64: k_deinterleave<64><<<(unsigned)grid, kBlock, 0, stream>>>(in, out, nvars, n_inst, tpg); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // Tightened (instance, variable) pairs are deferred to an LDS list and
 // marked by the whole workgroup at the end of the work item; a pair that
 // does not fit marks its whole instance for the next sweep instead (its

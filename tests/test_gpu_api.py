@@ -220,3 +220,58 @@ def test_load_from_device_memory(gpu, nvars):
         ok = np.repeat(~np.array([bool(r.failed) for r in r_h]), nvars)
         assert np.array_equal(l_d[ok], l_h[ok]) and np.array_equal(u_d[ok], u_h[ok])
         dev.reset()
+
+
+@pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
+@pytest.mark.parametrize("gi", [16, 32, 64])
+def test_device_load_into_the_lockstep_copy(gpu, gi, events):
+    """A device load into a lockstep batch writes the stores and the
+    interleaved copy in one pass, and the reset image lives in that copy
+    until a run would overwrite it.  Every sequence -- reset right after the
+    load, run / run again without a reset / reset / run, a host load after
+    it -- gives the host-loaded batch's results; 131 instances leave a
+    partial last group."""
+    from pcx.device import DeviceArray
+    V, P, n = 700, 2800, 131
+    op, x, y, z, lb, ub, _ = pcx.gen_network(pcx.gen_params(seed=gi + events, nvars=V, nprops=P))
+    pir = pcx.PIR(lockstep=gi, event_driven=events)
+    pir.tell(op, x, y, z, lb, ub)
+    root_l, root_u = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_l, root_u).failed
+    lo, hi = pcx.gen_splits(gi, root_l, root_u, n, 1, 20)
+    off, idx = pcx.gen_subsets(gi + 1, P, n, 100, P)
+    lo[5 * V + 3], hi[5 * V + 3] = 9, 8                        # instance 5 starts empty
+    ref = pcx.Batch(pir, n, V, off, idx)
+    ref.load(lo, hi)
+    r_ref, st = ref.run(pcx.MODE_BATCHED)
+    assert st.lockstep == gi
+    l_ref, u_ref = ref.read()
+    fail = np.array([bool(r.failed) for r in r_ref])
+    ok = np.repeat(~fail, V)
+
+    def same(b, what):
+        res, _ = b.run(pcx.MODE_BATCHED)
+        lg, ug = b.read()
+        assert np.array_equal(np.array([bool(r.failed) for r in res]), fail), what
+        assert np.array_equal(lg[ok], l_ref[ok]) and np.array_equal(ug[ok], u_ref[ok]), what
+
+    d_lo, d_hi = DeviceArray(lo), DeviceArray(hi)
+    b = pcx.Batch(pir, n, V, off, idx)
+    b.load_device(d_lo.ptr, d_hi.ptr)
+    b.reset()                                                  # image from the interleaved copy
+    l0, u0 = b.read()
+    assert np.array_equal(l0, lo) and np.array_equal(u0, hi)
+    same(b, "after the load and a reset")
+    b.run(pcx.MODE_BATCHED)                                    # from the fixpoint: rewrites the copy
+    b.reset()
+    l0, u0 = b.read()
+    assert np.array_equal(l0, lo) and np.array_equal(u0, hi), "reset image lost"
+    same(b, "after a run without reset and a reset")
+    b.load_device(d_lo.ptr, d_hi.ptr)
+    same(b, "straight after a second device load")
+    b.load(lo, hi)                                             # host load: the image is d_image again
+    b.run(pcx.MODE_PLAIN)
+    b.reset()
+    same(b, "after a host load")
+    d_lo.close()
+    d_hi.close()
