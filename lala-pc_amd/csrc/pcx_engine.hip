@@ -50,6 +50,7 @@ bool valid_ppw(int p) {
 }
 
 bool valid_lockstep(int g) { return g == -1 || g == 0 || g == 16 || g == 32 || g == 64; }
+bool valid_mark_queue(int q) { return q == -1 || q == 0 || (q >= 10 && q <= 26); }
 
 template <class T>
 void dfree(T*& p) {
@@ -547,6 +548,30 @@ pcx_status ensure_shadow(pcx_batch* b, int gi) {
 // Batched mode: per sweep one scheduler launch and one grid-stride sweep
 // launch over the work items of every running instance (or, for full sweeps
 // in lockstep, over (group, table block) items).
+// Queue planes of a batch (queued marked sweeps), allocated on first use:
+// room for a sixteenth of the batch's (instance, propagator) pairs per plane,
+// between 2^16 and 2^23 entries; a plane that overflows falls back to bitmaps.
+pcx_status ensure_queue(pcx_batch* b) {
+  const int mq = b->e->cfg.mark_queue;
+  const int64_t pairs = b->n_inst * std::max<int64_t>(b->table_n, 1);
+  const int64_t cap = mq > 0 ? (int64_t)1 << mq
+                             : std::min<int64_t>(std::max<int64_t>(pairs / 16, (int64_t)1 << 16), (int64_t)1 << 23);
+  if (b->d_q && b->qcap == cap) return PCX_OK;
+  dfree(b->d_q);
+  dfree(b->d_qctl);
+  b->qcap = cap;
+  PCX_HIP(hipMalloc(&b->d_q, 2 * b->qcap * sizeof(unsigned long long)));
+  PCX_HIP(hipMalloc(&b->d_qctl, kQCtlWords * sizeof(int32_t)));
+  return PCX_OK;
+}
+
+// Batched mode: per sweep one scheduler launch and one grid-stride sweep
+// launch over the work items of every running instance (or, for full sweeps
+// in lockstep, over (group, table block) items).  After a lockstep full
+// sweep with marks, the marked sweeps run from queue planes: groups of
+// (turn, queue pass) launches replayed as one graph until a turn finds no
+// marks, or finds a plane that lost marks, from which sweep on the bitmap
+// sweeps take over.
 pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
   const int ppw = ppw_of(e);
@@ -560,7 +585,16 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   const WorkList wl = worklist_of(b->d_act, b->d_item_base, b->d_ctl);
   int32_t live = 1, sweeps = 0;
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
-  const EventSet ev = event_set(b);
+  EventSet ev = event_set(b);
+  const bool qmode = b->events && gi > 0 && !b->seeded_run && e->cfg.mark_queue != 0;
+  if (qmode) {
+    pcx_status qs = ensure_queue(b);
+    if (qs != PCX_OK) return qs;
+    ev.q = b->d_q;
+    ev.qctl = b->d_qctl;
+    ev.qcap = b->qcap;
+    PCX_HIP(hipMemsetAsync(b->d_qctl, 0, kQCtlWords * sizeof(int32_t), e->stream));
+  }
   auto launch_marked = [&]() -> pcx_status {
     PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
                             &ev));
@@ -568,26 +602,95 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
                                 b->d_tight + kTightSlots, ev, dom_of(e), grid, e->stream));
     return PCX_OK;
   };
-  // Groups after the first hold only marked sweeps, whose launches have the
-  // same arguments every time (the sweep number lives on the device): they
-  // are captured once into a graph and replayed while every argument the
-  // capture used is unchanged (the batch's own buffers never move).
-  const pcx_batch::MarkedGraphArgs args{e->version, group, dom_of(e), maxs, ppw,
-                                        e->d_xo, e->d_y, e->d_z, e->d_adj_off, e->d_adj};
-  if (b->events && (!b->marked_graph || !(b->marked_graph_args == args))) {
-    if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
-    b->marked_graph = nullptr;
+  auto launch_queued = [&]() -> pcx_status {
+    PCX_HIP(launch_queue_turn(b->d_qctl, maxs, e->stream));
+    PCX_HIP(launch_sweep_queue(table_of(e), b->d_store, b->nvars, b->d_flags, b->d_tight + kTightSlots, ev,
+                               dom_of(e), grid, e->stream));
+    return PCX_OK;
+  };
+  // A full sweep (scheduled first): lockstep over the interleaved copy, or
+  // per-instance work items.
+  auto launch_full = [&]() -> pcx_status {
+    PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
+                            nullptr));
+    pcx_status ts = tm.begin(e->stream);
+    if (ts != PCX_OK) return ts;
+    if (gi) {                            // snapshot of the stores, then the sweep reading it
+      // A run that starts from the image reuses the image's interleaved
+      // copy (the sweep only reads it) until the next load.
+      const bool from_image = b->run_from_image && sweeps == 0;
+      const bool reuse = from_image && b->shadow_is_image && b->shadow_gi == gi;
+      if (!reuse) ts = materialize_image(b);
+      if (ts == PCX_OK) ts = ensure_shadow(b, gi);
+      if (ts == PCX_OK) ts = ensure_members(b);
+      if (ts != PCX_OK) return ts;
+      if (!reuse) {
+        PCX_HIP(launch_interleave(b->d_store, b->d_shadow, b->nvars, b->n_inst, gi, e->stream));
+        b->shadow_is_image = from_image;
+        b->shadow_gi = gi;
+      }
+      PCX_HIP(launch_sweep_lockstep(table_of(e), b->d_store, b->d_shadow, b->nvars,
+                                    (int32_t)b->n_inst, b->h_off.empty() ? nullptr : b->d_member,
+                                    bitmap_words(b), b->d_flags, wl, b->d_tight, gi, dom_of(e),
+                                    grid, e->stream, b->events ? &ev : nullptr));
+    } else {
+      PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
+                                   b->d_tight, ppw, dom_of(e), b->stream, grid, e->stream,
+                                   b->events ? &ev : nullptr));
+    }
+    return tm.end(e->stream, true);
+  };
+  // Groups of marked sweeps have the same launch arguments every time (the
+  // sweep number lives on the device): each kind is captured once into a
+  // graph and replayed while every argument the capture used is unchanged
+  // (the batch's own buffers never move).
+  auto capture = [&](hipGraphExec_t& gx, pcx_batch::MarkedGraphArgs& have, auto&& body) -> pcx_status {
+    const pcx_batch::MarkedGraphArgs args{e->version, group, dom_of(e), maxs, ppw,
+                                          e->d_xo, e->d_y, e->d_z, e->d_adj_off, e->d_adj, ev.q};
+    if (gx && have == args) return PCX_OK;
+    if (gx) (void)hipGraphExecDestroy(gx);
+    gx = nullptr;
     hipGraph_t graph = nullptr;
     PCX_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     pcx_status cs = PCX_OK;
-    for (int g = 0; g < group && cs == PCX_OK; ++g) cs = launch_marked();
+    for (int g = 0; g < group && cs == PCX_OK; ++g) cs = body();
     const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
     if (cs != PCX_OK) { if (graph) (void)hipGraphDestroy(graph); return cs; }
     PCX_HIP(ce);
-    const hipError_t ie = hipGraphInstantiate(&b->marked_graph, graph, nullptr, nullptr, 0);
+    const hipError_t ie = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     PCX_HIP(ie);
-    b->marked_graph_args = args;
+    have = args;
+    return PCX_OK;
+  };
+  if (b->events) {
+    pcx_status cs = capture(b->marked_graph, b->marked_graph_args, launch_marked);
+    if (cs == PCX_OK && qmode) cs = capture(b->queue_graph, b->queue_graph_args, launch_queued);
+    if (cs != PCX_OK) return cs;
+  }
+  if (qmode) {
+    pcx_status ts = launch_full();
+    if (ts != PCX_OK) return ts;
+    sweeps = 1;
+    for (;;) {
+      ts = tm.begin(e->stream);
+      if (ts != PCX_OK) return ts;
+      PCX_HIP(hipGraphLaunch(b->queue_graph, e->stream));
+      ts = tm.end(e->stream, false, group);
+      if (ts != PCX_OK) return ts;
+      int32_t q[kQCtlWords];
+      ts = read_small(b, b->d_qctl, sizeof(q), q);
+      if (ts != PCX_OK) return ts;
+      ts = tm.drain();
+      if (ts != PCX_OK) return ts;
+      if (q[kQStop] == 1) { live = 0; break; }
+      if (q[kQStop] == 2) {              // bitmap sweeps from sweep q[kQAt] on
+        sweeps = q[kQAt];
+        const int32_t prev = sweeps - 1;
+        PCX_HIP(hipMemcpyAsync(wl.sweep, &prev, sizeof(prev), hipMemcpyHostToDevice, e->stream));
+        break;
+      }
+    }
   }
   while (live > 0) {
     if (b->events && sweeps >= group) {     // groups after the first: all marked sweeps
@@ -609,43 +712,14 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     for (int g = 0; g < group; ++g) {
       const bool marked = b->events && (sweeps > 0 || b->seeded_run);
       pcx_status ts = PCX_OK;
-      if (marked && n_marked == 0) ts = tm.begin(e->stream);
-      if (ts != PCX_OK) return ts;
-      PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
-                              marked ? &ev : nullptr));
       if (marked) {
-        PCX_HIP(launch_sweep_marked(table_of(e), b->d_store, b->nvars, b->d_flags, wl,
-                                    b->d_tight + kTightSlots, ev, dom_of(e), grid, e->stream));
+        if (n_marked == 0) ts = tm.begin(e->stream);
+        if (ts == PCX_OK) ts = launch_marked();
         ++n_marked;
       } else {
-        ts = tm.begin(e->stream);
-        if (ts != PCX_OK) return ts;
-        if (gi) {                            // snapshot of the stores, then the sweep reading it
-          // A run that starts from the image reuses the image's interleaved
-          // copy (the sweep only reads it) until the next load.
-          const bool from_image = b->run_from_image && sweeps == 0;
-          const bool reuse = from_image && b->shadow_is_image && b->shadow_gi == gi;
-          if (!reuse) ts = materialize_image(b);
-          if (ts == PCX_OK) ts = ensure_shadow(b, gi);
-          if (ts == PCX_OK) ts = ensure_members(b);
-          if (ts != PCX_OK) return ts;
-          if (!reuse) {
-            PCX_HIP(launch_interleave(b->d_store, b->d_shadow, b->nvars, b->n_inst, gi, e->stream));
-            b->shadow_is_image = from_image;
-            b->shadow_gi = gi;
-          }
-          PCX_HIP(launch_sweep_lockstep(table_of(e), b->d_store, b->d_shadow, b->nvars,
-                                        (int32_t)b->n_inst, b->h_off.empty() ? nullptr : b->d_member,
-                                        bitmap_words(b), b->d_flags, wl, b->d_tight, gi, dom_of(e),
-                                        grid, e->stream, b->events ? &ev : nullptr));
-        } else {
-          PCX_HIP(launch_sweep_batched(t, b->d_store, b->nvars, sub, b->table_n, b->d_flags, wl,
-                                       b->d_tight, ppw, dom_of(e), b->stream, grid, e->stream,
-                                       b->events ? &ev : nullptr));
-        }
-        ts = tm.end(e->stream, true);
-        if (ts != PCX_OK) return ts;
+        ts = launch_full();
       }
+      if (ts != PCX_OK) return ts;
       ++sweeps;
     }
     if (n_marked > 0) {
@@ -713,6 +787,7 @@ void pcx_config_default(pcx_config* c) {
   c->stream_tables = -1;
   c->event_driven = -1;
   c->lockstep = -1;
+  c->mark_queue = -1;
 }
 
 int32_t pcx_device_count(void) {
@@ -730,6 +805,8 @@ pcx_status pcx_create(const pcx_config* cfg, pcx_engine** out) {
     return fail_with(PCX_EINVAL, "unknown domain");
   if (c.props_per_wave != 0 && !valid_ppw(c.props_per_wave))
     return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto) or a power of two in [64, 2048]");
+  if (!valid_lockstep(c.lockstep) || !valid_mark_queue(c.mark_queue))
+    return fail_with(PCX_EINVAL, "lockstep or mark_queue out of range");
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail_with(PCX_EDEVICE, "no HIP device");
   if (c.device < 0 || c.device >= n) return fail_with(PCX_EINVAL, "device ordinal out of range");
@@ -826,6 +903,8 @@ pcx_status pcx_set_config(pcx_engine* e, const pcx_config* cfg) {
     return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto) or a power of two in [64, 2048]");
   if (!valid_lockstep(cfg->lockstep))
     return fail_with(PCX_EINVAL, "lockstep must be -1 (auto), 0 (off), 16, 32 or 64");
+  if (!valid_mark_queue(cfg->mark_queue))
+    return fail_with(PCX_EINVAL, "mark_queue must be -1 (auto), 0 (off) or a log2 capacity in [10, 26]");
   e->cfg = *cfg;
   return PCX_OK;
 }
@@ -936,6 +1015,8 @@ void pcx_batch_destroy(pcx_batch* b) {
   dfree(b->d_summary); dfree(b->d_image_empty); dfree(b->d_unpack); dfree(b->d_shadow);
   if (b->h_pin) (void)hipHostFree(b->h_pin);
   if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
+  if (b->queue_graph) (void)hipGraphExecDestroy(b->queue_graph);
+  dfree(b->d_q); dfree(b->d_qctl);
   delete b;
 }
 

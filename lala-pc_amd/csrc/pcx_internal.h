@@ -71,14 +71,22 @@ struct pcx_batch {
   struct MarkedGraphArgs {
     uint64_t version;                 // engine table version
     int32_t group, dom, max_sweeps, ppw;
-    const void *xo, *y, *z, *adj_off, *adj;
+    const void *xo, *y, *z, *adj_off, *adj, *q;
     bool operator==(const MarkedGraphArgs& o) const {
       return version == o.version && group == o.group && dom == o.dom && max_sweeps == o.max_sweeps &&
-             ppw == o.ppw && xo == o.xo && y == o.y && z == o.z && adj_off == o.adj_off && adj == o.adj;
+             ppw == o.ppw && xo == o.xo && y == o.y && z == o.z && adj_off == o.adj_off && adj == o.adj &&
+             q == o.q;
     }
   };
   hipGraphExec_t marked_graph = nullptr;
   MarkedGraphArgs marked_graph_args{};
+  // queued marked sweeps: two planes of qcap entries, control words, and the
+  // captured group of (turn, queue pass) launches
+  unsigned long long* d_q = nullptr;
+  int32_t* d_qctl = nullptr;
+  int64_t qcap = 0;
+  hipGraphExec_t queue_graph = nullptr;
+  MarkedGraphArgs queue_graph_args{};
   pcx::InstFlags* d_flags = nullptr;
   unsigned long long* d_tight = nullptr;
   int32_t* d_act = nullptr;

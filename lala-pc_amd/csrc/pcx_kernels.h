@@ -73,7 +73,23 @@ struct EventSet {
   int64_t words;
   int64_t swords;
   int64_t n_inst;
+  // Queued marks (batched runs after a lockstep full sweep; q == nullptr:
+  // none).  Every mark that sets a clear bit of sweep s's bitmap is also
+  // appended to queue plane s & 1 as (instance << 32 | propagator), so a
+  // marked sweep is one flat pass over its plane; the bitmaps stay complete,
+  // so a plane that overflowed (or an instance whose marks went to its
+  // all_marked word) is swept from the bitmaps instead.  qctl: kQ* words.
+  unsigned long long* q;    // [2 * qcap]
+  int32_t* qctl;
+  int64_t qcap;
 };
+// Queue control words.
+constexpr int kQTail = 0;       // [2] entries appended to each plane (may pass qcap)
+constexpr int kQOver = 2;       // [2] the plane lost marks: its sweep needs the bitmaps
+constexpr int kQSweep = 4;      // the sweep the next queue pass runs
+constexpr int kQStop = 5;       // 1: nothing marked (fixpoint), 2: bitmaps needed from kQAt
+constexpr int kQAt = 6;         // the sweep at which the queue passes stopped
+constexpr int kQCtlWords = 8;
 
 // Per-instance propagator subset: prop_idx[off[k] .. off[k+1]) of the shared
 // table, the whole table when off == nullptr, or -- off without idx -- the
@@ -148,6 +164,12 @@ hipError_t launch_sweep_marked(const Table& t, int2* store, int64_t nvars, InstF
 // member bitmaps from subsets
 hipError_t launch_build_members(const int64_t* off, const int32_t* idx, int64_t n_inst,
                                 uint32_t* member, int64_t words, hipStream_t st);
+// Queued marked sweeps: one turn (advances kQSweep, or stops) and one flat
+// pass over the plane of that sweep; both return at once after a stop.
+hipError_t launch_queue_turn(int32_t* qctl, int32_t max_sweeps, hipStream_t st);
+hipError_t launch_sweep_queue(const Table& t, int2* store, int64_t nvars, InstFlags* flags,
+                              unsigned long long* tight, const EventSet& ev, int dom, int grid,
+                              hipStream_t st);
 // marked != nullptr: schedule a marked sweep (items = chunks with marks).
 hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags, int32_t n_inst,
                            WorkList wl, int ppw, int32_t max_sweeps, hipStream_t st,

@@ -189,6 +189,77 @@ __device__ __forceinline__ void mark_var(const Marker& mk, int32_t v) {
   }
 }
 
+// Queued marks are staged per workgroup in LDS and reserved in their plane
+// with one atomic per flush (a single tail counter per plane would otherwise
+// take an atomic per wave from every workgroup of the chip).  Entries that
+// find the staging area full take a slot of their own.
+constexpr int kQStage = 1024;
+struct QueueStage {
+  unsigned long long* buf;   // LDS, kQStage entries
+  int32_t* n;                // LDS counter
+};
+
+__device__ __forceinline__ void stage_entry(const QueueStage& qs, const EventSet& ev, int pl,
+                                            unsigned long long entry) {
+  const int at = atomicAdd(qs.n, 1);
+  if (at < kQStage) { qs.buf[at] = entry; return; }
+  const int64_t g = atomicAdd(ev.qctl + kQTail + pl, 1);
+  if (g < ev.qcap) ev.q[(int64_t)pl * ev.qcap + g] = entry;
+  else ev.qctl[kQOver + pl] = 1;
+}
+
+// Every thread of the workgroup calls this after a flush of marks; the
+// caller's barrier orders the staging before it.
+__device__ __forceinline__ void flush_stage(const QueueStage& qs, const EventSet& ev, int pl) {
+  __shared__ int32_t s_base;
+  const int n = min(*qs.n, kQStage);
+  if (threadIdx.x == 0) s_base = n ? atomicAdd(ev.qctl + kQTail + pl, n) : 0;
+  __syncthreads();
+  const int64_t base = s_base;
+  unsigned long long* plane = ev.q + (int64_t)pl * ev.qcap;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    if (base + i < ev.qcap) plane[base + i] = qs.buf[i];
+    else ev.qctl[kQOver + pl] = 1;
+  }
+  __syncthreads();                     // every thread has read the count and the entries
+  if (threadIdx.x == 0) *qs.n = 0;
+}
+
+// mark_var for queued marks: a propagator whose bit this call sets is
+// staged for the next sweep's plane (a returning atomic tells; a plain load
+// that already shows the bit saves the atomic, since bits of the next sweep
+// are only set during this one).  Summary bits are kept for a sweep that
+// falls back to the bitmaps.
+__device__ __forceinline__ void mark_var_q(const Marker& mk, const EventSet& ev, const QueueStage& qs,
+                                           int64_t k, int32_t v, int32_t next) {
+  constexpr int kB = 8;
+  const int pl = next & 1;
+  const int32_t e = mk.adj_off[v + 1];
+  for (int32_t i = mk.adj_off[v]; i < e; i += kB) {
+    int32_t p[kB];
+#pragma unroll
+    for (int j = 0; j < kB; ++j) p[j] = i + j < e ? mk.adj[i + j] : -1;
+    uint32_t in[kB];
+#pragma unroll
+    for (int j = 0; j < kB; ++j)
+      in[j] = (p[j] >= 0 && mk.member) ? mk.member[p[j] >> 5] : ~0u;
+    uint32_t seen[kB];
+#pragma unroll
+    for (int j = 0; j < kB; ++j)
+      seen[j] = p[j] >= 0 ? __hip_atomic_load(&mk.dirty[p[j] >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                          : ~0u;
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      if (p[j] < 0) continue;
+      const uint32_t bit = 1u << (p[j] & 31);
+      if (!(in[j] & bit) || (seen[j] & bit)) continue;
+      if (atomicOr(&mk.dirty[p[j] >> 5], bit) & bit) continue;
+      if (mk.sum) set_summary(nullptr, mk.sum, p[j] >> kChunkShift);
+      stage_entry(qs, ev, pl, ((unsigned long long)k << 32) | (uint32_t)p[j]);
+    }
+  }
+}
+
 // Every marking kernel gives its workgroup an LDS list (mk.list).  When the
 // list is full the variable is not recorded; the flush then marks every
 // propagator of the instance for the next sweep instead (a superset of the
@@ -654,6 +725,11 @@ __global__ __launch_bounds__(kSchedThreads) void k_schedule(Subsets sub, int64_t
   const int32_t s = *wl.sweep + 1;       // the sweep being scheduled
   const int tid = threadIdx.x;
   if (tid < kXcds) wl.xq[tid] = 0;
+  if (*wl.live == 0) {                   // every instance retired earlier: n_act is already 0
+    __syncthreads();
+    if (tid == 0) *wl.sweep = s;
+    return;
+  }
   const int per = (n_inst + kSchedThreads - 1) / kSchedThreads;
   const int lo = min(n_inst, tid * per), hi = min(n_inst, lo + per);
   int64_t items = 0;
@@ -1010,6 +1086,60 @@ __global__ __launch_bounds__(kBlock) void k_deinterleave(const u64* __restrict__
   }
 }
 
+typedef int32_t v4i __attribute__((ext_vector_type(4)));
+
+// The same with 16-byte accesses: four variables of one instance per thread
+// on the way in (nvars % 4 == 0, 16-byte aligned lb / ub), two slots of the
+// interleaved copy per thread on the way out.
+template <int GI>
+__global__ __launch_bounds__(kBlock) void k_ingest_interleave4(const v4i* __restrict__ lb,
+                                                              const v4i* __restrict__ ub,
+                                                              v4i* __restrict__ store, v4i* __restrict__ sh,
+                                                              int64_t nvars, int64_t n_inst,
+                                                              int64_t tiles_per_group,
+                                                              uint8_t* __restrict__ empty) {
+  constexpr int kTV = 4096 / GI;
+  constexpr int kQ = kTV / 4;                  // 4-variable groups per tile row
+  __shared__ u64 tile[GI][kTV + 1];
+  const int64_t g = blockIdx.x / tiles_per_group;
+  const int64_t v0 = (blockIdx.x - g * tiles_per_group) * kTV;
+#pragma unroll
+  for (int r = 0; r < 1024 / kBlock; ++r) {
+    const int e = r * kBlock + threadIdx.x;
+    const int i = e / kQ, j = (e - i * kQ) * 4;
+    const int64_t k = g * GI + i;
+    u64 w[4] = {0, 0, 0, 0};
+    if (k < n_inst && v0 + j < nvars) {
+      const int64_t at4 = (k * nvars + v0 + j) / 4;
+      const v4i a = __builtin_nontemporal_load(lb + at4);
+      const v4i b = __builtin_nontemporal_load(ub + at4);
+      const v4i p0 = {a.x, b.x, a.y, b.y};
+      const v4i p1 = {a.z, b.z, a.w, b.w};
+      store[2 * at4] = p0;
+      store[2 * at4 + 1] = p1;
+      if ((a.x > b.x) | (a.y > b.y) | (a.z > b.z) | (a.w > b.w)) empty[k] = 1;
+      w[0] = (u64)(uint32_t)a.x | ((u64)(uint32_t)b.x << 32);
+      w[1] = (u64)(uint32_t)a.y | ((u64)(uint32_t)b.y << 32);
+      w[2] = (u64)(uint32_t)a.z | ((u64)(uint32_t)b.z << 32);
+      w[3] = (u64)(uint32_t)a.w | ((u64)(uint32_t)b.w << 32);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) tile[i][j + q] = w[q];
+  }
+  __syncthreads();
+  u64* out = reinterpret_cast<u64*>(sh) + (g * nvars + v0) * GI;
+  const int64_t lim = min((int64_t)kTV, nvars - v0) * GI;      // even: GI is
+#pragma unroll
+  for (int r = 0; r < 2048 / kBlock; ++r) {
+    const int e = 2 * (r * kBlock + threadIdx.x);
+    if (e < lim) {
+      const u64 lo = tile[e % GI][e / GI], hi = tile[(e + 1) % GI][(e + 1) / GI];
+      const v4i v = {(int32_t)lo, (int32_t)(lo >> 32), (int32_t)hi, (int32_t)(hi >> 32)};
+      *reinterpret_cast<v4i*>(out + e) = v;
+    }
+  }
+}
+
 hipError_t launch_ingest_interleave(const int32_t* lb, const int32_t* ub, int2* store, int2* shadow,
                                     int64_t nvars, int64_t n_inst, int gi, uint8_t* empty,
                                     hipStream_t stream) {
@@ -1019,6 +1149,22 @@ hipError_t launch_ingest_interleave(const int32_t* lb, const int32_t* ub, int2* 
   const int64_t tpg = (nvars + tv - 1) / tv;
   const int64_t grid = groups * tpg;
   if (grid > 0x7fffffff) return hipErrorInvalidValue;
+  const bool vec = nvars % 4 == 0 &&
+                   ((reinterpret_cast<uintptr_t>(lb) | reinterpret_cast<uintptr_t>(ub) |
+                     reinterpret_cast<uintptr_t>(store) | reinterpret_cast<uintptr_t>(shadow)) & 15) == 0;
+  if (vec) {
+    const v4i* l4 = reinterpret_cast<const v4i*>(lb);
+    const v4i* u4 = reinterpret_cast<const v4i*>(ub);
+    v4i* s4 = reinterpret_cast<v4i*>(store);
+    v4i* o4 = reinterpret_cast<v4i*>(shadow);
+    switch (gi) {
+      case 16: k_ingest_interleave4<16><<<(unsigned)grid, kBlock, 0, stream>>>(l4, u4, s4, o4, nvars, n_inst, tpg, empty); break;
+      case 32: k_ingest_interleave4<32><<<(unsigned)grid, kBlock, 0, stream>>>(l4, u4, s4, o4, nvars, n_inst, tpg, empty); break;
+      case 64: k_ingest_interleave4<64><<<(unsigned)grid, kBlock, 0, stream>>>(l4, u4, s4, o4, nvars, n_inst, tpg, empty); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   u64* st = reinterpret_cast<u64*>(store);
   u64* out = reinterpret_cast<u64*>(shadow);
   switch (gi) {
@@ -1089,7 +1235,10 @@ __global__ __launch_bounds__(kBlock, lock_min_waves<D>()) void k_sweep_lockstep(
   __shared__ int32_t s_next;
   __shared__ int32_t s_n;
   __shared__ int32_t s_mk[kLockMarkCap], s_mv[kLockMarkCap];
-  if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; }
+  __shared__ unsigned long long s_qbuf[kQStage];
+  __shared__ int32_t s_qn;
+  const QueueStage qs{s_qbuf, &s_qn};
+  if (threadIdx.x == 0) { s_next = atomicAdd(qc, 1); s_n = 0; s_qn = 0; }
   __syncthreads();
   int64_t item = first_item + __builtin_amdgcn_readfirstlane(s_next);
   __syncthreads();
@@ -1117,7 +1266,11 @@ __global__ __launch_bounds__(kBlock, lock_min_waves<D>()) void k_sweep_lockstep(
       if (am) return;
       const int i = atomicAdd(&s_n, 1);
       if (i < kLockMarkCap) { s_mk[i] = (int32_t)k; s_mv[i] = v; }
-      else { f->all_marked[(s + 1) & 1] = s + 1; am = true; }
+      else {
+        f->all_marked[(s + 1) & 1] = s + 1;
+        am = true;
+        if (ev.q) ev.qctl[kQOver + ((s + 1) & 1)] = 1;   // these marks are in no queue
+      }
     };
     // The member word and the table entries of the next word are loaded
     // while this word is swept.
@@ -1271,13 +1424,153 @@ __global__ __launch_bounds__(kBlock, lock_min_waves<D>()) void k_sweep_lockstep(
       atomicAdd(&tight[(blockIdx.x * 4 + wave) & (kTightSlots - 1)], (unsigned long long)sum);
     __syncthreads();                            // the item's marks are listed
     const int n = min(s_n, kLockMarkCap);
-    for (int i = threadIdx.x; i < n; i += kBlock) mark_var(marker_for(&ev, s_mk[i], s), s_mv[i]);
+    if (ev.q) {
+      for (int i = threadIdx.x; i < n; i += kBlock)
+        mark_var_q(marker_for(&ev, s_mk[i], s), ev, qs, s_mk[i], s_mv[i], s + 1);
+      __syncthreads();                          // the item's new marks are staged
+      flush_stage(qs, ev, (s + 1) & 1);
+    } else {
+      for (int i = threadIdx.x; i < n; i += kBlock) mark_var(marker_for(&ev, s_mk[i], s), s_mv[i]);
+    }
     __syncthreads();                            // every thread has read s_n and the list
     if (threadIdx.x == 0) { s_next = nxt; s_n = 0; }
     __syncthreads();
     item = first_item + __builtin_amdgcn_readfirstlane(s_next);
     __syncthreads();
   }
+}
+
+// ---- queued marked sweeps --------------------------------------------------
+// After a lockstep full sweep of a batch, the marks of each sweep are also
+// listed in a queue plane (mark_var_q).  A turn checks the plane of the next
+// sweep: empty means no propagator of any instance is marked, which by the
+// marked-sweep invariant is every instance's fixpoint; a plane that lost
+// marks stops the queue passes (the host resumes with bitmap sweeps); else
+// the turn clears the other plane for the sweep's own marks and advances.
+__global__ void k_queue_turn(int32_t* qctl, int32_t max_sweeps) {
+  if (threadIdx.x != 0 || qctl[kQStop]) return;
+  const int32_t s = qctl[kQSweep] + 1;
+  const int cu = s & 1;
+  if (qctl[kQOver + cu]) { qctl[kQStop] = 2; qctl[kQAt] = s; return; }
+  if (qctl[kQTail + cu] == 0 || (max_sweeps > 0 && s >= max_sweeps)) {
+    qctl[kQStop] = 1;
+    qctl[kQAt] = s;
+    return;
+  }
+  qctl[kQTail + (cu ^ 1)] = 0;
+  qctl[kQOver + (cu ^ 1)] = 0;
+  qctl[kQSweep] = s;
+}
+
+hipError_t launch_queue_turn(int32_t* qctl, int32_t max_sweeps, hipStream_t stream) {
+  k_queue_turn<<<1, 64, 0, stream>>>(qctl, max_sweeps);
+  return hipGetLastError();
+}
+
+// One marked sweep over its queue plane: thread i takes entry i, clears its
+// bit, evaluates the propagator on the instance's store and lists what it
+// tightened; the workgroup then marks those variables' adjacency (queued)
+// for the next sweep.  Entries of failed instances are skipped.
+constexpr int kQueueList = 3 * kBlock;     // at most three tightened variables per entry
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sweep_queue(Table t, int2* __restrict__ store, int64_t nvars,
+                                                      InstFlags* flags, unsigned long long* tight,
+                                                      EventSet ev) {
+  const int32_t* qctl = ev.qctl;
+  if (qctl[kQStop]) return;
+  const int32_t s = qctl[kQSweep];
+  const int cu = s & 1;
+  const int64_t n = min((int64_t)qctl[kQTail + cu], ev.qcap);
+  const unsigned long long* plane = ev.q + (int64_t)cu * ev.qcap;
+  uint32_t* cur = ev.dirty + (int64_t)cu * ev.n_inst * ev.words;
+  __shared__ int32_t s_k[kQueueList], s_v[kQueueList];
+  __shared__ int32_t s_n;
+  __shared__ unsigned long long s_qbuf[kQStage];
+  __shared__ int32_t s_qn;
+  const QueueStage qs{s_qbuf, &s_qn};
+  if (threadIdx.x == 0) { s_n = 0; s_qn = 0; }
+  __syncthreads();
+  int nt_all = 0;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = base + threadIdx.x;
+    if (i < n) {
+      const unsigned long long en = plane[i];
+      const int32_t k = (int32_t)(en >> 32), p = (int32_t)(uint32_t)en;
+      InstFlags* f = flags + k;
+      if (f->failed_at == kNoFail) {
+        atomicAnd(&cur[(int64_t)k * ev.words + (p >> 5)], ~(1u << (p & 31)));
+        int2* st = store + (int64_t)k * nvars;
+        const uint32_t xo = t.xo[p];
+        const int32_t y = t.y[p], z = t.z[p];
+        const int32_t x = (int32_t)(xo & kVarMask);
+        const u64 wx = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + x);
+        const u64 wy = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + y);
+        const u64 wz = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + z);
+        bool chg = false, fail = false, tx = false, ty = false, tz = false;
+        int nt = 0;
+        if constexpr (D == kBits) {
+          u64* ws = reinterpret_cast<u64*>(st);
+          if (!wx || !wy || !wz) {
+            fail = true;
+          } else {
+            u64 nx, ny, nz;
+            deduce_bits((int)(xo >> kOpShift), wx, wy, wz, nx, ny, nz);
+            tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(ws, x, wx, nx, chg, fail, nt);
+            ty = meet_word<__HIP_MEMORY_SCOPE_AGENT>(ws, y, wy, ny, chg, fail, nt);
+            tz = meet_word<__HIP_MEMORY_SCOPE_AGENT>(ws, z, wz, nz, chg, fail, nt);
+          }
+        } else {
+          const int2 X = as_itv(wx), Y = as_itv(wy), Z = as_itv(wz);
+          if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) {
+            fail = true;
+          } else {
+            int2 nx, ny, nz;
+            deduce<D == kItvDiv>((int)(xo >> kOpShift), X, Y, Z, nx, ny, nz);
+            if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
+            tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(st, x, X, nx, chg, fail, nt);
+            ty = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(st, y, Y, ny, chg, fail, nt);
+            tz = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(st, z, Z, nz, chg, fail, nt);
+          }
+        }
+        const int32_t vs[3] = {x, y, z};
+        const bool ts[3] = {tx, ty, tz};
+        for (int j = 0; j < 3; ++j)
+          if (ts[j]) {
+            const int at = atomicAdd(&s_n, 1);     // < kQueueList: three per entry at most
+            s_k[at] = k;
+            s_v[at] = vs[j];
+          }
+        if (chg) f->changed[s & 1] = s;
+        if (fail) f->failed_at = s;
+        atomicAdd((unsigned long long*)&f->marked_evals, 1ull);
+        nt_all += nt;
+      }
+    }
+    __syncthreads();                               // the pass's tightened variables are listed
+    const int nl = s_n;
+    for (int j = threadIdx.x; j < nl; j += kBlock) {
+      const int64_t k = s_k[j];
+      if (flags[k].failed_at == kNoFail) mark_var_q(marker_for(&ev, k, s), ev, qs, k, s_v[j], s + 1);
+    }
+    __syncthreads();                               // every thread has read the list; marks staged
+    if (threadIdx.x == 0) s_n = 0;
+    flush_stage(qs, ev, (s + 1) & 1);              // ends with a barrier
+  }
+  const int sum = wave_sum(nt_all);
+  if (lane_id() == 0 && sum)
+    atomicAdd(&tight[(blockIdx.x * 4 + (threadIdx.x >> 6)) & (kTightSlots - 1)], (unsigned long long)sum);
+}
+
+hipError_t launch_sweep_queue(const Table& t, int2* store, int64_t nvars, InstFlags* flags,
+                              unsigned long long* tight, const EventSet& ev, int dom, int grid,
+                              hipStream_t stream) {
+  switch (dom) {
+    case kItv: k_sweep_queue<kItv><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, tight, ev); break;
+    case kItvDiv: k_sweep_queue<kItvDiv><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, tight, ev); break;
+    case kBits: k_sweep_queue<kBits><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, tight, ev); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_sweep_lockstep(const Table& t, int2* store, int2* shadow, int64_t nvars,
@@ -2401,7 +2694,6 @@ hipError_t launch_copy_words(const void* src, void* dst, int64_t n, hipStream_t 
 // empty[k] = 1 when instance k holds some lb > ub (empty zeroed first), in
 // one pass.  Four variables per thread and step: two 16-byte loads, four
 // 16-byte stores; the source is read once, so its loads are non-temporal.
-typedef int32_t v4i __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_ingest4(const v4i* __restrict__ lb, const v4i* __restrict__ ub,
                                                  v4i* __restrict__ image, v4i* __restrict__ store,
                                                  int64_t n4, int64_t nvars, uint8_t* __restrict__ empty) {

@@ -50,7 +50,8 @@ class Config(ctypes.Structure):
                 ("max_sweeps", ctypes.c_int64), ("timing", ctypes.c_int32),
                 ("domain", ctypes.c_int32), ("compact_tables", ctypes.c_int32),
                 ("stream_tables", ctypes.c_int32), ("event_driven", ctypes.c_int32),
-                ("lockstep", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("lockstep", ctypes.c_int32), ("mark_queue", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 1)]
 
 
 class Result(ctypes.Structure):

@@ -222,17 +222,19 @@ def test_load_from_device_memory(gpu, nvars):
         dev.reset()
 
 
+@pytest.mark.parametrize("V", [700, 701], ids=["vector", "scalar"])
 @pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
 @pytest.mark.parametrize("gi", [16, 32, 64])
-def test_device_load_into_the_lockstep_copy(gpu, gi, events):
+def test_device_load_into_the_lockstep_copy(gpu, gi, events, V):
     """A device load into a lockstep batch writes the stores and the
     interleaved copy in one pass, and the reset image lives in that copy
     until a run would overwrite it.  Every sequence -- reset right after the
     load, run / run again without a reset / reset / run, a host load after
     it -- gives the host-loaded batch's results; 131 instances leave a
-    partial last group."""
+    partial last group; 700 variables take the 16-byte path, 701 the scalar
+    one."""
     from pcx.device import DeviceArray
-    V, P, n = 700, 2800, 131
+    P, n = 4 * V, 131
     op, x, y, z, lb, ub, _ = pcx.gen_network(pcx.gen_params(seed=gi + events, nvars=V, nprops=P))
     pir = pcx.PIR(lockstep=gi, event_driven=events)
     pir.tell(op, x, y, z, lb, ub)

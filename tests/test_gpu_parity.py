@@ -696,3 +696,33 @@ def test_branching_on_device(gpu, incremental, mode):
     child.reset()                             # the children are the child batch's image
     glo2, ghi2 = child.read()
     assert (glo2 == clo).all() and (ghi2 == chi).all()
+
+
+@pytest.mark.parametrize("queue", [-1, 0, 10], ids=["queue", "bitmaps", "overflow"])
+@pytest.mark.parametrize("gi", [16, 64])
+def test_queued_marked_sweeps(gpu, gi, queue):
+    """Marked sweeps after a lockstep full sweep, from queue planes (the
+    default), from the mark bitmaps, or from a queue of 1,024 entries that
+    overflows (the run falls back to the bitmaps mid-way): every instance
+    equals the oracle, subsets and failures included."""
+    V, P, n = 3000, 12000, 200
+    op, x, y, z, lb, ub, _ = _network(41 + gi, V, P)
+    pir = pcx.PIR(lockstep=gi, event_driven=1, mark_queue=queue)
+    pir.tell(op, x, y, z, lb, ub)
+    root_l, root_u = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_l, root_u).failed
+    lo, hi = pcx.gen_splits(gi + 7, root_l, root_u, n, 1, 20)
+    off, idx = pcx.gen_subsets(gi + 8, P, n, 500, P)
+    top, tx, ty, tz = pir.table()
+    lo_o, hi_o = lo.copy(), hi.copy()
+    ro = ol.fixpoint_batch(top, tx, ty, tz, lo_o, hi_o, V, idx=idx.astype(np.int64), off=off, threads=8)
+    b = pcx.Batch(pir, n, V, off, idx)
+    b.load(lo, hi)
+    for _ in range(2):
+        rg, st = b.run(pcx.MODE_BATCHED)
+        assert st.lockstep == gi
+        lg, ug = b.read()
+        _compare(lg, ug, rg, lo_o, hi_o, ro, V, f"queue {queue} lockstep {gi}")
+        b.reset()
+    with pytest.raises(pcx.PcxError):
+        pir.set_config(mark_queue=5)
