@@ -548,6 +548,10 @@ pcx_status ensure_shadow(pcx_batch* b, int gi) {
 // Batched mode: per sweep one scheduler launch and one grid-stride sweep
 // launch over the work items of every running instance (or, for full sweeps
 // in lockstep, over (group, table block) items).
+// Rounds a workgroup of a queued marked sweep spends on the marks it made
+// itself (chains advance one link per round within the sweep).
+constexpr int32_t kQueueRounds = 64;
+
 // Queue planes of a batch (queued marked sweeps), allocated on first use:
 // room for a sixteenth of the batch's (instance, propagator) pairs per plane,
 // between 2^16 and 2^23 entries; a plane that overflows falls back to bitmaps.
@@ -605,7 +609,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   auto launch_queued = [&]() -> pcx_status {
     PCX_HIP(launch_queue_turn(b->d_qctl, maxs, e->stream));
     PCX_HIP(launch_sweep_queue(table_of(e), b->d_store, b->nvars, b->d_flags, b->d_tight + kTightSlots, ev,
-                               dom_of(e), grid, e->stream));
+                               dom_of(e), grid, e->stream, kQueueRounds));
     return PCX_OK;
   };
   // A full sweep (scheduled first): lockstep over the interleaved copy, or
