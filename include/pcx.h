@@ -94,11 +94,12 @@ typedef struct {
                                16, 32 or 64: one propagator for that many instances
                                per step, over an instance-interleaved copy of the
                                stores (8 B per variable per instance); -1 auto     */
-  int32_t mark_queue;       /* batched mode, marked sweeps after a lockstep full
-                               sweep: -1 auto, 0 scan the mark bitmaps, k in 10..26:
+  int32_t mark_queue;       /* marked sweeps: 0 scan the mark bitmaps; k in 10..26:
                                also list every new mark in a queue of 2^k entries
                                per sweep parity and sweep the queue (a full queue
-                               falls back to the bitmaps; same fixpoint)        */
+                               falls back to the bitmaps; same fixpoint); -1 auto:
+                               a queue sized to the batch for batched runs whose
+                               full sweep is lockstep, the bitmaps otherwise     */
   int32_t reserved[1];
 } pcx_config;
 

@@ -439,6 +439,26 @@ pcx_status prepare_run(pcx_batch* b, int32_t mode) {
   return prepare_events(b, mode);
 }
 
+// Queue planes of a batch (queued marked sweeps), allocated on first use:
+// room for a sixteenth of the batch's (instance, propagator) pairs per plane,
+// between 2^16 and 2^23 entries; a plane that overflows falls back to bitmaps.
+pcx_status ensure_queue(pcx_batch* b) {
+  const int mq = b->e->cfg.mark_queue;
+  const int64_t pairs = b->n_inst * std::max<int64_t>(b->table_n, 1);
+  // a sixteenth of the pairs, or one whole instance (a dense first sweep of
+  // a single large instance marks every propagator), within [2^16, 2^23]
+  const int64_t want = std::max<int64_t>(pairs / 16, std::min<int64_t>(pairs, b->table_n));
+  const int64_t cap = mq > 0 ? (int64_t)1 << mq
+                             : std::min<int64_t>(std::max<int64_t>(want, (int64_t)1 << 16), (int64_t)1 << 23);
+  if (b->d_q && b->qcap == cap) return PCX_OK;
+  dfree(b->d_q);
+  dfree(b->d_qctl);
+  b->qcap = cap;
+  PCX_HIP(hipMalloc(&b->d_q, 2 * b->qcap * sizeof(unsigned long long)));
+  PCX_HIP(hipMalloc(&b->d_qctl, kQCtlWords * sizeof(int32_t)));
+  return PCX_OK;
+}
+
 // Plain mode: every instance on its own, one launch per sweep, host checks
 // the instance's flags every `group` sweeps.
 pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
@@ -450,6 +470,48 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   ev.snap = nullptr;
   SweepTimer tm{b, e->cfg.timing != 0};
   InstFlags hf;
+  const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
+  // Queued marked sweeps (as in batched mode), when asked for with a queue
+  // capacity (mark_queue > 0; automatic mode keeps the bitmap passes, which
+  // advance a long chain several links per sweep where a queue pass
+  // advances one): after an instance's full sweep its marks are listed in a
+  // queue plane, and groups of (turn, queue pass) launches, captured once,
+  // run until no mark is left; a plane that overflows hands the instance
+  // back to the bitmap sweeps.
+  const bool qmode = b->events && !b->seeded_run && e->cfg.mark_queue > 0;
+  EventSet evq = ev;
+  const int qgrid = pcx_internal::device_cus(e->cfg.device) * 8;   // queried outside any capture
+  if (qmode) {
+    pcx_status qs = ensure_queue(b);
+    if (qs != PCX_OK) return qs;
+    evq.q = b->d_q;
+    evq.qctl = b->d_qctl;
+    evq.qcap = b->qcap;
+    const pcx_batch::MarkedGraphArgs args{e->version, group, dom_of(e), maxs, 0,
+                                          e->d_xo, e->d_y, e->d_z, e->d_adj_off, e->d_adj, evq.q};
+    if (!b->plain_queue_graph || !(b->plain_queue_graph_args == args)) {
+      if (b->plain_queue_graph) (void)hipGraphExecDestroy(b->plain_queue_graph);
+      b->plain_queue_graph = nullptr;
+      hipGraph_t graph = nullptr;
+      PCX_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t ce = hipSuccess;
+      for (int g = 0; g < group && ce == hipSuccess; ++g) {
+        ce = launch_queue_turn(b->d_qctl, maxs, e->stream);
+        if (ce == hipSuccess)
+          ce = launch_sweep_queue(t, b->d_store, b->nvars, b->d_flags, b->d_tight + kTightSlots, evq,
+                                  dom_of(e), qgrid, e->stream);
+      }
+      const hipError_t ee = hipStreamEndCapture(e->stream, &graph);
+      if (ce != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        PCX_HIP(ce != hipSuccess ? ce : ee);
+      }
+      const hipError_t ie = hipGraphInstantiate(&b->plain_queue_graph, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      PCX_HIP(ie);
+      b->plain_queue_graph_args = args;
+    }
+  }
   for (int64_t k = 0; k < b->n_inst; ++k) {
     const int64_t m = inst_size(b, k);
     const int ppw = ppw_of(e, m);
@@ -463,7 +525,33 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     int2* st = b->d_store + k * b->nvars;
     InstFlags* f = b->d_flags + k;
     int32_t s = 0;
-    for (;;) {
+    bool done = false;
+    if (qmode && (maxs <= 0 || maxs > 1)) {
+      pcx_status ts = tm.begin(e->stream);
+      if (ts != PCX_OK) return ts;
+      PCX_HIP(launch_sweep_plain(tk, st, idx, m, 0, f, b->d_tight, ppw, dom_of(e), b->stream, e->stream,
+                                 &ev, k));
+      ts = tm.end(e->stream, true);
+      if (ts != PCX_OK) return ts;
+      s = 1;
+      PCX_HIP(hipMemsetAsync(b->d_qctl, 0, kQCtlWords * sizeof(int32_t), e->stream));
+      PCX_HIP(launch_queue_from_bitmap(evq, b->d_flags, k, 1, b->table_n, e->stream));
+      for (;;) {
+        ts = tm.begin(e->stream);
+        if (ts != PCX_OK) return ts;
+        PCX_HIP(hipGraphLaunch(b->plain_queue_graph, e->stream));
+        ts = tm.end(e->stream, false, group);
+        if (ts != PCX_OK) return ts;
+        int32_t q[kQCtlWords];
+        ts = read_small(b, b->d_qctl, sizeof(q), q);
+        if (ts != PCX_OK) return ts;
+        ts = tm.drain();
+        if (ts != PCX_OK) return ts;
+        if (q[kQStop] == 1) { done = true; break; }
+        if (q[kQStop] == 2) { s = q[kQAt]; break; }      // bitmap sweeps from sweep s on
+      }
+    }
+    for (; !done;) {
       // Full sweeps are timed one by one; consecutive marked sweeps as one span.
       int n_marked = 0;
       for (int g = 0; g < group; ++g, ++s) {
@@ -548,27 +636,6 @@ pcx_status ensure_shadow(pcx_batch* b, int gi) {
 // Batched mode: per sweep one scheduler launch and one grid-stride sweep
 // launch over the work items of every running instance (or, for full sweeps
 // in lockstep, over (group, table block) items).
-// Rounds a workgroup of a queued marked sweep spends on the marks it made
-// itself (chains advance one link per round within the sweep).
-constexpr int32_t kQueueRounds = 64;
-
-// Queue planes of a batch (queued marked sweeps), allocated on first use:
-// room for a sixteenth of the batch's (instance, propagator) pairs per plane,
-// between 2^16 and 2^23 entries; a plane that overflows falls back to bitmaps.
-pcx_status ensure_queue(pcx_batch* b) {
-  const int mq = b->e->cfg.mark_queue;
-  const int64_t pairs = b->n_inst * std::max<int64_t>(b->table_n, 1);
-  const int64_t cap = mq > 0 ? (int64_t)1 << mq
-                             : std::min<int64_t>(std::max<int64_t>(pairs / 16, (int64_t)1 << 16), (int64_t)1 << 23);
-  if (b->d_q && b->qcap == cap) return PCX_OK;
-  dfree(b->d_q);
-  dfree(b->d_qctl);
-  b->qcap = cap;
-  PCX_HIP(hipMalloc(&b->d_q, 2 * b->qcap * sizeof(unsigned long long)));
-  PCX_HIP(hipMalloc(&b->d_qctl, kQCtlWords * sizeof(int32_t)));
-  return PCX_OK;
-}
-
 // Batched mode: per sweep one scheduler launch and one grid-stride sweep
 // launch over the work items of every running instance (or, for full sweeps
 // in lockstep, over (group, table block) items).  After a lockstep full
@@ -609,7 +676,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   auto launch_queued = [&]() -> pcx_status {
     PCX_HIP(launch_queue_turn(b->d_qctl, maxs, e->stream));
     PCX_HIP(launch_sweep_queue(table_of(e), b->d_store, b->nvars, b->d_flags, b->d_tight + kTightSlots, ev,
-                               dom_of(e), grid, e->stream, kQueueRounds));
+                               dom_of(e), grid, e->stream));
     return PCX_OK;
   };
   // A full sweep (scheduled first): lockstep over the interleaved copy, or
@@ -1020,6 +1087,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   if (b->h_pin) (void)hipHostFree(b->h_pin);
   if (b->marked_graph) (void)hipGraphExecDestroy(b->marked_graph);
   if (b->queue_graph) (void)hipGraphExecDestroy(b->queue_graph);
+  if (b->plain_queue_graph) (void)hipGraphExecDestroy(b->plain_queue_graph);
   dfree(b->d_q); dfree(b->d_qctl);
   delete b;
 }

@@ -87,6 +87,8 @@ struct pcx_batch {
   int64_t qcap = 0;
   hipGraphExec_t queue_graph = nullptr;
   MarkedGraphArgs queue_graph_args{};
+  hipGraphExec_t plain_queue_graph = nullptr;     // the same for plain-mode runs
+  MarkedGraphArgs plain_queue_graph_args{};
   pcx::InstFlags* d_flags = nullptr;
   unsigned long long* d_tight = nullptr;
   int32_t* d_act = nullptr;

@@ -167,11 +167,13 @@ hipError_t launch_build_members(const int64_t* off, const int32_t* idx, int64_t 
 // Queued marked sweeps: one turn (advances kQSweep, or stops) and one flat
 // pass over the plane of that sweep; both return at once after a stop.
 hipError_t launch_queue_turn(int32_t* qctl, int32_t max_sweeps, hipStream_t st);
-// rounds: passes of a workgroup over the marks it made itself, after its
-// share of the queue (0: none).
+// Instance k's marks for sweep s (bitmap row, or every member propagator
+// when its all_marked word is set) appended to queue plane s & 1.
+hipError_t launch_queue_from_bitmap(const EventSet& ev, InstFlags* flags, int64_t k, int32_t s,
+                                    int64_t n_props, hipStream_t st);
 hipError_t launch_sweep_queue(const Table& t, int2* store, int64_t nvars, InstFlags* flags,
                               unsigned long long* tight, const EventSet& ev, int dom, int grid,
-                              hipStream_t st, int32_t rounds);
+                              hipStream_t st);
 // marked != nullptr: schedule a marked sweep (items = chunks with marks).
 hipError_t launch_schedule(const Subsets& sub, int64_t n_props, InstFlags* flags, int32_t n_inst,
                            WorkList wl, int ppw, int32_t max_sweeps, hipStream_t st,

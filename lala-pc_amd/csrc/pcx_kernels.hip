@@ -226,10 +226,10 @@ __device__ __forceinline__ void flush_stage(const QueueStage& qs, const EventSet
 }
 
 // mark_var for queued marks: a propagator whose bit this call sets is
-// staged for the next sweep's plane (a returning atomic tells; a plain load
-// that already shows the bit saves the atomic, since bits of the next sweep
-// are only set during this one).  Summary bits are kept for a sweep that
-// falls back to the bitmaps.
+// staged for the next sweep's plane (a returning atomic tells).  An atomic
+// load that already shows the bit saves the atomic: the next sweep's bits
+// are only set, never cleared, during a launch.  Summary bits are kept for a
+// sweep that falls back to the bitmaps.
 __device__ __forceinline__ void mark_var_q(const Marker& mk, const EventSet& ev, const QueueStage& qs,
                                            int64_t k, int32_t v, int32_t next) {
   constexpr int kB = 8;
@@ -1440,6 +1440,32 @@ __global__ __launch_bounds__(kBlock, lock_min_waves<D>()) void k_sweep_lockstep(
   }
 }
 
+// Exclusive scan of v over a workgroup of BT threads; returns the offset,
+// *total = the sum.  s_scan holds BT / kWave entries.
+template <int BT>
+__device__ __forceinline__ int block_scan(int v, int32_t* s_scan, int* total) {
+  const int lane = lane_id();
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int u = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += u;
+  }
+  __syncthreads();
+  const int wv = threadIdx.x / kWave;
+  if (lane == kWave - 1) s_scan[wv] = incl;
+  __syncthreads();
+  int off = incl - v, t = 0;
+#pragma unroll
+  for (int i = 0; i < BT / kWave; ++i) {
+    const int u = s_scan[i];
+    if (i < wv) off += u;
+    t += u;
+  }
+  *total = t;
+  return off;
+}
+
 // ---- queued marked sweeps --------------------------------------------------
 // After a lockstep full sweep of a batch, the marks of each sweep are also
 // listed in a queue plane (mark_var_q).  A turn checks the plane of the next
@@ -1467,169 +1493,156 @@ hipError_t launch_queue_turn(int32_t* qctl, int32_t max_sweeps, hipStream_t stre
   return hipGetLastError();
 }
 
-// One marked sweep over its queue plane, with chasing inside each
-// workgroup.  Phase 1: thread i of a pass takes entry i of the plane, clears
-// its bit, evaluates the propagator on the instance's store and lists what it
-// tightened; the workgroup then marks those variables' adjacency for the
-// next sweep.  New marks (bits this workgroup set) are kept in an LDS ring of
-// one pass (kBlock entries) and the rest go to the next sweep's plane.
-// Phase 2: up to `rounds` rounds evaluate the ring's entries in the same way
-// (a chain of tightenings advances one link per round instead of one per
-// sweep); what is left in the ring at the end goes to the next plane.  A
-// propagator's bit is cleared, and the clear waited for, before its operands
-// are read: a workgroup that tightens an operand later finds the bit clear and
-// marks it again, so every tightening is followed by an evaluation.
+// One marked sweep over its queue plane: thread i takes entry i, clears its
+// bit, evaluates the propagator on the instance's store and lists what it
+// tightened; the workgroup then marks those variables' adjacency (queued)
+// for the next sweep.  Entries of failed instances are skipped.
 constexpr int kQueueList = 3 * kBlock;     // at most three tightened variables per entry
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_sweep_queue(Table t, int2* __restrict__ store, int64_t nvars,
                                                       InstFlags* flags, unsigned long long* tight,
-                                                      EventSet ev, int32_t rounds) {
+                                                      EventSet ev) {
   const int32_t* qctl = ev.qctl;
   if (qctl[kQStop]) return;
   const int32_t s = qctl[kQSweep];
-  const int cu = s & 1, nx = cu ^ 1;
+  const int cu = s & 1;
   const int64_t n = min((int64_t)qctl[kQTail + cu], ev.qcap);
   const unsigned long long* plane = ev.q + (int64_t)cu * ev.qcap;
-  const int64_t bplane = ev.n_inst * ev.words;
-  uint32_t* bm_cu = ev.dirty + (int64_t)cu * bplane;
-  uint32_t* bm_nx = ev.dirty + (int64_t)nx * bplane;
+  uint32_t* cur = ev.dirty + (int64_t)cu * ev.n_inst * ev.words;
   __shared__ int32_t s_k[kQueueList], s_v[kQueueList];
   __shared__ int32_t s_n;
   __shared__ unsigned long long s_qbuf[kQStage];
   __shared__ int32_t s_qn;
-  __shared__ unsigned long long s_ring[kBlock];
-  __shared__ int32_t s_rn, s_base_q;
   const QueueStage qs{s_qbuf, &s_qn};
-  if (threadIdx.x == 0) { s_n = 0; s_qn = 0; s_rn = 0; }
+  if (threadIdx.x == 0) { s_n = 0; s_qn = 0; }
   __syncthreads();
   int nt_all = 0;
-  // evaluate entry `en` whose bit is in bitmap plane `bm` (every thread
-  // calls it; `has` says whether this thread holds an entry)
-  auto eval_entry = [&](bool has, unsigned long long en, uint32_t* bm) {
-    if (!has) return;
-    const int32_t k = (int32_t)(en >> 32), p = (int32_t)(uint32_t)en;
-    InstFlags* f = flags + k;
-    if (f->failed_at != kNoFail) return;
-    (void)__hip_atomic_fetch_and(&bm[(int64_t)k * ev.words + (p >> 5)], ~(1u << (p & 31)),
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_s_waitcnt(0);               // the clear is done before the operands are read
-    int2* st = store + (int64_t)k * nvars;
-    const uint32_t xo = t.xo[p];
-    const int32_t y = t.y[p], z = t.z[p];
-    const int32_t x = (int32_t)(xo & kVarMask);
-    const u64 wx = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + x);
-    const u64 wy = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + y);
-    const u64 wz = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + z);
-    bool chg = false, fail = false, tx = false, ty = false, tz = false;
-    int nt = 0;
-    if constexpr (D == kBits) {
-      u64* ws = reinterpret_cast<u64*>(st);
-      if (!wx || !wy || !wz) {
-        fail = true;
-      } else {
-        u64 nxw, nyw, nzw;
-        deduce_bits((int)(xo >> kOpShift), wx, wy, wz, nxw, nyw, nzw);
-        tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(ws, x, wx, nxw, chg, fail, nt);
-        ty = meet_word<__HIP_MEMORY_SCOPE_AGENT>(ws, y, wy, nyw, chg, fail, nt);
-        tz = meet_word<__HIP_MEMORY_SCOPE_AGENT>(ws, z, wz, nzw, chg, fail, nt);
-      }
-    } else {
-      const int2 X = as_itv(wx), Y = as_itv(wy), Z = as_itv(wz);
-      if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) {
-        fail = true;
-      } else {
-        int2 nxi, nyi, nzi;
-        deduce<D == kItvDiv>((int)(xo >> kOpShift), X, Y, Z, nxi, nyi, nzi);
-        if (nxi.x > nxi.y || nyi.x > nyi.y || nzi.x > nzi.y) fail = true;
-        tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(st, x, X, nxi, chg, fail, nt);
-        ty = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(st, y, Y, nyi, chg, fail, nt);
-        tz = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(st, z, Z, nzi, chg, fail, nt);
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = base + threadIdx.x;
+    if (i < n) {
+      const unsigned long long en = plane[i];
+      const int32_t k = (int32_t)(en >> 32), p = (int32_t)(uint32_t)en;
+      InstFlags* f = flags + k;
+      if (f->failed_at == kNoFail) {
+        atomicAnd(&cur[(int64_t)k * ev.words + (p >> 5)], ~(1u << (p & 31)));
+        int2* st = store + (int64_t)k * nvars;
+        const uint32_t xo = t.xo[p];
+        const int32_t y = t.y[p], z = t.z[p];
+        const int32_t x = (int32_t)(xo & kVarMask);
+        const u64 wx = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + x);
+        const u64 wy = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + y);
+        const u64 wz = load_word<__HIP_MEMORY_SCOPE_AGENT>(st + z);
+        bool chg = false, fail = false, tx = false, ty = false, tz = false;
+        int nt = 0;
+        if constexpr (D == kBits) {
+          u64* ws = reinterpret_cast<u64*>(st);
+          if (!wx || !wy || !wz) {
+            fail = true;
+          } else {
+            u64 nx, ny, nz;
+            deduce_bits((int)(xo >> kOpShift), wx, wy, wz, nx, ny, nz);
+            tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(ws, x, wx, nx, chg, fail, nt);
+            ty = meet_word<__HIP_MEMORY_SCOPE_AGENT>(ws, y, wy, ny, chg, fail, nt);
+            tz = meet_word<__HIP_MEMORY_SCOPE_AGENT>(ws, z, wz, nz, chg, fail, nt);
+          }
+        } else {
+          const int2 X = as_itv(wx), Y = as_itv(wy), Z = as_itv(wz);
+          if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) {
+            fail = true;
+          } else {
+            int2 nx, ny, nz;
+            deduce<D == kItvDiv>((int)(xo >> kOpShift), X, Y, Z, nx, ny, nz);
+            if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
+            tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(st, x, X, nx, chg, fail, nt);
+            ty = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(st, y, Y, ny, chg, fail, nt);
+            tz = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(st, z, Z, nz, chg, fail, nt);
+          }
+        }
+        const int32_t vs[3] = {x, y, z};
+        const bool ts[3] = {tx, ty, tz};
+        for (int j = 0; j < 3; ++j)
+          if (ts[j]) {
+            const int at = atomicAdd(&s_n, 1);     // < kQueueList: three per entry at most
+            s_k[at] = k;
+            s_v[at] = vs[j];
+          }
+        if (chg) f->changed[s & 1] = s;
+        if (fail) f->failed_at = s;
+        atomicAdd((unsigned long long*)&f->marked_evals, 1ull);
+        nt_all += nt;
       }
     }
-    const int32_t vs[3] = {x, y, z};
-    const bool ts[3] = {tx, ty, tz};
-    for (int j = 0; j < 3; ++j)
-      if (ts[j]) {
-        const int at = atomicAdd(&s_n, 1);         // < kQueueList: three per entry at most
-        s_k[at] = k;
-        s_v[at] = vs[j];
-      }
-    if (chg) f->changed[s & 1] = s;
-    if (fail) f->failed_at = s;
-    atomicAdd((unsigned long long*)&f->marked_evals, 1ull);
-    nt_all += nt;
-  };
-  // mark the listed variables' adjacency; new marks fill the ring up to one
-  // pass and the rest go to the next plane (every thread calls it)
-  auto mark_listed = [&]() {
     __syncthreads();                               // the pass's tightened variables are listed
     const int nl = s_n;
     for (int j = threadIdx.x; j < nl; j += kBlock) {
       const int64_t k = s_k[j];
       if (flags[k].failed_at == kNoFail) mark_var_q(marker_for(&ev, k, s), ev, qs, k, s_v[j], s + 1);
     }
-    __syncthreads();                               // marks staged; the list is read
-    const int ns = min(s_qn, kQStage);
-    const int room = kBlock - s_rn;
-    const int keep = min(ns, room);
-    for (int j = threadIdx.x; j < keep; j += kBlock) s_ring[s_rn + j] = s_qbuf[j];
-    __syncthreads();                               // every thread has read s_rn and the kept entries
-    if (threadIdx.x == 0) {
-      s_n = 0;
-      s_rn += keep;
-      s_base_q = ns - keep > 0 ? atomicAdd(ev.qctl + kQTail + nx, ns - keep) : 0;
-    }
-    __syncthreads();
-    unsigned long long* out = ev.q + (int64_t)nx * ev.qcap;
-    for (int j = keep + threadIdx.x; j < ns; j += kBlock) {
-      const int64_t at = (int64_t)s_base_q + (j - keep);
-      if (at < ev.qcap) out[at] = s_qbuf[j];
-      else ev.qctl[kQOver + nx] = 1;
-    }
-    __syncthreads();                               // the staging area is read
-    if (threadIdx.x == 0) s_qn = 0;
-    __syncthreads();
-  };
-  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
-    const int64_t i = base + threadIdx.x;
-    eval_entry(i < n, i < n ? plane[i] : 0ull, bm_cu);
-    mark_listed();
-  }
-  for (int r = 0; r < rounds; ++r) {
-    const int nr = s_rn;
-    if (nr == 0) break;
-    const bool has = threadIdx.x < nr;
-    const unsigned long long en = has ? s_ring[threadIdx.x] : 0ull;
-    __syncthreads();                               // every thread has taken its entry
-    if (threadIdx.x == 0) s_rn = 0;
-    __syncthreads();
-    eval_entry(has, en, bm_nx);
-    mark_listed();
-  }
-  // the ring's leftovers: their bits are set, they go to the next plane
-  const int left = s_rn;
-  if (left > 0) {
-    if (threadIdx.x == 0) s_base_q = atomicAdd(ev.qctl + kQTail + nx, left);
-    __syncthreads();
-    unsigned long long* out = ev.q + (int64_t)nx * ev.qcap;
-    for (int j = threadIdx.x; j < left; j += kBlock) {
-      const int64_t at = (int64_t)s_base_q + j;
-      if (at < ev.qcap) out[at] = s_ring[j];
-      else ev.qctl[kQOver + nx] = 1;
-    }
+    __syncthreads();                               // every thread has read the list; marks staged
+    if (threadIdx.x == 0) s_n = 0;
+    flush_stage(qs, ev, (s + 1) & 1);              // ends with a barrier
   }
   const int sum = wave_sum(nt_all);
   if (lane_id() == 0 && sum)
     atomicAdd(&tight[(blockIdx.x * 4 + (threadIdx.x >> 6)) & (kTightSlots - 1)], (unsigned long long)sum);
 }
 
+// The marks of instance k for sweep s, from its bitmap row (or, when its
+// all_marked word says every member propagator is marked, from its member
+// row, whose bits are then set in the bitmap too) into queue plane s & 1:
+// a block scan of the popcounts and one reservation per workgroup.
+__global__ __launch_bounds__(kBlock) void k_queue_from_bitmap(EventSet ev, InstFlags* flags, int64_t k,
+                                                            int32_t s, int64_t n_props) {
+  const int pl = s & 1;
+  uint32_t* row = ev.dirty + (int64_t)pl * ev.n_inst * ev.words + k * ev.words;
+  const uint32_t* mrow = ev.member ? ev.member + k * ev.words : nullptr;
+  const bool all = flags[k].all_marked[pl] == s;
+  unsigned long long* plane = ev.q + (int64_t)pl * ev.qcap;
+  __shared__ int32_t s_scan[kBlock / kWave];
+  __shared__ int32_t s_base;
+  for (int64_t w0 = (int64_t)blockIdx.x * kBlock; w0 < ev.words; w0 += (int64_t)gridDim.x * kBlock) {
+    const int64_t w = w0 + threadIdx.x;
+    uint32_t bits = 0;
+    if (w < ev.words) {
+      if (all) {
+        bits = mrow ? mrow[w] : ~0u;
+        const int64_t left = n_props - w * 32;
+        if (left < 32) bits &= (1u << left) - 1u;
+        if (bits) atomicOr(&row[w], bits);
+      } else {
+        bits = row[w];
+      }
+    }
+    int total;
+    int off = block_scan<kBlock>(__builtin_popcount(bits), s_scan, &total);
+    if (threadIdx.x == 0) s_base = total ? atomicAdd(ev.qctl + kQTail + pl, total) : 0;
+    __syncthreads();
+    for (uint32_t b = bits; b; b &= b - 1, ++off) {
+      const int64_t at = (int64_t)s_base + off;
+      const unsigned long long en = ((unsigned long long)k << 32) | (uint32_t)(w * 32 + __builtin_ctz(b));
+      if (at < ev.qcap) plane[at] = en;
+      else ev.qctl[kQOver + pl] = 1;
+    }
+    __syncthreads();                               // s_base is read
+  }
+}
+
+hipError_t launch_queue_from_bitmap(const EventSet& ev, InstFlags* flags, int64_t k, int32_t s,
+                                    int64_t n_props, hipStream_t stream) {
+  int64_t grid = (ev.words + kBlock - 1) / kBlock;
+  if (grid > 2048) grid = 2048;
+  if (grid < 1) grid = 1;
+  k_queue_from_bitmap<<<(unsigned)grid, kBlock, 0, stream>>>(ev, flags, k, s, n_props);
+  return hipGetLastError();
+}
+
 hipError_t launch_sweep_queue(const Table& t, int2* store, int64_t nvars, InstFlags* flags,
                               unsigned long long* tight, const EventSet& ev, int dom, int grid,
-                              hipStream_t stream, int32_t rounds) {
+                              hipStream_t stream) {
   switch (dom) {
-    case kItv: k_sweep_queue<kItv><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, tight, ev, rounds); break;
-    case kItvDiv: k_sweep_queue<kItvDiv><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, tight, ev, rounds); break;
-    case kBits: k_sweep_queue<kBits><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, tight, ev, rounds); break;
+    case kItv: k_sweep_queue<kItv><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, tight, ev); break;
+    case kItvDiv: k_sweep_queue<kItvDiv><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, tight, ev); break;
+    case kBits: k_sweep_queue<kBits><<<grid, kBlock, 0, stream>>>(t, store, nvars, flags, tight, ev); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -2042,31 +2055,6 @@ constexpr int kBurstPop = kMarkCap / 3;   // propagators evaluated per round
 constexpr int kBurstChunks = 256;    // marked chunks gathered at the start
 static_assert(kBurstSummaryMax <= kBurstBlock, "one summary word per thread");
 
-// Exclusive scan of v over a workgroup of BT threads; returns the offset,
-// *total = the sum.  s_scan holds BT / kWave entries.
-template <int BT>
-__device__ __forceinline__ int block_scan(int v, int32_t* s_scan, int* total) {
-  const int lane = lane_id();
-  int incl = v;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const int u = __shfl_up(incl, o, kWave);
-    if (lane >= o) incl += u;
-  }
-  __syncthreads();
-  const int wv = threadIdx.x / kWave;
-  if (lane == kWave - 1) s_scan[wv] = incl;
-  __syncthreads();
-  int off = incl - v, t = 0;
-#pragma unroll
-  for (int i = 0; i < BT / kWave; ++i) {
-    const int u = s_scan[i];
-    if (i < wv) off += u;
-    t += u;
-  }
-  *total = t;
-  return off;
-}
 
 template <int D>
 __global__ __launch_bounds__(kBurstBlock) void k_burst_slots(const SlotDesc* slots,

@@ -726,3 +726,32 @@ def test_queued_marked_sweeps(gpu, gi, queue):
         b.reset()
     with pytest.raises(pcx.PcxError):
         pir.set_config(mark_queue=5)
+
+
+@pytest.mark.parametrize("queue", [23, 0, 12], ids=["queue", "bitmaps", "overflow"])
+def test_queued_marked_sweeps_plain(gpu, queue):
+    """Single instances with marked sweeps (plain mode): from queue planes
+    (asked for with a capacity of 2^23), from the bitmaps (the default), or
+    from a queue of 4,096 entries that overflows; chained networks (long
+    chains of tightenings) and a batch of search nodes, every one equal to
+    the oracle."""
+    V, P = 20000, 80000
+    op, x, y, z, lb, ub, _ = _network(77, V, P, chain_frac=0.3, chain_len=200)
+    pir = pcx.PIR(event_driven=1, mark_queue=queue)
+    g_lb, g_ub = lb.copy(), ub.copy()
+    pir.tell(op, x, y, z, g_lb, g_ub)
+    root_l, root_u = g_lb.copy(), g_ub.copy()
+    r = pir.fixpoint(root_l, root_u)
+    ro, o_lb, o_ub = _oracle_fix(op, x, y, z, lb, ub)
+    assert bool(r.failed) == bool(ro.failed)
+    if not ro.failed:
+        assert np.array_equal(root_l, o_lb) and np.array_equal(root_u, o_ub)
+    lo, hi = pcx.gen_splits(78, root_l, root_u, 12, 1, 20)
+    top, tx, ty, tz = pir.table()
+    lo_o, hi_o = lo.copy(), hi.copy()
+    ros = ol.fixpoint_batch(top, tx, ty, tz, lo_o, hi_o, V, threads=8)
+    b = pcx.Batch(pir, 12, V)
+    b.load(lo, hi)
+    rg, _ = b.run(pcx.MODE_PLAIN)
+    lg, ug = b.read()
+    _compare(lg, ug, rg, lo_o, hi_o, ros, V, f"plain queue {queue}")
