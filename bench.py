@@ -145,6 +145,14 @@ def full_kernel(mode, sums):
     return "k_sweep_lockstep" if sums.lockstep else "k_sweep_batched"
 
 
+def marked_kernel(pir, mode, sums, suffix=""):
+    """Name of the kernel of the marked sweeps: batched runs whose full sweep
+    is lockstep list their marks in queues by default (mark_queue)."""
+    if mode == pcx.MODE_BATCHED and sums.lockstep and pir.get_config().mark_queue != 0:
+        return "k_sweep_queue" + suffix
+    return "k_sweep_marked" + suffix
+
+
 def autotune(args, pir, batch, key):
     """Configure the engine from the tuning cache, or run the autotuner on the
     loaded batch (setup, outside the timed region) and cache its choice."""
@@ -261,7 +269,7 @@ def bench_batched(args):
         "failed_instances": int(failed),
         "sweeps_max": int(smax),
         "useful_lane_frac": useful_lane_fraction(off, n_inst, P, sums.lockstep),
-        "roofline": sums.roofline(args.steps, full_kernel(mode, sums), "k_sweep_marked", "cfg2"),
+        "roofline": sums.roofline(args.steps, full_kernel(mode, sums), marked_kernel(pir, mode, sums), "cfg2"),
     }
 
     if args.search_step:
@@ -353,9 +361,9 @@ def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n
            "evals_per_s": sums.v["evals"] / elapsed,
            "kernel_ms_per_step": sums.v["kernel_ms"] / args.steps,
            "failed_instances": int(sum(r.failed for r in res)),
-           "what": "each step loads 4,096 new stores from HBM (pcx_batch_load_device: stores, reset "
-                   "image and emptiness in one pass) and propagates them; the interleaved copy for "
-                   "the lockstep full sweep is rebuilt inside the step"}
+           "what": "each step loads 4,096 new stores from HBM (pcx_batch_load_device: the stores, "
+                   "their interleaved copy for the lockstep full sweep and the emptiness in one pass) "
+                   "and propagates them"}
     log(f"fresh: {out['ms_per_step']:.2f} ms/step, {fps:.0f} fixpoints/s")
     if not args.no_cpu:
         ol = cpu_oracle()
@@ -512,8 +520,8 @@ def bench_bits(args):
     evals = sums.v["evals"]
     fps = n_inst * args.steps / elapsed
     words_g = batch.read_words()
-    rf = sums.roofline(args.steps, full_kernel(mode, sums) + "_bits", "k_sweep_marked_bits", "cfg3",
-                       tight_bytes=8)
+    rf = sums.roofline(args.steps, full_kernel(mode, sums) + "_bits", marked_kernel(pir, mode, sums, "_bits"),
+                       "cfg3", tight_bytes=8)
     out = {"value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
            "config": {"workload": "bitset store: 1,024 bit-removal subproblems (1-20 removals each) of "
                                   "the root fixpoint of one 200,000-word / 800,000-propagator planted "
