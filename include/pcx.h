@@ -100,7 +100,14 @@ typedef struct {
                                falls back to the bitmaps; same fixpoint); -1 auto:
                                a queue sized to the batch for batched runs whose
                                full sweep is lockstep, the bitmaps otherwise     */
-  int32_t reserved[1];
+  int32_t base_seed;        /* batched mode, event-driven: a run that starts from
+                               the loaded stores first evaluates only the
+                               propagators adjacent to a value that differs from
+                               instance 0's store and those that instance 0's
+                               store does not satisfy (the others compute in
+                               instance k what they compute in instance 0), when
+                               that is much less than a full sweep; 0 off, 1 or
+                               -1 (auto) on.  The fixpoints are the same.        */
 } pcx_config;
 
 typedef struct {

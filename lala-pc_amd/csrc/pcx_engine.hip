@@ -51,6 +51,9 @@ bool valid_ppw(int p) {
 
 bool valid_lockstep(int g) { return g == -1 || g == 0 || g == 16 || g == 32 || g == 64; }
 bool valid_mark_queue(int q) { return q == -1 || q == 0 || (q >= 10 && q <= 26); }
+bool valid_base_seed(int v) { return v == -1 || v == 0 || v == 1; }
+// d_seedctl: the difference sub-lists' counts, then the unsatisfied count
+constexpr int kSeedCtlWords = kDiffLists + 1;
 
 template <class T>
 void dfree(T*& p) {
@@ -197,7 +200,8 @@ struct SweepTimer {
 
 // Readbacks go through pinned host memory: a copy into pageable memory is
 // staged by the runtime and cost ~70 us per call here.
-constexpr size_t kPinScratch = 256;
+constexpr size_t kPinScratch = 2048;     // small readbacks: at most the seeding counters
+static_assert(kPinScratch >= (kDiffLists + 1) * sizeof(int32_t), "seeding counters fit the scratch");
 size_t pinned_bytes(const pcx_batch* b) {
   return kPinScratch + (size_t)b->n_inst * sizeof(InstFlags) + 2 * kTightSlots * sizeof(unsigned long long);
 }
@@ -207,6 +211,7 @@ pcx_status ensure_pinned(pcx_batch* b) {
 }
 // Copy `bytes` of device memory into the pinned scratch and wait for it.
 pcx_status read_small(pcx_batch* b, const void* d, size_t bytes, void* out) {
+  if (bytes > kPinScratch) return fail_with(PCX_EINVAL, "readback larger than the pinned scratch");
   PCX_HIP(hipMemcpyAsync(b->h_pin, d, bytes, hipMemcpyDeviceToHost, b->e->stream));
   PCX_HIP(hipStreamSynchronize(b->e->stream));
   std::memcpy(out, b->h_pin, bytes);
@@ -599,6 +604,8 @@ int lockstep_of(const pcx_batch* b) {
   const pcx_engine* e = b->e;
   int g = e->cfg.lockstep;
   if (b->n_inst < 2 || b->table_n == 0) return 0;
+  // The sweep addresses a group's copy with 32-bit offsets.
+  if (g > 0 && (uint64_t)g * (uint64_t)b->nvars * 8u > 0xffffffffull) return 0;
   if (g < 0) {
     g = 0;
     for (int c : {64, 32, 16})
@@ -635,16 +642,58 @@ pcx_status ensure_shadow(pcx_batch* b, int gi) {
 
 // Batched mode: per sweep one scheduler launch and one grid-stride sweep
 // launch over the work items of every running instance (or, for full sweeps
-// in lockstep, over (group, table block) items).
-// Batched mode: per sweep one scheduler launch and one grid-stride sweep
-// launch over the work items of every running instance (or, for full sweeps
 // in lockstep, over (group, table block) items).  After a lockstep full
 // sweep with marks, the marked sweeps run from queue planes: groups of
 // (turn, queue pass) launches replayed as one graph until a turn finds no
 // marks, or finds a plane that lost marks, from which sweep on the bitmap
 // sweeps take over.
+// A run that starts from the loaded stores, whose load listed the values that
+// differ from instance 0's, needs in its first sweep only (a) the member
+// propagators adjacent to a listed value of their instance and (b) the
+// propagators instance 0's store does not satisfy, in every instance that
+// holds them: any other propagator reads in instance k the values it reads
+// in instance 0, where it tightens nothing.  These are seeded as the first
+// sweep's marks, which makes that sweep a marked sweep, when their count is
+// under 1/16 of the pairs a full sweep evaluates (and the list is complete).
+// This decides; the run launches the seeding (*n_unsat propagators listed
+// in d_unsat) with its own event set.
+pcx_status plan_seed(pcx_batch* b, bool* seeded, int64_t* n_unsat_out) {
+  *seeded = false;
+  *n_unsat_out = 0;
+  pcx_engine* e = b->e;
+  if (!b->events || b->seeded_run || !b->run_from_image || !b->diffs_listed || e->cfg.base_seed == 0 ||
+      e->cfg.domain != PCX_DOMAIN_INTERVAL || b->table_n == 0)
+    return PCX_OK;
+  if (!b->d_unsat) PCX_HIP(hipMalloc(&b->d_unsat, b->table_n * sizeof(int32_t)));
+  int32_t* unsat_n = b->d_seedctl + kDiffLists;
+  PCX_HIP(hipMemsetAsync(unsat_n, 0, sizeof(int32_t), e->stream));
+  PCX_HIP(launch_base_probe(table_of(e), b->d_store, b->d_unsat, unsat_n, dom_of(e), e->stream));
+  int32_t ctl[kDiffLists + 1];
+  pcx_status st = read_small(b, b->d_seedctl, sizeof(ctl), ctl);
+  if (st != PCX_OK) return st;
+  int64_t n_diff = 0;
+  for (int c = 0; c < kDiffLists; ++c) {
+    if (ctl[c] > b->diff_cap) return PCX_OK;          // a sub-list lost entries
+    n_diff += ctl[c];
+  }
+  const int64_t n_unsat = ctl[kDiffLists];
+  const int64_t per_var = 3 * b->table_n / std::max<int64_t>(b->nvars, 1) + 1;   // mean adjacency
+  const int64_t pairs = b->h_off.empty() ? b->n_inst * b->table_n : b->h_off.back();
+  if ((n_diff * per_var + n_unsat * b->n_inst) * 16 > pairs) return PCX_OK;
+  *seeded = true;
+  *n_unsat_out = n_unsat;
+  return PCX_OK;
+}
+
 pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
+  bool base_seeded = false;
+  int64_t n_unsat = 0;
+  {
+    pcx_status ss = plan_seed(b, &base_seeded, &n_unsat);
+    if (ss != PCX_OK) return ss;
+    if (base_seeded) b->seeded_run = true;
+  }
   const int ppw = ppw_of(e);
   const int gi = lockstep_of(b);
   SweepTimer tm{b, e->cfg.timing != 0};
@@ -657,7 +706,10 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   int32_t live = 1, sweeps = 0;
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
   EventSet ev = event_set(b);
-  const bool qmode = b->events && gi > 0 && !b->seeded_run && e->cfg.mark_queue != 0;
+  // Queued marked sweeps follow a lockstep full sweep, or start at sweep 0
+  // from the seeds of a base store (seeds of a branching keep the bitmaps).
+  const bool qmode = b->events && e->cfg.mark_queue != 0 &&
+                     (base_seeded || (gi > 0 && !b->seeded_run));
   if (qmode) {
     pcx_status qs = ensure_queue(b);
     if (qs != PCX_OK) return qs;
@@ -665,7 +717,12 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     ev.qctl = b->d_qctl;
     ev.qcap = b->qcap;
     PCX_HIP(hipMemsetAsync(b->d_qctl, 0, kQCtlWords * sizeof(int32_t), e->stream));
+    // the first turn advances to sweep 0, whose plane the seeds fill
+    if (base_seeded) PCX_HIP(hipMemsetAsync(b->d_qctl + kQSweep, 0xff, sizeof(int32_t), e->stream));
   }
+  if (base_seeded)
+    PCX_HIP(launch_seed_from_base(ev, DiffSink{b->d_diff, b->d_seedctl, b->diff_cap}, b->d_unsat, n_unsat,
+                                  e->stream));
   auto launch_marked = [&]() -> pcx_status {
     PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
                             &ev));
@@ -740,9 +797,12 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     if (cs != PCX_OK) return cs;
   }
   if (qmode) {
-    pcx_status ts = launch_full();
-    if (ts != PCX_OK) return ts;
-    sweeps = 1;
+    pcx_status ts = PCX_OK;
+    if (!base_seeded) {
+      ts = launch_full();
+      if (ts != PCX_OK) return ts;
+      sweeps = 1;
+    }
     for (;;) {
       ts = tm.begin(e->stream);
       if (ts != PCX_OK) return ts;
@@ -859,6 +919,7 @@ void pcx_config_default(pcx_config* c) {
   c->event_driven = -1;
   c->lockstep = -1;
   c->mark_queue = -1;
+  c->base_seed = -1;
 }
 
 int32_t pcx_device_count(void) {
@@ -876,8 +937,8 @@ pcx_status pcx_create(const pcx_config* cfg, pcx_engine** out) {
     return fail_with(PCX_EINVAL, "unknown domain");
   if (c.props_per_wave != 0 && !valid_ppw(c.props_per_wave))
     return fail_with(PCX_EINVAL, "props_per_wave must be 0 (auto) or a power of two in [64, 2048]");
-  if (!valid_lockstep(c.lockstep) || !valid_mark_queue(c.mark_queue))
-    return fail_with(PCX_EINVAL, "lockstep or mark_queue out of range");
+  if (!valid_lockstep(c.lockstep) || !valid_mark_queue(c.mark_queue) || !valid_base_seed(c.base_seed))
+    return fail_with(PCX_EINVAL, "lockstep, mark_queue or base_seed out of range");
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail_with(PCX_EDEVICE, "no HIP device");
   if (c.device < 0 || c.device >= n) return fail_with(PCX_EINVAL, "device ordinal out of range");
@@ -976,6 +1037,8 @@ pcx_status pcx_set_config(pcx_engine* e, const pcx_config* cfg) {
     return fail_with(PCX_EINVAL, "lockstep must be -1 (auto), 0 (off), 16, 32 or 64");
   if (!valid_mark_queue(cfg->mark_queue))
     return fail_with(PCX_EINVAL, "mark_queue must be -1 (auto), 0 (off) or a log2 capacity in [10, 26]");
+  if (!valid_base_seed(cfg->base_seed))
+    return fail_with(PCX_EINVAL, "base_seed must be -1 (auto), 0 (off) or 1 (on)");
   e->cfg = *cfg;
   return PCX_OK;
 }
@@ -1089,13 +1152,48 @@ void pcx_batch_destroy(pcx_batch* b) {
   if (b->queue_graph) (void)hipGraphExecDestroy(b->queue_graph);
   if (b->plain_queue_graph) (void)hipGraphExecDestroy(b->plain_queue_graph);
   dfree(b->d_q); dfree(b->d_qctl);
+  dfree(b->d_diff); dfree(b->d_unsat); dfree(b->d_seedctl);
   delete b;
 }
 
 namespace {
+// Room for the differences a load lists: a seeded first sweep pays only
+// while they are a small share of the values (see seed_from_base).
+pcx_status ensure_diff_list(pcx_batch* b) {
+  const int64_t want = std::min<int64_t>(std::max<int64_t>(b->n_inst * b->nvars / 128, 4096), 1 << 24);
+  // Twice the even share (at least 256): workgroups fill the sub-lists
+  // unevenly when a load has few of them.
+  const int64_t per_list = std::max<int64_t>(2 * ((want + kDiffLists - 1) / kDiffLists), 256);
+  if (!b->d_seedctl) PCX_HIP(hipMalloc(&b->d_seedctl, kSeedCtlWords * sizeof(int32_t)));
+  if (b->diff_cap < per_list) {
+    dfree(b->d_diff);
+    b->diff_cap = 0;
+    PCX_HIP(hipMalloc(&b->d_diff, per_list * kDiffLists * sizeof(unsigned long long)));
+    b->diff_cap = per_list;
+  }
+  return PCX_OK;
+}
+
+// Where this load lists the differences from instance 0 (interval batches of
+// several instances with seeding on), after clearing the count; an empty
+// sink otherwise.
+pcx_status diff_sink(pcx_batch* b, DiffSink* ds) {
+  *ds = DiffSink{nullptr, nullptr, 0};
+  b->diffs_listed = false;
+  const pcx_engine* e = b->e;
+  if (e->cfg.base_seed == 0 || e->cfg.domain != PCX_DOMAIN_INTERVAL || b->n_inst < 2 || b->nvars <= 0)
+    return PCX_OK;
+  pcx_status st = ensure_diff_list(b);
+  if (st != PCX_OK) return st;
+  PCX_HIP(hipMemsetAsync(b->d_seedctl, 0, kSeedCtlWords * sizeof(int32_t), e->stream));
+  *ds = DiffSink{b->d_diff, b->d_seedctl, b->diff_cap};
+  return PCX_OK;
+}
+
 // After a load: which instances' images are empty, and the stores are the image.
 pcx_status image_loaded(pcx_batch* b) {
   pcx_engine* e = b->e;
+  b->diffs_listed = false;
   PCX_HIP(hipMemsetAsync(b->d_image_empty, 0, b->n_inst, e->stream));
   PCX_HIP(launch_store_empty_map(b->d_image, b->nvars, (int32_t)b->n_inst, b->d_image_empty, dom_of(e),
                                  e->stream));
@@ -1120,7 +1218,14 @@ pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub) {
   PCX_HIP(hipMemcpyAsync(d_ub, ub, n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream));
   PCX_HIP(launch_store_pack(d_lb, d_ub, b->d_image, n, e->stream));
   PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, n * sizeof(int2), hipMemcpyDeviceToDevice, e->stream));
-  return image_loaded(b);
+  pcx_status st = image_loaded(b);
+  if (st != PCX_OK) return st;
+  DiffSink ds;
+  st = diff_sink(b, &ds);
+  if (st != PCX_OK || !ds.list) return st;
+  PCX_HIP(launch_list_diffs(b->d_store, b->nvars, b->n_inst, ds, e->stream));
+  b->diffs_listed = true;
+  return PCX_OK;
 }
 
 pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_t* d_ub) {
@@ -1130,6 +1235,9 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
   PCX_HIP(hipSetDevice(e->cfg.device));
   const int64_t n = b->n_inst * b->nvars;
   PCX_HIP(hipMemsetAsync(b->d_image_empty, 0, b->n_inst, e->stream));
+  DiffSink ds;
+  pcx_status dst = diff_sink(b, &ds);
+  if (dst != PCX_OK) return dst;
   const int gi = lockstep_of(b);
   if (gi > 0) {
     // Batched runs sweep a lockstep copy first: write it in the same pass
@@ -1139,17 +1247,19 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
     pcx_status st = ensure_shadow(b, gi);
     if (st != PCX_OK) return st;
     PCX_HIP(launch_ingest_interleave(d_lb, d_ub, b->d_store, b->d_shadow, b->nvars, b->n_inst, gi,
-                                     b->d_image_empty, e->stream));
+                                     b->d_image_empty, ds, e->stream));
     b->shadow_is_image = true;
     b->shadow_gi = gi;
     b->image_in_shadow = true;
   } else {
     PCX_HIP(launch_ingest(d_lb, d_ub, b->d_image, b->d_store, n, b->nvars, b->d_image_empty, e->stream));
+    PCX_HIP(launch_list_diffs(b->d_store, b->nvars, b->n_inst, ds, e->stream));
     b->shadow_is_image = false;
     b->image_in_shadow = false;
   }
   PCX_HIP(hipStreamSynchronize(e->stream));
   b->store_is_image = true;
+  b->diffs_listed = ds.list != nullptr;
   return PCX_OK;
 }
 

@@ -66,6 +66,14 @@ struct pcx_batch {
   bool events = false;                // this run uses marked-only sweeps
   bool seeded = false;                // the next run may start from seeded marks
   bool seeded_run = false;            // this run starts with a marked sweep
+  // first sweeps seeded from instance 0's store: the values of the image
+  // that differ from instance 0's (listed by the load), and the propagators
+  // instance 0's store does not satisfy (found by the run)
+  unsigned long long* d_diff = nullptr;   // kDiffLists sub-lists of diff_cap entries
+  int64_t diff_cap = 0;
+  int32_t* d_unsat = nullptr;         // [table_n]
+  int32_t* d_seedctl = nullptr;       // differences per sub-list, then propagators in d_unsat
+  bool diffs_listed = false;          // d_diff describes the current image
   // captured group of marked sweeps (batched mode), rebuilt when any launch
   // argument it was captured with differs
   struct MarkedGraphArgs {

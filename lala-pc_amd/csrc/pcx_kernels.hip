@@ -58,6 +58,7 @@ __device__ __forceinline__ u64 load_word(const void* p) {
 __device__ __forceinline__ int2 as_itv(u64 v) {
   return make_int2((int32_t)(uint32_t)v, (int32_t)(uint32_t)(v >> 32));
 }
+__device__ __forceinline__ u64 itv_bits(int2 v) { return (u64)(uint32_t)v.x | ((u64)(uint32_t)v.y << 32); }
 
 // Meet interval `n` into variable v, whose value was read as `cur`.
 // Returns true when this call tightened a bound of v.
@@ -974,6 +975,32 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
 // slot i at word (g * nvars + v) * GI + i.  One workgroup transposes a tile
 // of GI instances x (4096 / GI) variables through LDS; both sides are
 // contiguous runs.
+// Appends (k << 32 | v) for the lanes whose d is set, with one counter
+// atomic per wave.  Called by every lane of the wave.
+__device__ __forceinline__ void diff_append(const DiffSink& ds, bool d, int64_t k, int64_t v) {
+  const u64 mask = __ballot(d);
+  if (mask == 0) return;
+  const int c = blockIdx.x % kDiffLists;
+  const int leader = __builtin_ctzll(mask);
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(&ds.count[c], (int)__popcll(mask));
+  base = __builtin_amdgcn_readlane(base, leader);
+  const int64_t at = (int64_t)base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+  if (d && at < ds.cap) ds.list[c * ds.cap + at] = ((u64)k << 32) | (uint32_t)v;
+}
+// Whether this workgroup lists differences: a sub-list that is already full
+// is not contended for (the list is unused anyway).
+__device__ __forceinline__ bool diff_listing(const DiffSink& ds) {
+  __shared__ int s_listing;
+  if (threadIdx.x == 0)
+    s_listing = ds.list != nullptr && ds.count[blockIdx.x % kDiffLists] < ds.cap;
+  __syncthreads();
+  const bool r = s_listing != 0;
+  __syncthreads();
+  return r;
+}
+
 template <int GI>
 __global__ __launch_bounds__(kBlock) void k_interleave(const u64* __restrict__ store,
                                                       u64* __restrict__ sh, int64_t nvars,
@@ -1028,17 +1055,19 @@ __global__ __launch_bounds__(kBlock) void k_ingest_interleave(const int32_t* __r
                                                              u64* __restrict__ store, u64* __restrict__ sh,
                                                              int64_t nvars, int64_t n_inst,
                                                              int64_t tiles_per_group,
-                                                             uint8_t* __restrict__ empty) {
+                                                             uint8_t* __restrict__ empty, DiffSink ds) {
   constexpr int kTV = 4096 / GI;
   __shared__ u64 tile[GI][kTV + 1];
   const int64_t g = blockIdx.x / tiles_per_group;
   const int64_t v0 = (blockIdx.x - g * tiles_per_group) * kTV;
+  const bool listing = diff_listing(ds);
 #pragma unroll
   for (int r = 0; r < 4096 / kBlock; ++r) {
     const int e = r * kBlock + threadIdx.x;
     const int i = e / kTV, j = e - i * kTV;
     const int64_t k = g * GI + i;
     u64 val = 0;
+    bool dif = false;
     if (k < n_inst && v0 + j < nvars) {
       const int64_t at = k * nvars + v0 + j;
       const int32_t l = __builtin_nontemporal_load(lb + at);
@@ -1046,7 +1075,9 @@ __global__ __launch_bounds__(kBlock) void k_ingest_interleave(const int32_t* __r
       val = (u64)(uint32_t)l | ((u64)(uint32_t)u << 32);     // int2 {lb, ub}
       store[at] = val;
       if (l > u) empty[k] = 1;
+      if (listing && k > 0) dif = l != lb[v0 + j] || u != ub[v0 + j];   // instance 0's row
     }
+    if (listing) diff_append(ds, dif, k, v0 + j);
     tile[i][j] = val;
   }
   __syncthreads();
@@ -1097,22 +1128,29 @@ __global__ __launch_bounds__(kBlock) void k_ingest_interleave4(const v4i* __rest
                                                               v4i* __restrict__ store, v4i* __restrict__ sh,
                                                               int64_t nvars, int64_t n_inst,
                                                               int64_t tiles_per_group,
-                                                              uint8_t* __restrict__ empty) {
+                                                              uint8_t* __restrict__ empty, DiffSink ds) {
   constexpr int kTV = 4096 / GI;
   constexpr int kQ = kTV / 4;                  // 4-variable groups per tile row
   __shared__ u64 tile[GI][kTV + 1];
   const int64_t g = blockIdx.x / tiles_per_group;
   const int64_t v0 = (blockIdx.x - g * tiles_per_group) * kTV;
+  const bool listing = diff_listing(ds);
 #pragma unroll
   for (int r = 0; r < 1024 / kBlock; ++r) {
     const int e = r * kBlock + threadIdx.x;
     const int i = e / kQ, j = (e - i * kQ) * 4;
     const int64_t k = g * GI + i;
     u64 w[4] = {0, 0, 0, 0};
+    int dif = 0;                               // bit q: variable j + q differs from instance 0's
     if (k < n_inst && v0 + j < nvars) {
       const int64_t at4 = (k * nvars + v0 + j) / 4;
       const v4i a = __builtin_nontemporal_load(lb + at4);
       const v4i b = __builtin_nontemporal_load(ub + at4);
+      if (listing && k > 0) {
+        const v4i ra = lb[(v0 + j) / 4], rb = ub[(v0 + j) / 4];
+        dif = ((a.x != ra.x) | (b.x != rb.x)) | (((a.y != ra.y) | (b.y != rb.y)) << 1) |
+              (((a.z != ra.z) | (b.z != rb.z)) << 2) | (((a.w != ra.w) | (b.w != rb.w)) << 3);
+      }
       const v4i p0 = {a.x, b.x, a.y, b.y};
       const v4i p1 = {a.z, b.z, a.w, b.w};
       store[2 * at4] = p0;
@@ -1122,6 +1160,10 @@ __global__ __launch_bounds__(kBlock) void k_ingest_interleave4(const v4i* __rest
       w[1] = (u64)(uint32_t)a.y | ((u64)(uint32_t)b.y << 32);
       w[2] = (u64)(uint32_t)a.z | ((u64)(uint32_t)b.z << 32);
       w[3] = (u64)(uint32_t)a.w | ((u64)(uint32_t)b.w << 32);
+    }
+    if (listing && __ballot(dif != 0)) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) diff_append(ds, (dif >> q) & 1, k, v0 + j + q);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) tile[i][j + q] = w[q];
@@ -1142,7 +1184,7 @@ __global__ __launch_bounds__(kBlock) void k_ingest_interleave4(const v4i* __rest
 
 hipError_t launch_ingest_interleave(const int32_t* lb, const int32_t* ub, int2* store, int2* shadow,
                                     int64_t nvars, int64_t n_inst, int gi, uint8_t* empty,
-                                    hipStream_t stream) {
+                                    const DiffSink& ds, hipStream_t stream) {
   if (nvars <= 0 || n_inst <= 0) return hipSuccess;
   const int64_t groups = (n_inst + gi - 1) / gi;
   const int64_t tv = 4096 / gi;
@@ -1158,9 +1200,9 @@ hipError_t launch_ingest_interleave(const int32_t* lb, const int32_t* ub, int2* 
     v4i* s4 = reinterpret_cast<v4i*>(store);
     v4i* o4 = reinterpret_cast<v4i*>(shadow);
     switch (gi) {
-      case 16: k_ingest_interleave4<16><<<(unsigned)grid, kBlock, 0, stream>>>(l4, u4, s4, o4, nvars, n_inst, tpg, empty); break;
-      case 32: k_ingest_interleave4<32><<<(unsigned)grid, kBlock, 0, stream>>>(l4, u4, s4, o4, nvars, n_inst, tpg, empty); break;
-      case 64: k_ingest_interleave4<64><<<(unsigned)grid, kBlock, 0, stream>>>(l4, u4, s4, o4, nvars, n_inst, tpg, empty); break;
+      case 16: k_ingest_interleave4<16><<<(unsigned)grid, kBlock, 0, stream>>>(l4, u4, s4, o4, nvars, n_inst, tpg, empty, ds); break;
+      case 32: k_ingest_interleave4<32><<<(unsigned)grid, kBlock, 0, stream>>>(l4, u4, s4, o4, nvars, n_inst, tpg, empty, ds); break;
+      case 64: k_ingest_interleave4<64><<<(unsigned)grid, kBlock, 0, stream>>>(l4, u4, s4, o4, nvars, n_inst, tpg, empty, ds); break;
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1168,9 +1210,9 @@ hipError_t launch_ingest_interleave(const int32_t* lb, const int32_t* ub, int2* 
   u64* st = reinterpret_cast<u64*>(store);
   u64* out = reinterpret_cast<u64*>(shadow);
   switch (gi) {
-    case 16: k_ingest_interleave<16><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, out, nvars, n_inst, tpg, empty); break;
-    case 32: k_ingest_interleave<32><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, out, nvars, n_inst, tpg, empty); break;
-    case 64: k_ingest_interleave<64><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, out, nvars, n_inst, tpg, empty); break;
+    case 16: k_ingest_interleave<16><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, out, nvars, n_inst, tpg, empty, ds); break;
+    case 32: k_ingest_interleave<32><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, out, nvars, n_inst, tpg, empty, ds); break;
+    case 64: k_ingest_interleave<64><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, out, nvars, n_inst, tpg, empty, ds); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1251,16 +1293,23 @@ __global__ __launch_bounds__(kBlock, lock_min_waves<D>()) void k_sweep_lockstep(
     const int64_t g = uni(item / ipg);
     const int64_t w0 = (item - g * ipg) * kWpi + wave * kLockWordsPerWave;
     const int64_t k = g * GI + slot;
-    bool run = false, am = true;
+    // Per-lane flags carried across the item's rounds are ints: as lane
+    // masks every round would merge them under the exec mask.
+    bool run = false;
+    int am = 1;
     InstFlags* f = flags + (k < n_inst ? k : 0);
     if (k < n_inst) {
       run = !(f->done_at >= 0 && f->done_at <= s) && f->failed_at == kNoFail;
       am = !ev.dirty || f->all_marked[(s + 1) & 1] == s + 1;
     }
     int2* sk = store + k * nvars;
-    const int2* sg = shadow + g * nvars * GI + slot;
+    // The group's copy starts at a wave-uniform address and spans less than
+    // 4 GiB (the host checks nvars * GI * 8), so an operand is one 32-bit
+    // offset from it.
+    const char* sgb = reinterpret_cast<const char*>(shadow + g * nvars * GI);
+    const uint32_t slot_off = (uint32_t)slot * 8u;
     const uint32_t* mrow = member ? member + k * words : nullptr;
-    bool chg = false, fail = false;
+    int chg = 0, fail = 0;
     int nt = 0;
     auto mark = [&](int32_t v) {
       if (am) return;
@@ -1268,7 +1317,7 @@ __global__ __launch_bounds__(kBlock, lock_min_waves<D>()) void k_sweep_lockstep(
       if (i < kLockMarkCap) { s_mk[i] = (int32_t)k; s_mv[i] = v; }
       else {
         f->all_marked[(s + 1) & 1] = s + 1;
-        am = true;
+        am = 1;
         if (ev.q) ev.qctl[kQOver + ((s + 1) & 1)] = 1;   // these marks are in no queue
       }
     };
@@ -1302,11 +1351,11 @@ __global__ __launch_bounds__(kBlock, lock_min_waves<D>()) void k_sweep_lockstep(
       auto rounds = [&](auto opc) {
         constexpr int OPC = decltype(opc)::value;
         constexpr bool kFlat = OPC >= 0 && (D == kBits || OPC < kTdiv || OPC > kEdiv);
+        constexpr int kStep = kPpi * kLockUnroll;
 #pragma unroll 1
-        for (int j0 = 0; j0 < 32; j0 += kPpi * kLockUnroll) {
+        for (int j0 = 0; j0 < 32; j0 += kStep) {
           uint32_t xo[kLockUnroll];
           int32_t y[kLockUnroll], z[kLockUnroll];
-          bool on[kLockUnroll];
           u64 wx[kLockUnroll], wy[kLockUnroll], wz[kLockUnroll];
 #pragma unroll
           for (int u = 0; u < kLockUnroll; ++u) {
@@ -1320,73 +1369,109 @@ __global__ __launch_bounds__(kBlock, lock_min_waves<D>()) void k_sweep_lockstep(
               y[u] = __shfl(ty, src, kWave);
               z[u] = __shfl(tz, src, kWave);
             }
-            on[u] = (m >> src) & 1u;
           }
           // Every lane loads (its table entry names valid variables and its
           // slot lies inside the padded copy), so the loads issue together.
 #pragma unroll
           for (int u = 0; u < kLockUnroll; ++u) {
             const int32_t x = (int32_t)(xo[u] & kVarMask);
-            wx[u] = *reinterpret_cast<const u64*>(sg + (int64_t)x * GI);
-            wy[u] = *reinterpret_cast<const u64*>(sg + (int64_t)y[u] * GI);
-            wz[u] = *reinterpret_cast<const u64*>(sg + (int64_t)z[u] * GI);
+            wx[u] = *reinterpret_cast<const u64*>(sgb + ((uint32_t)x * (uint32_t)(GI * 8) + slot_off));
+            wy[u] = *reinterpret_cast<const u64*>(sgb + ((uint32_t)y[u] * (uint32_t)(GI * 8) + slot_off));
+            wz[u] = *reinterpret_cast<const u64*>(sgb + ((uint32_t)z[u] * (uint32_t)(GI * 8) + slot_off));
           }
+          bool on[kLockUnroll];
 #pragma unroll
-          for (int u = 0; u < kLockUnroll; ++u) {
-            const int op = OPC >= 0 ? OPC : (int)(xo[u] >> kOpShift);
-            const int32_t x = (int32_t)(xo[u] & kVarMask);
-            bool tx = false, ty2 = false, tz2 = false;
-            if constexpr (D == kBits) {
-              u64 nx, ny, nz;
-              if constexpr (kFlat) {
-                deduce_bits(op, wx[u], wy[u], wz[u], nx, ny, nz);
-                const bool empty_in = !wx[u] || !wy[u] || !wz[u];
-                fail |= on[u] && empty_in;
-                const bool tight = on[u] && !empty_in &&
-                                   ((wx[u] & nx) != wx[u] || (wy[u] & ny) != wy[u] || (wz[u] & nz) != wz[u]);
-                if (tight) {
-                  u64* sk64 = reinterpret_cast<u64*>(sk);
-                  tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, x, wx[u], nx, chg, fail, nt);
-                  ty2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, y[u], wy[u], ny, chg, fail, nt);
-                  tz2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, z[u], wz[u], nz, chg, fail, nt);
-                }
+          for (int u = 0; u < kLockUnroll; ++u) on[u] = (m >> (j0 + u * kPpi + sub)) & 1u;
+          if constexpr (kFlat) {
+            // One opcode, no division: the propagators are evaluated
+            // branch-free, and what is rare (a bound tightens, a store is
+            // empty) runs under one wave-uniform branch per round, so the
+            // common round carries no per-lane control flow.
+            using Val = std::conditional_t<D == kBits, u64, int2>;
+            Val nx[kLockUnroll], ny[kLockUnroll], nz[kLockUnroll];
+            bool need[kLockUnroll];
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < kLockUnroll; ++u) {
+              bool empty_in, tight;
+              if constexpr (D == kBits) {
+                deduce_bits(OPC, wx[u], wy[u], wz[u], nx[u], ny[u], nz[u]);
+                empty_in = !wx[u] | !wy[u] | !wz[u];
+                // A deduced empty word is a tightening too.
+                tight = ((wx[u] & nx[u]) != wx[u]) | ((wy[u] & ny[u]) != wy[u]) |
+                        ((wz[u] & nz[u]) != wz[u]);
               } else {
-                if (!on[u]) continue;
-                if (!wx[u] || !wy[u] || !wz[u]) { fail = true; continue; }
-                deduce_bits(op, wx[u], wy[u], wz[u], nx, ny, nz);
-                u64* sk64 = reinterpret_cast<u64*>(sk);
-                tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, x, wx[u], nx, chg, fail, nt);
-                ty2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, y[u], wy[u], ny, chg, fail, nt);
-                tz2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, z[u], wz[u], nz, chg, fail, nt);
+                const int2 X = as_itv(wx[u]), Y = as_itv(wy[u]), Z = as_itv(wz[u]);
+                deduce<D == kItvDiv>(OPC, X, Y, Z, nx[u], ny[u], nz[u]);
+                // ub - lb saturated is negative iff the interval is empty.
+                empty_in = (ssub(X.y, X.x) | ssub(Y.y, Y.x) | ssub(Z.y, Z.x)) < 0;
+                // Bounds only move inward, so an interval differs iff it
+                // tightens; a deduced empty interval of a non-empty one
+                // tightens too.
+                tight = (itv_bits(nx[u]) != wx[u]) | (itv_bits(ny[u]) != wy[u]) |
+                        (itv_bits(nz[u]) != wz[u]);
               }
-            } else {
-              const int2 X = as_itv(wx[u]), Y = as_itv(wy[u]), Z = as_itv(wz[u]);
-              int2 nx, ny, nz;
-              if constexpr (kFlat) {
-                deduce<D == kItvDiv>(op, X, Y, Z, nx, ny, nz);
-                const bool empty_in = X.x > X.y || Y.x > Y.y || Z.x > Z.y;
-                fail |= on[u] && (empty_in || nx.x > nx.y || ny.x > ny.y || nz.x > nz.y);
-                const bool tight = on[u] && !empty_in &&
-                                   (nx.x > X.x || nx.y < X.y || ny.x > Y.x || ny.y < Y.y ||
-                                    nz.x > Z.x || nz.y < Z.y);
-                if (tight) {
-                  tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, x, X, nx, chg, fail, nt);
-                  ty2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, y[u], Y, ny, chg, fail, nt);
-                  tz2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, z[u], Z, nz, chg, fail, nt);
+              need[u] = on[u] & (empty_in | tight);
+              any |= need[u];
+            }
+            if (__builtin_expect(__ballot(any) != 0, 0)) {
+#pragma unroll
+              for (int u = 0; u < kLockUnroll; ++u) {
+                if (!need[u]) continue;
+                const int32_t x = (int32_t)(xo[u] & kVarMask);
+                bool tx, ty2, tz2, c = false, fl = false;
+                if constexpr (D == kBits) {
+                  if (!wx[u] || !wy[u] || !wz[u]) { fail = 1; continue; }
+                  u64* sk64 = reinterpret_cast<u64*>(sk);
+                  tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, x, wx[u], nx[u], c, fl, nt);
+                  ty2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, y[u], wy[u], ny[u], c, fl, nt);
+                  tz2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, z[u], wz[u], nz[u], c, fl, nt);
+                } else {
+                  const int2 X = as_itv(wx[u]), Y = as_itv(wy[u]), Z = as_itv(wz[u]);
+                  if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = 1; continue; }
+                  if (nx[u].x > nx[u].y || ny[u].x > ny[u].y || nz[u].x > nz[u].y) fl = true;
+                  tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, x, X, nx[u], c, fl, nt);
+                  ty2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, y[u], Y, ny[u], c, fl, nt);
+                  tz2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, z[u], Z, nz[u], c, fl, nt);
                 }
-              } else {
-                if (!on[u]) continue;
-                if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = true; continue; }
-                deduce<D == kItvDiv>(op, X, Y, Z, nx, ny, nz);
-                if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fail = true;
-                tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, x, X, nx, chg, fail, nt);
-                ty2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, y[u], Y, ny, chg, fail, nt);
-                tz2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, z[u], Z, nz, chg, fail, nt);
+                chg |= c;
+                fail |= fl;
+                if (tx) mark(x);
+                if (ty2) mark(y[u]);
+                if (tz2) mark(z[u]);
               }
             }
-            if (tx) mark(x);
-            if (ty2) mark(y[u]);
-            if (tz2) mark(z[u]);
+          } else {
+#pragma unroll
+            for (int u = 0; u < kLockUnroll; ++u) {
+              if (!on[u]) continue;
+              const int op = OPC >= 0 ? OPC : (int)(xo[u] >> kOpShift);
+              const int32_t x = (int32_t)(xo[u] & kVarMask);
+              bool tx, ty2, tz2, c = false, fl = false;
+              if constexpr (D == kBits) {
+                if (!wx[u] || !wy[u] || !wz[u]) { fail = 1; continue; }
+                u64 nx, ny, nz;
+                deduce_bits(op, wx[u], wy[u], wz[u], nx, ny, nz);
+                u64* sk64 = reinterpret_cast<u64*>(sk);
+                tx = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, x, wx[u], nx, c, fl, nt);
+                ty2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, y[u], wy[u], ny, c, fl, nt);
+                tz2 = meet_word<__HIP_MEMORY_SCOPE_AGENT>(sk64, z[u], wz[u], nz, c, fl, nt);
+              } else {
+                const int2 X = as_itv(wx[u]), Y = as_itv(wy[u]), Z = as_itv(wz[u]);
+                if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) { fail = 1; continue; }
+                int2 nx, ny, nz;
+                deduce<D == kItvDiv>(op, X, Y, Z, nx, ny, nz);
+                if (nx.x > nx.y || ny.x > ny.y || nz.x > nz.y) fl = true;
+                tx = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, x, X, nx, c, fl, nt);
+                ty2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, y[u], Y, ny, c, fl, nt);
+                tz2 = meet_itv<__HIP_MEMORY_SCOPE_AGENT>(sk, z[u], Z, nz, c, fl, nt);
+              }
+              chg |= c;
+              fail |= fl;
+              if (tx) mark(x);
+              if (ty2) mark(y[u]);
+              if (tz2) mark(z[u]);
+            }
           }
         }
       };
@@ -2656,6 +2741,151 @@ __global__ __launch_bounds__(kBlock) void k_branch_seed(EventSet ev, const int32
 hipError_t launch_branch_seed(const EventSet& ev, const int32_t* var, int64_t n_child, hipStream_t stream) {
   const int64_t grid = n_child < 65536 ? n_child : 65536;
   if (grid > 0) k_branch_seed<<<(unsigned)grid, kBlock, 0, stream>>>(ev, var, n_child);
+  return hipGetLastError();
+}
+
+// ---- first sweeps seeded from a base store --------------------------------
+// Instances loaded together usually share most of their values (search nodes
+// of one root).  A propagator whose three operands hold the same values in
+// instance k as in instance 0 computes the same thing in both; if it does
+// not tighten instance 0's store, it does not tighten k's.  So a batch's
+// first sweep needs only the propagators adjacent to a value that differs
+// from instance 0's, and those that instance 0's store does not satisfy.
+
+// Differences from instance 0 in stores already in device memory: one
+// workgroup per (instance, 2048-variable block).
+__global__ __launch_bounds__(kBlock) void k_list_diffs(const u64* __restrict__ store, int64_t nvars,
+                                                       int64_t n_inst, int64_t vblocks, DiffSink ds) {
+  constexpr int kPer = 8;
+  const bool listing = diff_listing(ds);
+  if (!listing) return;
+  const int64_t k = 1 + blockIdx.x / vblocks;
+  const int64_t v0 = (blockIdx.x % vblocks) * (kBlock * kPer);
+  if (k >= n_inst) return;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int64_t v = v0 + q * kBlock + threadIdx.x;
+    const bool d = v < nvars && store[k * nvars + v] != store[v];
+    diff_append(ds, d, k, v);
+  }
+}
+
+hipError_t launch_list_diffs(const int2* store, int64_t nvars, int64_t n_inst, const DiffSink& ds,
+                             hipStream_t stream) {
+  if (!ds.list || nvars <= 0 || n_inst < 2) return hipSuccess;
+  const int64_t vblocks = (nvars + kBlock * 8 - 1) / (kBlock * 8);
+  const int64_t grid = (n_inst - 1) * vblocks;
+  if (grid > 0x7fffffff) return hipErrorInvalidValue;
+  k_list_diffs<<<(unsigned)grid, kBlock, 0, stream>>>(reinterpret_cast<const u64*>(store), nvars, n_inst,
+                                                      vblocks, ds);
+  return hipGetLastError();
+}
+
+// One thread per propagator, on the base store alone (nothing is written
+// back): positions of the propagators that would tighten it or read an
+// empty interval.
+__global__ __launch_bounds__(kBlock) void k_base_probe(Table t, const int2* __restrict__ base,
+                                                       int32_t* __restrict__ list, int32_t* count) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t p0 = (int64_t)blockIdx.x * kBlock; p0 < t.n; p0 += stride) {
+    const int64_t p = p0 + threadIdx.x;
+    bool need = false;
+    if (p < t.n) {
+      const uint32_t xo = t.xo[p];
+      const int2 X = base[xo & kVarMask], Y = base[t.y[p]], Z = base[t.z[p]];
+      if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) {
+        need = true;
+      } else {
+        int2 nx, ny, nz;
+        deduce<true>((int)(xo >> kOpShift), X, Y, Z, nx, ny, nz);
+        need = nx.x > X.x || nx.y < X.y || ny.x > Y.x || ny.y < Y.y || nz.x > Z.x || nz.y < Z.y;
+      }
+    }
+    // the same wave-aggregated append, of table positions
+    const u64 mask = __ballot(need);
+    if (mask) {
+      const int leader = __builtin_ctzll(mask);
+      int b = 0;
+      if (lane_id() == leader) b = atomicAdd(count, (int)__popcll(mask));
+      b = __builtin_amdgcn_readlane(b, leader);
+      const int at = b + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+      if (need) list[at] = (int32_t)p;     // at < t.n: each position is listed once
+    }
+  }
+}
+
+hipError_t launch_base_probe(const Table& t, const int2* base, int32_t* list, int32_t* count, int dom,
+                             hipStream_t stream) {
+  if (dom != kItv && dom != kItvDiv) return hipErrorInvalidValue;
+  if (t.n <= 0) return hipSuccess;
+  int64_t grid = (t.n + kBlock - 1) / kBlock;
+  if (grid > 65536) grid = 65536;
+  k_base_probe<<<(unsigned)grid, kBlock, 0, stream>>>(t, base, list, count);
+  return hipGetLastError();
+}
+
+// One seed: propagator p of instance k marked for sweep 0 (plane 0), when
+// its subset holds it; a bit this call sets is also staged for queue plane 0
+// when the run sweeps from queues.
+__device__ __forceinline__ void seed_prop(const Marker& mk, const EventSet& ev, const QueueStage& qs,
+                                          int64_t k, int32_t p) {
+  const uint32_t bit = 1u << (p & 31);
+  if (mk.member && !(mk.member[p >> 5] & bit)) return;
+  if (atomicOr(&mk.dirty[p >> 5], bit) & bit) return;
+  if (mk.sum) set_summary(nullptr, mk.sum, p >> kChunkShift);
+  if (ev.q) stage_entry(qs, ev, 0, ((unsigned long long)k << 32) | (uint32_t)p);
+}
+
+// Plane-0 marks, one block-wide step at a time (each step ends with a flush
+// of the staged queue entries): steps over the difference sub-lists take 64
+// slots, four threads walking a filled slot's adjacency; the steps after
+// them take kBlock (propagator of `unsat`, instance) pairs.
+constexpr int kSeedSlotsPerStep = kBlock / 4;
+__global__ __launch_bounds__(kBlock) void k_seed_from_base(EventSet ev, DiffSink ds,
+                                                           const int32_t* __restrict__ unsat,
+                                                           int64_t n_unsat) {
+  __shared__ unsigned long long s_qbuf[kQStage];
+  __shared__ int32_t s_qn;
+  const QueueStage qs{s_qbuf, &s_qn};
+  if (threadIdx.x == 0) s_qn = 0;
+  __syncthreads();
+  const int64_t n_slots = kDiffLists * ds.cap;
+  const int64_t diff_steps = (n_slots + kSeedSlotsPerStep - 1) / kSeedSlotsPerStep;
+  const int64_t n_pairs = n_unsat * ev.n_inst;
+  const int64_t steps = diff_steps + (n_pairs + kBlock - 1) / kBlock;
+  for (int64_t it = blockIdx.x; it < steps; it += gridDim.x) {
+    if (it < diff_steps) {
+      const int64_t slot = it * kSeedSlotsPerStep + threadIdx.x / 4;
+      if (slot < n_slots && slot % ds.cap < ds.count[slot / ds.cap]) {
+        const u64 d = ds.list[slot];
+        const int64_t k = (int64_t)(d >> 32);
+        const int32_t v = (int32_t)(uint32_t)d;
+        const Marker mk = marker_for(&ev, k, -1);          // sweep -1 marks plane 0 (sweep 0's)
+        for (int32_t a = ev.adj_off[v] + (threadIdx.x & 3); a < ev.adj_off[v + 1]; a += 4)
+          seed_prop(mk, ev, qs, k, ev.adj[a]);
+      }
+    } else {
+      const int64_t j = (it - diff_steps) * kBlock + threadIdx.x;
+      if (j < n_pairs) {
+        const int64_t k = j % ev.n_inst;
+        seed_prop(marker_for(&ev, k, -1), ev, qs, k, unsat[j / ev.n_inst]);
+      }
+    }
+    if (ev.q) {
+      __syncthreads();                                   // the step's entries are staged
+      flush_stage(qs, ev, 0);
+    }
+  }
+}
+
+hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, const int32_t* unsat,
+                                 int64_t n_unsat, hipStream_t stream) {
+  const int64_t steps = (kDiffLists * diffs.cap + kSeedSlotsPerStep - 1) / kSeedSlotsPerStep +
+                        (n_unsat * ev.n_inst + kBlock - 1) / kBlock;
+  if (steps <= 0 || !ev.dirty) return hipSuccess;
+  const int64_t grid = steps > 4096 ? 4096 : steps;
+  k_seed_from_base<<<(unsigned)grid, kBlock, 0, stream>>>(ev, diffs, unsat, n_unsat);
   return hipGetLastError();
 }
 

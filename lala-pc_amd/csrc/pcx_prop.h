@@ -372,12 +372,16 @@ __device__ __forceinline__ void deduce(int op, int2 X, int2 Y, int2 Z, int2& nx,
       return;
     }
     case kAdd: {
-      if (yl != kMinf && zl != kMinf) xl = max(xl, sadd(yl, zl));
-      if (yu != kInf && zu != kInf)   xu = min(xu, sadd(yu, zu));
-      if (xl != kMinf && zu != kInf)  yl = max(yl, ssub(xl, zu));
-      if (xu != kInf && zl != kMinf)  yu = min(yu, ssub(xu, zl));
-      if (xl != kMinf && yu != kInf)  zl = max(zl, ssub(xl, yu));
-      if (xu != kInf && yl != kMinf)  zu = min(zu, ssub(xu, yl));
+      // A sum or difference involving an infinite bound is that bound: the
+      // projection then leaves its target alone.  Each test "a is not -inf
+      // and b is not +inf" is min(a, ~b) != kMinf (~kInf == kMinf), one
+      // comparison instead of two.
+      if (min(yl, zl) != kMinf)  xl = max(xl, sadd(yl, zl));
+      if (max(yu, zu) != kInf)   xu = min(xu, sadd(yu, zu));
+      if (min(xl, ~zu) != kMinf) yl = max(yl, ssub(xl, zu));
+      if (max(xu, ~zl) != kInf)  yu = min(yu, ssub(xu, zl));
+      if (min(xl, ~yu) != kMinf) zl = max(zl, ssub(xl, yu));
+      if (max(xu, ~yl) != kInf)  zu = min(zu, ssub(xu, yl));
       break;
     }
     case kMul: {

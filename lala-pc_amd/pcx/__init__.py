@@ -51,7 +51,7 @@ class Config(ctypes.Structure):
                 ("domain", ctypes.c_int32), ("compact_tables", ctypes.c_int32),
                 ("stream_tables", ctypes.c_int32), ("event_driven", ctypes.c_int32),
                 ("lockstep", ctypes.c_int32), ("mark_queue", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 1)]
+                ("base_seed", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):

@@ -213,7 +213,7 @@ def test_marked_sweeps_grouped_items(gpu):
     with full sweeps only."""
     V, P, n_inst = 50000, 200000, 2048
     op, x, y, z, lb, ub, w = _network(21, V, P)
-    pir = pcx.PIR(event_driven=1)
+    pir = pcx.PIR(event_driven=1, base_seed=0)     # a full first sweep, then marked ones
     pir.tell(op, x, y, z, lb, ub)
     root_lb, root_ub = lb.copy(), ub.copy()
     assert not pir.fixpoint(root_lb, root_ub).failed
@@ -269,7 +269,7 @@ def test_lockstep_full_sweeps(gpu, gi, events):
     equals the oracle, and a rerun after reset gives the same stores."""
     V, P, n_inst = 5000, 20000, 100
     op, x, y, z, lb, ub, w = _network(41, V, P)
-    pir = pcx.PIR(event_driven=events, lockstep=gi)
+    pir = pcx.PIR(event_driven=events, lockstep=gi, base_seed=0)
     pir.tell(op, x, y, z, lb, ub)
     root_lb, root_ub = lb.copy(), ub.copy()
     assert not pir.fixpoint(root_lb, root_ub).failed
@@ -707,7 +707,7 @@ def test_queued_marked_sweeps(gpu, gi, queue):
     equals the oracle, subsets and failures included."""
     V, P, n = 3000, 12000, 200
     op, x, y, z, lb, ub, _ = _network(41 + gi, V, P)
-    pir = pcx.PIR(lockstep=gi, event_driven=1, mark_queue=queue)
+    pir = pcx.PIR(lockstep=gi, event_driven=1, mark_queue=queue, base_seed=0)
     pir.tell(op, x, y, z, lb, ub)
     root_l, root_u = lb.copy(), ub.copy()
     assert not pir.fixpoint(root_l, root_u).failed
@@ -755,3 +755,118 @@ def test_queued_marked_sweeps_plain(gpu, queue):
     rg, _ = b.run(pcx.MODE_PLAIN)
     lg, ug = b.read()
     _compare(lg, ug, rg, lo_o, hi_o, ros, V, f"plain queue {queue}")
+
+
+def _seed_case(seed, V, P, n_inst, subsets):
+    op, x, y, z, lb, ub, w = _network(seed, V, P)
+    pir = pcx.PIR(event_driven=1)
+    pir.tell(op, x, y, z, lb, ub)
+    root_lb, root_ub = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_lb, root_ub).failed
+    slb, sub = pcx.gen_splits(seed + 1, root_lb, root_ub, n_inst, 1, 20)
+    off = idx = None
+    if subsets:
+        off, idx = pcx.gen_subsets(seed + 2, P, n_inst, 500, P)
+    return pir, lb, ub, root_lb, root_ub, slb, sub, off, idx
+
+
+def _oracle_batch(pir, slb, sub, V, off, idx):
+    top, tx, ty, tz = pir.table()
+    lo, hi = slb.copy(), sub.copy()
+    kw = {} if off is None else {"idx": idx.astype(np.int64), "off": off}
+    res = ol.fixpoint_batch(top, tx, ty, tz, lo, hi, V, threads=8, **kw)
+    return lo, hi, res
+
+
+@pytest.mark.parametrize("subsets", [True, False], ids=["subsets", "whole"])
+@pytest.mark.parametrize("load", ["host", "device"])
+@pytest.mark.parametrize("gi", [0, 64])
+def test_first_sweep_seeded_from_instance_zero(gpu, gi, load, subsets):
+    """Splits of the root fixpoint share all but a few values with instance
+    0: the run evaluates first only the propagators adjacent to a value that
+    differs from instance 0's and those instance 0's store does not satisfy
+    (no full sweep), and every instance equals the oracle and the same run
+    with a full first sweep; after a reset the seeded run repeats.  Loads from
+    host and device memory, with and without lockstep copies and subsets;
+    instance 5 holds an empty interval."""
+    from pcx.device import DeviceArray
+    V, P, n_inst = 4000, 16000, 150
+    pir, lb, ub, rl, ru, slb, sub, off, idx = _seed_case(81, V, P, n_inst, subsets)
+    slb[5 * V + 17], sub[5 * V + 17] = 9, 3                 # instance 5 starts failed
+    pir.set_config(lockstep=gi)
+    lo_o, hi_o, res_o = _oracle_batch(pir, slb, sub, V, off, idx)
+    b = pcx.Batch(pir, n_inst, V, off, idx) if subsets else pcx.Batch(pir, n_inst, V)
+    dev = []
+    if load == "host":
+        b.load(slb, sub)
+    else:
+        dev = [DeviceArray(slb), DeviceArray(sub)]
+        b.load_device(dev[0].ptr, dev[1].ptr)
+    pairs = int(off[-1]) if subsets else n_inst * P
+    res_g, st = b.run(pcx.MODE_BATCHED)
+    assert st.full_evals == 0 and st.evals < pairs // 4, (st.full_evals, st.evals, pairs)
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lo_o, hi_o, res_o, V, f"seeded, lockstep {gi}, {load} load")
+    assert res_g[5].failed
+    b.reset()
+    res_2, st2 = b.run(pcx.MODE_BATCHED)
+    assert st2.full_evals == 0
+    lb_2, ub_2 = b.read()
+    _compare(lb_2, ub_2, res_2, lo_o, hi_o, res_o, V, "seeded after reset")
+    pir.set_config(base_seed=0)
+    b.reset()
+    res_f, stf = b.run(pcx.MODE_BATCHED)
+    assert stf.full_evals > 0
+    lb_f, ub_f = b.read()
+    _compare(lb_f, ub_f, res_f, lo_o, hi_o, res_o, V, "full first sweep")
+    for d in dev:
+        d.close()
+
+
+@pytest.mark.parametrize("case", ["widened", "empty", "unpropagated"])
+def test_seeded_first_sweep_when_instance_zero_is_not_a_fixpoint(gpu, case):
+    """Instance 0 itself need not be a fixpoint: with one variable widened
+    back to its told bounds, the propagators its store does not satisfy are
+    seeded in every instance; with an empty interval in instance 0, the
+    propagators reading it are; an unpropagated instance 0 leaves too many
+    such propagators, and the run takes a full first sweep.  Every instance
+    equals the oracle."""
+    V, P, n_inst = 4000, 16000, 120
+    pir, lb, ub, rl, ru, slb, sub, off, idx = _seed_case(91, V, P, n_inst, True)
+    top, tx, ty, tz = pir.table()
+    if case == "widened":
+        v = int(tx[0])
+        slb[v], sub[v] = lb[v], ub[v]
+    elif case == "empty":
+        slb[33], sub[33] = 5, 4
+    else:
+        slb[:V], sub[:V] = lb, ub
+    lo_o, hi_o, res_o = _oracle_batch(pir, slb, sub, V, off, idx)
+    b = pcx.Batch(pir, n_inst, V, off, idx)
+    b.load(slb, sub)
+    res_g, st = b.run(pcx.MODE_BATCHED)
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lo_o, hi_o, res_o, V, f"instance 0 {case}")
+    if case == "unpropagated":
+        assert st.full_evals > 0
+    else:
+        assert st.full_evals == 0
+    if case == "empty":
+        assert res_g[0].failed
+
+
+@pytest.mark.parametrize("queue", [0, 10], ids=["bitmaps", "overflow"])
+def test_seeded_first_sweep_marks_from_bitmaps(gpu, queue):
+    """Seeded first sweeps without queues (mark bitmaps only), and with a
+    queue of 1,024 entries that the seeds overflow (the run sweeps from the
+    bitmaps from sweep 0 on): every instance equals the oracle."""
+    V, P, n_inst = 4000, 16000, 150
+    pir, lb, ub, rl, ru, slb, sub, off, idx = _seed_case(101, V, P, n_inst, True)
+    pir.set_config(mark_queue=queue)
+    lo_o, hi_o, res_o = _oracle_batch(pir, slb, sub, V, off, idx)
+    b = pcx.Batch(pir, n_inst, V, off, idx)
+    b.load(slb, sub)
+    res_g, st = b.run(pcx.MODE_BATCHED)
+    assert st.full_evals == 0
+    lb_g, ub_g = b.read()
+    _compare(lb_g, ub_g, res_g, lo_o, hi_o, res_o, V, f"seeded, mark_queue {queue}")
