@@ -103,11 +103,12 @@ typedef struct {
   int32_t base_seed;        /* batched mode, event-driven: a run that starts from
                                the loaded stores first evaluates only the
                                propagators adjacent to a value that differs from
-                               instance 0's store and those that instance 0's
-                               store does not satisfy (the others compute in
-                               instance k what they compute in instance 0), when
-                               that is much less than a full sweep; 0 off, 1 or
-                               -1 (auto) on.  The fixpoints are the same.        */
+                               a base store (per variable the value two of
+                               instances 0-2 hold) and those the base store does
+                               not satisfy (the others compute in instance k what
+                               they compute on the base), when that is much less
+                               than a full sweep; 0 off, 1 or -1 (auto) on.  The
+                               fixpoints are the same.                           */
 } pcx_config;
 
 typedef struct {

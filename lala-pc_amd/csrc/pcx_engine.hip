@@ -648,17 +648,18 @@ pcx_status ensure_shadow(pcx_batch* b, int gi) {
 // marks, or finds a plane that lost marks, from which sweep on the bitmap
 // sweeps take over.
 // A run that starts from the loaded stores, whose load listed the values that
-// differ from instance 0's, needs in its first sweep only (a) the member
-// propagators adjacent to a listed value of their instance and (b) the
-// propagators instance 0's store does not satisfy, in every instance that
-// holds them: any other propagator reads in instance k the values it reads
-// in instance 0, where it tightens nothing.  These are seeded as the first
+// differ from the base store (the values instances 0-2 agree on), needs in
+// its first sweep only (a) the member propagators adjacent to a listed value
+// of their instance and (b) the propagators the base store does not satisfy,
+// in every instance that holds them: any other propagator reads in instance
+// k the values it reads in the base store, where it tightens nothing.  These are seeded as the first
 // sweep's marks, which makes that sweep a marked sweep, when their count is
 // under 1/16 of the pairs a full sweep evaluates (and the list is complete).
 // This decides; the run launches the seeding (*n_unsat propagators listed
 // in d_unsat) with its own event set.
-pcx_status plan_seed(pcx_batch* b, bool* seeded, int64_t* n_unsat_out) {
+pcx_status plan_seed(pcx_batch* b, bool* seeded, int64_t* n_diff_out, int64_t* n_unsat_out) {
   *seeded = false;
+  *n_diff_out = 0;
   *n_unsat_out = 0;
   pcx_engine* e = b->e;
   if (!b->events || b->seeded_run || !b->run_from_image || !b->diffs_listed || e->cfg.base_seed == 0 ||
@@ -667,7 +668,7 @@ pcx_status plan_seed(pcx_batch* b, bool* seeded, int64_t* n_unsat_out) {
   if (!b->d_unsat) PCX_HIP(hipMalloc(&b->d_unsat, b->table_n * sizeof(int32_t)));
   int32_t* unsat_n = b->d_seedctl + kDiffLists;
   PCX_HIP(hipMemsetAsync(unsat_n, 0, sizeof(int32_t), e->stream));
-  PCX_HIP(launch_base_probe(table_of(e), b->d_store, b->d_unsat, unsat_n, dom_of(e), e->stream));
+  PCX_HIP(launch_base_probe(table_of(e), b->d_base, b->d_unsat, unsat_n, dom_of(e), e->stream));
   int32_t ctl[kDiffLists + 1];
   pcx_status st = read_small(b, b->d_seedctl, sizeof(ctl), ctl);
   if (st != PCX_OK) return st;
@@ -682,15 +683,16 @@ pcx_status plan_seed(pcx_batch* b, bool* seeded, int64_t* n_unsat_out) {
   if ((n_diff * per_var + n_unsat * b->n_inst) * 16 > pairs) return PCX_OK;
   *seeded = true;
   *n_unsat_out = n_unsat;
+  *n_diff_out = n_diff;
   return PCX_OK;
 }
 
 pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   pcx_engine* e = b->e;
   bool base_seeded = false;
-  int64_t n_unsat = 0;
+  int64_t n_diff = 0, n_unsat = 0;
   {
-    pcx_status ss = plan_seed(b, &base_seeded, &n_unsat);
+    pcx_status ss = plan_seed(b, &base_seeded, &n_diff, &n_unsat);
     if (ss != PCX_OK) return ss;
     if (base_seeded) b->seeded_run = true;
   }
@@ -721,8 +723,8 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     if (base_seeded) PCX_HIP(hipMemsetAsync(b->d_qctl + kQSweep, 0xff, sizeof(int32_t), e->stream));
   }
   if (base_seeded)
-    PCX_HIP(launch_seed_from_base(ev, DiffSink{b->d_diff, b->d_seedctl, b->diff_cap}, b->d_unsat, n_unsat,
-                                  e->stream));
+    PCX_HIP(launch_seed_from_base(ev, DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base}, n_diff,
+                                  b->d_unsat, n_unsat, e->stream));
   auto launch_marked = [&]() -> pcx_status {
     PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
                             &ev));
@@ -1152,7 +1154,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   if (b->queue_graph) (void)hipGraphExecDestroy(b->queue_graph);
   if (b->plain_queue_graph) (void)hipGraphExecDestroy(b->plain_queue_graph);
   dfree(b->d_q); dfree(b->d_qctl);
-  dfree(b->d_diff); dfree(b->d_unsat); dfree(b->d_seedctl);
+  dfree(b->d_diff); dfree(b->d_unsat); dfree(b->d_seedctl); dfree(b->d_base);
   delete b;
 }
 
@@ -1165,6 +1167,7 @@ pcx_status ensure_diff_list(pcx_batch* b) {
   // unevenly when a load has few of them.
   const int64_t per_list = std::max<int64_t>(2 * ((want + kDiffLists - 1) / kDiffLists), 256);
   if (!b->d_seedctl) PCX_HIP(hipMalloc(&b->d_seedctl, kSeedCtlWords * sizeof(int32_t)));
+  if (!b->d_base) PCX_HIP(hipMalloc(&b->d_base, b->nvars * sizeof(int2)));
   if (b->diff_cap < per_list) {
     dfree(b->d_diff);
     b->diff_cap = 0;
@@ -1174,7 +1177,7 @@ pcx_status ensure_diff_list(pcx_batch* b) {
   return PCX_OK;
 }
 
-// Where this load lists the differences from instance 0 (interval batches of
+// Where this load lists the differences from the base store (interval batches of
 // several instances with seeding on), after clearing the count; an empty
 // sink otherwise.
 pcx_status diff_sink(pcx_batch* b, DiffSink* ds) {
@@ -1186,7 +1189,7 @@ pcx_status diff_sink(pcx_batch* b, DiffSink* ds) {
   pcx_status st = ensure_diff_list(b);
   if (st != PCX_OK) return st;
   PCX_HIP(hipMemsetAsync(b->d_seedctl, 0, kSeedCtlWords * sizeof(int32_t), e->stream));
-  *ds = DiffSink{b->d_diff, b->d_seedctl, b->diff_cap};
+  *ds = DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base};
   return PCX_OK;
 }
 

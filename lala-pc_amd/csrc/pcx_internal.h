@@ -66,9 +66,11 @@ struct pcx_batch {
   bool events = false;                // this run uses marked-only sweeps
   bool seeded = false;                // the next run may start from seeded marks
   bool seeded_run = false;            // this run starts with a marked sweep
-  // first sweeps seeded from instance 0's store: the values of the image
-  // that differ from instance 0's (listed by the load), and the propagators
-  // instance 0's store does not satisfy (found by the run)
+  // first sweeps seeded from a base store (per variable the value instances
+  // 0-2 agree on): the base, the values of the image that differ from it
+  // (both written by the load), and the propagators the base does not
+  // satisfy (found by the run)
+  int2* d_base = nullptr;             // [nvars]
   unsigned long long* d_diff = nullptr;   // kDiffLists sub-lists of diff_cap entries
   int64_t diff_cap = 0;
   int32_t* d_unsat = nullptr;         // [table_n]

@@ -188,32 +188,37 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
 // bitmaps of a subset batch (nullptr: every propagator).
 hipError_t launch_interleave(const int2* store, int2* shadow, int64_t nvars, int64_t n_inst, int gi,
                              hipStream_t st);
-// Where a load lists the values that differ from the first instance's
-// store: entry (instance << 32 | variable) for every such value of
-// instances 1.., appended in no particular order to one of kDiffLists
-// sub-lists (the workgroup index picks it, so each sub-list's counter is
-// updated from one XCD only: a single counter for the whole device
-// serialized its atomics across XCDs).  Sub-list c holds
-// list[c * cap, c * cap + count[c]); a count past cap means the list is
-// incomplete (and unused); a workgroup stops listing once it sees its
-// sub-list full.  list == nullptr: nothing is listed.
+// Where a load lists how its stores differ from a base store: the base
+// holds, per variable, the value instances 0, 1 and 2 agree on (two of the
+// three; instance 0's when they all differ or the batch is smaller), i.e.
+// the value most instances of a search-node batch share.  Entry
+// (instance << 32 | variable) for every value that differs from the base,
+// appended in no particular order to one of kDiffLists sub-lists (the
+// workgroup index picks it, so each sub-list's counter is updated from one
+// XCD only: a single counter for the whole device serialized its atomics
+// across XCDs).  Sub-list c holds list[c * cap, c * cap + count[c]); a count
+// past cap means the list is incomplete (and unused); a workgroup stops
+// listing once it sees its sub-list full.  list == nullptr: nothing is
+// listed.
 constexpr int kDiffLists = 256;
 struct DiffSink {
   unsigned long long* list;   // [kDiffLists * cap]
   int32_t* count;             // [kDiffLists]
   int64_t cap;                // entries per sub-list
+  int2* base;                 // [nvars]: the base store, written by the load
 };
 // Device-resident lb / ub (n_inst * nvars each) into the stores and their
 // interleaved copy in one pass; empty[k] = 1 where instance k has lb > ub
-// (empty zeroed by the caller); differences from the first instance into ds.
+// (empty zeroed by the caller); the base store and the differences from it
+// into ds.
 hipError_t launch_ingest_interleave(const int32_t* lb, const int32_t* ub, int2* store, int2* shadow,
                                     int64_t nvars, int64_t n_inst, int gi, uint8_t* empty,
                                     const DiffSink& ds, hipStream_t st);
-// The same differences listed from stores already in device memory
+// The same base and differences from stores already in device memory
 // (n_inst * nvars words, row by row).
 hipError_t launch_list_diffs(const int2* store, int64_t nvars, int64_t n_inst, const DiffSink& ds,
                              hipStream_t st);
-// Propagators the store `base` (one instance, nvars words) does not satisfy:
+// Propagators the store `base` (nvars words) does not satisfy:
 // those that would tighten it, or that read an empty interval.  Their table
 // positions are appended to list (room for t.n), their number to *count.
 hipError_t launch_base_probe(const Table& t, const int2* base, int32_t* list, int32_t* count,
@@ -222,8 +227,8 @@ hipError_t launch_base_probe(const Table& t, const int2* base, int32_t* list, in
 // every listed difference (k, v) (complete sub-lists) the member
 // propagators of instance k adjacent to v, and every propagator of `unsat`
 // in every instance whose subset holds it.
-hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, const int32_t* unsat,
-                                 int64_t n_unsat, hipStream_t st);
+hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, int64_t n_diffs,
+                                 const int32_t* unsat, int64_t n_unsat, hipStream_t st);
 // dst (n_inst * nvars words, row by row) from an interleaved copy
 hipError_t launch_deinterleave(const int2* shadow, int2* dst, int64_t nvars, int64_t n_inst, int gi,
                                hipStream_t st);

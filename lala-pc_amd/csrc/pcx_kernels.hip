@@ -989,12 +989,41 @@ __device__ __forceinline__ void diff_append(const DiffSink& ds, bool d, int64_t 
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
   if (d && at < ds.cap) ds.list[c * ds.cap + at] = ((u64)k << 32) | (uint32_t)v;
 }
-// Whether this workgroup lists differences: a sub-list that is already full
-// is not contended for (the list is unused anyway).
+// The base value of one variable: what two of instances 0, 1 and 2 hold
+// (instance 0's when no two agree or the batch is smaller).
+__device__ __forceinline__ u64 base_value(u64 a, u64 b, u64 c, int64_t n_inst) {
+  if (n_inst < 3) return a;
+  return (a == b || a == c) ? a : (b == c ? b : a);
+}
+__device__ __forceinline__ u64 pack_bounds(int32_t l, int32_t u) {
+  return (u64)(uint32_t)l | ((u64)(uint32_t)u << 32);
+}
+// Base values of a tile's variables [v0, v0 + n) into LDS (s_base), read from
+// rows 0-2 of lb / ub; the workgroups of group 0 also write them out.
+__device__ __forceinline__ void tile_base(const DiffSink& ds, const int32_t* lb, const int32_t* ub,
+                                          int64_t nvars, int64_t n_inst, int64_t v0, int n, bool writer,
+                                          u64* s_base) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int64_t v = v0 + t;
+    if (v >= nvars) continue;
+    const u64 a = pack_bounds(lb[v], ub[v]);
+    const u64 b = n_inst > 1 ? pack_bounds(lb[nvars + v], ub[nvars + v]) : a;
+    const u64 c = n_inst > 2 ? pack_bounds(lb[2 * nvars + v], ub[2 * nvars + v]) : a;
+    const u64 r = base_value(a, b, c, n_inst);
+    s_base[t] = r;
+    if (writer) reinterpret_cast<u64*>(ds.base)[v] = r;
+  }
+  __syncthreads();
+}
+
+// Whether this workgroup lists differences: a sub-list that has already
+// lost entries (count past cap) is not contended for, as the list is unused
+// anyway.  (A full sub-list is still appended to, so that its count passes
+// cap and records the loss.)
 __device__ __forceinline__ bool diff_listing(const DiffSink& ds) {
   __shared__ int s_listing;
   if (threadIdx.x == 0)
-    s_listing = ds.list != nullptr && ds.count[blockIdx.x % kDiffLists] < ds.cap;
+    s_listing = ds.list != nullptr && ds.count[blockIdx.x % kDiffLists] <= ds.cap;
   __syncthreads();
   const bool r = s_listing != 0;
   __syncthreads();
@@ -1061,6 +1090,8 @@ __global__ __launch_bounds__(kBlock) void k_ingest_interleave(const int32_t* __r
   const int64_t g = blockIdx.x / tiles_per_group;
   const int64_t v0 = (blockIdx.x - g * tiles_per_group) * kTV;
   const bool listing = diff_listing(ds);
+  __shared__ u64 s_base[kTV];
+  if (listing) tile_base(ds, lb, ub, nvars, n_inst, v0, kTV, g == 0, s_base);
 #pragma unroll
   for (int r = 0; r < 4096 / kBlock; ++r) {
     const int e = r * kBlock + threadIdx.x;
@@ -1075,7 +1106,7 @@ __global__ __launch_bounds__(kBlock) void k_ingest_interleave(const int32_t* __r
       val = (u64)(uint32_t)l | ((u64)(uint32_t)u << 32);     // int2 {lb, ub}
       store[at] = val;
       if (l > u) empty[k] = 1;
-      if (listing && k > 0) dif = l != lb[v0 + j] || u != ub[v0 + j];   // instance 0's row
+      if (listing) dif = val != s_base[j];
     }
     if (listing) diff_append(ds, dif, k, v0 + j);
     tile[i][j] = val;
@@ -1135,22 +1166,24 @@ __global__ __launch_bounds__(kBlock) void k_ingest_interleave4(const v4i* __rest
   const int64_t g = blockIdx.x / tiles_per_group;
   const int64_t v0 = (blockIdx.x - g * tiles_per_group) * kTV;
   const bool listing = diff_listing(ds);
+  __shared__ u64 s_base[kTV];
+  if (listing)
+    tile_base(ds, reinterpret_cast<const int32_t*>(lb), reinterpret_cast<const int32_t*>(ub), nvars, n_inst,
+              v0, kTV, g == 0, s_base);
 #pragma unroll
   for (int r = 0; r < 1024 / kBlock; ++r) {
     const int e = r * kBlock + threadIdx.x;
     const int i = e / kQ, j = (e - i * kQ) * 4;
     const int64_t k = g * GI + i;
     u64 w[4] = {0, 0, 0, 0};
-    int dif = 0;                               // bit q: variable j + q differs from instance 0's
+    int dif = 0;                               // bit q: variable j + q differs from the base
     if (k < n_inst && v0 + j < nvars) {
       const int64_t at4 = (k * nvars + v0 + j) / 4;
       const v4i a = __builtin_nontemporal_load(lb + at4);
       const v4i b = __builtin_nontemporal_load(ub + at4);
-      if (listing && k > 0) {
-        const v4i ra = lb[(v0 + j) / 4], rb = ub[(v0 + j) / 4];
-        dif = ((a.x != ra.x) | (b.x != rb.x)) | (((a.y != ra.y) | (b.y != rb.y)) << 1) |
-              (((a.z != ra.z) | (b.z != rb.z)) << 2) | (((a.w != ra.w) | (b.w != rb.w)) << 3);
-      }
+      if (listing)
+        dif = (pack_bounds(a.x, b.x) != s_base[j]) | ((pack_bounds(a.y, b.y) != s_base[j + 1]) << 1) |
+              ((pack_bounds(a.z, b.z) != s_base[j + 2]) << 2) | ((pack_bounds(a.w, b.w) != s_base[j + 3]) << 3);
       const v4i p0 = {a.x, b.x, a.y, b.y};
       const v4i p1 = {a.z, b.z, a.w, b.w};
       store[2 * at4] = p0;
@@ -2747,25 +2780,34 @@ hipError_t launch_branch_seed(const EventSet& ev, const int32_t* var, int64_t n_
 // ---- first sweeps seeded from a base store --------------------------------
 // Instances loaded together usually share most of their values (search nodes
 // of one root).  A propagator whose three operands hold the same values in
-// instance k as in instance 0 computes the same thing in both; if it does
-// not tighten instance 0's store, it does not tighten k's.  So a batch's
-// first sweep needs only the propagators adjacent to a value that differs
-// from instance 0's, and those that instance 0's store does not satisfy.
+// instance k as in a base store computes the same thing in both; if it does
+// not tighten the base store, it does not tighten k's.  So a batch's first
+// sweep needs only the propagators adjacent to a value that differs from the
+// base, and those the base store does not satisfy.  The base holds, per
+// variable, the value two of instances 0-2 agree on: the root's value in a
+// batch of splits of one root.
 
-// Differences from instance 0 in stores already in device memory: one
-// workgroup per (instance, 2048-variable block).
+// The base store and the differences from it in stores already in device
+// memory: one workgroup per (instance, 2048-variable block); the workgroups
+// of instance 0 write the base.
 __global__ __launch_bounds__(kBlock) void k_list_diffs(const u64* __restrict__ store, int64_t nvars,
                                                        int64_t n_inst, int64_t vblocks, DiffSink ds) {
   constexpr int kPer = 8;
   const bool listing = diff_listing(ds);
   if (!listing) return;
-  const int64_t k = 1 + blockIdx.x / vblocks;
+  const int64_t k = blockIdx.x / vblocks;
   const int64_t v0 = (blockIdx.x % vblocks) * (kBlock * kPer);
-  if (k >= n_inst) return;
 #pragma unroll
   for (int q = 0; q < kPer; ++q) {
     const int64_t v = v0 + q * kBlock + threadIdx.x;
-    const bool d = v < nvars && store[k * nvars + v] != store[v];
+    bool d = false;
+    if (v < nvars) {
+      const u64 a = store[v];
+      const u64 r = base_value(a, n_inst > 1 ? store[nvars + v] : a, n_inst > 2 ? store[2 * nvars + v] : a,
+                               n_inst);
+      if (k == 0) reinterpret_cast<u64*>(ds.base)[v] = r;
+      d = store[k * nvars + v] != r;
+    }
     diff_append(ds, d, k, v);
   }
 }
@@ -2774,7 +2816,7 @@ hipError_t launch_list_diffs(const int2* store, int64_t nvars, int64_t n_inst, c
                              hipStream_t stream) {
   if (!ds.list || nvars <= 0 || n_inst < 2) return hipSuccess;
   const int64_t vblocks = (nvars + kBlock * 8 - 1) / (kBlock * 8);
-  const int64_t grid = (n_inst - 1) * vblocks;
+  const int64_t grid = n_inst * vblocks;
   if (grid > 0x7fffffff) return hipErrorInvalidValue;
   k_list_diffs<<<(unsigned)grid, kBlock, 0, stream>>>(reinterpret_cast<const u64*>(store), nvars, n_inst,
                                                       vblocks, ds);
@@ -2838,27 +2880,40 @@ __device__ __forceinline__ void seed_prop(const Marker& mk, const EventSet& ev, 
 }
 
 // Plane-0 marks, one block-wide step at a time (each step ends with a flush
-// of the staged queue entries): steps over the difference sub-lists take 64
-// slots, four threads walking a filled slot's adjacency; the steps after
-// them take kBlock (propagator of `unsat`, instance) pairs.
+// of the staged queue entries): steps over the difference sub-lists take up
+// to 64 listed entries of one sub-list, four threads walking an entry's
+// adjacency; the steps after them take kBlock (propagator of `unsat`,
+// instance) pairs.
 constexpr int kSeedSlotsPerStep = kBlock / 4;
 __global__ __launch_bounds__(kBlock) void k_seed_from_base(EventSet ev, DiffSink ds,
                                                            const int32_t* __restrict__ unsat,
                                                            int64_t n_unsat) {
+  static_assert(kDiffLists == kBlock, "one thread per difference sub-list");
   __shared__ unsigned long long s_qbuf[kQStage];
   __shared__ int32_t s_qn;
+  __shared__ int32_t s_scan[kBlock / kWave];
+  __shared__ int32_t s_first[kDiffLists + 1];           // first step of each sub-list
   const QueueStage qs{s_qbuf, &s_qn};
   if (threadIdx.x == 0) s_qn = 0;
+  const int own = (int)min<int64_t>(ds.count[threadIdx.x], ds.cap);
+  int diff_steps;
+  const int first = block_scan<kBlock>((own + kSeedSlotsPerStep - 1) / kSeedSlotsPerStep, s_scan, &diff_steps);
+  s_first[threadIdx.x] = first;
+  if (threadIdx.x == 0) s_first[kDiffLists] = diff_steps;
   __syncthreads();
-  const int64_t n_slots = kDiffLists * ds.cap;
-  const int64_t diff_steps = (n_slots + kSeedSlotsPerStep - 1) / kSeedSlotsPerStep;
   const int64_t n_pairs = n_unsat * ev.n_inst;
   const int64_t steps = diff_steps + (n_pairs + kBlock - 1) / kBlock;
   for (int64_t it = blockIdx.x; it < steps; it += gridDim.x) {
     if (it < diff_steps) {
-      const int64_t slot = it * kSeedSlotsPerStep + threadIdx.x / 4;
-      if (slot < n_slots && slot % ds.cap < ds.count[slot / ds.cap]) {
-        const u64 d = ds.list[slot];
+      int lo = 0, hi = kDiffLists;                          // the sub-list c with s_first[c] <= it < s_first[c+1]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (s_first[mid] <= it) lo = mid; else hi = mid;
+      }
+      const int c = lo;
+      const int64_t at = (it - s_first[c]) * kSeedSlotsPerStep + threadIdx.x / 4;
+      if (at < min<int64_t>(ds.count[c], ds.cap)) {
+        const u64 d = ds.list[c * ds.cap + at];
         const int64_t k = (int64_t)(d >> 32);
         const int32_t v = (int32_t)(uint32_t)d;
         const Marker mk = marker_for(&ev, k, -1);          // sweep -1 marks plane 0 (sweep 0's)
@@ -2879,11 +2934,11 @@ __global__ __launch_bounds__(kBlock) void k_seed_from_base(EventSet ev, DiffSink
   }
 }
 
-hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, const int32_t* unsat,
-                                 int64_t n_unsat, hipStream_t stream) {
-  const int64_t steps = (kDiffLists * diffs.cap + kSeedSlotsPerStep - 1) / kSeedSlotsPerStep +
-                        (n_unsat * ev.n_inst + kBlock - 1) / kBlock;
-  if (steps <= 0 || !ev.dirty) return hipSuccess;
+hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, int64_t n_diffs,
+                                 const int32_t* unsat, int64_t n_unsat, hipStream_t stream) {
+  // steps: at most one partial step per sub-list beyond the full ones
+  const int64_t steps = n_diffs / kSeedSlotsPerStep + kDiffLists + (n_unsat * ev.n_inst + kBlock - 1) / kBlock;
+  if (!ev.dirty || (n_diffs == 0 && n_unsat == 0)) return hipSuccess;
   const int64_t grid = steps > 4096 ? 4096 : steps;
   k_seed_from_base<<<(unsigned)grid, kBlock, 0, stream>>>(ev, diffs, unsat, n_unsat);
   return hipGetLastError();

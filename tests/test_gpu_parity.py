@@ -823,30 +823,33 @@ def test_first_sweep_seeded_from_instance_zero(gpu, gi, load, subsets):
         d.close()
 
 
-@pytest.mark.parametrize("case", ["widened", "empty", "unpropagated"])
-def test_seeded_first_sweep_when_instance_zero_is_not_a_fixpoint(gpu, case):
-    """Instance 0 itself need not be a fixpoint: with one variable widened
-    back to its told bounds, the propagators its store does not satisfy are
-    seeded in every instance; with an empty interval in instance 0, the
-    propagators reading it are; an unpropagated instance 0 leaves too many
-    such propagators, and the run takes a full first sweep.  Every instance
-    equals the oracle."""
+@pytest.mark.parametrize("case", ["widened", "empty", "base-unsatisfied", "unpropagated"])
+def test_seeded_first_sweep_around_the_base_store(gpu, case):
+    """The base store holds per variable the value two of instances 0-2
+    agree on.  One instance with a variable widened back to its told bounds,
+    or holding an empty interval, differs from it there; with that variable
+    widened in instances 0-2 the base itself is no fixpoint, and the
+    propagators it does not satisfy are seeded in every instance; with every
+    instance unpropagated the base leaves too many of them and the run takes
+    a full first sweep.  Every instance equals the oracle."""
     V, P, n_inst = 4000, 16000, 120
     pir, lb, ub, rl, ru, slb, sub, off, idx = _seed_case(91, V, P, n_inst, True)
-    top, tx, ty, tz = pir.table()
+    v = int(np.nonzero((lb != rl) | (ub != ru))[0][0])         # a variable the root tightened
     if case == "widened":
-        v = int(tx[0])
         slb[v], sub[v] = lb[v], ub[v]
     elif case == "empty":
         slb[33], sub[33] = 5, 4
+    elif case == "base-unsatisfied":
+        for k in range(3):
+            slb[k * V + v], sub[k * V + v] = lb[v], ub[v]
     else:
-        slb[:V], sub[:V] = lb, ub
+        slb, sub = pcx.gen_splits(92, lb, ub, n_inst, 1, 20)
     lo_o, hi_o, res_o = _oracle_batch(pir, slb, sub, V, off, idx)
     b = pcx.Batch(pir, n_inst, V, off, idx)
     b.load(slb, sub)
     res_g, st = b.run(pcx.MODE_BATCHED)
     lb_g, ub_g = b.read()
-    _compare(lb_g, ub_g, res_g, lo_o, hi_o, res_o, V, f"instance 0 {case}")
+    _compare(lb_g, ub_g, res_g, lo_o, hi_o, res_o, V, f"base store, {case}")
     if case == "unpropagated":
         assert st.full_evals > 0
     else:
