@@ -446,7 +446,8 @@ pcx_status prepare_run(pcx_batch* b, int32_t mode) {
 
 // Queue planes of a batch (queued marked sweeps), allocated on first use:
 // room for a sixteenth of the batch's (instance, propagator) pairs per plane,
-// between 2^16 and 2^23 entries; a plane that overflows falls back to bitmaps.
+// between 2^16 and 2^23 entries, spread over kQLists sub-lists; a plane that
+// overflows (one sub-list of it) falls back to bitmaps.
 pcx_status ensure_queue(pcx_batch* b) {
   const int mq = b->e->cfg.mark_queue;
   const int64_t pairs = b->n_inst * std::max<int64_t>(b->table_n, 1);
@@ -455,11 +456,12 @@ pcx_status ensure_queue(pcx_batch* b) {
   const int64_t want = std::max<int64_t>(pairs / 16, std::min<int64_t>(pairs, b->table_n));
   const int64_t cap = mq > 0 ? (int64_t)1 << mq
                              : std::min<int64_t>(std::max<int64_t>(want, (int64_t)1 << 16), (int64_t)1 << 23);
-  if (b->d_q && b->qcap == cap) return PCX_OK;
+  const int64_t per_list = (cap + kQLists - 1) / kQLists;
+  if (b->d_q && b->qcap == per_list) return PCX_OK;
   dfree(b->d_q);
   dfree(b->d_qctl);
-  b->qcap = cap;
-  PCX_HIP(hipMalloc(&b->d_q, 2 * b->qcap * sizeof(unsigned long long)));
+  b->qcap = per_list;
+  PCX_HIP(hipMalloc(&b->d_q, 2 * kQLists * b->qcap * sizeof(unsigned long long)));
   PCX_HIP(hipMalloc(&b->d_qctl, kQCtlWords * sizeof(int32_t)));
   return PCX_OK;
 }
@@ -547,7 +549,7 @@ pcx_status run_plain(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
         PCX_HIP(hipGraphLaunch(b->plain_queue_graph, e->stream));
         ts = tm.end(e->stream, false, group);
         if (ts != PCX_OK) return ts;
-        int32_t q[kQCtlWords];
+        int32_t q[kQHeadWords];
         ts = read_small(b, b->d_qctl, sizeof(q), q);
         if (ts != PCX_OK) return ts;
         ts = tm.drain();
@@ -723,7 +725,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     if (base_seeded) PCX_HIP(hipMemsetAsync(b->d_qctl + kQSweep, 0xff, sizeof(int32_t), e->stream));
   }
   if (base_seeded)
-    PCX_HIP(launch_seed_from_base(ev, DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base}, n_diff,
+    PCX_HIP(launch_seed_from_base(ev, DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val}, n_diff,
                                   b->d_unsat, n_unsat, e->stream));
   auto launch_marked = [&]() -> pcx_status {
     PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
@@ -811,7 +813,7 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       PCX_HIP(hipGraphLaunch(b->queue_graph, e->stream));
       ts = tm.end(e->stream, false, group);
       if (ts != PCX_OK) return ts;
-      int32_t q[kQCtlWords];
+      int32_t q[kQHeadWords];
       ts = read_small(b, b->d_qctl, sizeof(q), q);
       if (ts != PCX_OK) return ts;
       ts = tm.drain();
@@ -1154,7 +1156,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   if (b->queue_graph) (void)hipGraphExecDestroy(b->queue_graph);
   if (b->plain_queue_graph) (void)hipGraphExecDestroy(b->plain_queue_graph);
   dfree(b->d_q); dfree(b->d_qctl);
-  dfree(b->d_diff); dfree(b->d_unsat); dfree(b->d_seedctl); dfree(b->d_base);
+  dfree(b->d_diff); dfree(b->d_diff_val); dfree(b->d_unsat); dfree(b->d_seedctl); dfree(b->d_base);
   delete b;
 }
 
@@ -1170,8 +1172,10 @@ pcx_status ensure_diff_list(pcx_batch* b) {
   if (!b->d_base) PCX_HIP(hipMalloc(&b->d_base, b->nvars * sizeof(int2)));
   if (b->diff_cap < per_list) {
     dfree(b->d_diff);
+    dfree(b->d_diff_val);
     b->diff_cap = 0;
     PCX_HIP(hipMalloc(&b->d_diff, per_list * kDiffLists * sizeof(unsigned long long)));
+    PCX_HIP(hipMalloc(&b->d_diff_val, per_list * kDiffLists * sizeof(int2)));
     b->diff_cap = per_list;
   }
   return PCX_OK;
@@ -1181,7 +1185,7 @@ pcx_status ensure_diff_list(pcx_batch* b) {
 // several instances with seeding on), after clearing the count; an empty
 // sink otherwise.
 pcx_status diff_sink(pcx_batch* b, DiffSink* ds) {
-  *ds = DiffSink{nullptr, nullptr, 0};
+  *ds = DiffSink{nullptr, nullptr, 0, nullptr, nullptr};
   b->diffs_listed = false;
   const pcx_engine* e = b->e;
   if (e->cfg.base_seed == 0 || e->cfg.domain != PCX_DOMAIN_INTERVAL || b->n_inst < 2 || b->nvars <= 0)
@@ -1189,7 +1193,7 @@ pcx_status diff_sink(pcx_batch* b, DiffSink* ds) {
   pcx_status st = ensure_diff_list(b);
   if (st != PCX_OK) return st;
   PCX_HIP(hipMemsetAsync(b->d_seedctl, 0, kSeedCtlWords * sizeof(int32_t), e->stream));
-  *ds = DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base};
+  *ds = DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val};
   return PCX_OK;
 }
 
@@ -1204,6 +1208,20 @@ pcx_status image_loaded(pcx_batch* b) {
   b->store_is_image = true;
   b->shadow_is_image = false;
   b->image_in_shadow = false;
+  b->image_in_diffs = false;
+  return PCX_OK;
+}
+
+// Whether every difference sub-list of the last load is complete (reads the
+// counts; the stream is synchronized).
+pcx_status diffs_complete(pcx_batch* b, bool* complete) {
+  pcx_status st = ensure_pinned(b);
+  if (st != PCX_OK) return st;
+  int32_t cnt[kDiffLists];
+  st = read_small(b, b->d_seedctl, sizeof(cnt), cnt);
+  if (st != PCX_OK) return st;
+  *complete = true;
+  for (int c = 0; c < kDiffLists; ++c) *complete = *complete && cnt[c] <= b->diff_cap;
   return PCX_OK;
 }
 }  // namespace
@@ -1242,7 +1260,35 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
   pcx_status dst = diff_sink(b, &ds);
   if (dst != PCX_OK) return dst;
   const int gi = lockstep_of(b);
-  if (gi > 0) {
+  b->image_in_diffs = false;
+  if (ds.list && b->shared_loads) {
+    // Stores that share most values (the last load's did): write the stores
+    // only.  A complete difference list (base store + listed values) is then
+    // the image; otherwise the image is copied from the stores now, as the
+    // interleaved copy a lockstep full sweep reads when the batch has one,
+    // and later loads write it in the ingest pass again.
+    PCX_HIP(launch_ingest_rows(d_lb, d_ub, b->d_store, b->nvars, b->n_inst, b->d_image_empty, ds, e->stream));
+    bool complete = false;
+    pcx_status st = diffs_complete(b, &complete);
+    if (st != PCX_OK) return st;
+    b->shadow_is_image = false;
+    b->image_in_shadow = false;
+    if (complete) {
+      b->image_in_diffs = true;
+    } else {
+      b->shared_loads = false;
+      if (gi > 0) {
+        st = ensure_shadow(b, gi);
+        if (st != PCX_OK) return st;
+        PCX_HIP(launch_interleave(b->d_store, b->d_shadow, b->nvars, b->n_inst, gi, e->stream));
+        b->shadow_is_image = true;
+        b->shadow_gi = gi;
+        b->image_in_shadow = true;
+      } else {
+        PCX_HIP(launch_copy_words(b->d_store, b->d_image, n, e->stream));
+      }
+    }
+  } else if (gi > 0) {
     // Batched runs sweep a lockstep copy first: write it in the same pass
     // as the stores.  That copy is then also the reset image (d_image is
     // written from it only if a run is about to overwrite it).
@@ -1261,6 +1307,14 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
     b->image_in_shadow = false;
   }
   PCX_HIP(hipStreamSynchronize(e->stream));
+  if (ds.list && !b->shared_loads) {
+    // a load of the other kind whose stores share most values: the next load
+    // writes the stores only
+    bool complete = false;
+    pcx_status st = diffs_complete(b, &complete);
+    if (st != PCX_OK) return st;
+    b->shared_loads = complete;
+  }
   b->store_is_image = true;
   b->diffs_listed = ds.list != nullptr;
   return PCX_OK;
@@ -1269,7 +1323,10 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
 pcx_status pcx_batch_reset(pcx_batch* b) {
   if (!b) return fail_with(PCX_EINVAL, "NULL argument");
   pcx_engine* e = b->e;
-  if (b->image_in_shadow)
+  if (b->image_in_diffs)
+    PCX_HIP(launch_image_from_diffs(DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val},
+                                    b->d_store, b->nvars, b->n_inst, e->stream));
+  else if (b->image_in_shadow)
     PCX_HIP(launch_deinterleave(b->d_shadow, b->d_store, b->nvars, b->n_inst, b->shadow_gi, e->stream));
   else
     PCX_HIP(launch_copy_words(b->d_image, b->d_store, b->n_inst * b->nvars, e->stream));

@@ -72,6 +72,9 @@ struct pcx_batch {
   // satisfy (found by the run)
   int2* d_base = nullptr;             // [nvars]
   unsigned long long* d_diff = nullptr;   // kDiffLists sub-lists of diff_cap entries
+  int2* d_diff_val = nullptr;         // the listed values (device loads)
+  bool image_in_diffs = false;        // the image is the base store + the listed values
+  bool shared_loads = true;           // the last device load's stores shared most values
   int64_t diff_cap = 0;
   int32_t* d_unsat = nullptr;         // [table_n]
   int32_t* d_seedctl = nullptr;       // differences per sub-list, then propagators in d_unsat
@@ -90,8 +93,8 @@ struct pcx_batch {
   };
   hipGraphExec_t marked_graph = nullptr;
   MarkedGraphArgs marked_graph_args{};
-  // queued marked sweeps: two planes of qcap entries, control words, and the
-  // captured group of (turn, queue pass) launches
+  // queued marked sweeps: two planes of kQLists sub-lists of qcap entries,
+  // control words, and the captured group of (turn, queue pass) launches
   unsigned long long* d_q = nullptr;
   int32_t* d_qctl = nullptr;
   int64_t qcap = 0;

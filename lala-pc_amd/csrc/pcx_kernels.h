@@ -73,23 +73,29 @@ struct EventSet {
   int64_t words;
   int64_t swords;
   int64_t n_inst;
-  // Queued marks (batched runs after a lockstep full sweep; q == nullptr:
-  // none).  Every mark that sets a clear bit of sweep s's bitmap is also
-  // appended to queue plane s & 1 as (instance << 32 | propagator), so a
-  // marked sweep is one flat pass over its plane; the bitmaps stay complete,
-  // so a plane that overflowed (or an instance whose marks went to its
-  // all_marked word) is swept from the bitmaps instead.  qctl: kQ* words.
-  unsigned long long* q;    // [2 * qcap]
+  // Queued marks (q == nullptr: none).  Every mark that sets a clear bit of
+  // sweep s's bitmap is also appended to queue plane s & 1 as
+  // (instance << 32 | propagator), so a marked sweep is one flat pass over
+  // its plane; the bitmaps stay complete, so a plane that overflowed (or an
+  // instance whose marks went to its all_marked word) is swept from the
+  // bitmaps instead.  A plane is kQLists sub-lists of qcap entries; the
+  // workgroup index picks the sub-list a workgroup appends to, so each
+  // sub-list's tail counter is updated from one XCD only (one tail for the
+  // whole plane serialized the workgroups' reservations across XCDs).
+  // qctl: kQ* words.
+  unsigned long long* q;    // [2 * kQLists * qcap]
   int32_t* qctl;
-  int64_t qcap;
+  int64_t qcap;             // entries per sub-list
 };
+constexpr int kQLists = 256;
 // Queue control words.
-constexpr int kQTail = 0;       // [2] entries appended to each plane (may pass qcap)
 constexpr int kQOver = 2;       // [2] the plane lost marks: its sweep needs the bitmaps
 constexpr int kQSweep = 4;      // the sweep the next queue pass runs
 constexpr int kQStop = 5;       // 1: nothing marked (fixpoint), 2: bitmaps needed from kQAt
 constexpr int kQAt = 6;         // the sweep at which the queue passes stopped
-constexpr int kQCtlWords = 8;
+constexpr int kQHeadWords = 8;  // the words above (what the host reads)
+constexpr int kQTails = 8;      // [2][kQLists] entries appended to each sub-list (may pass qcap)
+constexpr int kQCtlWords = kQTails + 2 * kQLists;
 
 // Per-instance propagator subset: prop_idx[off[k] .. off[k+1]) of the shared
 // table, the whole table when off == nullptr, or -- off without idx -- the
@@ -206,6 +212,7 @@ struct DiffSink {
   int32_t* count;             // [kDiffLists]
   int64_t cap;                // entries per sub-list
   int2* base;                 // [nvars]: the base store, written by the load
+  int2* vals;                 // [kDiffLists * cap]: each entry's value (nullptr: not kept)
 };
 // Device-resident lb / ub (n_inst * nvars each) into the stores and their
 // interleaved copy in one pass; empty[k] = 1 where instance k has lb > ub
@@ -214,6 +221,15 @@ struct DiffSink {
 hipError_t launch_ingest_interleave(const int32_t* lb, const int32_t* ub, int2* store, int2* shadow,
                                     int64_t nvars, int64_t n_inst, int gi, uint8_t* empty,
                                     const DiffSink& ds, hipStream_t st);
+// Device-resident lb / ub into the stores only (no image copy), with the
+// base store, the differences from it and their values into ds, and the
+// emptiness; a complete list is the image (launch_image_from_diffs).
+hipError_t launch_ingest_rows(const int32_t* lb, const int32_t* ub, int2* store, int64_t nvars,
+                              int64_t n_inst, uint8_t* empty, const DiffSink& ds, hipStream_t st);
+// store[k][v] = the base value, then every listed difference's value: the
+// stores a complete difference list (with values) was taken from.
+hipError_t launch_image_from_diffs(const DiffSink& ds, int2* store, int64_t nvars, int64_t n_inst,
+                                   hipStream_t st);
 // The same base and differences from stores already in device memory
 // (n_inst * nvars words, row by row).
 hipError_t launch_list_diffs(const int2* store, int64_t nvars, int64_t n_inst, const DiffSink& ds,

@@ -200,12 +200,20 @@ struct QueueStage {
   int32_t* n;                // LDS counter
 };
 
+// This workgroup's sub-list of plane pl: its tail counter and first entry.
+__device__ __forceinline__ int32_t* queue_tail(const EventSet& ev, int pl) {
+  return ev.qctl + kQTails + pl * kQLists + blockIdx.x % kQLists;
+}
+__device__ __forceinline__ unsigned long long* queue_list(const EventSet& ev, int pl) {
+  return ev.q + ((int64_t)pl * kQLists + blockIdx.x % kQLists) * ev.qcap;
+}
+
 __device__ __forceinline__ void stage_entry(const QueueStage& qs, const EventSet& ev, int pl,
                                             unsigned long long entry) {
   const int at = atomicAdd(qs.n, 1);
   if (at < kQStage) { qs.buf[at] = entry; return; }
-  const int64_t g = atomicAdd(ev.qctl + kQTail + pl, 1);
-  if (g < ev.qcap) ev.q[(int64_t)pl * ev.qcap + g] = entry;
+  const int64_t g = atomicAdd(queue_tail(ev, pl), 1);
+  if (g < ev.qcap) queue_list(ev, pl)[g] = entry;
   else ev.qctl[kQOver + pl] = 1;
 }
 
@@ -214,12 +222,12 @@ __device__ __forceinline__ void stage_entry(const QueueStage& qs, const EventSet
 __device__ __forceinline__ void flush_stage(const QueueStage& qs, const EventSet& ev, int pl) {
   __shared__ int32_t s_base;
   const int n = min(*qs.n, kQStage);
-  if (threadIdx.x == 0) s_base = n ? atomicAdd(ev.qctl + kQTail + pl, n) : 0;
+  if (threadIdx.x == 0) s_base = n ? atomicAdd(queue_tail(ev, pl), n) : 0;
   __syncthreads();
   const int64_t base = s_base;
-  unsigned long long* plane = ev.q + (int64_t)pl * ev.qcap;
+  unsigned long long* list = queue_list(ev, pl);
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    if (base + i < ev.qcap) plane[base + i] = qs.buf[i];
+    if (base + i < ev.qcap) list[base + i] = qs.buf[i];
     else ev.qctl[kQOver + pl] = 1;
   }
   __syncthreads();                     // every thread has read the count and the entries
@@ -977,7 +985,7 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
 // contiguous runs.
 // Appends (k << 32 | v) for the lanes whose d is set, with one counter
 // atomic per wave.  Called by every lane of the wave.
-__device__ __forceinline__ void diff_append(const DiffSink& ds, bool d, int64_t k, int64_t v) {
+__device__ __forceinline__ void diff_append(const DiffSink& ds, bool d, int64_t k, int64_t v, u64 val) {
   const u64 mask = __ballot(d);
   if (mask == 0) return;
   const int c = blockIdx.x % kDiffLists;
@@ -987,7 +995,10 @@ __device__ __forceinline__ void diff_append(const DiffSink& ds, bool d, int64_t 
   base = __builtin_amdgcn_readlane(base, leader);
   const int64_t at = (int64_t)base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-  if (d && at < ds.cap) ds.list[c * ds.cap + at] = ((u64)k << 32) | (uint32_t)v;
+  if (d && at < ds.cap) {
+    ds.list[c * ds.cap + at] = ((u64)k << 32) | (uint32_t)v;
+    if (ds.vals) reinterpret_cast<u64*>(ds.vals)[c * ds.cap + at] = val;
+  }
 }
 // The base value of one variable: what two of instances 0, 1 and 2 hold
 // (instance 0's when no two agree or the batch is smaller).
@@ -1108,7 +1119,7 @@ __global__ __launch_bounds__(kBlock) void k_ingest_interleave(const int32_t* __r
       if (l > u) empty[k] = 1;
       if (listing) dif = val != s_base[j];
     }
-    if (listing) diff_append(ds, dif, k, v0 + j);
+    if (listing) diff_append(ds, dif, k, v0 + j, val);
     tile[i][j] = val;
   }
   __syncthreads();
@@ -1196,7 +1207,7 @@ __global__ __launch_bounds__(kBlock) void k_ingest_interleave4(const v4i* __rest
     }
     if (listing && __ballot(dif != 0)) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) diff_append(ds, (dif >> q) & 1, k, v0 + j + q);
+      for (int q = 0; q < 4; ++q) diff_append(ds, (dif >> q) & 1, k, v0 + j + q, w[q]);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) tile[i][j + q] = w[q];
@@ -1591,23 +1602,26 @@ __device__ __forceinline__ int block_scan(int v, int32_t* s_scan, int* total) {
 // marked-sweep invariant is every instance's fixpoint; a plane that lost
 // marks stops the queue passes (the host resumes with bitmap sweeps); else
 // the turn clears the other plane for the sweep's own marks and advances.
-__global__ void k_queue_turn(int32_t* qctl, int32_t max_sweeps) {
-  if (threadIdx.x != 0 || qctl[kQStop]) return;
+// One workgroup, a thread per sub-list.
+__global__ __launch_bounds__(kQLists) void k_queue_turn(int32_t* qctl, int32_t max_sweeps) {
+  if (qctl[kQStop]) return;                       // uniform: every thread reads the same word
   const int32_t s = qctl[kQSweep] + 1;
   const int cu = s & 1;
-  if (qctl[kQOver + cu]) { qctl[kQStop] = 2; qctl[kQAt] = s; return; }
-  if (qctl[kQTail + cu] == 0 || (max_sweeps > 0 && s >= max_sweeps)) {
-    qctl[kQStop] = 1;
-    qctl[kQAt] = s;
+  const bool marked = __syncthreads_or(qctl[kQTails + cu * kQLists + threadIdx.x] != 0);
+  if (qctl[kQOver + cu]) {
+    if (threadIdx.x == 0) { qctl[kQStop] = 2; qctl[kQAt] = s; }
     return;
   }
-  qctl[kQTail + (cu ^ 1)] = 0;
-  qctl[kQOver + (cu ^ 1)] = 0;
-  qctl[kQSweep] = s;
+  if (!marked || (max_sweeps > 0 && s >= max_sweeps)) {
+    if (threadIdx.x == 0) { qctl[kQStop] = 1; qctl[kQAt] = s; }
+    return;
+  }
+  qctl[kQTails + (cu ^ 1) * kQLists + threadIdx.x] = 0;
+  if (threadIdx.x == 0) { qctl[kQOver + (cu ^ 1)] = 0; qctl[kQSweep] = s; }
 }
 
 hipError_t launch_queue_turn(int32_t* qctl, int32_t max_sweeps, hipStream_t stream) {
-  k_queue_turn<<<1, 64, 0, stream>>>(qctl, max_sweeps);
+  k_queue_turn<<<1, kQLists, 0, stream>>>(qctl, max_sweeps);
   return hipGetLastError();
 }
 
@@ -1620,25 +1634,37 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void k_sweep_queue(Table t, int2* __restrict__ store, int64_t nvars,
                                                       InstFlags* flags, unsigned long long* tight,
                                                       EventSet ev) {
+  static_assert(kQLists == kBlock, "one thread per queue sub-list");
   const int32_t* qctl = ev.qctl;
   if (qctl[kQStop]) return;
   const int32_t s = qctl[kQSweep];
   const int cu = s & 1;
-  const int64_t n = min((int64_t)qctl[kQTail + cu], ev.qcap);
-  const unsigned long long* plane = ev.q + (int64_t)cu * ev.qcap;
+  const unsigned long long* plane = ev.q + (int64_t)cu * kQLists * ev.qcap;
   uint32_t* cur = ev.dirty + (int64_t)cu * ev.n_inst * ev.words;
   __shared__ int32_t s_k[kQueueList], s_v[kQueueList];
   __shared__ int32_t s_n;
   __shared__ unsigned long long s_qbuf[kQStage];
   __shared__ int32_t s_qn;
+  __shared__ int32_t s_scan[kBlock / kWave];
+  __shared__ int32_t s_first[kQLists + 1];          // the plane's entries in sub-list order
   const QueueStage qs{s_qbuf, &s_qn};
   if (threadIdx.x == 0) { s_n = 0; s_qn = 0; }
+  int n32;
+  s_first[threadIdx.x] =
+      block_scan<kBlock>((int)min<int64_t>(qctl[kQTails + cu * kQLists + threadIdx.x], ev.qcap), s_scan, &n32);
+  if (threadIdx.x == 0) s_first[kQLists] = n32;
   __syncthreads();
+  const int64_t n = n32;
   int nt_all = 0;
   for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
     const int64_t i = base + threadIdx.x;
     if (i < n) {
-      const unsigned long long en = plane[i];
+      int lo = 0, hi = kQLists;                   // sub-list c with s_first[c] <= i < s_first[c + 1]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (s_first[mid] <= i) lo = mid; else hi = mid;
+      }
+      const unsigned long long en = plane[(int64_t)lo * ev.qcap + (i - s_first[lo])];
       const int32_t k = (int32_t)(en >> 32), p = (int32_t)(uint32_t)en;
       InstFlags* f = flags + k;
       if (f->failed_at == kNoFail) {
@@ -1715,7 +1741,7 @@ __global__ __launch_bounds__(kBlock) void k_queue_from_bitmap(EventSet ev, InstF
   uint32_t* row = ev.dirty + (int64_t)pl * ev.n_inst * ev.words + k * ev.words;
   const uint32_t* mrow = ev.member ? ev.member + k * ev.words : nullptr;
   const bool all = flags[k].all_marked[pl] == s;
-  unsigned long long* plane = ev.q + (int64_t)pl * ev.qcap;
+  unsigned long long* list = queue_list(ev, pl);
   __shared__ int32_t s_scan[kBlock / kWave];
   __shared__ int32_t s_base;
   for (int64_t w0 = (int64_t)blockIdx.x * kBlock; w0 < ev.words; w0 += (int64_t)gridDim.x * kBlock) {
@@ -1733,12 +1759,12 @@ __global__ __launch_bounds__(kBlock) void k_queue_from_bitmap(EventSet ev, InstF
     }
     int total;
     int off = block_scan<kBlock>(__builtin_popcount(bits), s_scan, &total);
-    if (threadIdx.x == 0) s_base = total ? atomicAdd(ev.qctl + kQTail + pl, total) : 0;
+    if (threadIdx.x == 0) s_base = total ? atomicAdd(queue_tail(ev, pl), total) : 0;
     __syncthreads();
     for (uint32_t b = bits; b; b &= b - 1, ++off) {
       const int64_t at = (int64_t)s_base + off;
       const unsigned long long en = ((unsigned long long)k << 32) | (uint32_t)(w * 32 + __builtin_ctz(b));
-      if (at < ev.qcap) plane[at] = en;
+      if (at < ev.qcap) list[at] = en;
       else ev.qctl[kQOver + pl] = 1;
     }
     __syncthreads();                               // s_base is read
@@ -2787,9 +2813,27 @@ hipError_t launch_branch_seed(const EventSet& ev, const int32_t* var, int64_t n_
 // variable, the value two of instances 0-2 agree on: the root's value in a
 // batch of splits of one root.
 
-// The base store and the differences from it in stores already in device
-// memory: one workgroup per (instance, 2048-variable block); the workgroups
-// of instance 0 write the base.
+// The base store (base_value per variable) from rows 0-2 of lb / ub, or of
+// packed stores (lb == nullptr: rows of `store`).
+__global__ __launch_bounds__(kBlock) void k_make_base(const int32_t* __restrict__ lb, const int32_t* __restrict__ ub,
+                                                      const u64* __restrict__ store, int64_t nvars,
+                                                      int64_t n_inst, u64* __restrict__ base) {
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < nvars; v += (int64_t)gridDim.x * kBlock) {
+    auto row = [&](int64_t k) { return lb ? pack_bounds(lb[k * nvars + v], ub[k * nvars + v]) : store[k * nvars + v]; };
+    const u64 a = row(0);
+    base[v] = base_value(a, n_inst > 1 ? row(1) : a, n_inst > 2 ? row(2) : a, n_inst);
+  }
+}
+hipError_t launch_make_base(const int32_t* lb, const int32_t* ub, const u64* store, int64_t nvars,
+                            int64_t n_inst, u64* base, hipStream_t stream) {
+  int64_t grid = (nvars + kBlock - 1) / kBlock;
+  if (grid > 8192) grid = 8192;
+  k_make_base<<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, store, nvars, n_inst, base);
+  return hipGetLastError();
+}
+
+// The differences from the base store (made first) in stores already in
+// device memory: one workgroup per (instance, 2048-variable block).
 __global__ __launch_bounds__(kBlock) void k_list_diffs(const u64* __restrict__ store, int64_t nvars,
                                                        int64_t n_inst, int64_t vblocks, DiffSink ds) {
   constexpr int kPer = 8;
@@ -2801,14 +2845,12 @@ __global__ __launch_bounds__(kBlock) void k_list_diffs(const u64* __restrict__ s
   for (int q = 0; q < kPer; ++q) {
     const int64_t v = v0 + q * kBlock + threadIdx.x;
     bool d = false;
+    u64 val = 0;
     if (v < nvars) {
-      const u64 a = store[v];
-      const u64 r = base_value(a, n_inst > 1 ? store[nvars + v] : a, n_inst > 2 ? store[2 * nvars + v] : a,
-                               n_inst);
-      if (k == 0) reinterpret_cast<u64*>(ds.base)[v] = r;
-      d = store[k * nvars + v] != r;
+      val = store[k * nvars + v];
+      d = val != reinterpret_cast<const u64*>(ds.base)[v];
     }
-    diff_append(ds, d, k, v);
+    diff_append(ds, d, k, v, val);
   }
 }
 
@@ -2818,8 +2860,117 @@ hipError_t launch_list_diffs(const int2* store, int64_t nvars, int64_t n_inst, c
   const int64_t vblocks = (nvars + kBlock * 8 - 1) / (kBlock * 8);
   const int64_t grid = n_inst * vblocks;
   if (grid > 0x7fffffff) return hipErrorInvalidValue;
+  const hipError_t e = launch_make_base(nullptr, nullptr, reinterpret_cast<const u64*>(store), nvars, n_inst,
+                                        reinterpret_cast<u64*>(ds.base), stream);
+  if (e != hipSuccess) return e;
   k_list_diffs<<<(unsigned)grid, kBlock, 0, stream>>>(reinterpret_cast<const u64*>(store), nvars, n_inst,
                                                       vblocks, ds);
+  return hipGetLastError();
+}
+
+// Rows of the stores straight from lb / ub (no image copy): one workgroup
+// per (instance, 2048-variable block), four variables per thread and step
+// (nvars % 4 == 0, 16-byte aligned) or one; the differences from the base
+// store (made first) with their values.
+template <bool V4>
+__global__ __launch_bounds__(kBlock) void k_ingest_rows(const int32_t* __restrict__ lb,
+                                                        const int32_t* __restrict__ ub, u64* __restrict__ store,
+                                                        int64_t nvars, int64_t n_inst, int64_t vblocks,
+                                                        uint8_t* __restrict__ empty, DiffSink ds) {
+  constexpr int kVars = kBlock * 8;
+  const bool listing = diff_listing(ds);
+  const int64_t k = blockIdx.x / vblocks;
+  const int64_t v0 = (blockIdx.x % vblocks) * kVars;
+  const int32_t* lk = lb + k * nvars;
+  const int32_t* uk = ub + k * nvars;
+  u64* sk = store + k * nvars;
+  bool bad = false;
+  constexpr int kPer = V4 ? 4 : 1;
+#pragma unroll
+  for (int q = 0; q < 8 / kPer; ++q) {
+    const int64_t v = v0 + (int64_t)(q * kBlock + threadIdx.x) * kPer;
+    u64 w[kPer];
+    bool in = v < nvars;
+    if (in) {
+      if constexpr (V4) {
+        const v4i a = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(lk + v));
+        const v4i b = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(uk + v));
+        w[0] = pack_bounds(a.x, b.x); w[1] = pack_bounds(a.y, b.y);
+        w[2] = pack_bounds(a.z, b.z); w[3] = pack_bounds(a.w, b.w);
+        const v4i p0 = {a.x, b.x, a.y, b.y}, p1 = {a.z, b.z, a.w, b.w};
+        reinterpret_cast<v4i*>(sk + v)[0] = p0;
+        reinterpret_cast<v4i*>(sk + v)[1] = p1;
+        bad |= (a.x > b.x) | (a.y > b.y) | (a.z > b.z) | (a.w > b.w);
+      } else {
+        const int32_t a = __builtin_nontemporal_load(lk + v), b = __builtin_nontemporal_load(uk + v);
+        w[0] = pack_bounds(a, b);
+        sk[v] = w[0];
+        bad |= a > b;
+      }
+    }
+    if (listing) {
+      const u64* base = reinterpret_cast<const u64*>(ds.base);
+      bool dif[kPer];
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) dif[j] = in && w[j] != base[v + j];
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) diff_append(ds, dif[j], k, v + j, in ? w[j] : 0);
+    }
+  }
+  if (bad) empty[k] = 1;
+}
+
+hipError_t launch_ingest_rows(const int32_t* lb, const int32_t* ub, int2* store, int64_t nvars,
+                              int64_t n_inst, uint8_t* empty, const DiffSink& ds, hipStream_t stream) {
+  if (nvars <= 0 || n_inst <= 0) return hipSuccess;
+  const int64_t vblocks = (nvars + kBlock * 8 - 1) / (kBlock * 8);
+  const int64_t grid = n_inst * vblocks;
+  if (grid > 0x7fffffff) return hipErrorInvalidValue;
+  const bool v4 = nvars % 4 == 0 &&
+                  ((reinterpret_cast<uintptr_t>(lb) | reinterpret_cast<uintptr_t>(ub) |
+                    reinterpret_cast<uintptr_t>(store)) & 15) == 0;
+  u64* st = reinterpret_cast<u64*>(store);
+  if (ds.list) {
+    const hipError_t e = launch_make_base(lb, ub, nullptr, nvars, n_inst, reinterpret_cast<u64*>(ds.base), stream);
+    if (e != hipSuccess) return e;
+  }
+  if (v4) k_ingest_rows<true><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, vblocks, empty, ds);
+  else k_ingest_rows<false><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, vblocks, empty, ds);
+  return hipGetLastError();
+}
+
+// The image from a complete difference list: every row the base, then the
+// listed values (a second launch, so that they land after the base).
+__global__ __launch_bounds__(kBlock) void k_fill_base(const u64* __restrict__ base, u64* __restrict__ store,
+                                                      int64_t nvars, int64_t vblocks) {
+  const int64_t k = blockIdx.x / vblocks;
+  const int64_t v0 = (blockIdx.x % vblocks) * (kBlock * 8);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int64_t v = v0 + q * kBlock + threadIdx.x;
+    if (v < nvars) store[k * nvars + v] = base[v];
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_scatter_diffs(DiffSink ds, u64* __restrict__ store, int64_t nvars) {
+  const int64_t n_slots = kDiffLists * ds.cap;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n_slots; i += (int64_t)gridDim.x * kBlock) {
+    if (i % ds.cap >= ds.count[i / ds.cap]) continue;
+    const u64 d = ds.list[i];
+    store[(int64_t)(d >> 32) * nvars + (uint32_t)d] = reinterpret_cast<const u64*>(ds.vals)[i];
+  }
+}
+
+hipError_t launch_image_from_diffs(const DiffSink& ds, int2* store, int64_t nvars, int64_t n_inst,
+                                   hipStream_t stream) {
+  if (nvars <= 0 || n_inst <= 0) return hipSuccess;
+  const int64_t vblocks = (nvars + kBlock * 8 - 1) / (kBlock * 8);
+  const int64_t grid = n_inst * vblocks;
+  if (grid > 0x7fffffff || !ds.vals) return hipErrorInvalidValue;
+  u64* st = reinterpret_cast<u64*>(store);
+  k_fill_base<<<(unsigned)grid, kBlock, 0, stream>>>(reinterpret_cast<const u64*>(ds.base), st, nvars, vblocks);
+  int64_t g2 = (kDiffLists * ds.cap + kBlock - 1) / kBlock;
+  if (g2 > 8192) g2 = 8192;
+  k_scatter_diffs<<<(unsigned)g2, kBlock, 0, stream>>>(ds, st, nvars);
   return hipGetLastError();
 }
 

@@ -277,3 +277,53 @@ def test_device_load_into_the_lockstep_copy(gpu, gi, events, V):
     same(b, "after a host load")
     d_lo.close()
     d_hi.close()
+
+
+@pytest.mark.parametrize("gi", [0, 64])
+def test_device_loads_switch_between_image_forms(gpu, gi):
+    """A device load whose stores share most values writes the stores only
+    (its reset image is the base store plus the listed differences); one
+    whose stores do not overflows the list and copies the image from the
+    stores, and the loads after it write the image in the ingest pass until
+    a load of shared stores again.  Along a sequence of unshared and shared
+    loads, every run equals the same stores loaded from the host, and every
+    reset gives back the loaded stores."""
+    from pcx.device import DeviceArray
+    V, P, n = 700, 2800, 131
+    op, x, y, z, lb, ub, _ = pcx.gen_network(pcx.gen_params(seed=7 + gi, nvars=V, nprops=P))
+    pir = pcx.PIR(lockstep=gi)
+    pir.tell(op, x, y, z, lb, ub)
+    root_l, root_u = lb.copy(), ub.copy()
+    assert not pir.fixpoint(root_l, root_u).failed
+    shared = pcx.gen_splits(gi + 3, root_l, root_u, n, 1, 20)
+    rng = np.random.default_rng(gi)
+    a = rng.integers(np.repeat(lb, n).reshape(V, n).T.ravel(), np.repeat(ub, n).reshape(V, n).T.ravel() + 1)
+    b_ = rng.integers(np.repeat(lb, n).reshape(V, n).T.ravel(), np.repeat(ub, n).reshape(V, n).T.ravel() + 1)
+    unshared = (np.minimum(a, b_).astype(np.int32), np.maximum(a, b_).astype(np.int32))
+
+    def reference(lo, hi):
+        r = pcx.Batch(pir, n, V)
+        r.load(lo, hi)
+        res, _ = r.run(pcx.MODE_BATCHED)
+        lg, ug = r.read()
+        return np.array([bool(x.failed) for x in res]), lg, ug
+
+    refs = {"shared": reference(*shared), "unshared": reference(*unshared)}
+    data = {"shared": shared, "unshared": unshared}
+    batch = pcx.Batch(pir, n, V)
+    for kind in ["unshared", "unshared", "shared", "shared", "unshared", "shared"]:
+        lo, hi = data[kind]
+        d_lo, d_hi = DeviceArray(lo), DeviceArray(hi)
+        batch.load_device(d_lo.ptr, d_hi.ptr)
+        d_lo.close()
+        d_hi.close()
+        for rep in range(2):                     # the run, then the run after a reset
+            res, _ = batch.run(pcx.MODE_BATCHED)
+            lg, ug = batch.read()
+            fail, l_ref, u_ref = refs[kind]
+            assert np.array_equal(np.array([bool(x.failed) for x in res]), fail), (kind, rep)
+            ok = np.repeat(~fail, V)
+            assert np.array_equal(lg[ok], l_ref[ok]) and np.array_equal(ug[ok], u_ref[ok]), (kind, rep)
+            batch.reset()
+            l0, u0 = batch.read()
+            assert np.array_equal(l0, lo) and np.array_equal(u0, hi), ("reset", kind, rep)
