@@ -710,10 +710,13 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   int32_t live = 1, sweeps = 0;
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
   EventSet ev = event_set(b);
-  // Queued marked sweeps follow a lockstep full sweep, or start at sweep 0
-  // from the seeds of a base store (seeds of a branching keep the bitmaps).
+  // A run seeded from a base store drains each instance's marks in one
+  // launch (one workgroup per instance) when it runs to the fixpoint; with a
+  // sweep limit it sweeps from queue planes.  Queued marked sweeps also
+  // follow a lockstep full sweep (seeds of a branching keep the bitmaps).
+  const bool drain = base_seeded && e->cfg.max_sweeps <= 0;
   const bool qmode = b->events && e->cfg.mark_queue != 0 &&
-                     (base_seeded || (gi > 0 && !b->seeded_run));
+                     ((base_seeded && !drain) || (gi > 0 && !b->seeded_run));
   if (qmode) {
     pcx_status qs = ensure_queue(b);
     if (qs != PCX_OK) return qs;
@@ -727,6 +730,21 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   if (base_seeded)
     PCX_HIP(launch_seed_from_base(ev, DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val}, n_diff,
                                   b->d_unsat, n_unsat, e->stream));
+  if (drain) {
+    PCX_HIP(hipMemsetAsync(b->d_ctl + 3, 0, sizeof(int32_t), e->stream));
+    const int dgrid = (int)std::min<int64_t>(b->n_inst, (int64_t)pcx_internal::device_cus(e->cfg.device) * 16);
+    pcx_status ts = tm.begin(e->stream);
+    if (ts != PCX_OK) return ts;
+    PCX_HIP(launch_drain_marks(table_of(e), b->d_store, b->nvars, b->d_flags, ev, b->d_ctl + 3,
+                               b->d_tight + kTightSlots, dom_of(e), dgrid, e->stream));
+    ts = tm.end(e->stream, false);
+    if (ts != PCX_OK) return ts;
+    PCX_HIP(hipStreamSynchronize(e->stream));
+    ts = tm.drain();
+    if (ts != PCX_OK) return ts;
+    std::vector<InstFlags> flags;
+    return collect(b, flags, results, stats, tm);
+  }
   auto launch_marked = [&]() -> pcx_status {
     PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
                             &ev));

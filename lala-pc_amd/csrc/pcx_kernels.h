@@ -245,6 +245,13 @@ hipError_t launch_base_probe(const Table& t, const int2* base, int32_t* list, in
 // in every instance whose subset holds it.
 hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, int64_t n_diffs,
                                  const int32_t* unsat, int64_t n_unsat, hipStream_t st);
+// The marked phase of a seeded run (marks in plane 0), one instance per
+// one-wave workgroup taken from *next_inst (zeroed by the caller): every
+// instance to its fixpoint in rounds separated by workgroup barriers, in
+// one launch; flags record the rounds as sweeps.  Interval domains.
+hipError_t launch_drain_marks(const Table& t, int2* store, int64_t nvars, InstFlags* flags, const EventSet& ev,
+                              int32_t* next_inst, unsigned long long* tight, int dom, int grid,
+                              hipStream_t st);
 // dst (n_inst * nvars words, row by row) from an interleaved copy
 hipError_t launch_deinterleave(const int2* shadow, int2* dst, int64_t nvars, int64_t n_inst, int gi,
                                hipStream_t st);

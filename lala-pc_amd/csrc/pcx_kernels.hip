@@ -2868,74 +2868,85 @@ hipError_t launch_list_diffs(const int2* store, int64_t nvars, int64_t n_inst, c
   return hipGetLastError();
 }
 
-// Rows of the stores straight from lb / ub (no image copy): one workgroup
-// per (instance, 2048-variable block), four variables per thread and step
-// (nvars % 4 == 0, 16-byte aligned) or one; the differences from the base
-// store (made first) with their values.
+// Rows of the stores straight from lb / ub (no image copy), streamed like a
+// copy: a grid-stride loop over groups of four values (nvars % 4 == 0,
+// 16-byte aligned) or single values (a grid as large as the data measured
+// fastest: 0.57 ms for 4,096 x 50,000 values against 0.67 ms with 4,096
+// workgroups); each thread carries its (instance, variable) position from
+// step to step.  The differences from the base store (made first) are listed
+// with their values; a list that overflowed is not worth a check per
+// workgroup here (the load then copies the image and later loads take the
+// fused ingest).
 template <bool V4>
 __global__ __launch_bounds__(kBlock) void k_ingest_rows(const int32_t* __restrict__ lb,
                                                         const int32_t* __restrict__ ub, u64* __restrict__ store,
-                                                        int64_t nvars, int64_t n_inst, int64_t vblocks,
+                                                        int64_t nvars, int64_t n_inst,
                                                         uint8_t* __restrict__ empty, DiffSink ds) {
-  constexpr int kVars = kBlock * 8;
-  const bool listing = diff_listing(ds);
-  const int64_t k = blockIdx.x / vblocks;
-  const int64_t v0 = (blockIdx.x % vblocks) * kVars;
-  const int32_t* lk = lb + k * nvars;
-  const int32_t* uk = ub + k * nvars;
-  u64* sk = store + k * nvars;
-  bool bad = false;
   constexpr int kPer = V4 ? 4 : 1;
-#pragma unroll
-  for (int q = 0; q < 8 / kPer; ++q) {
-    const int64_t v = v0 + (int64_t)(q * kBlock + threadIdx.x) * kPer;
+  const bool listing = ds.list != nullptr;
+  const u64* base = reinterpret_cast<const u64*>(ds.base);
+  const int64_t n = n_inst * nvars;
+  const int64_t stride = (int64_t)gridDim.x * kBlock * kPer;
+  const int64_t dk = stride / nvars, dv = stride % nvars;
+  int64_t i = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * kPer;
+  int64_t k = i / nvars, v = i % nvars;
+  for (int64_t i0 = (int64_t)blockIdx.x * kBlock * kPer; i0 < n; i0 += stride) {
+    const bool in = i < n;
     u64 w[kPer];
-    bool in = v < nvars;
+    bool bad = false;
     if (in) {
       if constexpr (V4) {
-        const v4i a = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(lk + v));
-        const v4i b = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(uk + v));
+        const v4i a = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(lb + i));
+        const v4i b = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(ub + i));
         w[0] = pack_bounds(a.x, b.x); w[1] = pack_bounds(a.y, b.y);
         w[2] = pack_bounds(a.z, b.z); w[3] = pack_bounds(a.w, b.w);
         const v4i p0 = {a.x, b.x, a.y, b.y}, p1 = {a.z, b.z, a.w, b.w};
-        reinterpret_cast<v4i*>(sk + v)[0] = p0;
-        reinterpret_cast<v4i*>(sk + v)[1] = p1;
-        bad |= (a.x > b.x) | (a.y > b.y) | (a.z > b.z) | (a.w > b.w);
+        reinterpret_cast<v4i*>(store + i)[0] = p0;
+        reinterpret_cast<v4i*>(store + i)[1] = p1;
+        bad = (a.x > b.x) | (a.y > b.y) | (a.z > b.z) | (a.w > b.w);
       } else {
-        const int32_t a = __builtin_nontemporal_load(lk + v), b = __builtin_nontemporal_load(uk + v);
+        const int32_t a = __builtin_nontemporal_load(lb + i), b = __builtin_nontemporal_load(ub + i);
         w[0] = pack_bounds(a, b);
-        sk[v] = w[0];
-        bad |= a > b;
+        store[i] = w[0];
+        bad = a > b;
       }
+      if (bad) empty[k] = 1;
     }
     if (listing) {
-      const u64* base = reinterpret_cast<const u64*>(ds.base);
-      bool dif[kPer];
+      bool dif[kPer], any = false;
 #pragma unroll
-      for (int j = 0; j < kPer; ++j) dif[j] = in && w[j] != base[v + j];
+      for (int j = 0; j < kPer; ++j) {
+        dif[j] = in && w[j] != base[v + j];
+        any |= dif[j];
+      }
+      if (__ballot(any)) {
 #pragma unroll
-      for (int j = 0; j < kPer; ++j) diff_append(ds, dif[j], k, v + j, in ? w[j] : 0);
+        for (int j = 0; j < kPer; ++j) diff_append(ds, dif[j], k, v + j, in ? w[j] : 0);
+      }
     }
+    i += stride;
+    k += dk;
+    v += dv;
+    if (v >= nvars) { v -= nvars; ++k; }
   }
-  if (bad) empty[k] = 1;
 }
 
 hipError_t launch_ingest_rows(const int32_t* lb, const int32_t* ub, int2* store, int64_t nvars,
                               int64_t n_inst, uint8_t* empty, const DiffSink& ds, hipStream_t stream) {
   if (nvars <= 0 || n_inst <= 0) return hipSuccess;
-  const int64_t vblocks = (nvars + kBlock * 8 - 1) / (kBlock * 8);
-  const int64_t grid = n_inst * vblocks;
-  if (grid > 0x7fffffff) return hipErrorInvalidValue;
   const bool v4 = nvars % 4 == 0 &&
                   ((reinterpret_cast<uintptr_t>(lb) | reinterpret_cast<uintptr_t>(ub) |
                     reinterpret_cast<uintptr_t>(store)) & 15) == 0;
+  const int64_t n = n_inst * nvars;
+  int64_t grid = (n / (v4 ? 4 : 1) + kBlock - 1) / kBlock;
+  if (grid > (1 << 20)) grid = 1 << 20;
   u64* st = reinterpret_cast<u64*>(store);
   if (ds.list) {
     const hipError_t e = launch_make_base(lb, ub, nullptr, nvars, n_inst, reinterpret_cast<u64*>(ds.base), stream);
     if (e != hipSuccess) return e;
   }
-  if (v4) k_ingest_rows<true><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, vblocks, empty, ds);
-  else k_ingest_rows<false><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, vblocks, empty, ds);
+  if (v4) k_ingest_rows<true><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds);
+  else k_ingest_rows<false><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds);
   return hipGetLastError();
 }
 
@@ -3083,6 +3094,213 @@ __global__ __launch_bounds__(kBlock) void k_seed_from_base(EventSet ev, DiffSink
       flush_stage(qs, ev, 0);
     }
   }
+}
+
+// The marked phase of a seeded run, one instance per workgroup of one wave:
+// the workgroup owns the instance (its store, its bitmap row of plane 0 and
+// its flags), so instead of device-wide sweeps it drains the instance's
+// marks in rounds separated by workgroup barriers, and the whole phase is
+// one launch.  Round r: (A) the round's propagators are taken and their
+// bits cleared -- from the LDS list the last round filled, or, when that
+// list overflowed (and in round 0, whose marks are the seeds), by scanning
+// the bitmap row for up to kWorkCap of them; (B) they are evaluated, and a
+// tightening sets the bits of the member propagators adjacent to the
+// variable, listing each bit it sets for round r + 1.  A propagator whose
+// operand another lane tightens in phase B has its bit clear (phase A) and
+// is listed again, so the barrier-separated rounds keep the marked-sweep
+// invariant: at the end of round r every propagator not marked is at its
+// fixpoint on the store.  Workgroup-scope atomics and loads, as in the
+// persistent sweeps (one workgroup per instance).
+constexpr int kWorkCap = 1024;
+constexpr int kDrainVars = 1024;          // tightened variables listed per round
+
+// Marks of the member propagators adjacent to v in an instance's bitmap row,
+// eight adjacency entries in flight at a time; every bit this sets is
+// listed for the next round (or, when the list is full, left for a scan).
+__device__ __forceinline__ void drain_mark(uint32_t* row, const uint32_t* mrow, const EventSet& ev, int32_t v,
+                                           int32_t* list, int32_t* n, int32_t* over) {
+  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+  constexpr int kB = 8;
+  const int32_t e = ev.adj_off[v + 1];
+  for (int32_t i = ev.adj_off[v]; i < e; i += kB) {
+    int32_t q[kB];
+#pragma unroll
+    for (int j = 0; j < kB; ++j) q[j] = i + j < e ? ev.adj[i + j] : -1;
+    uint32_t in[kB], seen[kB];
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      in[j] = (q[j] >= 0 && mrow) ? mrow[q[j] >> 5] : ~0u;
+      seen[j] = q[j] >= 0 ? __hip_atomic_load(&row[q[j] >> 5], __ATOMIC_RELAXED, WG) : ~0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      if (q[j] < 0) continue;
+      const uint32_t bit = 1u << (q[j] & 31);
+      if (!(in[j] & bit) || (seen[j] & bit)) continue;
+      if (__hip_atomic_fetch_or(&row[q[j] >> 5], bit, __ATOMIC_RELAXED, WG) & bit) continue;
+      const int at = atomicAdd(n, 1);
+      if (at < kWorkCap) list[at] = q[j];
+      else *over = 1;                                  // the bit stays: a later round scans for it
+    }
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(kWave) void k_drain_marks(Table t, int2* __restrict__ store, int64_t nvars,
+                                                       InstFlags* flags, EventSet ev, int32_t* next_inst,
+                                                       unsigned long long* tight) {
+  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+  __shared__ int32_t s_list[2][kWorkCap];
+  __shared__ int32_t s_tv[kDrainVars];
+  __shared__ int32_t s_n[2], s_over[2];
+  __shared__ int32_t s_k, s_chg, s_fail, s_ntv;
+  const int lane = threadIdx.x;
+  int nt_all = 0;
+  for (;;) {
+    if (lane == 0) s_k = atomicAdd(next_inst, 1);
+    __syncthreads();
+    const int64_t k = s_k;
+    __syncthreads();                                   // every lane has read s_k
+    if (k >= ev.n_inst) break;
+    InstFlags* f = flags + k;
+    if (f->failed_at != kNoFail) continue;             // the same word for every lane: uniform
+    uint32_t* row = ev.dirty + k * ev.words;           // plane 0: the seeds, then every later mark
+    const uint32_t* mrow = ev.member ? ev.member + k * ev.words : nullptr;
+    int2* st = store + k * nvars;
+    if (lane == 0) { s_n[0] = 0; s_over[0] = 1; }      // round 0 scans the row
+    int cur = 0;
+    int64_t evals = 0;
+    for (int32_t r = 0;; ++r) {
+      __syncthreads();
+      // (A) take the round's propagators
+      if (s_over[cur]) {
+        __syncthreads();                               // every lane has read s_over
+        if (lane == 0) { s_n[cur] = 0; s_over[cur] = 0; }
+        __syncthreads();
+        // Round 0 reads the seeds with plain 16-byte loads, four in flight per
+        // lane (the row was written by earlier launches; the 16-byte chunks
+        // start at the aligned address below the row and stay inside the
+        // bitmap planes, the second of which follows this one); later scans
+        // read this workgroup's own atomics with workgroup-scope atomic loads.
+        const int o = (int)((reinterpret_cast<uintptr_t>(row) & 15) / 4);
+        const uint4* row4 = reinterpret_cast<const uint4*>(row - o);
+        const int64_t w4n = r == 0 ? (ev.words + o + 3) / 4 : 0;
+        for (int64_t g = lane; g < w4n; g += 4 * kWave) {
+          uint4 v4[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) v4[u] = g + u * kWave < w4n ? row4[g + u * kWave] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t ws[4] = {v4[u].x, v4[u].y, v4[u].z, v4[u].w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+              const int64_t w = 4 * (g + u * kWave) + h - o;
+              const uint32_t bits = (w >= 0 && w < ev.words) ? ws[h] : 0u;
+              if (!bits) continue;
+              const int c = __builtin_popcount(bits);
+              const int at = atomicAdd(&s_n[cur], c);
+              uint32_t take = 0;
+              int room = kWorkCap - at;
+              for (uint32_t b = bits; b && room > 0; b &= b - 1, --room) take |= b & (~b + 1u);
+              int j = at;
+              for (uint32_t b = take; b; b &= b - 1) s_list[cur][j++] = (int32_t)(w * 32 + __builtin_ctz(b));
+              if (take) __hip_atomic_fetch_and(&row[w], ~take, __ATOMIC_RELAXED, WG);
+              if (take != bits) s_over[cur] = 1;       // marks left for a later round
+            }
+          }
+        }
+        for (int64_t w = w4n ? ev.words : lane; w < ev.words; w += kWave) {
+          const uint32_t bits = __hip_atomic_load(&row[w], __ATOMIC_RELAXED, WG);
+          if (!bits) continue;
+          const int c = __builtin_popcount(bits);
+          const int at = atomicAdd(&s_n[cur], c);
+          uint32_t take = 0;
+          int room = kWorkCap - at;
+          for (uint32_t b = bits; b && room > 0; b &= b - 1, --room) take |= b & (~b + 1u);
+          int j = at;
+          for (uint32_t b = take; b; b &= b - 1) s_list[cur][j++] = (int32_t)(w * 32 + __builtin_ctz(b));
+          if (take) __hip_atomic_fetch_and(&row[w], ~take, __ATOMIC_RELAXED, WG);
+          if (take != bits) s_over[cur] = 1;           // marks left for a later round
+        }
+      } else {
+        const int n = min(s_n[cur], kWorkCap);
+        for (int i = lane; i < n; i += kWave) {
+          const int32_t p = s_list[cur][i];
+          __hip_atomic_fetch_and(&row[p >> 5], ~(1u << (p & 31)), __ATOMIC_RELAXED, WG);
+        }
+      }
+      const int nx = cur ^ 1;
+      __syncthreads();                                 // the round is taken: its bits are clear
+      const int n = min(s_n[cur], kWorkCap);
+      if (n == 0) break;                               // nothing marked: the fixpoint
+      if (lane == 0) { s_n[nx] = 0; s_over[nx] = s_over[cur]; s_chg = 0; s_fail = 0; s_ntv = 0; }
+      __syncthreads();
+      // (B) evaluate them, listing the variables they tighten
+      bool chg = false, fail = false;
+      for (int i = lane; i < n; i += kWave) {
+        const int32_t p = s_list[cur][i];
+        const uint32_t xo = t.xo[p];
+        const int32_t y = t.y[p], z = t.z[p], x = (int32_t)(xo & kVarMask);
+        const u64 wx = load_word<WG>(st + x), wy = load_word<WG>(st + y), wz = load_word<WG>(st + z);
+        bool c = false, fl = false, tx = false, ty = false, tz = false;
+        int nt = 0;
+        const int2 X = as_itv(wx), Y = as_itv(wy), Z = as_itv(wz);
+        if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) {
+          fl = true;
+        } else {
+          int2 nxv, nyv, nzv;
+          deduce<D == kItvDiv>((int)(xo >> kOpShift), X, Y, Z, nxv, nyv, nzv);
+          if (nxv.x > nxv.y || nyv.x > nyv.y || nzv.x > nzv.y) fl = true;
+          tx = meet_itv<WG>(st, x, X, nxv, c, fl, nt);
+          ty = meet_itv<WG>(st, y, Y, nyv, c, fl, nt);
+          tz = meet_itv<WG>(st, z, Z, nzv, c, fl, nt);
+        }
+        ++evals;
+        nt_all += nt;
+        chg |= c;
+        fail |= fl;
+        const int32_t vs[3] = {x, y, z};
+        const bool ts[3] = {tx, ty, tz};
+        for (int j = 0; j < 3; ++j) {
+          if (!ts[j]) continue;
+          const int at = atomicAdd(&s_ntv, 1);
+          if (at < kDrainVars) s_tv[at] = vs[j];
+          else drain_mark(row, mrow, ev, vs[j], s_list[nx], &s_n[nx], &s_over[nx]);
+        }
+      }
+      __syncthreads();                                 // the round's tightened variables are listed
+      // (C) mark their adjacency, a variable per lane
+      const int ntv = min(s_ntv, kDrainVars);
+      for (int i = lane; i < ntv; i += kWave) drain_mark(row, mrow, ev, s_tv[i], s_list[nx], &s_n[nx], &s_over[nx]);
+      if (__ballot(chg) && lane == 0) s_chg = 1;
+      if (__ballot(fail) && lane == 0) s_fail = 1;
+      __syncthreads();                                 // the round's marks and flags are in
+      if (lane == 0) {
+        if (s_chg) f->changed[r & 1] = r;
+        if (s_fail) f->failed_at = r;
+      }
+      if (s_fail) break;
+      cur = nx;
+    }
+    const int64_t we = wave_sum((int)evals);
+    if (lane == 0 && we) f->marked_evals += we;
+    __syncthreads();
+  }
+  const int sum = wave_sum(nt_all);
+  if (lane == 0 && sum)
+    atomicAdd(&tight[blockIdx.x & (kTightSlots - 1)], (unsigned long long)sum);
+}
+
+hipError_t launch_drain_marks(const Table& t, int2* store, int64_t nvars, InstFlags* flags, const EventSet& ev,
+                              int32_t* next_inst, unsigned long long* tight, int dom, int grid,
+                              hipStream_t stream) {
+  if (ev.n_inst <= 0 || !ev.dirty) return hipSuccess;
+  switch (dom) {
+    case kItv: k_drain_marks<kItv><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
+    case kItvDiv: k_drain_marks<kItvDiv><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, int64_t n_diffs,

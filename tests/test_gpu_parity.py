@@ -858,14 +858,17 @@ def test_seeded_first_sweep_around_the_base_store(gpu, case):
         assert res_g[0].failed
 
 
-@pytest.mark.parametrize("queue", [0, 10], ids=["bitmaps", "overflow"])
-def test_seeded_first_sweep_marks_from_bitmaps(gpu, queue):
-    """Seeded first sweeps without queues (mark bitmaps only), and with a
-    queue of 1,024 entries that the seeds overflow (the run sweeps from the
-    bitmaps from sweep 0 on): every instance equals the oracle."""
+@pytest.mark.parametrize("queue", [-1, 0, 10], ids=["queue", "bitmaps", "overflow"])
+@pytest.mark.parametrize("limit", [0, 1000], ids=["drain", "swept"])
+def test_seeded_first_sweep_marks_from_bitmaps(gpu, queue, limit):
+    """A seeded run to the fixpoint drains each instance's marks in one
+    launch; with a sweep limit (here one the run never reaches) it sweeps:
+    from queue planes, without queues (mark bitmaps only), or from a queue of
+    1,024 entries that the seeds overflow (bitmap sweeps from sweep 0 on).
+    Every instance equals the oracle."""
     V, P, n_inst = 4000, 16000, 150
     pir, lb, ub, rl, ru, slb, sub, off, idx = _seed_case(101, V, P, n_inst, True)
-    pir.set_config(mark_queue=queue)
+    pir.set_config(mark_queue=queue, max_sweeps=limit)
     lo_o, hi_o, res_o = _oracle_batch(pir, slb, sub, V, off, idx)
     b = pcx.Batch(pir, n_inst, V, off, idx)
     b.load(slb, sub)
