@@ -12,13 +12,20 @@ their fixpoints):
   random subset of 2,000-200,000 active propagators; int32 interval stores.
 A step loads a new set of 4,096 stores from device memory (one of four sets
 staged in HBM before the timed region, each drawn with its own seed) and
-runs the batched engine to every instance's fixpoint: the load, the
-emptiness scan and the interleaved copy of the stores are inside the timing.
-Beside it: the same batch replayed from its own reset image (no new
+runs the batched engine to every instance's fixpoint: the load (stores,
+emptiness, base store and the values that differ from it) is inside the
+timing.  The engine's default first sweep evaluates only the propagators
+adjacent to a value that differs from the base store and those the base
+store does not satisfy (the others compute in each instance what they
+compute on the base, where they tighten nothing); the same steps with a full
+first sweep (lockstep) are timed beside it and must reach the same stores.
+Beside them: the same batch replayed from its own reset image (no new
 stores), the plain engine (one sweep launch per instance and sweep), a
 search step on the device, and a bounded sample of instances on the CPU
 oracle (the reference's Gauss-Seidel algorithm, restated) on the host
-cores, whose results are checked bit-for-bit against the GPU's.
+cores, whose results are checked bit-for-bit against the GPU's.  The
+roofline is the dominant kernel's of the headline step (the one with the
+most device time per step: the load, the full sweep or the marked phase).
 
 The default line also carries the other workloads, each measured the same
 way (HBM-resident inputs, HIP-event kernel timing, CPU oracle beside it):
@@ -91,10 +98,14 @@ def measured_traffic(kernel, workload):
     return None
 
 
-def roofline(evals, tightened, kernel_ms, launches, kernel, workload, tight_bytes=BYTES_PER_TIGHTENING):
+def roofline(evals, tightened, kernel_ms, launches, kernel, workload, tight_bytes=BYTES_PER_TIGHTENING,
+             algo_bytes=None):
     """Algorithmic bytes per launch / average launch duration (HIP events on
-    the engine's stream around every launch of the dominant kernel)."""
-    algo_bytes = evals * BYTES_READ_PER_EVAL + tightened * tight_bytes
+    the engine's stream around every launch of the dominant kernel).  Sweep
+    kernels: 40 B per evaluation and the bytes of each tightening; a load
+    passes its own algorithmic bytes."""
+    if algo_bytes is None:
+        algo_bytes = evals * BYTES_READ_PER_EVAL + tightened * tight_bytes
     launches = max(1, launches)
     achieved = (algo_bytes / launches) / (kernel_ms * 1e-3 / launches) / 1e9 if kernel_ms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -109,7 +120,7 @@ class StatSum:
     active propagator) and marked-only sweeps."""
 
     FIELDS = ("kernel_ms", "kernel_launches", "evals", "tightened", "full_ms", "full_launches",
-              "full_evals", "full_tightened")
+              "full_evals", "full_tightened", "load_ms")
 
     def __init__(self):
         self.v = {k: 0 for k in self.FIELDS}
@@ -120,20 +131,25 @@ class StatSum:
             self.v[k] += getattr(st, k)
         self.lockstep = int(st.lockstep)
 
-    def roofline(self, steps, full_kernel, marked_kernel, workload, tight_bytes=BYTES_PER_TIGHTENING):
-        """Roofline of the dominant sweep kernel (the class with more kernel
-        time per step), plus the per-step figures of both classes."""
+    def roofline(self, steps, full_kernel, marked_kernel, workload, tight_bytes=BYTES_PER_TIGHTENING,
+                 load=None):
+        """Roofline of the dominant kernel (the class with the most kernel
+        time per step): the full sweeps, the marked sweeps, or -- load =
+        (kernel, algorithmic bytes per load) for steps that load new stores
+        -- the device load; plus the per-step figures of every class."""
         v = {k: x / steps for k, x in self.v.items()}
         full = (v["full_evals"], v["full_tightened"], v["full_ms"], v["full_launches"])
         marked = (v["evals"] - v["full_evals"], v["tightened"] - v["full_tightened"],
                   v["kernel_ms"] - v["full_ms"], v["kernel_launches"] - v["full_launches"])
-        dom, name = (full, full_kernel) if full[2] >= marked[2] else (marked, marked_kernel)
-        rf = roofline(*dom, name, workload, tight_bytes)
-        rf["sweep_classes_per_step"] = {
-            "full": {"kernel": full_kernel, "launches": full[3], "kernel_ms": round(full[2], 3),
-                     "evals": int(full[0])},
-            "marked": {"kernel": marked_kernel, "launches": marked[3], "kernel_ms": round(marked[2], 3),
-                       "evals": int(marked[0])}}
+        classes = {"full": (full, full_kernel, None), "marked": (marked, marked_kernel, None)}
+        if load is not None and v["load_ms"] > 0:
+            classes["load"] = ((0, 0, v["load_ms"], 1), load[0], load[1])
+        name = max(classes, key=lambda c: classes[c][0][2])
+        dom, kernel, ab = classes[name]
+        rf = roofline(*dom, kernel, workload, tight_bytes, algo_bytes=ab)
+        rf["kernel_classes_per_step"] = {
+            c: {"kernel": k, "launches": d[3], "kernel_ms": round(d[2], 3), "evals": int(d[0])}
+            for c, (d, k, _) in classes.items()}
         return rf
 
 
@@ -146,9 +162,14 @@ def full_kernel(mode, sums):
 
 
 def marked_kernel(pir, mode, sums, suffix=""):
-    """Name of the kernel of the marked sweeps: batched runs whose full sweep
-    is lockstep list their marks in queues by default (mark_queue)."""
-    if mode == pcx.MODE_BATCHED and sums.lockstep and pir.get_config().mark_queue != 0:
+    """Name of the kernel of the marked sweeps: batched runs seeded from a
+    base store drain each instance's marks in one launch; batched runs whose
+    full sweep is lockstep list their marks in queues by default
+    (mark_queue)."""
+    cfg = pir.get_config()
+    if mode == pcx.MODE_BATCHED and sums.v["full_evals"] == 0 and cfg.base_seed != 0 and not suffix:
+        return "k_drain_marks"
+    if mode == pcx.MODE_BATCHED and sums.lockstep and cfg.mark_queue != 0:
         return "k_sweep_queue" + suffix
     return "k_sweep_marked" + suffix
 
@@ -268,9 +289,9 @@ def bench_batched(args):
         "evals_per_s": evals / replay_s,
         "failed_instances": int(failed),
         "sweeps_max": int(smax),
-        "useful_lane_frac": useful_lane_fraction(off, n_inst, P, sums.lockstep),
-        "roofline": sums.roofline(args.steps, full_kernel(mode, sums), marked_kernel(pir, mode, sums), "cfg2"),
     }
+    out["replay"]["roofline"] = sums.roofline(args.steps, full_kernel(mode, sums),
+                                              marked_kernel(pir, mode, sums), "cfg2")
 
     if args.search_step:
         out["search_step"] = search_step(args, pir, batch, off, idx, res, lb_g, ub_g, V, n_inst)
@@ -319,10 +340,33 @@ def bench_batched(args):
     out["value"] = fr.pop("fixpoints_per_s")
     out["unit"] = "fixpoints/s"
     out["ms_per_step"] = fr.pop("ms_per_step")
+    out["roofline"] = fr.pop("roofline")
     out["fresh"] = fr
     out["batched_vs_plain"] = out["value"] / plain_fps if plain_fps else None
     batch.close()
     return out
+
+
+def ingest_kernel(pir, n_inst, V):
+    """(kernel, algorithmic bytes) of a device load of n_inst stores: with
+    base seeding the load writes the stores only (8 B read + 8 B written per
+    value), otherwise it also writes the interleaved copy (+8 B)."""
+    if pir.get_config().base_seed != 0:
+        return "k_ingest_rows", 16 * n_inst * V
+    return "k_ingest_interleave4", 24 * n_inst * V
+
+
+def timed_fresh(args, batch, sets, mode, steps):
+    """`steps` steps, each a device load of the next staged store set and a
+    run; returns (seconds, StatSum, the last step's results)."""
+    sums = StatSum()
+    t_start = time.perf_counter()
+    for s in range(steps):
+        d_lo, d_hi = sets[s % len(sets)]
+        batch.load_device(d_lo.ptr, d_hi.ptr)
+        res, st = batch.run(mode)
+        sums.add(st)
+    return time.perf_counter() - t_start, sums, res
 
 
 def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n_sets=4):
@@ -330,7 +374,9 @@ def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n
     root fixpoint drawn with their own seed) from device memory into the batch
     and propagates it.  The sets are staged in HBM before the timed region;
     consecutive steps load different sets.  The last step's results are
-    checked on a bounded sample against the oracle."""
+    checked on a bounded sample against the oracle.  The same steps with the
+    first sweep a full sweep (base seeding off) are timed beside them, and
+    their last stores compared with the seeded run's."""
     from pcx.device import DeviceArray
     t0 = time.time()
     sets = []
@@ -345,30 +391,27 @@ def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n
         d_lo, d_hi = sets[s % n_sets]
         batch.load_device(d_lo.ptr, d_hi.ptr)
         batch.run(mode)
-    sums = StatSum()
-    t_start = time.perf_counter()
-    for s in range(args.steps):
-        d_lo, d_hi = sets[s % n_sets]
-        batch.load_device(d_lo.ptr, d_hi.ptr)
-        res, st = batch.run(mode)
-        sums.add(st)
-    elapsed = time.perf_counter() - t_start
-    for d_lo, d_hi in sets:
-        d_lo.close()
-        d_hi.close()
+    elapsed, sums, res = timed_fresh(args, batch, sets, mode, args.steps)
     fps = n_inst * args.steps / elapsed
+    seeded = pir.get_config().base_seed != 0 and sums.v["full_evals"] == 0
+    load = ingest_kernel(pir, n_inst, V)
     out = {"fixpoints_per_s": fps, "ms_per_step": elapsed / args.steps * 1e3, "store_sets": n_sets,
            "evals_per_s": sums.v["evals"] / elapsed,
            "kernel_ms_per_step": sums.v["kernel_ms"] / args.steps,
+           "load_ms_per_step": sums.v["load_ms"] / args.steps,
            "failed_instances": int(sum(r.failed for r in res)),
-           "what": "each step loads 4,096 new stores from HBM (pcx_batch_load_device: the stores, "
-                   "their interleaved copy for the lockstep full sweep and the emptiness in one pass) "
-                   "and propagates them"}
-    log(f"fresh: {out['ms_per_step']:.2f} ms/step, {fps:.0f} fixpoints/s")
+           "first_sweep": "seeded from the base store" if seeded else "full",
+           "roofline": sums.roofline(args.steps, full_kernel(mode, sums), marked_kernel(pir, mode, sums),
+                                     "cfg2", load=load),
+           "what": "each step loads 4,096 new stores from HBM (pcx_batch_load_device: the stores, the "
+                   "emptiness, the base store and the values that differ from it, in one pass) and "
+                   "propagates them; the first sweep evaluates only the propagators adjacent to a "
+                   "listed value and those the base store does not satisfy"}
+    log(f"fresh: {out['ms_per_step']:.2f} ms/step, {fps:.0f} fixpoints/s, first sweep {out['first_sweep']}")
+    lb_s, ub_s = batch.read()
     if not args.no_cpu:
         ol = cpu_oracle()
         top, tx, ty, tz = pir.table()
-        lb_g, ub_g = batch.read()
         k = min(n_inst, args.fresh_check)
         lo, hi = last_host[0][:k * V].copy(), last_host[1][:k * V].copy()
         r = ol.fixpoint_batch(top, tx, ty, tz, lo, hi, V, idx=idx[:off[k]].astype(np.int64),
@@ -377,11 +420,36 @@ def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n
         for j in range(k):
             if bool(res[j].failed) != bool(r[j].failed):
                 mism += 1
-            elif not r[j].failed and not (np.array_equal(lb_g[j * V:(j + 1) * V], lo[j * V:(j + 1) * V]) and
-                                          np.array_equal(ub_g[j * V:(j + 1) * V], hi[j * V:(j + 1) * V])):
+            elif not r[j].failed and not (np.array_equal(lb_s[j * V:(j + 1) * V], lo[j * V:(j + 1) * V]) and
+                                          np.array_equal(ub_s[j * V:(j + 1) * V], hi[j * V:(j + 1) * V])):
                 mism += 1
         out["parity"] = {"checked_instances": k, "mismatches": mism, "of_step": args.steps}
         log(f"fresh parity: {k} instances of the last step, mismatches {mism}")
+    if seeded:
+        # the same steps with a full first sweep (lockstep), for comparison
+        pir.set_config(base_seed=0)
+        for s in range(max(1, args.warmup)):
+            d_lo, d_hi = sets[s % n_sets]
+            batch.load_device(d_lo.ptr, d_hi.ptr)
+            batch.run(mode)
+        el_f, sums_f, res_f = timed_fresh(args, batch, sets, mode, args.steps)
+        lb_f, ub_f = batch.read()
+        ok = ~np.repeat(np.array([bool(x.failed) for x in res]), V)
+        agree = (np.array_equal(np.array([bool(x.failed) for x in res_f]), np.array([bool(x.failed) for x in res]))
+                 and np.array_equal(lb_f[ok], lb_s[ok]) and np.array_equal(ub_f[ok], ub_s[ok]))
+        out["full_first_sweep"] = {
+            "fixpoints_per_s": n_inst * args.steps / el_f, "ms_per_step": el_f / args.steps * 1e3,
+            "load_ms_per_step": sums_f.v["load_ms"] / args.steps,
+            "same_fixpoints_as_seeded": bool(agree),
+            "useful_lane_frac": useful_lane_fraction(off, n_inst, len(pir.table()[0]), sums_f.lockstep),
+            "roofline": sums_f.roofline(args.steps, full_kernel(mode, sums_f), marked_kernel(pir, mode, sums_f),
+                                        "cfg2", load=ingest_kernel(pir, n_inst, V))}
+        pir.set_config(base_seed=-1)
+        log(f"fresh, full first sweep: {out['full_first_sweep']['ms_per_step']:.2f} ms/step, "
+            f"same fixpoints: {agree}")
+    for d_lo, d_hi in sets:
+        d_lo.close()
+        d_hi.close()
     return out
 
 

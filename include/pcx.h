@@ -133,6 +133,9 @@ typedef struct {
   int64_t full_tightened;
   int64_t lockstep;         /* instances per lockstep group of the full sweeps
                                (batched mode), 0 for per-instance work items  */
+  double  load_ms;          /* device time of the pcx_batch_load_device this
+                               run started from (config.timing; reported by
+                               the first run after the load, else 0)         */
 } pcx_stats;
 
 /* ---- library ---------------------------------------------------------- */
@@ -295,8 +298,9 @@ typedef struct {
 /* Time every point of {batched: props_per_wave x compact x stream x
  * event_driven, and lockstep 16/32/64 with event_driven} U {persistent: lds (where the store fits) x compact} on the
  * batch's loaded image (one warm-up and `reps` timed runs per point, best
- * kept), set the engine's configuration to the fastest point, and restore
- * the stores to the image.  results (may be NULL): up to max_results points
+ * kept; the points shape the full sweeps and the sweeps after them, so the
+ * runs take a full first sweep whatever base_seed says), set the engine's
+ * configuration to the fastest point, and restore the stores to the image.  results (may be NULL): up to max_results points
  * in the order tried; *n_results (may be NULL) receives how many were tried.
  * No counterpart in the reference (its fixpoint has no tuning parameters). */
 pcx_status pcx_batch_autotune(pcx_batch* b, int32_t reps, pcx_tuning* best,

@@ -265,7 +265,9 @@ pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* resu
     stats->full_evals = full_evals;
     stats->full_tightened = (int64_t)tf;
     stats->lockstep = tm.lockstep;
+    stats->load_ms = b->run_from_image ? b->load_ms : 0.0;
   }
+  b->load_ms = 0.0;
   return PCX_OK;
 }
 
@@ -1175,6 +1177,8 @@ void pcx_batch_destroy(pcx_batch* b) {
   if (b->plain_queue_graph) (void)hipGraphExecDestroy(b->plain_queue_graph);
   dfree(b->d_q); dfree(b->d_qctl);
   dfree(b->d_diff); dfree(b->d_diff_val); dfree(b->d_unsat); dfree(b->d_seedctl); dfree(b->d_base);
+  for (hipEvent_t ev : b->load_ev)
+    if (ev) (void)hipEventDestroy(ev);
   delete b;
 }
 
@@ -1279,6 +1283,16 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
   if (dst != PCX_OK) return dst;
   const int gi = lockstep_of(b);
   b->image_in_diffs = false;
+  // config.timing: HIP events around the load's kernels
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  bool t1_recorded = false;
+  if (e->cfg.timing) {
+    for (hipEvent_t& ev : b->load_ev)
+      if (!ev) PCX_HIP(hipEventCreate(&ev));
+    t0 = b->load_ev[0];
+    t1 = b->load_ev[1];
+    PCX_HIP(hipEventRecord(t0, e->stream));
+  }
   if (ds.list && b->shared_loads) {
     // Stores that share most values (the last load's did): write the stores
     // only.  A complete difference list (base store + listed values) is then
@@ -1286,6 +1300,8 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
     // interleaved copy a lockstep full sweep reads when the batch has one,
     // and later loads write it in the ingest pass again.
     PCX_HIP(launch_ingest_rows(d_lb, d_ub, b->d_store, b->nvars, b->n_inst, b->d_image_empty, ds, e->stream));
+    if (t1) PCX_HIP(hipEventRecord(t1, e->stream));
+    t1_recorded = true;                 // (a copy of the image below records it again)
     bool complete = false;
     pcx_status st = diffs_complete(b, &complete);
     if (st != PCX_OK) return st;
@@ -1305,6 +1321,7 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
       } else {
         PCX_HIP(launch_copy_words(b->d_store, b->d_image, n, e->stream));
       }
+      if (t1) PCX_HIP(hipEventRecord(t1, e->stream));
     }
   } else if (gi > 0) {
     // Batched runs sweep a lockstep copy first: write it in the same pass
@@ -1324,7 +1341,13 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
     b->shadow_is_image = false;
     b->image_in_shadow = false;
   }
+  if (t1 && !t1_recorded) PCX_HIP(hipEventRecord(t1, e->stream));
   PCX_HIP(hipStreamSynchronize(e->stream));
+  if (t1) {
+    float ms = 0.f;
+    PCX_HIP(hipEventElapsedTime(&ms, t0, t1));
+    b->load_ms = ms;
+  }
   if (ds.list && !b->shared_loads) {
     // a load of the other kind whose stores share most values: the next load
     // writes the stores only
@@ -1509,6 +1532,7 @@ pcx_status pcx_batch_autotune(pcx_batch* b, int32_t reps, pcx_tuning* best, pcx_
     e->cfg.event_driven = p.event_driven;
     e->cfg.lockstep = p.lockstep;
     e->cfg.timing = 0;
+    e->cfg.base_seed = 0;               // the points shape full sweeps: time those
     double bt = -1.0;
     for (int r = 0; r <= reps; ++r) {
       pcx_status s = pcx_batch_reset(b);

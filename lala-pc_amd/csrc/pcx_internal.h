@@ -75,6 +75,8 @@ struct pcx_batch {
   int2* d_diff_val = nullptr;         // the listed values (device loads)
   bool image_in_diffs = false;        // the image is the base store + the listed values
   bool shared_loads = true;           // the last device load's stores shared most values
+  double load_ms = 0.0;               // device time of the last device load (timing), until reported
+  hipEvent_t load_ev[2] = {nullptr, nullptr};
   int64_t diff_cap = 0;
   int32_t* d_unsat = nullptr;         // [table_n]
   int32_t* d_seedctl = nullptr;       // differences per sub-list, then propagators in d_unsat

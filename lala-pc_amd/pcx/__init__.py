@@ -69,7 +69,7 @@ class Stats(ctypes.Structure):
                 ("tightened", ctypes.c_int64), ("sweeps_max", ctypes.c_int64),
                 ("full_ms", ctypes.c_double), ("full_launches", ctypes.c_int64),
                 ("full_evals", ctypes.c_int64), ("full_tightened", ctypes.c_int64),
-                ("lockstep", ctypes.c_int64)]
+                ("lockstep", ctypes.c_int64), ("load_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
