@@ -167,8 +167,8 @@ def marked_kernel(pir, mode, sums, suffix=""):
     full sweep is lockstep list their marks in queues by default
     (mark_queue)."""
     cfg = pir.get_config()
-    if mode == pcx.MODE_BATCHED and sums.v["full_evals"] == 0 and cfg.base_seed != 0 and not suffix:
-        return "k_drain_marks"
+    if mode == pcx.MODE_BATCHED and sums.v["full_evals"] == 0 and cfg.base_seed != 0:
+        return "k_drain_marks" + suffix
     if mode == pcx.MODE_BATCHED and sums.lockstep and cfg.mark_queue != 0:
         return "k_sweep_queue" + suffix
     return "k_sweep_marked" + suffix
@@ -590,6 +590,27 @@ def bench_bits(args):
     words_g = batch.read_words()
     rf = sums.roofline(args.steps, full_kernel(mode, sums) + "_bits", marked_kernel(pir, mode, sums, "_bits"),
                        "cfg3", tight_bytes=8)
+    full_first = None
+    if mode == pcx.MODE_BATCHED and sums.v["full_evals"] == 0:
+        # the same steps with a full first sweep (seeding off): same fixpoints
+        pir.set_config(base_seed=0)
+        fsums = StatSum()
+        batch.reset()
+        batch.run(mode)
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.reset()
+            fres, fst = batch.run(mode)
+            fsums.add(fst)
+        f_el = time.perf_counter() - t1
+        same = (np.array_equal(batch.read_words(), words_g) and
+                all(bool(a.failed) == bool(b.failed) for a, b in zip(fres, res)))
+        full_first = {"fixpoints_per_s": n_inst * args.steps / f_el, "ms_per_step": f_el / args.steps * 1e3,
+                      "same_fixpoints_as_seeded": same,
+                      "roofline": fsums.roofline(args.steps, full_kernel(mode, fsums) + "_bits",
+                                                 marked_kernel(pir, mode, fsums, "_bits"), "cfg3",
+                                                 tight_bytes=8)}
+        pir.set_config(base_seed=-1)
     out = {"value": fps, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
            "config": {"workload": "bitset store: 1,024 bit-removal subproblems (1-20 removals each) of "
                                   "the root fixpoint of one 200,000-word / 800,000-propagator planted "
@@ -601,7 +622,7 @@ def bench_bits(args):
                       "lockstep": sums.lockstep, "autotuned": tuning},
            "single_fixpoint_ms": single_ms, "single_sweeps": int(r_single.sweeps),
            "evals_per_s": evals / elapsed, "failed_instances": int(sum(r.failed for r in res)),
-           "roofline": rf}
+           "roofline": rf, "full_first_sweep": full_first}
     if not args.no_cpu:
         ol = cpu_oracle()
         top, tx, ty, tz = pir.table()

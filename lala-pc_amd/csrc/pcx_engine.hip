@@ -667,7 +667,7 @@ pcx_status plan_seed(pcx_batch* b, bool* seeded, int64_t* n_diff_out, int64_t* n
   *n_unsat_out = 0;
   pcx_engine* e = b->e;
   if (!b->events || b->seeded_run || !b->run_from_image || !b->diffs_listed || e->cfg.base_seed == 0 ||
-      e->cfg.domain != PCX_DOMAIN_INTERVAL || b->table_n == 0)
+      b->table_n == 0)
     return PCX_OK;
   if (!b->d_unsat) PCX_HIP(hipMalloc(&b->d_unsat, b->table_n * sizeof(int32_t)));
   int32_t* unsat_n = b->d_seedctl + kDiffLists;
@@ -1203,14 +1203,14 @@ pcx_status ensure_diff_list(pcx_batch* b) {
   return PCX_OK;
 }
 
-// Where this load lists the differences from the base store (interval batches of
-// several instances with seeding on), after clearing the count; an empty
+// Where this load lists the differences from the base store (batches of
+// several instances with seeding on), after clearing the counts; an empty
 // sink otherwise.
 pcx_status diff_sink(pcx_batch* b, DiffSink* ds) {
   *ds = DiffSink{nullptr, nullptr, 0, nullptr, nullptr};
   b->diffs_listed = false;
   const pcx_engine* e = b->e;
-  if (e->cfg.base_seed == 0 || e->cfg.domain != PCX_DOMAIN_INTERVAL || b->n_inst < 2 || b->nvars <= 0)
+  if (e->cfg.base_seed == 0 || b->n_inst < 2 || b->nvars <= 0)
     return PCX_OK;
   pcx_status st = ensure_diff_list(b);
   if (st != PCX_OK) return st;
@@ -1246,6 +1246,22 @@ pcx_status diffs_complete(pcx_batch* b, bool* complete) {
   for (int c = 0; c < kDiffLists; ++c) *complete = *complete && cnt[c] <= b->diff_cap;
   return PCX_OK;
 }
+
+// After a load from host memory (the stores and d_image hold the image):
+// list the differences from the base store.  A complete list is then also
+// the reset image, which a reset writes without reading d_image.
+pcx_status list_load_diffs(pcx_batch* b) {
+  DiffSink ds;
+  pcx_status st = diff_sink(b, &ds);
+  if (st != PCX_OK || !ds.list) return st;
+  PCX_HIP(launch_list_diffs(b->d_store, b->nvars, b->n_inst, ds, b->e->stream));
+  bool complete = false;
+  st = diffs_complete(b, &complete);
+  if (st != PCX_OK) return st;
+  b->diffs_listed = true;
+  b->image_in_diffs = complete;
+  return PCX_OK;
+}
 }  // namespace
 
 pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub) {
@@ -1263,12 +1279,7 @@ pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub) {
   PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, n * sizeof(int2), hipMemcpyDeviceToDevice, e->stream));
   pcx_status st = image_loaded(b);
   if (st != PCX_OK) return st;
-  DiffSink ds;
-  st = diff_sink(b, &ds);
-  if (st != PCX_OK || !ds.list) return st;
-  PCX_HIP(launch_list_diffs(b->d_store, b->nvars, b->n_inst, ds, e->stream));
-  b->diffs_listed = true;
-  return PCX_OK;
+  return list_load_diffs(b);
 }
 
 pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_t* d_ub) {
@@ -1673,7 +1684,9 @@ pcx_status pcx_bits_batch_load(pcx_batch* b, const uint64_t* words) {
   const int64_t n = b->n_inst * b->nvars;
   PCX_HIP(hipMemcpyAsync(b->d_image, words, n * sizeof(uint64_t), hipMemcpyHostToDevice, e->stream));
   PCX_HIP(hipMemcpyAsync(b->d_store, b->d_image, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, e->stream));
-  return image_loaded(b);
+  pcx_status st = image_loaded(b);
+  if (st != PCX_OK) return st;
+  return list_load_diffs(b);
 }
 
 pcx_status pcx_bits_batch_read(pcx_batch* b, uint64_t* words) {

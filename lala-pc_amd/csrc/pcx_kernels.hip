@@ -3000,7 +3000,8 @@ hipError_t launch_image_from_diffs(const DiffSink& ds, int2* store, int64_t nvar
 
 // One thread per propagator, on the base store alone (nothing is written
 // back): positions of the propagators that would tighten it or read an
-// empty interval.
+// empty domain.
+template <int D>
 __global__ __launch_bounds__(kBlock) void k_base_probe(Table t, const int2* __restrict__ base,
                                                        int32_t* __restrict__ list, int32_t* count) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -3009,13 +3010,26 @@ __global__ __launch_bounds__(kBlock) void k_base_probe(Table t, const int2* __re
     bool need = false;
     if (p < t.n) {
       const uint32_t xo = t.xo[p];
-      const int2 X = base[xo & kVarMask], Y = base[t.y[p]], Z = base[t.z[p]];
-      if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) {
-        need = true;
+      const int op = (int)(xo >> kOpShift);
+      if constexpr (D == kBits) {
+        const u64* bw = reinterpret_cast<const u64*>(base);
+        const u64 X = bw[xo & kVarMask], Y = bw[t.y[p]], Z = bw[t.z[p]];
+        if (!X || !Y || !Z) {
+          need = true;
+        } else {
+          u64 nx, ny, nz;
+          deduce_bits(op, X, Y, Z, nx, ny, nz);
+          need = (X & nx) != X || (Y & ny) != Y || (Z & nz) != Z;
+        }
       } else {
-        int2 nx, ny, nz;
-        deduce<true>((int)(xo >> kOpShift), X, Y, Z, nx, ny, nz);
-        need = nx.x > X.x || nx.y < X.y || ny.x > Y.x || ny.y < Y.y || nz.x > Z.x || nz.y < Z.y;
+        const int2 X = base[xo & kVarMask], Y = base[t.y[p]], Z = base[t.z[p]];
+        if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) {
+          need = true;
+        } else {
+          int2 nx, ny, nz;
+          deduce<true>(op, X, Y, Z, nx, ny, nz);
+          need = nx.x > X.x || nx.y < X.y || ny.x > Y.x || ny.y < Y.y || nz.x > Z.x || nz.y < Z.y;
+        }
       }
     }
     // the same wave-aggregated append, of table positions
@@ -3034,11 +3048,14 @@ __global__ __launch_bounds__(kBlock) void k_base_probe(Table t, const int2* __re
 
 hipError_t launch_base_probe(const Table& t, const int2* base, int32_t* list, int32_t* count, int dom,
                              hipStream_t stream) {
-  if (dom != kItv && dom != kItvDiv) return hipErrorInvalidValue;
   if (t.n <= 0) return hipSuccess;
   int64_t grid = (t.n + kBlock - 1) / kBlock;
   if (grid > 65536) grid = 65536;
-  k_base_probe<<<(unsigned)grid, kBlock, 0, stream>>>(t, base, list, count);
+  switch (dom) {
+    case kItv: case kItvDiv: k_base_probe<kItv><<<(unsigned)grid, kBlock, 0, stream>>>(t, base, list, count); break;
+    case kBits: k_base_probe<kBits><<<(unsigned)grid, kBlock, 0, stream>>>(t, base, list, count); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -3257,16 +3274,29 @@ __global__ __launch_bounds__(kWave) void k_drain_marks(Table t, int2* __restrict
         const u64 wx = load_word<WG>(st + x), wy = load_word<WG>(st + y), wz = load_word<WG>(st + z);
         bool c = false, fl = false, tx = false, ty = false, tz = false;
         int nt = 0;
-        const int2 X = as_itv(wx), Y = as_itv(wy), Z = as_itv(wz);
-        if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) {
-          fl = true;
+        if constexpr (D == kBits) {
+          if (!wx || !wy || !wz) {
+            fl = true;
+          } else {
+            u64 nxw, nyw, nzw;
+            deduce_bits((int)(xo >> kOpShift), wx, wy, wz, nxw, nyw, nzw);
+            u64* sw = reinterpret_cast<u64*>(st);
+            tx = meet_word<WG>(sw, x, wx, nxw, c, fl, nt);
+            ty = meet_word<WG>(sw, y, wy, nyw, c, fl, nt);
+            tz = meet_word<WG>(sw, z, wz, nzw, c, fl, nt);
+          }
         } else {
-          int2 nxv, nyv, nzv;
-          deduce<D == kItvDiv>((int)(xo >> kOpShift), X, Y, Z, nxv, nyv, nzv);
-          if (nxv.x > nxv.y || nyv.x > nyv.y || nzv.x > nzv.y) fl = true;
-          tx = meet_itv<WG>(st, x, X, nxv, c, fl, nt);
-          ty = meet_itv<WG>(st, y, Y, nyv, c, fl, nt);
-          tz = meet_itv<WG>(st, z, Z, nzv, c, fl, nt);
+          const int2 X = as_itv(wx), Y = as_itv(wy), Z = as_itv(wz);
+          if (X.x > X.y || Y.x > Y.y || Z.x > Z.y) {
+            fl = true;
+          } else {
+            int2 nxv, nyv, nzv;
+            deduce<D == kItvDiv>((int)(xo >> kOpShift), X, Y, Z, nxv, nyv, nzv);
+            if (nxv.x > nxv.y || nyv.x > nyv.y || nzv.x > nzv.y) fl = true;
+            tx = meet_itv<WG>(st, x, X, nxv, c, fl, nt);
+            ty = meet_itv<WG>(st, y, Y, nyv, c, fl, nt);
+            tz = meet_itv<WG>(st, z, Z, nzv, c, fl, nt);
+          }
         }
         ++evals;
         nt_all += nt;
@@ -3311,6 +3341,7 @@ hipError_t launch_drain_marks(const Table& t, int2* store, int64_t nvars, InstFl
   switch (dom) {
     case kItv: k_drain_marks<kItv><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
     case kItvDiv: k_drain_marks<kItvDiv><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
+    case kBits: k_drain_marks<kBits><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -105,7 +105,7 @@ def test_bits_lockstep(gpu, gi, events):
     """Bitset full sweeps in lockstep groups of gi over 100 instances."""
     V, P, n = 4000, 16000, 100
     op, x, y, z, w = pcx.gen_bits_network(61, V, P)
-    pir = pcx.BitsPIR(event_driven=events, lockstep=gi)
+    pir = pcx.BitsPIR(event_driven=events, lockstep=gi, base_seed=0)
     pir.tell(op, x, y, z, w)
     root = w.copy()
     assert not pir.fixpoint(root).failed
@@ -118,6 +118,80 @@ def test_bits_lockstep(gpu, gi, events):
     rg, st = b.run(pcx.MODE_BATCHED)
     assert st.lockstep == gi
     _compare(b.read_words(), rg, wo, ro, V, f"bits lockstep {gi}")
+
+
+def _bits_seed_case(seed, V, P, n):
+    op, x, y, z, w = pcx.gen_bits_network(seed, V, P)
+    pir = pcx.BitsPIR()
+    pir.tell(op, x, y, z, w)
+    root = w.copy()
+    assert not pir.fixpoint(root).failed
+    return pir, w, root, pcx.gen_bits_splits(seed + 1, root, n, 1, 40)
+
+
+@pytest.mark.parametrize("case", ["splits", "empty", "base-unsatisfied", "unpropagated"])
+@pytest.mark.parametrize("gi", [0, 64])
+def test_bits_seeded_first_sweep(gpu, gi, case):
+    """Bit-removal splits of the root fixpoint share all but a few words with
+    the base store (per word the value two of instances 0-2 hold): the first
+    sweep evaluates only the propagators adjacent to a differing word and
+    those the base does not satisfy, and the marks drain in one launch.  One
+    instance holding an empty word, the base widened back to a told word in
+    instances 0-2 (not a fixpoint), and unpropagated stores (too many
+    unsatisfied propagators: a full first sweep).  Every instance equals the
+    oracle, after a reset too, and the run with a full first sweep agrees."""
+    V, P, n = 4000, 16000, 150
+    pir, w, root, inst = _bits_seed_case(71, V, P, n)
+    v = int(np.nonzero(w != root)[0][0])                   # a word the root narrowed
+    if case == "empty":
+        inst[33] = 0
+    elif case == "base-unsatisfied":
+        for k in range(3):
+            inst[k * V + v] = w[v]
+    elif case == "unpropagated":
+        inst = pcx.gen_bits_splits(72, w, n, 1, 40)
+    pir.set_config(lockstep=gi)
+    top, tx, ty, tz = pir.table()
+    wo = inst.copy()
+    ro = ol.bits_fixpoint_batch(top, tx, ty, tz, wo, V, threads=8)
+    b = pcx.Batch(pir, n, V)
+    b.load_words(inst)
+    rg, st = b.run(pcx.MODE_BATCHED)
+    _compare(b.read_words(), rg, wo, ro, V, f"bits seeded, {case}, lockstep {gi}")
+    if case == "unpropagated":
+        assert st.full_evals > 0
+        return
+    assert st.full_evals == 0 and st.evals < n * P // 4, (st.full_evals, st.evals)
+    if case == "empty":
+        assert rg[0].failed
+    b.reset()
+    r2, st2 = b.run(pcx.MODE_BATCHED)
+    assert st2.full_evals == 0
+    _compare(b.read_words(), r2, wo, ro, V, "bits seeded after reset")
+    pir.set_config(base_seed=0)
+    b.reset()
+    rf, stf = b.run(pcx.MODE_BATCHED)
+    assert stf.full_evals > 0
+    _compare(b.read_words(), rf, wo, ro, V, "bits full first sweep")
+
+
+@pytest.mark.parametrize("queue", [-1, 0, 10], ids=["queue", "bitmaps", "overflow"])
+def test_bits_seeded_swept(gpu, queue):
+    """With a sweep limit (one the run never reaches) a seeded bitset run
+    sweeps instead of draining: from queue planes, from the mark bitmaps
+    alone, or from a queue the seeds overflow.  Every instance equals the
+    oracle."""
+    V, P, n = 4000, 16000, 150
+    pir, w, root, inst = _bits_seed_case(75, V, P, n)
+    pir.set_config(mark_queue=queue, max_sweeps=1000)
+    top, tx, ty, tz = pir.table()
+    wo = inst.copy()
+    ro = ol.bits_fixpoint_batch(top, tx, ty, tz, wo, V, threads=8)
+    b = pcx.Batch(pir, n, V)
+    b.load_words(inst)
+    rg, st = b.run(pcx.MODE_BATCHED)
+    assert st.full_evals == 0
+    _compare(b.read_words(), rg, wo, ro, V, f"bits seeded, mark_queue {queue}")
 
 
 @pytest.mark.parametrize("lds", [1, 0])
