@@ -357,6 +357,7 @@ pcx_status clear_marks(pcx_batch* b) {
   const int64_t srows = b->n_inst * summary_words(b);
   PCX_HIP(hipMemsetAsync(b->d_dirty, 0, 2 * rows * sizeof(uint32_t), e->stream));
   PCX_HIP(hipMemsetAsync(b->d_summary, 0, 2 * srows * sizeof(uint32_t), e->stream));
+  b->marks_clean = true;
   return PCX_OK;
 }
 
@@ -395,8 +396,10 @@ pcx_status prepare_events(pcx_batch* b, int32_t mode) {
   }
   pcx_status st = ensure_event_buffers(b);
   if (st != PCX_OK) return st;
-  st = clear_marks(b);
+  // A run that drained every instance's marks left the planes clear.
+  if (!b->marks_clean) st = clear_marks(b);
   if (st != PCX_OK) return st;
+  b->marks_clean = false;                   // until this run ends drained
   b->events = true;
   return PCX_OK;
 }
@@ -729,12 +732,16 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     // the first turn advances to sweep 0, whose plane the seeds fill
     if (base_seeded) PCX_HIP(hipMemsetAsync(b->d_qctl + kQSweep, 0xff, sizeof(int32_t), e->stream));
   }
-  if (base_seeded)
-    PCX_HIP(launch_seed_from_base(ev, DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val}, n_diff,
+  if (base_seeded) {
+    // A drain reads neither the summaries nor queues: plain marks.
+    EventSet sev = ev;
+    if (drain) sev.summary = nullptr;
+    PCX_HIP(launch_seed_from_base(sev, DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val}, n_diff,
                                   b->d_unsat, n_unsat, e->stream));
+  }
   if (drain) {
     PCX_HIP(hipMemsetAsync(b->d_ctl + 3, 0, sizeof(int32_t), e->stream));
-    const int dgrid = (int)std::min<int64_t>(b->n_inst, (int64_t)pcx_internal::device_cus(e->cfg.device) * 16);
+    const int dgrid = (int)std::min<int64_t>(b->n_inst, (int64_t)pcx_internal::device_cus(e->cfg.device) * 24);
     pcx_status ts = tm.begin(e->stream);
     if (ts != PCX_OK) return ts;
     PCX_HIP(launch_drain_marks(table_of(e), b->d_store, b->nvars, b->d_flags, ev, b->d_ctl + 3,
@@ -745,7 +752,10 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     ts = tm.drain();
     if (ts != PCX_OK) return ts;
     std::vector<InstFlags> flags;
-    return collect(b, flags, results, stats, tm);
+    ts = collect(b, flags, results, stats, tm);
+    // the drain took and cleared every mark (rows of failed instances too)
+    if (ts == PCX_OK) b->marks_clean = true;
+    return ts;
   }
   auto launch_marked = [&]() -> pcx_status {
     PCX_HIP(launch_schedule(sub, b->table_n, b->d_flags, (int32_t)b->n_inst, wl, ppw, maxs, e->stream,
@@ -1438,6 +1448,7 @@ pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var, i
     if (st == PCX_OK) st = clear_marks(child);
     if (st != PCX_OK) { (void)hipFree(d_var); return st; }
     he = launch_branch_seed(event_set(child), d_var, child->n_inst, e->stream);
+    child->marks_clean = false;
     child->seeded = he == hipSuccess;
   }
   if (he == hipSuccess && var)

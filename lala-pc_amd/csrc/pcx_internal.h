@@ -66,6 +66,7 @@ struct pcx_batch {
   bool events = false;                // this run uses marked-only sweeps
   bool seeded = false;                // the next run may start from seeded marks
   bool seeded_run = false;            // this run starts with a marked sweep
+  bool marks_clean = false;           // both mark planes and the summaries are zero
   // first sweeps seeded from a base store (per variable the value instances
   // 0-2 agree on): the base, the values of the image that differ from it
   // (both written by the load), and the propagators the base does not

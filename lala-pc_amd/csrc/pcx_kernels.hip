@@ -1452,8 +1452,8 @@ __global__ __launch_bounds__(kBlock, lock_min_waves<D>()) void k_sweep_lockstep(
                 // Bounds only move inward, so an interval differs iff it
                 // tightens; a deduced empty interval of a non-empty one
                 // tightens too.
-                tight = (itv_bits(nx[u]) != wx[u]) | (itv_bits(ny[u]) != wy[u]) |
-                        (itv_bits(nz[u]) != wz[u]);
+                tight = ((itv_bits(nx[u]) ^ wx[u]) | (itv_bits(ny[u]) ^ wy[u]) |
+                         (itv_bits(nz[u]) ^ wz[u])) != 0;
               }
               need[u] = on[u] & (empty_in | tight);
               any |= need[u];
@@ -3066,6 +3066,10 @@ __device__ __forceinline__ void seed_prop(const Marker& mk, const EventSet& ev, 
                                           int64_t k, int32_t p) {
   const uint32_t bit = 1u << (p & 31);
   if (mk.member && !(mk.member[p >> 5] & bit)) return;
+  if (!mk.sum && !ev.q) {                  // bits only: a non-returning atomic
+    atomicOr(&mk.dirty[p >> 5], bit);
+    return;
+  }
   if (atomicOr(&mk.dirty[p >> 5], bit) & bit) return;
   if (mk.sum) set_summary(nullptr, mk.sum, p >> kChunkShift);
   if (ev.q) stage_entry(qs, ev, 0, ((unsigned long long)k << 32) | (uint32_t)p);
@@ -3141,8 +3145,8 @@ __global__ __launch_bounds__(kBlock) void k_seed_from_base(EventSet ev, DiffSink
 // invariant: at the end of round r every propagator not marked is at its
 // fixpoint on the store.  Workgroup-scope atomics and loads, as in the
 // persistent sweeps (one workgroup per instance).
-constexpr int kWorkCap = 1024;
-constexpr int kDrainVars = 1024;          // tightened variables listed per round
+constexpr int kWorkCap = 512;
+constexpr int kDrainVars = 512;           // tightened variables listed per round
 
 // Marks of the member propagators adjacent to v in an instance's bitmap row,
 // eight adjacency entries in flight at a time; every bit this sets is
@@ -3193,8 +3197,11 @@ __global__ __launch_bounds__(kWave) void k_drain_marks(Table t, int2* __restrict
     __syncthreads();                                   // every lane has read s_k
     if (k >= ev.n_inst) break;
     InstFlags* f = flags + k;
-    if (f->failed_at != kNoFail) continue;             // the same word for every lane: uniform
     uint32_t* row = ev.dirty + k * ev.words;           // plane 0: the seeds, then every later mark
+    if (f->failed_at != kNoFail) {                     // the same word for every lane: uniform
+      for (int64_t w = lane; w < ev.words; w += kWave) row[w] = 0;   // its seeds: the row ends clear
+      continue;
+    }
     const uint32_t* mrow = ev.member ? ev.member + k * ev.words : nullptr;
     int2* st = store + k * nvars;
     if (lane == 0) { s_n[0] = 0; s_over[0] = 1; }      // round 0 scans the row
@@ -3322,7 +3329,10 @@ __global__ __launch_bounds__(kWave) void k_drain_marks(Table t, int2* __restrict
         if (s_chg) f->changed[r & 1] = r;
         if (s_fail) f->failed_at = r;
       }
-      if (s_fail) break;
+      if (s_fail) {                                    // its marks are left: the row ends clear
+        for (int64_t w = lane; w < ev.words; w += kWave) __hip_atomic_store(&row[w], 0u, __ATOMIC_RELAXED, WG);
+        break;
+      }
       cur = nx;
     }
     const int64_t we = wave_sum((int)evals);

@@ -858,6 +858,46 @@ def test_seeded_first_sweep_around_the_base_store(gpu, case):
         assert res_g[0].failed
 
 
+def test_drained_runs_leave_the_marks_clear(gpu):
+    """A drained run takes and clears every mark it evaluates, and clears the
+    rows of instances that start failed or fail while draining, so the next
+    run on the batch skips clearing the mark planes.  Instances 7 and 20 fail
+    in their first round (an add whose operands are fixed, and its result
+    fixed to another value than their sum), instance 3 starts failed.  Seeded runs after a
+    reset, then runs with a full first sweep (and marked sweeps after it) and
+    with a sweep limit: every instance equals the oracle each time."""
+    V, P, n_inst = 4000, 16000, 60
+    pir, lb, ub, rl, ru, slb, sub, off, idx = _seed_case(111, V, P, n_inst, False)
+    top, tx, ty, tz = pir.table()
+    made = 0
+    for p in np.nonzero(top == pcx.ADD)[0]:
+        x, y, z = int(tx[p]), int(ty[p]), int(tz[p])
+        if len({x, y, z}) < 3:
+            continue
+        v = int(rl[y]) + int(rl[z]) + 1                       # not the sum
+        for k in (7, 20):
+            slb[k * V + y] = sub[k * V + y] = rl[y]
+            slb[k * V + z] = sub[k * V + z] = rl[z]
+            slb[k * V + x] = sub[k * V + x] = v
+        made = 1
+        break
+    assert made
+    slb[3 * V + 11], sub[3 * V + 11] = 7, 6
+    lo_o, hi_o, res_o = _oracle_batch(pir, slb, sub, V, None, None)
+    assert res_o[7].failed and res_o[20].failed and res_o[3].failed
+    b = pcx.Batch(pir, n_inst, V)
+    b.load(slb, sub)
+    for what, cfg in [("seeded", {}), ("seeded again", {}), ("full first sweep", {"base_seed": 0}),
+                      ("swept", {"base_seed": -1, "max_sweeps": 1000}), ("seeded last", {"max_sweeps": 0})]:
+        if cfg:
+            pir.set_config(**cfg)
+        b.reset()
+        res_g, st = b.run(pcx.MODE_BATCHED)
+        assert (st.full_evals == 0) == (what != "full first sweep"), what
+        lb_g, ub_g = b.read()
+        _compare(lb_g, ub_g, res_g, lo_o, hi_o, res_o, V, what)
+
+
 @pytest.mark.parametrize("queue", [-1, 0, 10], ids=["queue", "bitmaps", "overflow"])
 @pytest.mark.parametrize("limit", [0, 1000], ids=["drain", "swept"])
 def test_seeded_first_sweep_marks_from_bitmaps(gpu, queue, limit):
