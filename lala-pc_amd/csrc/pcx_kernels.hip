@@ -3130,151 +3130,163 @@ __global__ __launch_bounds__(kBlock) void k_seed_from_base(EventSet ev, DiffSink
   }
 }
 
-// The marked phase of a seeded run, one instance per workgroup of one wave:
+// The marked phase of a seeded run, one instance per workgroup of B threads:
 // the workgroup owns the instance (its store, its bitmap row of plane 0 and
 // its flags), so instead of device-wide sweeps it drains the instance's
 // marks in rounds separated by workgroup barriers, and the whole phase is
-// one launch.  Round r: (A) the round's propagators are taken and their
-// bits cleared -- from the LDS list the last round filled, or, when that
-// list overflowed (and in round 0, whose marks are the seeds), by scanning
-// the bitmap row for up to kWorkCap of them; (B) they are evaluated, and a
-// tightening sets the bits of the member propagators adjacent to the
-// variable, listing each bit it sets for round r + 1.  A propagator whose
-// operand another lane tightens in phase B has its bit clear (phase A) and
-// is listed again, so the barrier-separated rounds keep the marked-sweep
-// invariant: at the end of round r every propagator not marked is at its
-// fixpoint on the store.  Workgroup-scope atomics and loads, as in the
-// persistent sweeps (one workgroup per instance).
+// one launch.  Round r evaluates the list L_r; each variable it tightens
+// puts its adjacent member propagators in L_{r+1}, once each (an LDS hash
+// set of L_{r+1}).  A propagator marked while it is evaluated is listed
+// again, so the rounds keep the marked-sweep invariant: at the end of round
+// r every propagator not in L_{r+1} (or set in the row) is at its fixpoint
+// on the store.  The bitmap row holds the seeds (round 0 scans it) and the
+// marks that find L_{r+1} full (the next round also scans the row, for up to
+// the list's capacity, and leaves the rest for a later scan).  The marks
+// themselves are LDS operations: a round waits on memory for the table,
+// the operands and the meets (phase B), then the adjacency (phase C).
+// Workgroup-scope atomics and loads, as in the persistent sweeps.
 constexpr int kWorkCap = 512;
-constexpr int kDrainVars = 512;           // tightened variables listed per round
+constexpr int kDrainVars = 256;           // tightened variables listed per round
+constexpr int kDrainHash = 1024;          // LDS hash slots of the next list
 
-// Marks of the member propagators adjacent to v in an instance's bitmap row,
-// eight adjacency entries in flight at a time; every bit this sets is
-// listed for the next round (or, when the list is full, left for a scan).
+// List q for the next round: into the LDS hash set of the list (each
+// propagator once) and at the end of the list; a propagator that finds the
+// list full sets its bit in the row instead, for a scan.  Insertions start
+// only while the list has room, so the set holds at most kWorkCap plus one
+// per thread of the workgroup (< kDrainHash): a probe always ends.
+__device__ __forceinline__ void drain_list(int32_t q, uint32_t* row, int32_t* hs, int32_t* list, int32_t* n,
+                                           int32_t* over) {
+  if (__hip_atomic_load(n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < kWorkCap) {
+    uint32_t h = ((uint32_t)q * 2654435761u) >> 22;   // Fibonacci hashing: the top 10 bits
+    for (;;) {
+      const int32_t old = atomicCAS(&hs[h], -1, q);
+      if (old == q) return;                            // listed already
+      if (old == -1) break;
+      h = (h + 1) & (kDrainHash - 1);
+    }
+    const int at = atomicAdd(n, 1);
+    if (at < kWorkCap) {
+      list[at] = q;
+      return;
+    }
+  }
+  __hip_atomic_fetch_or(&row[q >> 5], 1u << (q & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  *over = 1;
+}
+
+// The member propagators adjacent to v, listed for the next round; eight
+// adjacency entries (and their member words) in flight at a time.
 __device__ __forceinline__ void drain_mark(uint32_t* row, const uint32_t* mrow, const EventSet& ev, int32_t v,
-                                           int32_t* list, int32_t* n, int32_t* over) {
-  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+                                           int32_t* hs, int32_t* list, int32_t* n, int32_t* over) {
   constexpr int kB = 8;
   const int32_t e = ev.adj_off[v + 1];
   for (int32_t i = ev.adj_off[v]; i < e; i += kB) {
     int32_t q[kB];
 #pragma unroll
     for (int j = 0; j < kB; ++j) q[j] = i + j < e ? ev.adj[i + j] : -1;
-    uint32_t in[kB], seen[kB];
+    uint32_t in[kB];
 #pragma unroll
-    for (int j = 0; j < kB; ++j) {
-      in[j] = (q[j] >= 0 && mrow) ? mrow[q[j] >> 5] : ~0u;
-      seen[j] = q[j] >= 0 ? __hip_atomic_load(&row[q[j] >> 5], __ATOMIC_RELAXED, WG) : ~0u;
-    }
+    for (int j = 0; j < kB; ++j) in[j] = (q[j] >= 0 && mrow) ? mrow[q[j] >> 5] : ~0u;
 #pragma unroll
-    for (int j = 0; j < kB; ++j) {
-      if (q[j] < 0) continue;
-      const uint32_t bit = 1u << (q[j] & 31);
-      if (!(in[j] & bit) || (seen[j] & bit)) continue;
-      if (__hip_atomic_fetch_or(&row[q[j] >> 5], bit, __ATOMIC_RELAXED, WG) & bit) continue;
-      const int at = atomicAdd(n, 1);
-      if (at < kWorkCap) list[at] = q[j];
-      else *over = 1;                                  // the bit stays: a later round scans for it
-    }
+    for (int j = 0; j < kB; ++j)
+      if (q[j] >= 0 && ((in[j] >> (q[j] & 31)) & 1u)) drain_list(q[j], row, hs, list, n, over);
   }
 }
 
-template <int D>
-__global__ __launch_bounds__(kWave) void k_drain_marks(Table t, int2* __restrict__ store, int64_t nvars,
-                                                       InstFlags* flags, EventSet ev, int32_t* next_inst,
-                                                       unsigned long long* tight) {
+// Round 0 reads the seeds with plain 16-byte loads, four in flight per
+// thread (the row was written by earlier launches; the 16-byte chunks start
+// at the aligned address below the row and stay inside the bitmap planes,
+// the second of which follows this one); later scans read this workgroup's
+// own atomics with workgroup-scope atomic loads.  Taken bits are cleared;
+// bits beyond the list's room stay, and set *over.
+template <int B>
+__device__ __forceinline__ void drain_scan(uint32_t* row, int64_t words, bool first, int32_t* list,
+                                           int32_t* n, int32_t* over) {
+  constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+  const int tid = threadIdx.x;
+  auto take = [&](int64_t w, uint32_t bits) {
+    const int c = __builtin_popcount(bits);
+    const int at = atomicAdd(n, c);
+    uint32_t t = 0;
+    int room = kWorkCap - at;
+    for (uint32_t b = bits; b && room > 0; b &= b - 1, --room) t |= b & (~b + 1u);
+    int j = at;
+    for (uint32_t b = t; b; b &= b - 1) list[j++] = (int32_t)(w * 32 + __builtin_ctz(b));
+    if (t) __hip_atomic_fetch_and(&row[w], ~t, __ATOMIC_RELAXED, WG);
+    if (t != bits) *over = 1;                          // marks left for a later scan
+  };
+  const int o = (int)((reinterpret_cast<uintptr_t>(row) & 15) / 4);
+  const uint4* row4 = reinterpret_cast<const uint4*>(row - o);
+  const int64_t w4n = first ? (words + o + 3) / 4 : 0;
+  for (int64_t g = tid; g < w4n; g += 4 * B) {
+    uint4 v4[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v4[u] = g + u * B < w4n ? row4[g + u * B] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t ws[4] = {v4[u].x, v4[u].y, v4[u].z, v4[u].w};
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int64_t w = 4 * (g + u * B) + h - o;
+        const uint32_t bits = (w >= 0 && w < words) ? ws[h] : 0u;
+        if (bits) take(w, bits);
+      }
+    }
+  }
+  for (int64_t w = w4n ? words : tid; w < words; w += B) {
+    const uint32_t bits = __hip_atomic_load(&row[w], __ATOMIC_RELAXED, WG);
+    if (bits) take(w, bits);
+  }
+}
+
+template <int D, int B>
+__global__ __launch_bounds__(B) void k_drain_marks(Table t, int2* __restrict__ store, int64_t nvars,
+                                                   InstFlags* flags, EventSet ev, int32_t* next_inst,
+                                                   unsigned long long* tight) {
   constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
   __shared__ int32_t s_list[2][kWorkCap];
+  __shared__ int32_t s_hash[kDrainHash];
   __shared__ int32_t s_tv[kDrainVars];
   __shared__ int32_t s_n[2], s_over[2];
   __shared__ int32_t s_k, s_chg, s_fail, s_ntv;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   int nt_all = 0;
   for (;;) {
-    if (lane == 0) s_k = atomicAdd(next_inst, 1);
+    if (tid == 0) s_k = atomicAdd(next_inst, 1);
     __syncthreads();
     const int64_t k = s_k;
-    __syncthreads();                                   // every lane has read s_k
+    __syncthreads();                                   // every thread has read s_k
     if (k >= ev.n_inst) break;
     InstFlags* f = flags + k;
-    uint32_t* row = ev.dirty + k * ev.words;           // plane 0: the seeds, then every later mark
-    if (f->failed_at != kNoFail) {                     // the same word for every lane: uniform
-      for (int64_t w = lane; w < ev.words; w += kWave) row[w] = 0;   // its seeds: the row ends clear
+    uint32_t* row = ev.dirty + k * ev.words;           // plane 0: the seeds, and marks that overflow
+    if (f->failed_at != kNoFail) {                     // the same word for every thread: uniform
+      for (int64_t w = tid; w < ev.words; w += B) row[w] = 0;   // its seeds: the row ends clear
       continue;
     }
     const uint32_t* mrow = ev.member ? ev.member + k * ev.words : nullptr;
     int2* st = store + k * nvars;
-    if (lane == 0) { s_n[0] = 0; s_over[0] = 1; }      // round 0 scans the row
+    if (tid == 0) { s_n[0] = 0; s_over[0] = 1; }       // round 0 scans the row
     int cur = 0;
     int64_t evals = 0;
     for (int32_t r = 0;; ++r) {
       __syncthreads();
-      // (A) take the round's propagators
-      if (s_over[cur]) {
-        __syncthreads();                               // every lane has read s_over
-        if (lane == 0) { s_n[cur] = 0; s_over[cur] = 0; }
-        __syncthreads();
-        // Round 0 reads the seeds with plain 16-byte loads, four in flight per
-        // lane (the row was written by earlier launches; the 16-byte chunks
-        // start at the aligned address below the row and stay inside the
-        // bitmap planes, the second of which follows this one); later scans
-        // read this workgroup's own atomics with workgroup-scope atomic loads.
-        const int o = (int)((reinterpret_cast<uintptr_t>(row) & 15) / 4);
-        const uint4* row4 = reinterpret_cast<const uint4*>(row - o);
-        const int64_t w4n = r == 0 ? (ev.words + o + 3) / 4 : 0;
-        for (int64_t g = lane; g < w4n; g += 4 * kWave) {
-          uint4 v4[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) v4[u] = g + u * kWave < w4n ? row4[g + u * kWave] : make_uint4(0, 0, 0, 0);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const uint32_t ws[4] = {v4[u].x, v4[u].y, v4[u].z, v4[u].w};
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-              const int64_t w = 4 * (g + u * kWave) + h - o;
-              const uint32_t bits = (w >= 0 && w < ev.words) ? ws[h] : 0u;
-              if (!bits) continue;
-              const int c = __builtin_popcount(bits);
-              const int at = atomicAdd(&s_n[cur], c);
-              uint32_t take = 0;
-              int room = kWorkCap - at;
-              for (uint32_t b = bits; b && room > 0; b &= b - 1, --room) take |= b & (~b + 1u);
-              int j = at;
-              for (uint32_t b = take; b; b &= b - 1) s_list[cur][j++] = (int32_t)(w * 32 + __builtin_ctz(b));
-              if (take) __hip_atomic_fetch_and(&row[w], ~take, __ATOMIC_RELAXED, WG);
-              if (take != bits) s_over[cur] = 1;       // marks left for a later round
-            }
-          }
-        }
-        for (int64_t w = w4n ? ev.words : lane; w < ev.words; w += kWave) {
-          const uint32_t bits = __hip_atomic_load(&row[w], __ATOMIC_RELAXED, WG);
-          if (!bits) continue;
-          const int c = __builtin_popcount(bits);
-          const int at = atomicAdd(&s_n[cur], c);
-          uint32_t take = 0;
-          int room = kWorkCap - at;
-          for (uint32_t b = bits; b && room > 0; b &= b - 1, --room) take |= b & (~b + 1u);
-          int j = at;
-          for (uint32_t b = take; b; b &= b - 1) s_list[cur][j++] = (int32_t)(w * 32 + __builtin_ctz(b));
-          if (take) __hip_atomic_fetch_and(&row[w], ~take, __ATOMIC_RELAXED, WG);
-          if (take != bits) s_over[cur] = 1;           // marks left for a later round
-        }
-      } else {
-        const int n = min(s_n[cur], kWorkCap);
-        for (int i = lane; i < n; i += kWave) {
-          const int32_t p = s_list[cur][i];
-          __hip_atomic_fetch_and(&row[p >> 5], ~(1u << (p & 31)), __ATOMIC_RELAXED, WG);
-        }
-      }
       const int nx = cur ^ 1;
-      __syncthreads();                                 // the round is taken: its bits are clear
+      // (A) a round after an overflow (and round 0) also takes marks from the row
+      if (s_over[cur]) {
+        __syncthreads();                               // every thread has read s_over
+        if (tid == 0) s_over[cur] = 0;
+        __syncthreads();
+        drain_scan<B>(row, ev.words, r == 0, s_list[cur], &s_n[cur], &s_over[cur]);
+        __syncthreads();                               // the round's list is complete
+      }
       const int n = min(s_n[cur], kWorkCap);
       if (n == 0) break;                               // nothing marked: the fixpoint
-      if (lane == 0) { s_n[nx] = 0; s_over[nx] = s_over[cur]; s_chg = 0; s_fail = 0; s_ntv = 0; }
+      if (tid == 0) { s_n[nx] = 0; s_over[nx] = s_over[cur]; s_chg = 0; s_fail = 0; s_ntv = 0; }
+      for (int i = tid; i < kDrainHash; i += B) s_hash[i] = -1;
       __syncthreads();
-      // (B) evaluate them, listing the variables they tighten
+      // (B) evaluate the list, listing the variables it tightens
       bool chg = false, fail = false;
-      for (int i = lane; i < n; i += kWave) {
+      for (int i = tid; i < n; i += B) {
         const int32_t p = s_list[cur][i];
         const uint32_t xo = t.xo[p];
         const int32_t y = t.y[p], z = t.z[p], x = (int32_t)(xo & kVarMask);
@@ -3315,32 +3327,32 @@ __global__ __launch_bounds__(kWave) void k_drain_marks(Table t, int2* __restrict
           if (!ts[j]) continue;
           const int at = atomicAdd(&s_ntv, 1);
           if (at < kDrainVars) s_tv[at] = vs[j];
-          else drain_mark(row, mrow, ev, vs[j], s_list[nx], &s_n[nx], &s_over[nx]);
+          else drain_mark(row, mrow, ev, vs[j], s_hash, s_list[nx], &s_n[nx], &s_over[nx]);
         }
       }
       __syncthreads();                                 // the round's tightened variables are listed
-      // (C) mark their adjacency, a variable per lane
+      // (C) list their adjacency, a variable per thread
       const int ntv = min(s_ntv, kDrainVars);
-      for (int i = lane; i < ntv; i += kWave) drain_mark(row, mrow, ev, s_tv[i], s_list[nx], &s_n[nx], &s_over[nx]);
-      if (__ballot(chg) && lane == 0) s_chg = 1;
-      if (__ballot(fail) && lane == 0) s_fail = 1;
-      __syncthreads();                                 // the round's marks and flags are in
-      if (lane == 0) {
+      for (int i = tid; i < ntv; i += B) drain_mark(row, mrow, ev, s_tv[i], s_hash, s_list[nx], &s_n[nx], &s_over[nx]);
+      if (__ballot(chg) && lane_id() == 0) s_chg = 1;
+      if (__ballot(fail) && lane_id() == 0) s_fail = 1;
+      __syncthreads();                                 // the next list and the flags are in
+      if (tid == 0) {
         if (s_chg) f->changed[r & 1] = r;
         if (s_fail) f->failed_at = r;
       }
-      if (s_fail) {                                    // its marks are left: the row ends clear
-        for (int64_t w = lane; w < ev.words; w += kWave) __hip_atomic_store(&row[w], 0u, __ATOMIC_RELAXED, WG);
+      if (s_fail) {                                    // marks may be left: the row ends clear
+        for (int64_t w = tid; w < ev.words; w += B) __hip_atomic_store(&row[w], 0u, __ATOMIC_RELAXED, WG);
         break;
       }
       cur = nx;
     }
     const int64_t we = wave_sum((int)evals);
-    if (lane == 0 && we) f->marked_evals += we;
+    if (lane_id() == 0 && we) atomicAdd((unsigned long long*)&f->marked_evals, (unsigned long long)we);
     __syncthreads();
   }
   const int sum = wave_sum(nt_all);
-  if (lane == 0 && sum)
+  if (lane_id() == 0 && sum)
     atomicAdd(&tight[blockIdx.x & (kTightSlots - 1)], (unsigned long long)sum);
 }
 
@@ -3349,9 +3361,9 @@ hipError_t launch_drain_marks(const Table& t, int2* store, int64_t nvars, InstFl
                               hipStream_t stream) {
   if (ev.n_inst <= 0 || !ev.dirty) return hipSuccess;
   switch (dom) {
-    case kItv: k_drain_marks<kItv><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
-    case kItvDiv: k_drain_marks<kItvDiv><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
-    case kBits: k_drain_marks<kBits><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
+    case kItv: k_drain_marks<kItv, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
+    case kItvDiv: k_drain_marks<kItvDiv, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
+    case kBits: k_drain_marks<kBits, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
