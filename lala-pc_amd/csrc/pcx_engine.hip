@@ -54,6 +54,7 @@ bool valid_mark_queue(int q) { return q == -1 || q == 0 || (q >= 10 && q <= 26);
 bool valid_base_seed(int v) { return v == -1 || v == 0 || v == 1; }
 // d_seedctl: the difference sub-lists' counts, then the unsatisfied count
 constexpr int kSeedCtlWords = kDiffLists + 1;
+constexpr int32_t kTrailCap = 1024;     // variables a drained instance lists for a restore
 
 template <class T>
 void dfree(T*& p) {
@@ -442,6 +443,7 @@ pcx_status prepare_run(pcx_batch* b, int32_t mode) {
   }
   b->run_from_image = b->store_is_image;
   b->store_is_image = false;
+  b->trail_valid = false;                   // until a drained run from the image lists its changes
   pcx_status st = ensure_pinned(b);
   if (st != PCX_OK) return st;
   st = prepare_tables(b);
@@ -744,8 +746,13 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     const int dgrid = (int)std::min<int64_t>(b->n_inst, (int64_t)pcx_internal::device_cus(e->cfg.device) * 24);
     pcx_status ts = tm.begin(e->stream);
     if (ts != PCX_OK) return ts;
+    if (!b->d_trail) {
+      PCX_HIP(hipMalloc(&b->d_trail, b->n_inst * kTrailCap * sizeof(int32_t)));
+      PCX_HIP(hipMalloc(&b->d_trail_n, b->n_inst * sizeof(int32_t)));
+    }
     PCX_HIP(launch_drain_marks(table_of(e), b->d_store, b->nvars, b->d_flags, ev, b->d_ctl + 3,
-                               b->d_tight + kTightSlots, dom_of(e), dgrid, e->stream));
+                               b->d_tight + kTightSlots, Trail{b->d_trail, b->d_trail_n, kTrailCap}, dom_of(e),
+                               dgrid, e->stream));
     ts = tm.end(e->stream, false);
     if (ts != PCX_OK) return ts;
     PCX_HIP(hipStreamSynchronize(e->stream));
@@ -753,8 +760,13 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     if (ts != PCX_OK) return ts;
     std::vector<InstFlags> flags;
     ts = collect(b, flags, results, stats, tm);
-    // the drain took and cleared every mark (rows of failed instances too)
-    if (ts == PCX_OK) b->marks_clean = true;
+    // the drain took and cleared every mark (rows of failed instances too),
+    // and it started from the image: the stores differ from it only in the
+    // trail's variables
+    if (ts == PCX_OK) {
+      b->marks_clean = true;
+      b->trail_valid = b->run_from_image;
+    }
     return ts;
   }
   auto launch_marked = [&]() -> pcx_status {
@@ -1186,7 +1198,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   if (b->queue_graph) (void)hipGraphExecDestroy(b->queue_graph);
   if (b->plain_queue_graph) (void)hipGraphExecDestroy(b->plain_queue_graph);
   dfree(b->d_q); dfree(b->d_qctl);
-  dfree(b->d_diff); dfree(b->d_diff_val); dfree(b->d_unsat); dfree(b->d_seedctl); dfree(b->d_base);
+  dfree(b->d_diff); dfree(b->d_diff_val); dfree(b->d_unsat); dfree(b->d_trail); dfree(b->d_trail_n); dfree(b->d_seedctl); dfree(b->d_base);
   for (hipEvent_t ev : b->load_ev)
     if (ev) (void)hipEventDestroy(ev);
   delete b;
@@ -1197,9 +1209,10 @@ namespace {
 // while they are a small share of the values (see seed_from_base).
 pcx_status ensure_diff_list(pcx_batch* b) {
   const int64_t want = std::min<int64_t>(std::max<int64_t>(b->n_inst * b->nvars / 128, 4096), 1 << 24);
-  // Twice the even share (at least 256): workgroups fill the sub-lists
-  // unevenly when a load has few of them.
-  const int64_t per_list = std::max<int64_t>(2 * ((want + kDiffLists - 1) / kDiffLists), 256);
+  // Twice the even share, and at least the 2,048 values one listing
+  // workgroup covers: workgroups fill the sub-lists unevenly when a load has
+  // few of them, and an instance's differences may sit in one block.
+  const int64_t per_list = std::max<int64_t>(2 * ((want + kDiffLists - 1) / kDiffLists), 2048);
   if (!b->d_seedctl) PCX_HIP(hipMalloc(&b->d_seedctl, kSeedCtlWords * sizeof(int32_t)));
   if (!b->d_base) PCX_HIP(hipMalloc(&b->d_base, b->nvars * sizeof(int2)));
   if (b->diff_cap < per_list) {
@@ -1241,6 +1254,7 @@ pcx_status image_loaded(pcx_batch* b) {
   b->shadow_is_image = false;
   b->image_in_shadow = false;
   b->image_in_diffs = false;
+  b->trail_valid = false;
   return PCX_OK;
 }
 
@@ -1385,7 +1399,13 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
 pcx_status pcx_batch_reset(pcx_batch* b) {
   if (!b) return fail_with(PCX_EINVAL, "NULL argument");
   pcx_engine* e = b->e;
-  if (b->image_in_diffs)
+  if (b->store_is_image) return PCX_OK;
+  const DiffSink ds{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val};
+  if (b->trail_valid && (b->image_in_diffs || !b->image_in_shadow))
+    PCX_HIP(launch_restore_trail(Trail{b->d_trail, b->d_trail_n, kTrailCap}, ds,
+                                 b->image_in_diffs ? nullptr : b->d_image, b->d_store, b->nvars, b->n_inst,
+                                 e->stream));
+  else if (b->image_in_diffs)
     PCX_HIP(launch_image_from_diffs(DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val},
                                     b->d_store, b->nvars, b->n_inst, e->stream));
   else if (b->image_in_shadow)

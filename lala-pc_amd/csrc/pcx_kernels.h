@@ -248,10 +248,23 @@ hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, int6
 // The marked phase of a seeded run (marks in plane 0), one instance per
 // one-wave workgroup taken from *next_inst (zeroed by the caller): every
 // instance to its fixpoint in rounds separated by workgroup barriers, in
-// one launch; flags record the rounds as sweeps.  Interval domains.
+// one launch; flags record the rounds as sweeps.  With a trail, each
+// instance also lists the variables it tightens (n[k] of them; more than
+// cap: the list is incomplete), so that a restore rewrites only those.
+struct Trail {
+  int32_t* list;     // [n_inst * cap], or nullptr
+  int32_t* n;        // [n_inst]
+  int32_t cap;
+};
 hipError_t launch_drain_marks(const Table& t, int2* store, int64_t nvars, InstFlags* flags, const EventSet& ev,
-                              int32_t* next_inst, unsigned long long* tight, int dom, int grid,
-                              hipStream_t st);
+                              int32_t* next_inst, unsigned long long* tight, const Trail& trail, int dom,
+                              int grid, hipStream_t st);
+// The stores back to their image after a drained run with a trail: each
+// instance's listed variables (its whole row when the list is incomplete)
+// from the image, which is either d_image or the base store plus the listed
+// differences (ds.vals non-null: written after the base values).
+hipError_t launch_restore_trail(const Trail& trail, const DiffSink& ds, const int2* image, int2* store,
+                                int64_t nvars, int64_t n_inst, hipStream_t st);
 // dst (n_inst * nvars words, row by row) from an interleaved copy
 hipError_t launch_deinterleave(const int2* shadow, int2* dst, int64_t nvars, int64_t n_inst, int gi,
                                hipStream_t st);

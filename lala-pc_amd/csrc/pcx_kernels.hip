@@ -2998,6 +2998,44 @@ hipError_t launch_image_from_diffs(const DiffSink& ds, int2* store, int64_t nvar
   return hipGetLastError();
 }
 
+// One workgroup per instance: its trail's variables (or, past the cap, its
+// whole row) rewritten from the image: d_image, or the base store when the
+// listed differences are scattered afterwards.
+__global__ __launch_bounds__(kBlock) void k_restore_trail(Trail tr, const u64* __restrict__ base,
+                                                          const u64* __restrict__ image, u64* __restrict__ store,
+                                                          int64_t nvars, int64_t n_inst) {
+  for (int64_t k = blockIdx.x; k < n_inst; k += gridDim.x) {
+    const int32_t n = tr.n[k];
+    u64* row = store + k * nvars;
+    const u64* irow = image ? image + k * nvars : nullptr;
+    if (n <= tr.cap) {
+      const int32_t* list = tr.list + k * tr.cap;
+      for (int i = threadIdx.x; i < n; i += kBlock) {
+        const int32_t v = list[i];
+        row[v] = irow ? irow[v] : base[v];
+      }
+    } else {
+      for (int64_t v = threadIdx.x; v < nvars; v += kBlock) row[v] = irow ? irow[v] : base[v];
+    }
+  }
+}
+
+hipError_t launch_restore_trail(const Trail& tr, const DiffSink& ds, const int2* image, int2* store,
+                                int64_t nvars, int64_t n_inst, hipStream_t stream) {
+  if (nvars <= 0 || n_inst <= 0) return hipSuccess;
+  if (!tr.list || (!image && !ds.vals)) return hipErrorInvalidValue;
+  const int64_t grid = n_inst < 65536 ? n_inst : 65536;
+  u64* st = reinterpret_cast<u64*>(store);
+  k_restore_trail<<<(unsigned)grid, kBlock, 0, stream>>>(tr, reinterpret_cast<const u64*>(ds.base),
+                                                         reinterpret_cast<const u64*>(image), st, nvars, n_inst);
+  if (!image) {
+    int64_t g2 = (kDiffLists * ds.cap + kBlock - 1) / kBlock;
+    if (g2 > 8192) g2 = 8192;
+    k_scatter_diffs<<<(unsigned)g2, kBlock, 0, stream>>>(ds, st, nvars);
+  }
+  return hipGetLastError();
+}
+
 // One thread per propagator, on the base store alone (nothing is written
 // back): positions of the propagators that would tighten it or read an
 // empty domain.
@@ -3242,13 +3280,13 @@ __device__ __forceinline__ void drain_scan(uint32_t* row, int64_t words, bool fi
 template <int D, int B>
 __global__ __launch_bounds__(B) void k_drain_marks(Table t, int2* __restrict__ store, int64_t nvars,
                                                    InstFlags* flags, EventSet ev, int32_t* next_inst,
-                                                   unsigned long long* tight) {
+                                                   unsigned long long* tight, Trail trail) {
   constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
   __shared__ int32_t s_list[2][kWorkCap];
   __shared__ int32_t s_hash[kDrainHash];
   __shared__ int32_t s_tv[kDrainVars];
   __shared__ int32_t s_n[2], s_over[2];
-  __shared__ int32_t s_k, s_chg, s_fail, s_ntv;
+  __shared__ int32_t s_k, s_chg, s_fail, s_ntv, s_tn;
   const int tid = threadIdx.x;
   int nt_all = 0;
   for (;;) {
@@ -3261,11 +3299,19 @@ __global__ __launch_bounds__(B) void k_drain_marks(Table t, int2* __restrict__ s
     uint32_t* row = ev.dirty + k * ev.words;           // plane 0: the seeds, and marks that overflow
     if (f->failed_at != kNoFail) {                     // the same word for every thread: uniform
       for (int64_t w = tid; w < ev.words; w += B) row[w] = 0;   // its seeds: the row ends clear
+      if (tid == 0 && trail.list) trail.n[k] = 0;
       continue;
     }
+    int32_t* tlist = trail.list ? trail.list + k * trail.cap : nullptr;
+    // a tightened variable on the trail (past the cap only counted)
+    auto on_trail = [&](int32_t v) {
+      if (!tlist) return;
+      const int at = atomicAdd(&s_tn, 1);
+      if (at < trail.cap) tlist[at] = v;
+    };
     const uint32_t* mrow = ev.member ? ev.member + k * ev.words : nullptr;
     int2* st = store + k * nvars;
-    if (tid == 0) { s_n[0] = 0; s_over[0] = 1; }       // round 0 scans the row
+    if (tid == 0) { s_n[0] = 0; s_over[0] = 1; s_tn = 0; }   // round 0 scans the row
     int cur = 0;
     int64_t evals = 0;
     for (int32_t r = 0;; ++r) {
@@ -3326,14 +3372,21 @@ __global__ __launch_bounds__(B) void k_drain_marks(Table t, int2* __restrict__ s
         for (int j = 0; j < 3; ++j) {
           if (!ts[j]) continue;
           const int at = atomicAdd(&s_ntv, 1);
-          if (at < kDrainVars) s_tv[at] = vs[j];
-          else drain_mark(row, mrow, ev, vs[j], s_hash, s_list[nx], &s_n[nx], &s_over[nx]);
+          if (at < kDrainVars) {
+            s_tv[at] = vs[j];
+          } else {
+            drain_mark(row, mrow, ev, vs[j], s_hash, s_list[nx], &s_n[nx], &s_over[nx]);
+            on_trail(vs[j]);
+          }
         }
       }
       __syncthreads();                                 // the round's tightened variables are listed
       // (C) list their adjacency, a variable per thread
       const int ntv = min(s_ntv, kDrainVars);
-      for (int i = tid; i < ntv; i += B) drain_mark(row, mrow, ev, s_tv[i], s_hash, s_list[nx], &s_n[nx], &s_over[nx]);
+      for (int i = tid; i < ntv; i += B) {
+        drain_mark(row, mrow, ev, s_tv[i], s_hash, s_list[nx], &s_n[nx], &s_over[nx]);
+        on_trail(s_tv[i]);
+      }
       if (__ballot(chg) && lane_id() == 0) s_chg = 1;
       if (__ballot(fail) && lane_id() == 0) s_fail = 1;
       __syncthreads();                                 // the next list and the flags are in
@@ -3349,6 +3402,7 @@ __global__ __launch_bounds__(B) void k_drain_marks(Table t, int2* __restrict__ s
     }
     const int64_t we = wave_sum((int)evals);
     if (lane_id() == 0 && we) atomicAdd((unsigned long long*)&f->marked_evals, (unsigned long long)we);
+    if (tid == 0 && tlist) trail.n[k] = s_tn;          // the rounds' writes of s_tn ended at a barrier
     __syncthreads();
   }
   const int sum = wave_sum(nt_all);
@@ -3357,13 +3411,13 @@ __global__ __launch_bounds__(B) void k_drain_marks(Table t, int2* __restrict__ s
 }
 
 hipError_t launch_drain_marks(const Table& t, int2* store, int64_t nvars, InstFlags* flags, const EventSet& ev,
-                              int32_t* next_inst, unsigned long long* tight, int dom, int grid,
-                              hipStream_t stream) {
+                              int32_t* next_inst, unsigned long long* tight, const Trail& trail, int dom,
+                              int grid, hipStream_t stream) {
   if (ev.n_inst <= 0 || !ev.dirty) return hipSuccess;
   switch (dom) {
-    case kItv: k_drain_marks<kItv, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
-    case kItvDiv: k_drain_marks<kItvDiv, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
-    case kBits: k_drain_marks<kBits, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight); break;
+    case kItv: k_drain_marks<kItv, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, trail); break;
+    case kItvDiv: k_drain_marks<kItvDiv, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, trail); break;
+    case kBits: k_drain_marks<kBits, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, trail); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -165,6 +165,7 @@ def test_bits_seeded_first_sweep(gpu, gi, case):
     if case == "empty":
         assert rg[0].failed
     b.reset()
+    assert np.array_equal(b.read_words(), inst), "reset after a seeded run"
     r2, st2 = b.run(pcx.MODE_BATCHED)
     assert st2.full_evals == 0
     _compare(b.read_words(), r2, wo, ro, V, "bits seeded after reset")

@@ -809,6 +809,8 @@ def test_first_sweep_seeded_from_instance_zero(gpu, gi, load, subsets):
     _compare(lb_g, ub_g, res_g, lo_o, hi_o, res_o, V, f"seeded, lockstep {gi}, {load} load")
     assert res_g[5].failed
     b.reset()
+    lb_r, ub_r = b.read()
+    assert np.array_equal(lb_r, slb) and np.array_equal(ub_r, sub), "reset after a seeded run"
     res_2, st2 = b.run(pcx.MODE_BATCHED)
     assert st2.full_evals == 0
     lb_2, ub_2 = b.read()
@@ -861,7 +863,9 @@ def test_seeded_first_sweep_around_the_base_store(gpu, case):
 def test_drained_runs_leave_the_marks_clear(gpu):
     """A drained run takes and clears every mark it evaluates, and clears the
     rows of instances that start failed or fail while draining, so the next
-    run on the batch skips clearing the mark planes.  Instances 7 and 20 fail
+    run on the batch skips clearing the mark planes; its trail of tightened
+    variables lets a reset rewrite only those (every reset gives the loaded
+    stores back).  Instances 7 and 20 fail
     in their first round (an add whose operands are fixed, and its result
     fixed to another value than their sum), instance 3 starts failed.  Seeded runs after a
     reset, then runs with a full first sweep (and marked sweeps after it) and
@@ -883,6 +887,11 @@ def test_drained_runs_leave_the_marks_clear(gpu):
         break
     assert made
     slb[3 * V + 11], sub[3 * V + 11] = 7, 6
+    # instance 12: 1,500 variables back at their told bounds (more tightenings
+    # than its trail holds: its restore rewrites the whole row)
+    wide = np.nonzero((lb != rl) | (ub != ru))[0][:1500]
+    assert len(wide) == 1500
+    slb[12 * V + wide], sub[12 * V + wide] = lb[wide], ub[wide]
     lo_o, hi_o, res_o = _oracle_batch(pir, slb, sub, V, None, None)
     assert res_o[7].failed and res_o[20].failed and res_o[3].failed
     b = pcx.Batch(pir, n_inst, V)
@@ -892,6 +901,8 @@ def test_drained_runs_leave_the_marks_clear(gpu):
         if cfg:
             pir.set_config(**cfg)
         b.reset()
+        lb_r, ub_r = b.read()                                  # the reset gives the loaded stores back
+        assert np.array_equal(lb_r, slb) and np.array_equal(ub_r, sub), f"reset before {what}"
         res_g, st = b.run(pcx.MODE_BATCHED)
         assert (st.full_evals == 0) == (what != "full first sweep"), what
         lb_g, ub_g = b.read()
