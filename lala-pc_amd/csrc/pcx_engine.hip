@@ -55,6 +55,7 @@ bool valid_base_seed(int v) { return v == -1 || v == 0 || v == 1; }
 // d_seedctl: the difference sub-lists' counts, then the unsatisfied count
 constexpr int kSeedCtlWords = kDiffLists + 1;
 constexpr int32_t kTrailCap = 1024;     // variables a drained instance lists for a restore
+constexpr int32_t kSeedCap = 1024;      // seeds listed per instance for a drain
 
 template <class T>
 void dfree(T*& p) {
@@ -674,19 +675,26 @@ pcx_status plan_seed(pcx_batch* b, bool* seeded, int64_t* n_diff_out, int64_t* n
   if (!b->events || b->seeded_run || !b->run_from_image || !b->diffs_listed || e->cfg.base_seed == 0 ||
       b->table_n == 0)
     return PCX_OK;
-  if (!b->d_unsat) PCX_HIP(hipMalloc(&b->d_unsat, b->table_n * sizeof(int32_t)));
-  int32_t* unsat_n = b->d_seedctl + kDiffLists;
-  PCX_HIP(hipMemsetAsync(unsat_n, 0, sizeof(int32_t), e->stream));
-  PCX_HIP(launch_base_probe(table_of(e), b->d_base, b->d_unsat, unsat_n, dom_of(e), e->stream));
-  int32_t ctl[kDiffLists + 1];
-  pcx_status st = read_small(b, b->d_seedctl, sizeof(ctl), ctl);
-  if (st != PCX_OK) return st;
-  int64_t n_diff = 0;
-  for (int c = 0; c < kDiffLists; ++c) {
-    if (ctl[c] > b->diff_cap) return PCX_OK;          // a sub-list lost entries
-    n_diff += ctl[c];
+  // The probe and the counts depend on the image and the table only: runs
+  // from the same image (after resets) reuse them.
+  if (!b->probe_valid || b->probe_version != e->version) {
+    if (!b->d_unsat) PCX_HIP(hipMalloc(&b->d_unsat, b->table_n * sizeof(int32_t)));
+    int32_t* unsat_n = b->d_seedctl + kDiffLists;
+    PCX_HIP(hipMemsetAsync(unsat_n, 0, sizeof(int32_t), e->stream));
+    PCX_HIP(launch_base_probe(table_of(e), b->d_base, b->d_unsat, unsat_n, dom_of(e), e->stream));
+    int32_t ctl[kDiffLists + 1];
+    pcx_status st = read_small(b, b->d_seedctl, sizeof(ctl), ctl);
+    if (st != PCX_OK) return st;
+    int64_t n = 0;
+    for (int c = 0; c < kDiffLists; ++c)
+      n = (ctl[c] > b->diff_cap || n < 0) ? -1 : n + ctl[c];   // -1: a sub-list lost entries
+    b->probe_diff = n;
+    b->probe_unsat = ctl[kDiffLists];
+    b->probe_version = e->version;
+    b->probe_valid = true;
   }
-  const int64_t n_unsat = ctl[kDiffLists];
+  if (b->probe_diff < 0) return PCX_OK;
+  const int64_t n_diff = b->probe_diff, n_unsat = b->probe_unsat;
   const int64_t per_var = 3 * b->table_n / std::max<int64_t>(b->nvars, 1) + 1;   // mean adjacency
   const int64_t pairs = b->h_off.empty() ? b->n_inst * b->table_n : b->h_off.back();
   if ((n_diff * per_var + n_unsat * b->n_inst) * 16 > pairs) return PCX_OK;
@@ -734,25 +742,32 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     // the first turn advances to sweep 0, whose plane the seeds fill
     if (base_seeded) PCX_HIP(hipMemsetAsync(b->d_qctl + kQSweep, 0xff, sizeof(int32_t), e->stream));
   }
+  // A drain takes its seeds from per-instance lists (those past a list's
+  // cap as plain marks) and keeps a trail; it reads neither the summaries
+  // nor queues.
+  InstLists seeds{nullptr, nullptr, 0};
+  if (drain) {
+    if (!b->d_trail) {
+      PCX_HIP(hipMalloc(&b->d_trail, b->n_inst * (kTrailCap + kSeedCap) * sizeof(int32_t)));
+      PCX_HIP(hipMalloc(&b->d_trail_n, 2 * b->n_inst * sizeof(int32_t)));
+    }
+    seeds = InstLists{b->d_trail + b->n_inst * kTrailCap, b->d_trail_n + b->n_inst, kSeedCap};
+    PCX_HIP(hipMemsetAsync(seeds.n, 0, b->n_inst * sizeof(int32_t), e->stream));
+  }
   if (base_seeded) {
-    // A drain reads neither the summaries nor queues: plain marks.
     EventSet sev = ev;
     if (drain) sev.summary = nullptr;
     PCX_HIP(launch_seed_from_base(sev, DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val}, n_diff,
-                                  b->d_unsat, n_unsat, e->stream));
+                                  b->d_unsat, n_unsat, seeds, e->stream));
   }
   if (drain) {
     PCX_HIP(hipMemsetAsync(b->d_ctl + 3, 0, sizeof(int32_t), e->stream));
     const int dgrid = (int)std::min<int64_t>(b->n_inst, (int64_t)pcx_internal::device_cus(e->cfg.device) * 24);
     pcx_status ts = tm.begin(e->stream);
     if (ts != PCX_OK) return ts;
-    if (!b->d_trail) {
-      PCX_HIP(hipMalloc(&b->d_trail, b->n_inst * kTrailCap * sizeof(int32_t)));
-      PCX_HIP(hipMalloc(&b->d_trail_n, b->n_inst * sizeof(int32_t)));
-    }
     PCX_HIP(launch_drain_marks(table_of(e), b->d_store, b->nvars, b->d_flags, ev, b->d_ctl + 3,
-                               b->d_tight + kTightSlots, Trail{b->d_trail, b->d_trail_n, kTrailCap}, dom_of(e),
-                               dgrid, e->stream));
+                               b->d_tight + kTightSlots, seeds, InstLists{b->d_trail, b->d_trail_n, kTrailCap},
+                               dom_of(e), dgrid, e->stream));
     ts = tm.end(e->stream, false);
     if (ts != PCX_OK) return ts;
     PCX_HIP(hipStreamSynchronize(e->stream));
@@ -1232,6 +1247,7 @@ pcx_status ensure_diff_list(pcx_batch* b) {
 pcx_status diff_sink(pcx_batch* b, DiffSink* ds) {
   *ds = DiffSink{nullptr, nullptr, 0, nullptr, nullptr};
   b->diffs_listed = false;
+  b->probe_valid = false;
   const pcx_engine* e = b->e;
   if (e->cfg.base_seed == 0 || b->n_inst < 2 || b->nvars <= 0)
     return PCX_OK;
@@ -1255,6 +1271,7 @@ pcx_status image_loaded(pcx_batch* b) {
   b->image_in_shadow = false;
   b->image_in_diffs = false;
   b->trail_valid = false;
+  b->probe_valid = false;
   return PCX_OK;
 }
 
@@ -1402,7 +1419,7 @@ pcx_status pcx_batch_reset(pcx_batch* b) {
   if (b->store_is_image) return PCX_OK;
   const DiffSink ds{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val};
   if (b->trail_valid && (b->image_in_diffs || !b->image_in_shadow))
-    PCX_HIP(launch_restore_trail(Trail{b->d_trail, b->d_trail_n, kTrailCap}, ds,
+    PCX_HIP(launch_restore_trail(InstLists{b->d_trail, b->d_trail_n, kTrailCap}, ds,
                                  b->image_in_diffs ? nullptr : b->d_image, b->d_store, b->nvars, b->n_inst,
                                  e->stream));
   else if (b->image_in_diffs)

@@ -80,9 +80,13 @@ struct pcx_batch {
   hipEvent_t load_ev[2] = {nullptr, nullptr};
   int64_t diff_cap = 0;
   int32_t* d_unsat = nullptr;         // [table_n]
-  int32_t* d_trail = nullptr;         // [n_inst * kTrailCap] variables a drained run tightened
-  int32_t* d_trail_n = nullptr;       // [n_inst]
+  int32_t* d_trail = nullptr;         // [n_inst * kTrailCap] variables a drained run tightened,
+                                      // then [n_inst * kSeedCap] the drain's seeds
+  int32_t* d_trail_n = nullptr;       // [2 * n_inst]: the two lists' counts
   bool trail_valid = false;           // the stores are the image but for the trail's variables
+  bool probe_valid = false;           // d_unsat and the counts below hold this image's probe
+  uint64_t probe_version = 0;         // of this table version
+  int64_t probe_diff = 0, probe_unsat = 0;
   int32_t* d_seedctl = nullptr;       // differences per sub-list, then propagators in d_unsat
   bool diffs_listed = false;          // d_diff describes the current image
   // captured group of marked sweeps (batched mode), rebuilt when any launch

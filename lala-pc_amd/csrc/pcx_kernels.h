@@ -239,31 +239,37 @@ hipError_t launch_list_diffs(const int2* store, int64_t nvars, int64_t n_inst, c
 // positions are appended to list (room for t.n), their number to *count.
 hipError_t launch_base_probe(const Table& t, const int2* base, int32_t* list, int32_t* count,
                              int dom, hipStream_t st);
-// Marks of a batch's first sweep (plane 0) seeded from a base store: for
-// every listed difference (k, v) (complete sub-lists) the member
-// propagators of instance k adjacent to v, and every propagator of `unsat`
-// in every instance whose subset holds it.
-hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, int64_t n_diffs,
-                                 const int32_t* unsat, int64_t n_unsat, hipStream_t st);
-// The marked phase of a seeded run (marks in plane 0), one instance per
-// one-wave workgroup taken from *next_inst (zeroed by the caller): every
-// instance to its fixpoint in rounds separated by workgroup barriers, in
-// one launch; flags record the rounds as sweeps.  With a trail, each
-// instance also lists the variables it tightens (n[k] of them; more than
-// cap: the list is incomplete), so that a restore rewrites only those.
-struct Trail {
+// Lists kept per instance: instance k's n[k] entries at list + k * cap
+// (entries past cap are only counted: the list is then incomplete).
+struct InstLists {
   int32_t* list;     // [n_inst * cap], or nullptr
   int32_t* n;        // [n_inst]
   int32_t cap;
 };
+// Marks of a batch's first sweep (plane 0) seeded from a base store: for
+// every listed difference (k, v) (complete sub-lists) the member
+// propagators of instance k adjacent to v, and every propagator of `unsat`
+// in every instance whose subset holds it.  With seed lists (their counts
+// zeroed by the caller; only without summaries and queues) the seeds are
+// listed per instance instead, and only those past a list's cap are marked.
+hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, int64_t n_diffs,
+                                 const int32_t* unsat, int64_t n_unsat, const InstLists& seeds,
+                                 hipStream_t st);
+// The marked phase of a seeded run (marks in plane 0), one instance per
+// one-wave workgroup taken from *next_inst (zeroed by the caller): every
+// instance to its fixpoint in rounds separated by workgroup barriers, in
+// one launch; flags record the rounds as sweeps.  Round 0 takes the seed
+// lists when given (and the row's marks when a list is incomplete).  With a
+// trail, each instance also lists the variables it tightens, so that a
+// restore rewrites only those.
 hipError_t launch_drain_marks(const Table& t, int2* store, int64_t nvars, InstFlags* flags, const EventSet& ev,
-                              int32_t* next_inst, unsigned long long* tight, const Trail& trail, int dom,
-                              int grid, hipStream_t st);
+                              int32_t* next_inst, unsigned long long* tight, const InstLists& seeds,
+                              const InstLists& trail, int dom, int grid, hipStream_t st);
 // The stores back to their image after a drained run with a trail: each
 // instance's listed variables (its whole row when the list is incomplete)
 // from the image, which is either d_image or the base store plus the listed
 // differences (ds.vals non-null: written after the base values).
-hipError_t launch_restore_trail(const Trail& trail, const DiffSink& ds, const int2* image, int2* store,
+hipError_t launch_restore_trail(const InstLists& trail, const DiffSink& ds, const int2* image, int2* store,
                                 int64_t nvars, int64_t n_inst, hipStream_t st);
 // dst (n_inst * nvars words, row by row) from an interleaved copy
 hipError_t launch_deinterleave(const int2* shadow, int2* dst, int64_t nvars, int64_t n_inst, int gi,

@@ -3001,7 +3001,7 @@ hipError_t launch_image_from_diffs(const DiffSink& ds, int2* store, int64_t nvar
 // One workgroup per instance: its trail's variables (or, past the cap, its
 // whole row) rewritten from the image: d_image, or the base store when the
 // listed differences are scattered afterwards.
-__global__ __launch_bounds__(kBlock) void k_restore_trail(Trail tr, const u64* __restrict__ base,
+__global__ __launch_bounds__(kBlock) void k_restore_trail(InstLists tr, const u64* __restrict__ base,
                                                           const u64* __restrict__ image, u64* __restrict__ store,
                                                           int64_t nvars, int64_t n_inst) {
   for (int64_t k = blockIdx.x; k < n_inst; k += gridDim.x) {
@@ -3020,7 +3020,7 @@ __global__ __launch_bounds__(kBlock) void k_restore_trail(Trail tr, const u64* _
   }
 }
 
-hipError_t launch_restore_trail(const Trail& tr, const DiffSink& ds, const int2* image, int2* store,
+hipError_t launch_restore_trail(const InstLists& tr, const DiffSink& ds, const int2* image, int2* store,
                                 int64_t nvars, int64_t n_inst, hipStream_t stream) {
   if (nvars <= 0 || n_inst <= 0) return hipSuccess;
   if (!tr.list || (!image && !ds.vals)) return hipErrorInvalidValue;
@@ -3101,9 +3101,16 @@ hipError_t launch_base_probe(const Table& t, const int2* base, int32_t* list, in
 // its subset holds it; a bit this call sets is also staged for queue plane 0
 // when the run sweeps from queues.
 __device__ __forceinline__ void seed_prop(const Marker& mk, const EventSet& ev, const QueueStage& qs,
-                                          int64_t k, int32_t p) {
+                                          const InstLists& sl, int64_t k, int32_t p) {
   const uint32_t bit = 1u << (p & 31);
   if (mk.member && !(mk.member[p >> 5] & bit)) return;
+  if (sl.list) {                           // listed for the instance; past the cap, marked
+    const int at = atomicAdd(&sl.n[k], 1);
+    if (at < sl.cap) {
+      sl.list[k * sl.cap + at] = p;
+      return;
+    }
+  }
   if (!mk.sum && !ev.q) {                  // bits only: a non-returning atomic
     atomicOr(&mk.dirty[p >> 5], bit);
     return;
@@ -3121,7 +3128,7 @@ __device__ __forceinline__ void seed_prop(const Marker& mk, const EventSet& ev, 
 constexpr int kSeedSlotsPerStep = kBlock / 4;
 __global__ __launch_bounds__(kBlock) void k_seed_from_base(EventSet ev, DiffSink ds,
                                                            const int32_t* __restrict__ unsat,
-                                                           int64_t n_unsat) {
+                                                           int64_t n_unsat, InstLists sl) {
   static_assert(kDiffLists == kBlock, "one thread per difference sub-list");
   __shared__ unsigned long long s_qbuf[kQStage];
   __shared__ int32_t s_qn;
@@ -3152,13 +3159,13 @@ __global__ __launch_bounds__(kBlock) void k_seed_from_base(EventSet ev, DiffSink
         const int32_t v = (int32_t)(uint32_t)d;
         const Marker mk = marker_for(&ev, k, -1);          // sweep -1 marks plane 0 (sweep 0's)
         for (int32_t a = ev.adj_off[v] + (threadIdx.x & 3); a < ev.adj_off[v + 1]; a += 4)
-          seed_prop(mk, ev, qs, k, ev.adj[a]);
+          seed_prop(mk, ev, qs, sl, k, ev.adj[a]);
       }
     } else {
       const int64_t j = (it - diff_steps) * kBlock + threadIdx.x;
       if (j < n_pairs) {
         const int64_t k = j % ev.n_inst;
-        seed_prop(marker_for(&ev, k, -1), ev, qs, k, unsat[j / ev.n_inst]);
+        seed_prop(marker_for(&ev, k, -1), ev, qs, sl, k, unsat[j / ev.n_inst]);
       }
     }
     if (ev.q) {
@@ -3280,7 +3287,8 @@ __device__ __forceinline__ void drain_scan(uint32_t* row, int64_t words, bool fi
 template <int D, int B>
 __global__ __launch_bounds__(B) void k_drain_marks(Table t, int2* __restrict__ store, int64_t nvars,
                                                    InstFlags* flags, EventSet ev, int32_t* next_inst,
-                                                   unsigned long long* tight, Trail trail) {
+                                                   unsigned long long* tight, InstLists seeds,
+                                                   InstLists trail) {
   constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
   __shared__ int32_t s_list[2][kWorkCap];
   __shared__ int32_t s_hash[kDrainHash];
@@ -3311,7 +3319,16 @@ __global__ __launch_bounds__(B) void k_drain_marks(Table t, int2* __restrict__ s
     };
     const uint32_t* mrow = ev.member ? ev.member + k * ev.words : nullptr;
     int2* st = store + k * nvars;
-    if (tid == 0) { s_n[0] = 0; s_over[0] = 1; s_tn = 0; }   // round 0 scans the row
+    // Round 0 takes the seed list (and scans the row when the list is
+    // incomplete: the seeds past its cap were marked there), or scans the row.
+    const int32_t m = seeds.list ? seeds.n[k] : 0;
+    if (tid == 0) { s_n[0] = 0; s_over[0] = seeds.list ? m > seeds.cap : 1; s_tn = 0; }
+    if (seeds.list) {
+      for (int i = tid; i < kDrainHash; i += B) s_hash[i] = -1;
+      __syncthreads();
+      const int32_t* sl = seeds.list + k * seeds.cap;
+      for (int i = tid; i < min(m, seeds.cap); i += B) drain_list(sl[i], row, s_hash, s_list[0], &s_n[0], &s_over[0]);
+    }
     int cur = 0;
     int64_t evals = 0;
     for (int32_t r = 0;; ++r) {
@@ -3411,25 +3428,27 @@ __global__ __launch_bounds__(B) void k_drain_marks(Table t, int2* __restrict__ s
 }
 
 hipError_t launch_drain_marks(const Table& t, int2* store, int64_t nvars, InstFlags* flags, const EventSet& ev,
-                              int32_t* next_inst, unsigned long long* tight, const Trail& trail, int dom,
-                              int grid, hipStream_t stream) {
+                              int32_t* next_inst, unsigned long long* tight, const InstLists& seeds,
+                              const InstLists& trail, int dom, int grid, hipStream_t stream) {
   if (ev.n_inst <= 0 || !ev.dirty) return hipSuccess;
   switch (dom) {
-    case kItv: k_drain_marks<kItv, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, trail); break;
-    case kItvDiv: k_drain_marks<kItvDiv, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, trail); break;
-    case kBits: k_drain_marks<kBits, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, trail); break;
+    case kItv: k_drain_marks<kItv, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, seeds, trail); break;
+    case kItvDiv: k_drain_marks<kItvDiv, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, seeds, trail); break;
+    case kBits: k_drain_marks<kBits, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, seeds, trail); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, int64_t n_diffs,
-                                 const int32_t* unsat, int64_t n_unsat, hipStream_t stream) {
+                                 const int32_t* unsat, int64_t n_unsat, const InstLists& seeds,
+                                 hipStream_t stream) {
   // steps: at most one partial step per sub-list beyond the full ones
   const int64_t steps = n_diffs / kSeedSlotsPerStep + kDiffLists + (n_unsat * ev.n_inst + kBlock - 1) / kBlock;
   if (!ev.dirty || (n_diffs == 0 && n_unsat == 0)) return hipSuccess;
   const int64_t grid = steps > 4096 ? 4096 : steps;
-  k_seed_from_base<<<(unsigned)grid, kBlock, 0, stream>>>(ev, diffs, unsat, n_unsat);
+  if (seeds.list && (ev.q || ev.summary)) return hipErrorInvalidValue;
+  k_seed_from_base<<<(unsigned)grid, kBlock, 0, stream>>>(ev, diffs, unsat, n_unsat, seeds);
   return hipGetLastError();
 }
 
