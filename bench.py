@@ -794,6 +794,29 @@ def all_workloads(args):
     return out
 
 
+COPY_CALIBRATION_BYTES = 640 * 50000 * 8
+
+
+def copy_calibration():
+    """Three launches of the library's plain device copy (a batch reset that
+    copies the reset image: 640 stores of 50,000 variables, 256,000,000
+    bytes read and written) after the workload: the known-size kernel that
+    calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    V, n = 50000, 640
+    pir = pcx.PIR(base_seed=0)
+    lb = np.zeros(V, np.int32)
+    ub = np.full(V, 100, np.int32)
+    pir.tell(np.array([pcx.ADD], np.int32), np.array([0], np.int32), np.array([1], np.int32),
+             np.array([2], np.int32), lb, ub)
+    b = pcx.Batch(pir, n, V)
+    b.load(np.tile(lb, n), np.tile(ub, n))
+    for _ in range(3):
+        b.run(pcx.MODE_BATCHED)
+        b.reset()
+    b.close()
+    log(f"copy calibration: 3 resets of {COPY_CALIBRATION_BYTES:,} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -841,6 +864,8 @@ def main():
     ap.add_argument("--requests", type=int, default=4000, help="serving: requests in the trace")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: Poisson arrival rate (1/s)")
     ap.add_argument("--slots", type=int, default=256, help="serving: requests on the GPU at once")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the workload, three plain device copies of known size (counter calibration)")
     ap.add_argument("--sweeps-per-tick", type=int, default=4, help="serving: sweeps between host checks")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -853,6 +878,8 @@ def main():
         sys.exit(1)
     out = {0: all_workloads, 1: bench_single, 2: bench_batched, 3: bench_bits,
            4: bench_serving}[args.config](args)
+    if args.copy_calibration:
+        copy_calibration()
     line = {"metric": METRIC, "value": out.pop("value"), "unit": out.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": out.pop("ms_per_step"),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",

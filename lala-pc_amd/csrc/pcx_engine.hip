@@ -1275,16 +1275,33 @@ pcx_status image_loaded(pcx_batch* b) {
   return PCX_OK;
 }
 
-// Whether every difference sub-list of the last load is complete (reads the
-// counts; the stream is synchronized).
-pcx_status diffs_complete(pcx_batch* b, bool* complete) {
-  pcx_status st = ensure_pinned(b);
+// After a load listed its differences: the probe of its base store, then
+// one read of the counts (differences per sub-list, unsatisfied
+// propagators; the stream is synchronized).  *complete: no sub-list lost
+// entries.  The probe and the counts are kept for the runs from this image.
+pcx_status probe_loaded(pcx_batch* b, bool* complete) {
+  pcx_engine* e = b->e;
+  pcx_status st = sync_table(e);
+  if (st == PCX_OK) st = ensure_pinned(b);
   if (st != PCX_OK) return st;
-  int32_t cnt[kDiffLists];
-  st = read_small(b, b->d_seedctl, sizeof(cnt), cnt);
+  const bool probe = b->table_n > 0 && (int64_t)e->op.size() == b->table_n;
+  if (probe) {
+    if (!b->d_unsat) PCX_HIP(hipMalloc(&b->d_unsat, b->table_n * sizeof(int32_t)));
+    PCX_HIP(launch_base_probe(table_of(e), b->d_base, b->d_unsat, b->d_seedctl + kDiffLists, dom_of(e),
+                              e->stream));
+  }
+  int32_t ctl[kDiffLists + 1];
+  st = read_small(b, b->d_seedctl, sizeof(ctl), ctl);
   if (st != PCX_OK) return st;
-  *complete = true;
-  for (int c = 0; c < kDiffLists; ++c) *complete = *complete && cnt[c] <= b->diff_cap;
+  int64_t n = 0;
+  for (int c = 0; c < kDiffLists; ++c) n = (ctl[c] > b->diff_cap || n < 0) ? -1 : n + ctl[c];
+  *complete = n >= 0;
+  if (probe) {
+    b->probe_diff = n;                                 // -1: a sub-list lost entries
+    b->probe_unsat = ctl[kDiffLists];
+    b->probe_version = e->version;
+    b->probe_valid = true;
+  }
   return PCX_OK;
 }
 
@@ -1297,7 +1314,7 @@ pcx_status list_load_diffs(pcx_batch* b) {
   if (st != PCX_OK || !ds.list) return st;
   PCX_HIP(launch_list_diffs(b->d_store, b->nvars, b->n_inst, ds, b->e->stream));
   bool complete = false;
-  st = diffs_complete(b, &complete);
+  st = probe_loaded(b, &complete);
   if (st != PCX_OK) return st;
   b->diffs_listed = true;
   b->image_in_diffs = complete;
@@ -1355,7 +1372,7 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
     if (t1) PCX_HIP(hipEventRecord(t1, e->stream));
     t1_recorded = true;                 // (a copy of the image below records it again)
     bool complete = false;
-    pcx_status st = diffs_complete(b, &complete);
+    pcx_status st = probe_loaded(b, &complete);
     if (st != PCX_OK) return st;
     b->shadow_is_image = false;
     b->image_in_shadow = false;
@@ -1404,7 +1421,7 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
     // a load of the other kind whose stores share most values: the next load
     // writes the stores only
     bool complete = false;
-    pcx_status st = diffs_complete(b, &complete);
+    pcx_status st = probe_loaded(b, &complete);
     if (st != PCX_OK) return st;
     b->shared_loads = complete;
   }
