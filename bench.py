@@ -148,8 +148,9 @@ class StatSum:
         dom, kernel, ab = classes[name]
         rf = roofline(*dom, kernel, workload, tight_bytes, algo_bytes=ab)
         rf["kernel_classes_per_step"] = {
-            c: {"kernel": k, "launches": d[3], "kernel_ms": round(d[2], 3), "evals": int(d[0])}
-            for c, (d, k, _) in classes.items()}
+            c: {"kernel": k, "launches": d[3], "kernel_ms": round(d[2], 3), "evals": int(d[0]),
+                "algorithmic_bytes": int(ab if ab is not None else d[0] * BYTES_READ_PER_EVAL + d[1] * tight_bytes)}
+            for c, (d, k, ab) in classes.items()}
         return rf
 
 
@@ -803,7 +804,7 @@ def copy_calibration():
     bytes read and written) after the workload: the known-size kernel that
     calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
     V, n = 50000, 640
-    pir = pcx.PIR(base_seed=0)
+    pir = pcx.PIR(base_seed=0, lockstep=0, event_driven=0)     # no kernel the workloads time
     lb = np.zeros(V, np.int32)
     ub = np.full(V, 100, np.int32)
     pir.tell(np.array([pcx.ADD], np.int32), np.array([0], np.int32), np.array([1], np.int32),

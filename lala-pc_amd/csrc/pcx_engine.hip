@@ -1352,7 +1352,9 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
   if (dst != PCX_OK) return dst;
   const int gi = lockstep_of(b);
   b->image_in_diffs = false;
-  // config.timing: HIP events around the load's kernels
+  const bool rows = ds.list && b->shared_loads;
+  if (rows) PCX_HIP(launch_make_base_rows(d_lb, d_ub, b->nvars, b->n_inst, ds, e->stream));
+  // config.timing: HIP events around the load's pass over the stores
   hipEvent_t t0 = nullptr, t1 = nullptr;
   bool t1_recorded = false;
   if (e->cfg.timing) {
@@ -1362,7 +1364,7 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
     t1 = b->load_ev[1];
     PCX_HIP(hipEventRecord(t0, e->stream));
   }
-  if (ds.list && b->shared_loads) {
+  if (rows) {
     // Stores that share most values (the last load's did): write the stores
     // only.  A complete difference list (base store + listed values) is then
     // the image; otherwise the image is copied from the stores now, as the

@@ -221,9 +221,13 @@ struct DiffSink {
 hipError_t launch_ingest_interleave(const int32_t* lb, const int32_t* ub, int2* store, int2* shadow,
                                     int64_t nvars, int64_t n_inst, int gi, uint8_t* empty,
                                     const DiffSink& ds, hipStream_t st);
+// The base store of device-resident lb / ub rows (rows 0-2) into ds.base,
+// before launch_ingest_rows lists the differences from it.
+hipError_t launch_make_base_rows(const int32_t* lb, const int32_t* ub, int64_t nvars, int64_t n_inst,
+                                 const DiffSink& ds, hipStream_t st);
 // Device-resident lb / ub into the stores only (no image copy), with the
-// base store, the differences from it and their values into ds, and the
-// emptiness; a complete list is the image (launch_image_from_diffs).
+// differences from the base store (made first) and their values into ds,
+// and the emptiness; a complete list is the image (launch_image_from_diffs).
 hipError_t launch_ingest_rows(const int32_t* lb, const int32_t* ub, int2* store, int64_t nvars,
                               int64_t n_inst, uint8_t* empty, const DiffSink& ds, hipStream_t st);
 // store[k][v] = the base value, then every listed difference's value: the

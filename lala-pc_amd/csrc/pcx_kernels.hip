@@ -2944,6 +2944,12 @@ __global__ __launch_bounds__(kBlock) void k_ingest_rows(const int32_t* __restric
   }
 }
 
+hipError_t launch_make_base_rows(const int32_t* lb, const int32_t* ub, int64_t nvars, int64_t n_inst,
+                                 const DiffSink& ds, hipStream_t stream) {
+  if (!ds.list || nvars <= 0 || n_inst <= 0) return hipSuccess;
+  return launch_make_base(lb, ub, nullptr, nvars, n_inst, reinterpret_cast<u64*>(ds.base), stream);
+}
+
 hipError_t launch_ingest_rows(const int32_t* lb, const int32_t* ub, int2* store, int64_t nvars,
                               int64_t n_inst, uint8_t* empty, const DiffSink& ds, hipStream_t stream) {
   if (nvars <= 0 || n_inst <= 0) return hipSuccess;
@@ -2954,10 +2960,6 @@ hipError_t launch_ingest_rows(const int32_t* lb, const int32_t* ub, int2* store,
   int64_t grid = (n / (v4 ? 4 : 1) + kBlock - 1) / kBlock;
   if (grid > (1 << 20)) grid = 1 << 20;
   u64* st = reinterpret_cast<u64*>(store);
-  if (ds.list) {
-    const hipError_t e = launch_make_base(lb, ub, nullptr, nvars, n_inst, reinterpret_cast<u64*>(ds.base), stream);
-    if (e != hipSuccess) return e;
-  }
   if (v4) k_ingest_rows<true><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds);
   else k_ingest_rows<false><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds);
   return hipGetLastError();
