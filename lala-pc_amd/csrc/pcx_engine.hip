@@ -1213,7 +1213,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   if (b->queue_graph) (void)hipGraphExecDestroy(b->queue_graph);
   if (b->plain_queue_graph) (void)hipGraphExecDestroy(b->plain_queue_graph);
   dfree(b->d_q); dfree(b->d_qctl);
-  dfree(b->d_diff); dfree(b->d_diff_val); dfree(b->d_unsat); dfree(b->d_trail); dfree(b->d_trail_n); dfree(b->d_seedctl); dfree(b->d_base);
+  dfree(b->d_diff); dfree(b->d_diff_val); dfree(b->d_unsat); dfree(b->d_trail); dfree(b->d_trail_n); dfree(b->d_bvar); dfree(b->d_seedctl); dfree(b->d_base);
   for (hipEvent_t ev : b->load_ev)
     if (ev) (void)hipEventDestroy(ev);
   delete b;
@@ -1259,12 +1259,14 @@ pcx_status diff_sink(pcx_batch* b, DiffSink* ds) {
 }
 
 // After a load: which instances' images are empty, and the stores are the image.
-pcx_status image_loaded(pcx_batch* b) {
+pcx_status image_loaded(pcx_batch* b, bool empty_known = false) {
   pcx_engine* e = b->e;
   b->diffs_listed = false;
-  PCX_HIP(hipMemsetAsync(b->d_image_empty, 0, b->n_inst, e->stream));
-  PCX_HIP(launch_store_empty_map(b->d_image, b->nvars, (int32_t)b->n_inst, b->d_image_empty, dom_of(e),
-                                 e->stream));
+  if (!empty_known) {
+    PCX_HIP(hipMemsetAsync(b->d_image_empty, 0, b->n_inst, e->stream));
+    PCX_HIP(launch_store_empty_map(b->d_image, b->nvars, (int32_t)b->n_inst, b->d_image_empty, dom_of(e),
+                                   e->stream));
+  }
   PCX_HIP(hipStreamSynchronize(e->stream));
   b->store_is_image = true;
   b->shadow_is_image = false;
@@ -1493,16 +1495,17 @@ pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var, i
   if (parent == child) return fail_with(PCX_EINVAL, "parent and child must differ");
   pcx_engine* e = parent->e;
   PCX_HIP(hipSetDevice(e->cfg.device));
-  int32_t* d_var = nullptr;
-  PCX_HIP(hipMalloc(&d_var, std::max<int64_t>(parent->n_inst, 1) * sizeof(int32_t)));
+  if (!parent->d_bvar)
+    PCX_HIP(hipMalloc(&parent->d_bvar, std::max<int64_t>(parent->n_inst, 1) * sizeof(int32_t)));
+  int32_t* d_var = parent->d_bvar;
   hipError_t he = launch_branch(parent->d_store, parent->nvars, parent->n_inst, d_var, child->d_store,
-                                child->d_image, dom_of(e), e->stream);
+                                child->d_image, child->d_image_empty, dom_of(e), e->stream);
   child->seeded = false;
   if (he == hipSuccess && from_fixpoint && events_possible(child, PCX_MODE_BATCHED)) {
     pcx_status st = sync_table(e);
     if (st == PCX_OK) st = ensure_event_buffers(child);
     if (st == PCX_OK) st = clear_marks(child);
-    if (st != PCX_OK) { (void)hipFree(d_var); return st; }
+    if (st != PCX_OK) return st;
     he = launch_branch_seed(event_set(child), d_var, child->n_inst, e->stream);
     child->marks_clean = false;
     child->seeded = he == hipSuccess;
@@ -1510,10 +1513,9 @@ pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var, i
   if (he == hipSuccess && var)
     he = hipMemcpyAsync(var, d_var, parent->n_inst * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
   if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
-  (void)hipFree(d_var);
   if (he != hipSuccess) return fail_with(PCX_EDEVICE, std::string("branch: ") + hipGetErrorString(he));
   const bool seeded = child->seeded;
-  pcx_status st = image_loaded(child);
+  pcx_status st = image_loaded(child, true);        // the branch wrote the children's emptiness
   child->seeded = seeded;
   return st;
 }

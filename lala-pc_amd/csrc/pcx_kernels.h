@@ -325,9 +325,10 @@ hipError_t launch_batch_ask(const Table& t, const int2* store, int64_t nvars, co
                             int64_t n_props, int64_t n_inst, int64_t* live, int32_t* empty,
                             const int64_t* out_off, int32_t* out_idx, hipStream_t st);
 // Branching: var[k] = split variable of parent instance k (or -1); children
-// 2k / 2k+1 written to child and child_image.
+// 2k / 2k+1 written to child and child_image, and their emptiness (that of
+// their parent) to child_empty when given; one pass over the parents.
 hipError_t launch_branch(const int2* parent, int64_t nvars, int64_t n_inst, int32_t* var, int2* child,
-                         int2* child_image, int dom, hipStream_t st);
+                         int2* child_image, uint8_t* child_empty, int dom, hipStream_t st);
 hipError_t launch_branch_seed(const EventSet& ev, const int32_t* var, int64_t n_child, hipStream_t st);
 hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
                       hipStream_t st);

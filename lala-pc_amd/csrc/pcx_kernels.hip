@@ -2713,68 +2713,92 @@ hipError_t launch_batch_ask(const Table& t, const int2* store, int64_t nvars, co
 }
 
 // ---- branching: two children per instance ---------------------------------
-// var[k] = the first variable (input order) of instance k's store that is not
-// fixed, or -1 when the store is empty or every variable is fixed.
+// One pass per parent k (a workgroup each): children 2k and 2k+1 (stores
+// and image) are written as copies of the parent while the workgroup finds
+// var[k], the first variable (input order) that is not fixed (-1 when the
+// store is empty or every variable is fixed), and whether the store is
+// empty; then that variable is split in both children: intervals
+// [lb, mid] / [mid+1, ub] (mid = lb + (ub - lb) / 2, floor), bitsets {min} /
+// the rest.  The split halves are not empty, so each child is empty exactly
+// when its parent is (child_empty, may be nullptr).
 template <bool BITS>
-__global__ __launch_bounds__(kBlock) void k_branch_pick(const int2* __restrict__ store,
-                                                        int64_t nvars, int64_t n_inst,
-                                                        int32_t* var) {
+__device__ __forceinline__ void branch_scan(int2 b, int32_t v, int32_t& first, bool& empty) {
+  if (BITS) {
+    const u64 wv = ((u64)(uint32_t)b.y << 32) | (uint32_t)b.x;
+    empty |= wv == 0;
+    if (wv & (wv - 1)) first = min(first, v);
+  } else {
+    empty |= b.x > b.y;
+    if (b.x < b.y) first = min(first, v);
+  }
+}
+
+// V2: two variables per 16-byte access (nvars even, rows 16-byte aligned).
+template <bool BITS, bool V2>
+__global__ __launch_bounds__(kBlock) void k_branch(const int2* __restrict__ parent, int64_t nvars,
+                                                   int64_t n_inst, int32_t* __restrict__ var, int2* child,
+                                                   int2* child_image, uint8_t* child_empty) {
   __shared__ int32_t s_first, s_empty;
   for (int64_t k = blockIdx.x; k < n_inst; k += gridDim.x) {
     if (threadIdx.x == 0) { s_first = 0x7fffffff; s_empty = 0; }
     __syncthreads();
-    const int2* st = store + k * nvars;
+    const int2* p = parent + k * nvars;
+    int2* c0 = child + 2 * k * nvars;
+    int2* c1 = c0 + nvars;
+    int2* i0 = child_image + 2 * k * nvars;
+    int2* i1 = i0 + nvars;
     int32_t first = 0x7fffffff;
     bool empty = false;
-    for (int64_t v = threadIdx.x; v < nvars; v += kBlock) {
-      const int2 b = st[v];
-      if (BITS) {
-        const u64 wv = ((u64)(uint32_t)b.y << 32) | (uint32_t)b.x;
-        empty |= wv == 0;
-        if (wv & (wv - 1)) first = min(first, (int32_t)v);
-      } else {
-        empty |= b.x > b.y;
-        if (b.x < b.y) first = min(first, (int32_t)v);
+    if constexpr (V2) {
+      const int4* p4 = reinterpret_cast<const int4*>(p);
+      int4 *c04 = reinterpret_cast<int4*>(c0), *c14 = reinterpret_cast<int4*>(c1);
+      int4 *i04 = reinterpret_cast<int4*>(i0), *i14 = reinterpret_cast<int4*>(i1);
+      for (int64_t h = threadIdx.x; h < nvars / 2; h += kBlock) {
+        const int4 b = p4[h];
+        c04[h] = b;
+        c14[h] = b;
+        i04[h] = b;
+        i14[h] = b;
+        branch_scan<BITS>(make_int2(b.x, b.y), (int32_t)(2 * h), first, empty);
+        branch_scan<BITS>(make_int2(b.z, b.w), (int32_t)(2 * h + 1), first, empty);
+      }
+    } else {
+      for (int64_t v = threadIdx.x; v < nvars; v += kBlock) {
+        const int2 b = p[v];
+        c0[v] = b;
+        c1[v] = b;
+        i0[v] = b;
+        i1[v] = b;
+        branch_scan<BITS>(b, (int32_t)v, first, empty);
       }
     }
     if (first != 0x7fffffff) atomicMin(&s_first, first);
     if (__ballot(empty) && lane_id() == 0) s_empty = 1;
-    __syncthreads();
-    if (threadIdx.x == 0) var[k] = (s_empty || s_first == 0x7fffffff) ? -1 : s_first;
-    __syncthreads();
-  }
-}
-
-// child[2k] and child[2k+1] = parent k with variable var[k] split: intervals
-// [lb, mid] / [mid+1, ub] (mid = lb + (ub - lb) / 2, floor), bitsets {min} /
-// the rest.  var[k] == -1: both children are copies.  Writes the stores and
-// their image.
-template <bool BITS>
-__global__ void k_branch_write(const int2* __restrict__ parent, int64_t nvars, int64_t n_inst,
-                               const int32_t* __restrict__ var, int2* child, int2* child_image) {
-  const int64_t total = n_inst * nvars;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t k = i / nvars, v = i - k * nvars;
-    const int2 b = parent[i];
-    int2 left = b, right = b;
-    if (var[k] == v) {
-      if (BITS) {
-        const u64 wv = ((u64)(uint32_t)b.y << 32) | (uint32_t)b.x;
-        const u64 lo = wv & (~wv + 1), hi = wv & ~lo;
-        left = make_int2((int32_t)(uint32_t)lo, (int32_t)(uint32_t)(lo >> 32));
-        right = make_int2((int32_t)(uint32_t)hi, (int32_t)(uint32_t)(hi >> 32));
-      } else {
-        const int64_t mid = (int64_t)b.x + (((int64_t)b.y - (int64_t)b.x) >> 1);
-        left.y = (int32_t)mid;
-        right.x = (int32_t)(mid + 1);
+    __syncthreads();                             // the copies are written: the split overwrites them
+    if (threadIdx.x == 0) {
+      const int32_t sv = (s_empty || s_first == 0x7fffffff) ? -1 : s_first;
+      var[k] = sv;
+      if (child_empty) child_empty[2 * k] = child_empty[2 * k + 1] = (uint8_t)s_empty;
+      if (sv >= 0) {
+        const int2 b = p[sv];
+        int2 left = b, right = b;
+        if (BITS) {
+          const u64 wv = ((u64)(uint32_t)b.y << 32) | (uint32_t)b.x;
+          const u64 lo = wv & (~wv + 1), hi = wv & ~lo;
+          left = make_int2((int32_t)(uint32_t)lo, (int32_t)(uint32_t)(lo >> 32));
+          right = make_int2((int32_t)(uint32_t)hi, (int32_t)(uint32_t)(hi >> 32));
+        } else {
+          const int64_t mid = (int64_t)b.x + (((int64_t)b.y - (int64_t)b.x) >> 1);
+          left.y = (int32_t)mid;
+          right.x = (int32_t)(mid + 1);
+        }
+        c0[sv] = left;
+        c1[sv] = right;
+        i0[sv] = left;
+        i1[sv] = right;
       }
     }
-    const int64_t c0 = (2 * k) * nvars + v, c1 = c0 + nvars;
-    child[c0] = left;
-    child[c1] = right;
-    child_image[c0] = left;
-    child_image[c1] = right;
+    __syncthreads();                             // s_first / s_empty read before the next parent
   }
 }
 
@@ -3455,18 +3479,18 @@ hipError_t launch_seed_from_base(const EventSet& ev, const DiffSink& diffs, int6
 }
 
 hipError_t launch_branch(const int2* parent, int64_t nvars, int64_t n_inst, int32_t* var, int2* child,
-                         int2* child_image, int dom, hipStream_t stream) {
+                         int2* child_image, uint8_t* child_empty, int dom, hipStream_t stream) {
   if (n_inst <= 0) return hipSuccess;
-  const int64_t g1 = n_inst < 65536 ? n_inst : 65536;
-  const int64_t total = n_inst * nvars;
-  int64_t g2 = (total + 255) / 256;
-  if (g2 > 65536) g2 = 65536;
+  const int64_t grid = n_inst < 65536 ? n_inst : 65536;
+  const bool v2 = nvars % 2 == 0 && ((reinterpret_cast<uintptr_t>(parent) | reinterpret_cast<uintptr_t>(child) |
+                                      reinterpret_cast<uintptr_t>(child_image)) & 15) == 0;
+  const unsigned g = (unsigned)grid;
   if (dom == kBits) {
-    k_branch_pick<true><<<(unsigned)g1, kBlock, 0, stream>>>(parent, nvars, n_inst, var);
-    k_branch_write<true><<<(unsigned)g2, 256, 0, stream>>>(parent, nvars, n_inst, var, child, child_image);
+    if (v2) k_branch<true, true><<<g, kBlock, 0, stream>>>(parent, nvars, n_inst, var, child, child_image, child_empty);
+    else k_branch<true, false><<<g, kBlock, 0, stream>>>(parent, nvars, n_inst, var, child, child_image, child_empty);
   } else {
-    k_branch_pick<false><<<(unsigned)g1, kBlock, 0, stream>>>(parent, nvars, n_inst, var);
-    k_branch_write<false><<<(unsigned)g2, 256, 0, stream>>>(parent, nvars, n_inst, var, child, child_image);
+    if (v2) k_branch<false, true><<<g, kBlock, 0, stream>>>(parent, nvars, n_inst, var, child, child_image, child_empty);
+    else k_branch<false, false><<<g, kBlock, 0, stream>>>(parent, nvars, n_inst, var, child, child_image, child_empty);
   }
   return hipGetLastError();
 }

@@ -213,8 +213,9 @@ def test_bits_persistent_lds(gpu, lds):
     _compare(b.read_words(), rg, wo, ro, V, f"bits persistent lds {lds}")
 
 
-def test_bits_branching(gpu):
-    V, P, n = 1000, 4000, 16
+@pytest.mark.parametrize("V", [1000, 999], ids=["even", "odd"])
+def test_bits_branching(gpu, V):
+    P, n = 4000, 16
     op, x, y, z, w = pcx.gen_bits_network(81, V, P)
     pir = pcx.BitsPIR()
     pir.tell(op, x, y, z, w)

@@ -658,18 +658,22 @@ def _host_children(lo, hi, n_inst, V):
 
 @pytest.mark.parametrize("mode", [pcx.MODE_BATCHED, pcx.MODE_PLAIN])
 @pytest.mark.parametrize("incremental", [False, True], ids=["full", "from-fixpoint"])
-def test_branching_on_device(gpu, incremental, mode):
+@pytest.mark.parametrize("V", [3000, 2999], ids=["even", "odd"])
+def test_branching_on_device(gpu, incremental, mode, V):
     """pcx_batch_branch splits each propagated node on its first unfixed
     variable; the children equal the host restatement and propagate to the
     oracle's fixpoints, also when their run starts from the split variable's
-    propagators only (the parents are at their fixpoints)."""
-    V, P, n_inst = 3000, 12000, 24
+    propagators only (the parents are at their fixpoints).  Stores of an
+    even number of variables are copied two variables per access.  Parent 3
+    is empty: so are both its children."""
+    P, n_inst = 12000, 24
     op, x, y, z, lb, ub, w = _network(71, V, P)
     pir = pcx.PIR()
     pir.tell(op, x, y, z, lb, ub)
     root_lb, root_ub = lb.copy(), ub.copy()
     assert not pir.fixpoint(root_lb, root_ub).failed
     slb, sub = pcx.gen_splits(72, root_lb, root_ub, n_inst, 1, 6)
+    slb[3 * V + 5], sub[3 * V + 5] = 4, 3
     off, idx = pcx.gen_subsets(73, P, n_inst, 1000, P)
     parent = pcx.Batch(pir, n_inst, V, off, idx)
     parent.load(slb, sub)
@@ -693,6 +697,7 @@ def test_branching_on_device(gpu, incremental, mode):
     lo_o, hi_o = clo.copy(), chi.copy()
     res_o = ol.fixpoint_batch(top, tx, ty, tz, lo_o, hi_o, V, idx=cidx.astype(np.int64), off=coff, threads=8)
     _compare(lb_g, ub_g, res, lo_o, hi_o, res_o, V, "children")
+    assert res[6].failed and res[7].failed
     child.reset()                             # the children are the child batch's image
     glo2, ghi2 = child.read()
     assert (glo2 == clo).all() and (ghi2 == chi).all()
