@@ -662,8 +662,10 @@ def bench_serving(args):
     tr = make_trace(args.seed, args.requests, rate=args.rate, embed=embed)
     srv = pcx.Server(max_slots=args.slots, sweeps_per_tick=args.sweeps_per_tick,
                      props_per_wave=args.ppw)
-    for p in pirs:
+    for j, p in enumerate(pirs):
         srv.add_network(p)
+        if args.serve_root:               # requests inside the told store start from the root fixpoint
+            srv.set_root(j, tr.nets[j][4], tr.nets[j][5])
     n = len(tr)
     sizes = np.array([tr.nprops(i) for i in range(n)])
     log(f"setup {time.time() - t0:.1f}s: {n} requests over {tr.arrival_s[-1]:.2f}s, "
@@ -696,8 +698,11 @@ def bench_serving(args):
         "value": n * args.steps / elapsed, "unit": "fixpoints/s", "ms_per_step": elapsed / args.steps * 1e3,
         "config": {"workload": f"serving stream: Poisson arrivals at {args.rate:.0f} requests/s, "
                                f"{n} requests over {len(pirs)} seeded networks of 1k-1M propagators "
-                               "(log-uniform, vars = propagators/4), 1-10 splits each",
-                   "requests": n, "rate": args.rate, "slots": args.slots,
+                               "(log-uniform, vars = propagators/4), 1-10 splits each of the network's told "
+                               "store" + ("; each network's told store registered with the server "
+                                          "(pcx_server_set_root), so a request starts from its meet "
+                                          "with the root fixpoint" if args.serve_root else ""),
+                   "requests": n, "rate": args.rate, "slots": args.slots, "roots": bool(args.serve_root),
                    "sweeps_per_tick": args.sweeps_per_tick, "props_per_wave": args.ppw},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},
@@ -867,6 +872,8 @@ def main():
     ap.add_argument("--slots", type=int, default=256, help="serving: requests on the GPU at once")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the workload, three plain device copies of known size (counter calibration)")
+    ap.add_argument("--serve-root", type=int, default=1, choices=[0, 1],
+                    help="serving: register each network's told store (requests start from the root fixpoint)")
     ap.add_argument("--sweeps-per-tick", type=int, default=4, help="serving: sweeps between host checks")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -392,6 +392,17 @@ void       pcx_server_destroy(pcx_server* s);
 /* Register an engine's table as network *net.  The engine must outlive the
  * server and tell nothing more while registered. */
 pcx_status pcx_server_add_network(pcx_server* s, pcx_engine* e, int32_t* net);
+/* Give network `net` its told store (lb/ub, nvars entries, host memory, read
+ * during the call): the server propagates it once (the root fixpoint) and
+ * starts each later request of the network whose store lies inside the told
+ * store from the request's meet with the root fixpoint.  The result is the
+ * same -- every fixpoint below such a store is below the root fixpoint -- and
+ * a request then needs a few sweeps instead of its whole propagation.  A
+ * request outside the told store starts from its own store.  Calling it
+ * again replaces the root.  No counterpart in the reference (which has no
+ * server); search-tree serving, where requests are subproblems of a root. */
+pcx_status pcx_server_set_root(pcx_server* s, int32_t net, const int32_t* lb, const int32_t* ub,
+                               int64_t nvars);
 /* Queue a request.  lb/ub (nvars entries, host memory) stay the caller's
  * until the request completes; they then hold its fixpoint. */
 pcx_status pcx_server_submit(pcx_server* s, int32_t net, int32_t* lb, int32_t* ub,

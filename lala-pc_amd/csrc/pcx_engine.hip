@@ -221,7 +221,7 @@ pcx_status read_small(pcx_batch* b, const void* d, size_t bytes, void* out) {
 }
 
 pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* results,
-                   pcx_stats* stats, const SweepTimer& tm) {
+                   pcx_stats* stats, SweepTimer& tm) {
   pcx_engine* e = b->e;
   flags.resize(b->n_inst);
   const size_t fbytes = b->n_inst * sizeof(InstFlags);
@@ -230,6 +230,8 @@ pcx_status collect(pcx_batch* b, std::vector<InstFlags>& flags, pcx_result* resu
   PCX_HIP(hipMemcpyAsync(b->h_pin + kPinScratch + fbytes, b->d_tight, sizeof(tight),
                          hipMemcpyDeviceToHost, e->stream));
   PCX_HIP(hipStreamSynchronize(e->stream));
+  pcx_status ts = tm.drain();                    // launches still timed (the stream is idle now)
+  if (ts != PCX_OK) return ts;
   std::memcpy(flags.data(), b->h_pin + kPinScratch, fbytes);
   std::memcpy(tight, b->h_pin + kPinScratch + fbytes, sizeof(tight));
   int64_t evals = 0, smax = 0, full_evals = 0;
@@ -770,11 +772,8 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
                                dom_of(e), dgrid, e->stream));
     ts = tm.end(e->stream, false);
     if (ts != PCX_OK) return ts;
-    PCX_HIP(hipStreamSynchronize(e->stream));
-    ts = tm.drain();
-    if (ts != PCX_OK) return ts;
     std::vector<InstFlags> flags;
-    ts = collect(b, flags, results, stats, tm);
+    ts = collect(b, flags, results, stats, tm);        // one synchronization for the drain and the flags
     // the drain took and cleared every mark (rows of failed instances too),
     // and it started from the image: the stores differ from it only in the
     // trail's variables

@@ -92,6 +92,34 @@ struct Slot {
 
 constexpr int32_t kFreeFailed = -2;   // failed_at marker of a free slot
 
+// A network's told store and root fixpoint (pcx_server_set_root).
+struct NetRoot {
+  int2* told = nullptr;
+  int2* root = nullptr;
+  int64_t nvars = 0;
+};
+
+// out[0] = 1 when the store leaves the told store somewhere.
+__global__ __launch_bounds__(256) void k_root_within(const int2* __restrict__ st, const int2* __restrict__ told,
+                                                     int64_t nvars, int32_t* out) {
+  bool outside = false;
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nvars; v += (int64_t)gridDim.x * 256) {
+    const int2 a = st[v], t = told[v];
+    outside |= a.x < t.x || a.y > t.y;
+  }
+  if (__ballot(outside) && (threadIdx.x & 63) == 0) *out = 1;
+}
+
+// Unless out[0] is set: the store meets the root fixpoint.
+__global__ __launch_bounds__(256) void k_root_meet(int2* __restrict__ st, const int2* __restrict__ root,
+                                                   int64_t nvars, const int32_t* out) {
+  if (*out) return;
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nvars; v += (int64_t)gridDim.x * 256) {
+    const int2 a = st[v], r = root[v];
+    st[v] = make_int2(max(a.x, r.x), min(a.y, r.y));
+  }
+}
+
 // Side lanes: concurrent burst launches, each on a stream of its own (with
 // the slot batch's stream, four streams: the hardware queues of a process).
 constexpr int kSideLanes = 3;
@@ -132,6 +160,8 @@ struct pcx_server {
   // Side lanes: bursts of long-running slots beside the slot batch.
   SideLane lanes[kSideLanes];
   hipEvent_t handed = nullptr;        // slot batch: hand-over flags uploaded
+  std::vector<NetRoot> roots;         // per network (pcx_server_set_root)
+  int32_t* d_root_out = nullptr;      // per slot: its store left the told store
 };
 
 namespace {
@@ -208,6 +238,15 @@ pcx_status admit(pcx_server* s) {
     for (int64_t v = 0; v < rq.nvars; ++v) stage[v] = make_int2(rq.lb[v], rq.ub[v]);
     PCX_HIP(hipMemcpyAsync(s->d_arena + off, stage, rq.nvars * sizeof(int2), hipMemcpyHostToDevice, s->stream));
     staged += rq.nvars;
+    if (rq.net < (int32_t)s->roots.size() && s->roots[rq.net].root && s->roots[rq.net].nvars == rq.nvars) {
+      // inside the told store: start from the meet with the root fixpoint
+      const NetRoot& nr = s->roots[rq.net];
+      const unsigned g = (unsigned)std::min<int64_t>((rq.nvars + 255) / 256, 2048);
+      PCX_HIP(hipMemsetAsync(s->d_root_out + k, 0, sizeof(int32_t), s->stream));
+      k_root_within<<<g, 256, 0, s->stream>>>(s->d_arena + off, nr.told, rq.nvars, s->d_root_out + k);
+      k_root_meet<<<g, 256, 0, s->stream>>>(s->d_arena + off, nr.root, rq.nvars, s->d_root_out + k);
+      PCX_HIP(hipGetLastError());
+    }
     pcx_engine* e = s->nets[rq.net];
     const int32_t S = s->next_sweep;
     SlotDesc sd{};
@@ -294,9 +333,17 @@ pcx_status complete(pcx_server* s, const std::vector<int32_t>& ks, const InstFla
   for (size_t i = 0; i < ks.size(); ++i) {
     const int32_t k = ks[i];
     Slot& sl = s->slots[k];
-    const pcx_result r = result_of(s, k, fl[k]);
+    pcx_result r = result_of(s, k, fl[k]);
     const int2* stage = s->h_stage + at[i];
-    for (int64_t v = 0; v < sl.req.nvars; ++v) { sl.req.lb[v] = stage[v].x; sl.req.ub[v] = stage[v].y; }
+    bool moved = false;                          // the fixpoint differs from the request's store
+    for (int64_t v = 0; v < sl.req.nvars; ++v) {
+      moved |= sl.req.lb[v] != stage[v].x || sl.req.ub[v] != stage[v].y;
+      sl.req.lb[v] = stage[v].x;
+      sl.req.ub[v] = stage[v].y;
+    }
+    // (a request started from its meet with a root fixpoint may change
+    // nothing on the device and still differ from its own store)
+    if (!r.failed) r.has_changed = moved;
     finish(s, sl.req, r);
     s->arena.release(sl.off, sl.units);
     sl.busy = false;
@@ -568,6 +615,11 @@ void pcx_server_destroy(pcx_server* s) {
     if (ln.stream) (void)hipStreamDestroy(ln.stream);
   }
   if (s->handed) (void)hipEventDestroy(s->handed);
+  for (NetRoot& nr : s->roots) {
+    if (nr.told) (void)hipFree(nr.told);
+    if (nr.root) (void)hipFree(nr.root);
+  }
+  if (s->d_root_out) (void)hipFree(s->d_root_out);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -656,6 +708,39 @@ pcx_status pcx_server_add_network(pcx_server* s, pcx_engine* e, int32_t* net) {
     if (st != PCX_OK) { s->nets.pop_back(); return st; }
   }
   *net = (int32_t)s->nets.size() - 1;
+  return PCX_OK;
+}
+
+pcx_status pcx_server_set_root(pcx_server* s, int32_t net, const int32_t* lb, const int32_t* ub,
+                               int64_t nvars) {
+  if (!s || !lb || !ub) return fail_with(PCX_EINVAL, "NULL argument");
+  if (net < 0 || net >= (int32_t)s->nets.size()) return fail_with(PCX_EINVAL, "unknown network");
+  pcx_engine* e = s->nets[net];
+  if (e->dirty) return fail_with(PCX_EINVAL, "network told more propagators after registration");
+  if (nvars <= e->max_var) return fail_with(PCX_EINVAL, "store smaller than the network's variables");
+  PCX_HIP(hipSetDevice(s->cfg.device));
+  // the root fixpoint, by the engine's own fixpoint call
+  std::vector<int32_t> rl(lb, lb + nvars), ru(ub, ub + nvars);
+  pcx_result r{};
+  pcx_status st = pcx_fixpoint(e, rl.data(), ru.data(), nvars, &r);
+  if (st != PCX_OK) return st;
+  std::vector<int2> told(nvars), root(nvars);
+  for (int64_t v = 0; v < nvars; ++v) {
+    told[v] = make_int2(lb[v], ub[v]);
+    root[v] = make_int2(rl[v], ru[v]);
+  }
+  if (!s->d_root_out) PCX_HIP(hipMalloc(&s->d_root_out, s->n_slots * sizeof(int32_t)));
+  if ((int32_t)s->roots.size() <= net) s->roots.resize(net + 1);
+  NetRoot& nr = s->roots[net];
+  PCX_HIP(hipStreamSynchronize(s->stream));        // no admission still reads the old root
+  if (nr.told) (void)hipFree(nr.told);
+  if (nr.root) (void)hipFree(nr.root);
+  nr = NetRoot{};
+  PCX_HIP(hipMalloc(&nr.told, nvars * sizeof(int2)));
+  PCX_HIP(hipMalloc(&nr.root, nvars * sizeof(int2)));
+  PCX_HIP(hipMemcpy(nr.told, told.data(), nvars * sizeof(int2), hipMemcpyHostToDevice));
+  PCX_HIP(hipMemcpy(nr.root, root.data(), nvars * sizeof(int2), hipMemcpyHostToDevice));
+  nr.nvars = nvars;
   return PCX_OK;
 }
 

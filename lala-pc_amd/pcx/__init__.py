@@ -163,6 +163,7 @@ SIGNATURES = [
     ("pcx_server_create", ctypes.c_int32, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_VP)]),
     ("pcx_server_destroy", None, [_VP]),
     ("pcx_server_add_network", ctypes.c_int32, [_VP, _VP, _P32]),
+    ("pcx_server_set_root", ctypes.c_int32, [_VP, ctypes.c_int32, _P32, _P32, ctypes.c_int64]),
     ("pcx_server_submit", ctypes.c_int32, [_VP, ctypes.c_int32, _P32, _P32, ctypes.c_int64,
                                            ctypes.c_int64]),
     ("pcx_server_step", ctypes.c_int32, [_VP, _P32]),
@@ -538,6 +539,12 @@ class Server:
         _check(lib().pcx_server_add_network(self._h, pir._h, ctypes.byref(n)))
         self._nets.append(pir)          # keeps the engine alive while registered
         return n.value
+
+    def set_root(self, net, lb, ub):
+        """The network's told store: later requests inside it start from
+        their meet with its root fixpoint (pcx_server_set_root)."""
+        nv = _store_pair(lb, ub)
+        _check(lib().pcx_server_set_root(self._h, int(net), _p32(lb, nv), _p32(ub, nv), nv))
 
     def submit(self, net, lb, ub, tag):
         nv = _store_pair(lb, ub)

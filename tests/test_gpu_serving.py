@@ -48,16 +48,24 @@ def small(gpu):
     return tr, pirs, alone
 
 
-def _serve(tr, pirs, **cfg):
+def _serve(tr, pirs, roots=False, extra=(), **cfg):
+    """Serve every request of the trace (and `extra` (net, lb, ub) requests,
+    tagged after them); roots: give each network its told store."""
     srv = pcx.Server(**cfg)
     ids = [srv.add_network(p) for p in pirs]
     assert ids == list(range(len(pirs)))
+    if roots:
+        for j, net in enumerate(tr.nets):
+            srv.set_root(j, net[4], net[5])
     stores = [tuple(a.copy() for a in tr.store(i)) for i in range(len(tr))]
+    for j, lo, hi in extra:
+        stores.append((lo.copy(), hi.copy()))
+    nets = list(tr.net) + [j for j, _, _ in extra]
     for i, (lo, hi) in enumerate(stores):
-        srv.submit(int(tr.net[i]), lo, hi, tag=i)
+        srv.submit(int(nets[i]), lo, hi, tag=i)
     got = {}
     steps = 0
-    while len(got) < len(tr):
+    while len(got) < len(stores):
         srv.step()
         for tag, r, lat in srv.poll():
             assert tag not in got
@@ -109,6 +117,37 @@ def test_mix_equals_alone(small, events):
     for i in range(len(tr)):
         lo, hi = stores[i]
         assert _same(lo, hi, got[i], *alone[i]), f"request {i} differs in the mix"
+
+
+@pytest.mark.parametrize("events", [1, 0, -1], ids=["marked", "full", "auto"])
+def test_requests_start_from_the_root_fixpoint(small, events):
+    """With each network's told store given (pcx_server_set_root), a request
+    inside it starts from its meet with the root fixpoint: the results are
+    the same as alone (stores, failure, has_changed), in fewer sweeps.
+    Requests that leave the told store (a bound widened by one) start from
+    their own store and match too."""
+    tr, pirs, alone = small
+    extra, extra_alone = [], []
+    for i in range(0, len(tr), 9):
+        j = int(tr.net[i])
+        lo, hi = (a.copy() for a in tr.store(i))
+        v = int(np.nonzero(lo == tr.nets[j][4])[0][0])
+        lo[v] -= 1                                        # below the told lower bound
+        extra.append((j, lo, hi))
+        a_lo, a_hi = lo.copy(), hi.copy()
+        extra_alone.append((a_lo, a_hi, pirs[j].fixpoint(a_lo, a_hi)))
+    stores, got, st = _serve(tr, pirs, roots=True, extra=extra, max_slots=64, sweeps_per_tick=4,
+                             event_driven=events)
+    assert st.completed == len(tr) + len(extra)
+    sweeps_root = 0
+    for i in range(len(tr)):
+        lo, hi = stores[i]
+        assert _same(lo, hi, got[i], *alone[i]), f"request {i} differs when started from the root"
+        sweeps_root += int(got[i].sweeps)
+    for e, (a_lo, a_hi, a_r) in enumerate(extra_alone):
+        lo, hi = stores[len(tr) + e]
+        assert _same(lo, hi, got[len(tr) + e], a_lo, a_hi, a_r), f"request outside the told store {e}"
+    assert sweeps_root < sum(int(alone[i][2].sweeps) for i in range(len(tr)))
 
 
 @pytest.mark.parametrize("events", [1, 0], ids=["marked", "full"])
