@@ -727,11 +727,12 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
   int32_t live = 1, sweeps = 0;
   const int32_t maxs = (int32_t)std::min<int64_t>(e->cfg.max_sweeps, 1 << 30);
   EventSet ev = event_set(b);
-  // A run seeded from a base store drains each instance's marks in one
-  // launch (one workgroup per instance) when it runs to the fixpoint; with a
-  // sweep limit it sweeps from queue planes.  Queued marked sweeps also
-  // follow a lockstep full sweep (seeds of a branching keep the bitmaps).
-  const bool drain = base_seeded && e->cfg.max_sweeps <= 0;
+  // A seeded run (from a base store, or the split variables of a branching)
+  // drains each instance's marks in one launch (one workgroup per instance)
+  // when it runs to the fixpoint; with a sweep limit a base-seeded run sweeps
+  // from queue planes, a branch-seeded one from the bitmaps.  Queued marked
+  // sweeps also follow a lockstep full sweep.
+  const bool drain = b->seeded_run && e->cfg.max_sweeps <= 0;
   const bool qmode = b->events && e->cfg.mark_queue != 0 &&
                      ((base_seeded && !drain) || (gi > 0 && !b->seeded_run));
   if (qmode) {
@@ -753,8 +754,10 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
       PCX_HIP(hipMalloc(&b->d_trail, b->n_inst * (kTrailCap + kSeedCap) * sizeof(int32_t)));
       PCX_HIP(hipMalloc(&b->d_trail_n, 2 * b->n_inst * sizeof(int32_t)));
     }
-    seeds = InstLists{b->d_trail + b->n_inst * kTrailCap, b->d_trail_n + b->n_inst, kSeedCap};
-    PCX_HIP(hipMemsetAsync(seeds.n, 0, b->n_inst * sizeof(int32_t), e->stream));
+    if (base_seeded) {                   // a branching's seeds are bits in the rows
+      seeds = InstLists{b->d_trail + b->n_inst * kTrailCap, b->d_trail_n + b->n_inst, kSeedCap};
+      PCX_HIP(hipMemsetAsync(seeds.n, 0, b->n_inst * sizeof(int32_t), e->stream));
+    }
   }
   if (base_seeded) {
     EventSet sev = ev;
@@ -774,11 +777,11 @@ pcx_status run_batched(pcx_batch* b, pcx_result* results, pcx_stats* stats) {
     if (ts != PCX_OK) return ts;
     std::vector<InstFlags> flags;
     ts = collect(b, flags, results, stats, tm);        // one synchronization for the drain and the flags
-    // the drain took and cleared every mark (rows of failed instances too),
-    // and it started from the image: the stores differ from it only in the
-    // trail's variables
+    // the drain took and cleared every mark (rows of failed instances too;
+    // a branching's seeds also set summary bits, which stay), and it started
+    // from the image: the stores differ from it only in the trail's variables
     if (ts == PCX_OK) {
-      b->marks_clean = true;
+      if (base_seeded) b->marks_clean = true;
       b->trail_valid = b->run_from_image;
     }
     return ts;
@@ -1429,6 +1432,7 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
     b->shared_loads = complete;
   }
   b->store_is_image = true;
+  b->trail_valid = false;                // the trail described the last image
   b->diffs_listed = ds.list != nullptr;
   return PCX_OK;
 }
