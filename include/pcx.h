@@ -229,11 +229,15 @@ void       pcx_batch_destroy(pcx_batch* b);
 pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub);
 /* The same from DEVICE memory: d_lb / d_ub hold n_inst*nvars bounds in the
  * same order (e.g. stores a caller's kernel produced, or inputs staged in HBM
- * ahead of time); no host transfer.  One pass writes the stores, the reset
- * image and the per-instance emptiness.  The source may be reused or freed
- * once the call returns.  Interval batches. */
+ * ahead of time); no host transfer.  One pass writes the stores and the
+ * per-instance emptiness, and keeps the reset image (as a copy, or -- stores
+ * that share most values -- as a base store and the values that differ from
+ * it).  The source may be reused or freed once the call returns.  Interval
+ * batches. */
 pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_t* d_ub);
-/* Restore every store from the reset image (device to device). */
+/* Restore every store from the reset image (device to device).  After a
+ * run that drained its marks from the image, only the variables that run
+ * tightened are rewritten; stores that are the image are left as they are. */
 pcx_status pcx_batch_reset(pcx_batch* b);
 pcx_status pcx_batch_read(pcx_batch* b, int32_t* lb, int32_t* ub);
 
