@@ -221,6 +221,40 @@ def test_empty_store_and_errors(gpu):
     assert e.value.status == pcx.EINVAL
     with pytest.raises(pcx.PcxError):
         srv.submit(5, lb.copy(), ub.copy(), tag=11)                         # unknown network
+    with pytest.raises(pcx.PcxError) as e:
+        srv.set_root(5, lb, ub)                                             # unknown network
+    assert e.value.status == pcx.EINVAL
+    with pytest.raises(pcx.PcxError) as e:
+        srv.set_root(n, lb[:2].copy(), ub[:2].copy())                       # too few variables
+    assert e.value.status == pcx.EINVAL
+
+
+def test_root_without_a_fixpoint(gpu):
+    """A told store whose propagation fails: its root fixpoint is empty, so
+    every request inside it fails (as it does alone); a request outside it
+    (y widened past the told bound) runs from its own store and does not."""
+    op = np.array([pcx.ADD], np.int32)
+    x, y, z = (np.array([v], np.int32) for v in (0, 1, 2))
+    lb = np.array([20, 0, 0], np.int32)
+    ub = np.array([30, 3, 3], np.int32)                 # x = y + z <= 6 < 20: no fixpoint
+    p = pcx.PIR()
+    p.tell(op, x, y, z, lb, ub)
+    srv = pcx.Server(max_slots=4)
+    n = srv.add_network(p)
+    srv.set_root(n, lb, ub)
+    inside = (np.array([20, 1, 1], np.int32), np.array([25, 3, 3], np.int32))
+    outside = (np.array([20, 0, 0], np.int32), np.array([30, 30, 3], np.int32))
+    for tag, (lo, hi) in enumerate([inside, outside]):
+        srv.submit(n, lo, hi, tag=tag)
+    done = {}
+    while len(done) < 2:
+        srv.step()
+        done.update({t: r for t, r, _ in srv.poll()})
+    assert done[0].failed
+    assert not done[1].failed and list(outside[0]) == [20, 17, 0] and list(outside[1]) == [30, 30, 3]
+    a_lo, a_hi = (v.copy() for v in (np.array([20, 0, 0], np.int32), np.array([30, 30, 3], np.int32)))
+    r = p.fixpoint(a_lo, a_hi)
+    assert not r.failed and np.array_equal(a_lo, outside[0]) and np.array_equal(a_hi, outside[1])
 
 
 def test_auto_marks_large_requests_only(gpu):
