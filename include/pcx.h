@@ -235,6 +235,16 @@ pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub);
  * it).  The source may be reused or freed once the call returns.  Interval
  * batches. */
 pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_t* d_ub);
+/* PC's is_extractable on every store of the batch (pc.hpp:727-738: the
+ * store is not empty and every formula is asked), for formula trees in the
+ * postorder node form of pcx_model.h (nodes: 4 * n_nodes int32, off:
+ * n_formulas + 1 offsets; pcx_model_formulas gives a model's trees).  One
+ * GPU thread per (store, formula) interprets its tree.  extractable[k] is
+ * 1 or 0.  Errors: PCX_EINVAL for a malformed tree (unknown kind, a variable
+ * outside the store, a stack deeper than 32 or left with more than one
+ * value).  Interval batches. */
+pcx_status pcx_batch_ask_formulas(pcx_batch* b, const int32_t* nodes, int64_t n_nodes, const int64_t* off,
+                                  int64_t n_formulas, int32_t* extractable);
 /* Restore every store from the reset image (device to device).  After a
  * run that drained its marks from the image, only the variables that run
  * tightened are rewritten; stores that are the image are left as they are. */

@@ -69,6 +69,32 @@ int32_t    pcx_model_var(const pcx_model* m, const char* name);
 /* Name of variable v (auxiliary: __AUX_k, constants: __CONSTANT_c), or NULL. */
 const char* pcx_model_var_name(const pcx_model* m, int32_t v);
 
+/* PC's formula trees.  The reference's PC keeps each constraint as a tree
+ * of formulas over terms (include/lala/formula.hpp, terms.hpp) and its
+ * is_extractable asks every tree (pc.hpp:727-738).  The model keeps each
+ * constraint told so far as such a tree, in postorder: nodes of four
+ * int32 {kind, a, b, c}.  Terms push an interval: VAR (a = variable),
+ * CONST (a = value), NEG, ABS (one operand), ADD, SUB, MUL, TDIV, FDIV,
+ * CDIV, EDIV, MIN, MAX (two).  Formulas push a truth value (entailed,
+ * disentailed, or neither): LE, LT, EQ, NE (two terms), AND, OR (a =
+ * number of operands), NOT, EQUIV, XOR (formulas), LIT (a term read as a
+ * boolean: true when 0 is not in it, false when it is {0}), PROP (one
+ * ternary propagator of pcx.h: a = op, b = x, c = y, and the next node's a =
+ * z; a constraint this tree form does not cover is the conjunction of the
+ * propagators it was decomposed into), TRUE, FALSE. */
+enum {
+  PCX_FT_VAR = 1, PCX_FT_CONST, PCX_FT_NEG, PCX_FT_ABS, PCX_FT_ADD, PCX_FT_SUB, PCX_FT_MUL,
+  PCX_FT_TDIV, PCX_FT_FDIV, PCX_FT_CDIV, PCX_FT_EDIV, PCX_FT_MIN, PCX_FT_MAX,
+  PCX_FT_LE, PCX_FT_LT, PCX_FT_EQ, PCX_FT_NE, PCX_FT_AND, PCX_FT_OR, PCX_FT_NOT, PCX_FT_EQUIV,
+  PCX_FT_XOR, PCX_FT_LIT, PCX_FT_PROP, PCX_FT_PROP_Z, PCX_FT_TRUE, PCX_FT_FALSE
+};
+/* Formulas (constraints told so far) and their nodes in total. */
+int64_t    pcx_model_num_formulas(const pcx_model* m);
+int64_t    pcx_model_num_formula_nodes(const pcx_model* m);
+/* nodes: 4 * num_formula_nodes int32; off: num_formulas + 1 offsets (in
+ * nodes) of each formula's postorder list. */
+pcx_status pcx_model_formulas(const pcx_model* m, int32_t* nodes, int64_t* off);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "pcx_internal.h"
+#include "../../include/pcx_model.h"
 
 using namespace pcx;
 
@@ -1156,6 +1157,7 @@ pcx_status pcx_batch_create(pcx_engine* e, int64_t n_inst, int64_t nvars, const 
   auto* b = new (std::nothrow) pcx_batch();
   if (!b) return fail_with(PCX_ENOMEM, "host allocation failed");
   b->e = e;
+  b->device = e->cfg.device;
   b->n_inst = n_inst;
   b->nvars = nvars;
   b->table_n = tn;
@@ -1204,7 +1206,9 @@ pcx_status pcx_batch_create(pcx_engine* e, int64_t n_inst, int64_t nvars, const 
 
 void pcx_batch_destroy(pcx_batch* b) {
   if (!b) return;
-  if (b->e) (void)hipSetDevice(b->e->cfg.device);
+  // the engine may be gone already (a garbage collector frees both in any
+  // order): only the batch's own device number is read
+  (void)hipSetDevice(b->device);
   for (hipEvent_t ev : b->ev) (void)hipEventDestroy(ev);
   dfree(b->d_store); dfree(b->d_image); dfree(b->d_off); dfree(b->d_idx); dfree(b->d_flags);
   dfree(b->d_tight); dfree(b->d_act); dfree(b->d_item_base); dfree(b->d_ctl); dfree(b->d_order);
@@ -1521,6 +1525,98 @@ pcx_status pcx_batch_branch(pcx_batch* parent, pcx_batch* child, int32_t* var, i
   pcx_status st = image_loaded(child, true);        // the branch wrote the children's emptiness
   child->seeded = seeded;
   return st;
+}
+
+namespace {
+// A formula tree list is well formed: known kinds, variables inside the
+// store, ternary opcodes, a stack within kTreeMaxDepth that ends each formula
+// with one value.
+constexpr int kTreeMaxDepth = 32;
+bool trees_valid(const int32_t* nodes, int64_t n_nodes, const int64_t* off, int64_t n_formulas,
+                 int64_t nvars, std::string* why) {
+  if (off[0] != 0 || off[n_formulas] != n_nodes) { *why = "offsets do not cover the nodes"; return false; }
+  for (int64_t f = 0; f < n_formulas; ++f) {
+    if (off[f + 1] < off[f]) { *why = "offsets decrease"; return false; }
+    int depth = 0;
+    for (int64_t j = off[f]; j < off[f + 1]; ++j) {
+      const int32_t* nd = nodes + 4 * j;
+      int pop = 0, push = 1;
+      switch (nd[0]) {
+        case PCX_FT_VAR:
+          if (nd[1] < 0 || nd[1] >= nvars) { *why = "variable outside the store"; return false; }
+          break;
+        case PCX_FT_CONST: case PCX_FT_TRUE: case PCX_FT_FALSE: break;
+        case PCX_FT_NEG: case PCX_FT_ABS: case PCX_FT_NOT: case PCX_FT_LIT: pop = 1; break;
+        case PCX_FT_ADD: case PCX_FT_SUB: case PCX_FT_MUL: case PCX_FT_TDIV: case PCX_FT_FDIV:
+        case PCX_FT_CDIV: case PCX_FT_EDIV: case PCX_FT_MIN: case PCX_FT_MAX: case PCX_FT_LE: case PCX_FT_LT:
+        case PCX_FT_EQ: case PCX_FT_NE: case PCX_FT_EQUIV: case PCX_FT_XOR: pop = 2; break;
+        case PCX_FT_AND: case PCX_FT_OR:
+          if (nd[1] < 0) { *why = "negative operand count"; return false; }
+          pop = nd[1];
+          break;
+        case PCX_FT_PROP: {
+          const int32_t* zn = nd + 4;
+          if (j + 1 >= off[f + 1] || zn[0] != PCX_FT_PROP_Z) { *why = "PROP without its PROP_Z node"; return false; }
+          if (nd[1] < 0 || nd[1] >= PCX_NUM_OPS) { *why = "unknown opcode in PROP"; return false; }
+          for (int32_t v : {nd[2], nd[3], zn[1]})
+            if (v < 0 || v >= nvars) { *why = "variable outside the store"; return false; }
+          ++j;
+          break;
+        }
+        default: *why = "unknown node kind " + std::to_string(nd[0]); return false;
+      }
+      if (depth < pop) { *why = "node without its operands"; return false; }
+      depth += push - pop;
+      if (depth > kTreeMaxDepth) { *why = "tree deeper than 32"; return false; }
+    }
+    if (depth != 1) { *why = "formula leaves " + std::to_string(depth) + " values"; return false; }
+  }
+  return true;
+}
+}  // namespace
+
+pcx_status pcx_batch_ask_formulas(pcx_batch* b, const int32_t* nodes, int64_t n_nodes, const int64_t* off,
+                                  int64_t n_formulas, int32_t* extractable) {
+  if (!b || !off || !extractable || (n_nodes > 0 && !nodes)) return fail_with(PCX_EINVAL, "NULL argument");
+  if (n_nodes < 0 || n_formulas < 0) return fail_with(PCX_EINVAL, "negative size");
+  pcx_engine* e = b->e;
+  if (e->cfg.domain != PCX_DOMAIN_INTERVAL) return fail_with(PCX_EINVAL, "formula trees are asked on interval batches");
+  std::string why;
+  if (!trees_valid(nodes, n_nodes, off, n_formulas, b->nvars, &why)) return fail_with(PCX_EINVAL, "formula trees: " + why);
+  PCX_HIP(hipSetDevice(e->cfg.device));
+  int4* d_nodes = nullptr;
+  int64_t* d_off = nullptr;
+  int32_t* d_ok = nullptr;
+  uint8_t* d_empty = nullptr;
+  auto release = [&]() {
+    for (void* p : {(void*)d_nodes, (void*)d_off, (void*)d_ok, (void*)d_empty}) if (p) (void)hipFree(p);
+  };
+  hipError_t he = hipMalloc(&d_nodes, std::max<int64_t>(n_nodes, 1) * sizeof(int4));
+  if (he == hipSuccess) he = hipMalloc(&d_off, (n_formulas + 1) * sizeof(int64_t));
+  if (he == hipSuccess) he = hipMalloc(&d_ok, b->n_inst * sizeof(int32_t));
+  if (he == hipSuccess) he = hipMalloc(&d_empty, b->n_inst);
+  if (he == hipSuccess && n_nodes > 0)
+    he = hipMemcpyAsync(d_nodes, nodes, n_nodes * sizeof(int4), hipMemcpyHostToDevice, e->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(d_off, off, (n_formulas + 1) * sizeof(int64_t), hipMemcpyHostToDevice, e->stream);
+  std::vector<int32_t> ones(b->n_inst, 1);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(d_ok, ones.data(), b->n_inst * sizeof(int32_t), hipMemcpyHostToDevice, e->stream);
+  if (he == hipSuccess) he = hipMemsetAsync(d_empty, 0, b->n_inst, e->stream);
+  if (he == hipSuccess)
+    he = launch_store_empty_map(b->d_store, b->nvars, (int32_t)b->n_inst, d_empty, dom_of(e), e->stream);
+  if (he == hipSuccess)
+    he = launch_formula_ask(d_nodes, d_off, n_formulas, b->d_store, b->nvars, b->n_inst, d_ok, e->stream);
+  std::vector<uint8_t> empty(b->n_inst);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(extractable, d_ok, b->n_inst * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(empty.data(), d_empty, b->n_inst, hipMemcpyDeviceToHost, e->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+  release();
+  if (he != hipSuccess) return fail_with(PCX_EDEVICE, std::string("formula ask: ") + hipGetErrorString(he));
+  for (int64_t k = 0; k < b->n_inst; ++k) extractable[k] = extractable[k] && !empty[k];
+  return PCX_OK;
 }
 
 pcx_status pcx_batch_ask(pcx_batch* b, int32_t* extractable, int64_t* live_off, int32_t* live_idx,

@@ -84,6 +84,7 @@ struct pcx_batch {
                                       // then [n_inst * kSeedCap] the drain's seeds
   int32_t* d_trail_n = nullptr;       // [2 * n_inst]: the two lists' counts
   bool trail_valid = false;           // the stores are the image but for the trail's variables
+  int32_t device = 0;                 // the engine's device (destroy reads it, not the engine)
   int32_t* d_bvar = nullptr;          // [n_inst] split variables of this batch's last branching
   bool probe_valid = false;           // d_unsat and the counts below hold this image's probe
   uint64_t probe_version = 0;         // of this table version

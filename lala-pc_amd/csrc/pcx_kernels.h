@@ -332,6 +332,10 @@ hipError_t launch_branch(const int2* parent, int64_t nvars, int64_t n_inst, int3
 hipError_t launch_branch_seed(const EventSet& ev, const int32_t* var, int64_t n_child, hipStream_t st);
 hipError_t launch_ask(const Table& t, const int2* store, uint8_t* ent, int32_t* all_flag,
                       hipStream_t st);
+// PC's formula trees (pcx_model.h node lists, validated): ok[k] = 0 where a
+// formula is not asked on instance k's store (ok set to 1 by the caller).
+hipError_t launch_formula_ask(const int4* nodes, const int64_t* off, int64_t n_formulas, const int2* store,
+                              int64_t nvars, int64_t n_inst, int32_t* ok, hipStream_t st);
 hipError_t launch_store_pack(const int32_t* lb, const int32_t* ub, int2* store, int64_t n, hipStream_t st);
 // image[v] = store[v] = {lb[v], ub[v]} for n device-resident bounds, and
 // empty[v / nvars] = 1 where lb > ub (empty zeroed by the caller)

@@ -14,6 +14,7 @@
 //   kItvDiv  interval store, all ten opcodes
 //   kBits    bitset store
 #include "pcx_kernels.h"
+#include "../../include/pcx_model.h"
 #include "pcx_prop.h"
 
 #include <type_traits>
@@ -3492,6 +3493,167 @@ hipError_t launch_branch(const int2* parent, int64_t nvars, int64_t n_inst, int3
     if (v2) k_branch<false, true><<<g, kBlock, 0, stream>>>(parent, nvars, n_inst, var, child, child_image, child_empty);
     else k_branch<false, false><<<g, kBlock, 0, stream>>>(parent, nvars, n_inst, var, child, child_image, child_empty);
   }
+  return hipGetLastError();
+}
+
+// ---- PC's formula trees: is_extractable --------------------------------------
+// One thread per (instance, formula) interprets the formula's postorder
+// nodes with a stack of int2: intervals for terms, {ask, nask} for formulas
+// (PC's Formula::ask / nask, include/lala/formula.hpp; the arithmetic of
+// the terms' projections over int32 bounds whose extremes are the
+// infinities).  A formula not asked clears its instance's flag.
+namespace tree {
+constexpr int32_t kMinf = INT32_MIN, kInf = INT32_MAX;
+__device__ __forceinline__ bool inf(i64 v) { return v == kMinf || v == kInf; }
+__device__ __forceinline__ int32_t sat(i64 v) { return v <= kMinf ? kMinf : v >= kInf ? kInf : (int32_t)v; }
+// a bound of a sum: the infinity it leans to wins over the other
+__device__ __forceinline__ int32_t add(i64 a, i64 b, i64 first) {
+  if (a == first || b == first) return (int32_t)first;
+  if (inf(a)) return (int32_t)a;
+  if (inf(b)) return (int32_t)b;
+  return sat(a + b);
+}
+__device__ __forceinline__ int32_t neg(i64 a) { return a == kMinf ? kInf : a == kInf ? kMinf : (int32_t)-a; }
+__device__ __forceinline__ int sgn(i64 a) { return (a > 0) - (a < 0); }
+__device__ __forceinline__ i64 mul(i64 a, i64 b) {
+  if (a == 0 || b == 0) return 0;
+  if (inf(a) || inf(b)) return sgn(a) * sgn(b) > 0 ? kInf : kMinf;
+  return a * b;
+}
+__device__ __forceinline__ i64 div(int kind, i64 a, i64 b) {
+  if (inf(a)) return sgn(a) * sgn(b) > 0 ? kInf : kMinf;
+  if (inf(b)) return 0;
+  i64 q = a / b;                                   // truncated
+  const i64 r = a - q * b;
+  if (kind == PCX_FT_FDIV) { if (r != 0 && ((r < 0) != (b < 0))) --q; }
+  else if (kind == PCX_FT_CDIV) { if (r != 0 && ((r < 0) == (b < 0))) ++q; }
+  else if (kind == PCX_FT_EDIV) { if (r < 0) q += b > 0 ? -1 : 1; }
+  return q;
+}
+}  // namespace tree
+
+constexpr int kTreeDepth = 32;
+
+__global__ __launch_bounds__(kBlock) void k_formula_ask(const int4* __restrict__ nodes, const int64_t* __restrict__ off,
+                                                        int64_t n_formulas, const int2* __restrict__ store,
+                                                        int64_t nvars, int64_t n_inst, int32_t* ok) {
+  using tree::add;
+  using tree::div;
+  using tree::mul;
+  using tree::neg;
+  using tree::sat;
+  constexpr int32_t kTMinf = tree::kMinf, kTInf = tree::kInf;
+  const int64_t total = n_inst * n_formulas;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t k = i / n_formulas, f = i - k * n_formulas;
+    const int2* st = store + k * nvars;
+    int2 s[kTreeDepth];
+    int sp = 0;
+    for (int64_t j = off[f]; j < off[f + 1]; ++j) {
+      const int4 nd = nodes[j];
+      switch (nd.x) {
+        case PCX_FT_VAR: s[sp++] = st[nd.y]; break;
+        case PCX_FT_CONST: s[sp++] = make_int2(nd.y, nd.y); break;
+        case PCX_FT_TRUE: s[sp++] = make_int2(1, 0); break;
+        case PCX_FT_FALSE: s[sp++] = make_int2(0, 1); break;
+        case PCX_FT_NEG: { const int2 a = s[sp - 1]; s[sp - 1] = make_int2(neg(a.y), neg(a.x)); break; }
+        case PCX_FT_ABS: {
+          const int2 a = s[sp - 1];
+          if (a.x >= 0) break;
+          s[sp - 1] = a.y <= 0 ? make_int2(neg(a.y), neg(a.x)) : make_int2(0, max((int32_t)neg(a.x), a.y));
+          break;
+        }
+        case PCX_FT_ADD: case PCX_FT_SUB: {
+          const int2 a = s[sp - 2];
+          int2 b = s[sp - 1];
+          if (nd.x == PCX_FT_SUB) b = make_int2(neg(b.y), neg(b.x));
+          s[sp - 2] = make_int2(add(a.x, b.x, kTMinf), add(a.y, b.y, kTInf));
+          --sp;
+          break;
+        }
+        case PCX_FT_MUL: {
+          const int2 a = s[sp - 2], b = s[sp - 1];
+          const i64 c0 = mul(a.x, b.x), c1 = mul(a.x, b.y), c2 = mul(a.y, b.x), c3 = mul(a.y, b.y);
+          s[sp - 2] = make_int2(sat(min(min(c0, c1), min(c2, c3))), sat(max(max(c0, c1), max(c2, c3))));
+          --sp;
+          break;
+        }
+        case PCX_FT_TDIV: case PCX_FT_FDIV: case PCX_FT_CDIV: case PCX_FT_EDIV: {
+          const int2 a = s[sp - 2], b = s[sp - 1];
+          if (b.x <= 0 && 0 <= b.y) {
+            s[sp - 2] = make_int2(kTMinf, kTInf);
+          } else {
+            const i64 c0 = div(nd.x, a.x, b.x), c1 = div(nd.x, a.x, b.y), c2 = div(nd.x, a.y, b.x),
+                      c3 = div(nd.x, a.y, b.y);
+            s[sp - 2] = make_int2(sat(min(min(c0, c1), min(c2, c3))), sat(max(max(c0, c1), max(c2, c3))));
+          }
+          --sp;
+          break;
+        }
+        case PCX_FT_MIN: case PCX_FT_MAX: {
+          const int2 a = s[sp - 2], b = s[sp - 1];
+          s[sp - 2] = nd.x == PCX_FT_MIN ? make_int2(min(a.x, b.x), min(a.y, b.y))
+                                         : make_int2(max(a.x, b.x), max(a.y, b.y));
+          --sp;
+          break;
+        }
+        case PCX_FT_LE: case PCX_FT_LT: case PCX_FT_EQ: case PCX_FT_NE: {
+          const int2 l = s[sp - 2], r = s[sp - 1];
+          int2 t;
+          if (nd.x == PCX_FT_LE) t = make_int2(l.y <= r.x, l.x > r.y);
+          else if (nd.x == PCX_FT_LT) t = make_int2(l.y < r.x, l.x >= r.y);
+          else {
+            const int same = l.x == l.y && r.x == r.y && l.x == r.x;
+            const int apart = l.y < r.x || r.y < l.x;
+            t = nd.x == PCX_FT_EQ ? make_int2(same, apart) : make_int2(apart, same);
+          }
+          s[sp - 2] = t;
+          --sp;
+          break;
+        }
+        case PCX_FT_AND: case PCX_FT_OR: {
+          const int n = nd.y;
+          int all_a = 1, any_a = 0, all_n = 1, any_n = 0;
+          for (int q = sp - n; q < sp; ++q) {
+            all_a &= s[q].x; any_a |= s[q].x; all_n &= s[q].y; any_n |= s[q].y;
+          }
+          sp -= n;
+          s[sp++] = nd.x == PCX_FT_AND ? make_int2(all_a, any_n) : make_int2(any_a, all_n);
+          break;
+        }
+        case PCX_FT_NOT: { const int2 a = s[sp - 1]; s[sp - 1] = make_int2(a.y, a.x); break; }
+        case PCX_FT_EQUIV: case PCX_FT_XOR: {
+          const int2 fm = s[sp - 2], g = s[sp - 1];
+          const int same = (fm.x & g.x) | (fm.y & g.y), differ = (fm.x & g.y) | (fm.y & g.x);
+          s[sp - 2] = nd.x == PCX_FT_EQUIV ? make_int2(same, differ) : make_int2(differ, same);
+          --sp;
+          break;
+        }
+        case PCX_FT_LIT: {
+          const int2 a = s[sp - 1];
+          s[sp - 1] = make_int2(!(a.x <= 0 && 0 <= a.y), a.x >= 0 && a.y <= 0);
+          break;
+        }
+        case PCX_FT_PROP: {
+          const int4 zn = nodes[j + 1];
+          s[sp++] = make_int2(entailed(nd.y, st[nd.z], st[nd.w], st[zn.y]) ? 1 : 0, 0);
+          ++j;                                       // the PROP_Z node
+          break;
+        }
+        default: break;                              // (validated on the host)
+      }
+    }
+    if (!s[0].x) ok[k] = 0;
+  }
+}
+
+hipError_t launch_formula_ask(const int4* nodes, const int64_t* off, int64_t n_formulas, const int2* store,
+                              int64_t nvars, int64_t n_inst, int32_t* ok, hipStream_t stream) {
+  const int64_t total = n_inst * n_formulas;
+  if (total <= 0) return hipSuccess;
+  int64_t grid = (total + kBlock - 1) / kBlock;
+  if (grid > 65536) grid = 65536;
+  k_formula_ask<<<(unsigned)grid, kBlock, 0, stream>>>(nodes, off, n_formulas, store, nvars, n_inst, ok);
   return hipGetLastError();
 }
 

@@ -130,6 +130,8 @@ struct pcx_model {
   std::unordered_map<long long, int32_t> consts;
   std::vector<int32_t> op, x, y, z;
   int32_t n_aux = 0;
+  std::vector<int32_t> tnodes;                 // PC's formula trees, 4 int32 per node
+  std::vector<int64_t> toff{0};                // node offset of each formula
   std::string_view own(std::string_view s) { names.emplace_back(s); return names.back(); }
 };
 
@@ -647,6 +649,203 @@ struct Builder {
   }
 };
 
+// ---- PC's formula trees ------------------------------------------------------
+// Each constraint as the tree PC asks (formula.hpp: Inequality 760-770,
+// Equality / Disequality 617-628, VariableLiteral 101-110, Conjunction,
+// Disjunction, Biconditional, Xor, Implication 268-555), over the model's
+// variables and constants.  Built after the constraint was decomposed, so a
+// construct the tree form does not cover takes the conjunction of the
+// ternary propagators the constraint was decomposed into.
+struct TreeBuilder {
+  Builder& b;
+  std::vector<int32_t> out;
+  bool ok = true;
+
+  void node(int k, int32_t a = 0, int32_t bb = 0, int32_t c = 0) {
+    out.push_back(k); out.push_back(a); out.push_back(bb); out.push_back(c);
+  }
+  static bool arith2(std::string_view f, int* k) {
+    if (f == "int_plus") *k = PCX_FT_ADD;
+    else if (f == "int_minus") *k = PCX_FT_SUB;
+    else if (f == "int_times") *k = PCX_FT_MUL;
+    else if (f == "int_div" || f == "int_tdiv") *k = PCX_FT_TDIV;
+    else if (f == "int_fdiv") *k = PCX_FT_FDIV;
+    else if (f == "int_cdiv") *k = PCX_FT_CDIV;
+    else if (f == "int_ediv") *k = PCX_FT_EDIV;
+    else if (f == "int_min") *k = PCX_FT_MIN;
+    else if (f == "int_max") *k = PCX_FT_MAX;
+    else return false;
+    return true;
+  }
+  void term(const Expr& e) {
+    long long c;
+    int32_t v;
+    if (!ok) return;
+    if (b.const_of(e, &c)) {
+      if (c < INT32_MIN || c > INT32_MAX) { ok = false; return; }
+      node(PCX_FT_CONST, (int32_t)c);
+      return;
+    }
+    if (b.var_of(e, &v)) { node(PCX_FT_VAR, v); return; }
+    if (e.kind != eCall) { ok = false; return; }
+    int k;
+    if (arith2(e.name, &k) && e.args.size() == 2) {
+      term(e.args[0]);
+      term(e.args[1]);
+      node(k);
+      return;
+    }
+    if ((e.name == "int_neg" || e.name == "int_abs") && e.args.size() == 1) {
+      term(e.args[0]);
+      node(e.name == "int_neg" ? PCX_FT_NEG : PCX_FT_ABS);
+      return;
+    }
+    if (e.name == "bool2int" && e.args.size() == 1) { term(e.args[0]); return; }
+    ok = false;                                  // a formula used as a term
+  }
+  void lit(const Expr& e) { term(e); node(PCX_FT_LIT); }
+  void lits(const Expr& arr, bool neg) {
+    for (const Expr& a : b.elements(arr)) {
+      lit(a);
+      if (neg) node(PCX_FT_NOT);
+    }
+  }
+  void formula(const Expr& e) {
+    long long c;
+    int32_t v;
+    if (!ok) return;
+    if (b.const_of(e, &c)) { node(c ? PCX_FT_TRUE : PCX_FT_FALSE); return; }
+    if (b.var_of(e, &v)) { node(PCX_FT_VAR, v); node(PCX_FT_LIT); return; }
+    if (e.kind != eCall) { ok = false; return; }
+    const std::string_view f = e.name;
+    const size_t n = e.args.size();
+    if (f.size() > 5 && f.compare(f.size() - 5, 5, "_reif") == 0 && n >= 1) {
+      Expr p = e;
+      p.name = f.substr(0, f.size() - 5);
+      p.args.pop_back();
+      formula(p);
+      lit(e.args.back());
+      node(PCX_FT_EQUIV);
+      return;
+    }
+    int cmp = -1;
+    bool swap = false;
+    if (f == "int_le" || f == "bool_le") cmp = PCX_FT_LE;
+    else if (f == "int_ge" || f == "bool_ge") { cmp = PCX_FT_LE; swap = true; }
+    else if (f == "int_lt" || f == "bool_lt") cmp = PCX_FT_LT;
+    else if (f == "int_gt" || f == "bool_gt") { cmp = PCX_FT_LT; swap = true; }
+    else if (f == "int_eq" || f == "bool_eq" || f == "bool2int") cmp = PCX_FT_EQ;
+    else if (f == "int_ne" || f == "bool_ne") cmp = PCX_FT_NE;
+    if (cmp >= 0 && n == 2) {
+      term(e.args[swap ? 1 : 0]);
+      term(e.args[swap ? 0 : 1]);
+      node(cmp);
+      return;
+    }
+    if ((f == "bool_equiv" || f == "nbool_equiv") && n == 2) {
+      lit(e.args[0]);
+      lit(e.args[1]);
+      node(f == "bool_equiv" ? PCX_FT_EQUIV : PCX_FT_XOR);
+      return;
+    }
+    int k;
+    if (arith2(f, &k) && n == 3) {                 // a op b = r
+      Expr t = e;
+      t.args.pop_back();
+      term(t);
+      term(e.args[2]);
+      node(PCX_FT_EQ);
+      return;
+    }
+    if ((f == "int_neg" || f == "int_abs") && n == 2) {
+      Expr t = e;
+      t.args.pop_back();
+      term(t);
+      term(e.args[1]);
+      node(PCX_FT_EQ);
+      return;
+    }
+    if (f == "int_lin_le" || f == "int_lin_eq" || f == "int_lin_ne") {
+      if (n != 3) { ok = false; return; }
+      const std::vector<Expr> as = b.elements(e.args[0]), xs = b.elements(e.args[1]);
+      if (as.size() != xs.size()) { ok = false; return; }
+      if (as.empty()) node(PCX_FT_CONST, 0);
+      for (size_t i = 0; i < as.size(); ++i) {
+        term(as[i]);
+        term(xs[i]);
+        node(PCX_FT_MUL);
+        if (i) node(PCX_FT_ADD);
+      }
+      term(e.args[2]);
+      node(f == "int_lin_le" ? PCX_FT_LE : f == "int_lin_eq" ? PCX_FT_EQ : PCX_FT_NE);
+      return;
+    }
+    if (f == "bool_clause" && n == 2) {
+      const size_t k2 = b.elements(e.args[0]).size() + b.elements(e.args[1]).size();
+      lits(e.args[0], false);
+      lits(e.args[1], true);
+      node(PCX_FT_OR, (int32_t)k2);
+      return;
+    }
+    if (f == "bool_not") {
+      if (n == 1) { formula(e.args[0]); node(PCX_FT_NOT); return; }
+      if (n == 2) { lit(e.args[0]); lit(e.args[1]); node(PCX_FT_XOR); return; }
+    }
+    if ((f == "bool_and" || f == "bool_or" || f == "bool_xor") && (n == 2 || n == 3)) {
+      lit(e.args[0]);
+      lit(e.args[1]);
+      if (f == "bool_xor") node(PCX_FT_XOR);
+      else node(f == "bool_and" ? PCX_FT_AND : PCX_FT_OR, 2);
+      if (n == 3) { lit(e.args[2]); node(PCX_FT_EQUIV); }
+      return;
+    }
+    if ((f == "array_bool_and" || f == "array_bool_or") && (n == 1 || n == 2)) {
+      const size_t k2 = b.elements(e.args[0]).size();
+      lits(e.args[0], false);
+      node(f == "array_bool_and" ? PCX_FT_AND : PCX_FT_OR, (int32_t)k2);
+      if (n == 2) { lit(e.args[1]); node(PCX_FT_EQUIV); }
+      return;
+    }
+    ok = false;
+  }
+};
+
+// A declared domain with holes ({1, 3}) is a propagator of PC's (the
+// disjunction of the equalities); a range is the store's own domain.
+void record_domain_tree(Builder& b, int32_t v, const Expr& set) {
+  bool rng;
+  long long lo = 0, hi = 0;
+  const std::vector<long long> vals = b.set_values(set, &rng, &lo, &hi);
+  if (rng || vals.empty()) return;
+  TreeBuilder t{b};
+  for (long long c : vals) {
+    t.node(PCX_FT_VAR, v);
+    t.node(PCX_FT_CONST, (int32_t)c);
+    t.node(PCX_FT_EQ);
+  }
+  t.node(PCX_FT_OR, (int32_t)vals.size());
+  b.m.tnodes.insert(b.m.tnodes.end(), t.out.begin(), t.out.end());
+  b.m.toff.push_back((int64_t)(b.m.tnodes.size() / 4));
+}
+
+// Record constraint e's tree (props [p0, end) are what it was decomposed into).
+void record_tree(Builder& b, const Expr& e, size_t p0) {
+  pcx_model& m = b.m;
+  TreeBuilder t{b};
+  t.formula(e);
+  if (!t.ok) {
+    t.out.clear();
+    const size_t p1 = m.op.size();
+    for (size_t p = p0; p < p1; ++p) {
+      t.node(PCX_FT_PROP, m.op[p], m.x[p], m.y[p]);
+      t.node(PCX_FT_PROP_Z, m.z[p]);
+    }
+    t.node(PCX_FT_AND, (int32_t)(p1 - p0));
+  }
+  m.tnodes.insert(m.tnodes.end(), t.out.begin(), t.out.end());
+  m.toff.push_back((int64_t)(m.tnodes.size() / 4));
+}
+
 // ---- parser -------------------------------------------------------------------
 struct Parser {
   Lexer lx;
@@ -792,7 +991,7 @@ struct Parser {
       fail("name declared twice: " + std::string(name));
     const int32_t v = b.new_var(name, lo, hi);
     b.m.scalars[b.m.vars[v].name] = v;
-    if (set.kind == eSet && !set.args.empty()) b.post_set_in(v, set);
+    if (set.kind == eSet && !set.args.empty()) { b.post_set_in(v, set); record_domain_tree(b, v, set); }
     if (is("=")) {
       skip();
       Expr eq;
@@ -858,7 +1057,7 @@ struct Parser {
       check_int(dhi);
       for (long long k = 1; k <= hi; ++k) {
         const int32_t v = b.new_var(std::string(name) + "[" + std::to_string(k) + "]", dlo, dhi);
-        if (set.kind == eSet && !set.args.empty()) b.post_set_in(v, set);
+        if (set.kind == eSet && !set.args.empty()) { b.post_set_in(v, set); record_domain_tree(b, v, set); }
         elems.push_back(v);
       }
     }
@@ -890,7 +1089,9 @@ struct Parser {
         skip();
         const Expr e = expr();
         skip_annotations();
+        const size_t p0 = b.m.op.size();
         b.post(e);
+        record_tree(b, e, p0);
       } else if (is("solve") || is("output")) {
         while (peek().kind != tEnd && !is(";")) skip();
       } else if (is("predicate") || is("function")) {
@@ -978,6 +1179,17 @@ int32_t pcx_model_var(const pcx_model* m, const char* name) {
   for (size_t v = 0; v < m->vars.size(); ++v)
     if (m->vars[v].name == std::string_view(s)) return (int32_t)v;
   return -1;
+}
+
+int64_t pcx_model_num_formulas(const pcx_model* m) { return m ? (int64_t)m->toff.size() - 1 : 0; }
+
+int64_t pcx_model_num_formula_nodes(const pcx_model* m) { return m ? (int64_t)m->tnodes.size() / 4 : 0; }
+
+pcx_status pcx_model_formulas(const pcx_model* m, int32_t* nodes, int64_t* off) {
+  if (!m || !nodes || !off) return pcx_internal::fail_with(PCX_EINVAL, "NULL argument");
+  std::copy(m->tnodes.begin(), m->tnodes.end(), nodes);
+  std::copy(m->toff.begin(), m->toff.end(), off);
+  return PCX_OK;
 }
 
 const char* pcx_model_var_name(const pcx_model* m, int32_t v) {

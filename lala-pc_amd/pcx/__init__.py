@@ -153,6 +153,7 @@ SIGNATURES = [
     ("pcx_batch_run", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Result), ctypes.POINTER(Stats)]),
     ("pcx_batch_branch", ctypes.c_int32, [_VP, _VP, _P32, ctypes.c_int32]),
     ("pcx_batch_ask", ctypes.c_int32, [_VP, _P32, _P64, _P32, ctypes.c_int64]),
+    ("pcx_batch_ask_formulas", ctypes.c_int32, [_VP, _P32, ctypes.c_int64, _P64, ctypes.c_int64, _P32]),
     ("pcx_batch_autotune", ctypes.c_int32, [_VP, ctypes.c_int32, ctypes.POINTER(Tuning),
                                             ctypes.POINTER(Tuning), ctypes.c_int32, _P32]),
     ("pcx_bits_tell", ctypes.c_int32, [_VP, _P32, _P32, _P32, _P32, ctypes.c_int64, _PU64, ctypes.c_int64]),
@@ -192,6 +193,9 @@ SIGNATURES = [
     ("pcx_model_store", ctypes.c_int32, [_VP, _P32, _P32]),
     ("pcx_model_var", ctypes.c_int32, [_VP, ctypes.c_char_p]),
     ("pcx_model_var_name", ctypes.c_char_p, [_VP, ctypes.c_int32]),
+    ("pcx_model_num_formulas", ctypes.c_int64, [_VP]),
+    ("pcx_model_num_formula_nodes", ctypes.c_int64, [_VP]),
+    ("pcx_model_formulas", ctypes.c_int32, [_VP, _P32, _P64]),
 ]
 
 _lib = None
@@ -477,6 +481,17 @@ class Batch:
         _check(lib().pcx_batch_branch(self._h, child._h, _p32(var), 1 if from_fixpoint else 0))
         return var[:self.n_inst]
 
+    def ask_formulas(self, nodes, off):
+        """PC's is_extractable of every store for formula trees (nodes an
+        (n, 4) int32 array, off the n_formulas + 1 offsets, as
+        Model.formulas() gives them): a bool array, one per instance."""
+        nodes = np.ascontiguousarray(nodes, np.int32).reshape(-1, 4)
+        off = np.ascontiguousarray(off, np.int64)
+        out = np.zeros(self.n_inst, np.int32)
+        _check(lib().pcx_batch_ask_formulas(self._h, nodes.ctypes.data_as(_P32), len(nodes), _p64(off),
+                                            len(off) - 1, _p32(out)))
+        return out.astype(bool)
+
     def ask(self, live=True):
         """Entailment on the current stores: (extractable bool[n_inst],
         live_off int64[n_inst+1], live_idx int32[...]) -- the propagators of
@@ -740,6 +755,17 @@ class Model:
     def var_name(self, v):
         r = lib().pcx_model_var_name(self._h, v)
         return None if r is None else r.decode()
+
+    def formulas(self):
+        """PC's formula trees of the constraints told so far: (nodes, off),
+        nodes an (n, 4) int32 array in postorder (kinds PCX_FT_* of
+        pcx_model.h), formula f at nodes[off[f]:off[f + 1]]."""
+        n = int(lib().pcx_model_num_formula_nodes(self._h))
+        nf = int(lib().pcx_model_num_formulas(self._h))
+        nodes = np.zeros((max(n, 1), 4), np.int32)
+        off = np.zeros(nf + 1, np.int64)
+        _check(lib().pcx_model_formulas(self._h, nodes.ctypes.data_as(_P32), _p64(off)))
+        return nodes[:n], off
 
     def meet_store(self, lb, ub):
         """The current store met with the model's domains; variables created

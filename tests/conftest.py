@@ -7,6 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(ROOT, "lala-pc_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))            # the oracle's Python parts (pc_ask)
 
 
 def pytest_configure(config):

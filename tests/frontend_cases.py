@@ -44,6 +44,7 @@ def run_case(case, backend):
         elif step["text"]:
             model.add(step["text"])
         lb, ub = model.meet_store(lb, ub)
+        backend.model = model                    # a backend may read the model (its formulas)
         op, x, y, z = model.table(told)
         told = model.num_props()
         counts.append(told)

@@ -12,7 +12,10 @@ reported, not pinned.  Parse errors and unsupported predicates are errors.
 import numpy as np
 import pytest
 
+import os
+
 import oracle_lib as ol
+import pc_ask
 import pcx
 from frontend_cases import load_cases, load_pc_cases, run_case
 
@@ -102,6 +105,47 @@ def test_extractable_and_changed_flags():
     n, ua, ch = _flag_agreement(PC_CASES)
     print(f"PC tests: is_extractable {ua}/{n}, has_changed {ch}/{n}")
     assert ua >= n - 10 and ch >= n - 11
+
+
+class TreeAskingOracle(Oracle):
+    """The oracle's fixpoint, then PC's is_extractable over the model's
+    formula trees (oracle/pc_ask.py) per step."""
+
+    def __init__(self):
+        super().__init__()
+        self.steps = []
+
+    def __call__(self, op, x, y, z, lb, ub, before):
+        failed = super().__call__(op, x, y, z, lb, ub, before)
+        nodes, off = self.model.formulas()
+        self.steps.append(pc_ask.is_extractable(nodes, off, lb, ub, ol.ask))
+        return failed
+
+
+def test_pc_formula_trees_give_pc_is_extractable():
+    """PC asks each constraint as a formula tree.  With the model's trees
+    (pcx_model_formulas), every is_extractable flag of tests/pc_test.cpp's
+    model tests is the reference's (through the ternary propagators, ten of
+    them differ: test_extractable_and_changed_flags): 98 of 98 steps."""
+    n = 0
+    for c in PC_CASES:
+        o = TreeAskingOracle()
+        errs, _ = run_case(c, o)
+        assert not errs
+        for k, (st, ua) in enumerate(zip(c["steps"], o.steps)):
+            if st["failed"]:
+                continue
+            n += 1
+            assert ua == st["ua"], f"{c['name']} ({c['ref']}) step {k}: is_extractable {ua}, expected {st['ua']}"
+    assert n >= 90
+
+
+def test_formula_tree_kinds_match_the_header():
+    import re
+    text = open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "include", "pcx_model.h")).read()
+    body = text[text.index("PCX_FT_VAR = 1"):text.index("};", text.index("PCX_FT_VAR = 1"))]
+    names = re.findall(r"PCX_FT_(\w+)", body)
+    assert [getattr(pc_ask, n) for n in names] == list(range(1, len(names) + 1))
 
 
 def test_fixture_covers_the_model_tests():

@@ -327,3 +327,23 @@ def test_device_loads_switch_between_image_forms(gpu, gi):
             batch.reset()
             l0, u0 = batch.read()
             assert np.array_equal(l0, lo) and np.array_equal(u0, hi), ("reset", kind, rep)
+
+
+def test_batch_outlives_its_engine(gpu):
+    """A garbage collector may free an engine before a batch over it (a
+    reference cycle through a test's traceback does): destroying the batch
+    then reads nothing of the engine, and the device stays usable."""
+    op = np.array([pcx.ADD], np.int32)
+    x, y, z = (np.array([v], np.int32) for v in (0, 1, 2))
+    lb = np.array([0, 0, 0], np.int32)
+    ub = np.array([10, 3, 3], np.int32)
+    pir = pcx.PIR()
+    pir.tell(op, x, y, z, lb.copy(), ub.copy())
+    b = pcx.Batch(pir, 2, 3)
+    b.load(np.tile(lb, 2), np.tile(ub, 2))
+    pir.close()                                  # the engine first
+    b.close()
+    p2 = pcx.PIR()
+    lo, hi = lb.copy(), ub.copy()
+    p2.tell(op, x, y, z, lo, hi)
+    assert not p2.fixpoint(lo, hi).failed and list(hi) == [6, 3, 3]

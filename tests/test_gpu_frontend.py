@@ -48,6 +48,41 @@ def test_reference_pc_models_on_the_gpu(gpu, case):
     assert not errs, f"{case['name']} ({case['ref']}): " + "; ".join(errs)
 
 
+class TreeEngine(Engine):
+    """The GPU fixpoint, then PC's is_extractable of the model's formula trees
+    asked on the GPU (pcx_batch_ask_formulas) per step."""
+
+    def __init__(self):
+        super().__init__()
+        self.steps = []
+
+    def __call__(self, op, x, y, z, lb, ub, before):
+        failed = super().__call__(op, x, y, z, lb, ub, before)
+        nodes, off = self.model.formulas()
+        b = pcx.Batch(self.pir, 1, len(lb))
+        b.load(lb, ub)
+        self.steps.append(bool(b.ask_formulas(nodes, off)[0]))
+        b.close()
+        return failed
+
+
+def test_pc_is_extractable_on_the_gpu(gpu):
+    """tests/pc_test.cpp's is_extractable flags (pc.hpp:727-738) from the
+    model's formula trees, each tree interpreted by a GPU thread: every step
+    of every PC model test that does not fail."""
+    n = 0
+    for c in PC_CASES:
+        eng = TreeEngine()
+        errs, _ = run_case(c, eng)
+        assert not errs, c["name"]
+        for k, (st, ua) in enumerate(zip(c["steps"], eng.steps)):
+            if st["failed"]:
+                continue
+            n += 1
+            assert ua == st["ua"], f"{c['name']} ({c['ref']}) step {k}: is_extractable {ua}, expected {st['ua']}"
+    assert n >= 90
+
+
 @pytest.mark.parametrize("seed", [3, 4])
 def test_generated_model_matches_oracle(gpu, seed):
     p = pcx.gen_params(nvars=1000, nprops=3000, seed=seed)
