@@ -1613,6 +1613,7 @@ pcx_status pcx_batch_ask_formulas(pcx_batch* b, const int32_t* nodes, int64_t n_
   if (he == hipSuccess)
     he = hipMemcpyAsync(empty.data(), d_empty, b->n_inst, hipMemcpyDeviceToHost, e->stream);
   if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+  else (void)hipStreamSynchronize(e->stream);     // nothing queued still uses the buffers
   release();
   if (he != hipSuccess) return fail_with(PCX_EDEVICE, std::string("formula ask: ") + hipGetErrorString(he));
   for (int64_t k = 0; k < b->n_inst; ++k) extractable[k] = extractable[k] && !empty[k];
