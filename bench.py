@@ -296,6 +296,8 @@ def bench_batched(args):
 
     if args.search_step:
         out["search_step"] = search_step(args, pir, batch, off, idx, res, lb_g, ub_g, V, n_inst)
+    if args.formula_ask:
+        out["pc_formula_ask"] = formula_ask(pir, batch, op, x, y, z, lb, ub, w, n_inst, V)
 
     if not args.no_cpu:
         ol = cpu_oracle()
@@ -451,6 +453,41 @@ def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n
     for d_lo, d_hi in sets:
         d_lo.close()
         d_hi.close()
+    return out
+
+
+def formula_ask(pir, batch, op, x, y, z, lb, ub, w, n_inst, V):
+    """PC's is_extractable over formula trees: the network written as
+    FlatZinc, read back by the front-end, which keeps each constraint as PC's
+    tree (x = y op z as an equality of a term, a reified comparison as an
+    equivalence); one GPU thread per (store, tree) (pcx_batch_ask_formulas).
+    On the batch's fixpoints (no store is a solution: each store stops at its
+    first tree not asked, as PC's loop does), and on 4,096 copies of the
+    planted solution (every tree asked: the full work, and every store must
+    come out extractable).  Host parsing is setup; the timed part is the call
+    (tree validation, upload, the kernel, the flags back)."""
+    t0 = time.time()
+    m = pcx.Model(pcx.flatzinc(op, x, y, z, lb, ub))
+    nodes, off = m.formulas()
+    sol = pcx.Batch(pir, n_inst, V)
+    ws = np.tile(w.astype(np.int32), n_inst)
+    sol.load(ws, ws)
+    setup = time.time() - t0
+    out = {"formulas": int(len(off) - 1), "nodes": int(len(nodes)), "instances": n_inst,
+           "setup_s": round(setup, 2)}
+    for name, b in (("fixpoints", batch), ("solutions", sol)):
+        b.ask_formulas(nodes, off)
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            ext = b.ask_formulas(nodes, off)
+            ts.append(time.perf_counter() - t1)
+        t = min(ts)
+        out[name] = {"ms": t * 1e3, "extractable": int(ext.sum())}
+        log(f"formula ask on the {name}: {out['formulas']} trees x {n_inst} stores in {t * 1e3:.2f} ms, "
+            f"{int(ext.sum())} extractable")
+    out["solutions"]["trees_per_s"] = (len(off) - 1) * n_inst / (out["solutions"]["ms"] * 1e-3)
+    sol.close()
     return out
 
 
@@ -872,6 +909,8 @@ def main():
     ap.add_argument("--slots", type=int, default=256, help="serving: requests on the GPU at once")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the workload, three plain device copies of known size (counter calibration)")
+    ap.add_argument("--formula-ask", type=int, default=1, choices=[0, 1],
+                    help="config 2: PC's is_extractable over formula trees on the batch")
     ap.add_argument("--serve-root", type=int, default=1, choices=[0, 1],
                     help="serving: register each network's told store (requests start from the root fixpoint)")
     ap.add_argument("--sweeps-per-tick", type=int, default=4, help="serving: sweeps between host checks")

@@ -3543,9 +3543,16 @@ __global__ __launch_bounds__(kBlock) void k_formula_ask(const int4* __restrict__
   using tree::neg;
   using tree::sat;
   constexpr int32_t kTMinf = tree::kMinf, kTInf = tree::kInf;
-  const int64_t total = n_inst * n_formulas;
+  // Threads take chunks of kBlock formulas of one instance, chunk by chunk
+  // over all instances, so that once a formula of an instance is not asked
+  // (is_extractable is false, as PC's loop returns at the first one) its
+  // later chunks only read the flag.
+  const int64_t chunks = (n_formulas + kBlock - 1) / kBlock;
+  const int64_t total = chunks * n_inst * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock) {
-    const int64_t k = i / n_formulas, f = i - k * n_formulas;
+    const int64_t row = i / kBlock, k = row % n_inst;
+    const int64_t f = (row / n_inst) * kBlock + (i - row * kBlock);
+    if (f >= n_formulas || !__hip_atomic_load(&ok[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
     const int2* st = store + k * nvars;
     int2 s[kTreeDepth];
     int sp = 0;
@@ -3643,16 +3650,15 @@ __global__ __launch_bounds__(kBlock) void k_formula_ask(const int4* __restrict__
         default: break;                              // (validated on the host)
       }
     }
-    if (!s[0].x) ok[k] = 0;
+    if (!s[0].x) __hip_atomic_store(&ok[k], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
 hipError_t launch_formula_ask(const int4* nodes, const int64_t* off, int64_t n_formulas, const int2* store,
                               int64_t nvars, int64_t n_inst, int32_t* ok, hipStream_t stream) {
-  const int64_t total = n_inst * n_formulas;
-  if (total <= 0) return hipSuccess;
-  int64_t grid = (total + kBlock - 1) / kBlock;
-  if (grid > 65536) grid = 65536;
+  if (n_inst <= 0 || n_formulas <= 0) return hipSuccess;
+  int64_t grid = ((n_formulas + kBlock - 1) / kBlock) * n_inst;     // one workgroup per (chunk, instance)
+  if (grid > 8192) grid = 8192;
   k_formula_ask<<<(unsigned)grid, kBlock, 0, stream>>>(nodes, off, n_formulas, store, nvars, n_inst, ok);
   return hipGetLastError();
 }

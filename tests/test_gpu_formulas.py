@@ -140,3 +140,28 @@ def test_malformed_trees_are_refused(gpu):
     ok = np.array([(pa.VAR, 0, 0, 0), (pa.CONST, 1, 0, 0), (pa.LE, 0, 0, 0)], np.int32)
     assert b.ask_formulas(ok, np.array([0, 3], np.int64)).all()
     assert b.ask_formulas(np.zeros((0, 4), np.int32), np.array([0], np.int64)).all()   # no formula
+
+
+def test_planted_solutions_are_extractable(gpu):
+    """A generated network as FlatZinc, read back as PC's trees: stores
+    holding its planted solution are extractable on the GPU, and a store
+    with one value moved off the solution is not (a tree it breaks is
+    disproved); the oracle agrees on every store."""
+    p = pcx.gen_params(seed=9, nvars=5000, nprops=20000)
+    op, x, y, z, lb, ub, w = pcx.gen_network(p)
+    pir = pcx.PIR()
+    pir.tell(op, x, y, z, lb, ub)
+    m = pcx.Model(pcx.flatzinc(op, x, y, z, lb, ub))
+    nodes, off = m.formulas()
+    n, V = 6, len(w)
+    stores = np.tile(w.astype(np.int32), (n, 1))
+    free = np.nonzero(lb != ub)[0]
+    for k in (2, 5):
+        v = int(free[k])
+        stores[k, v] = lb[v] if w[v] != lb[v] else ub[v]     # another value of its domain
+    b = pcx.Batch(pir, n, V)
+    b.load(stores.ravel(), stores.ravel())
+    got = b.ask_formulas(nodes, off)
+    want = np.array([pa.is_extractable(nodes, off, stores[k], stores[k], ol.ask) for k in range(n)])
+    assert np.array_equal(got, want)
+    assert got[[0, 1, 3, 4]].all()
