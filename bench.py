@@ -350,13 +350,29 @@ def bench_batched(args):
     return out
 
 
-def ingest_kernel(pir, n_inst, V):
+def ingest_kernel(pir, n_inst, V, written):
     """(kernel, algorithmic bytes) of a device load of n_inst stores: with
-    base seeding the load writes the stores only (8 B read + 8 B written per
-    value), otherwise it also writes the interleaved copy (+8 B)."""
+    base seeding, into stores that hold the base store (a load after a
+    drained run puts them back to it), the load reads the stores (8 B per
+    value) and writes the `written` values that differ from the base (8 B
+    each); without it, it writes the stores and their interleaved copy (8 B
+    read + 16 B written per value)."""
     if pir.get_config().base_seed != 0:
-        return "k_ingest_rows", 16 * n_inst * V
+        return "k_ingest_rows", 8 * n_inst * V + 8 * written
     return "k_ingest_interleave4", 24 * n_inst * V
+
+
+def differing_values(lo, hi, n_inst, V):
+    """Values of a store set that differ from its base store (per variable
+    the value two of instances 0-2 hold, instance 0's when none agree)."""
+    lo2, hi2 = lo.reshape(n_inst, V), hi.reshape(n_inst, V)
+    if n_inst < 3:
+        return int(((lo2 != lo2[0]) | (hi2 != hi2[0])).sum())
+    same = [(lo2[a] == lo2[b]) & (hi2[a] == hi2[b]) for a, b in ((0, 1), (0, 2), (1, 2))]
+    first = same[0] | same[1]
+    blo = np.where(first, lo2[0], np.where(same[2], lo2[1], lo2[0]))
+    bhi = np.where(first, hi2[0], np.where(same[2], hi2[1], hi2[0]))
+    return int(((lo2 != blo) | (hi2 != bhi)).sum())
 
 
 def timed_fresh(args, batch, sets, mode, steps):
@@ -382,10 +398,11 @@ def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n
     their last stores compared with the seeded run's."""
     from pcx.device import DeviceArray
     t0 = time.time()
-    sets = []
+    sets, written = [], []
     for j in range(n_sets):
         lo, hi = pcx.gen_splits(args.seed + 100 + j, root_lb, root_ub, n_inst, 1, 20)
         sets.append((DeviceArray(lo), DeviceArray(hi)))
+        written.append(differing_values(lo, hi, n_inst, V))
         if j == (args.steps - 1) % n_sets:
             last_host = (lo, hi)
         del lo, hi
@@ -397,7 +414,7 @@ def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n
     elapsed, sums, res = timed_fresh(args, batch, sets, mode, args.steps)
     fps = n_inst * args.steps / elapsed
     seeded = pir.get_config().base_seed != 0 and sums.v["full_evals"] == 0
-    load = ingest_kernel(pir, n_inst, V)
+    load = ingest_kernel(pir, n_inst, V, sum(written) // n_sets)
     out = {"fixpoints_per_s": fps, "ms_per_step": elapsed / args.steps * 1e3, "store_sets": n_sets,
            "evals_per_s": sums.v["evals"] / elapsed,
            "kernel_ms_per_step": sums.v["kernel_ms"] / args.steps,
@@ -406,8 +423,10 @@ def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n
            "first_sweep": "seeded from the base store" if seeded else "full",
            "roofline": sums.roofline(args.steps, full_kernel(mode, sums), marked_kernel(pir, mode, sums),
                                      "cfg2", load=load),
-           "what": "each step loads 4,096 new stores from HBM (pcx_batch_load_device: the stores, the "
-                   "emptiness, the base store and the values that differ from it, in one pass) and "
+           "values_differing_from_base": sum(written) // n_sets,
+           "what": "each step loads 4,096 new stores from HBM (pcx_batch_load_device: the stores, put "
+                   "back to the base store after the last run, get the values that differ from it; the "
+                   "emptiness, the base store and the listed differences in the same pass) and "
                    "propagates them; the first sweep evaluates only the propagators adjacent to a "
                    "listed value and those the base store does not satisfy"}
     log(f"fresh: {out['ms_per_step']:.2f} ms/step, {fps:.0f} fixpoints/s, first sweep {out['first_sweep']}")
@@ -446,7 +465,7 @@ def fresh_steps(args, pir, batch, root_lb, root_ub, off, idx, mode, n_inst, V, n
             "same_fixpoints_as_seeded": bool(agree),
             "useful_lane_frac": useful_lane_fraction(off, n_inst, len(pir.table()[0]), sums_f.lockstep),
             "roofline": sums_f.roofline(args.steps, full_kernel(mode, sums_f), marked_kernel(pir, mode, sums_f),
-                                        "cfg2", load=ingest_kernel(pir, n_inst, V))}
+                                        "cfg2", load=ingest_kernel(pir, n_inst, V, 0))}
         pir.set_config(base_seed=-1)
         log(f"fresh, full first sweep: {out['full_first_sweep']['ms_per_step']:.2f} ms/step, "
             f"same fixpoints: {agree}")

@@ -1219,7 +1219,7 @@ void pcx_batch_destroy(pcx_batch* b) {
   if (b->queue_graph) (void)hipGraphExecDestroy(b->queue_graph);
   if (b->plain_queue_graph) (void)hipGraphExecDestroy(b->plain_queue_graph);
   dfree(b->d_q); dfree(b->d_qctl);
-  dfree(b->d_diff); dfree(b->d_diff_val); dfree(b->d_unsat); dfree(b->d_trail); dfree(b->d_trail_n); dfree(b->d_bvar); dfree(b->d_seedctl); dfree(b->d_base);
+  dfree(b->d_diff); dfree(b->d_diff_val); dfree(b->d_unsat); dfree(b->d_trail); dfree(b->d_trail_n); dfree(b->d_bvar); dfree(b->d_seedctl); dfree(b->d_base); dfree(b->d_base_alt);
   for (hipEvent_t ev : b->load_ev)
     if (ev) (void)hipEventDestroy(ev);
   delete b;
@@ -1354,12 +1354,29 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
   pcx_engine* e = b->e;
   PCX_HIP(hipSetDevice(e->cfg.device));
   const int64_t n = b->n_inst * b->nvars;
+  // Stores that hold their image as a base store plus a complete difference
+  // list (or that image but for the trail of a drained run from it) go back
+  // to the base store first: the ingest then writes only the values that
+  // differ from it (loads of nodes of one search tree share most values).
+  // The new base store goes to the other base buffer.
+  const bool held = b->shared_loads && b->image_in_diffs && (b->store_is_image || b->trail_valid) &&
+                    e->cfg.base_seed != 0 && b->n_inst >= 2 && b->nvars > 0;
+  if (held) {
+    if (!b->d_base_alt) PCX_HIP(hipMalloc(&b->d_base_alt, b->nvars * sizeof(int2)));
+    const InstLists trail{b->d_trail, b->d_trail_n, kTrailCap};
+    PCX_HIP(launch_stores_to_base(b->store_is_image ? nullptr : &trail,
+                                  DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val},
+                                  b->d_store, b->nvars, b->n_inst, e->stream));
+    std::swap(b->d_base, b->d_base_alt);   // d_base_alt: what every row now holds
+  }
+  b->store_is_image = false;               // until the load below completes
+  b->trail_valid = false;
+  b->image_in_diffs = false;
   PCX_HIP(hipMemsetAsync(b->d_image_empty, 0, b->n_inst, e->stream));
   DiffSink ds;
   pcx_status dst = diff_sink(b, &ds);
   if (dst != PCX_OK) return dst;
   const int gi = lockstep_of(b);
-  b->image_in_diffs = false;
   const bool rows = ds.list && b->shared_loads;
   if (rows) PCX_HIP(launch_make_base_rows(d_lb, d_ub, b->nvars, b->n_inst, ds, e->stream));
   // config.timing: HIP events around the load's pass over the stores
@@ -1378,7 +1395,8 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
     // the image; otherwise the image is copied from the stores now, as the
     // interleaved copy a lockstep full sweep reads when the batch has one,
     // and later loads write it in the ingest pass again.
-    PCX_HIP(launch_ingest_rows(d_lb, d_ub, b->d_store, b->nvars, b->n_inst, b->d_image_empty, ds, e->stream));
+    PCX_HIP(launch_ingest_rows(d_lb, d_ub, b->d_store, b->nvars, b->n_inst, b->d_image_empty, ds,
+                               held ? b->d_base_alt : nullptr, e->stream));
     if (t1) PCX_HIP(hipEventRecord(t1, e->stream));
     t1_recorded = true;                 // (a copy of the image below records it again)
     bool complete = false;

@@ -72,6 +72,7 @@ struct pcx_batch {
   // (both written by the load), and the propagators the base does not
   // satisfy (found by the run)
   int2* d_base = nullptr;             // [nvars]
+  int2* d_base_alt = nullptr;         // [nvars]: the previous load's base store (device loads)
   unsigned long long* d_diff = nullptr;   // kDiffLists sub-lists of diff_cap entries
   int2* d_diff_val = nullptr;         // the listed values (device loads)
   bool image_in_diffs = false;        // the image is the base store + the listed values

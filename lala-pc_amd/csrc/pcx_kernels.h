@@ -228,8 +228,11 @@ hipError_t launch_make_base_rows(const int32_t* lb, const int32_t* ub, int64_t n
 // Device-resident lb / ub into the stores only (no image copy), with the
 // differences from the base store (made first) and their values into ds,
 // and the emptiness; a complete list is the image (launch_image_from_diffs).
+// held (nullable, nvars values): stores that hold these values in every row
+// (launch_stores_to_base), of which only the differing values are written.
 hipError_t launch_ingest_rows(const int32_t* lb, const int32_t* ub, int2* store, int64_t nvars,
-                              int64_t n_inst, uint8_t* empty, const DiffSink& ds, hipStream_t st);
+                              int64_t n_inst, uint8_t* empty, const DiffSink& ds, const int2* held,
+                              hipStream_t st);
 // store[k][v] = the base value, then every listed difference's value: the
 // stores a complete difference list (with values) was taken from.
 hipError_t launch_image_from_diffs(const DiffSink& ds, int2* store, int64_t nvars, int64_t n_inst,
@@ -275,6 +278,12 @@ hipError_t launch_drain_marks(const Table& t, int2* store, int64_t nvars, InstFl
 // differences (ds.vals non-null: written after the base values).
 hipError_t launch_restore_trail(const InstLists& trail, const DiffSink& ds, const int2* image, int2* store,
                                 int64_t nvars, int64_t n_inst, hipStream_t st);
+// Stores that hold their image (base store plus a complete difference list)
+// except at a drained run's trail (trail non-null) back to the base store in
+// every row: the trail's variables (a row past its cap whole) and the listed
+// differences' positions get their base values.
+hipError_t launch_stores_to_base(const InstLists* trail, const DiffSink& ds, int2* store, int64_t nvars,
+                                 int64_t n_inst, hipStream_t st);
 // dst (n_inst * nvars words, row by row) from an interleaved copy
 hipError_t launch_deinterleave(const int2* shadow, int2* dst, int64_t nvars, int64_t n_inst, int gi,
                                hipStream_t st);

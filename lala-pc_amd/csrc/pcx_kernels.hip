@@ -2901,12 +2901,15 @@ hipError_t launch_list_diffs(const int2* store, int64_t nvars, int64_t n_inst, c
 // step to step.  The differences from the base store (made first) are listed
 // with their values; a list that overflowed is not worth a check per
 // workgroup here (the load then copies the image and later loads take the
-// fused ingest).
-template <bool V4>
+// fused ingest).  With `held` (stores that hold these base values in every
+// row) only the values that differ from it are written: the pass then reads
+// the rows and writes a few values.
+template <bool V4, bool HELD>
 __global__ __launch_bounds__(kBlock) void k_ingest_rows(const int32_t* __restrict__ lb,
                                                         const int32_t* __restrict__ ub, u64* __restrict__ store,
                                                         int64_t nvars, int64_t n_inst,
-                                                        uint8_t* __restrict__ empty, DiffSink ds) {
+                                                        uint8_t* __restrict__ empty, DiffSink ds,
+                                                        const u64* __restrict__ held) {
   constexpr int kPer = V4 ? 4 : 1;
   const bool listing = ds.list != nullptr;
   const u64* base = reinterpret_cast<const u64*>(ds.base);
@@ -2925,14 +2928,23 @@ __global__ __launch_bounds__(kBlock) void k_ingest_rows(const int32_t* __restric
         const v4i b = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(ub + i));
         w[0] = pack_bounds(a.x, b.x); w[1] = pack_bounds(a.y, b.y);
         w[2] = pack_bounds(a.z, b.z); w[3] = pack_bounds(a.w, b.w);
-        const v4i p0 = {a.x, b.x, a.y, b.y}, p1 = {a.z, b.z, a.w, b.w};
-        reinterpret_cast<v4i*>(store + i)[0] = p0;
-        reinterpret_cast<v4i*>(store + i)[1] = p1;
+        if constexpr (HELD) {
+          const ulonglong2 h0 = reinterpret_cast<const ulonglong2*>(held + v)[0];
+          const ulonglong2 h1 = reinterpret_cast<const ulonglong2*>(held + v)[1];
+          const u64 h[4] = {h0.x, h0.y, h1.x, h1.y};
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (w[j] != h[j]) store[i + j] = w[j];
+        } else {
+          const v4i p0 = {a.x, b.x, a.y, b.y}, p1 = {a.z, b.z, a.w, b.w};
+          reinterpret_cast<v4i*>(store + i)[0] = p0;
+          reinterpret_cast<v4i*>(store + i)[1] = p1;
+        }
         bad = (a.x > b.x) | (a.y > b.y) | (a.z > b.z) | (a.w > b.w);
       } else {
         const int32_t a = __builtin_nontemporal_load(lb + i), b = __builtin_nontemporal_load(ub + i);
         w[0] = pack_bounds(a, b);
-        store[i] = w[0];
+        if (!HELD || w[0] != held[v]) store[i] = w[0];
         bad = a > b;
       }
       if (bad) empty[k] = 1;
@@ -2976,17 +2988,21 @@ hipError_t launch_make_base_rows(const int32_t* lb, const int32_t* ub, int64_t n
 }
 
 hipError_t launch_ingest_rows(const int32_t* lb, const int32_t* ub, int2* store, int64_t nvars,
-                              int64_t n_inst, uint8_t* empty, const DiffSink& ds, hipStream_t stream) {
+                              int64_t n_inst, uint8_t* empty, const DiffSink& ds, const int2* held,
+                              hipStream_t stream) {
   if (nvars <= 0 || n_inst <= 0) return hipSuccess;
   const bool v4 = nvars % 4 == 0 &&
                   ((reinterpret_cast<uintptr_t>(lb) | reinterpret_cast<uintptr_t>(ub) |
-                    reinterpret_cast<uintptr_t>(store)) & 15) == 0;
+                    reinterpret_cast<uintptr_t>(store) | reinterpret_cast<uintptr_t>(held)) & 15) == 0;
   const int64_t n = n_inst * nvars;
   int64_t grid = (n / (v4 ? 4 : 1) + kBlock - 1) / kBlock;
   if (grid > (1 << 20)) grid = 1 << 20;
   u64* st = reinterpret_cast<u64*>(store);
-  if (v4) k_ingest_rows<true><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds);
-  else k_ingest_rows<false><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds);
+  const u64* h = reinterpret_cast<const u64*>(held);
+  if (v4 && h) k_ingest_rows<true, true><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds, h);
+  else if (v4) k_ingest_rows<true, false><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds, h);
+  else if (h) k_ingest_rows<false, true><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds, h);
+  else k_ingest_rows<false, false><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds, h);
   return hipGetLastError();
 }
 
@@ -3060,6 +3076,34 @@ hipError_t launch_restore_trail(const InstLists& tr, const DiffSink& ds, const i
     if (g2 > 8192) g2 = 8192;
     k_scatter_diffs<<<(unsigned)g2, kBlock, 0, stream>>>(ds, st, nvars);
   }
+  return hipGetLastError();
+}
+
+// The base value at every listed difference's position.
+__global__ __launch_bounds__(kBlock) void k_unscatter_diffs(DiffSink ds, u64* __restrict__ store, int64_t nvars) {
+  const int64_t n_slots = kDiffLists * ds.cap;
+  const u64* base = reinterpret_cast<const u64*>(ds.base);
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n_slots; i += (int64_t)gridDim.x * kBlock) {
+    if (i % ds.cap >= ds.count[i / ds.cap]) continue;
+    const u64 d = ds.list[i];
+    const uint32_t v = (uint32_t)d;
+    store[(int64_t)(d >> 32) * nvars + v] = base[v];
+  }
+}
+
+hipError_t launch_stores_to_base(const InstLists* trail, const DiffSink& ds, int2* store, int64_t nvars,
+                                 int64_t n_inst, hipStream_t stream) {
+  if (nvars <= 0 || n_inst <= 0) return hipSuccess;
+  if (!ds.list || !ds.count || !ds.base || (trail && !trail->list)) return hipErrorInvalidValue;
+  u64* st = reinterpret_cast<u64*>(store);
+  if (trail) {
+    const int64_t grid = n_inst < 65536 ? n_inst : 65536;
+    k_restore_trail<<<(unsigned)grid, kBlock, 0, stream>>>(*trail, reinterpret_cast<const u64*>(ds.base),
+                                                           nullptr, st, nvars, n_inst);
+  }
+  int64_t g2 = (kDiffLists * ds.cap + kBlock - 1) / kBlock;
+  if (g2 > 8192) g2 = 8192;
+  k_unscatter_diffs<<<(unsigned)g2, kBlock, 0, stream>>>(ds, st, nvars);
   return hipGetLastError();
 }
 

@@ -914,6 +914,79 @@ def test_drained_runs_leave_the_marks_clear(gpu):
         _compare(lb_g, ub_g, res_g, lo_o, hi_o, res_o, V, what)
 
 
+def _failing_set(pir, lb, ub, rl, ru, slb, sub, V):
+    """Instances 7 and 20 fail in their first round (an add whose operands
+    are fixed and whose result is fixed to another value), instance 3 starts
+    failed, and instance 12 has 1,500 variables back at their told bounds
+    (more tightenings than a drained run's trail holds)."""
+    top, tx, ty, tz = pir.table()
+    slb, sub = slb.copy(), sub.copy()
+    for p in np.nonzero(top == pcx.ADD)[0]:
+        x, y, z = int(tx[p]), int(ty[p]), int(tz[p])
+        if len({x, y, z}) < 3:
+            continue
+        for k in (7, 20):
+            slb[k * V + y] = sub[k * V + y] = rl[y]
+            slb[k * V + z] = sub[k * V + z] = rl[z]
+            slb[k * V + x] = sub[k * V + x] = int(rl[y]) + int(rl[z]) + 1
+        break
+    slb[3 * V + 11], sub[3 * V + 11] = 7, 6
+    wide = np.nonzero((lb != rl) | (ub != ru))[0][:1500]
+    assert len(wide) == 1500
+    slb[12 * V + wide], sub[12 * V + wide] = lb[wide], ub[wide]
+    return slb, sub
+
+
+@pytest.mark.parametrize("V", [4000, 3999], ids=["v4", "v1"])
+def test_device_loads_write_only_what_differs(gpu, V):
+    """A device load into a batch whose stores hold the last image (base
+    store plus its complete difference list), or that image but for the
+    trail of a drained run from it, puts the stores back to the base and then
+    writes only the loaded values that differ from it.  Loads in a row: after
+    drained runs (trails with failed instances and one past its cap), after
+    a reset, with a new base store (splits of the told store instead of the
+    root fixpoint), after a swept run (no trail: every value written) and
+    after a host load.  After each load the stores read back equal the
+    loaded ones, and every run equals the oracle."""
+    from pcx.device import DeviceArray
+    n_inst = 60
+    pir, lb, ub, rl, ru, slb, sub, off, idx = _seed_case(121, V, 4 * V, n_inst, False)
+    sets = {"A": _failing_set(pir, lb, ub, rl, ru, slb, sub, V),
+            "B": pcx.gen_splits(123, rl, ru, n_inst, 1, 20),
+            "C": pcx.gen_splits(124, lb, ub, n_inst, 1, 20)}
+    oracle = {k: _oracle_batch(pir, lo, hi, V, None, None) for k, (lo, hi) in sets.items()}
+    assert oracle["A"][2][7].failed and oracle["A"][2][3].failed
+    dev = {k: (DeviceArray(lo), DeviceArray(hi)) for k, (lo, hi) in sets.items()}
+    b = pcx.Batch(pir, n_inst, V)
+    steps = [("A", "drain"), ("B", "drain"), ("A", "drain"), ("B", "reset"), ("C", "drain"),
+             ("B", "swept"), ("B", "drain"), ("A", "host"), ("B", "drain"), ("C", "drain")]
+    try:
+        for i, (k, how) in enumerate(steps):
+            what = f"step {i}: load {k} ({how})"
+            if how == "host":
+                b.load(*sets[k])
+                b.run(pcx.MODE_BATCHED)
+                b.reset()
+                continue
+            b.load_device(dev[k][0].ptr, dev[k][1].ptr)
+            lb_r, ub_r = b.read()
+            assert np.array_equal(lb_r, sets[k][0]) and np.array_equal(ub_r, sets[k][1]), what
+            pir.set_config(max_sweeps=1000 if how == "swept" else 0)
+            res_g, st = b.run(pcx.MODE_BATCHED)
+            lb_g, ub_g = b.read()
+            lo_o, hi_o, res_o = oracle[k]
+            _compare(lb_g, ub_g, res_g, lo_o, hi_o, res_o, V, what)
+            if how == "reset":
+                b.reset()
+                lb_r, ub_r = b.read()
+                assert np.array_equal(lb_r, sets[k][0]) and np.array_equal(ub_r, sets[k][1]), what
+    finally:
+        pir.set_config(max_sweeps=0)
+        for d in dev.values():
+            d[0].close()
+            d[1].close()
+
+
 @pytest.mark.parametrize("queue", [-1, 0, 10], ids=["queue", "bitmaps", "overflow"])
 @pytest.mark.parametrize("limit", [0, 1000], ids=["drain", "swept"])
 def test_seeded_first_sweep_marks_from_bitmaps(gpu, queue, limit):
