@@ -1295,8 +1295,9 @@ pcx_status probe_loaded(pcx_batch* b, bool* complete) {
   const bool probe = b->table_n > 0 && (int64_t)e->op.size() == b->table_n;
   if (probe) {
     if (!b->d_unsat) PCX_HIP(hipMalloc(&b->d_unsat, b->table_n * sizeof(int32_t)));
-    PCX_HIP(launch_base_probe(table_of(e), b->d_base, b->d_unsat, b->d_seedctl + kDiffLists, dom_of(e),
-                              e->stream));
+    int32_t* unsat_n = b->d_seedctl + kDiffLists;      // the probe appends from zero
+    PCX_HIP(hipMemsetAsync(unsat_n, 0, sizeof(int32_t), e->stream));
+    PCX_HIP(launch_base_probe(table_of(e), b->d_base, b->d_unsat, unsat_n, dom_of(e), e->stream));
   }
   int32_t ctl[kDiffLists + 1];
   st = read_small(b, b->d_seedctl, sizeof(ctl), ctl);
@@ -1445,9 +1446,9 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
     PCX_HIP(hipEventElapsedTime(&ms, t0, t1));
     b->load_ms = ms;
   }
-  if (ds.list && !b->shared_loads) {
+  if (ds.list && !rows) {
     // a load of the other kind whose stores share most values: the next load
-    // writes the stores only
+    // writes the stores only (a rows load has probed its list already)
     bool complete = false;
     pcx_status st = probe_loaded(b, &complete);
     if (st != PCX_OK) return st;

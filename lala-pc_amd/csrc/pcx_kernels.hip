@@ -984,12 +984,11 @@ hipError_t launch_sweep_batched(const Table& t, int2* store, int64_t nvars, cons
 // slot i at word (g * nvars + v) * GI + i.  One workgroup transposes a tile
 // of GI instances x (4096 / GI) variables through LDS; both sides are
 // contiguous runs.
-// Appends (k << 32 | v) for the lanes whose d is set, with one counter
-// atomic per wave.  Called by every lane of the wave.
-__device__ __forceinline__ void diff_append(const DiffSink& ds, bool d, int64_t k, int64_t v, u64 val) {
+// Appends (k << 32 | v) to sub-list c for the lanes whose d is set, with one
+// counter atomic per wave.  Called by every lane of the wave.
+__device__ __forceinline__ void diff_append_to(const DiffSink& ds, int c, bool d, int64_t k, int64_t v, u64 val) {
   const u64 mask = __ballot(d);
   if (mask == 0) return;
-  const int c = blockIdx.x % kDiffLists;
   const int leader = __builtin_ctzll(mask);
   int base = 0;
   if (lane_id() == leader) base = atomicAdd(&ds.count[c], (int)__popcll(mask));
@@ -1000,6 +999,10 @@ __device__ __forceinline__ void diff_append(const DiffSink& ds, bool d, int64_t 
     ds.list[c * ds.cap + at] = ((u64)k << 32) | (uint32_t)v;
     if (ds.vals) reinterpret_cast<u64*>(ds.vals)[c * ds.cap + at] = val;
   }
+}
+// The same, to the workgroup's sub-list.
+__device__ __forceinline__ void diff_append(const DiffSink& ds, bool d, int64_t k, int64_t v, u64 val) {
+  diff_append_to(ds, blockIdx.x % kDiffLists, d, k, v, val);
 }
 // The base value of one variable: what two of instances 0, 1 and 2 hold
 // (instance 0's when no two agree or the batch is smaller).
@@ -2981,6 +2984,64 @@ __global__ __launch_bounds__(kBlock) void k_ingest_rows(const int32_t* __restric
   }
 }
 
+// The same load into stores that hold the base store (`held`): one
+// workgroup per (tile of 1,024 variables, chunk of kHeldChunk instances),
+// each thread four variables whose held and base values it loads once, then
+// the chunk's rows (4 KB of each row per array and workgroup).  Only the
+// values that differ from `held` are written.
+constexpr int kHeldChunk = 16;
+__global__ __launch_bounds__(kBlock) void k_ingest_held(const int32_t* __restrict__ lb,
+                                                        const int32_t* __restrict__ ub, u64* __restrict__ store,
+                                                        int64_t nvars, int64_t n_inst, int64_t vtiles,
+                                                        uint8_t* __restrict__ empty, DiffSink ds,
+                                                        const u64* __restrict__ held) {
+  const bool listing = ds.list != nullptr;
+  const u64* base = reinterpret_cast<const u64*>(ds.base);
+  const int64_t v = (blockIdx.x % vtiles) * (kBlock * 4) + threadIdx.x * 4;
+  const int64_t k0 = (blockIdx.x / vtiles) * kHeldChunk;
+  const int64_t k1 = min(n_inst, k0 + kHeldChunk);
+  const bool in = v < nvars;                           // nvars % 4 == 0: then v + 3 < nvars
+  u64 h[4] = {0, 0, 0, 0}, bv[4] = {0, 0, 0, 0};
+  if (in) {
+    const ulonglong2 h0 = reinterpret_cast<const ulonglong2*>(held + v)[0];
+    const ulonglong2 h1 = reinterpret_cast<const ulonglong2*>(held + v)[1];
+    h[0] = h0.x; h[1] = h0.y; h[2] = h1.x; h[3] = h1.y;
+    if (listing) {
+      const ulonglong2 b0 = reinterpret_cast<const ulonglong2*>(base + v)[0];
+      const ulonglong2 b1 = reinterpret_cast<const ulonglong2*>(base + v)[1];
+      bv[0] = b0.x; bv[1] = b0.y; bv[2] = b1.x; bv[3] = b1.y;
+    }
+  }
+#pragma unroll 4
+  for (int64_t k = k0; k < k1; ++k) {                  // the same trip count for the workgroup
+    const int64_t i = k * nvars + v;
+    u64 w[4] = {0, 0, 0, 0};
+    if (in) {
+      const v4i a = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(lb + i));
+      const v4i b = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(ub + i));
+      w[0] = pack_bounds(a.x, b.x); w[1] = pack_bounds(a.y, b.y);
+      w[2] = pack_bounds(a.z, b.z); w[3] = pack_bounds(a.w, b.w);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (w[j] != h[j]) store[i + j] = w[j];
+      if ((a.x > b.x) | (a.y > b.y) | (a.z > b.z) | (a.w > b.w)) empty[k] = 1;
+    }
+    if (listing) {
+      bool dif[4], any = false;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        dif[j] = in && w[j] != bv[j];
+        any |= dif[j];
+      }
+      if (__ballot(any)) {                             // a sub-list per (workgroup, row)
+        const int c = (int)((blockIdx.x * kHeldChunk + (k - k0)) % kDiffLists);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) diff_append_to(ds, c, dif[j], k, v + j, w[j]);
+      }
+    }
+  }
+}
+
 hipError_t launch_make_base_rows(const int32_t* lb, const int32_t* ub, int64_t nvars, int64_t n_inst,
                                  const DiffSink& ds, hipStream_t stream) {
   if (!ds.list || nvars <= 0 || n_inst <= 0) return hipSuccess;
@@ -2999,8 +3060,12 @@ hipError_t launch_ingest_rows(const int32_t* lb, const int32_t* ub, int2* store,
   if (grid > (1 << 20)) grid = 1 << 20;
   u64* st = reinterpret_cast<u64*>(store);
   const u64* h = reinterpret_cast<const u64*>(held);
-  if (v4 && h) k_ingest_rows<true, true><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds, h);
-  else if (v4) k_ingest_rows<true, false><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds, h);
+  if (v4 && h) {
+    const int64_t vtiles = (nvars + kBlock * 4 - 1) / (kBlock * 4);
+    const int64_t g = vtiles * ((n_inst + kHeldChunk - 1) / kHeldChunk);
+    if (g > 0x7fffffff) return hipErrorInvalidValue;
+    k_ingest_held<<<(unsigned)g, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, vtiles, empty, ds, h);
+  } else if (v4) k_ingest_rows<true, false><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds, h);
   else if (h) k_ingest_rows<false, true><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds, h);
   else k_ingest_rows<false, false><<<(unsigned)grid, kBlock, 0, stream>>>(lb, ub, st, nvars, n_inst, empty, ds, h);
   return hipGetLastError();
@@ -3150,7 +3215,7 @@ __global__ __launch_bounds__(kBlock) void k_base_probe(Table t, const int2* __re
       b = __builtin_amdgcn_readlane(b, leader);
       const int at = b + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-      if (need) list[at] = (int32_t)p;     // at < t.n: each position is listed once
+      if (need && at < t.n) list[at] = (int32_t)p;   // each position is listed once: at < t.n
     }
   }
 }

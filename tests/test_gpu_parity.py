@@ -945,21 +945,28 @@ def test_device_loads_write_only_what_differs(gpu, V):
     writes only the loaded values that differ from it.  Loads in a row: after
     drained runs (trails with failed instances and one past its cap), after
     a reset, with a new base store (splits of the told store instead of the
-    root fixpoint), after a swept run (no trail: every value written) and
-    after a host load.  After each load the stores read back equal the
-    loaded ones, and every run equals the oracle."""
+    root fixpoint), with random stores (whose differences overflow the list
+    when a row's land in one sub-list: the image is then copied and the next
+    load writes every value), after a swept run (no trail: every value
+    written) and after a host load.  After each load the stores read back
+    equal the loaded ones, and every run equals the oracle."""
     from pcx.device import DeviceArray
     n_inst = 60
     pir, lb, ub, rl, ru, slb, sub, off, idx = _seed_case(121, V, 4 * V, n_inst, False)
+    rng = np.random.default_rng(125)
+    lo_t, hi_t = np.tile(lb, n_inst).astype(np.int64), np.tile(ub, n_inst).astype(np.int64)
+    ra, rb = rng.integers(lo_t, hi_t + 1), rng.integers(lo_t, hi_t + 1)
     sets = {"A": _failing_set(pir, lb, ub, rl, ru, slb, sub, V),
             "B": pcx.gen_splits(123, rl, ru, n_inst, 1, 20),
-            "C": pcx.gen_splits(124, lb, ub, n_inst, 1, 20)}
+            "C": pcx.gen_splits(124, lb, ub, n_inst, 1, 20),
+            "D": (np.minimum(ra, rb).astype(np.int32), np.maximum(ra, rb).astype(np.int32))}
     oracle = {k: _oracle_batch(pir, lo, hi, V, None, None) for k, (lo, hi) in sets.items()}
     assert oracle["A"][2][7].failed and oracle["A"][2][3].failed
     dev = {k: (DeviceArray(lo), DeviceArray(hi)) for k, (lo, hi) in sets.items()}
     b = pcx.Batch(pir, n_inst, V)
     steps = [("A", "drain"), ("B", "drain"), ("A", "drain"), ("B", "reset"), ("C", "drain"),
-             ("B", "swept"), ("B", "drain"), ("A", "host"), ("B", "drain"), ("C", "drain")]
+             ("D", "drain"), ("B", "drain"), ("B", "drain"), ("B", "swept"), ("B", "drain"),
+             ("A", "host"), ("B", "drain"), ("C", "drain")]
     try:
         for i, (k, how) in enumerate(steps):
             what = f"step {i}: load {k} ({how})"
