@@ -358,7 +358,7 @@ def ingest_kernel(pir, n_inst, V, written):
     each); without it, it writes the stores and their interleaved copy (8 B
     read + 16 B written per value)."""
     if pir.get_config().base_seed != 0:
-        return "k_ingest_rows", 8 * n_inst * V + 8 * written
+        return ("k_ingest_held" if V % 4 == 0 else "k_ingest_rows"), 8 * n_inst * V + 8 * written
     return "k_ingest_interleave4", 24 * n_inst * V
 
 
