@@ -1362,17 +1362,18 @@ pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_
   // The new base store goes to the other base buffer.
   const bool held = b->shared_loads && b->image_in_diffs && (b->store_is_image || b->trail_valid) &&
                     e->cfg.base_seed != 0 && b->n_inst >= 2 && b->nvars > 0;
+  const bool from_image = b->store_is_image;
+  b->store_is_image = false;               // until the load below completes
+  b->trail_valid = false;
+  b->image_in_diffs = false;
   if (held) {
     if (!b->d_base_alt) PCX_HIP(hipMalloc(&b->d_base_alt, b->nvars * sizeof(int2)));
     const InstLists trail{b->d_trail, b->d_trail_n, kTrailCap};
-    PCX_HIP(launch_stores_to_base(b->store_is_image ? nullptr : &trail,
+    PCX_HIP(launch_stores_to_base(from_image ? nullptr : &trail,
                                   DiffSink{b->d_diff, b->d_seedctl, b->diff_cap, b->d_base, b->d_diff_val},
                                   b->d_store, b->nvars, b->n_inst, e->stream));
     std::swap(b->d_base, b->d_base_alt);   // d_base_alt: what every row now holds
   }
-  b->store_is_image = false;               // until the load below completes
-  b->trail_valid = false;
-  b->image_in_diffs = false;
   PCX_HIP(hipMemsetAsync(b->d_image_empty, 0, b->n_inst, e->stream));
   DiffSink ds;
   pcx_status dst = diff_sink(b, &ds);
