@@ -2906,13 +2906,15 @@ hipError_t launch_list_diffs(const int2* store, int64_t nvars, int64_t n_inst, c
 // workgroup here (the load then copies the image and later loads take the
 // fused ingest).  With `held` (stores that hold these base values in every
 // row) only the values that differ from it are written: the pass then reads
-// the rows and writes a few values.
+// the rows and writes a few values (single values only: k_ingest_held takes
+// held loads four values at a time).
 template <bool V4, bool HELD>
 __global__ __launch_bounds__(kBlock) void k_ingest_rows(const int32_t* __restrict__ lb,
                                                         const int32_t* __restrict__ ub, u64* __restrict__ store,
                                                         int64_t nvars, int64_t n_inst,
                                                         uint8_t* __restrict__ empty, DiffSink ds,
                                                         const u64* __restrict__ held) {
+  static_assert(!(V4 && HELD), "held loads of four values take k_ingest_held");
   constexpr int kPer = V4 ? 4 : 1;
   const bool listing = ds.list != nullptr;
   const u64* base = reinterpret_cast<const u64*>(ds.base);
@@ -2931,18 +2933,9 @@ __global__ __launch_bounds__(kBlock) void k_ingest_rows(const int32_t* __restric
         const v4i b = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(ub + i));
         w[0] = pack_bounds(a.x, b.x); w[1] = pack_bounds(a.y, b.y);
         w[2] = pack_bounds(a.z, b.z); w[3] = pack_bounds(a.w, b.w);
-        if constexpr (HELD) {
-          const ulonglong2 h0 = reinterpret_cast<const ulonglong2*>(held + v)[0];
-          const ulonglong2 h1 = reinterpret_cast<const ulonglong2*>(held + v)[1];
-          const u64 h[4] = {h0.x, h0.y, h1.x, h1.y};
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (w[j] != h[j]) store[i + j] = w[j];
-        } else {
-          const v4i p0 = {a.x, b.x, a.y, b.y}, p1 = {a.z, b.z, a.w, b.w};
-          reinterpret_cast<v4i*>(store + i)[0] = p0;
-          reinterpret_cast<v4i*>(store + i)[1] = p1;
-        }
+        const v4i p0 = {a.x, b.x, a.y, b.y}, p1 = {a.z, b.z, a.w, b.w};
+        reinterpret_cast<v4i*>(store + i)[0] = p0;
+        reinterpret_cast<v4i*>(store + i)[1] = p1;
         bad = (a.x > b.x) | (a.y > b.y) | (a.z > b.z) | (a.w > b.w);
       } else {
         const int32_t a = __builtin_nontemporal_load(lb + i), b = __builtin_nontemporal_load(ub + i);

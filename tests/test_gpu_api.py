@@ -289,7 +289,7 @@ def test_device_loads_switch_between_image_forms(gpu, gi):
     loads, every run equals the same stores loaded from the host, and every
     reset gives back the loaded stores."""
     from pcx.device import DeviceArray
-    V, P, n = 700, 2800, 131
+    V, P, n = 700, 2800, 800                 # unshared loads overflow sub-lists of 2,048 entries
     op, x, y, z, lb, ub, _ = pcx.gen_network(pcx.gen_params(seed=7 + gi, nvars=V, nprops=P))
     pir = pcx.PIR(lockstep=gi)
     pir.tell(op, x, y, z, lb, ub)
