@@ -232,8 +232,10 @@ pcx_status pcx_batch_load(pcx_batch* b, const int32_t* lb, const int32_t* ub);
  * ahead of time); no host transfer.  One pass writes the stores and the
  * per-instance emptiness, and keeps the reset image (as a copy, or -- stores
  * that share most values -- as a base store and the values that differ from
- * it).  The source may be reused or freed once the call returns.  Interval
- * batches. */
+ * it).  When the batch's stores hold the last such image (or that image but
+ * for the variables a run tightened), the pass writes only the loaded values
+ * that differ from the base store.  The source may be reused or freed once
+ * the call returns.  Interval batches. */
 pcx_status pcx_batch_load_device(pcx_batch* b, const int32_t* d_lb, const int32_t* d_ub);
 /* PC's is_extractable on every store of the batch (pc.hpp:727-738: the
  * store is not empty and every formula is asked), for formula trees in the
