@@ -3562,14 +3562,15 @@ hipError_t launch_drain_marks(const Table& t, int2* store, int64_t nvars, InstFl
                               int32_t* next_inst, unsigned long long* tight, const InstLists& seeds,
                               const InstLists& trail, int dom, int grid, hipStream_t stream) {
   if (ev.n_inst <= 0 || !ev.dirty) return hipSuccess;
-  // Interval drains in workgroups of two waves: at 4,096 instances of 50,000
-  // variables 0.228 ms against 0.236 with one wave and 0.255 with four (the
-  // replayed step 0.33 ms against 0.35-0.36).
-  constexpr int kItvDrain = 2 * kWave;
+  // Workgroups of two waves: at 4,096 interval instances of 50,000 variables
+  // 0.228 ms against 0.236 with one wave and 0.255 with four (the replayed
+  // step 0.33 ms against 0.35-0.36); at 1,024 bitset instances of 200,000
+  // words 98 us against 105 with one wave.
+  constexpr int kDrainB = 2 * kWave;
   switch (dom) {
-    case kItv: k_drain_marks<kItv, kItvDrain><<<grid, kItvDrain, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, seeds, trail); break;
-    case kItvDiv: k_drain_marks<kItvDiv, kItvDrain><<<grid, kItvDrain, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, seeds, trail); break;
-    case kBits: k_drain_marks<kBits, kWave><<<grid, kWave, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, seeds, trail); break;
+    case kItv: k_drain_marks<kItv, kDrainB><<<grid, kDrainB, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, seeds, trail); break;
+    case kItvDiv: k_drain_marks<kItvDiv, kDrainB><<<grid, kDrainB, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, seeds, trail); break;
+    case kBits: k_drain_marks<kBits, kDrainB><<<grid, kDrainB, 0, stream>>>(t, store, nvars, flags, ev, next_inst, tight, seeds, trail); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
