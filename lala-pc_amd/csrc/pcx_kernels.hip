@@ -3078,13 +3078,33 @@ __global__ __launch_bounds__(kBlock) void k_fill_base(const u64* __restrict__ ba
     if (v < nvars) store[k * nvars + v] = base[v];
   }
 }
-__global__ __launch_bounds__(kBlock) void k_scatter_diffs(DiffSink ds, u64* __restrict__ store, int64_t nvars) {
-  const int64_t n_slots = kDiffLists * ds.cap;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n_slots; i += (int64_t)gridDim.x * kBlock) {
-    if (i % ds.cap >= ds.count[i / ds.cap]) continue;
-    const u64 d = ds.list[i];
-    store[(int64_t)(d >> 32) * nvars + (uint32_t)d] = reinterpret_cast<const u64*>(ds.vals)[i];
+// The listed differences, a group of workgroups per sub-list: workgroup b
+// walks sub-list b % kDiffLists from chunk b / kDiffLists on, kBlock entries
+// a chunk, up to the sub-list's count (read once), so the slots past the
+// counts cost nothing.  The grid is a multiple of kDiffLists (diff_grid).
+template <typename F>
+__device__ __forceinline__ void for_listed_diffs(const DiffSink& ds, F f) {
+  const int c = blockIdx.x % kDiffLists;
+  const int64_t step = (int64_t)(gridDim.x / kDiffLists) * kBlock;
+  const int64_t n = min<int64_t>(ds.count[c], ds.cap);
+  for (int64_t at0 = (int64_t)(blockIdx.x / kDiffLists) * kBlock; at0 < n; at0 += step) {
+    const int64_t at = at0 + threadIdx.x;
+    if (at < n) {
+      const int64_t slot = c * ds.cap + at;
+      const u64 d = ds.list[slot];
+      f(slot, (int64_t)(d >> 32), (uint32_t)d);
+    }
   }
+}
+unsigned diff_grid(const DiffSink& ds) {
+  const int64_t chunks = (ds.cap + kBlock - 1) / kBlock;
+  return (unsigned)(kDiffLists * (chunks < 16 ? chunks : 16));
+}
+
+// Every listed difference's value into the stores.
+__global__ __launch_bounds__(kBlock) void k_scatter_diffs(DiffSink ds, u64* __restrict__ store, int64_t nvars) {
+  const u64* vals = reinterpret_cast<const u64*>(ds.vals);
+  for_listed_diffs(ds, [&](int64_t slot, int64_t k, uint32_t v) { store[k * nvars + v] = vals[slot]; });
 }
 
 hipError_t launch_image_from_diffs(const DiffSink& ds, int2* store, int64_t nvars, int64_t n_inst,
@@ -3095,9 +3115,7 @@ hipError_t launch_image_from_diffs(const DiffSink& ds, int2* store, int64_t nvar
   if (grid > 0x7fffffff || !ds.vals) return hipErrorInvalidValue;
   u64* st = reinterpret_cast<u64*>(store);
   k_fill_base<<<(unsigned)grid, kBlock, 0, stream>>>(reinterpret_cast<const u64*>(ds.base), st, nvars, vblocks);
-  int64_t g2 = (kDiffLists * ds.cap + kBlock - 1) / kBlock;
-  if (g2 > 8192) g2 = 8192;
-  k_scatter_diffs<<<(unsigned)g2, kBlock, 0, stream>>>(ds, st, nvars);
+  k_scatter_diffs<<<diff_grid(ds), kBlock, 0, stream>>>(ds, st, nvars);
   return hipGetLastError();
 }
 
@@ -3131,24 +3149,14 @@ hipError_t launch_restore_trail(const InstLists& tr, const DiffSink& ds, const i
   u64* st = reinterpret_cast<u64*>(store);
   k_restore_trail<<<(unsigned)grid, kBlock, 0, stream>>>(tr, reinterpret_cast<const u64*>(ds.base),
                                                          reinterpret_cast<const u64*>(image), st, nvars, n_inst);
-  if (!image) {
-    int64_t g2 = (kDiffLists * ds.cap + kBlock - 1) / kBlock;
-    if (g2 > 8192) g2 = 8192;
-    k_scatter_diffs<<<(unsigned)g2, kBlock, 0, stream>>>(ds, st, nvars);
-  }
+  if (!image) k_scatter_diffs<<<diff_grid(ds), kBlock, 0, stream>>>(ds, st, nvars);
   return hipGetLastError();
 }
 
 // The base value at every listed difference's position.
 __global__ __launch_bounds__(kBlock) void k_unscatter_diffs(DiffSink ds, u64* __restrict__ store, int64_t nvars) {
-  const int64_t n_slots = kDiffLists * ds.cap;
   const u64* base = reinterpret_cast<const u64*>(ds.base);
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n_slots; i += (int64_t)gridDim.x * kBlock) {
-    if (i % ds.cap >= ds.count[i / ds.cap]) continue;
-    const u64 d = ds.list[i];
-    const uint32_t v = (uint32_t)d;
-    store[(int64_t)(d >> 32) * nvars + v] = base[v];
-  }
+  for_listed_diffs(ds, [&](int64_t, int64_t k, uint32_t v) { store[k * nvars + v] = base[v]; });
 }
 
 hipError_t launch_stores_to_base(const InstLists* trail, const DiffSink& ds, int2* store, int64_t nvars,
@@ -3161,9 +3169,7 @@ hipError_t launch_stores_to_base(const InstLists* trail, const DiffSink& ds, int
     k_restore_trail<<<(unsigned)grid, kBlock, 0, stream>>>(*trail, reinterpret_cast<const u64*>(ds.base),
                                                            nullptr, st, nvars, n_inst);
   }
-  int64_t g2 = (kDiffLists * ds.cap + kBlock - 1) / kBlock;
-  if (g2 > 8192) g2 = 8192;
-  k_unscatter_diffs<<<(unsigned)g2, kBlock, 0, stream>>>(ds, st, nvars);
+  k_unscatter_diffs<<<diff_grid(ds), kBlock, 0, stream>>>(ds, st, nvars);
   return hipGetLastError();
 }
 
